@@ -1,0 +1,178 @@
+"""Benchmark of the fused ensemble propagate on one MI355X.
+
+Workload (default `c2`): the standard map at K in {0.97, 2.0, 6.0}, a 1024 x 1024 grid of cell centres
+(2^20 orbits) per K, 10^4 steps, k = 2, tau = 1, with MEGNO + FLI + OFLI + SALI on one tangent frame and
+QRSpectrum(HouseholderQR()) on a second frame, all FP64, fused in one kernel pass. One timed step is one
+full propagate of one K's grid (frames formed on the device, states initialised, 10^4 steps); the steps
+cycle through the three K values. Inputs are resident in HBM before the timed region; results stay there.
+
+`--workload c1`: the reference's CPU-runnable case (K = 6, 4096 starts, 10^5 steps, QRSpectrum(MGS())).
+
+Prints one JSON line with the metric, the roofline of the orbit kernel (HIP events on the launch stream)
+and the CPU baseline (the oracle, glibc math, on the host's cores, on a bounded sample of the workload).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "orbit-steps/s at equal λ error vs Double64 oracle; surrogate-triage speedup"
+FP64_PEAK_TFLOPS = 78.6
+# Algorithmic FP64 operations per orbit-step (fma = 2, add/mul/div/sqrt = 1, each sin/cos/log = 20):
+# c2 = map step and two 2x2 frame products (~70) + column norms and logs (~55) + MEGNO (~55) + FLI (~10)
+#      + OFLI on the map displacement (~145) + SALI (~30) + Householder QR spectrum (~150), about 500;
+# c1 = map step and one frame product (~57) + MGS, G_w, orthogonality loss and two logs (~117), about 174.
+FLOPS_PER_ORBIT_STEP = {"c2": 500.0, "c1": 174.0}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    from chaos_amd import MEGNO, FLI, OFLI, SALI, QRSpectrum, HouseholderQR, MGS
+    from chaos_amd.inputs import grid_centres, torus_uniform
+    if name == "c2":
+        return dict(name="c2", Ks=[0.97, 2.0, 6.0], x0=grid_centres(1024), N=10_000, tau=1, seed=1,
+                    red=(MEGNO(replace=False), FLI(replace=False), OFLI(replace=False), SALI()),
+                    red2=(QRSpectrum(HouseholderQR()),),
+                    desc="standard map K in {0.97,2,6}, 1024^2 grid centres per K, 1e4 steps, k=2, tau=1, "
+                         "MEGNO+FLI+OFLI+SALI on frame 1 + QR(Householder) on frame 2, FP64")
+    if name == "c1":
+        return dict(name="c1", Ks=[6.0], x0=torus_uniform(1, 4096), N=100_000, tau=1, seed=1,
+                    red=(QRSpectrum(MGS()),), red2=(),
+                    desc="standard map K=6, 4096 starts uniform on the torus (seed 1), 1e5 steps, k=2, tau=1, "
+                         "QRSpectrum(MGS()), FP64")
+    raise SystemExit(f"unknown workload {name}")
+
+
+def gpu_run(wl, steps, warmup):
+    import torch
+    from chaos_amd import StandardMap, _lib as L
+    from chaos_amd.tangent import make_chain, _context
+    lib, ctx = _context()
+    x0 = wl["x0"]
+    m = len(x0)
+    d = k = 2
+    chain, allr = make_chain(wl["red"], wl["red2"])
+    nf = 2 if wl["red2"] else 1
+    sd0 = StandardMap(1.0).descriptor()
+    words = lib.cl_state_bytes(ctypes.byref(sd0), ctypes.byref(chain), 1, d, k) // 8
+    dev = torch.device("cuda")
+    x_master = torch.from_numpy(np.ascontiguousarray(x0.T)).to(dev)
+    dx = torch.empty_like(x_master)
+    dW = torch.empty((d * k, m), dtype=torch.float64, device=dev)
+    dW2 = torch.empty_like(dW) if nf > 1 else None
+    dst = torch.empty((words, m), dtype=torch.int64, device=dev)
+    dcode = torch.zeros(m, dtype=torch.int8, device=dev)
+    dh = torch.zeros(m, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    N, tau = wl["N"], wl["tau"]
+    kernel_ms = []
+
+    def one_step(i, timed):
+        K = wl["Ks"][i % len(wl["Ks"])]
+        sd = StandardMap(K).descriptor()
+        dx.copy_(x_master)
+        rc = lib.cl_initial_deviations_device(ctx, wl["seed"], 1, m, d, k, dW.data_ptr(), sh)
+        assert rc == 0
+        if dW2 is not None:
+            dW2.copy_(dW)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        rc = lib.cl_propagate_device(ctx, ctypes.byref(sd), ctypes.byref(chain), 1, m, 2, d, k, 0, N, N, tau,
+                                     dx.data_ptr(), dW.data_ptr(), dW2.data_ptr() if dW2 is not None else None,
+                                     dst.data_ptr(), dcode.data_ptr(), dh.data_ptr(), sh)
+        e1.record(stream)
+        assert rc == 0, lib.cl_last_error(ctx)
+        if timed:
+            kernel_ms.append((e0, e1))
+
+    for i in range(warmup):
+        one_step(i, False)
+        torch.cuda.synchronize()
+        log(f"warmup step {i + 1}/{warmup} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        one_step(warmup + i, True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    log(f"timed {steps} steps in {wall:.3f} s")
+    kms = [a.elapsed_time(b) for a, b in kernel_ms]
+    bad = int((dcode != 0).sum().item())
+    return dict(wall=wall, kernel_ms=kms, orbit_steps=float(m) * N, m=m, bad=bad)
+
+
+def cpu_baseline(wl, budget_orbits):
+    """The oracle (glibc sin/cos/log) on the host's cores over a bounded sample of the workload."""
+    from chaos_amd import StandardMap
+    from chaos_amd.tangent import make_chain
+    from tests import oracle_lib as O
+    x0 = wl["x0"]
+    stride = max(1, len(x0) // budget_orbits)
+    xs = x0[::stride]
+    m = len(xs)
+    W0 = O.frames(wl["seed"], 1, len(x0), 2, 2)[::stride]
+    chain, _ = make_chain(wl["red"], wl["red2"])
+    nthreads = min(16, os.cpu_count() or 1)
+    total, t = 0.0, 0.0
+    for K in wl["Ks"]:
+        t0 = time.perf_counter()
+        O.run(StandardMap(K).descriptor(), chain, xs, W0, wl["N"], wl["tau"], 2,
+              W2=W0 if wl["red2"] else None, math=O.MATH_LIBM, nthreads=nthreads)
+        t += time.perf_counter() - t0
+        total += float(m) * wl["N"]
+        log(f"cpu baseline K={K}: {m} orbits done")
+    return dict(value=total / t, unit="orbit-steps/s", cores=nthreads, kind="port",
+                sample=f"every {stride}th orbit of each K's set ({m} orbits x {wl['N']} steps x {len(wl['Ks'])} K), "
+                       f"oracle/oracle.cpp with glibc sin/cos/log, {nthreads} threads, {t:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1"])
+    ap.add_argument("--cpu-orbits", type=int, default=32768, help="orbits per K in the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: the ensemble shards by orbit ranges with no exchange during propagation, "
+              f"and this build runs on single-GPU boxes only (asked for --gpus {a.gpus}).", file=sys.stderr)
+        sys.exit(2)
+    wl = workload(a.workload)
+    log("workload", wl["desc"])
+    g = gpu_run(wl, a.steps, a.warmup)
+    value = g["orbit_steps"] * a.steps / g["wall"]
+    avg_ms = float(np.mean(g["kernel_ms"]))
+    flops = FLOPS_PER_ORBIT_STEP[wl["name"]] * g["orbit_steps"]
+    achieved = flops / (avg_ms * 1e-3) / 1e12
+    cpu = None if a.no_cpu else cpu_baseline(wl, a.cpu_orbits if wl["name"] == "c2" else 512)
+    out = {
+        "metric": METRIC, "value": value, "unit": "orbit-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": g["wall"] * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wl["desc"], "orbits_per_step": g["m"], "steps_per_orbit": wl["N"],
+                   "tau": wl["tau"], "k": 2, "nonfinite_orbits": g["bad"]},
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel_ms": avg_ms, "flops_per_orbit_step": FLOPS_PER_ORBIT_STEP[wl["name"]]},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

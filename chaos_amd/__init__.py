@@ -1,0 +1,13 @@
+"""chaos_amd: MI355X-native ensemble tangent stream (the `propagate` hot path of Chaoslab).
+
+Hand-written HIP kernels for gfx950 behind a C ABI (include/chaoslab_hip.h); this package is the
+Python host side of that ABI, mirroring the reference's `propagate` API and reducer types.
+"""
+from .systems import StandardMap, WrappedStandardMap, torus_state
+from .reducers import (QRSpectrum, CGS, MGS, RGS, HouseholderQR, AugmentedQR, GivensQR, Benettin, MEGNO, SALI, FLI,
+                       OFLI, GALI)
+from .tangent import propagate, PropagationResult, NumericalHealthError, initial_deviations
+
+__all__ = ["StandardMap", "WrappedStandardMap", "torus_state", "QRSpectrum", "CGS", "MGS", "RGS", "HouseholderQR",
+           "AugmentedQR", "GivensQR", "Benettin", "MEGNO", "SALI", "FLI", "OFLI", "GALI", "propagate",
+           "PropagationResult", "NumericalHealthError", "initial_deviations"]

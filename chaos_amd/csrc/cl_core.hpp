@@ -1,0 +1,979 @@
+// cl_core.hpp - per-orbit numerics of the ensemble tangent stream, compiled for gfx950 kernels.
+//
+// One orbit lives in one lane: the state x, the deviation frame(s) W (column-major d x k, held in
+// VGPRs through compile-time sizes) and the reducer states. The arithmetic follows the reference
+// operation for operation (reference paths relative to the Chaoslab repository root):
+//   per-orbit body           src/Tangent/orbit.jl:44-60
+//   fused frame product      src/Foundation/product.jl:9-20 (explicit fma, ascending q)
+//   Neumaier sums, norms     src/Foundation/Foundation.jl:48-58, src/Foundation/norm.jl:8-12
+//   standard map             src/Systems/standard_map.jl:18-21,30-34
+//   wrapped standard map     src/Systems/wrapped_standard_map.jl:82-111
+//   QR spectrum              src/Lyapunov/qr_spectrum.jl:54-73, orthonormalize.jl:38-151
+//   Householder/Givens       src/Lyapunov/householder.jl:17-123
+//   Benettin                 src/Lyapunov/benettin.jl:159-164
+//   MEGNO / SALI / FLI       src/Indicators/megno.jl:63-80, sali.jl:166-179, fli.jl:320-341
+//   OFLI                     src/Indicators/ofli.jl:59-102
+//   GALI                     src/Indicators/gali.jl:220-341
+// so that, with cl_math.h providing the same sin/cos/log bits on host and device, a kernel
+// reproduces the host restatement of the same run bit for bit.
+#pragma once
+#include <stdint.h>
+#include "cl_math.h"
+
+#if !defined(__HIPCC__)
+#define __device__
+#define __host__
+#endif
+
+#define CL_EPS64 0x1p-52
+
+namespace cl {
+
+enum : int8_t { STATUS_OK = 0, STATUS_STATE = 1, STATUS_W = 2, STATUS_REDUCER = 3 };
+
+// ------------------------------------------------------------------------------ foundations
+struct CSum {
+  double s, c;
+};
+
+CL_HD CSum csum_zero() { return CSum{0.0, 0.0}; }
+
+// Neumaier update, branch on the larger magnitude (Foundation.jl:48-55).
+CL_HD CSum csum_add(CSum a, double y) {
+  double t = a.s + y;
+  double c = cl_fabs(a.s) >= cl_fabs(y) ? (a.s - t) + y : (y - t) + a.s;
+  return CSum{t, a.c + c};
+}
+CL_HD double csum_total(CSum a) { return a.s + a.c; }
+
+// Scaled Euclidean norm of v[0..n) with stride `st` (norm.jl:8-12): maximum of |v| (foldl with
+// Julia's max), then m * sqrt(sum((v/m)^2)) summed left to right.
+template <int N>
+CL_HD double safe_norm(const double* v, int st = 1) {
+  double m = cl_fabs(v[0]);
+#pragma unroll
+  for (int i = 1; i < N; ++i) m = cl_jmax(m, cl_fabs(v[i * st]));
+  if (!(cl_isfinite(m) && m > 0.0)) return m;
+  double a = v[0] / m;
+  double s = a * a;
+#pragma unroll
+  for (int i = 1; i < N; ++i) {
+    double b = v[i * st] / m;
+    s = s + b * b;
+  }
+  return m * cl_sqrt(s);
+}
+
+// Plain dot product in StaticArrays order: a1*b1 + a2*b2 + ... (no fusion).
+template <int N>
+CL_HD double dot(const double* a, const double* b) {
+  double s = a[0] * b[0];
+#pragma unroll
+  for (int i = 1; i < N; ++i) s = s + a[i] * b[i];
+  return s;
+}
+
+// ------------------------------------------------------------------------------ frames
+template <int D, int K>
+struct Frame {
+  double w[D * K];  // column-major: w[col * D + row]
+  CL_HD double& at(int r, int c) { return w[c * D + r]; }
+  CL_HD double at(int r, int c) const { return w[c * D + r]; }
+  CL_HD double* col(int c) { return w + c * D; }
+  CL_HD const double* col(int c) const { return w + c * D; }
+};
+
+template <int D, int K>
+CL_HD bool frame_finite(const Frame<D, K>& W) {
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < D * K; ++i) ok = ok && cl_isfinite(W.w[i]);
+  return ok;
+}
+
+// ------------------------------------------------------------------------------ systems
+// Standard map p' = p + K sin x, x' = x + p', Jacobian [[1 + c, 1], [c, 1]], c = K cos x.
+// The kernel carries sin and cos of the current point, so one reduction serves the Jacobian,
+// the step and a map displacement field read at the handoff.
+struct StdMap {
+  static constexpr int D = 2;
+  static constexpr int S = 2;
+  double K;
+  struct Aux {
+    double sn, cs;
+  };
+  CL_HD void aux(const double* x, Aux& a) const { cl_sincos(x[0], &a.sn, &a.cs); }
+  // W <- M W with fused_product order: C[i,j] = M[i,1] W[1,j], then fma(M[i,2], W[2,j], acc).
+  template <int K2>
+  CL_HD void tangent_apply(const Aux& a, Frame<2, K2>& W) const {
+    double c = K * a.cs;
+    double m11 = 1.0 + c;
+#pragma unroll
+    for (int j = 0; j < K2; ++j) {
+      double w0 = W.at(0, j), w1 = W.at(1, j);
+      W.at(0, j) = cl_fma(1.0, w1, m11 * w0);
+      W.at(1, j) = cl_fma(1.0, w1, c * w0);
+    }
+  }
+  CL_HD void forward(const Aux& a, double* x) const {
+    double p = x[1] + K * a.sn;
+    x[0] = x[0] + p;
+    x[1] = p;
+  }
+  // Displacement field F(x) - x = (p + K sin x, K sin x) at the current point (the direction
+  // the OFLI reducer projects out on a map).
+  CL_HD void field(const double* x, const Aux& a, double* f) const {
+    double ks = K * a.sn;
+    f[0] = x[1] + ks;
+    f[1] = ks;
+  }
+};
+
+// Wrapped standard map on the torus with exact winding counts (state x, p, m, n).
+struct WrappedStdMap {
+  static constexpr int D = 2;
+  static constexpr int S = 4;
+  double K;
+  struct Aux {
+    double sn, cs;
+  };
+  CL_HD void aux(const double* x, Aux& a) const { cl_sincos(x[0], &a.sn, &a.cs); }
+  template <int K2>
+  CL_HD void tangent_apply(const Aux& a, Frame<2, K2>& W) const {
+    double c = K * a.cs;
+    double m11 = 1.0 + c;
+#pragma unroll
+    for (int j = 0; j < K2; ++j) {
+      double w0 = W.at(0, j), w1 = W.at(1, j);
+      W.at(0, j) = cl_fma(1.0, w1, m11 * w0);
+      W.at(1, j) = cl_fma(1.0, w1, c * w0);
+    }
+  }
+  static CL_HD void reduce(double u, double tp, double* r, double* a) {
+    double q = cl_floor(u / tp);
+    double rr = u - tp * q;
+    if (rr < 0.0) {
+      rr += tp;
+      q -= 1.0;
+    }
+    if (rr >= tp) {
+      rr -= tp;
+      q += 1.0;
+    }
+    *r = rr;
+    *a = q;
+  }
+  CL_HD void forward(const Aux& au, double* z) const {
+    // 2 * (pi_hi + pi_lo) in Float64 is 2 pi rounded (wrapped_standard_map.jl:106, standard_map.jl:48)
+    const double tp = 2.0 * (3.141592653589793 + 1.2246467991473532e-16);
+    double p, a;
+    reduce(z[1] + K * au.sn, tp, &p, &a);
+    double n = z[3] + a;
+    double x, b;
+    reduce(z[0] + p, tp, &x, &b);
+    double m = z[2] + n + b;
+    const double lim = 9007199254740992.0;  // maxintfloat(Float64)
+    if (cl_fabs(m) < lim && cl_fabs(n) < lim) {
+      z[0] = x; z[1] = p; z[2] = m; z[3] = n;
+    } else {
+      double nan = __builtin_nan("");
+      z[0] = nan; z[1] = nan; z[2] = nan; z[3] = nan;
+    }
+  }
+  CL_HD void field(const double* x, const Aux& a, double* f) const {
+    double ks = K * a.sn;
+    f[0] = x[1] + ks;
+    f[1] = ks;
+  }
+};
+
+// ------------------------------------------------------------------------------ reducer states
+// Byte layouts of the reference's isbits states (Julia lays struct fields out with C alignment),
+// so a host can rebuild them and call the reference's reducer_final unchanged.
+template <int K>
+struct QRState {
+  CSum sums[K];
+  int64_t n;
+  double G_max, orth_max;
+  int64_t count_max;
+  int8_t code;
+  int8_t pad_[7];
+  int64_t column;
+  double value, threshold;
+  int64_t step;
+};
+
+struct BenettinState {
+  CSum sum;
+  int64_t n;
+  int8_t code;
+  int8_t pad_[7];
+  double value;
+  int64_t step;
+};
+
+struct MegnoSums {
+  CSum s, R1, R2, Q, Q2;
+};
+struct MEGNOState {
+  MegnoSums c1, c2;
+  double value;
+  int64_t n, K, step;
+  int8_t code, column;
+  int8_t pad_[6];
+};
+
+struct SALIState {
+  CSum s1, s2;
+  double sali, value;
+  int64_t n, step;
+  int8_t code, column;
+  int8_t pad_[6];
+};
+
+struct FLIState {
+  CSum s1, s2;
+  double f1, fB, value;
+  int64_t n, step, step1, stepB;
+  int8_t code, column, colB;
+  int8_t pad_[5];
+};
+
+template <int K>
+struct OFLIState {
+  CSum acc[K];
+  double g1, gB, inst, f1, fB, sin1, sinB, sinmin, value;
+  int64_t nminf, n, step, step1, stepB;
+  int8_t colB, code, column;
+  int8_t pad_[5];
+};
+
+template <int K>
+struct GALIState {
+  double s[K];
+  double c[K];
+  double gali[K - 1];
+  double value;
+  int64_t n, step;
+  int8_t code;
+  int8_t pad_[7];
+  int64_t column;
+};
+
+struct MonitorState {
+  double det_max, symp_max, orth_max, sum_max, pair_max, G_max, M_max;
+  int8_t code;
+  int8_t pad_[7];
+  double value, threshold;
+  int64_t step;
+};
+
+// ------------------------------------------------------------------------------ QR family
+enum QRAlg : int { QR_CGS = 1, QR_MGS = 2, QR_RGS = 3, QR_HOUSEHOLDER = 4, QR_AUGMENTED = 5, QR_GIVENS = 6 };
+enum : int8_t { QR_OK = 0, QR_PIVOT = 1, QR_PASSES = 2 };
+constexpr int MAX_PASSES = 4;
+
+template <int D, int K>
+struct QRFact {
+  Frame<D, K> Q;
+  double R[K * K];  // column-major
+  int count;
+  int8_t status;
+  int column;  // 1-based failing column, 0 if none
+};
+
+// Gram-Schmidt family (orthonormalize.jl:281-348).
+template <int ALG, int D, int K>
+CL_HD void orthonormalize_gs(const Frame<D, K>& W, QRFact<D, K>& f) {
+#pragma unroll
+  for (int i = 0; i < D * K; ++i) f.Q.w[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < K * K; ++i) f.R[i] = 0.0;
+  f.count = 1;
+  f.status = QR_OK;
+  f.column = 0;
+  bool done = false;
+#pragma unroll
+  for (int m = 0; m < K; ++m) {
+    if (!done) {
+      double u[D], r[K];
+#pragma unroll
+      for (int i = 0; i < D; ++i) u[i] = W.at(i, m);
+#pragma unroll
+      for (int i = 0; i < K; ++i) r[i] = 0.0;
+      int passes = 1;
+      if (ALG == QR_CGS || ALG == QR_MGS) {
+        double w0[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) w0[i] = u[i];
+#pragma unroll
+        for (int i = 0; i < m; ++i) {
+          const double* qi = f.Q.col(i);
+          double c = (ALG == QR_CGS) ? dot<D>(qi, w0) : dot<D>(qi, u);
+          r[i] = c;
+#pragma unroll
+          for (int l = 0; l < D; ++l) u[l] = u[l] - c * qi[l];
+        }
+      } else {  // RGS, eta = 1/sqrt(2), at most MAX_PASSES passes
+        const double eta = 1.0 / cl_sqrt(2.0);
+        double nold = safe_norm<D>(u);
+        passes = 0;
+        for (;;) {
+          passes += 1;
+          double vold[D];
+#pragma unroll
+          for (int i = 0; i < D; ++i) vold[i] = u[i];
+#pragma unroll
+          for (int i = 0; i < m; ++i) {
+            const double* qi = f.Q.col(i);
+            double c = dot<D>(qi, vold);
+            r[i] = r[i] + c;
+#pragma unroll
+            for (int l = 0; l < D; ++l) u[l] = u[l] - c * qi[l];
+          }
+          double nnew = safe_norm<D>(u);
+          if (nnew >= nold * eta || !(cl_isfinite(nnew) && nnew > 0.0)) break;
+          if (passes == MAX_PASSES) {
+            passes = MAX_PASSES + 1;
+            break;
+          }
+          nold = nnew;
+        }
+      }
+      double gamma = safe_norm<D>(u);
+#pragma unroll
+      for (int i = 0; i < K; ++i) f.R[m * K + i] = (i == m) ? gamma : r[i];
+      if (passes > MAX_PASSES) {
+        f.status = QR_PASSES;
+        f.column = m + 1;
+      } else if (!(cl_isfinite(gamma) && gamma > 0.0)) {
+        f.status = QR_PIVOT;
+        f.column = m + 1;
+      }
+      if (f.status != QR_OK) {
+        done = true;
+      } else {
+#pragma unroll
+        for (int l = 0; l < D; ++l) f.Q.at(l, m) = u[l] / gamma;
+        f.count = f.count > passes ? f.count : passes;
+      }
+    }
+  }
+}
+
+// (I - 2 u u') A for A with NC columns of D rows (householder.jl:30-33).
+template <int D, int NC>
+CL_HD void hh_reflect(double* A, const double* u) {
+#pragma unroll
+  for (int l = 0; l < NC; ++l) {
+    double w = dot<D>(u, A + l * D);
+#pragma unroll
+    for (int i = 0; i < D; ++i) A[l * D + i] = A[l * D + i] - (2.0 * u[i]) * w;
+  }
+}
+
+// Reflector sweep (householder.jl:20-48) over the first NREF columns of A (D x NC).
+template <int D, int NC, int NREF>
+CL_HD int hh_sweep(double* A, double* U) {
+  int count = 0;
+#pragma unroll
+  for (int j = 0; j < NREF; ++j) {
+    double a[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) a[i] = (i >= j) ? A[j * D + i] : 0.0;
+    double c = safe_norm<D>(a);
+    double rho;
+    if (cl_isfinite(c) && c > 0.0) {
+      double s = a[j] < 0.0 ? -1.0 : 1.0;
+      a[j] = a[j] + s * c;
+      double nv = safe_norm<D>(a);
+#pragma unroll
+      for (int i = 0; i < D; ++i) a[i] = a[i] / nv;
+      rho = -s * c;
+      hh_reflect<D, NC>(A, a);
+#pragma unroll
+      for (int i = 0; i < D; ++i) U[j * D + i] = a[i];
+      count += 1;
+    } else {
+      rho = c;
+    }
+    A[j * D + j] = rho;
+  }
+  return count;
+}
+
+// Sign fix and status (householder.jl:66-75); A has leading dimension LD.
+template <int D, int K, int LD>
+CL_HD void hh_finish(QRFact<D, K>& f, const double* A) {
+  double sg[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) sg[j] = A[j * LD + j] < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+#pragma unroll
+    for (int i = 0; i < K; ++i) f.R[j * K + i] = (i > j) ? 0.0 : sg[i] * A[j * LD + i];
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+#pragma unroll
+    for (int i = 0; i < D; ++i) f.Q.at(i, j) = f.Q.at(i, j) * sg[j];
+  f.status = QR_OK;
+  f.column = 0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    double p = f.R[j * K + j];
+    if (f.status == QR_OK && !(cl_isfinite(p) && p > 0.0)) {
+      f.status = QR_PIVOT;
+      f.column = j + 1;
+    }
+  }
+}
+
+template <int D, int K>
+CL_HD void orthonormalize_householder(const Frame<D, K>& W, QRFact<D, K>& f) {
+  constexpr int NREF = K < D ? K : K - 1;
+  double A[D * K], U[D * K];
+#pragma unroll
+  for (int i = 0; i < D * K; ++i) {
+    A[i] = W.w[i];
+    U[i] = 0.0;
+  }
+  f.count = hh_sweep<D, K, NREF>(A, U);
+  // Q = H_1 ... H_r applied to I[:, 1:K], reflectors in reverse order
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+#pragma unroll
+    for (int i = 0; i < D; ++i) f.Q.at(i, j) = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+  for (int j = NREF - 1; j >= 0; --j) hh_reflect<D, K>(f.Q.w, U + j * D);
+  hh_finish<D, K, D>(f, A);
+}
+
+template <int D, int K>
+CL_HD void orthonormalize_augmented(const Frame<D, K>& W, QRFact<D, K>& f) {
+  constexpr int NREF = K < D ? K : K - 1;
+  constexpr int NC = K + D;
+  double A[D * NC], U[D * K];
+#pragma unroll
+  for (int l = 0; l < D * NC; ++l) {
+    int c = l / D, r = l % D;
+    A[l] = (l < D * K) ? W.w[l] : ((r == c - K) ? 1.0 : 0.0);
+  }
+#pragma unroll
+  for (int i = 0; i < D * K; ++i) U[i] = 0.0;
+  f.count = hh_sweep<D, NC, NREF>(A, U);
+#pragma unroll
+  for (int b = 0; b < K; ++b)
+#pragma unroll
+    for (int a = 0; a < D; ++a) f.Q.at(a, b) = A[(K + a) * D + b];
+  hh_finish<D, K, D>(f, A);
+}
+
+// Givens rotations on the rows of [W, I_d] (householder.jl:94-123).
+template <int D, int K>
+CL_HD void orthonormalize_givens(const Frame<D, K>& W, QRFact<D, K>& f) {
+  constexpr int WD = K + D;
+  constexpr int J = K < D - 1 ? K : D - 1;
+  double rows[D][WD];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int l = 0; l < WD; ++l) rows[i][l] = (l < K) ? W.at(i, l) : ((l - K == i) ? 1.0 : 0.0);
+  int count = 0;
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+#pragma unroll
+    for (int i = D - 1; i >= j + 1; --i) {
+      double ab[2] = {rows[i - 1][j], rows[i][j]};
+      double rho = safe_norm<2>(ab);
+      if (cl_isfinite(rho) && rho > 0.0) {
+        double c = ab[0] / rho, s = ab[1] / rho;
+#pragma unroll
+        for (int l = 0; l < WD; ++l) {
+          double ra = rows[i - 1][l], rb = rows[i][l];
+          rows[i - 1][l] = c * ra + s * rb;
+          rows[i][l] = c * rb - s * ra;
+        }
+        rows[i - 1][j] = rho;
+        rows[i][j] = 0.0;
+        count += 1;
+      }
+    }
+  }
+  double R0[K * K];
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+#pragma unroll
+    for (int i = 0; i < K; ++i) R0[j * K + i] = (i > j) ? 0.0 : rows[i][j];
+#pragma unroll
+  for (int b = 0; b < K; ++b)
+#pragma unroll
+    for (int a = 0; a < D; ++a) f.Q.at(a, b) = rows[b][K + a];
+  f.count = count;
+  hh_finish<D, K, K>(f, R0);
+}
+
+template <int ALG, int D, int K>
+CL_HD void orthonormalize(const Frame<D, K>& W, QRFact<D, K>& f) {
+  if (ALG == QR_HOUSEHOLDER) orthonormalize_householder<D, K>(W, f);
+  else if (ALG == QR_AUGMENTED) orthonormalize_augmented<D, K>(W, f);
+  else if (ALG == QR_GIVENS) orthonormalize_givens<D, K>(W, f);
+  else orthonormalize_gs<ALG, D, K>(W, f);
+}
+
+// G_w = ||R||_F ||R^-1||_F by back substitution, rescaled when far from unit (orthonormalize.jl:112-127).
+template <int K>
+CL_HD double window_Gw(const double* R0) {
+  double R[K * K];
+  double m = cl_fabs(R0[0]);
+#pragma unroll
+  for (int i = 1; i < K * K; ++i) m = cl_jmax(m, cl_fabs(R0[i]));
+  bool scale = (m > 1e100 || m < 1e-100) && cl_isfinite(m) && m > 0.0;
+#pragma unroll
+  for (int i = 0; i < K * K; ++i) R[i] = scale ? R0[i] / m : R0[i];
+  double Ri[K * K];
+#pragma unroll
+  for (int i = 0; i < K * K; ++i) Ri[i] = 0.0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    Ri[j * K + j] = 1.0 / R[j * K + j];
+#pragma unroll
+    for (int i = j - 1; i >= 0; --i) {
+      double s = 0.0;
+#pragma unroll
+      for (int l = i + 1; l <= j; ++l) s += R[l * K + i] * Ri[j * K + l];
+      Ri[j * K + i] = -s / R[i * K + i];
+    }
+  }
+  double a = R[0] * R[0], b = Ri[0] * Ri[0];
+#pragma unroll
+  for (int i = 1; i < K * K; ++i) {
+    a = a + R[i] * R[i];
+    b = b + Ri[i] * Ri[i];
+  }
+  return cl_sqrt(a) * cl_sqrt(b);
+}
+
+template <int K>
+CL_HD double G_from_Gw(double Gw) {
+  if (K == 1) return 1.0;
+  if (K == 2) return (Gw + cl_sqrt(cl_jmax(Gw * Gw - 4.0, 0.0))) / 2.0;
+  return Gw;
+}
+
+// max |Q'Q - I| with each entry formed as Q[1,i] Q[1,j] then fma down the rows.
+template <int D, int K>
+CL_HD double orth_loss(const Frame<D, K>& Q) {
+  double mx = 0.0;
+  bool first = true;
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      double acc = Q.at(0, i) * Q.at(0, j);
+#pragma unroll
+      for (int l = 1; l < D; ++l) acc = cl_fma(Q.at(l, i), Q.at(l, j), acc);
+      double e = cl_fabs(i == j ? acc - 1.0 : acc - 0.0);
+      mx = first ? e : cl_jmax(mx, e);
+      first = false;
+    }
+  return mx;
+}
+
+template <int ALG, int D, int K>
+struct QRSpectrum {
+  double tol;  // orthogonality_tol
+  typedef QRState<K> State;
+  static constexpr bool checks_conditioning = (ALG == QR_CGS || ALG == QR_MGS || ALG == QR_RGS);
+  CL_HD void init(State& st) const {
+#pragma unroll
+    for (int i = 0; i < K; ++i) st.sums[i] = csum_zero();
+    st.n = 0;
+    st.G_max = 0.0;
+    st.orth_max = 0.0;
+    st.count_max = 0;
+    st.code = 0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) st.pad_[i] = 0;
+    st.column = 0;
+    st.value = 0.0;
+    st.threshold = 0.0;
+    st.step = 0;
+  }
+  CL_HD static void fail(State& st, int code, int64_t column, double value, double threshold, int64_t n) {
+    st.code = (int8_t)code;
+    st.column = column;
+    st.value = value;
+    st.threshold = threshold;
+    st.step = n;
+  }
+  CL_HD void step(State& st, int64_t n, Frame<D, K>& W) const {
+    if (st.code != 0) return;
+    QRFact<D, K> f;
+    orthonormalize<ALG, D, K>(W, f);
+    if (f.status != QR_OK) {
+      if (f.status == QR_PASSES) {
+        fail(st, 2, f.column, (double)(MAX_PASSES + 1), (double)MAX_PASSES, n);
+      } else {
+        double pivot = 0.0;  // R[column, column], selected without a dynamic register index
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+          if (j + 1 == f.column) pivot = f.R[j * K + j];
+        fail(st, 1, f.column, pivot, cl_isfinite(pivot) ? 0.0 : __builtin_inf(), n);
+      }
+      return;
+    }
+    double Gw = window_Gw<K>(f.R);
+    if (checks_conditioning && !(Gw * CL_EPS64 < 0.5)) {
+      fail(st, 3, 0, Gw * CL_EPS64, 0.5, n);
+      return;
+    }
+    double orth = K >= 2 ? orth_loss<D, K>(f.Q) : 0.0;
+    double g = (ALG == QR_CGS || ALG == QR_MGS) ? Gw : 1.0;
+    double thr = tol * CL_EPS64 * (double)K * g;
+    if (K >= 2 && !(orth <= thr)) {
+      fail(st, 4, 0, orth, thr, n);
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < K; ++i) st.sums[i] = csum_add(st.sums[i], cl_log(f.R[i * K + i]));
+    double G = G_from_Gw<K>(Gw);
+    st.n = n;
+    st.G_max = cl_jmax(st.G_max, G);
+    st.orth_max = cl_jmax(st.orth_max, orth);
+    st.count_max = st.count_max > f.count ? st.count_max : f.count;
+    st.column = 0;
+    W = f.Q;
+  }
+};
+
+// ------------------------------------------------------------------------------ Benettin (k = 1)
+template <int D>
+struct Benettin {
+  typedef BenettinState State;
+  CL_HD void init(State& st) const {
+    st.sum = csum_zero();
+    st.n = 0;
+    st.code = 0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) st.pad_[i] = 0;
+    st.value = 0.0;
+    st.step = 0;
+  }
+  CL_HD void step(State& st, int64_t n, Frame<D, 1>& W) const {
+    if (st.code != 0) return;
+    double alpha = safe_norm<D>(W.w);
+    if (!(cl_isfinite(alpha) && alpha > 0.0)) {
+      st.code = 1;
+      st.value = alpha;
+      st.step = n;
+      return;
+    }
+    st.sum = csum_add(st.sum, cl_log(alpha));
+    st.n = n;
+#pragma unroll
+    for (int i = 0; i < D; ++i) W.w[i] = W.w[i] / alpha;
+  }
+};
+
+// Column norms and logs shared by MEGNO, SALI and FLI at one handoff: the three reducers perform
+// the same operations on the same arriving columns, so their log sums are bitwise equal.
+struct ColLogs {
+  double a1, a2, l1, l2;
+  bool ok1, ok2;
+};
+template <int D>
+CL_HD ColLogs col_logs(const Frame<D, 2>& W) {
+  ColLogs c;
+  c.a1 = safe_norm<D>(W.col(0));
+  c.a2 = safe_norm<D>(W.col(1));
+  c.ok1 = cl_isfinite(c.a1) && c.a1 > 0.0;
+  c.ok2 = cl_isfinite(c.a2) && c.a2 > 0.0;
+  c.l1 = 0.0;
+  c.l2 = 0.0;
+  return c;
+}
+
+// ------------------------------------------------------------------------------ MEGNO (k = 2)
+CL_HD MegnoSums megno_zero() {
+  MegnoSums m;
+  m.s = csum_zero(); m.R1 = csum_zero(); m.R2 = csum_zero(); m.Q = csum_zero(); m.Q2 = csum_zero();
+  return m;
+}
+CL_HD MegnoSums megno_add(MegnoSums m, double a, int64_t n) {
+  double tn = (double)n;
+  CSum R1 = csum_add(m.R1, tn * a);
+  CSum R2 = csum_add(m.R2, tn * tn * a);
+  MegnoSums o;
+  o.s = csum_add(m.s, a);
+  o.R1 = R1;
+  o.R2 = R2;
+  o.Q = csum_add(m.Q, 2.0 * csum_total(R1) / tn);
+  o.Q2 = csum_add(m.Q2, 3.0 * csum_total(R2));
+  return o;
+}
+
+template <int D>
+struct MEGNO {
+  bool replace;
+  typedef MEGNOState State;
+  CL_HD void init(State& st) const {
+    st.c1 = megno_zero();
+    st.c2 = megno_zero();
+    st.value = 0.0;
+    st.n = 0; st.K = 0; st.step = 0;
+    st.code = 0; st.column = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) st.pad_[i] = 0;
+  }
+  CL_HD void step_logs(State& st, int64_t n, Frame<D, 2>& W, ColLogs& c) const {
+    if (st.code != 0) return;
+    if (!(c.ok1 && c.ok2)) {
+      st.column = c.ok1 ? 2 : 1;
+      st.value = c.ok1 ? c.a2 : c.a1;
+      st.step = n;
+      st.code = 1;
+      return;
+    }
+    st.c1 = megno_add(st.c1, c.l1, n);
+    st.c2 = megno_add(st.c2, c.l2, n);
+    st.n = n;
+    st.K = st.K + 1;
+    if (replace) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        W.at(i, 0) = W.at(i, 0) / c.a1;
+        W.at(i, 1) = W.at(i, 1) / c.a2;
+      }
+    }
+  }
+};
+
+// ------------------------------------------------------------------------------ SALI (k = 2)
+template <int D>
+struct SALI {
+  typedef SALIState State;
+  CL_HD void init(State& st) const {
+    st.s1 = csum_zero(); st.s2 = csum_zero();
+    st.sali = 0.0; st.value = 0.0;
+    st.n = 0; st.step = 0;
+    st.code = 0; st.column = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) st.pad_[i] = 0;
+  }
+  CL_HD void step_logs(State& st, int64_t n, Frame<D, 2>& W, ColLogs& c) const {
+    if (st.code != 0) return;
+    if (!(c.ok1 && c.ok2)) {
+      st.column = c.ok1 ? 2 : 1;
+      st.value = c.ok1 ? c.a2 : c.a1;
+      st.step = n;
+      st.code = 1;
+      return;
+    }
+    double u1[D], u2[D], dm[D], dp[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      u1[i] = W.at(i, 0) / c.a1;
+      u2[i] = W.at(i, 1) / c.a2;
+      dm[i] = u1[i] - u2[i];
+      dp[i] = u1[i] + u2[i];
+    }
+    double d_minus = safe_norm<D>(dm), d_plus = safe_norm<D>(dp);
+    st.s1 = csum_add(st.s1, c.l1);
+    st.s2 = csum_add(st.s2, c.l2);
+    st.sali = cl_jmin(d_minus, d_plus);
+    st.n = n;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      W.at(i, 0) = u1[i];
+      W.at(i, 1) = u2[i];
+    }
+  }
+};
+
+// ------------------------------------------------------------------------------ FLI (k = 2)
+template <int D>
+struct FLI {
+  bool replace;
+  typedef FLIState State;
+  CL_HD void init(State& st) const {
+    st.s1 = csum_zero(); st.s2 = csum_zero();
+    st.f1 = 0.0; st.fB = 0.0; st.value = 0.0;
+    st.n = 0; st.step = 0; st.step1 = 0; st.stepB = 0;
+    st.code = 0; st.column = 0; st.colB = 0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) st.pad_[i] = 0;
+  }
+  CL_HD void step_logs(State& st, int64_t n, Frame<D, 2>& W, ColLogs& c) const {
+    if (st.code != 0) return;
+    if (!(c.ok1 && c.ok2)) {
+      st.column = c.ok1 ? 2 : 1;
+      st.value = c.ok1 ? c.a2 : c.a1;
+      st.step = n;
+      st.code = 1;
+      return;
+    }
+    CSum s1 = csum_add(st.s1, c.l1), s2 = csum_add(st.s2, c.l2);
+    double S1 = csum_total(s1), S2 = csum_total(s2);
+    if (S1 > st.f1) { st.f1 = S1; st.step1 = n; }
+    if (S1 > st.fB) { st.fB = S1; st.stepB = n; st.colB = 1; }
+    if (S2 > st.fB) { st.fB = S2; st.stepB = n; st.colB = 2; }
+    st.s1 = s1;
+    st.s2 = s2;
+    st.n = n;
+    if (replace) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        W.at(i, 0) = W.at(i, 0) / c.a1;
+        W.at(i, 1) = W.at(i, 1) / c.a2;
+      }
+    }
+  }
+};
+
+// ------------------------------------------------------------------------------ OFLI
+// The direction is a field f(x) supplied by the system: the vector field of a flow, or on a map
+// the displacement F(x) - x (a definition this build adds; the reference defines OFLI for flows).
+template <int D, int K>
+struct OFLI {
+  bool replace;
+  typedef OFLIState<K> State;
+  CL_HD void init(State& st) const {
+#pragma unroll
+    for (int i = 0; i < K; ++i) st.acc[i] = csum_zero();
+    double ninf = -__builtin_inf(), inf = __builtin_inf();
+    st.g1 = ninf; st.gB = ninf; st.inst = ninf; st.f1 = 0.0; st.fB = 0.0;
+    st.sin1 = inf; st.sinB = inf; st.sinmin = inf; st.value = 0.0;
+    st.nminf = 0; st.n = 0; st.step = 0; st.step1 = 0; st.stepB = 0;
+    st.colB = 0; st.code = 0; st.column = 0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) st.pad_[i] = 0;
+  }
+  CL_HD void step(State& st, int64_t n, const double* fld, Frame<D, K>& W) const {
+    if (st.code != 0) return;
+    double fn = safe_norm<D>(fld);
+    if (!(cl_isfinite(fn) && fn > 0x1p-1022)) {
+      st.value = fn; st.step = n; st.code = 2; st.column = 0;
+      return;
+    }
+    double u[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) u[i] = fld[i] / fn;
+    double al[K], be[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const double* w = W.col(j);
+      al[j] = safe_norm<D>(w);
+      double dw = dot<D>(u, w);
+      double v[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) v[i] = w[i] - dw * u[i];
+      be[j] = safe_norm<D>(v);
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      if (st.code == 0 && !(cl_isfinite(al[j]) && al[j] > 0.0)) {
+        st.value = al[j]; st.step = n; st.code = 1; st.column = (int8_t)(j + 1);
+      }
+    }
+    if (st.code != 0) return;
+    double g1 = st.g1, gB = st.gB, inst = -__builtin_inf(), f1 = st.f1, fB = st.fB;
+    double sin1 = st.sin1, sinB = st.sinB, sinmin = st.sinmin;
+    int64_t step1 = st.step1, stepB = st.stepB, nminf = st.nminf;
+    int8_t colB = st.colB;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      CSum na = csum_add(st.acc[i], cl_log(al[i]));
+      double g = be[i] > 0.0 ? csum_total(st.acc[i]) + cl_log(be[i]) : -__builtin_inf();
+      double sn = be[i] / al[i], S = csum_total(na);
+      sinmin = cl_jmin(sinmin, sn);
+      nminf += be[i] > 0.0 ? 0 : 1;
+      inst = cl_jmax(inst, g);
+      if (g > gB) { gB = g; stepB = n; colB = (int8_t)(i + 1); sinB = sn; }
+      if (i == 0 && g > g1) { g1 = g; step1 = n; sin1 = sn; }
+      if (i == 0 && S > f1) f1 = S;
+      if (S > fB) fB = S;
+      st.acc[i] = na;
+    }
+    st.g1 = g1; st.gB = gB; st.inst = inst; st.f1 = f1; st.fB = fB;
+    st.sin1 = sin1; st.sinB = sinB; st.sinmin = sinmin;
+    st.nminf = nminf; st.n = n; st.step1 = step1; st.stepB = stepB; st.colB = colB;
+    if (replace) {
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+#pragma unroll
+        for (int i = 0; i < D; ++i) W.at(i, j) = W.at(i, j) / al[j];
+    }
+  }
+};
+
+// ------------------------------------------------------------------------------ GALI (2 <= K <= D)
+template <int D, int K>
+struct GALI {
+  typedef GALIState<K> State;
+  CL_HD void init(State& st) const {
+#pragma unroll
+    for (int i = 0; i < K; ++i) { st.s[i] = 0.0; st.c[i] = 0.0; }
+#pragma unroll
+    for (int i = 0; i < K - 1; ++i) st.gali[i] = 0.0;
+    st.value = 0.0; st.n = 0; st.step = 0; st.code = 0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) st.pad_[i] = 0;
+    st.column = 0;
+  }
+  CL_HD void step(State& st, int64_t n, Frame<D, K>& W) const {
+    if (st.code != 0) return;
+    double a[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) a[j] = safe_norm<D>(W.col(j));
+    int bad = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      if (bad == 0 && !(cl_isfinite(a[j]) && a[j] >= 0x1p-1022)) bad = j + 1;
+    if (bad != 0) {
+      double v = a[0];
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+        if (j + 1 == bad) v = a[j];
+      st.value = v; st.step = n; st.code = 1; st.column = bad;
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      CSum acc = csum_add(CSum{st.s[i], st.c[i]}, cl_log(a[i]));
+      st.s[i] = acc.s;
+      st.c[i] = acc.c;
+    }
+    Frame<D, K> U;
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+#pragma unroll
+      for (int i = 0; i < D; ++i) U.at(i, j) = W.at(i, j) / a[j];
+    W = U;
+    // nested MGS, each column orthogonalised twice (gali.jl:254-274)
+    double g = 1.0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+#pragma unroll
+      for (int pass = 0; pass < 2; ++pass)
+#pragma unroll
+        for (int p = 0; p < j; ++p) {
+          double c = 0.0;
+#pragma unroll
+          for (int i = 0; i < D; ++i) c += U.at(i, p) * U.at(i, j);
+#pragma unroll
+          for (int i = 0; i < D; ++i) U.at(i, j) -= c * U.at(i, p);
+        }
+      double r = safe_norm<D>(U.col(j));
+      double s = r == 0.0 ? 1.0 : r;
+#pragma unroll
+      for (int i = 0; i < D; ++i) U.at(i, j) /= s;
+      if (j > 0) {
+        g *= r;
+        st.gali[j - 1] = g;
+      }
+    }
+    st.n = n;
+  }
+};
+
+}  // namespace cl

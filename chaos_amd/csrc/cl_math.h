@@ -1,0 +1,318 @@
+/* cl_math.h - FP64 elementary functions and error-free transforms shared by the gfx950 kernels
+ * and by host code.
+ *
+ * Everything here is written with IEEE-754 binary64 operations that are correctly rounded on
+ * both x86-64 and CDNA4 (+, -, *, /, fma, sqrt, floor, rint, comparisons) plus exact integer
+ * arithmetic. Compiled with -ffp-contract=off on both sides, a call returns the same bits on
+ * the CPU and on the GPU, which is what lets a chaotic FP64 orbit computed by a kernel be
+ * compared bit for bit with the same orbit computed on the host.
+ *
+ * Methods (published, not taken from any vendor math library):
+ *  - sin/cos: argument reduction x = q*pi/2 + r with r carried as a double-double.
+ *    |x| < 2^50: Cody and Waite (1980) with pi/2 split over four doubles and the leading
+ *    product made exact with fma (the reduced value is exact by Sterbenz' lemma).
+ *    |x| >= 2^50: Payne and Hanek (1983): a 192-bit window of the bits of 2/pi selected by the
+ *    exponent of x, an exact 53x192-bit integer product, the quadrant from its integer bits.
+ *    Then the Taylor series of sin and cos (coefficients 1/n! rounded to double) on
+ *    |r| <= 1, with the first-order correction for the low part of r.
+ *  - log: x = 2^k m, m in [sqrt(2)/2, sqrt(2)), f = m - 1, s = f/(2+f), and the series
+ *    log(1+f) = 2 atanh(s) = f - f^2/2 + s (f^2/2 + R(s^2)), R(z) = sum_i 2 z^i/(2i+1),
+ *    with ln 2 split so that k*LN2_HI is exact.
+ *  - TwoSum (Knuth), TwoProd with fma (Dekker's product made exact by the fused multiply-add).
+ * The constants come from chaos_amd/csrc/gen_math_constants.py.
+ */
+#ifndef CL_MATH_H
+#define CL_MATH_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define CL_HD __host__ __device__ inline
+#else
+#include <math.h>
+#define CL_HD static inline
+#endif
+
+/* pi/2 = CL_PIO2_1 + CL_PIO2_2 + CL_PIO2_3 + CL_PIO2_4 (to about 2^-216) */
+#define CL_PIO2_1 0x1.921fb54442d18p+0
+#define CL_PIO2_2 0x1.1a62633145c07p-54
+#define CL_PIO2_3 -0x1.f1976b7ed8fbcp-110
+#define CL_PIO2_4 0x1.4cf98e804177dp-164
+#define CL_INV_PIO2 0x1.45f306dc9c883p-1
+/* ln 2 = CL_LN2_HI (42 significant bits) + CL_LN2_LO */
+#define CL_LN2_HI 0x1.62e42fefa3800p-1
+#define CL_LN2_LO 0x1.ef35793c76730p-45
+
+/* ---------------------------------------------------------------- bit helpers */
+CL_HD uint64_t cl_bits(double x) {
+  union { double d; uint64_t u; } v;
+  v.d = x;
+  return v.u;
+}
+CL_HD double cl_from_bits(uint64_t u) {
+  union { double d; uint64_t u; } v;
+  v.u = u;
+  return v.d;
+}
+CL_HD int cl_isfinite(double x) { return (cl_bits(x) & 0x7ff0000000000000ull) != 0x7ff0000000000000ull; }
+CL_HD int cl_isnan(double x) { return (cl_bits(x) & 0x7fffffffffffffffull) > 0x7ff0000000000000ull; }
+CL_HD int cl_signbit(double x) { return (int)(cl_bits(x) >> 63); }
+CL_HD double cl_fabs(double x) { return cl_from_bits(cl_bits(x) & 0x7fffffffffffffffull); }
+CL_HD double cl_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+CL_HD double cl_sqrt(double x) { return __builtin_sqrt(x); }
+CL_HD double cl_floor(double x) { return __builtin_floor(x); }
+CL_HD double cl_rint(double x) { return __builtin_rint(x); }
+/* 2^e for e in [-1022, 1023] */
+CL_HD double cl_pow2i(int e) { return cl_from_bits((uint64_t)(e + 1023) << 52); }
+
+CL_HD uint64_t cl_mulhi64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umul64hi(a, b);
+#else
+  return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+/* Julia's min/max on floats: NaN wins, -0.0 < +0.0. */
+CL_HD double cl_jmin(double x, double y) {
+  int takey = (y < x) || (cl_signbit(y) > cl_signbit(x));
+  return takey ? (cl_isnan(x) ? x : y) : (cl_isnan(y) ? y : x);
+}
+CL_HD double cl_jmax(double x, double y) {
+  int takey = (y > x) || (cl_signbit(y) < cl_signbit(x));
+  return takey ? (cl_isnan(x) ? x : y) : (cl_isnan(y) ? y : x);
+}
+
+/* ---------------------------------------------------------------- error-free transforms */
+/* s + e = a + b exactly (Knuth's TwoSum, no ordering assumption). */
+CL_HD void cl_two_sum(double a, double b, double *s, double *e) {
+  double t = a + b;
+  double bb = t - a;
+  *s = t;
+  *e = (a - (t - bb)) + (b - bb);
+}
+/* s + e = a + b exactly when |a| >= |b| or a == 0 (Dekker's FastTwoSum). */
+CL_HD void cl_fast_two_sum(double a, double b, double *s, double *e) {
+  double t = a + b;
+  *s = t;
+  *e = b - (t - a);
+}
+/* p + e = a * b exactly (no overflow/underflow). */
+CL_HD void cl_two_prod(double a, double b, double *p, double *e) {
+  double t = a * b;
+  *p = t;
+  *e = cl_fma(a, b, -t);
+}
+
+/* ---------------------------------------------------------------- argument reduction */
+#define CL_TWO_OVER_PI_WORDS \
+  0xa2f9836e4e441529ull, 0xfc2757d1f534ddc0ull, 0xdb6295993c439041ull, \
+  0xfe5163abdebbc561ull, 0xb7246e3a424dd2e0ull, 0x06492eea09d1921cull, \
+  0xfe1deb1cb129a73eull, 0xe88235f52ebb4484ull, 0xe99c7026b45f7e41ull, \
+  0x3991d639835339f4ull, 0x9c845f8bbdf9283bull, 0x1ff897ffde05980full, \
+  0xef2f118b5a0a6d1full, 0x6d367ecf27cb09b7ull, 0x4f463f669e5fea2dull, \
+  0x7527bac7ebe5f17bull, 0x3d0739f78a5292eaull, 0x6bfb5fb11f8d5d08ull, \
+  0x56033046fc7b6babull, 0xf0cfbc209af4361dull, 0xa9e391615ee61b08ull, \
+  0x6599855f14a06840ull, 0x8dffd8804d732731ull, 0x06061556ca73a8c9ull, \
+
+/* The table lives in constant memory for the device pass and in ordinary read-only memory for
+ * the host pass (a __constant__ variable read from host code would be its uninitialised shadow). */
+#if defined(__HIPCC__)
+__device__ __constant__ uint64_t cl_two_over_pi_dev[24] = {CL_TWO_OVER_PI_WORDS};
+static const uint64_t cl_two_over_pi_host[24] = {CL_TWO_OVER_PI_WORDS};
+CL_HD uint64_t cl_two_over_pi(int i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return cl_two_over_pi_dev[i];
+#else
+  return cl_two_over_pi_host[i];
+#endif
+}
+#else
+static const uint64_t cl_two_over_pi_host[24] = {CL_TWO_OVER_PI_WORDS};
+CL_HD uint64_t cl_two_over_pi(int i) { return cl_two_over_pi_host[i]; }
+#endif
+
+/* 64 bits of the 256-bit number p[0..3] (p[0] least significant) starting at bit `pos`. */
+CL_HD uint64_t cl_get64(const uint64_t *p, int pos) {
+  int w = pos >> 6, sh = pos & 63;
+  uint64_t lo = p[w];
+  uint64_t hi = (w + 1 < 4) ? p[w + 1] : 0;
+  return sh == 0 ? lo : ((lo >> sh) | (hi << (64 - sh)));
+}
+
+/* Payne-Hanek reduction of a finite |x| >= 2^50: returns q mod 4 and r = x - q pi/2 as rh + rl,
+ * |r| <= pi/4. */
+CL_HD int cl_rem_pio2_large(double x, double *rh, double *rl) {
+  uint64_t b = cl_bits(x);
+  int neg = (int)(b >> 63);
+  int e = (int)((b >> 52) & 0x7ff) - 1075;             /* |x| = M 2^e, e >= -2 */
+  uint64_t M = (b & 0x000fffffffffffffull) | 0x0010000000000000ull;
+  int j0 = e - 1 > 1 ? e - 1 : 1;                       /* first bit of 2/pi that matters mod 4 */
+  int sft = j0 + 191 - e;                              /* value = (M * window) 2^-sft */
+  int bit = j0 - 1, w = bit >> 6, sh = bit & 63;
+  uint64_t win[3];                                      /* win[2] most significant */
+  for (int i = 0; i < 3; ++i) {
+    uint64_t a = cl_two_over_pi(w + i), c = cl_two_over_pi(w + i + 1);
+    win[2 - i] = sh == 0 ? a : ((a << sh) | (c >> (64 - sh)));
+  }
+  uint64_t l0 = M * win[0], h0 = cl_mulhi64(M, win[0]);
+  uint64_t l1 = M * win[1], h1 = cl_mulhi64(M, win[1]);
+  uint64_t l2 = M * win[2], h2 = cl_mulhi64(M, win[2]);
+  uint64_t p[4];
+  p[0] = l0;
+  p[1] = h0 + l1;
+  uint64_t c1 = p[1] < h0;
+  uint64_t t2 = h1 + l2;
+  uint64_t c2 = t2 < h1;
+  p[2] = t2 + c1;
+  c2 += p[2] < t2;
+  p[3] = h2 + c2;
+  int q = (int)(cl_get64(p, sft) & 3);
+  uint64_t f1 = cl_get64(p, sft - 64), f0 = cl_get64(p, sft - 128);
+  int negf = 0;
+  if (f1 >> 63) { /* fraction >= 1/2: take the next quadrant, fraction - 1 */
+    q = (q + 1) & 3;
+    negf = 1;
+    f0 = ~f0 + 1;
+    f1 = ~f1 + (f0 == 0);
+  }
+  int scale = -64;
+  if (f1 == 0) {
+    f1 = f0;
+    f0 = 0;
+    scale -= 64;
+  }
+  double hi = 0.0, lo = 0.0;
+  if (f1 != 0) {
+    int lz = __builtin_clzll(f1);
+    if (lz) {
+      f1 = (f1 << lz) | (f0 >> (64 - lz));
+      f0 <<= lz;
+    }
+    scale -= lz; /* fraction = (f1 . f0) 2^(scale) with the top bit of f1 set */
+    hi = (double)(f1 >> 11) * cl_pow2i(scale + 11);
+    lo = (double)(((f1 & 0x7ffull) << 42) | (f0 >> 22)) * cl_pow2i(scale - 42);
+  }
+  /* r = fraction * pi/2 as a double-double */
+  double ph, pe;
+  cl_two_prod(hi, CL_PIO2_1, &ph, &pe);
+  pe = pe + (hi * CL_PIO2_2 + lo * CL_PIO2_1);
+  double s, t;
+  cl_fast_two_sum(ph, pe, &s, &t);
+  if (negf) { s = -s; t = -t; }
+  if (neg) { s = -s; t = -t; q = (4 - q) & 3; }
+  *rh = s;
+  *rl = t;
+  return q;
+}
+
+/* x = q pi/2 + (rh + rl) for finite x; returns q mod 4. */
+CL_HD int cl_rem_pio2(double x, double *rh, double *rl) {
+  double ax = cl_fabs(x);
+  if (ax <= 0x1.921fb54442d18p-1) { /* pi/4 */
+    *rh = x;
+    *rl = 0.0;
+    return 0;
+  }
+  if (ax >= 0x1p50) return cl_rem_pio2_large(x, rh, rl);
+  double qd = cl_rint(x * CL_INV_PIO2);
+  double p1, e1;
+  cl_two_prod(qd, CL_PIO2_1, &p1, &e1);
+  double t = x - p1; /* exact (Sterbenz) */
+  double s, se;
+  cl_two_sum(t, -e1, &s, &se);
+  double p2, e2;
+  cl_two_prod(qd, CL_PIO2_2, &p2, &e2);
+  double s2, se2;
+  cl_two_sum(s, -p2, &s2, &se2);
+  double lo = ((se + se2) - e2) - qd * CL_PIO2_3;
+  lo = lo - qd * CL_PIO2_4;
+  cl_two_sum(s2, lo, rh, rl);
+  int64_t qi = (int64_t)qd;
+  return (int)(qi & 3);
+}
+
+/* ---------------------------------------------------------------- sin / cos kernels, |r| <= 1 */
+CL_HD double cl_sin_kernel(double rh, double rl) {
+  double z = rh * rh;
+  double p = -0x1.ae64567f544e4p-26 + z * (0x1.6124613a86d09p-33 + z * (-0x1.ae7f3e733b81fp-41 +
+             z * (0x1.952c77030ad4ap-49 + z * (-0x1.2f49b46814157p-57 + z * 0x1.71b8ef6dcf572p-66))));
+  p = -0x1.5555555555555p-3 + z * (0x1.1111111111111p-7 + z * (-0x1.a01a01a01a01ap-13 +
+      z * (0x1.71de3a556c734p-19 + z * p)));
+  double v = z * rh;
+  return rh + (rl * (1.0 - 0.5 * z) + v * p);
+}
+
+CL_HD double cl_cos_kernel(double rh, double rl) {
+  double z = rh * rh;
+  double p = 0x1.1eed8eff8d898p-29 + z * (-0x1.93974a8c07c9dp-37 + z * (0x1.ae7f3e733b81fp-45 +
+             z * (-0x1.6827863b97d97p-53 + z * (0x1.e542ba4020225p-62 + z * -0x1.0ce396db7f853p-70))));
+  p = 0x1.5555555555555p-5 + z * (-0x1.6c16c16c16c17p-10 + z * (0x1.a01a01a01a01ap-16 +
+      z * (-0x1.27e4fb7789f5cp-22 + z * p)));
+  double hz = 0.5 * z;
+  double w = 1.0 - hz;
+  return w + (((1.0 - w) - hz) + (z * z * p - rh * rl));
+}
+
+/* sin and cos of one argument (one reduction). Non-finite x gives x - x (NaN). */
+CL_HD void cl_sincos(double x, double *sn, double *cs) {
+  if (!cl_isfinite(x)) {
+    *sn = x - x;
+    *cs = x - x;
+    return;
+  }
+  double rh, rl;
+  int q = cl_rem_pio2(x, &rh, &rl);
+  double s = cl_sin_kernel(rh, rl), c = cl_cos_kernel(rh, rl);
+  switch (q) {
+    case 0: *sn = s; *cs = c; break;
+    case 1: *sn = c; *cs = -s; break;
+    case 2: *sn = -s; *cs = -c; break;
+    default: *sn = -c; *cs = s; break;
+  }
+}
+CL_HD double cl_sin(double x) {
+  double s, c;
+  cl_sincos(x, &s, &c);
+  return s;
+}
+CL_HD double cl_cos(double x) {
+  double s, c;
+  cl_sincos(x, &s, &c);
+  return c;
+}
+
+/* ---------------------------------------------------------------- log */
+CL_HD double cl_log(double x) {
+  if (cl_isnan(x)) return x;
+  if (x < 0.0) return __builtin_nan("");
+  if (x == 0.0) return -__builtin_inf();
+  if (!cl_isfinite(x)) return x;
+  int k = 0;
+  if (x < 0x1p-1022) { /* subnormal */
+    x = x * 0x1p54;
+    k = -54;
+  }
+  uint64_t b = cl_bits(x);
+  k += (int)(b >> 52) - 1023;
+  uint64_t mant = b & 0x000fffffffffffffull;
+  double m = cl_from_bits(mant | 0x3ff0000000000000ull); /* [1, 2) */
+  if (m > 0x1.6a09e667f3bcdp+0) {                         /* sqrt(2) */
+    m = cl_from_bits(mant | 0x3fe0000000000000ull);       /* [0.7, 1) */
+    k += 1;
+  }
+  double f = m - 1.0;
+  double s = f / (2.0 + f);
+  double z = s * s;
+  double R = z * (0x1.5555555555555p-1 + z * (0x1.999999999999ap-2 + z * (0x1.2492492492492p-2 +
+             z * (0x1.c71c71c71c71cp-3 + z * (0x1.745d1745d1746p-3 + z * (0x1.3b13b13b13b14p-3 +
+             z * (0x1.1111111111111p-3 + z * (0x1.e1e1e1e1e1e1ep-4 + z * (0x1.af286bca1af28p-4 +
+             z * (0x1.8618618618618p-4 + z * (0x1.642c8590b2164p-4 + z * 0x1.47ae147ae147bp-4)))))))))));
+  double hfsq = 0.5 * f * f;
+  double dk = (double)k;
+  double lo = dk * CL_LN2_LO + s * (hfsq + R);
+  return dk * CL_LN2_HI + (f - (hfsq - lo));
+}
+
+#endif /* CL_MATH_H */

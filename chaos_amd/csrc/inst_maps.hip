@@ -1,0 +1,60 @@
+// inst_maps.hip - kernel instances for the two-dimensional maps (standard map, wrapped standard map).
+#include "registry.hpp"
+
+namespace cl {
+
+template <class R, int FR = 0>
+using S_ = Slot<R, FR>;
+
+// k = 2
+typedef Nil C_none;
+#define QR2(A, FR) S_<QRSpectrum<A, 2, 2>, FR>
+#define QR1(A) S_<QRSpectrum<A, 2, 1>, 0>
+typedef Cons<QR2(QR_CGS, 0), Nil> C_cgs;
+typedef Cons<QR2(QR_MGS, 0), Nil> C_mgs;
+typedef Cons<QR2(QR_RGS, 0), Nil> C_rgs;
+typedef Cons<QR2(QR_HOUSEHOLDER, 0), Nil> C_hh;
+typedef Cons<QR2(QR_AUGMENTED, 0), Nil> C_aug;
+typedef Cons<QR2(QR_GIVENS, 0), Nil> C_giv;
+typedef Cons<S_<MEGNO<2>>, Nil> C_megno;
+typedef Cons<S_<SALI<2>>, Nil> C_sali;
+typedef Cons<S_<FLI<2>>, Nil> C_fli;
+typedef Cons<S_<OFLI<2, 2>>, Nil> C_ofli;
+typedef Cons<S_<GALI<2, 2>>, Nil> C_gali;
+typedef Cons<S_<MEGNO<2>>, Cons<S_<SALI<2>>, Nil>> C_megno_sali;
+typedef Cons<S_<FLI<2>>, Cons<S_<SALI<2>>, Nil>> C_fli_sali;
+typedef Cons<S_<MEGNO<2>>, Cons<S_<FLI<2>>, Cons<S_<SALI<2>>, Nil>>> C_mfs;
+typedef Cons<S_<MEGNO<2>>, Cons<S_<FLI<2>>, Cons<S_<OFLI<2, 2>>, Cons<S_<SALI<2>>, Nil>>>> C_mfos;
+// the fused indicator pass: indicators on the first frame, a QR spectrum on the second
+typedef Cons<S_<MEGNO<2>>, Cons<S_<FLI<2>>, Cons<S_<OFLI<2, 2>>, Cons<S_<SALI<2>>, Cons<QR2(QR_HOUSEHOLDER, 1), Nil>>>>> C_mfos_hh;
+typedef Cons<S_<MEGNO<2>>, Cons<S_<FLI<2>>, Cons<S_<OFLI<2, 2>>, Cons<S_<SALI<2>>, Cons<QR2(QR_MGS, 1), Nil>>>>> C_mfos_mgs;
+typedef Cons<S_<MEGNO<2>>, Cons<S_<FLI<2>>, Cons<S_<SALI<2>>, Cons<QR2(QR_HOUSEHOLDER, 1), Nil>>>> C_mfs_hh;
+// k = 1
+typedef Cons<S_<Benettin<2>>, Nil> C_ben;
+typedef Cons<QR1(QR_CGS), Nil> C1_cgs;
+typedef Cons<QR1(QR_MGS), Nil> C1_mgs;
+typedef Cons<QR1(QR_RGS), Nil> C1_rgs;
+typedef Cons<QR1(QR_HOUSEHOLDER), Nil> C1_hh;
+typedef Cons<QR1(QR_AUGMENTED), Nil> C1_aug;
+typedef Cons<QR1(QR_GIVENS), Nil> C1_giv;
+typedef Cons<S_<OFLI<2, 1>>, Nil> C1_ofli;
+
+#define ADD2(SK, SYS, CH, NF) if (*n < cap) out[(*n)++] = make_instance<SK, SYS, 2, NF, CH>();
+#define ADD1(SK, SYS, CH) if (*n < cap) out[(*n)++] = make_instance<SK, SYS, 1, 1, CH>();
+#define ALL_FOR(SK, SYS)                                                                     \
+  ADD2(SK, SYS, C_none, 1) ADD2(SK, SYS, C_cgs, 1) ADD2(SK, SYS, C_mgs, 1) ADD2(SK, SYS, C_rgs, 1) \
+  ADD2(SK, SYS, C_hh, 1) ADD2(SK, SYS, C_aug, 1) ADD2(SK, SYS, C_giv, 1) ADD2(SK, SYS, C_megno, 1) \
+  ADD2(SK, SYS, C_sali, 1) ADD2(SK, SYS, C_fli, 1) ADD2(SK, SYS, C_ofli, 1) ADD2(SK, SYS, C_gali, 1) \
+  ADD2(SK, SYS, C_megno_sali, 1) ADD2(SK, SYS, C_fli_sali, 1) ADD2(SK, SYS, C_mfs, 1)           \
+  ADD2(SK, SYS, C_mfos, 1) ADD2(SK, SYS, C_mfos_hh, 2) ADD2(SK, SYS, C_mfos_mgs, 2)            \
+  ADD2(SK, SYS, C_mfs_hh, 2)                                                                   \
+  ADD1(SK, SYS, C_none) ADD1(SK, SYS, C_ben) ADD1(SK, SYS, C1_cgs) ADD1(SK, SYS, C1_mgs)        \
+  ADD1(SK, SYS, C1_rgs) ADD1(SK, SYS, C1_hh) ADD1(SK, SYS, C1_aug) ADD1(SK, SYS, C1_giv)        \
+  ADD1(SK, SYS, C1_ofli)
+
+void register_instances_map(Instance* out, int* n, int cap) {
+  ALL_FOR(CL_SYS_STANDARD_MAP, StdMap)
+  ALL_FOR(CL_SYS_WRAPPED_STANDARD_MAP, WrappedStdMap)
+}
+
+}  // namespace cl

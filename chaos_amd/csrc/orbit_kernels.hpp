@@ -1,0 +1,262 @@
+// orbit_kernels.hpp - the fused ensemble kernel: one orbit per lane, structure-of-arrays HBM
+// layout, state/frames/reducer states in VGPRs for the whole launch.
+//
+// Per step (src/Tangent/orbit.jl:48-58): the tangent at the OLD point is applied to every frame
+// with the explicit-fma product, then the map advances; sin and cos of the new point are formed
+// once and serve the next Jacobian, the next step and any reducer that reads the field. Every
+// `tau` steps the handoff checks finiteness (orbit.jl:171-172) and runs the reducer chain in order
+// (orbit.jl:139-145), each reducer seeing the frame as replaced by the reducers before it.
+#pragma once
+#include <stdint.h>
+#include "cl_core.hpp"
+#include "../../include/chaoslab_hip.h"
+
+namespace cl {
+
+struct KArgs {
+  double* x;     // [S][m]
+  double* W0;    // [D*K][m]
+  double* W1;    // second frame or null
+  uint64_t* st;  // [words][m]
+  int8_t* code;  // [m]
+  int64_t* hstep;
+  int64_t m, n0, steps, N;
+  int tau;
+};
+
+// Per-orbit context the reducers read: the point, the carried sin/cos, the frames and a cache of
+// the column log-norms of each frame (valid until a reducer replaces that frame).
+template <class Sys, int K, int NF>
+struct OrbitCtx {
+  Sys sys;
+  double x[Sys::S];
+  typename Sys::Aux aux;
+  Frame<Sys::D, K> W[NF];
+  bool logs_valid[NF];
+  ColLogs logs[NF];
+};
+
+// ------------------------------------------------------------------------------ chain slots
+template <class R, int FR>
+struct Slot {
+  R r;
+  typename R::State st;
+  static constexpr int frame = FR;
+};
+
+struct Nil {};
+template <class H, class T>
+struct Cons {
+  H head;
+  T tail;
+};
+
+template <int D, int K, int NF, class Ctx>
+CL_HD void ensure_logs(Ctx& c, int fr) {
+  if (!c.logs_valid[fr]) {
+    ColLogs L = col_logs<D>(c.W[fr]);
+    if (L.ok1 && L.ok2) {
+      L.l1 = cl_log(L.a1);
+      L.l2 = cl_log(L.a2);
+    }
+    c.logs[fr] = L;
+    c.logs_valid[fr] = true;
+  }
+}
+
+// One reducer at a handoff, by reducer type.
+template <int ALG, int D, int K, int FR, class Ctx>
+CL_HD void slot_step(Slot<QRSpectrum<ALG, D, K>, FR>& s, int64_t n, Ctx& c) {
+  s.r.step(s.st, n, c.W[FR]);
+  c.logs_valid[FR] = false;
+}
+template <int D, int FR, class Ctx>
+CL_HD void slot_step(Slot<Benettin<D>, FR>& s, int64_t n, Ctx& c) {
+  s.r.step(s.st, n, c.W[FR]);
+  c.logs_valid[FR] = false;
+}
+template <int D, int FR, class Ctx>
+CL_HD void slot_step(Slot<MEGNO<D>, FR>& s, int64_t n, Ctx& c) {
+  ensure_logs<D, 2, 0>(c, FR);
+  s.r.step_logs(s.st, n, c.W[FR], c.logs[FR]);
+  if (s.r.replace) c.logs_valid[FR] = false;
+}
+template <int D, int FR, class Ctx>
+CL_HD void slot_step(Slot<FLI<D>, FR>& s, int64_t n, Ctx& c) {
+  ensure_logs<D, 2, 0>(c, FR);
+  s.r.step_logs(s.st, n, c.W[FR], c.logs[FR]);
+  if (s.r.replace) c.logs_valid[FR] = false;
+}
+template <int D, int FR, class Ctx>
+CL_HD void slot_step(Slot<SALI<D>, FR>& s, int64_t n, Ctx& c) {
+  ensure_logs<D, 2, 0>(c, FR);
+  s.r.step_logs(s.st, n, c.W[FR], c.logs[FR]);
+  c.logs_valid[FR] = false;
+}
+template <int D, int K, int FR, class Ctx>
+CL_HD void slot_step(Slot<OFLI<D, K>, FR>& s, int64_t n, Ctx& c) {
+  double f[D];
+  c.sys.field(c.x, c.aux, f);
+  s.r.step(s.st, n, f, c.W[FR]);
+  if (s.r.replace) c.logs_valid[FR] = false;
+}
+template <int D, int K, int FR, class Ctx>
+CL_HD void slot_step(Slot<GALI<D, K>, FR>& s, int64_t n, Ctx& c) {
+  s.r.step(s.st, n, c.W[FR]);
+  c.logs_valid[FR] = false;
+}
+
+// Reducer parameters from the descriptor.
+template <int ALG, int D, int K>
+CL_HD void slot_setup(QRSpectrum<ALG, D, K>& r, const cl_reducer& d) { r.tol = d.tol; }
+template <int D>
+CL_HD void slot_setup(Benettin<D>&, const cl_reducer&) {}
+template <int D>
+CL_HD void slot_setup(MEGNO<D>& r, const cl_reducer& d) { r.replace = d.replace != 0; }
+template <int D>
+CL_HD void slot_setup(FLI<D>& r, const cl_reducer& d) { r.replace = d.replace != 0; }
+template <int D>
+CL_HD void slot_setup(SALI<D>&, const cl_reducer&) {}
+template <int D, int K>
+CL_HD void slot_setup(OFLI<D, K>& r, const cl_reducer& d) { r.replace = d.replace != 0; }
+template <int D, int K>
+CL_HD void slot_setup(GALI<D, K>&, const cl_reducer&) {}
+
+// ------------------------------------------------------------------------------ chain recursion
+CL_HD void chain_setup(Nil&, const cl_chain&, int) {}
+template <class H, class T>
+CL_HD void chain_setup(Cons<H, T>& c, const cl_chain& d, int i) {
+  slot_setup(c.head.r, d.r[i]);
+  chain_setup(c.tail, d, i + 1);
+}
+CL_HD void chain_init(Nil&) {}
+template <class H, class T>
+CL_HD void chain_init(Cons<H, T>& c) {
+  c.head.r.init(c.head.st);
+  chain_init(c.tail);
+}
+template <class Ctx>
+CL_HD void chain_step(Nil&, int64_t, Ctx&) {}
+template <class H, class T, class Ctx>
+CL_HD void chain_step(Cons<H, T>& c, int64_t n, Ctx& ctx) {
+  slot_step(c.head, n, ctx);
+  chain_step(c.tail, n, ctx);
+}
+CL_HD bool chain_failed(const Nil&) { return false; }
+template <class H, class T>
+CL_HD bool chain_failed(const Cons<H, T>& c) {
+  return c.head.st.code != 0 || chain_failed(c.tail);
+}
+template <class C>
+struct ChainWords;
+template <>
+struct ChainWords<Nil> {
+  static constexpr int value = 0;
+};
+template <class H, class T>
+struct ChainWords<Cons<H, T>> {
+  static constexpr int value = (int)(sizeof(decltype(H::st)) / 8) + ChainWords<T>::value;
+};
+
+template <class S>
+CL_HD void state_load(S& s, const uint64_t* st, int64_t m, int64_t i, int w0) {
+  constexpr int NW = sizeof(S) / 8;
+  uint64_t tmp[NW];
+#pragma unroll
+  for (int j = 0; j < NW; ++j) tmp[j] = st[(int64_t)(w0 + j) * m + i];
+  __builtin_memcpy(&s, tmp, sizeof(S));
+}
+template <class S>
+CL_HD void state_store(const S& s, uint64_t* st, int64_t m, int64_t i, int w0) {
+  constexpr int NW = sizeof(S) / 8;
+  uint64_t tmp[NW];
+  __builtin_memcpy(tmp, &s, sizeof(S));
+#pragma unroll
+  for (int j = 0; j < NW; ++j) st[(int64_t)(w0 + j) * m + i] = tmp[j];
+}
+CL_HD int chain_load(Nil&, const uint64_t*, int64_t, int64_t, int w) { return w; }
+template <class H, class T>
+CL_HD int chain_load(Cons<H, T>& c, const uint64_t* st, int64_t m, int64_t i, int w) {
+  state_load(c.head.st, st, m, i, w);
+  return chain_load(c.tail, st, m, i, w + (int)(sizeof(c.head.st) / 8));
+}
+CL_HD int chain_store(const Nil&, uint64_t*, int64_t, int64_t, int w) { return w; }
+template <class H, class T>
+CL_HD int chain_store(const Cons<H, T>& c, uint64_t* st, int64_t m, int64_t i, int w) {
+  state_store(c.head.st, st, m, i, w);
+  return chain_store(c.tail, st, m, i, w + (int)(sizeof(c.head.st) / 8));
+}
+template <class Sys, int K, int NF>
+CL_HD int8_t frame_status(const OrbitCtx<Sys, K, NF>& c) {
+  bool xf = true;
+#pragma unroll
+  for (int i = 0; i < Sys::S; ++i) xf = xf && cl_isfinite(c.x[i]);
+  if (!xf) return STATUS_STATE;
+  bool wf = true;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) wf = wf && frame_finite(c.W[f]);
+  return wf ? STATUS_OK : STATUS_W;
+}
+
+// ------------------------------------------------------------------------------ the kernel
+template <class Sys, int K, int NF, class Chain>
+__global__ void __launch_bounds__(256) orbit_kernel(KArgs a, Sys sys, Chain chain0) {
+  constexpr int D = Sys::D;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.m) return;
+  if (!(a.n0 == 0 || a.code[i] == STATUS_OK)) return;
+  OrbitCtx<Sys, K, NF> c;
+  c.sys = sys;
+#pragma unroll
+  for (int j = 0; j < Sys::S; ++j) c.x[j] = a.x[(int64_t)j * a.m + i];
+#pragma unroll
+  for (int e = 0; e < D * K; ++e) c.W[0].w[e] = a.W0[(int64_t)e * a.m + i];
+  if (NF > 1) {
+#pragma unroll
+    for (int e = 0; e < D * K; ++e) c.W[NF - 1].w[e] = a.W1[(int64_t)e * a.m + i];
+  }
+  Chain ch = chain0;
+  if (a.n0 == 0) chain_init(ch);
+  else chain_load(ch, a.st, a.m, i, 0);
+  int64_t n = a.n0;
+  int8_t code = STATUS_OK;
+  bool early = false;
+  sys.aux(c.x, c.aux);
+  const int64_t nh = a.steps / a.tau;
+  for (int64_t h = 0; h < nh; ++h) {
+    for (int t = 0; t < a.tau; ++t) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) sys.tangent_apply(c.aux, c.W[f]);
+      sys.forward(c.aux, c.x);
+      sys.aux(c.x, c.aux);
+    }
+    n += a.tau;
+    code = frame_status(c);
+    if (code != STATUS_OK) {
+      early = true;
+      break;
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) c.logs_valid[f] = false;
+    chain_step(ch, n, c);
+    if (chain_failed(ch)) {
+      code = STATUS_REDUCER;
+      early = true;
+      break;
+    }
+  }
+  if (!early) code = (n == a.N) ? frame_status(c) : STATUS_OK;
+#pragma unroll
+  for (int j = 0; j < Sys::S; ++j) a.x[(int64_t)j * a.m + i] = c.x[j];
+#pragma unroll
+  for (int e = 0; e < D * K; ++e) a.W0[(int64_t)e * a.m + i] = c.W[0].w[e];
+  if (NF > 1) {
+#pragma unroll
+    for (int e = 0; e < D * K; ++e) a.W1[(int64_t)e * a.m + i] = c.W[NF - 1].w[e];
+  }
+  chain_store(ch, a.st, a.m, i, 0);
+  a.code[i] = code;
+  a.hstep[i] = n;
+}
+
+}  // namespace cl

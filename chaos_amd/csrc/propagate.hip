@@ -1,0 +1,369 @@
+// propagate.hip - the C ABI of libchaoslab_hip.so: device probe, contexts, the chunk entry points
+// that replace `_orbit_kernel!` / `_run_chunk!` (src/Tangent/propagate.jl:32-50,92-119), seeded frames
+// and the elementary-function test entry points.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <mutex>
+#include "registry.hpp"
+#include "frames.hpp"
+
+using namespace cl;
+
+struct cl_ctx {
+  int device;
+  hipStream_t stream;
+  char err[256];
+  // grown-on-demand device buffers of the host entry point
+  void* buf[12];
+  size_t cap[12];
+};
+
+static char g_create_err[256] = "";
+
+// ------------------------------------------------------------------------------ instance table
+static std::once_flag g_once;
+static Instance g_inst[256];
+static int g_ninst = 0;
+
+static void load_table() {
+  std::call_once(g_once, [] { register_instances_map(g_inst, &g_ninst, 256); });
+}
+
+static const Instance* find_instance(const cl_system* sys, const cl_chain* ch, int d, int k) {
+  load_table();
+  for (int i = 0; i < g_ninst; ++i) {
+    const Instance& in = g_inst[i];
+    if (in.sys_kind != sys->kind || in.d != d || in.k != k || in.nred != ch->n) continue;
+    bool same = true;
+    for (int r = 0; r < ch->n && same; ++r) {
+      const cl_reducer& q = ch->r[r];
+      int alg = q.kind == CL_RED_QR ? q.alg : 0;
+      same = in.red[r][0] == q.kind && in.red[r][1] == alg && in.red[r][2] == q.frame;
+    }
+    if (same) return &in;
+  }
+  return nullptr;
+}
+
+static int state_dim(int kind) {
+  switch (kind) {
+    case CL_SYS_STANDARD_MAP: return 2;
+    case CL_SYS_WRAPPED_STANDARD_MAP: return 4;
+    default: return -1;
+  }
+}
+static int sys_dim(int kind) {
+  switch (kind) {
+    case CL_SYS_STANDARD_MAP:
+    case CL_SYS_WRAPPED_STANDARD_MAP: return 2;
+    default: return -1;
+  }
+}
+
+static int map_err(hipError_t e) {
+  if (e == hipSuccess) return CL_OK;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return CL_ERR_OOM;
+  return CL_ERR_DEVICE_LOST;
+}
+
+static int set_err(cl_ctx* ctx, int code, const char* msg) {
+  if (ctx) snprintf(ctx->err, sizeof(ctx->err), "%s", msg);
+  return code;
+}
+
+// Validation shared by both launch entry points (src/Tangent/propagate.jl:279-289, orbit.jl:187-191).
+static int validate(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m, int s, int d,
+                    int k, int64_t n0, int64_t steps, int64_t N, int tau, const Instance** out) {
+  if (!sys || !ch) return set_err(ctx, CL_ERR_ARGUMENT, "null system or chain descriptor");
+  if (ch->n < 0 || ch->n > CL_MAX_REDUCERS) return set_err(ctx, CL_ERR_ARGUMENT, "chain length out of range");
+  if (!(tau >= 1 && N >= 1 && N % tau == 0))
+    return set_err(ctx, CL_ERR_ARGUMENT, "N must be a positive multiple of tau");
+  if (!(steps >= 1 && steps % tau == 0 && n0 >= 0 && n0 % tau == 0 && n0 + steps <= N))
+    return set_err(ctx, CL_ERR_ARGUMENT, "launch steps must be a positive multiple of tau within N");
+  if (m < 0) return set_err(ctx, CL_ERR_ARGUMENT, "negative orbit count");
+  int sd = sys_dim(sys->kind);
+  if (sd < 0) return set_err(ctx, CL_ERR_UNSUPPORTED, "system kind has no kernel");
+  if (d != sd) return set_err(ctx, CL_ERR_ARGUMENT, "d is not the system dimension");
+  if (s != state_dim(sys->kind)) return set_err(ctx, CL_ERR_ARGUMENT, "state length does not match the system");
+  if (!(k >= 1 && k <= d)) return set_err(ctx, CL_ERR_ARGUMENT, "k must be in 1:d");
+  if (elt != CL_ELT_F64) return set_err(ctx, CL_ERR_UNSUPPORTED, "element type has no kernel for this system");
+  const Instance* in = find_instance(sys, ch, d, k);
+  if (!in) return set_err(ctx, CL_ERR_UNSUPPORTED, "no kernel instance for this system/chain/k");
+  *out = in;
+  return CL_OK;
+}
+
+// ------------------------------------------------------------------------------ helper kernels
+// AoS[i][w] <-> SoA[w][i] for 8-byte words.
+__global__ void k_aos_to_soa(const uint64_t* __restrict__ a, uint64_t* __restrict__ s, int64_t m, int w) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m * w) return;
+  int64_t j = t / m, i = t - j * m;
+  s[t] = a[i * w + j];
+}
+__global__ void k_soa_to_aos(const uint64_t* __restrict__ s, uint64_t* __restrict__ a, int64_t m, int w) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m * w) return;
+  int64_t j = t / m, i = t - j * m;
+  a[i * w + j] = s[t];
+}
+
+template <int D, int K>
+__global__ void k_frames(uint64_t seed, int64_t orbit0, int64_t m, double* __restrict__ W) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  Frame<D, K> f;
+  seeded_frame<D, K>(seed, orbit0 + i, f);
+#pragma unroll
+  for (int e = 0; e < D * K; ++e) W[(int64_t)e * m + i] = f.w[e];
+}
+
+__global__ void k_math(int64_t n, const double* __restrict__ x, double* sn, double* cs, double* lg) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s, c;
+  cl_sincos(x[i], &s, &c);
+  sn[i] = s;
+  cs[i] = c;
+  lg[i] = cl_log(x[i]);
+}
+
+static unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// ------------------------------------------------------------------------------ probe / context
+extern "C" int cl_probe(int elt, cl_probe_t* out) {
+  if (!out) return CL_ERR_ARGUMENT;
+  memset(out, 0, sizeof(*out));
+  out->loadable = 1;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return CL_ERR_NO_DEVICE;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, 0) != hipSuccess) return CL_ERR_NO_DEVICE;
+  out->functional = strncmp(p.gcnArchName, "gfx950", 6) == 0;
+  snprintf(out->device, sizeof(out->device), "%s (%s)", p.name, p.gcnArchName);
+  out->compiles = out->functional && elt == CL_ELT_F64;
+  out->gpu_faster = out->compiles;
+  return out->functional ? CL_OK : CL_ERR_NO_DEVICE;
+}
+
+extern "C" int64_t cl_free_memory(void) {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return -1;
+  return (int64_t)fr;
+}
+
+extern "C" cl_ctx* cl_create(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+    snprintf(g_create_err, sizeof(g_create_err), "no HIP device %d", device);
+    return nullptr;
+  }
+  if (hipSetDevice(device) != hipSuccess) {
+    snprintf(g_create_err, sizeof(g_create_err), "hipSetDevice(%d) failed", device);
+    return nullptr;
+  }
+  cl_ctx* c = (cl_ctx*)calloc(1, sizeof(cl_ctx));
+  c->device = device;
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    free(c);
+    snprintf(g_create_err, sizeof(g_create_err), "hipStreamCreate failed");
+    return nullptr;
+  }
+  load_table();
+  return c;
+}
+
+extern "C" void cl_destroy(cl_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  for (int i = 0; i < 12; ++i)
+    if (c->buf[i]) hipFree(c->buf[i]);
+  hipStreamDestroy(c->stream);
+  free(c);
+}
+
+extern "C" const char* cl_last_error(cl_ctx* c) { return c ? c->err : g_create_err; }
+
+extern "C" int64_t cl_state_bytes(const cl_system* sys, const cl_chain* ch, int elt, int d, int k) {
+  if (!sys || !ch || ch->n < 0 || ch->n > CL_MAX_REDUCERS) return -CL_ERR_ARGUMENT;
+  if (elt != CL_ELT_F64) return -CL_ERR_UNSUPPORTED;
+  const Instance* in = find_instance(sys, ch, d, k);
+  if (!in) return -CL_ERR_UNSUPPORTED;
+  return (int64_t)in->words * 8;
+}
+
+static int grow(cl_ctx* c, int slot, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  if (c->cap[slot] >= bytes) return CL_OK;
+  if (c->buf[slot]) hipFree(c->buf[slot]);
+  c->buf[slot] = nullptr;
+  c->cap[slot] = 0;
+  hipError_t e = hipMalloc(&c->buf[slot], bytes);
+  if (e != hipSuccess) return map_err(e);
+  c->cap[slot] = bytes;
+  return CL_OK;
+}
+
+static int launch(const Instance* in, const cl_system* sys, const cl_chain* ch, int64_t m, double* dx,
+                  double* dW, double* dW2, uint64_t* dst, int8_t* dcode, int64_t* dhstep, int64_t n0,
+                  int64_t steps, int64_t N, int tau, hipStream_t st) {
+  if (m == 0) return CL_OK;
+  KArgs a;
+  a.x = dx;
+  a.W0 = dW;
+  a.W1 = dW2;
+  a.st = dst;
+  a.code = dcode;
+  a.hstep = dhstep;
+  a.m = m;
+  a.n0 = n0;
+  a.steps = steps;
+  a.N = N;
+  a.tau = tau;
+  return in->launch(sys, ch, a, st);
+}
+
+extern "C" int cl_propagate_device(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m,
+                                   int s, int d, int k, int64_t n0, int64_t steps, int64_t N, int tau,
+                                   double* dx, double* dW, double* dW2, uint64_t* dst, int8_t* dcode,
+                                   int64_t* dhstep, void* stream) {
+  if (!ctx) return CL_ERR_ARGUMENT;
+  const Instance* in = nullptr;
+  int rc = validate(ctx, sys, ch, elt, m, s, d, k, n0, steps, N, tau, &in);
+  if (rc) return rc;
+  if (in->nf > 1 && !dW2) return set_err(ctx, CL_ERR_ARGUMENT, "the chain uses a second frame: W2 is required");
+  hipSetDevice(ctx->device);
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  rc = launch(in, sys, ch, m, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st);
+  if (rc) return set_err(ctx, rc, "kernel launch failed");
+  return CL_OK;
+}
+
+enum { B_X, B_W, B_W2, B_ST, B_CODE, B_H, B_XA, B_WA, B_W2A, B_STA };
+
+extern "C" int cl_propagate_chunk(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m,
+                                  int s, int d, int k, int64_t n0, int64_t steps, int64_t N, int tau, double* x,
+                                  double* W, double* W2, void* states, int8_t* code, int64_t* hstep,
+                                  uint32_t* ws) {
+  (void)ws;
+  if (!ctx) return CL_ERR_ARGUMENT;
+  const Instance* in = nullptr;
+  int rc = validate(ctx, sys, ch, elt, m, s, d, k, n0, steps, N, tau, &in);
+  if (rc) return rc;
+  if (m == 0) return CL_OK;
+  if (!x || !W || !code || !hstep || (in->words > 0 && !states) || (in->nf > 1 && !W2))
+    return set_err(ctx, CL_ERR_ARGUMENT, "null buffer");
+  hipSetDevice(ctx->device);
+  hipStream_t st = ctx->stream;
+  const int dk = d * k, nw = in->words;
+  size_t bx = (size_t)m * s * 8, bw = (size_t)m * dk * 8, bs = (size_t)m * nw * 8;
+  if ((rc = grow(ctx, B_X, bx)) || (rc = grow(ctx, B_W, bw)) || (rc = grow(ctx, B_ST, bs)) ||
+      (rc = grow(ctx, B_CODE, (size_t)m)) || (rc = grow(ctx, B_H, (size_t)m * 8)) || (rc = grow(ctx, B_XA, bx)) ||
+      (rc = grow(ctx, B_WA, bw)) || (rc = grow(ctx, B_STA, bs)))
+    return set_err(ctx, rc, "device allocation failed");
+  if (in->nf > 1 && ((rc = grow(ctx, B_W2, bw)) || (rc = grow(ctx, B_W2A, bw))))
+    return set_err(ctx, rc, "device allocation failed");
+  uint64_t *dx = (uint64_t*)ctx->buf[B_X], *dW = (uint64_t*)ctx->buf[B_W], *dW2 = (uint64_t*)ctx->buf[B_W2];
+  uint64_t *dst = (uint64_t*)ctx->buf[B_ST], *xa = (uint64_t*)ctx->buf[B_XA], *wa = (uint64_t*)ctx->buf[B_WA];
+  uint64_t *w2a = (uint64_t*)ctx->buf[B_W2A], *sta = (uint64_t*)ctx->buf[B_STA];
+  int8_t* dcode = (int8_t*)ctx->buf[B_CODE];
+  int64_t* dh = (int64_t*)ctx->buf[B_H];
+  hipError_t e = hipSuccess;
+#define CK(call)                                   \
+  if ((e = (call)) != hipSuccess) {                \
+    return set_err(ctx, map_err(e), #call " failed"); \
+  }
+  const int B = 256;
+  CK(hipMemcpyAsync(xa, x, bx, hipMemcpyHostToDevice, st));
+  CK(hipMemcpyAsync(wa, W, bw, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(m * s, B)), dim3(B), 0, st, xa, dx, m, s);
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(m * dk, B)), dim3(B), 0, st, wa, dW, m, dk);
+  if (in->nf > 1) {
+    CK(hipMemcpyAsync(w2a, W2, bw, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(m * dk, B)), dim3(B), 0, st, w2a, dW2, m, dk);
+  }
+  if (n0 > 0) {
+    if (nw > 0) {
+      CK(hipMemcpyAsync(sta, states, bs, hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(m * nw, B)), dim3(B), 0, st, sta, dst, m, nw);
+    }
+    CK(hipMemcpyAsync(dcode, code, (size_t)m, hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(dh, hstep, (size_t)m * 8, hipMemcpyHostToDevice, st));
+  }
+  rc = launch(in, sys, ch, m, (double*)dx, (double*)dW, (double*)dW2, dst, dcode, dh, n0, steps, N, tau, st);
+  if (rc) return set_err(ctx, rc, "kernel launch failed");
+  hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(m * s, B)), dim3(B), 0, st, dx, xa, m, s);
+  hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(m * dk, B)), dim3(B), 0, st, dW, wa, m, dk);
+  CK(hipMemcpyAsync(x, xa, bx, hipMemcpyDeviceToHost, st));
+  CK(hipMemcpyAsync(W, wa, bw, hipMemcpyDeviceToHost, st));
+  if (in->nf > 1) {
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(m * dk, B)), dim3(B), 0, st, dW2, w2a, m, dk);
+    CK(hipMemcpyAsync(W2, w2a, bw, hipMemcpyDeviceToHost, st));
+  }
+  if (nw > 0) {
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(m * nw, B)), dim3(B), 0, st, dst, sta, m, nw);
+    CK(hipMemcpyAsync(states, sta, bs, hipMemcpyDeviceToHost, st));
+  }
+  CK(hipMemcpyAsync(code, dcode, (size_t)m, hipMemcpyDeviceToHost, st));
+  CK(hipMemcpyAsync(hstep, dh, (size_t)m * 8, hipMemcpyDeviceToHost, st));
+  CK(hipStreamSynchronize(st));
+#undef CK
+  return CL_OK;
+}
+
+// ------------------------------------------------------------------------------ frames
+template <int D, int K>
+static void frames_host(uint64_t seed, int64_t orbit0, int64_t m, double* W) {
+  for (int64_t i = 0; i < m; ++i) {
+    Frame<D, K> f;
+    seeded_frame<D, K>(seed, orbit0 + i, f);
+    for (int e = 0; e < D * K; ++e) W[i * D * K + e] = f.w[e];
+  }
+}
+
+#define FRAME_SIZES(X) X(2, 1) X(2, 2) X(4, 1) X(4, 2) X(4, 3) X(4, 4)
+
+extern "C" int cl_initial_deviations(uint64_t seed, int64_t orbit0, int64_t m, int d, int k, double* W) {
+  if (m < 0 || !W) return CL_ERR_ARGUMENT;
+#define X(D, K) \
+  if (d == D && k == K) { frames_host<D, K>(seed, orbit0, m, W); return CL_OK; }
+  FRAME_SIZES(X)
+#undef X
+  return CL_ERR_UNSUPPORTED;
+}
+
+extern "C" int cl_initial_deviations_device(cl_ctx* ctx, uint64_t seed, int64_t orbit0, int64_t m, int d, int k,
+                                            double* dW, void* stream) {
+  if (!ctx || m < 0 || !dW) return CL_ERR_ARGUMENT;
+  if (m == 0) return CL_OK;
+  hipSetDevice(ctx->device);
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+#define X(D, K)                                                                                    \
+  if (d == D && k == K) {                                                                          \
+    hipLaunchKernelGGL((k_frames<D, K>), dim3(grid_for(m, 256)), dim3(256), 0, st, seed, orbit0, m, dW); \
+    return map_err(hipGetLastError());                                                             \
+  }
+  FRAME_SIZES(X)
+#undef X
+  return set_err(ctx, CL_ERR_UNSUPPORTED, "no frame kernel for this d, k");
+}
+
+// ------------------------------------------------------------------------------ math test entry points
+extern "C" void cl_math_host(int64_t n, const double* x, double* sn, double* cs, double* lg) {
+  for (int64_t i = 0; i < n; ++i) {
+    cl_sincos(x[i], &sn[i], &cs[i]);
+    lg[i] = cl_log(x[i]);
+  }
+}
+
+extern "C" int cl_math_device(cl_ctx* ctx, int64_t n, const double* dx, double* dsn, double* dcs, double* dlg,
+                              void* stream) {
+  if (!ctx || n < 0) return CL_ERR_ARGUMENT;
+  if (n == 0) return CL_OK;
+  hipSetDevice(ctx->device);
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  hipLaunchKernelGGL(k_math, dim3(grid_for(n, 256)), dim3(256), 0, st, n, dx, dsn, dcs, dlg);
+  return map_err(hipGetLastError());
+}

@@ -1,0 +1,111 @@
+// registry.hpp - table of the built kernel instances, keyed by system, frame shape and chain.
+//
+// The reference compiles one kernel per (system type, reducer tuple type) on first use. Here the
+// kernels are compiled ahead of time for gfx950; a chain that has no instance is reported as
+// "kernel does not compile for the element type" (FALLBACK_REASONS[3]), as the reference's probe
+// reports a chain its device cannot compile.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "orbit_kernels.hpp"
+
+namespace cl {
+
+struct Instance {
+  int sys_kind, d, k, nf, words, nred;
+  int red[CL_MAX_REDUCERS][3];  // kind, alg, frame
+  int (*launch)(const cl_system*, const cl_chain*, const KArgs&, hipStream_t);
+};
+
+template <class Sys>
+inline Sys make_sys(const cl_system& s);
+template <>
+inline StdMap make_sys<StdMap>(const cl_system& s) {
+  StdMap m;
+  m.K = s.K;
+  return m;
+}
+template <>
+inline WrappedStdMap make_sys<WrappedStdMap>(const cl_system& s) {
+  WrappedStdMap m;
+  m.K = s.K;
+  return m;
+}
+
+// Launch geometry: a big ensemble fills the chip with 256-lane groups; a small one (below 64K
+// orbits) uses 64-lane groups so its waves spread over as many CUs as possible.
+inline int block_for(int64_t m) { return m >= 65536 ? 256 : 64; }
+
+template <class Sys, int K, int NF, class Chain>
+int launch_impl(const cl_system* s, const cl_chain* d, const KArgs& a, hipStream_t st) {
+  Sys sys = make_sys<Sys>(*s);
+  Chain ch;
+  chain_setup(ch, *d, 0);
+  int block = block_for(a.m);
+  int64_t grid = (a.m + block - 1) / block;
+  hipLaunchKernelGGL((orbit_kernel<Sys, K, NF, Chain>), dim3((unsigned)grid), dim3(block), 0, st, a, sys, ch);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (e == hipErrorOutOfMemory ? CL_ERR_OOM : CL_ERR_DEVICE_LOST);
+}
+
+// Reducer-kind codes of the slot types, for the table keys.
+template <class R>
+struct Kind;
+template <int ALG, int D, int K>
+struct Kind<QRSpectrum<ALG, D, K>> {
+  static constexpr int kind = CL_RED_QR, alg = ALG;
+};
+template <int D>
+struct Kind<Benettin<D>> {
+  static constexpr int kind = CL_RED_BENETTIN, alg = 0;
+};
+template <int D>
+struct Kind<MEGNO<D>> {
+  static constexpr int kind = CL_RED_MEGNO, alg = 0;
+};
+template <int D>
+struct Kind<SALI<D>> {
+  static constexpr int kind = CL_RED_SALI, alg = 0;
+};
+template <int D>
+struct Kind<FLI<D>> {
+  static constexpr int kind = CL_RED_FLI, alg = 0;
+};
+template <int D, int K>
+struct Kind<OFLI<D, K>> {
+  static constexpr int kind = CL_RED_OFLI, alg = 0;
+};
+template <int D, int K>
+struct Kind<GALI<D, K>> {
+  static constexpr int kind = CL_RED_GALI, alg = 0;
+};
+
+inline void fill_key(Instance&, const Nil&, int) {}
+template <class H, class T>
+void fill_key(Instance& in, const Cons<H, T>& c, int i) {
+  in.red[i][0] = Kind<decltype(c.head.r)>::kind;
+  in.red[i][1] = Kind<decltype(c.head.r)>::alg;
+  in.red[i][2] = H::frame;
+  in.nred = i + 1;
+  fill_key(in, c.tail, i + 1);
+}
+
+template <int SYSK, class Sys, int K, int NF, class Chain>
+Instance make_instance() {
+  Instance in;
+  in.sys_kind = SYSK;
+  in.d = Sys::D;
+  in.k = K;
+  in.nf = NF;
+  in.words = ChainWords<Chain>::value;
+  in.nred = 0;
+  for (int i = 0; i < CL_MAX_REDUCERS; ++i) in.red[i][0] = in.red[i][1] = in.red[i][2] = 0;
+  Chain c;
+  fill_key(in, c, 0);
+  in.launch = &launch_impl<Sys, K, NF, Chain>;
+  return in;
+}
+
+// Each instance translation unit appends its entries.
+void register_instances_map(Instance* out, int* n, int cap);
+
+}  // namespace cl

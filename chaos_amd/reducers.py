@@ -1,0 +1,293 @@
+"""Reducer types of the tangent stream and their host-side summaries.
+
+Each reducer mirrors the reference type of the same name (constructor arguments, the state's
+byte layout, and `reducer_final`), so that a chain's per-orbit states written by the kernel decode
+to the same named fields the reference returns. States are read from the raw bytes of the chain's
+state tuple (one row per orbit), vectorised over orbits with numpy.
+"""
+import numpy as np
+
+from . import _lib as L
+
+EPS64 = 2.0 ** -52
+
+
+class Orthonormalizer:
+    code = 0
+    name = ""
+
+
+class CGS(Orthonormalizer):
+    """Classical Gram-Schmidt (src/Lyapunov/orthonormalize.jl:250-251)."""
+    code, name = L.CL_QR_CGS, "CGS"
+
+
+class MGS(Orthonormalizer):
+    """Modified Gram-Schmidt (src/Lyapunov/orthonormalize.jl:252-253)."""
+    code, name = L.CL_QR_MGS, "MGS"
+
+
+class RGS(Orthonormalizer):
+    """Repeated Gram-Schmidt, eta = 1/sqrt 2, at most 4 passes (src/Lyapunov/orthonormalize.jl:254-255)."""
+    code, name = L.CL_QR_RGS, "RGS"
+
+
+class HouseholderQR(Orthonormalizer):
+    """Householder reflectors, V-H (src/Lyapunov/householder.jl:6)."""
+    code, name = L.CL_QR_HOUSEHOLDER, "V-H"
+
+
+class AugmentedQR(Orthonormalizer):
+    """Householder on [W, I], V-A (src/Lyapunov/householder.jl:8)."""
+    code, name = L.CL_QR_AUGMENTED, "V-A"
+
+
+class GivensQR(Orthonormalizer):
+    """Givens rotations on the rows of [W, I], V-G (src/Lyapunov/householder.jl:10)."""
+    code, name = L.CL_QR_GIVENS, "V-G"
+
+
+def _acc(a):
+    """total() of CompensatedSum columns a[..., 0] (s) and a[..., 1] (c)."""
+    return a[..., 0] + a[..., 1]
+
+
+class Reducer:
+    kind = 0
+
+    def descriptor(self, frame):
+        r = L.cl_reducer()
+        r.kind = self.kind
+        r.frame = frame
+        if hasattr(self, "replace"):
+            r.replace = int(self.replace)
+        return r
+
+    def nbytes(self, d, k):
+        return self.dtype(d, k).itemsize
+
+    def check(self, d, k):
+        pass
+
+    def provenance(self):
+        return {}
+
+
+class QRSpectrum(Reducer):
+    """QRSpectrum(alg; orthogonality_tol=8.0) (src/Lyapunov/qr_spectrum.jl:161-165)."""
+    kind = L.CL_RED_QR
+
+    def __init__(self, alg, orthogonality_tol=8.0):
+        if isinstance(alg, type):
+            alg = alg()
+        self.alg = alg
+        self.orthogonality_tol = float(orthogonality_tol)
+
+    def descriptor(self, frame):
+        r = super().descriptor(frame)
+        r.alg = self.alg.code
+        r.tol = self.orthogonality_tol
+        return r
+
+    def dtype(self, d, k):
+        return np.dtype([("sums", "<f8", (k, 2)), ("n", "<i8"), ("G_max", "<f8"), ("orth_max", "<f8"),
+                         ("count_max", "<i8"), ("code", "i1"), ("pad", "u1", 7), ("column", "<i8"),
+                         ("value", "<f8"), ("threshold", "<f8"), ("step", "<i8")])
+
+    def final(self, st):
+        """(n, sum, comp, lambda, G_max, orth_max, count_max), qr_spectrum.jl:235-240."""
+        n = st["n"]
+        tot = _acc(st["sums"])
+        lam = np.where(n[:, None] == 0, 0.0, tot / np.where(n == 0, 1, n)[:, None])
+        return dict(n=n, sum=st["sums"][..., 0].copy(), comp=st["sums"][..., 1].copy(), **{"lambda": lam},
+                    G_max=st["G_max"].copy(), orth_max=st["orth_max"].copy(), count_max=st["count_max"].copy())
+
+    def error(self, row):
+        kind = "orthogonality" if row["code"] == 4 else "orthonormalisation breakdown"
+        return kind, float(row["value"]), float(row["threshold"]), int(row["step"]), int(row["column"])
+
+    def provenance(self):
+        return dict(estimator="qr", orthonormalizer=self.alg.name, orthogonality_tol=self.orthogonality_tol)
+
+
+class Benettin(Reducer):
+    """Benettin() for k = 1 (src/Lyapunov/benettin.jl:143-185)."""
+    kind = L.CL_RED_BENETTIN
+
+    def check(self, d, k):
+        if k != 1:
+            raise ValueError(f"Benettin needs k = 1, got k = {k}")
+
+    def dtype(self, d, k):
+        return np.dtype([("sum", "<f8", 2), ("n", "<i8"), ("code", "i1"), ("pad", "u1", 7), ("value", "<f8"),
+                         ("step", "<i8")])
+
+    def final(self, st):
+        n = st["n"]
+        lam = np.where(n == 0, 0.0, _acc(st["sum"]) / np.where(n == 0, 1, n))
+        return dict(n=n, sum=st["sum"][:, :1].copy(), comp=st["sum"][:, 1:].copy(), **{"lambda": lam[:, None]})
+
+    def error(self, row):
+        v = float(row["value"])
+        return "deviation matrix", v, 0.0 if np.isfinite(v) else np.inf, int(row["step"]), 1
+
+    def provenance(self):
+        return dict(estimator="benettin")
+
+
+def _k2(name, k):
+    if k != 2:
+        raise ValueError(f"{name} needs k = 2, got k = {k}")
+
+
+class MEGNO(Reducer):
+    """MEGNO(; replace=True) for k = 2 (src/Indicators/megno.jl:29-113)."""
+    kind = L.CL_RED_MEGNO
+
+    def __init__(self, replace=True):
+        self.replace = bool(replace)
+
+    def check(self, d, k):
+        _k2("MEGNO", k)
+
+    def dtype(self, d, k):
+        return np.dtype([("c", "<f8", (2, 5, 2)), ("value", "<f8"), ("n", "<i8"), ("K", "<i8"), ("step", "<i8"),
+                         ("code", "i1"), ("column", "i1"), ("pad", "u1", 6)])
+
+    def final(self, st):
+        """(n, handoffs, Y, Ybar, Ybar20, sum, comp, R1, R2, Q, Q2), megno.jl:97-111."""
+        c = st["c"]  # [orbit, column, sum(s,R1,R2,Q,Q2), (s,c)]
+        n = st["n"].astype(np.float64)[:, None]
+        K = st["K"][:, None]
+        tot = _acc(c)
+        R1, R2, Q, Q2 = tot[:, :, 1], tot[:, :, 2], tot[:, :, 3], tot[:, :, 4]
+        zero = K == 0
+        nn = np.where(zero, 1.0, n)
+        tau = (st["n"][:, None] // np.where(zero, 1, K)).astype(np.float64)
+        Y = np.where(zero, 0.0, 2 * R1 / nn)
+        Ybar = np.where(zero, 0.0, Q / np.where(zero, 1, K))
+        Ybar20 = np.where(zero, 0.0, tau * Q2 / nn ** 3)
+        return dict(n=st["n"].copy(), handoffs=st["K"].copy(), Y=Y, Ybar=Ybar, Ybar20=Ybar20,
+                    sum=c[:, :, 0, 0].copy(), comp=c[:, :, 0, 1].copy(), R1=R1, R2=R2, Q=Q, Q2=Q2)
+
+    def error(self, row):
+        v = float(row["value"])
+        return "deviation matrix", v, 0.0 if np.isfinite(v) else np.inf, int(row["step"]), int(row["column"])
+
+    def provenance(self):
+        return dict(indicator="megno", convention="weights at the end of the window, mean over the handoffs 1..K")
+
+
+class SALI(Reducer):
+    """SALI() for k = 2, replaces W by unit columns (src/Indicators/sali.jl:147-201)."""
+    kind = L.CL_RED_SALI
+
+    def check(self, d, k):
+        _k2("SALI", k)
+
+    def dtype(self, d, k):
+        return np.dtype([("s", "<f8", (2, 2)), ("sali", "<f8"), ("value", "<f8"), ("n", "<i8"), ("step", "<i8"),
+                         ("code", "i1"), ("column", "i1"), ("pad", "u1", 6)])
+
+    def final(self, st):
+        s, c = st["s"][:, :, 0], st["s"][:, :, 1]
+        L_ = np.where(st["n"] == 0, 0.0, -((s[:, 0] + s[:, 1]) + (c[:, 0] + c[:, 1])))
+        return dict(n=st["n"].copy(), sali=st["sali"].copy(), L=L_, sum=s.copy(), comp=c.copy(),
+                    floor=4 * EPS64)
+
+    error = MEGNO.error
+
+    def provenance(self):
+        return dict(indicator="sali")
+
+
+class FLI(Reducer):
+    """FLI(; replace=True) for k = 2 (src/Indicators/fli.jl:292-362)."""
+    kind = L.CL_RED_FLI
+
+    def __init__(self, replace=True):
+        self.replace = bool(replace)
+
+    def check(self, d, k):
+        _k2("FLI", k)
+
+    def dtype(self, d, k):
+        return np.dtype([("s", "<f8", (2, 2)), ("f1", "<f8"), ("fB", "<f8"), ("value", "<f8"), ("n", "<i8"),
+                         ("step", "<i8"), ("step1", "<i8"), ("stepB", "<i8"), ("code", "i1"), ("column", "i1"),
+                         ("colB", "i1"), ("pad", "u1", 5)])
+
+    def final(self, st):
+        return dict(n=st["n"].copy(), fli=st["fB"].copy(), fli1=st["f1"].copy(), sum=st["s"][:, :, 0].copy(),
+                    comp=st["s"][:, :, 1].copy(), stepB=st["stepB"].copy(), colB=st["colB"].astype(np.int64),
+                    step1=st["step1"].copy())
+
+    error = MEGNO.error
+
+    def provenance(self):
+        return dict(indicator="fli")
+
+
+class OFLI(Reducer):
+    """OFLI(field; replace=True): the FLI of the components orthogonal to a direction field
+    (src/Indicators/ofli.jl:23-126). The reference defines the field for flows only (their vector
+    field); on a map this build uses the map displacement F(x) - x."""
+    kind = L.CL_RED_OFLI
+
+    def __init__(self, flow=None, replace=True):
+        self.flow = flow
+        self.replace = bool(replace)
+
+    def dtype(self, d, k):
+        return np.dtype([("acc", "<f8", (k, 2)), ("g1", "<f8"), ("gB", "<f8"), ("inst", "<f8"), ("f1", "<f8"),
+                         ("fB", "<f8"), ("sin1", "<f8"), ("sinB", "<f8"), ("sinmin", "<f8"), ("value", "<f8"),
+                         ("nminf", "<i8"), ("n", "<i8"), ("step", "<i8"), ("step1", "<i8"), ("stepB", "<i8"),
+                         ("colB", "i1"), ("code", "i1"), ("column", "i1"), ("pad", "u1", 5)])
+
+    def final(self, st):
+        return dict(n=st["n"].copy(), ofli=st["gB"].copy(), ofli1=st["g1"].copy(), inst=st["inst"].copy(),
+                    fli=st["fB"].copy(), fli1=st["f1"].copy(), sum=st["acc"][..., 0].copy(),
+                    comp=st["acc"][..., 1].copy(), stepB=st["stepB"].copy(), colB=st["colB"].astype(np.int64),
+                    step1=st["step1"].copy(), sinB=st["sinB"].copy(), sin1=st["sin1"].copy(),
+                    sinmin=st["sinmin"].copy(), nminf=st["nminf"].copy())
+
+    def error(self, row):
+        v = float(row["value"])
+        if row["code"] == 2:
+            return "vector field", v, 2.0 ** -1022, int(row["step"]), 0
+        return "deviation matrix", v, 0.0 if np.isfinite(v) else np.inf, int(row["step"]), int(row["column"])
+
+    def provenance(self):
+        return dict(indicator="ofli", ofli_flow=str(self.flow), ofli_replace=self.replace)
+
+
+class GALI(Reducer):
+    """GALI() for 2 <= k <= d (src/Indicators/gali.jl:190-374)."""
+    kind = L.CL_RED_GALI
+
+    def check(self, d, k):
+        if not (2 <= k <= d):
+            raise ValueError("GALI needs 2 <= k <= d")
+
+    def dtype(self, d, k):
+        return np.dtype([("s", "<f8", k), ("c", "<f8", k), ("gali", "<f8", k - 1), ("value", "<f8"),
+                         ("n", "<i8"), ("step", "<i8"), ("code", "i1"), ("pad", "u1", 7), ("column", "<i8")])
+
+    def final(self, st, d=None):
+        k = st["s"].shape[1]
+        s, c = st["s"], st["c"]
+        Ls = s[:, 0].copy()
+        Lc = c[:, 0].copy()
+        for i in range(1, k):
+            Ls = Ls + s[:, i]
+            Lc = Lc + c[:, i]
+        L_ = np.where(st["n"] == 0, 0.0, -(Ls + Lc))
+        return dict(n=st["n"].copy(), gali=st["gali"].copy(), L=L_ if d == k else None, sum=s.copy(), comp=c.copy(),
+                    floor=np.array([2 * (i + 2) * EPS64 for i in range(k - 1)]))
+
+    def error(self, row):
+        v = float(row["value"])
+        thr = np.inf if not np.isfinite(v) else (0.0 if v == 0 else 2.0 ** -1022)
+        return "deviation matrix", v, thr, int(row["step"]), int(row["column"])
+
+    def provenance(self):
+        return dict(indicator="gali")

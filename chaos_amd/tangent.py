@@ -1,0 +1,212 @@
+"""`propagate`: the host mirror of Chaoslab's ensemble entry point on the MI355X library.
+
+`propagate(sys, x0, N; k, tau, seed, reducers, W0, ...)` keeps the reference's argument meaning,
+validation and error behaviour (src/Tangent/propagate.jl:279-290): N must be a positive multiple of
+tau, 1 <= k <= d, one W0 matrix per orbit; orbit ids are 1..len(x0); a non-finite state or frame,
+or a failed reducer, raises NumericalHealthError for the lowest failing orbit id
+(src/Tangent/propagate.jl:110-113). The work runs on the GPU through the C ABI; torch provides the
+device buffers and the stream. There is no CPU path: without the library or a device this raises.
+
+Extension over the reference: `reducers2` runs a second chain on a second frame of the same orbit
+(same seed frame, same Jacobians), so a replacing indicator chain (MEGNO/FLI/OFLI/SALI) and a QR
+spectrum can share one pass; each chain's results are bitwise those of a separate run.
+"""
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+
+class NumericalHealthError(Exception):
+    """The reference's NumericalHealthError (src/Foundation/Foundation.jl:21-36)."""
+
+    def __init__(self, what, value, threshold, step, orbit, backend="gpu", column=0):
+        super().__init__(f"NumericalHealthError: {what} = {value} (threshold {threshold}) at step {step}, "
+                         f"orbit {orbit}, backend {backend}")
+        self.what, self.value, self.threshold = what, value, threshold
+        self.step, self.orbit, self.backend, self.column = step, orbit, backend, column
+
+
+@dataclass
+class PropagationResult:
+    """Final points `x` (m, s), frames `W` (m, d, k), per-reducer summaries `finals` (a list with one
+    dict of per-orbit arrays per reducer, chain order), and the run `provenance`."""
+    x: np.ndarray
+    W: np.ndarray
+    finals: list
+    provenance: dict
+    W2: np.ndarray = None
+    code: np.ndarray = None
+    hstep: np.ndarray = None
+    states: np.ndarray = field(default=None, repr=False)
+
+
+_ctx = None
+
+
+def _context():
+    global _ctx
+    import torch
+    if not torch.cuda.is_available():
+        raise L.HIPError(2, "no functional HIP device (torch.cuda.is_available() is False)")
+    lib = L.load()
+    if _ctx is None:
+        torch.cuda.init()
+        c = lib.cl_create(torch.cuda.current_device())
+        if not c:
+            raise L.HIPError(2, lib.cl_last_error(None).decode())
+        _ctx = c
+    return lib, _ctx
+
+
+def make_chain(reducers, reducers2=()):
+    ch = L.cl_chain()
+    allr = [(r, 0) for r in reducers] + [(r, 1) for r in reducers2]
+    if len(allr) > L.CL_MAX_REDUCERS:
+        raise ValueError(f"at most {L.CL_MAX_REDUCERS} reducers per run")
+    ch.n = len(allr)
+    for i, (r, f) in enumerate(allr):
+        ch.r[i] = r.descriptor(f)
+    return ch, allr
+
+
+def state_layout(allr, d, k):
+    """Byte offset and numpy dtype of each reducer's state in the chain's state tuple."""
+    out, off = [], 0
+    for r, _ in allr:
+        dt = r.dtype(d, k)
+        out.append((off, dt))
+        off += dt.itemsize
+    return out, off
+
+
+def decode_states(raw, allr, d, k):
+    """Per-reducer structured arrays from the (m, bytes) uint8 state rows."""
+    lay, total = state_layout(allr, d, k)
+    assert raw.shape[1] == total, (raw.shape, total)
+    return [np.ascontiguousarray(raw[:, o:o + dt.itemsize]).view(dt)[:, 0] for o, dt in lay]
+
+
+def finals_of(states, allr, d, k):
+    out = []
+    for (r, _), st in zip(allr, states):
+        out.append(r.final(st, d) if r.kind == L.CL_RED_GALI else r.final(st))
+    return out
+
+
+def check_health(code, hstep, x, W, states, allr, orbit0=1):
+    """Raise NumericalHealthError for the lowest failing orbit (src/Tangent/propagate.jl:111-113)."""
+    bad = np.nonzero(code != 0)[0]
+    if bad.size == 0:
+        return
+    i = int(bad[0])
+    c = int(code[i])
+    orbit = orbit0 + i
+    if c == 3:
+        for (r, _), st in zip(allr, states):
+            if st["code"][i] != 0:
+                what, v, thr, step, col = r.error(st[i])
+                raise NumericalHealthError(what, v, thr, step, orbit, "gpu", col)
+    vals = x[i] if c == 1 else W[i]
+    vals = np.asarray(vals).ravel()
+    nf = vals[~np.isfinite(vals)]
+    first = float(nf[0]) if nf.size else float("nan")
+    raise NumericalHealthError("state" if c == 1 else "deviation matrix", first, math.inf, int(hstep[i]), orbit, "gpu")
+
+
+def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None, launch_steps=None,
+              check=True, keep_states=False):
+    """Propagate every orbit of `x0` (shape (m, s)) for N steps with k deviation vectors on the GPU."""
+    import torch
+    N, tau, k = int(N), int(tau), int(k)
+    if not (tau >= 1 and N >= 1 and N % tau == 0):
+        raise ValueError(f"N must be a positive multiple of tau, got N = {N}, tau = {tau}")
+    x0 = np.ascontiguousarray(np.asarray(x0, dtype=np.float64))
+    if x0.ndim != 2:
+        raise ValueError("x0 must have shape (orbits, state length)")
+    m, s = x0.shape
+    d = sys.dimension
+    if s < d:
+        raise ValueError(f"state length {s} is below the system dimension {d}")
+    if s != sys.state_length:
+        raise ValueError(f"state length {s} does not match {sys!r} (needs {sys.state_length})")
+    if not (1 <= k <= d):
+        raise ValueError(f"k must be in 1:{d}, got {k}")
+    if W0 is not None and len(W0) != m:
+        raise ValueError("W0 needs one matrix per orbit")
+    if launch_steps is not None and not (launch_steps >= 1 and launch_steps % tau == 0):
+        raise ValueError(f"forced steps per launch must be a positive multiple of tau, got L = {launch_steps}, tau = {tau}")
+    for r in list(reducers) + list(reducers2):
+        r.check(d, k)
+    lib, ctx = _context()
+    sysd = sys.descriptor()
+    chain, allr = make_chain(reducers, reducers2)
+    nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, d, k)
+    if nbytes < 0:
+        raise L.HIPError(-nbytes, "no kernel instance for this system, k and reducer chain")
+    words = nbytes // 8
+    nf = 2 if reducers2 else 1
+    dev = torch.device("cuda")
+    dx = torch.from_numpy(np.ascontiguousarray(x0.T)).to(dev)
+    dW = torch.empty((d * k, m), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    if W0 is None:
+        rc = lib.cl_initial_deviations_device(ctx, int(seed) & (2 ** 64 - 1), 1, m, d, k, dW.data_ptr(), stream)
+        if rc:
+            raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
+    else:
+        w = np.asarray(W0, dtype=np.float64).reshape(m, d, k)
+        dW.copy_(torch.from_numpy(np.ascontiguousarray(w.transpose(2, 1, 0).reshape(k * d, m))))
+    dW2 = dW.clone() if nf > 1 else None
+    dst = torch.zeros((max(words, 1), m), dtype=torch.int64, device=dev)
+    dcode = torch.zeros(m, dtype=torch.int8, device=dev)
+    dh = torch.zeros(m, dtype=torch.int64, device=dev)
+    L_ = N if launch_steps is None else min(int(launch_steps), N)
+    n0, launches = 0, 0
+    while n0 < N:
+        steps = min(L_, N - n0)
+        rc = lib.cl_propagate_device(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, m, s, d, k, n0,
+                                     steps, N, tau, dx.data_ptr(), dW.data_ptr(),
+                                     dW2.data_ptr() if dW2 is not None else None, dst.data_ptr(), dcode.data_ptr(),
+                                     dh.data_ptr(), stream)
+        if rc:
+            raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
+        n0 += steps
+        launches += 1
+    torch.cuda.synchronize()
+    x = dx.cpu().numpy().T.copy()
+    W = dW.cpu().numpy().reshape(k, d, m).transpose(2, 1, 0).copy()
+    W2 = dW2.cpu().numpy().reshape(k, d, m).transpose(2, 1, 0).copy() if dW2 is not None else None
+    raw = dst[:words].cpu().numpy().T.copy().view(np.uint8).reshape(m, words * 8)
+    code = dcode.cpu().numpy()
+    hstep = dh.cpu().numpy()
+    states = decode_states(raw, allr, d, k)
+    if check:
+        check_health(code, hstep, x, W, states, allr)
+    prov = dict(backend="gpu", device=device_name(), backend_fallback_reason="", precision="Float64",
+                seed=seed, tau=tau, k=k, chunk_size=m, gpu_chunks=1, gpu_chunk_size=m, launch_steps=L_,
+                launches_per_chunk=launches)
+    for r, _ in allr:
+        prov.update(r.provenance())
+    return PropagationResult(x=x, W=W, finals=finals_of(states, allr, d, k), provenance=prov, W2=W2, code=code,
+                             hstep=hstep, states=raw if keep_states else None)
+
+
+def device_name():
+    lib = L.load()
+    p = L.cl_probe_t()
+    lib.cl_probe(L.CL_ELT_F64, ctypes.byref(p))
+    return p.device.decode()
+
+
+def initial_deviations(seed, orbit, d, k):
+    """Seeded d x k orthonormal frame of orbit `orbit` (1-based), as the library forms it on the device."""
+    lib = L.load()
+    out = np.empty((1, k, d), dtype=np.float64)
+    rc = lib.cl_initial_deviations(int(seed) & (2 ** 64 - 1), int(orbit), 1, d, k, out.ctypes.data)
+    if rc:
+        raise L.HIPError(rc, "no frame rule for this d, k")
+    return out[0].T.copy()

@@ -1,0 +1,162 @@
+/* chaoslab_hip.h - C ABI of the MI355X (gfx950) ensemble tangent-stream library.
+ *
+ * The library replaces the device path of Chaoslab's `propagate`: the KernelAbstractions kernel
+ * `_orbit_kernel!` (src/Tangent/propagate.jl:32-50) and the chunk driver around it
+ * (`_run_chunk!`, src/Tangent/propagate.jl:92-119), plus the device hooks that the CUDA package
+ * extension installs (ext/ChaoslabCUDAExt.jl:22-91). A Julia `ChaoslabHIPExt` binds these symbols
+ * with `ccall`; Python binds them with ctypes. No torch or HIP types appear in the signatures:
+ * buffers are plain pointers and sizes, streams are opaque `void*`.
+ *
+ * Layouts
+ *   Host buffers (cl_propagate_chunk) are the reference's own array-of-structs layouts:
+ *     x      double[m][s]        (Vector{SVector{s,Float64}})
+ *     W      double[m][k][d]     (Vector{SMatrix{d,k,Float64}}, column-major per orbit)
+ *     states bytes[m][state_bytes] (the chain's tuple of isbits reducer states, Julia layout)
+ *     code   int8[m], hstep int64[m]
+ *   Device buffers (cl_propagate_device) are structure-of-arrays, one orbit per lane:
+ *     x[c*m + i], W[(col*d + row)*m + i], states as 8-byte words st[w*m + i].
+ *
+ * Return codes are the indices of the reference's FALLBACK_REASONS (src/Foundation/Foundation.jl:71-75):
+ *   0 ok, 2 no functional device, 3 kernel not built for this system/chain/element type,
+ *   4 device out of memory, 9 device lost; 10 is an invalid argument (the reference throws
+ *   ArgumentError before any work). Per-orbit numerical failures are not errors: they come back
+ *   in code[] / hstep[] and in the reducer states exactly as the reference kernel writes them.
+ */
+#ifndef CHAOSLAB_HIP_H
+#define CHAOSLAB_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status codes */
+#define CL_OK 0
+#define CL_ERR_NO_DEVICE 2
+#define CL_ERR_UNSUPPORTED 3
+#define CL_ERR_OOM 4
+#define CL_ERR_DEVICE_LOST 9
+#define CL_ERR_ARGUMENT 10
+
+/* element types: the orbit and frame arithmetic of a run */
+#define CL_ELT_F64 1         /* Float64 orbit and tangents */
+#define CL_ELT_F64_F32TAN 2  /* Float64 orbit, Float32 tangent frames, FP64 reorthonormalisation and log sums */
+#define CL_ELT_DD 3          /* Double64-equivalent orbit (double-double on fma), Float64 tangents */
+
+/* ------------------------------------------------------------------ system descriptors */
+#define CL_SYS_STANDARD_MAP 1          /* StandardMap(K), src/Systems/standard_map.jl:8-43 */
+#define CL_SYS_WRAPPED_STANDARD_MAP 2  /* WrappedStandardMap(K), src/Systems/wrapped_standard_map.jl:75-123 */
+#define CL_SYS_STANDARD_MAP_CHAIN 3    /* StandardMapChain(N, K, eps; boundary), src/Systems/standard_map_chain.jl:11-53 */
+#define CL_SYS_HENON_HEILES_GAUSS 4    /* IntegratorMap(Gauss(Float64, s; tol), HenonHeilesFlow(), h), src/Integrators/integrator_map.jl:12-23 */
+
+typedef struct {
+  int32_t kind;     /* CL_SYS_* */
+  int32_t N;        /* chain sites (kind 3) */
+  int32_t ring;     /* chain boundary: 1 ring, 0 open (kind 3) */
+  int32_t stages;   /* Gauss stages s (kind 4) */
+  int32_t maxiter;  /* Gauss fixed-point iteration cap (kind 4; 0 means the reference default 50) */
+  int32_t pad_;
+  double K;         /* kick strength (kinds 1-3) */
+  double eps;       /* chain coupling (kind 3) */
+  double h;         /* step size (kind 4) */
+  double tol;       /* Gauss acceptance bound; NaN means 8 eps max(1, |Y|) (kind 4) */
+} cl_system;
+
+/* ------------------------------------------------------------------ reducer chain */
+#define CL_RED_QR 1        /* QRSpectrum(alg; orthogonality_tol), src/Lyapunov/qr_spectrum.jl:161-243 */
+#define CL_RED_BENETTIN 2  /* Benettin(), src/Lyapunov/benettin.jl:143-185 */
+#define CL_RED_MEGNO 3     /* MEGNO(; replace), src/Indicators/megno.jl:29-113 */
+#define CL_RED_SALI 4      /* SALI(), src/Indicators/sali.jl:147-201 */
+#define CL_RED_FLI 5       /* FLI(; replace), src/Indicators/fli.jl:292-362 */
+#define CL_RED_OFLI 6      /* OFLI(flow; replace), src/Indicators/ofli.jl:23-126 */
+#define CL_RED_GALI 7      /* GALI(), src/Indicators/gali.jl:190-374 */
+
+#define CL_QR_CGS 1
+#define CL_QR_MGS 2
+#define CL_QR_RGS 3
+#define CL_QR_HOUSEHOLDER 4  /* HouseholderQR(), V-H */
+#define CL_QR_AUGMENTED 5    /* AugmentedQR(), V-A */
+#define CL_QR_GIVENS 6       /* GivensQR(), V-G */
+
+typedef struct {
+  int32_t kind;     /* CL_RED_* */
+  int32_t alg;      /* CL_QR_* for CL_RED_QR */
+  int32_t replace;  /* MEGNO / FLI / OFLI chain-position flag */
+  int32_t frame;    /* 0: the frame W; 1: the second frame W2 (a second tangent frame on the same
+                       orbit, so a replacing indicator chain and a QR spectrum can share one pass) */
+  double tol;       /* QRSpectrum orthogonality_tol */
+  double pad_;
+} cl_reducer;
+
+#define CL_MAX_REDUCERS 8
+typedef struct {
+  int32_t n;
+  int32_t pad_;
+  cl_reducer r[CL_MAX_REDUCERS];
+} cl_chain;
+
+/* ------------------------------------------------------------------ device probe and context */
+typedef struct {
+  int32_t loadable;    /* the library loaded (always 1 when this call returns) */
+  int32_t functional;  /* a gfx950 device is visible */
+  int32_t compiles;    /* kernels are built for the element type */
+  int32_t gpu_faster;  /* always 1 for Float64 on MI355X */
+  char device[64];
+} cl_probe_t;
+
+typedef struct cl_ctx cl_ctx;
+
+/* Foundation.probe(::HIPDevice, T) (src/Foundation/Foundation.jl:92-93; ext/ChaoslabCUDAExt.jl:87-91). */
+int cl_probe(int elt, cl_probe_t *out);
+/* Foundation.free_memory(::HIPDevice) (ext/ChaoslabCUDAExt.jl:31); -1 without a device. */
+int64_t cl_free_memory(void);
+/* A context owns one HIP stream (default priority) and grown-on-demand staging buffers. */
+cl_ctx *cl_create(int device);
+void cl_destroy(cl_ctx *ctx);
+/* Text of the last error on this context (or of the last cl_create failure when ctx is NULL). */
+const char *cl_last_error(cl_ctx *ctx);
+
+/* Bytes per orbit of the chain's state tuple (reducer_bytes, src/Tangent/Tangent.jl:108-111), or a
+ * negative code when the system/chain/d/k combination has no kernel (then -3) or is invalid (-10). */
+int64_t cl_state_bytes(const cl_system *sys, const cl_chain *chain, int elt, int d, int k);
+
+/* One launch of `steps` steps from global step n0 over m orbits, in place on host buffers in the
+ * reference layouts: the work of one `_orbit_kernel!` launch (src/Tangent/propagate.jl:32-50)
+ * including the host<->device copies of `_run_chunk!` (src/Tangent/propagate.jl:95-110).
+ * n0 == 0 initialises the reducer states (the input states are ignored); an orbit whose code is not
+ * 0 at n0 > 0 is left untouched. W2 is the second frame (NULL when no reducer uses frame 1).
+ * ws is the reducer workspace (no reducer here requests one: pass NULL). Synchronous. */
+int cl_propagate_chunk(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain, int elt,
+                       int64_t m, int s, int d, int k, int64_t n0, int64_t steps, int64_t N, int tau,
+                       double *x, double *W, double *W2, void *states, int8_t *code, int64_t *hstep,
+                       uint32_t *ws);
+
+/* The same launch on device-resident structure-of-arrays buffers, enqueued on `stream` (a
+ * hipStream_t, NULL for the context's stream). Asynchronous. */
+int cl_propagate_device(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain, int elt,
+                        int64_t m, int s, int d, int k, int64_t n0, int64_t steps, int64_t N, int tau,
+                        double *dx, double *dW, double *dW2, uint64_t *dstates, int8_t *dcode,
+                        int64_t *dhstep, void *stream);
+
+/* Seeded orthonormal frames, replacing initial_deviations(seed, orbit, Val(d), Val(k), T)
+ * (src/Tangent/initial.jl:27-34): orbit ids orbit0 .. orbit0+m-1 (1-based ids as the reference),
+ * key mix(mix(seed) + id) with the reference's splitmix64 finaliser, Gaussian d x k matrix from a
+ * counter stream, two modified Gram-Schmidt passes per column and unit columns (R_jj > 0).
+ * Column j does not depend on k.
+ * Host version: AoS output W[m][k][d]. Device version: SoA output, asynchronous. */
+int cl_initial_deviations(uint64_t seed, int64_t orbit0, int64_t m, int d, int k, double *W);
+int cl_initial_deviations_device(cl_ctx *ctx, uint64_t seed, int64_t orbit0, int64_t m, int d, int k,
+                                 double *dW, void *stream);
+
+/* Elementary functions of the kernels, evaluated on the host or on the device (for the parity
+ * tests of sin/cos/log; the kernels and these calls share one implementation). */
+void cl_math_host(int64_t n, const double *x, double *sn, double *cs, double *lg);
+int cl_math_device(cl_ctx *ctx, int64_t n, const double *dx, double *dsn, double *dcs, double *dlg,
+                   void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHAOSLAB_HIP_H */

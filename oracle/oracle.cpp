@@ -1,0 +1,1176 @@
+// ORACLE - test infrastructure only. A CPU restatement of the reference's ensemble tangent stream
+// (Chaoslab `propagate`), used by tests/, by __graft_entry__.smoke() and by bench.py's CPU-baseline
+// leg, always as the checker or the timed CPU reference, never as the product path.
+//
+// What it restates (reference file:line, paths relative to the Chaoslab repository root):
+//   or_propagate          _orbit_body, src/Tangent/orbit.jl:44-60; the chunk/thread split of the
+//                         KernelAbstractions CPU backend, src/Tangent/propagate.jl:59-60 (16 orbits per group)
+//   fused product         src/Foundation/product.jl:9-20
+//   CompensatedSum/add    src/Foundation/Foundation.jl:48-58;   safe_norm  src/Foundation/norm.jl:8-12
+//   StandardMap           src/Systems/standard_map.jl:18-43;   WrappedStandardMap  wrapped_standard_map.jl:82-123
+//   QRSpectrum            src/Lyapunov/qr_spectrum.jl:169-240; orthonormalize.jl:281-394; householder.jl:17-123
+//   Benettin              src/Lyapunov/benettin.jl:146-183
+//   MEGNO                 src/Indicators/megno.jl:35-111;       SALI sali.jl:150-199;   FLI fli.jl:299-360
+//   OFLI                  src/Indicators/ofli.jl:30-124 (direction field: the flow's, or the map displacement)
+//   GALI                  src/Indicators/gali.jl:193-372
+//   LogNorm (test reducer) test/tangent-core/tc_helpers.jl:56-70
+//
+// Independence from the product: the reducer chain is interpreted at run time over byte buffers
+// (the product compiles one kernel per chain), and every reducer is written out again here. The
+// elementary functions are selectable: MATH_LIBM (glibc sin/cos/log, fully independent),
+// MATH_CL (the project's portable sin/cos/log from chaos_amd/csrc/cl_math.h, which the kernels use;
+// selected when bit-for-bit agreement with a kernel is the test) and, in the __float128
+// instantiation, libquadmath (113-bit significand, finer than Double64's 106 bits).
+#include <math.h>
+#include <quadmath.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <atomic>
+#include <thread>
+#include <vector>
+
+#include "../include/chaoslab_hip.h"
+#include "../chaos_amd/csrc/cl_math.h"
+
+typedef __float128 quad;
+
+enum { MATH_LIBM = 0, MATH_CL = 1 };
+
+// ------------------------------------------------------------------ arithmetic policies
+struct F64Libm {
+  typedef double T;
+  static double sin(double x) { return cl_isfinite(x) ? ::sin(x) : x - x; }
+  static double cos(double x) { return cl_isfinite(x) ? ::cos(x) : x - x; }
+  static double log(double x) { return ::log(x); }
+  static double sqrt(double x) { return ::sqrt(x); }
+  static double fma(double a, double b, double c) { return ::fma(a, b, c); }
+  static double abs(double x) { return fabs(x); }
+  static bool finite(double x) { return cl_isfinite(x); }
+  static bool nan(double x) { return cl_isnan(x); }
+  static bool sign(double x) { return cl_signbit(x); }
+  static double eps() { return 0x1p-52; }
+  static double floatmin() { return 0x1p-1022; }
+  static double floor(double x) { return ::floor(x); }
+  static double pi() { return 3.141592653589793 + 1.2246467991473532e-16; }
+};
+struct F64Cl : F64Libm {
+  static double sin(double x) { return cl_sin(x); }
+  static double cos(double x) { return cl_cos(x); }
+  static double log(double x) { return cl_log(x); }
+};
+struct Quad {
+  typedef quad T;
+  static quad sin(quad x) { return finite(x) ? sinq(x) : x - x; }
+  static quad cos(quad x) { return finite(x) ? cosq(x) : x - x; }
+  static quad log(quad x) { return logq(x); }
+  static quad sqrt(quad x) { return sqrtq(x); }
+  static quad fma(quad a, quad b, quad c) { return fmaq(a, b, c); }
+  static quad abs(quad x) { return fabsq(x); }
+  static bool finite(quad x) { return finiteq(x); }
+  static bool nan(quad x) { return isnanq(x); }
+  static bool sign(quad x) { return signbitq(x) != 0; }
+  static quad eps() { return FLT128_EPSILON; }
+  static quad floatmin() { return FLT128_MIN; }
+  static quad floor(quad x) { return floorq(x); }
+  static quad pi() { return M_PIq; }
+};
+
+// Julia float max/min: NaN propagates, -0 < +0.
+template <class P, class T>
+static T jmax(T x, T y) {
+  bool takey = (y > x) || (P::sign(y) < P::sign(x));
+  return takey ? (P::nan(x) ? x : y) : (P::nan(y) ? y : x);
+}
+template <class P, class T>
+static T jmin(T x, T y) {
+  bool takey = (y < x) || (P::sign(y) > P::sign(x));
+  return takey ? (P::nan(x) ? x : y) : (P::nan(y) ? y : x);
+}
+
+template <class T>
+struct Acc {
+  T s, c;
+};
+template <class P, class T>
+static Acc<T> acc_add(Acc<T> a, T y) {
+  T t = a.s + y;
+  T c = P::abs(a.s) >= P::abs(y) ? (a.s - t) + y : (y - t) + a.s;
+  Acc<T> o;
+  o.s = t;
+  o.c = a.c + c;
+  return o;
+}
+template <class T>
+static T acc_total(Acc<T> a) {
+  return a.s + a.c;
+}
+
+// scaled norm of v[0], v[st], ..., n entries
+template <class P, class T>
+static T snorm(const T* v, int n, int st = 1) {
+  T m = P::abs(v[0]);
+  for (int i = 1; i < n; ++i) m = jmax<P>(m, P::abs(v[i * st]));
+  if (!(P::finite(m) && m > 0)) return m;
+  T s = (v[0] / m) * (v[0] / m);
+  for (int i = 1; i < n; ++i) {
+    T b = v[i * st] / m;
+    s = s + b * b;
+  }
+  return m * P::sqrt(s);
+}
+template <class T>
+static T vdot(const T* a, const T* b, int n) {
+  T s = a[0] * b[0];
+  for (int i = 1; i < n; ++i) s = s + a[i] * b[i];
+  return s;
+}
+
+// ------------------------------------------------------------------ Julia state layouts (Float64)
+// Byte writer/reader for the chain's state tuple: fields at their Julia offsets.
+struct Buf {
+  uint8_t* p;
+  int off;
+  template <class V>
+  void put(V v) {
+    memcpy(p + off, &v, sizeof(V));
+    off += sizeof(V);
+  }
+  template <class V>
+  V get() {
+    V v;
+    memcpy(&v, p + off, sizeof(V));
+    off += sizeof(V);
+    return v;
+  }
+  void pad(int n) {
+    memset(p + off, 0, n);
+    off += n;
+  }
+  void skip(int n) { off += n; }
+};
+
+static int red_bytes(const cl_reducer& r, int d, int k) {
+  switch (r.kind) {
+    case CL_RED_QR: return 16 * k + 72;
+    case CL_RED_BENETTIN: return 48;
+    case CL_RED_MEGNO: return 200;
+    case CL_RED_SALI: return 72;
+    case CL_RED_FLI: return 96;
+    case CL_RED_OFLI: return 16 * k + 120;
+    case CL_RED_GALI: return 24 * k + 32;
+    case 8: return 16 * k;  // LogNorm
+  }
+  return -1;
+}
+
+// ------------------------------------------------------------------ reducer states in T
+template <class T>
+struct QRSt {
+  Acc<T> sums[8];
+  int64_t n, count_max, column, step;
+  T G_max, orth_max, value, threshold;
+  int8_t code;
+};
+template <class T>
+struct BenSt {
+  Acc<T> sum;
+  int64_t n, step;
+  int8_t code;
+  T value;
+};
+template <class T>
+struct MSums {
+  Acc<T> s, R1, R2, Q, Q2;
+};
+template <class T>
+struct MegSt {
+  MSums<T> c[2];
+  T value;
+  int64_t n, K, step;
+  int8_t code, column;
+};
+template <class T>
+struct SaliSt {
+  Acc<T> s1, s2;
+  T sali, value;
+  int64_t n, step;
+  int8_t code, column;
+};
+template <class T>
+struct FliSt {
+  Acc<T> s1, s2;
+  T f1, fB, value;
+  int64_t n, step, step1, stepB;
+  int8_t code, column, colB;
+};
+template <class T>
+struct OfliSt {
+  Acc<T> acc[8];
+  T g1, gB, inst, f1, fB, sin1, sinB, sinmin, value;
+  int64_t nminf, n, step, step1, stepB;
+  int8_t colB, code, column;
+};
+template <class T>
+struct GaliSt {
+  T s[8], c[8], gali[8], value;
+  int64_t n, step, column;
+  int8_t code;
+};
+template <class T>
+struct LogNormSt {
+  Acc<T> a[8];
+};
+
+template <class T>
+struct RState {  // one of the above, by kind
+  QRSt<T> qr;
+  BenSt<T> ben;
+  MegSt<T> meg;
+  SaliSt<T> sali;
+  FliSt<T> fli;
+  OfliSt<T> ofli;
+  GaliSt<T> gali;
+  LogNormSt<T> ln;
+};
+
+static double dbl(double v) { return v; }
+static double dbl(quad v) { return (double)v; }
+
+// serialise a state to the Julia layout (Float64 values)
+template <class T>
+static void write_state(const cl_reducer& r, const RState<T>& s, int d, int k, uint8_t* out) {
+  Buf b{out, 0};
+  switch (r.kind) {
+    case CL_RED_QR: {
+      const QRSt<T>& q = s.qr;
+      for (int i = 0; i < k; ++i) {
+        b.put(dbl(q.sums[i].s));
+        b.put(dbl(q.sums[i].c));
+      }
+      b.put(q.n);
+      b.put(dbl(q.G_max));
+      b.put(dbl(q.orth_max));
+      b.put(q.count_max);
+      b.put(q.code);
+      b.pad(7);
+      b.put(q.column);
+      b.put(dbl(q.value));
+      b.put(dbl(q.threshold));
+      b.put(q.step);
+      break;
+    }
+    case CL_RED_BENETTIN: {
+      const BenSt<T>& q = s.ben;
+      b.put(dbl(q.sum.s));
+      b.put(dbl(q.sum.c));
+      b.put(q.n);
+      b.put(q.code);
+      b.pad(7);
+      b.put(dbl(q.value));
+      b.put(q.step);
+      break;
+    }
+    case CL_RED_MEGNO: {
+      const MegSt<T>& q = s.meg;
+      for (int c = 0; c < 2; ++c) {
+        const MSums<T>& m = q.c[c];
+        const Acc<T>* a[5] = {&m.s, &m.R1, &m.R2, &m.Q, &m.Q2};
+        for (int j = 0; j < 5; ++j) {
+          b.put(dbl(a[j]->s));
+          b.put(dbl(a[j]->c));
+        }
+      }
+      b.put(dbl(q.value));
+      b.put(q.n);
+      b.put(q.K);
+      b.put(q.step);
+      b.put(q.code);
+      b.put(q.column);
+      b.pad(6);
+      break;
+    }
+    case CL_RED_SALI: {
+      const SaliSt<T>& q = s.sali;
+      b.put(dbl(q.s1.s));
+      b.put(dbl(q.s1.c));
+      b.put(dbl(q.s2.s));
+      b.put(dbl(q.s2.c));
+      b.put(dbl(q.sali));
+      b.put(dbl(q.value));
+      b.put(q.n);
+      b.put(q.step);
+      b.put(q.code);
+      b.put(q.column);
+      b.pad(6);
+      break;
+    }
+    case CL_RED_FLI: {
+      const FliSt<T>& q = s.fli;
+      b.put(dbl(q.s1.s));
+      b.put(dbl(q.s1.c));
+      b.put(dbl(q.s2.s));
+      b.put(dbl(q.s2.c));
+      b.put(dbl(q.f1));
+      b.put(dbl(q.fB));
+      b.put(dbl(q.value));
+      b.put(q.n);
+      b.put(q.step);
+      b.put(q.step1);
+      b.put(q.stepB);
+      b.put(q.code);
+      b.put(q.column);
+      b.put(q.colB);
+      b.pad(5);
+      break;
+    }
+    case CL_RED_OFLI: {
+      const OfliSt<T>& q = s.ofli;
+      for (int i = 0; i < k; ++i) {
+        b.put(dbl(q.acc[i].s));
+        b.put(dbl(q.acc[i].c));
+      }
+      T v[9] = {q.g1, q.gB, q.inst, q.f1, q.fB, q.sin1, q.sinB, q.sinmin, q.value};
+      for (int i = 0; i < 9; ++i) b.put(dbl(v[i]));
+      b.put(q.nminf);
+      b.put(q.n);
+      b.put(q.step);
+      b.put(q.step1);
+      b.put(q.stepB);
+      b.put(q.colB);
+      b.put(q.code);
+      b.put(q.column);
+      b.pad(5);
+      break;
+    }
+    case CL_RED_GALI: {
+      const GaliSt<T>& q = s.gali;
+      for (int i = 0; i < k; ++i) b.put(dbl(q.s[i]));
+      for (int i = 0; i < k; ++i) b.put(dbl(q.c[i]));
+      for (int i = 0; i < k - 1; ++i) b.put(dbl(q.gali[i]));
+      b.put(dbl(q.value));
+      b.put(q.n);
+      b.put(q.step);
+      b.put(q.code);
+      b.pad(7);
+      b.put(q.column);
+      break;
+    }
+    case 8: {
+      for (int i = 0; i < k; ++i) {
+        b.put(dbl(s.ln.a[i].s));
+        b.put(dbl(s.ln.a[i].c));
+      }
+      break;
+    }
+  }
+}
+
+template <class T>
+static void init_state(const cl_reducer& r, RState<T>& s, int k) {
+  memset(&s, 0, sizeof(s));
+  if (r.kind == CL_RED_OFLI) {
+    OfliSt<T>& q = s.ofli;
+    T inf = (T)INFINITY;
+    q.g1 = q.gB = q.inst = -inf;
+    q.sin1 = q.sinB = q.sinmin = inf;
+  }
+  (void)k;
+}
+
+// ------------------------------------------------------------------ QR family
+template <class P, class T>
+struct Fact {
+  T Q[64], R[64];
+  int count, column;
+  int status;  // 0 ok, 1 pivot, 2 passes
+};
+
+// W, Q column-major d x k; R k x k column-major
+template <class P, class T>
+static void qr_gs(int alg, const T* W, int d, int k, Fact<P, T>& f) {
+  for (int i = 0; i < d * k; ++i) f.Q[i] = 0;
+  for (int i = 0; i < k * k; ++i) f.R[i] = 0;
+  f.count = 1;
+  f.status = 0;
+  f.column = 0;
+  for (int m = 0; m < k; ++m) {
+    T u[8], r[8], w[8];
+    for (int i = 0; i < d; ++i) u[i] = w[i] = W[m * d + i];
+    for (int i = 0; i < k; ++i) r[i] = 0;
+    int passes = 1;
+    if (alg == CL_QR_CGS || alg == CL_QR_MGS) {
+      for (int i = 0; i < m; ++i) {
+        const T* q = f.Q + i * d;
+        T c = alg == CL_QR_CGS ? vdot(q, w, d) : vdot(q, u, d);
+        r[i] = c;
+        for (int l = 0; l < d; ++l) u[l] = u[l] - c * q[l];
+      }
+    } else {
+      T eta = (T)1 / P::sqrt((T)2);
+      T nold = snorm<P>(u, d);
+      passes = 0;
+      while (true) {
+        passes++;
+        T vold[8];
+        for (int i = 0; i < d; ++i) vold[i] = u[i];
+        for (int i = 0; i < m; ++i) {
+          const T* q = f.Q + i * d;
+          T c = vdot(q, vold, d);
+          r[i] = r[i] + c;
+          for (int l = 0; l < d; ++l) u[l] = u[l] - c * q[l];
+        }
+        T nnew = snorm<P>(u, d);
+        if (nnew >= nold * eta || !(P::finite(nnew) && nnew > 0)) break;
+        if (passes == 4) {
+          passes = 5;
+          break;
+        }
+        nold = nnew;
+      }
+    }
+    T g = snorm<P>(u, d);
+    for (int i = 0; i < k; ++i) f.R[m * k + i] = i == m ? g : r[i];
+    if (passes > 4) {
+      f.status = 2;
+      f.column = m + 1;
+    } else if (!(P::finite(g) && g > 0)) {
+      f.status = 1;
+      f.column = m + 1;
+    }
+    if (f.status) return;
+    for (int l = 0; l < d; ++l) f.Q[m * d + l] = u[l] / g;
+    if (passes > f.count) f.count = passes;
+  }
+}
+
+// A (d rows, nc columns, column-major) <- (I - 2 u u') A
+template <class P, class T>
+static void reflect(T* A, const T* u, int d, int nc) {
+  for (int l = 0; l < nc; ++l) {
+    T w = vdot(u, A + l * d, d);
+    for (int i = 0; i < d; ++i) A[l * d + i] = A[l * d + i] - ((T)2 * u[i]) * w;
+  }
+}
+
+template <class P, class T>
+static int sweep(T* A, T* U, int d, int nc, int nref) {
+  int count = 0;
+  for (int j = 0; j < nref; ++j) {
+    T a[8];
+    for (int i = 0; i < d; ++i) a[i] = i >= j ? A[j * d + i] : (T)0;
+    T c = snorm<P>(a, d), rho;
+    if (P::finite(c) && c > 0) {
+      T s = a[j] < 0 ? (T)-1 : (T)1;
+      a[j] = a[j] + s * c;
+      T nv = snorm<P>(a, d);
+      for (int i = 0; i < d; ++i) a[i] = a[i] / nv;
+      rho = -s * c;
+      reflect<P>(A, a, d, nc);
+      for (int i = 0; i < d; ++i) U[j * d + i] = a[i];
+      count++;
+    } else {
+      rho = c;
+    }
+    A[j * d + j] = rho;
+  }
+  return count;
+}
+
+template <class P, class T>
+static void finish(Fact<P, T>& f, const T* A, int ld, int d, int k) {
+  T sg[8];
+  for (int j = 0; j < k; ++j) sg[j] = A[j * ld + j] < 0 ? (T)-1 : (T)1;
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < k; ++i) f.R[j * k + i] = i > j ? (T)0 : sg[i] * A[j * ld + i];
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < d; ++i) f.Q[j * d + i] = f.Q[j * d + i] * sg[j];
+  f.status = 0;
+  f.column = 0;
+  for (int j = 0; j < k; ++j) {
+    T p = f.R[j * k + j];
+    if (!(P::finite(p) && p > 0)) {
+      f.status = 1;
+      f.column = j + 1;
+      return;
+    }
+  }
+}
+
+template <class P, class T>
+static void qr_hh(int alg, const T* W, int d, int k, Fact<P, T>& f) {
+  int nref = k < d ? k : k - 1;
+  T U[64];
+  for (int i = 0; i < d * k; ++i) U[i] = 0;
+  if (alg == CL_QR_HOUSEHOLDER) {
+    T A[64];
+    for (int i = 0; i < d * k; ++i) A[i] = W[i];
+    f.count = sweep<P>(A, U, d, k, nref);
+    for (int j = 0; j < k; ++j)
+      for (int i = 0; i < d; ++i) f.Q[j * d + i] = i == j ? (T)1 : (T)0;
+    for (int j = nref - 1; j >= 0; --j) reflect<P>(f.Q, U + j * d, d, k);
+    finish<P>(f, A, d, d, k);
+  } else {  // augmented [W, I]
+    int nc = k + d;
+    T A[128];
+    for (int l = 0; l < d * nc; ++l) {
+      int c = l / d, r = l % d;
+      A[l] = l < d * k ? W[l] : (r == c - k ? (T)1 : (T)0);
+    }
+    f.count = sweep<P>(A, U, d, nc, nref);
+    for (int b = 0; b < k; ++b)
+      for (int a = 0; a < d; ++a) f.Q[b * d + a] = A[(k + a) * d + b];
+    finish<P>(f, A, d, d, k);
+  }
+}
+
+template <class P, class T>
+static void qr_givens(const T* W, int d, int k, Fact<P, T>& f) {
+  int wd = k + d, J = k < d - 1 ? k : d - 1;
+  T rows[8][16];
+  for (int i = 0; i < d; ++i)
+    for (int l = 0; l < wd; ++l) rows[i][l] = l < k ? W[l * d + i] : (l - k == i ? (T)1 : (T)0);
+  int count = 0;
+  for (int j = 0; j < J; ++j)
+    for (int i = d - 1; i >= j + 1; --i) {
+      T ab[2] = {rows[i - 1][j], rows[i][j]};
+      T rho = snorm<P>(ab, 2);
+      if (!(P::finite(rho) && rho > 0)) continue;
+      T c = ab[0] / rho, s = ab[1] / rho;
+      for (int l = 0; l < wd; ++l) {
+        T ra = rows[i - 1][l], rb = rows[i][l];
+        rows[i - 1][l] = c * ra + s * rb;
+        rows[i][l] = c * rb - s * ra;
+      }
+      rows[i - 1][j] = rho;
+      rows[i][j] = 0;
+      count++;
+    }
+  T R0[64];
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < k; ++i) R0[j * k + i] = i > j ? (T)0 : rows[i][j];
+  for (int b = 0; b < k; ++b)
+    for (int a = 0; a < d; ++a) f.Q[b * d + a] = rows[b][k + a];
+  f.count = count;
+  finish<P>(f, R0, k, d, k);
+}
+
+template <class P, class T>
+static T window_gw(const T* R0, int k) {
+  T R[64], Ri[64];
+  T m = P::abs(R0[0]);
+  for (int i = 1; i < k * k; ++i) m = jmax<P>(m, P::abs(R0[i]));
+  bool sc = (m > (T)1e100 || m < (T)1e-100) && P::finite(m) && m > 0;
+  for (int i = 0; i < k * k; ++i) R[i] = sc ? R0[i] / m : R0[i];
+  for (int i = 0; i < k * k; ++i) Ri[i] = 0;
+  for (int j = 0; j < k; ++j) {
+    Ri[j * k + j] = (T)1 / R[j * k + j];
+    for (int i = j - 1; i >= 0; --i) {
+      T s = 0;
+      for (int l = i + 1; l <= j; ++l) s += R[l * k + i] * Ri[j * k + l];
+      Ri[j * k + i] = -s / R[i * k + i];
+    }
+  }
+  T a = R[0] * R[0], b = Ri[0] * Ri[0];
+  for (int i = 1; i < k * k; ++i) {
+    a = a + R[i] * R[i];
+    b = b + Ri[i] * Ri[i];
+  }
+  return P::sqrt(a) * P::sqrt(b);
+}
+
+template <class P, class T>
+static T orth_loss(const T* Q, int d, int k) {
+  T mx = 0;
+  bool first = true;
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < k; ++i) {
+      T acc = Q[i * d] * Q[j * d];
+      for (int l = 1; l < d; ++l) acc = P::fma(Q[i * d + l], Q[j * d + l], acc);
+      T e = P::abs(i == j ? acc - (T)1 : acc - (T)0);
+      mx = first ? e : jmax<P>(mx, e);
+      first = false;
+    }
+  return mx;
+}
+
+// ------------------------------------------------------------------ reducers
+template <class P, class T>
+static void red_qr(const cl_reducer& r, QRSt<T>& st, int64_t n, T* W, int d, int k) {
+  if (st.code) return;
+  Fact<P, T> f;
+  if (r.alg == CL_QR_HOUSEHOLDER || r.alg == CL_QR_AUGMENTED) qr_hh<P>(r.alg, W, d, k, f);
+  else if (r.alg == CL_QR_GIVENS) qr_givens<P>(W, d, k, f);
+  else qr_gs<P>(r.alg, W, d, k, f);
+  auto fail = [&](int code, int64_t col, T v, T thr) {
+    st.code = (int8_t)code;
+    st.column = col;
+    st.value = v;
+    st.threshold = thr;
+    st.step = n;
+  };
+  if (f.status) {
+    if (f.status == 2) fail(2, f.column, (T)5, (T)4);
+    else {
+      T piv = f.R[(f.column - 1) * k + f.column - 1];
+      fail(1, f.column, piv, P::finite(piv) ? (T)0 : (T)INFINITY);
+    }
+    return;
+  }
+  T gw = window_gw<P>(f.R, k);
+  bool cond = r.alg == CL_QR_CGS || r.alg == CL_QR_MGS || r.alg == CL_QR_RGS;
+  if (cond && !(gw * P::eps() < (T)0.5)) {
+    fail(3, 0, gw * P::eps(), (T)0.5);
+    return;
+  }
+  T orth = k >= 2 ? orth_loss<P>(f.Q, d, k) : (T)0;
+  T g = (r.alg == CL_QR_CGS || r.alg == CL_QR_MGS) ? gw : (T)1;
+  T thr = (T)r.tol * P::eps() * (T)k * g;
+  if (k >= 2 && !(orth <= thr)) {
+    fail(4, 0, orth, thr);
+    return;
+  }
+  for (int i = 0; i < k; ++i) st.sums[i] = acc_add<P>(st.sums[i], P::log(f.R[i * k + i]));
+  T G = k == 1 ? (T)1 : (k == 2 ? (gw + P::sqrt(jmax<P>(gw * gw - (T)4, (T)0))) / (T)2 : gw);
+  st.n = n;
+  st.G_max = jmax<P>(st.G_max, G);
+  st.orth_max = jmax<P>(st.orth_max, orth);
+  if (f.count > st.count_max) st.count_max = f.count;
+  st.column = 0;
+  for (int i = 0; i < d * k; ++i) W[i] = f.Q[i];
+}
+
+template <class P, class T>
+static MSums<T> megno_add(MSums<T> m, T a, int64_t n) {
+  T tn = (T)n;
+  Acc<T> R1 = acc_add<P>(m.R1, tn * a), R2 = acc_add<P>(m.R2, tn * tn * a);
+  MSums<T> o;
+  o.s = acc_add<P>(m.s, a);
+  o.R1 = R1;
+  o.R2 = R2;
+  o.Q = acc_add<P>(m.Q, (T)2 * acc_total(R1) / tn);
+  o.Q2 = acc_add<P>(m.Q2, (T)3 * acc_total(R2));
+  return o;
+}
+
+// Column norms of a d x 2 frame, with the zero/non-finite rule shared by MEGNO, SALI, FLI.
+template <class P, class T>
+static bool two_norms(const T* W, int d, T& a1, T& a2, int8_t& col, T& val) {
+  a1 = snorm<P>(W, d);
+  a2 = snorm<P>(W + d, d);
+  bool ok1 = P::finite(a1) && a1 > 0, ok2 = P::finite(a2) && a2 > 0;
+  if (ok1 && ok2) return true;
+  col = ok1 ? 2 : 1;
+  val = ok1 ? a2 : a1;
+  return false;
+}
+
+template <class P, class T>
+static void red_megno(const cl_reducer& r, MegSt<T>& st, int64_t n, T* W, int d) {
+  if (st.code) return;
+  T a1, a2;
+  if (!two_norms<P>(W, d, a1, a2, st.column, st.value)) {
+    st.step = n;
+    st.code = 1;
+    return;
+  }
+  st.c[0] = megno_add<P>(st.c[0], P::log(a1), n);
+  st.c[1] = megno_add<P>(st.c[1], P::log(a2), n);
+  st.n = n;
+  st.K += 1;
+  if (r.replace)
+    for (int i = 0; i < d; ++i) {
+      W[i] = W[i] / a1;
+      W[d + i] = W[d + i] / a2;
+    }
+}
+
+template <class P, class T>
+static void red_sali(SaliSt<T>& st, int64_t n, T* W, int d) {
+  if (st.code) return;
+  T a1, a2;
+  if (!two_norms<P>(W, d, a1, a2, st.column, st.value)) {
+    st.step = n;
+    st.code = 1;
+    return;
+  }
+  T u1[8], u2[8], dm[8], dp[8];
+  for (int i = 0; i < d; ++i) {
+    u1[i] = W[i] / a1;
+    u2[i] = W[d + i] / a2;
+    dm[i] = u1[i] - u2[i];
+    dp[i] = u1[i] + u2[i];
+  }
+  T dmin = snorm<P>(dm, d), dplus = snorm<P>(dp, d);
+  st.s1 = acc_add<P>(st.s1, P::log(a1));
+  st.s2 = acc_add<P>(st.s2, P::log(a2));
+  st.sali = jmin<P>(dmin, dplus);
+  st.n = n;
+  for (int i = 0; i < d; ++i) {
+    W[i] = u1[i];
+    W[d + i] = u2[i];
+  }
+}
+
+template <class P, class T>
+static void red_fli(const cl_reducer& r, FliSt<T>& st, int64_t n, T* W, int d) {
+  if (st.code) return;
+  T a1, a2;
+  if (!two_norms<P>(W, d, a1, a2, st.column, st.value)) {
+    st.step = n;
+    st.code = 1;
+    return;
+  }
+  Acc<T> s1 = acc_add<P>(st.s1, P::log(a1)), s2 = acc_add<P>(st.s2, P::log(a2));
+  T S1 = acc_total(s1), S2 = acc_total(s2);
+  if (S1 > st.f1) {
+    st.f1 = S1;
+    st.step1 = n;
+  }
+  if (S1 > st.fB) {
+    st.fB = S1;
+    st.stepB = n;
+    st.colB = 1;
+  }
+  if (S2 > st.fB) {
+    st.fB = S2;
+    st.stepB = n;
+    st.colB = 2;
+  }
+  st.s1 = s1;
+  st.s2 = s2;
+  st.n = n;
+  if (r.replace)
+    for (int i = 0; i < d; ++i) {
+      W[i] = W[i] / a1;
+      W[d + i] = W[d + i] / a2;
+    }
+}
+
+template <class P, class T>
+static void red_ofli(const cl_reducer& r, OfliSt<T>& st, int64_t n, const T* fld, T* W, int d, int k) {
+  if (st.code) return;
+  T fn = snorm<P>(fld, d);
+  if (!(P::finite(fn) && fn > P::floatmin())) {
+    st.value = fn;
+    st.step = n;
+    st.code = 2;
+    st.column = 0;
+    return;
+  }
+  T u[8], al[8], be[8];
+  for (int i = 0; i < d; ++i) u[i] = fld[i] / fn;
+  for (int j = 0; j < k; ++j) {
+    const T* w = W + j * d;
+    al[j] = snorm<P>(w, d);
+    T dw = vdot(u, w, d), v[8];
+    for (int i = 0; i < d; ++i) v[i] = w[i] - dw * u[i];
+    be[j] = snorm<P>(v, d);
+  }
+  for (int j = 0; j < k; ++j)
+    if (!(P::finite(al[j]) && al[j] > 0)) {
+      st.value = al[j];
+      st.step = n;
+      st.code = 1;
+      st.column = (int8_t)(j + 1);
+      return;
+    }
+  Acc<T> na[8];
+  for (int i = 0; i < k; ++i) na[i] = acc_add<P>(st.acc[i], P::log(al[i]));
+  T inst = (T)-INFINITY;
+  for (int i = 0; i < k; ++i) {
+    T g = be[i] > 0 ? acc_total(st.acc[i]) + P::log(be[i]) : (T)-INFINITY;
+    T sn = be[i] / al[i], S = acc_total(na[i]);
+    st.sinmin = jmin<P>(st.sinmin, sn);
+    st.nminf += be[i] > 0 ? 0 : 1;
+    inst = jmax<P>(inst, g);
+    if (g > st.gB) {
+      st.gB = g;
+      st.stepB = n;
+      st.colB = (int8_t)(i + 1);
+      st.sinB = sn;
+    }
+    if (i == 0 && g > st.g1) {
+      st.g1 = g;
+      st.step1 = n;
+      st.sin1 = sn;
+    }
+    if (i == 0 && S > st.f1) st.f1 = S;
+    if (S > st.fB) st.fB = S;
+  }
+  for (int i = 0; i < k; ++i) st.acc[i] = na[i];
+  st.inst = inst;
+  st.n = n;
+  if (r.replace)
+    for (int j = 0; j < k; ++j)
+      for (int i = 0; i < d; ++i) W[j * d + i] = W[j * d + i] / al[j];
+}
+
+template <class P, class T>
+static void red_gali(GaliSt<T>& st, int64_t n, T* W, int d, int k) {
+  if (st.code) return;
+  T a[8];
+  for (int j = 0; j < k; ++j) a[j] = snorm<P>(W + j * d, d);
+  for (int j = 0; j < k; ++j)
+    if (!(P::finite(a[j]) && a[j] >= P::floatmin())) {
+      st.value = a[j];
+      st.step = n;
+      st.code = 1;
+      st.column = j + 1;
+      return;
+    }
+  for (int i = 0; i < k; ++i) {
+    Acc<T> q;
+    q.s = st.s[i];
+    q.c = st.c[i];
+    q = acc_add<P>(q, P::log(a[i]));
+    st.s[i] = q.s;
+    st.c[i] = q.c;
+  }
+  T U[64];
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < d; ++i) {
+      U[j * d + i] = W[j * d + i] / a[j];
+      W[j * d + i] = U[j * d + i];
+    }
+  T g = 1;
+  for (int j = 0; j < k; ++j) {
+    for (int pass = 0; pass < 2; ++pass)
+      for (int p = 0; p < j; ++p) {
+        T c = 0;
+        for (int i = 0; i < d; ++i) c += U[p * d + i] * U[j * d + i];
+        for (int i = 0; i < d; ++i) U[j * d + i] -= c * U[p * d + i];
+      }
+    T rr = snorm<P>(U + j * d, d);
+    T s = rr == 0 ? (T)1 : rr;
+    for (int i = 0; i < d; ++i) U[j * d + i] /= s;
+    if (j > 0) {
+      g *= rr;
+      st.gali[j - 1] = g;
+    }
+  }
+  st.n = n;
+}
+
+template <class P, class T>
+static void red_lognorm(LogNormSt<T>& st, T* W, int d, int k) {
+  for (int j = 0; j < k; ++j) {
+    T s = W[j * d] * W[j * d];
+    for (int i = 1; i < d; ++i) s = s + W[j * d + i] * W[j * d + i];
+    s = P::sqrt(s);
+    st.a[j] = acc_add<P>(st.a[j], P::log(s));
+    for (int i = 0; i < d; ++i) W[j * d + i] = W[j * d + i] / s;
+  }
+}
+
+template <class T>
+static bool red_failed(const cl_reducer& r, const RState<T>& s) {
+  switch (r.kind) {
+    case CL_RED_QR: return s.qr.code != 0;
+    case CL_RED_BENETTIN: return s.ben.code != 0;
+    case CL_RED_MEGNO: return s.meg.code != 0;
+    case CL_RED_SALI: return s.sali.code != 0;
+    case CL_RED_FLI: return s.fli.code != 0;
+    case CL_RED_OFLI: return s.ofli.code != 0;
+    case CL_RED_GALI: return s.gali.code != 0;
+  }
+  return false;
+}
+
+// ------------------------------------------------------------------ systems (d = 2 maps)
+#define OR_SYS_LINEAR 100  // test-only linear map x' = A x, tangent A (test/tangent-core/tc_helpers.jl:8-10,26-28)
+
+template <class P, class T>
+struct MapSys {
+  int kind;
+  T K;
+  int s;     // state length
+  T A[4];    // linear map, column-major (kind OR_SYS_LINEAR)
+  // tangent at x applied to W (d x k), explicit-fma product; then the step
+  void step(T* x, T* const* W, int nf, int k) const {
+    if (kind == OR_SYS_LINEAR) {
+      for (int f = 0; f < nf; ++f)
+        for (int j = 0; j < k; ++j) {
+          T w0 = W[f][2 * j], w1 = W[f][2 * j + 1];
+          W[f][2 * j] = P::fma(A[2], w1, A[0] * w0);
+          W[f][2 * j + 1] = P::fma(A[3], w1, A[1] * w0);
+        }
+      T x0 = x[0], x1 = x[1];
+      x[0] = A[0] * x0 + A[2] * x1;
+      x[1] = A[1] * x0 + A[3] * x1;
+      return;
+    }
+    T sn = P::sin(x[0]), cs = P::cos(x[0]);
+    T c = K * cs, m11 = (T)1 + c;
+    for (int f = 0; f < nf; ++f)
+      for (int j = 0; j < k; ++j) {
+        T w0 = W[f][2 * j], w1 = W[f][2 * j + 1];
+        W[f][2 * j] = P::fma((T)1, w1, m11 * w0);
+        W[f][2 * j + 1] = P::fma((T)1, w1, c * w0);
+      }
+    if (kind == CL_SYS_STANDARD_MAP) {
+      T p = x[1] + K * sn;
+      x[0] = x[0] + p;
+      x[1] = p;
+    } else {
+      T tp = (T)2 * P::pi();
+      auto red = [&](T u, T& rr, T& a) {
+        a = P::floor(u / tp);
+        rr = u - tp * a;
+        if (rr < 0) {
+          rr += tp;
+          a -= 1;
+        }
+        if (rr >= tp) {
+          rr -= tp;
+          a += 1;
+        }
+      };
+      T p, a, xx, b;
+      red(x[1] + K * sn, p, a);
+      T nn = x[3] + a;
+      red(x[0] + p, xx, b);
+      T mm = x[2] + nn + b;
+      T lim = (T)9007199254740992.0;
+      if (P::abs(mm) < lim && P::abs(nn) < lim) {
+        x[0] = xx;
+        x[1] = p;
+        x[2] = mm;
+        x[3] = nn;
+      } else {
+        T nan = (T)NAN;
+        x[0] = x[1] = x[2] = x[3] = nan;
+      }
+    }
+  }
+  // map displacement field at x (OFLI direction on a map)
+  void field(const T* x, T* f) const {
+    T ks = K * P::sin(x[0]);
+    f[0] = x[1] + ks;
+    f[1] = ks;
+  }
+};
+
+// ------------------------------------------------------------------ the per-orbit body
+template <class P, class T>
+static int8_t run_orbit(const MapSys<P, T>& sys, const cl_chain& ch, int d, int k, int64_t N, int tau, T* x,
+                        T* W0, T* W1, RState<T>* st, int64_t* hstep) {
+  for (int r = 0; r < ch.n; ++r) init_state(ch.r[r], st[r], k);
+  T* W[2] = {W0, W1};
+  int nf = 1;
+  for (int r = 0; r < ch.n; ++r)
+    if (ch.r[r].frame == 1) nf = 2;
+  auto status = [&]() -> int8_t {
+    for (int i = 0; i < sys.s; ++i)
+      if (!P::finite(x[i])) return 1;
+    for (int f = 0; f < nf; ++f)
+      for (int i = 0; i < d * k; ++i)
+        if (!P::finite(W[f][i])) return 2;
+    return 0;
+  };
+  int64_t n = 0;
+  for (int64_t h = 0; h < N / tau; ++h) {
+    for (int t = 0; t < tau; ++t) sys.step(x, W, nf, k);
+    n += tau;
+    int8_t code = status();
+    if (code) {
+      *hstep = n;
+      return code;
+    }
+    for (int r = 0; r < ch.n; ++r) {
+      const cl_reducer& rd = ch.r[r];
+      T* w = W[rd.frame];
+      switch (rd.kind) {
+        case CL_RED_QR: red_qr<P>(rd, st[r].qr, n, w, d, k); break;
+        case CL_RED_BENETTIN: {
+          BenSt<T>& b = st[r].ben;
+          if (b.code) break;
+          T al = snorm<P>(w, d);
+          if (!(P::finite(al) && al > 0)) {
+            b.code = 1;
+            b.value = al;
+            b.step = n;
+            break;
+          }
+          b.sum = acc_add<P>(b.sum, P::log(al));
+          b.n = n;
+          for (int i = 0; i < d; ++i) w[i] = w[i] / al;
+          break;
+        }
+        case CL_RED_MEGNO: red_megno<P>(rd, st[r].meg, n, w, d); break;
+        case CL_RED_SALI: red_sali<P>(st[r].sali, n, w, d); break;
+        case CL_RED_FLI: red_fli<P>(rd, st[r].fli, n, w, d); break;
+        case CL_RED_OFLI: {
+          T f[8];
+          sys.field(x, f);
+          red_ofli<P>(rd, st[r].ofli, n, f, w, d, k);
+          break;
+        }
+        case CL_RED_GALI: red_gali<P>(st[r].gali, n, w, d, k); break;
+        case 8: red_lognorm<P>(st[r].ln, w, d, k); break;
+      }
+    }
+    for (int r = 0; r < ch.n; ++r)
+      if (red_failed(ch.r[r], st[r])) {
+        *hstep = n;
+        return 3;
+      }
+  }
+  *hstep = n;
+  return status();
+}
+
+template <class P>
+static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int d, int k, int64_t N,
+                         int tau, double* x, double* W, double* W2, uint8_t* states, int8_t* code, int64_t* hstep,
+                         int nthreads, const double* A = nullptr) {
+  typedef typename P::T T;
+  MapSys<P, T> ms;
+  ms.kind = sys->kind;
+  ms.K = (T)sys->K;
+  ms.s = s;
+  for (int i = 0; i < 4; ++i) ms.A[i] = A ? (T)A[i] : (T)0;
+  int sb = 0;
+  int off[CL_MAX_REDUCERS];
+  for (int r = 0; r < ch->n; ++r) {
+    off[r] = sb;
+    sb += red_bytes(ch->r[r], d, k);
+  }
+  std::atomic<int64_t> next(0);
+  auto worker = [&]() {
+    std::vector<RState<T>> st(ch->n > 0 ? ch->n : 1);
+    while (true) {
+      int64_t lo = next.fetch_add(16);
+      if (lo >= m) break;
+      int64_t hi = lo + 16 < m ? lo + 16 : m;
+      for (int64_t i = lo; i < hi; ++i) {
+        T xx[4], w0[64], w1[64];
+        for (int c = 0; c < s; ++c) xx[c] = (T)x[i * s + c];
+        for (int e = 0; e < d * k; ++e) {
+          w0[e] = (T)W[i * d * k + e];
+          w1[e] = W2 ? (T)W2[i * d * k + e] : (T)0;
+        }
+        code[i] = run_orbit<P, T>(ms, *ch, d, k, N, tau, xx, w0, w1, st.data(), &hstep[i]);
+        for (int c = 0; c < s; ++c) x[i * s + c] = dbl(xx[c]);
+        for (int e = 0; e < d * k; ++e) {
+          W[i * d * k + e] = dbl(w0[e]);
+          if (W2) W2[i * d * k + e] = dbl(w1[e]);
+        }
+        if (states)
+          for (int r = 0; r < ch->n; ++r) write_state(ch->r[r], st[r], d, k, states + i * sb + off[r]);
+      }
+    }
+  };
+  if (nthreads <= 1) {
+    worker();
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+  }
+  return 0;
+}
+
+extern "C" {
+
+// Bytes per orbit of the chain's state tuple (Julia layout), -1 for an unknown reducer.
+int64_t or_state_bytes(const cl_chain* ch, int d, int k) {
+  int64_t b = 0;
+  for (int r = 0; r < ch->n; ++r) {
+    int x = red_bytes(ch->r[r], d, k);
+    if (x < 0) return -1;
+    b += x;
+  }
+  return b;
+}
+
+// Full runs of N steps from fresh states (n0 = 0) on host AoS buffers; `math` selects the
+// elementary functions (MATH_LIBM or MATH_CL); nthreads worker threads over groups of 16 orbits.
+int or_propagate(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int d, int k, int64_t N, int tau,
+                 double* x, double* W, double* W2, void* states, int8_t* code, int64_t* hstep, int math,
+                 int nthreads) {
+  if (sys->kind != CL_SYS_STANDARD_MAP && sys->kind != CL_SYS_WRAPPED_STANDARD_MAP) return 3;
+  if (d != 2 || k < 1 || k > 2 || N % tau) return 10;
+  if (math == MATH_CL)
+    return propagate_all<F64Cl>(sys, ch, m, s, d, k, N, tau, x, W, W2, (uint8_t*)states, code, hstep, nthreads);
+  return propagate_all<F64Libm>(sys, ch, m, s, d, k, N, tau, x, W, W2, (uint8_t*)states, code, hstep, nthreads);
+}
+
+// A linear map x' = A x (A column-major 2 x 2): the closed-form fixtures of the reference's suites
+// (Y5 / cat map [[2,1],[1,1]]). math as in or_propagate; quad != 0 runs in __float128.
+int or_propagate_linear(const double* A, const cl_chain* ch, int64_t m, int k, int64_t N, int tau, double* x,
+                        double* W, void* states, int8_t* code, int64_t* hstep, int quad_) {
+  cl_system sys;
+  memset(&sys, 0, sizeof(sys));
+  sys.kind = OR_SYS_LINEAR;
+  if (k < 1 || k > 2 || N % tau) return 10;
+  if (quad_)
+    return propagate_all<Quad>(&sys, ch, m, 2, 2, k, N, tau, x, W, nullptr, (uint8_t*)states, code, hstep, 1, A);
+  return propagate_all<F64Cl>(&sys, ch, m, 2, 2, k, N, tau, x, W, nullptr, (uint8_t*)states, code, hstep, 1, A);
+}
+
+// n additions of x into a compensated sum (Foundation.jl:48-58), and the plain sum, for the AC9 fixture.
+void or_csum(int64_t n, double x, double* comp, double* plain) {
+  Acc<double> a;
+  a.s = a.c = 0.0;
+  double p = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    a = acc_add<F64Libm>(a, x);
+    p += x;
+  }
+  *comp = acc_total(a);
+  *plain = p;
+}
+void or_csum_list(int64_t n, const double* v, double* comp) {
+  Acc<double> a;
+  a.s = a.c = 0.0;
+  for (int64_t i = 0; i < n; ++i) a = acc_add<F64Libm>(a, v[i]);
+  *comp = acc_total(a);
+}
+
+// The same run in __float128 (libquadmath): inputs/outputs are rounded to double, states are the
+// Julia layout with each value rounded to double.
+int or_propagate_quad(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int d, int k, int64_t N, int tau,
+                      double* x, double* W, double* W2, void* states, int8_t* code, int64_t* hstep, int nthreads) {
+  if (sys->kind != CL_SYS_STANDARD_MAP && sys->kind != CL_SYS_WRAPPED_STANDARD_MAP) return 3;
+  if (d != 2 || k < 1 || k > 2 || N % tau) return 10;
+  return propagate_all<Quad>(sys, ch, m, s, d, k, N, tau, x, W, W2, (uint8_t*)states, code, hstep, nthreads);
+}
+
+// Seeded frames of the documented counter rule (the same rule the library implements):
+// key = mix(mix(seed) + orbit), uniforms u(t), Box-Muller normals, two MGS passes per column, unit columns.
+static uint64_t mixr(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+int or_initial_deviations(uint64_t seed, int64_t orbit0, int64_t m, int d, int k, double* W) {
+  if (d > 8 || k > d || k < 1) return 10;
+  for (int64_t o = 0; o < m; ++o) {
+    uint64_t key = mixr(mixr(seed) + (uint64_t)(orbit0 + o));
+    double G[64];
+    for (int j = 0; j < k; ++j)
+      for (int i = 0; i < d; ++i) {
+        uint64_t t = (uint64_t)(j * d + i), p = t >> 1;
+        double u1 = ((double)(mixr(key + (2 * p + 1) * 0x9e3779b97f4a7c15ull) >> 12) + 0.5) * 0x1p-52;
+        double u2 = ((double)(mixr(key + (2 * p + 2) * 0x9e3779b97f4a7c15ull) >> 12) + 0.5) * 0x1p-52;
+        double r = sqrt(-2.0 * cl_log(u1)), th = 6.283185307179586 * u2;
+        G[j * d + i] = (t & 1) ? r * cl_sin(th) : r * cl_cos(th);
+      }
+    for (int j = 0; j < k; ++j) {
+      double* g = G + j * d;
+      for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < j; ++i) {
+          double c = vdot(G + i * d, g, d);
+          for (int l = 0; l < d; ++l) g[l] = g[l] - c * G[i * d + l];
+        }
+      double nr = snorm<F64Cl>(g, d);
+      for (int l = 0; l < d; ++l) g[l] = g[l] / nr;
+    }
+    for (int e = 0; e < d * k; ++e) W[o * d * k + e] = G[e];
+  }
+  return 0;
+}
+
+}  // extern "C"

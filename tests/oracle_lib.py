@@ -1,0 +1,94 @@
+"""ctypes binding of oracle/liboracle.so for the tests and the CPU-baseline leg of bench.py.
+
+The oracle is the checker: a CPU restatement of the reference path (see oracle/oracle.cpp)."""
+import ctypes
+import os
+
+import numpy as np
+
+from chaos_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+MATH_LIBM, MATH_CL = 0, 1
+_o = None
+
+
+def load():
+    global _o
+    if _o is None:
+        if not os.path.exists(PATH):
+            raise ImportError(f"{PATH} missing: run `make -C oracle`")
+        o = ctypes.CDLL(PATH)
+        P, I64, I = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+        o.or_state_bytes.restype = I64
+        o.or_state_bytes.argtypes = [ctypes.POINTER(L.cl_chain), I, I]
+        o.or_propagate.restype = I
+        o.or_propagate.argtypes = [ctypes.POINTER(L.cl_system), ctypes.POINTER(L.cl_chain), I64, I, I, I, I64, I,
+                                   P, P, P, P, P, P, I, I]
+        o.or_propagate_quad.restype = I
+        o.or_propagate_quad.argtypes = [ctypes.POINTER(L.cl_system), ctypes.POINTER(L.cl_chain), I64, I, I, I, I64,
+                                        I, P, P, P, P, P, P, I]
+        o.or_propagate_linear.restype = I
+        o.or_propagate_linear.argtypes = [P, ctypes.POINTER(L.cl_chain), I64, I, I64, I, P, P, P, P, P, I]
+        o.or_initial_deviations.restype = I
+        o.or_initial_deviations.argtypes = [ctypes.c_uint64, I64, I64, I, I, P]
+        o.or_csum.restype = None
+        o.or_csum.argtypes = [I64, ctypes.c_double, P, P]
+        o.or_csum_list.restype = None
+        o.or_csum_list.argtypes = [I64, P, P]
+        _o = o
+    return _o
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def run(sysd, chain, x0, W0, N, tau, k, W2=None, math=MATH_CL, nthreads=None, quad=False):
+    """Full N-step runs. x0 (m, s), W0 (m, d, k) as numpy; returns dict of x, W, W2, raw states, code, hstep."""
+    o = load()
+    x = np.ascontiguousarray(np.array(x0, dtype=np.float64))
+    m, s = x.shape
+    d = 2
+    W = np.array(np.asarray(W0, dtype=np.float64).reshape(m, d, k).transpose(0, 2, 1), order="C", copy=True)
+    W2a = None if W2 is None else np.array(np.asarray(W2, dtype=np.float64).reshape(m, d, k).transpose(0, 2, 1), order="C", copy=True)
+    nb = o.or_state_bytes(ctypes.byref(chain), d, k)
+    states = np.zeros((m, max(nb, 1)), dtype=np.uint8)
+    code = np.zeros(m, dtype=np.int8)
+    hstep = np.zeros(m, dtype=np.int64)
+    nt = nthreads or min(16, os.cpu_count() or 1)
+    if quad:
+        rc = o.or_propagate_quad(ctypes.byref(sysd), ctypes.byref(chain), m, s, d, k, N, tau, _ptr(x), _ptr(W),
+                                 _ptr(W2a), _ptr(states), _ptr(code), _ptr(hstep), nt)
+    else:
+        rc = o.or_propagate(ctypes.byref(sysd), ctypes.byref(chain), m, s, d, k, N, tau, _ptr(x), _ptr(W), _ptr(W2a),
+                            _ptr(states), _ptr(code), _ptr(hstep), math, nt)
+    assert rc == 0, rc
+    out = dict(x=x, W=W.transpose(0, 2, 1).copy(), states=states[:, :nb], code=code, hstep=hstep)
+    if W2a is not None:
+        out["W2"] = W2a.transpose(0, 2, 1).copy()
+    return out
+
+
+def run_linear(A, chain, x0, W0, N, tau, k, quad=False):
+    o = load()
+    x = np.ascontiguousarray(np.array(x0, dtype=np.float64))
+    m = x.shape[0]
+    W = np.array(np.asarray(W0, dtype=np.float64).reshape(m, 2, k).transpose(0, 2, 1), order="C", copy=True)
+    Ac = np.ascontiguousarray(np.asarray(A, dtype=np.float64).T.ravel())  # column-major
+    nb = o.or_state_bytes(ctypes.byref(chain), 2, k)
+    states = np.zeros((m, max(nb, 1)), dtype=np.uint8)
+    code = np.zeros(m, dtype=np.int8)
+    hstep = np.zeros(m, dtype=np.int64)
+    rc = o.or_propagate_linear(_ptr(Ac), ctypes.byref(chain), m, k, N, tau, _ptr(x), _ptr(W), _ptr(states), _ptr(code),
+                               _ptr(hstep), int(quad))
+    assert rc == 0, rc
+    return dict(x=x, W=W.transpose(0, 2, 1).copy(), states=states[:, :nb], code=code, hstep=hstep)
+
+
+def frames(seed, orbit0, m, d, k):
+    o = load()
+    out = np.zeros((m, k, d), dtype=np.float64)
+    assert o.or_initial_deviations(seed & (2 ** 64 - 1), orbit0, m, d, k, out.ctypes.data) == 0
+    return out.transpose(0, 2, 1).copy()
