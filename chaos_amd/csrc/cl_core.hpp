@@ -681,8 +681,8 @@ struct ColLogs {
   double a1, a2, l1, l2;
   bool ok1, ok2;
 };
-template <int D>
-CL_HD ColLogs col_logs(const Frame<D, 2>& W) {
+template <int D, int K>
+CL_HD ColLogs col_logs(const Frame<D, K>& W) {
   ColLogs c;
   c.a1 = safe_norm<D>(W.col(0));
   c.a2 = safe_norm<D>(W.col(1));
@@ -725,7 +725,9 @@ struct MEGNO {
 #pragma unroll
     for (int i = 0; i < 6; ++i) st.pad_[i] = 0;
   }
-  CL_HD void step_logs(State& st, int64_t n, Frame<D, 2>& W, ColLogs& c) const {
+  // Reads columns 1 and 2 of the frame (k = 2 in the reference; with k > 2 it only reads them).
+  template <int K>
+  CL_HD void step_logs(State& st, int64_t n, Frame<D, K>& W, ColLogs& c) const {
     if (st.code != 0) return;
     if (!(c.ok1 && c.ok2)) {
       st.column = c.ok1 ? 2 : 1;
@@ -848,7 +850,8 @@ struct OFLI {
 #pragma unroll
     for (int i = 0; i < 5; ++i) st.pad_[i] = 0;
   }
-  CL_HD void step(State& st, int64_t n, const double* fld, Frame<D, K>& W) const {
+  // `lc` (k = 2 only): column norms and logs of this frame already formed at this handoff.
+  CL_HD void step(State& st, int64_t n, const double* fld, Frame<D, K>& W, const ColLogs* lc = nullptr) const {
     if (st.code != 0) return;
     double fn = safe_norm<D>(fld);
     if (!(cl_isfinite(fn) && fn > 0x1p-1022)) {
@@ -862,7 +865,7 @@ struct OFLI {
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       const double* w = W.col(j);
-      al[j] = safe_norm<D>(w);
+      al[j] = (K == 2 && lc) ? (j == 0 ? lc->a1 : lc->a2) : safe_norm<D>(w);
       double dw = dot<D>(u, w);
       double v[D];
 #pragma unroll
@@ -882,7 +885,8 @@ struct OFLI {
     int8_t colB = st.colB;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
-      CSum na = csum_add(st.acc[i], cl_log(al[i]));
+      double la = (K == 2 && lc) ? (i == 0 ? lc->l1 : lc->l2) : cl_log(al[i]);
+      CSum na = csum_add(st.acc[i], la);
       double g = be[i] > 0.0 ? csum_total(st.acc[i]) + cl_log(be[i]) : -__builtin_inf();
       double sn = be[i] / al[i], S = csum_total(na);
       sinmin = cl_jmin(sinmin, sn);

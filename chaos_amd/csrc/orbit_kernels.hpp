@@ -9,6 +9,7 @@
 #pragma once
 #include <stdint.h>
 #include "cl_core.hpp"
+#include "cl_gauss.hpp"
 #include "../../include/chaoslab_hip.h"
 
 namespace cl {
@@ -54,7 +55,7 @@ struct Cons {
 template <int D, int K, int NF, class Ctx>
 CL_HD void ensure_logs(Ctx& c, int fr) {
   if (!c.logs_valid[fr]) {
-    ColLogs L = col_logs<D>(c.W[fr]);
+    ColLogs L = col_logs(c.W[fr]);
     if (L.ok1 && L.ok2) {
       L.l1 = cl_log(L.a1);
       L.l2 = cl_log(L.a2);
@@ -97,7 +98,12 @@ template <int D, int K, int FR, class Ctx>
 CL_HD void slot_step(Slot<OFLI<D, K>, FR>& s, int64_t n, Ctx& c) {
   double f[D];
   c.sys.field(c.x, c.aux, f);
-  s.r.step(s.st, n, f, c.W[FR]);
+  if constexpr (K == 2) {
+    ensure_logs<D, 2, 0>(c, FR);
+    s.r.step(s.st, n, f, c.W[FR], &c.logs[FR]);
+  } else {
+    s.r.step(s.st, n, f, c.W[FR]);
+  }
   if (s.r.replace) c.logs_valid[FR] = false;
 }
 template <int D, int K, int FR, class Ctx>

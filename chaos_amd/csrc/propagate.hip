@@ -28,7 +28,10 @@ static Instance g_inst[256];
 static int g_ninst = 0;
 
 static void load_table() {
-  std::call_once(g_once, [] { register_instances_map(g_inst, &g_ninst, 256); });
+  std::call_once(g_once, [] {
+    register_instances_map(g_inst, &g_ninst, 256);
+    register_instances_flow(g_inst, &g_ninst, 256);
+  });
 }
 
 static const Instance* find_instance(const cl_system* sys, const cl_chain* ch, int d, int k) {
@@ -51,6 +54,7 @@ static int state_dim(int kind) {
   switch (kind) {
     case CL_SYS_STANDARD_MAP: return 2;
     case CL_SYS_WRAPPED_STANDARD_MAP: return 4;
+    case CL_SYS_HENON_HEILES_GAUSS: return 4;
     default: return -1;
   }
 }
@@ -58,6 +62,7 @@ static int sys_dim(int kind) {
   switch (kind) {
     case CL_SYS_STANDARD_MAP:
     case CL_SYS_WRAPPED_STANDARD_MAP: return 2;
+    case CL_SYS_HENON_HEILES_GAUSS: return 4;
     default: return -1;
   }
 }
@@ -86,6 +91,8 @@ static int validate(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int e
   int sd = sys_dim(sys->kind);
   if (sd < 0) return set_err(ctx, CL_ERR_UNSUPPORTED, "system kind has no kernel");
   if (d != sd) return set_err(ctx, CL_ERR_ARGUMENT, "d is not the system dimension");
+  if (sys->kind == CL_SYS_HENON_HEILES_GAUSS && sys->stages != 3)
+    return set_err(ctx, CL_ERR_UNSUPPORTED, "Gauss kernels are built for s = 3 stages");
   if (s != state_dim(sys->kind)) return set_err(ctx, CL_ERR_ARGUMENT, "state length does not match the system");
   if (!(k >= 1 && k <= d)) return set_err(ctx, CL_ERR_ARGUMENT, "k must be in 1:d");
   if (elt != CL_ELT_F64) return set_err(ctx, CL_ERR_UNSUPPORTED, "element type has no kernel for this system");

@@ -31,6 +31,15 @@ inline WrappedStdMap make_sys<WrappedStdMap>(const cl_system& s) {
   return m;
 }
 
+template <>
+inline HHGauss make_sys<HHGauss>(const cl_system& s) {
+  HHGauss m;
+  m.h = s.h;
+  m.tol = s.tol;
+  m.maxiter = s.maxiter > 0 ? s.maxiter : 50;
+  return m;
+}
+
 // Launch geometry: a big ensemble fills the chip with 256-lane groups; a small one (below 64K
 // orbits) uses 64-lane groups so its waves spread over as many CUs as possible.
 inline int block_for(int64_t m) { return m >= 65536 ? 256 : 64; }
@@ -107,5 +116,6 @@ Instance make_instance() {
 
 // Each instance translation unit appends its entries.
 void register_instances_map(Instance* out, int* n, int cap);
+void register_instances_flow(Instance* out, int* n, int cap);
 
 }  // namespace cl

@@ -49,3 +49,40 @@ def uniform_k(seed, m, lo, hi, orbit0=1):
     """Per-orbit parameter drawn uniform on [lo, hi) from the orbit's initial-condition stream."""
     keys = orbit_keys(seed, m, orbit0) ^ IC_SALT
     return lo + (hi - lo) * uniforms(keys, 2)
+
+
+def hh_section_starts(seed, m, E, section="q2", orbit0=1, max_tries=64):
+    """(m, 4) starts on the energy surface H = E of the Henon-Heiles flow, uniform on a Poincare section.
+
+    section="q2": the literal reading of the workload (y = q2 = 0, p_y = p2 > 0 solved from E): (q1, p1) drawn
+    uniform in the box |q1|, |p1| <= sqrt(2E) and accepted where 2E - q1^2 - p1^2 > 0, then p2 = sqrt(that).
+    section="q1": the reference's convention (src/Systems/henon_heiles.jl:62-68): q1 = 0, (q2, p2) drawn in the
+    box [a, b] x [-sqrt(2E), sqrt(2E)] (b, a the roots of q2^2/2 - q2^3/3 = E around 0) and p1 = sqrt(2(E - p2^2/2
+    - q2^2/2 + q2^3/3)) > 0. Rejection draws use the orbit's initial-condition stream, attempt t drawing
+    uniforms 2t+3 and 2t+4."""
+    keys = orbit_keys(seed, m, orbit0) ^ IC_SALT
+    out = np.full((m, 4), np.nan)
+    todo = np.ones(m, dtype=bool)
+    r = np.sqrt(2 * E)
+    if section == "q1":
+        from scipy.optimize import brentq
+        u = lambda q: q * q / 2 - q ** 3 / 3 - E
+        a, b = (brentq(u, -1.0, 0.0), brentq(u, 0.0, 1.0)) if E < 1 / 6 else (-0.5, 1.0)
+    for t in range(max_tries):
+        if not todo.any():
+            break
+        u1, u2 = uniforms(keys, 2 * t + 3), uniforms(keys, 2 * t + 4)
+        if section == "q2":
+            q1, p1 = -r + 2 * r * u1, -r + 2 * r * u2
+            rem = 2 * E - q1 * q1 - p1 * p1
+            ok = todo & (rem > 0)
+            out[ok] = np.stack([q1, np.zeros(m), p1, np.sqrt(np.where(rem > 0, rem, 0))], axis=1)[ok]
+        else:
+            q2, p2 = a + (b - a) * u1, -r + 2 * r * u2
+            rem = 2 * (E - p2 * p2 / 2 - (q2 * q2 / 2 - q2 ** 3 / 3))
+            ok = todo & (rem > 0) & (np.abs(q2) <= 1)
+            out[ok] = np.stack([np.zeros(m), q2, np.sqrt(np.where(rem > 0, rem, 0)), p2], axis=1)[ok]
+        todo &= ~ok
+    if todo.any():
+        raise RuntimeError(f"{int(todo.sum())} starts found no admissible point in {max_tries} draws")
+    return out

@@ -148,7 +148,9 @@ class MEGNO(Reducer):
         self.replace = bool(replace)
 
     def check(self, d, k):
-        _k2("MEGNO", k)
+        # k = 2 as the reference; a wider frame is read (columns 1 and 2) only when MEGNO does not replace it
+        if k != 2 and (self.replace or k < 2):
+            raise ValueError(f"MEGNO needs k = 2, got k = {k} (k > 2 only with replace=False: columns 1-2 are read)")
 
     def dtype(self, d, k):
         return np.dtype([("c", "<f8", (2, 5, 2)), ("value", "<f8"), ("n", "<i8"), ("K", "<i8"), ("step", "<i8"),

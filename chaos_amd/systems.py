@@ -1,6 +1,8 @@
 """System descriptors: the maps and flows the kernels integrate, named as in the reference."""
 import math
 
+import numpy as np
+
 from . import _lib as L
 
 
@@ -39,6 +41,44 @@ class WrappedStandardMap:
 
     def __repr__(self):
         return f"WrappedStandardMap({self.K!r})"
+
+
+class HenonHeilesGauss:
+    """IntegratorMap(Gauss(Float64, 3; tol, maxiter), HenonHeilesFlow(), h): one step of the s = 3 Gauss-Legendre
+    method on H = |p|^2/2 + |q|^2/2 + q1^2 q2 - q2^3/3, state (q1, q2, p1, p2)
+    (src/Integrators/integrator_map.jl:12-23, src/Integrators/gauss.jl:49-69, src/Systems/flow_models.jl:134,205-207).
+    `tol=None` is the reference default acceptance 8 eps max(1, |Y|)."""
+    dimension = 4
+    state_length = 4
+
+    def __init__(self, h=0.01, tol=None, maxiter=50, stages=3):
+        if stages != 3:
+            raise ValueError("the kernels are built for the s = 3 Gauss method")
+        if maxiter < 2:
+            raise ValueError("maxiter must be at least 2")
+        self.h = float(h)
+        self.tol = float("nan") if tol is None else float(tol)
+        self.maxiter = int(maxiter)
+        self.stages = stages
+
+    def descriptor(self):
+        s = L.cl_system()
+        s.kind = L.CL_SYS_HENON_HEILES_GAUSS
+        s.stages = self.stages
+        s.maxiter = self.maxiter
+        s.h = self.h
+        s.tol = self.tol
+        return s
+
+    def __repr__(self):
+        return f"HenonHeilesGauss(h={self.h!r}, tol={self.tol!r}, maxiter={self.maxiter})"
+
+
+def hh_energy(z):
+    """H(z) of the Henon-Heiles flow in the reference's operation order (src/Systems/flow_models.jl:205)."""
+    z = np.asarray(z, dtype=np.float64)
+    q1, q2, p1, p2 = z[..., 0], z[..., 1], z[..., 2], z[..., 3]
+    return ((p1 * p1 + p2 * p2) / 2 + (q1 * q1 + q2 * q2) / 2 + (q1 * q1) * q2) - (q2 * q2) * q2 / 3
 
 
 TWO_PI = 2.0 * (3.141592653589793 + 1.2246467991473532e-16)

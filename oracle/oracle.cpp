@@ -26,6 +26,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
 #include <thread>
 #include <vector>
@@ -951,9 +952,185 @@ struct MapSys {
   }
 };
 
-// ------------------------------------------------------------------ the per-orbit body
+
+// ------------------------------------------------------------------ Henon-Heiles under Gauss s = 3
+// IntegratorMap(Gauss(Float64, 3; tol), HenonHeilesFlow(), h) (src/Integrators/integrator_map.jl:12-23):
+// fixed-point stage iteration with compensated sums (src/Integrators/gauss.jl:90-141), increment
+// h sum b_i F_i, and the one-step Jacobian of the stage equations. The reference solves the 12 x 12
+// system with LAPACK (gauss.jl:176-192); this restatement eliminates the position blocks first
+// (J_j = [[0, I], [-H_j, 0]]) and solves the remaining 6 x 6 system with partial pivoting, the order
+// the kernels use, so the two agree bit for bit.
+static const double OR_GA_HI[3][3] = {{0x1.1c71c71c71c72p-3, -0x1.26b88a4e09a62p-5, 0x1.40c7cef225974p-7},
+                                      {0x1.337831ea8a881p-2, 0x1.c71c71c71c71cp-3, -0x1.7066ace18c0fbp-6},
+                                      {0x1.126b88a4e09a6p-2, 0x1.ebf38310dda69p-2, 0x1.1c71c71c71c72p-3}};
+static const double OR_GA_LO[3][3] = {{-0x1.c71c71c71c71cp-58, -0x1.f9a1c81ea4ba2p-60, 0x1.94971b6f94297p-61},
+                                      {0x1.d03218eda4366p-56, 0x1.c71c71c71c71cp-57, 0x1.87f5c5d9b2176p-61},
+                                      {0x1.1f9a1c81ea4bap-57, -0x1.194971b6f9429p-56, -0x1.c71c71c71c71cp-58}};
+static const double OR_GB_HI[3] = {0x1.1c71c71c71c72p-2, 0x1.c71c71c71c71cp-2, 0x1.1c71c71c71c72p-2};
+static const double OR_GB_LO[3] = {-0x1.c71c71c71c71cp-57, 0x1.c71c71c71c71cp-56, -0x1.c71c71c71c71cp-57};
+
 template <class P, class T>
-static int8_t run_orbit(const MapSys<P, T>& sys, const cl_chain& ch, int d, int k, int64_t N, int tau, T* x,
+struct HHSys {
+  int s = 4, d = 4;
+  T h, tol;  // tol NaN: 8 eps max(1, |Y|)
+  int maxiter;
+  static void vfield(const T* z, T* f) {
+    f[0] = z[2];
+    f[1] = z[3];
+    f[2] = -(z[0] + (T)2 * z[0] * z[1]);
+    f[3] = -((z[1] + z[0] * z[0]) - z[1] * z[1]);
+  }
+  // sum_j (wh_j + wl_j) F_j, TwoProd by fma and TwoSum per term
+  static T cdot(const double* wh, const double* wl, const T* F) {
+    T s = 0, c = 0;
+    for (int j = 0; j < 3; ++j)
+      for (int t = 0; t < 2; ++t) {
+        T w = (T)(t ? wl[j] : wh[j]);
+        T p = w * F[j];
+        T pe = P::fma(w, F[j], -p);
+        T sn = s + p, bb = sn - s;
+        T se = (s - (sn - bb)) + (p - bb);
+        s = sn;
+        c = c + (se + pe);
+      }
+    return s + c;
+  }
+  void step(T* x, T* const* W, int nf, int k) const {
+    T f0[4], Y[3][4], F[3][4];
+    vfield(x, f0);
+    for (int i = 0; i < 3; ++i) {
+      T rs = h * (((T)OR_GA_HI[i][0] + (T)OR_GA_HI[i][1]) + (T)OR_GA_HI[i][2]);
+      for (int c = 0; c < 4; ++c) {
+        Y[i][c] = x[c] + rs * f0[c];
+        F[i][c] = f0[c];
+      }
+    }
+    T delta = 0, prev = (T)INFINITY;
+    bool stopped = false;
+    int it = 0;
+    while (it < maxiter && !stopped) {
+      ++it;
+      for (int i = 0; i < 3; ++i) vfield(Y[i], F[i]);
+      T dm = 0;
+      for (int i = 0; i < 3; ++i) {
+        T di = 0;
+        for (int c = 0; c < 4; ++c) {
+          T Fc[3] = {F[0][c], F[1][c], F[2][c]};
+          T yn = x[c] + ((T)0 + h * cdot(OR_GA_HI[i], OR_GA_LO[i], Fc));
+          T e = P::abs(yn - Y[i][c]);
+          di = c == 0 ? e : jmax<P>(di, e);
+          Y[i][c] = yn;
+        }
+        dm = i == 0 ? di : jmax<P>(dm, di);
+      }
+      delta = dm;
+      if (!P::finite(delta)) break;
+      stopped = delta == 0 || (it >= 2 && delta >= prev);
+      prev = delta;
+    }
+    T bound = tol;
+    if (P::nan(tol)) {
+      T mx = P::abs(Y[0][0]);
+      for (int i = 0; i < 3; ++i)
+        for (int c = 0; c < 4; ++c) mx = jmax<P>(mx, P::abs(Y[i][c]));
+      bound = (T)8 * P::eps() * jmax<P>((T)1, mx);
+    }
+    bool ok = stopped && P::finite(delta) && delta <= bound;
+    T inc[4];
+    for (int c = 0; c < 4; ++c) {
+      T Fc[3] = {F[0][c], F[1][c], F[2][c]};
+      inc[c] = h * cdot(OR_GB_HI, OR_GB_LO, Fc);
+    }
+    // Jacobian of the step from the converged stages
+    T ha[3][3], hb[3], H[3][2][2];
+    for (int i = 0; i < 3; ++i) {
+      for (int j = 0; j < 3; ++j) ha[i][j] = h * ((T)OR_GA_HI[i][j] + (T)OR_GA_LO[i][j]);
+      hb[i] = h * ((T)OR_GB_HI[i] + (T)OR_GB_LO[i]);
+      H[i][0][0] = (T)1 + (T)2 * Y[i][1];
+      H[i][1][0] = (T)2 * Y[i][0];
+      H[i][0][1] = (T)2 * Y[i][0];
+      H[i][1][1] = (T)1 - (T)2 * Y[i][1];
+    }
+    T B[6][6], R[6][4];
+    for (int i = 0; i < 3; ++i)
+      for (int r = 0; r < 2; ++r) {
+        for (int l = 0; l < 3; ++l)
+          for (int c = 0; c < 2; ++c) {
+            T v = (i == l && r == c) ? (T)1 : (T)0;
+            for (int j = 0; j < 3; ++j) v = v + (ha[i][j] * ha[j][l]) * H[j][r][c];
+            B[2 * i + r][2 * l + c] = v;
+          }
+        for (int c = 0; c < 2; ++c) {
+          T v = 0;
+          for (int j = 0; j < 3; ++j) v = v + ha[i][j] * H[j][r][c];
+          R[2 * i + r][c] = -v;
+          R[2 * i + r][2 + c] = r == c ? (T)1 : (T)0;
+        }
+      }
+    for (int p = 0; p < 6; ++p) {
+      int piv = p;
+      T best = P::abs(B[p][p]);
+      for (int r = p + 1; r < 6; ++r)
+        if (P::abs(B[r][p]) > best) {
+          best = P::abs(B[r][p]);
+          piv = r;
+        }
+      if (piv != p) {
+        for (int c = 0; c < 6; ++c) std::swap(B[p][c], B[piv][c]);
+        for (int c = 0; c < 4; ++c) std::swap(R[p][c], R[piv][c]);
+      }
+      for (int r = p + 1; r < 6; ++r) {
+        T f = B[r][p] / B[p][p];
+        for (int c = p + 1; c < 6; ++c) B[r][c] = B[r][c] - f * B[p][c];
+        for (int c = 0; c < 4; ++c) R[r][c] = R[r][c] - f * R[p][c];
+      }
+    }
+    T Q[6][4];
+    for (int r = 5; r >= 0; --r)
+      for (int c = 0; c < 4; ++c) {
+        T v = R[r][c];
+        for (int t = r + 1; t < 6; ++t) v = v - B[r][t] * Q[t][c];
+        Q[r][c] = v / B[r][r];
+      }
+    T J[4][4];
+    for (int r = 0; r < 4; ++r)
+      for (int c = 0; c < 4; ++c) J[r][c] = r == c ? (T)1 : (T)0;
+    for (int i = 0; i < 3; ++i) {
+      T Pm[2][4];
+      for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 4; ++c) {
+          T v = r == c ? (T)1 : (T)0;
+          for (int j = 0; j < 3; ++j) v = v + ha[i][j] * Q[2 * j + r][c];
+          Pm[r][c] = v;
+        }
+      for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 2; ++r) {
+          J[r][c] = J[r][c] + hb[i] * Q[2 * i + r][c];
+          T hp = H[i][r][0] * Pm[0][c] + H[i][r][1] * Pm[1][c];
+          J[2 + r][c] = J[2 + r][c] + hb[i] * (-hp);
+        }
+    }
+    if (!ok)
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) J[r][c] = (T)NAN;
+    for (int f = 0; f < nf; ++f) {
+      T O[16];
+      for (int j = 0; j < k; ++j)
+        for (int i = 0; i < 4; ++i) {
+          T acc = J[i][0] * W[f][j * 4];
+          for (int q = 1; q < 4; ++q) acc = P::fma(J[i][q], W[f][j * 4 + q], acc);
+          O[j * 4 + i] = acc;
+        }
+      for (int e = 0; e < 4 * k; ++e) W[f][e] = O[e];
+    }
+    for (int c = 0; c < 4; ++c) x[c] = ok ? x[c] + inc[c] : (T)NAN;
+  }
+  void field(const T* x, T* f) const { vfield(x, f); }
+};
+
+// ------------------------------------------------------------------ the per-orbit body
+template <class P, class T, class SYS>
+static int8_t run_orbit(const SYS& sys, const cl_chain& ch, int d, int k, int64_t N, int tau, T* x,
                         T* W0, T* W1, RState<T>* st, int64_t* hstep) {
   for (int r = 0; r < ch.n; ++r) init_state(ch.r[r], st[r], k);
   T* W[2] = {W0, W1};
@@ -1030,6 +1207,10 @@ static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, in
   ms.K = (T)sys->K;
   ms.s = s;
   for (int i = 0; i < 4; ++i) ms.A[i] = A ? (T)A[i] : (T)0;
+  HHSys<P, T> hs;
+  hs.h = (T)sys->h;
+  hs.tol = (T)sys->tol;
+  hs.maxiter = sys->maxiter > 0 ? sys->maxiter : 50;
   int sb = 0;
   int off[CL_MAX_REDUCERS];
   for (int r = 0; r < ch->n; ++r) {
@@ -1050,7 +1231,9 @@ static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, in
           w0[e] = (T)W[i * d * k + e];
           w1[e] = W2 ? (T)W2[i * d * k + e] : (T)0;
         }
-        code[i] = run_orbit<P, T>(ms, *ch, d, k, N, tau, xx, w0, w1, st.data(), &hstep[i]);
+        code[i] = sys->kind == CL_SYS_HENON_HEILES_GAUSS
+                      ? run_orbit<P, T>(hs, *ch, d, k, N, tau, xx, w0, w1, st.data(), &hstep[i])
+                      : run_orbit<P, T>(ms, *ch, d, k, N, tau, xx, w0, w1, st.data(), &hstep[i]);
         for (int c = 0; c < s; ++c) x[i * s + c] = dbl(xx[c]);
         for (int e = 0; e < d * k; ++e) {
           W[i * d * k + e] = dbl(w0[e]);
@@ -1071,6 +1254,12 @@ static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, in
   return 0;
 }
 
+static bool or_dims_ok(const cl_system* sys, int d, int k) {
+  int want = sys->kind == CL_SYS_HENON_HEILES_GAUSS ? 4
+           : (sys->kind == CL_SYS_STANDARD_MAP || sys->kind == CL_SYS_WRAPPED_STANDARD_MAP) ? 2 : -1;
+  return d == want && k >= 1 && k <= d;
+}
+
 extern "C" {
 
 // Bytes per orbit of the chain's state tuple (Julia layout), -1 for an unknown reducer.
@@ -1089,8 +1278,7 @@ int64_t or_state_bytes(const cl_chain* ch, int d, int k) {
 int or_propagate(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int d, int k, int64_t N, int tau,
                  double* x, double* W, double* W2, void* states, int8_t* code, int64_t* hstep, int math,
                  int nthreads) {
-  if (sys->kind != CL_SYS_STANDARD_MAP && sys->kind != CL_SYS_WRAPPED_STANDARD_MAP) return 3;
-  if (d != 2 || k < 1 || k > 2 || N % tau) return 10;
+  if (!or_dims_ok(sys, d, k) || N % tau) return 10;
   if (math == MATH_CL)
     return propagate_all<F64Cl>(sys, ch, m, s, d, k, N, tau, x, W, W2, (uint8_t*)states, code, hstep, nthreads);
   return propagate_all<F64Libm>(sys, ch, m, s, d, k, N, tau, x, W, W2, (uint8_t*)states, code, hstep, nthreads);
@@ -1132,8 +1320,7 @@ void or_csum_list(int64_t n, const double* v, double* comp) {
 // Julia layout with each value rounded to double.
 int or_propagate_quad(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int d, int k, int64_t N, int tau,
                       double* x, double* W, double* W2, void* states, int8_t* code, int64_t* hstep, int nthreads) {
-  if (sys->kind != CL_SYS_STANDARD_MAP && sys->kind != CL_SYS_WRAPPED_STANDARD_MAP) return 3;
-  if (d != 2 || k < 1 || k > 2 || N % tau) return 10;
+  if (!or_dims_ok(sys, d, k) || N % tau) return 10;
   return propagate_all<Quad>(sys, ch, m, s, d, k, N, tau, x, W, W2, (uint8_t*)states, code, hstep, nthreads);
 }
 

@@ -50,7 +50,7 @@ def run(sysd, chain, x0, W0, N, tau, k, W2=None, math=MATH_CL, nthreads=None, qu
     o = load()
     x = np.ascontiguousarray(np.array(x0, dtype=np.float64))
     m, s = x.shape
-    d = 2
+    d = 4 if sysd.kind == L.CL_SYS_HENON_HEILES_GAUSS else 2
     W = np.array(np.asarray(W0, dtype=np.float64).reshape(m, d, k).transpose(0, 2, 1), order="C", copy=True)
     W2a = None if W2 is None else np.array(np.asarray(W2, dtype=np.float64).reshape(m, d, k).transpose(0, 2, 1), order="C", copy=True)
     nb = o.or_state_bytes(ctypes.byref(chain), d, k)

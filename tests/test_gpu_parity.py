@@ -130,7 +130,7 @@ def test_launch_split_invariance():
 def test_host_chunk_entry_point():
     """cl_propagate_chunk on host buffers in the reference layouts, split over launches with n0 > 0, equals the
     device entry point bitwise."""
-    m, N, k = 200, 1200
+    m, N = 200, 1200
     chain = (QRSpectrum(MGS()),)
     say("host chunk API", "m", m, "N", N)
     lib, ctx = _context()
@@ -226,3 +226,28 @@ def test_config1_bitwise_full():
     # chaotic sea at K = 6: lambda_1 near the large-K law ln(K/2) (Chirikov), pairing lambda_1 + lambda_2 = 0
     assert abs(np.median(lam[:, 0]) - math.log(3.0)) < 0.1
     assert np.max(np.abs(lam[:, 0] + lam[:, 1])) < 1e-9
+
+
+HH_CHAINS = [((), (), 4), ((QRSpectrum(GivensQR()),), (), 4), ((GALI(),), (), 4), ((MEGNO(replace=False),), (), 4),
+             ((OFLI(),), (), 4), ((MEGNO(replace=False), GALI()), (QRSpectrum(GivensQR()),), 4),
+             ((MEGNO(replace=False), FLI(replace=False), OFLI(replace=False), SALI()), (), 2), ((Benettin(),), (), 1)]
+
+
+@pytest.mark.parametrize("chains", HH_CHAINS, ids=lambda c: cname(c[0] + c[1]) + f"-k{c[2]}")
+def test_henon_heiles_gauss_bitwise(chains):
+    """Henon-Heiles under the s = 3 Gauss method: the stage iteration, increment, step Jacobian and reducers
+    on the GPU equal the oracle bit for bit."""
+    from chaos_amd import HenonHeilesGauss
+    from chaos_amd.inputs import hh_section_starts
+    red, red2, k = chains
+    m, N = 256, 400
+    for E, tol in ((1 / 12, 1e-15), (1 / 6, None)):
+        say("henon-heiles", cname(red + red2), "k", k, "E", E, "tol", tol)
+        sysm = HenonHeilesGauss(h=0.01, tol=tol)
+        x0 = hh_section_starts(5, m, E)
+        res = propagate(sysm, x0, N, k=k, tau=10, seed=21, reducers=red, reducers2=red2, check=False,
+                        keep_states=True)
+        ch, allr = make_chain(red, red2)
+        W0 = O.frames(21, 1, m, 4, k)
+        orc = O.run(sysm.descriptor(), ch, x0, W0, N, 10, k, W2=W0 if red2 else None, math=O.MATH_CL)
+        compare(res, orc, allr, 4, k)
