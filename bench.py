@@ -145,6 +145,8 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c2", "c1"])
     ap.add_argument("--cpu-orbits", type=int, default=32768, help="orbits per K in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first copy 1 GiB device-to-device (the known-size kernel that calibrates FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: the ensemble shards by orbit ranges with no exchange during propagation, "
@@ -152,6 +154,14 @@ def main():
         sys.exit(2)
     wl = workload(a.workload)
     log("workload", wl["desc"])
+    if a.copy_probe:
+        import torch
+        src = torch.ones(2 ** 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: 1 GiB read + 1 GiB written by one device copy kernel")
+        del src, dst
     g = gpu_run(wl, a.steps, a.warmup)
     value = g["orbit_steps"] * a.steps / g["wall"]
     avg_ms = float(np.mean(g["kernel_ms"]))

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libchaoslab_hip.so")
 CL_OK, CL_ERR_NO_DEVICE, CL_ERR_UNSUPPORTED, CL_ERR_OOM, CL_ERR_DEVICE_LOST, CL_ERR_ARGUMENT = 0, 2, 3, 4, 9, 10
 CL_ELT_F64, CL_ELT_F64_F32TAN, CL_ELT_DD = 1, 2, 3
 CL_SYS_STANDARD_MAP, CL_SYS_WRAPPED_STANDARD_MAP, CL_SYS_STANDARD_MAP_CHAIN, CL_SYS_HENON_HEILES_GAUSS = 1, 2, 3, 4
+CL_SYS_STANDARD_MAP_K = 5
 CL_RED_QR, CL_RED_BENETTIN, CL_RED_MEGNO, CL_RED_SALI, CL_RED_FLI, CL_RED_OFLI, CL_RED_GALI = 1, 2, 3, 4, 5, 6, 7
 CL_QR_CGS, CL_QR_MGS, CL_QR_RGS, CL_QR_HOUSEHOLDER, CL_QR_AUGMENTED, CL_QR_GIVENS = 1, 2, 3, 4, 5, 6
 CL_MAX_REDUCERS = 8
@@ -61,10 +62,20 @@ SIGNATURES = {
                                I64, I, P, P, P, P, P, P, P]),
     "cl_propagate_device": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I64, I, I, I, I64, I64,
                                 I64, I, P, P, P, P, P, P, P]),
+    "cl_propagate_device_traced": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I64, I, I, I, I64,
+                                       I64, I64, I, P, P, P, P, P, P, P, I64, I, P]),
     "cl_initial_deviations": (I, [ctypes.c_uint64, I64, I64, I, I, P]),
     "cl_initial_deviations_device": (I, [P, ctypes.c_uint64, I64, I64, I, I, P, P]),
     "cl_math_host": (None, [I64, P, P, P, P]),
     "cl_math_device": (I, [P, I64, P, P, P, P, P]),
+    "cl_mlp_gemm": (I, [P, I, I, I, I, I, P, I, P, I, P, I, I, P, I, P, I, P]),
+    "cl_mlp_rowsum": (I, [P, I, I, P, I, P, P]),
+    "cl_mlp_loss": (I, [P, I, P, P, P, P, ctypes.c_float, P, P, P]),
+    "cl_mlp_adam": (I, [P, I64, P, P, P, P, P, I, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                        ctypes.c_float, P]),
+    "cl_mlp_normalize": (I, [P, I, I64, P, P, P, P, P]),
+    "cl_mlp_feature_stats": (I, [P, I, I64, P, P, P, P]),
+    "cl_f32_to_bf16": (I, [P, I64, P, P, P]),
 }
 
 _lib = None

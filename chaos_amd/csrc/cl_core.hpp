@@ -38,22 +38,33 @@ struct CSum {
 
 CL_HD CSum csum_zero() { return CSum{0.0, 0.0}; }
 
-// Neumaier update, branch on the larger magnitude (Foundation.jl:48-55).
+// Neumaier update (Foundation.jl:48-55). The reference picks (s - t) + y or (y - t) + s by the larger
+// magnitude; either is the exact rounding error of s + y, as is Knuth's branch-free TwoSum error used
+// here, so the compensation term is bit-identical while s + y does not overflow.
 CL_HD CSum csum_add(CSum a, double y) {
-  double t = a.s + y;
-  double c = cl_fabs(a.s) >= cl_fabs(y) ? (a.s - t) + y : (y - t) + a.s;
-  return CSum{t, a.c + c};
+  double t, e;
+  cl_two_sum(a.s, y, &t, &e);
+  return CSum{t, a.c + e};
 }
 CL_HD double csum_total(CSum a) { return a.s + a.c; }
 
-// Scaled Euclidean norm of v[0..n) with stride `st` (norm.jl:8-12): maximum of |v| (foldl with
-// Julia's max), then m * sqrt(sum((v/m)^2)) summed left to right.
+// Scaled Euclidean norm of v[0..n) with stride `st` (norm.jl:8-12): maximum of |v|, then
+// m * sqrt(sum((v/m)^2)) summed left to right. The maximum uses the hardware max: every caller passes
+// entries the handoff has already checked to be finite (or vectors formed from them), and on finite and
+// infinite absolute values the hardware max equals Julia's NaN-propagating `max` bit for bit.
 template <int N>
 CL_HD double safe_norm(const double* v, int st = 1) {
   double m = cl_fabs(v[0]);
 #pragma unroll
-  for (int i = 1; i < N; ++i) m = cl_jmax(m, cl_fabs(v[i * st]));
+  for (int i = 1; i < N; ++i) m = __builtin_fmax(m, cl_fabs(v[i * st]));
   if (!(cl_isfinite(m) && m > 0.0)) return m;
+  if (N == 2) {
+    // the larger entry over m is exactly +-1, its square exactly 1: one division instead of two, and
+    // 1 + b^2 equals b^2 + 1, so the sum is the same in either order
+    double other = cl_fabs(v[0]) >= cl_fabs(v[st]) ? v[st] : v[0];
+    double b = other / m;
+    return m * cl_sqrt(1.0 + b * b);
+  }
   double a = v[0] / m;
   double s = a * a;
 #pragma unroll
@@ -124,6 +135,42 @@ struct StdMap {
   // the OFLI reducer projects out on a map).
   CL_HD void field(const double* x, const Aux& a, double* f) const {
     double ks = K * a.sn;
+    f[0] = x[1] + ks;
+    f[1] = ks;
+  }
+};
+
+// Standard map whose kick strength rides in the state (x, p, K): an ensemble over K in one launch.
+// The reference holds one K per system (src/Systems/standard_map.jl:8-10); each orbit here steps exactly
+// as StandardMap(K) with its own K, and K passes through the step unchanged.
+struct StdMapK {
+  static constexpr int D = 2;
+  static constexpr int S = 3;
+  struct Aux {
+    double sn, cs, K;
+  };
+  CL_HD void aux(const double* x, Aux& a) const {
+    cl_sincos(x[0], &a.sn, &a.cs);
+    a.K = x[2];
+  }
+  template <int K2>
+  CL_HD void tangent_apply(const Aux& a, Frame<2, K2>& W) const {
+    double c = a.K * a.cs;
+    double m11 = 1.0 + c;
+#pragma unroll
+    for (int j = 0; j < K2; ++j) {
+      double w0 = W.at(0, j), w1 = W.at(1, j);
+      W.at(0, j) = cl_fma(1.0, w1, m11 * w0);
+      W.at(1, j) = cl_fma(1.0, w1, c * w0);
+    }
+  }
+  CL_HD void forward(const Aux& a, double* x) const {
+    double p = x[1] + a.K * a.sn;
+    x[0] = x[0] + p;
+    x[1] = p;
+  }
+  CL_HD void field(const double* x, const Aux& a, double* f) const {
+    double ks = a.K * a.sn;
     f[0] = x[1] + ks;
     f[1] = ks;
   }

@@ -52,9 +52,18 @@ typedef Cons<S_<OFLI<2, 1>>, Nil> C1_ofli;
   ADD1(SK, SYS, C1_rgs) ADD1(SK, SYS, C1_hh) ADD1(SK, SYS, C1_aug) ADD1(SK, SYS, C1_giv)        \
   ADD1(SK, SYS, C1_ofli)
 
+// the surrogate workload on the per-orbit-K map: the traced prefix (MEGNO + FLI on frame 1, the QR spectrum
+// on frame 2), the continuation and the plain method (the QR spectrum alone)
+typedef Cons<S_<MEGNO<2>>, Cons<S_<FLI<2>>, Cons<QR2(QR_MGS, 1), Nil>>> C_prefix_mgs;
+typedef Cons<S_<MEGNO<2>>, Cons<S_<FLI<2>>, Nil>> C_mf;
+
 void register_instances_map(Instance* out, int* n, int cap) {
   ALL_FOR(CL_SYS_STANDARD_MAP, StdMap)
   ALL_FOR(CL_SYS_WRAPPED_STANDARD_MAP, WrappedStdMap)
+  ADD2(CL_SYS_STANDARD_MAP_K, StdMapK, C_none, 1) ADD2(CL_SYS_STANDARD_MAP_K, StdMapK, C_mgs, 1)
+  ADD2(CL_SYS_STANDARD_MAP_K, StdMapK, C_hh, 1) ADD2(CL_SYS_STANDARD_MAP_K, StdMapK, C_mf, 1)
+  ADD2(CL_SYS_STANDARD_MAP_K, StdMapK, C_mfos_hh, 2) ADD2(CL_SYS_STANDARD_MAP_K, StdMapK, C_prefix_mgs, 2)
+  ADD2(CL_SYS_STANDARD_MAP, StdMap, C_prefix_mgs, 2) ADD2(CL_SYS_STANDARD_MAP, StdMap, C_mf, 1)
 }
 
 }  // namespace cl

@@ -23,6 +23,11 @@ struct KArgs {
   int64_t* hstep;
   int64_t m, n0, steps, N;
   int tau;
+  // optional per-handoff feature trace (float [3 * trace_len][m]); handoff t (1-based, counted from
+  // trace_n0) writes rows 3(t-1) .. 3(t-1)+2 while t <= trace_len
+  float* trace;
+  int64_t trace_n0;
+  int trace_len;
 };
 
 // Per-orbit context the reducers read: the point, the carried sin/cos, the frames and a cache of
@@ -192,6 +197,25 @@ CL_HD int chain_store(const Cons<H, T>& c, uint64_t* st, int64_t m, int64_t i, i
   state_store(c.head.st, st, m, i, w);
   return chain_store(c.tail, st, m, i, w + (int)(sizeof(c.head.st) / 8));
 }
+// Feature trace of the prefix pass. A chain that starts MEGNO (frame 0) then FLI (frame 0) records at
+// each handoff: MEGNO's Y of column 1 (2 R1 / n), FLI_B, and the log stretch log(alpha_1) of the handoff.
+template <class Chain, class Ctx>
+CL_HD void trace_hook(const Chain&, const Ctx&, int64_t, const KArgs&, int64_t) {}
+template <int D, class Rest, class Ctx>
+CL_HD void trace_hook(const Cons<Slot<MEGNO<D>, 0>, Cons<Slot<FLI<D>, 0>, Rest>>& ch, const Ctx& c, int64_t n,
+                      const KArgs& a, int64_t i) {
+  if (a.trace == nullptr) return;
+  int64_t t = (n - a.trace_n0) / a.tau;
+  if (t < 1 || t > a.trace_len) return;
+  double y = 2.0 * csum_total(ch.head.st.c1.R1) / (double)n;
+  double f = ch.tail.head.st.fB;
+  double l = c.logs[0].l1;
+  float* o = a.trace + (int64_t)(3 * (t - 1)) * a.m + i;
+  o[0] = (float)y;
+  o[a.m] = (float)f;
+  o[2 * a.m] = (float)l;
+}
+
 template <class Sys, int K, int NF>
 CL_HD int8_t frame_status(const OrbitCtx<Sys, K, NF>& c) {
   bool xf = true;
@@ -245,6 +269,7 @@ __global__ void __launch_bounds__(256) orbit_kernel(KArgs a, Sys sys, Chain chai
 #pragma unroll
     for (int f = 0; f < NF; ++f) c.logs_valid[f] = false;
     chain_step(ch, n, c);
+    trace_hook(ch, c, n, a, i);
     if (chain_failed(ch)) {
       code = STATUS_REDUCER;
       early = true;

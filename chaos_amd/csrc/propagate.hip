@@ -55,13 +55,15 @@ static int state_dim(int kind) {
     case CL_SYS_STANDARD_MAP: return 2;
     case CL_SYS_WRAPPED_STANDARD_MAP: return 4;
     case CL_SYS_HENON_HEILES_GAUSS: return 4;
+    case CL_SYS_STANDARD_MAP_K: return 3;
     default: return -1;
   }
 }
 static int sys_dim(int kind) {
   switch (kind) {
     case CL_SYS_STANDARD_MAP:
-    case CL_SYS_WRAPPED_STANDARD_MAP: return 2;
+    case CL_SYS_WRAPPED_STANDARD_MAP:
+    case CL_SYS_STANDARD_MAP_K: return 2;
     case CL_SYS_HENON_HEILES_GAUSS: return 4;
     default: return -1;
   }
@@ -193,6 +195,9 @@ extern "C" void cl_destroy(cl_ctx* c) {
 
 extern "C" const char* cl_last_error(cl_ctx* c) { return c ? c->err : g_create_err; }
 
+// The context's stream, for the other translation units of the library (not part of the C ABI).
+extern "C" __attribute__((visibility("hidden"))) hipStream_t cl_ctx_stream(cl_ctx* c) { return c->stream; }
+
 extern "C" int64_t cl_state_bytes(const cl_system* sys, const cl_chain* ch, int elt, int d, int k) {
   if (!sys || !ch || ch->n < 0 || ch->n > CL_MAX_REDUCERS) return -CL_ERR_ARGUMENT;
   if (elt != CL_ELT_F64) return -CL_ERR_UNSUPPORTED;
@@ -215,9 +220,13 @@ static int grow(cl_ctx* c, int slot, size_t bytes) {
 
 static int launch(const Instance* in, const cl_system* sys, const cl_chain* ch, int64_t m, double* dx,
                   double* dW, double* dW2, uint64_t* dst, int8_t* dcode, int64_t* dhstep, int64_t n0,
-                  int64_t steps, int64_t N, int tau, hipStream_t st) {
+                  int64_t steps, int64_t N, int tau, hipStream_t st, float* trace = nullptr,
+                  int64_t trace_n0 = 0, int trace_len = 0) {
   if (m == 0) return CL_OK;
   KArgs a;
+  a.trace = trace;
+  a.trace_n0 = trace_n0;
+  a.trace_len = trace_len;
   a.x = dx;
   a.W0 = dW;
   a.W1 = dW2;
@@ -244,6 +253,24 @@ extern "C" int cl_propagate_device(cl_ctx* ctx, const cl_system* sys, const cl_c
   hipSetDevice(ctx->device);
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
   rc = launch(in, sys, ch, m, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st);
+  if (rc) return set_err(ctx, rc, "kernel launch failed");
+  return CL_OK;
+}
+
+extern "C" int cl_propagate_device_traced(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt,
+                                          int64_t m, int s, int d, int k, int64_t n0, int64_t steps, int64_t N,
+                                          int tau, double* dx, double* dW, double* dW2, uint64_t* dst, int8_t* dcode,
+                                          int64_t* dhstep, float* dtrace, int64_t trace_n0, int trace_len,
+                                          void* stream) {
+  if (!ctx) return CL_ERR_ARGUMENT;
+  const Instance* in = nullptr;
+  int rc = validate(ctx, sys, ch, elt, m, s, d, k, n0, steps, N, tau, &in);
+  if (rc) return rc;
+  if (in->nf > 1 && !dW2) return set_err(ctx, CL_ERR_ARGUMENT, "the chain uses a second frame: W2 is required");
+  if (trace_len < 0 || (trace_len > 0 && !dtrace)) return set_err(ctx, CL_ERR_ARGUMENT, "bad trace buffer");
+  hipSetDevice(ctx->device);
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  rc = launch(in, sys, ch, m, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st, dtrace, trace_n0, trace_len);
   if (rc) return set_err(ctx, rc, "kernel launch failed");
   return CL_OK;
 }

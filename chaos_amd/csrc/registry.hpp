@@ -32,6 +32,10 @@ inline WrappedStdMap make_sys<WrappedStdMap>(const cl_system& s) {
 }
 
 template <>
+inline StdMapK make_sys<StdMapK>(const cl_system&) {
+  return StdMapK();
+}
+template <>
 inline HHGauss make_sys<HHGauss>(const cl_system& s) {
   HHGauss m;
   m.h = s.h;

@@ -43,6 +43,22 @@ class WrappedStandardMap:
         return f"WrappedStandardMap({self.K!r})"
 
 
+class StandardMapK:
+    """The standard map with its kick strength carried per orbit: state (x, p, K), each orbit stepping exactly as
+    StandardMap(K) (src/Systems/standard_map.jl:18-43). No reference counterpart (the reference holds one K per
+    system); it lets one launch cover an ensemble over K."""
+    dimension = 2
+    state_length = 3
+
+    def descriptor(self):
+        s = L.cl_system()
+        s.kind = L.CL_SYS_STANDARD_MAP_K
+        return s
+
+    def __repr__(self):
+        return "StandardMapK()"
+
+
 class HenonHeilesGauss:
     """IntegratorMap(Gauss(Float64, 3; tol, maxiter), HenonHeilesFlow(), h): one step of the s = 3 Gauss-Legendre
     method on H = |p|^2/2 + |q|^2/2 + q1^2 q2 - q2^3/3, state (q1, q2, p1, p2)

@@ -49,6 +49,8 @@ extern "C" {
 #define CL_SYS_WRAPPED_STANDARD_MAP 2  /* WrappedStandardMap(K), src/Systems/wrapped_standard_map.jl:75-123 */
 #define CL_SYS_STANDARD_MAP_CHAIN 3    /* StandardMapChain(N, K, eps; boundary), src/Systems/standard_map_chain.jl:11-53 */
 #define CL_SYS_HENON_HEILES_GAUSS 4    /* IntegratorMap(Gauss(Float64, s; tol), HenonHeilesFlow(), h), src/Integrators/integrator_map.jl:12-23 */
+#define CL_SYS_STANDARD_MAP_K 5        /* StandardMap with its K carried per orbit: state (x, p, K); no reference
+                                          counterpart (the reference holds one K per system, standard_map.jl:8-10) */
 
 typedef struct {
   int32_t kind;     /* CL_SYS_* */
@@ -139,6 +141,15 @@ int cl_propagate_device(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain
                         double *dx, double *dW, double *dW2, uint64_t *dstates, int8_t *dcode,
                         int64_t *dhstep, void *stream);
 
+/* cl_propagate_device plus a feature trace of the first `trace_len` handoffs after global step trace_n0:
+ * float trace[3 * trace_len][m], rows 3(t-1)..3(t-1)+2 of handoff t = MEGNO's Y of column 1, FLI_B and
+ * log(alpha_1). Only chains that start MEGNO then FLI on the first frame record (others write nothing).
+ * No reference counterpart: this is the prefix pass of the surrogate triage. */
+int cl_propagate_device_traced(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain, int elt,
+                               int64_t m, int s, int d, int k, int64_t n0, int64_t steps, int64_t N, int tau,
+                               double *dx, double *dW, double *dW2, uint64_t *dstates, int8_t *dcode,
+                               int64_t *dhstep, float *dtrace, int64_t trace_n0, int trace_len, void *stream);
+
 /* Seeded orthonormal frames, replacing initial_deviations(seed, orbit, Val(d), Val(k), T)
  * (src/Tangent/initial.jl:27-34): orbit ids orbit0 .. orbit0+m-1 (1-based ids as the reference),
  * key mix(mix(seed) + id) with the reference's splitmix64 finaliser, Gaussian d x k matrix from a
@@ -154,6 +165,34 @@ int cl_initial_deviations_device(cl_ctx *ctx, uint64_t seed, int64_t orbit0, int
 void cl_math_host(int64_t n, const double *x, double *sn, double *cs, double *lg);
 int cl_math_device(cl_ctx *ctx, int64_t n, const double *dx, double *dsn, double *dcs, double *dlg,
                    void *stream);
+
+/* ------------------------------------------------------------------ surrogate network kernels
+ * No reference counterpart: the learned triage of the ensemble (a dense network on the prefix trace).
+ * Matrices are row-major; bf16 values are uint16_t bit patterns; activations are feature-major
+ * (features x orbits). All calls are asynchronous on `stream` (NULL: the context's stream). */
+
+/* C = epi(op(A) op(B)): op(A) is M x K (A stored K x M when transA), op(B) is K x N (B stored N x K when
+ * transB); BF16 inputs, FP32 accumulation on the matrix cores. epi: 0 none, 1 + bias[m] then ReLU,
+ * 2 + bias[m], 3 zero where H[m][n] <= 0 (ReLU backward). C is FP32 when c_f32, else BF16 (round to nearest even). */
+int cl_mlp_gemm(cl_ctx *ctx, int transA, int transB, int M, int N, int K, const uint16_t *A, int lda,
+                const uint16_t *B, int ldb, void *C, int ldc, int c_f32, const float *bias, int epi,
+                const uint16_t *H, int ldh, void *stream);
+/* out[m] = sum_n X[m][n] (FP32 sums of BF16 values): the bias gradient. */
+int cl_mlp_rowsum(cl_ctx *ctx, int M, int N, const uint16_t *X, int ld, float *out, void *stream);
+/* Batch-mean loss: BCE-with-logits of out[0][:] against y, plus wreg * w * (out[1][:] - t)^2 / 2; writes
+ * d(loss)/d(out) as BF16 (2 x B) and per-group partial sums partial[2 g + {0, 1}] (ceil(B/256) groups). */
+int cl_mlp_loss(cl_ctx *ctx, int B, const float *out, const float *y, const float *t, const float *w, float wreg,
+                uint16_t *dout, float *partial, void *stream);
+/* One Adam step (step >= 1) on FP32 parameters p with moments m, v; writes the BF16 copy pb when non-NULL. */
+int cl_mlp_adam(cl_ctx *ctx, int64_t n, float *p, uint16_t *pb, const float *g, float *m, float *v, int step,
+                float lr, float b1, float b2, float eps, float wd, void *stream);
+/* X = bf16((trace - mean[f]) * istd[f]) for an F x B FP32 trace (non-finite results become 0). */
+int cl_mlp_normalize(cl_ctx *ctx, int F, int64_t B, const float *trace, const float *mean, const float *istd,
+                     uint16_t *X, void *stream);
+/* Per-feature mean and 1/std of an F x B FP32 trace. */
+int cl_mlp_feature_stats(cl_ctx *ctx, int F, int64_t B, const float *trace, float *mean, float *istd, void *stream);
+/* dst = bf16(src), n values. */
+int cl_f32_to_bf16(cl_ctx *ctx, int64_t n, const float *src, uint16_t *dst, void *stream);
 
 #ifdef __cplusplus
 }

@@ -902,15 +902,16 @@ struct MapSys {
       return;
     }
     T sn = P::sin(x[0]), cs = P::cos(x[0]);
-    T c = K * cs, m11 = (T)1 + c;
+    const T Kx = kind == CL_SYS_STANDARD_MAP_K ? x[2] : K;  // the per-orbit K rides in the state
+    T c = Kx * cs, m11 = (T)1 + c;
     for (int f = 0; f < nf; ++f)
       for (int j = 0; j < k; ++j) {
         T w0 = W[f][2 * j], w1 = W[f][2 * j + 1];
         W[f][2 * j] = P::fma((T)1, w1, m11 * w0);
         W[f][2 * j + 1] = P::fma((T)1, w1, c * w0);
       }
-    if (kind == CL_SYS_STANDARD_MAP) {
-      T p = x[1] + K * sn;
+    if (kind == CL_SYS_STANDARD_MAP || kind == CL_SYS_STANDARD_MAP_K) {
+      T p = x[1] + Kx * sn;
       x[0] = x[0] + p;
       x[1] = p;
     } else {
@@ -946,7 +947,7 @@ struct MapSys {
   }
   // map displacement field at x (OFLI direction on a map)
   void field(const T* x, T* f) const {
-    T ks = K * P::sin(x[0]);
+    T ks = (kind == CL_SYS_STANDARD_MAP_K ? x[2] : K) * P::sin(x[0]);
     f[0] = x[1] + ks;
     f[1] = ks;
   }
@@ -1256,7 +1257,8 @@ static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, in
 
 static bool or_dims_ok(const cl_system* sys, int d, int k) {
   int want = sys->kind == CL_SYS_HENON_HEILES_GAUSS ? 4
-           : (sys->kind == CL_SYS_STANDARD_MAP || sys->kind == CL_SYS_WRAPPED_STANDARD_MAP) ? 2 : -1;
+           : (sys->kind == CL_SYS_STANDARD_MAP || sys->kind == CL_SYS_WRAPPED_STANDARD_MAP ||
+              sys->kind == CL_SYS_STANDARD_MAP_K) ? 2 : -1;
   return d == want && k >= 1 && k <= d;
 }
 

@@ -270,3 +270,21 @@ def test_ofli_bounded_by_fli():
     assert np.all(fo["sinmin"] <= 1.0 + 4 * EPS)
     assert np.array_equal(fo["sum"], ff["sum"])
     assert np.all(fo["ofli"] <= ff["fli"] + 1e-12) or np.all(fo["ofli"] <= fo["fli"] + 1e-12)
+
+
+def test_per_orbit_k_map_equals_fixed_k_runs():
+    """The per-orbit-K standard map (state x, p, K) steps exactly as StandardMap(K) for each orbit's K."""
+    from chaos_amd import StandardMap, StandardMapK
+    from chaos_amd.inputs import torus_uniform
+    from chaos_amd.tangent import make_chain
+    from chaos_amd.reducers import MEGNO, QRSpectrum, MGS
+    ch, _ = make_chain((MEGNO(replace=False), QRSpectrum(MGS())))
+    Ks = np.array([0.5, 0.97, 2.0, 6.0])
+    x2 = torus_uniform(5, 4)
+    x3 = np.concatenate([x2, Ks[:, None]], axis=1)
+    W0 = O.frames(5, 1, 4, 2, 2)
+    a = O.run(StandardMapK().descriptor(), ch, x3, W0, 500, 1, 2)
+    for i, K in enumerate(Ks):
+        b = O.run(StandardMap(K).descriptor(), ch, x2[i:i + 1], W0[i:i + 1], 500, 1, 2)
+        assert np.array_equal(a["states"][i], b["states"][0])
+        assert np.array_equal(a["x"][i, :2], b["x"][0]) and a["x"][i, 2] == K
