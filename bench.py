@@ -8,6 +8,13 @@ cycle through the three K values. Inputs are resident in HBM before the timed re
 
 `--workload c1`: the reference's CPU-runnable case (K = 6, 4096 starts, 10^5 steps, QRSpectrum(MGS())).
 
+`--workload c5`: surrogate triage on the standard map with per-orbit K ~ U[0.5, 6]. The network is trained on
+1M orbits x 10^5 steps (prefix features + full-method labels), checked on 262144 held-out orbits, and then
+4M orbits are labelled twice: by the plain method (QR spectrum to N/10 and N for every orbit) and by the
+triage (256-step prefix, network, full method only where |logit| is below a margin chosen on the held-out
+set). `value` is the triaged rate in orbit-steps/s (4M x 10^5 over the triage wall time); the line also
+carries the speedup and the misclassified fraction against the plain labels.
+
 Prints one JSON line with the metric, the roofline of the orbit kernel (HIP events on the launch stream)
 and the CPU baseline (the oracle, glibc math, on the host's cores, on a bounded sample of the workload).
 """
@@ -137,12 +144,92 @@ def cpu_baseline(wl, budget_orbits):
                        f"oracle/oracle.cpp with glibc sin/cos/log, {nthreads} threads, {t:.1f} s")
 
 
+def c5_main(a):
+    import torch
+    from chaos_amd import surrogate as S
+    n_train, n_ho, n_tri, N = (int(v) for v in a.c5_sizes.split(","))
+    prefix = 256
+    desc = (f"standard map, per-orbit K ~ U[0.5,6], starts uniform on the torus; train {n_train} x {N} steps, "
+            f"held out {n_ho}, triage {n_tri} orbits; prefix {prefix} steps (MEGNO Y, FLI_B, log alpha_1), "
+            f"MLP 768-256-256-256-2 BF16/FP32-acc, margin |logit| calibrated on the held-out set for <= 0.4% "
+            f"misclassified; full method QRSpectrum(MGS()) k=2 FP64")
+    log("workload", desc)
+    t0 = time.perf_counter()
+    train = S.run_full(S.make_starts(11, n_train), 11, N, prefix)
+    t_data = time.perf_counter() - t0
+    log(f"training data: {n_train} orbits x {N} steps in {t_data:.1f} s")
+    t0 = time.perf_counter()
+    net, info = S.fit(train, epochs=a.c5_epochs, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del train
+    ho = S.run_full(S.make_starts(12, n_ho, orbit0=1 + n_train), 12, N, prefix)
+    margin = S.calibrate_margin(net, ho, target=0.004)
+    ev = S.evaluate(net, ho, margin)
+    ev["margin"] = margin
+    log("held-out", ev)
+    del ho
+    torch.cuda.empty_cache()
+    x0 = S.make_starts(13, n_tri, orbit0=1 + n_train + n_ho)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain = S.run_plain(x0, 13, N)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    lab_plain = S.p1_labels(plain["lam_N"].cpu().numpy(), plain["lam_N10"].cpu().numpy())
+    del plain
+    torch.cuda.empty_cache()
+    log(f"plain: {n_tri} orbits in {t_plain:.2f} s")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tri = S.triage_run(net, x0, 13, N, prefix, margin, log=log)
+    torch.cuda.synchronize()
+    t_tri = time.perf_counter() - t0
+    wrong = float(np.mean(tri["label"] != (lab_plain == 1)))
+    cpu = None
+    if not a.no_cpu:
+        from chaos_amd import QRSpectrum, MGS, StandardMapK
+        from chaos_amd.tangent import make_chain
+        from tests import oracle_lib as O
+        ms = 16384
+        nthreads = min(16, os.cpu_count() or 1)
+        chain, _ = make_chain((QRSpectrum(MGS()),))
+        xs = x0[:: max(1, n_tri // ms)][:ms]
+        W0 = O.frames(13, 1, n_tri, 2, 2)[:: max(1, n_tri // ms)][:ms]
+        tc = time.perf_counter()
+        O.run(StandardMapK().descriptor(), chain, xs, W0, N, 1, 2, math=O.MATH_LIBM, nthreads=nthreads)
+        tc = time.perf_counter() - tc
+        cpu = dict(value=float(len(xs)) * N / tc, unit="orbit-steps/s", cores=nthreads, kind="port",
+                   sample=f"plain method on {len(xs)} of the triaged orbits x {N} steps, oracle/oracle.cpp with "
+                          f"glibc sin/cos/log, {nthreads} threads, {tc:.1f} s")
+    value = float(n_tri) * N / t_tri
+    out = {
+        "metric": METRIC, "value": value, "unit": "orbit-steps/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": t_tri * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+bf16", "data": "synthetic",
+        "config": {"workload": desc, "orbits": n_tri, "steps_per_orbit": N},
+        "surrogate": {"speedup": t_plain / t_tri, "plain_s": t_plain, "triage_s": t_tri,
+                      "triage_prefix_s": tri["t_prefix"], "triage_rest_s": tri["t_rest"],
+                      "continued_fraction": float(1.0 - tri["confident"].mean()),
+                      "misclassified_vs_plain": wrong, "heldout": ev, "train_s": t_train, "data_s": t_data,
+                      "chaotic_fraction_train": info["chaotic_fraction"],
+                      "final_loss": [float(v) for v in info["history"][-1]]},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c5"])
+    ap.add_argument("--c5-sizes", default="1048576,262144,4194304,100000",
+                    help="c5: training orbits, held-out orbits, triaged orbits, steps N")
+    ap.add_argument("--c5-epochs", type=int, default=8)
     ap.add_argument("--cpu-orbits", type=int, default=32768, help="orbits per K in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -152,6 +239,8 @@ def main():
         print(f"bench.py measures one GPU: the ensemble shards by orbit ranges with no exchange during propagation, "
               f"and this build runs on single-GPU boxes only (asked for --gpus {a.gpus}).", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "c5":
+        return c5_main(a)
     wl = workload(a.workload)
     log("workload", wl["desc"])
     if a.copy_probe:

@@ -330,6 +330,21 @@ def fit(run, epochs=8, batch=4096, lr=1e-3, wreg=0.1, seed=0, log=None):
     return net, dict(history=hist, chaotic_fraction=float((lab == 1).mean()))
 
 
+def calibrate_margin(net, run, target=0.004, grid=(1.0, 1.5, 2.0, 2.5, 3.0, 3.5, 4.0, 5.0, 6.0, 8.0)):
+    """Smallest margin on `grid` whose misclassified fraction on a held-out full run is at most `target`
+    (the largest margin of the grid if none is)."""
+    tr = run["trace"]
+    m = tr.shape[1]
+    logit = net.predict(net.normalise(tr, m), m)[0].cpu().numpy()
+    _, _, lab = targets(run)
+    truth = lab == 1
+    for mg in grid:
+        conf = np.abs(logit) >= mg
+        if np.mean(conf & ((logit > 0) != truth)) <= target:
+            return mg
+    return grid[-1]
+
+
 def evaluate(net, run, margin=2.0):
     """Triage outcome on a held-out full run: orbits with |logit| >= margin take the network's class, the rest
     keep the full method's label. Returns the skipped fraction and the misclassified fraction (of all orbits)."""
@@ -382,3 +397,22 @@ def triage_run(net, x0, seed, N, prefix=256, margin=2.0, log=None):
     if log:
         log(f"triage: {mu} of {m} orbits continue ({mu / m:.3f}); prefix+net {t1 - t0:.2f}s, rest {t2 - t1:.2f}s")
     return dict(label=lab, lam_N10=lam10, lam_N=lamN, confident=conf_h, t_prefix=t1 - t0, t_rest=t2 - t1)
+
+
+def run_plain(x0, seed, N):
+    """The plain method on an ensemble: the QR spectrum (MGS, k = 2) to N/10 and N for every orbit."""
+    import torch
+    lib, ctx = _context()
+    m = len(x0)
+    dev = torch.device("cuda")
+    x = torch.from_numpy(np.ascontiguousarray(np.asarray(x0, dtype=np.float64).T)).to(dev)
+    W = torch.empty((4, m), dtype=torch.float64, device=dev)
+    _check(lib.cl_initial_deviations_device(ctx, seed, 1, m, 2, 2, W.data_ptr(),
+                                            torch.cuda.current_stream().cuda_stream), lib, ctx)
+    st = torch.zeros((13, m), dtype=torch.int64, device=dev)
+    code = torch.zeros(m, dtype=torch.int8, device=dev)
+    h = torch.zeros(m, dtype=torch.int64, device=dev)
+    continue_qr(x, W, st, code, h, 0, N, N // 10)
+    lam10 = qr_lambda1(st).clone()
+    continue_qr(x, W, st, code, h, N // 10, N, N - N // 10)
+    return dict(lam_N10=lam10, lam_N=qr_lambda1(st), code=code)

@@ -7,12 +7,17 @@ same BF16-valued weights and inputs. Its own check is a central-difference test 
 import numpy as np
 
 
-def forward(Ws, bs, X):
+def _same(a):
+    return a
+
+
+def forward(Ws, bs, X, rnd=_same):
+    """Activations per layer; `rnd` rounds each hidden activation (identity: exact FP64)."""
     acts = [X]
     A = X
     for l, (W, b) in enumerate(zip(Ws, bs)):
         Z = W @ A + b[:, None]
-        A = Z if l == len(Ws) - 1 else np.maximum(Z, 0.0)
+        A = Z if l == len(Ws) - 1 else rnd(np.maximum(Z, 0.0))
         acts.append(A)
     return acts
 
@@ -28,12 +33,14 @@ def loss_and_dout(out, y, t, w, wreg):
     return lb, lr, d
 
 
-def backward(Ws, acts, dout):
+def backward(Ws, acts, dout, rnd=_same):
+    """Weight and bias gradients; `rnd` rounds dout and each back-propagated error (identity: exact FP64).
+    With a BF16 rounding it models the error budget of a BF16-storage network, which bounds the GPU's error."""
     gW, gb = [None] * len(Ws), [None] * len(Ws)
-    dZ = dout
+    dZ = rnd(dout)
     for l in reversed(range(len(Ws))):
         gW[l] = dZ @ acts[l].T
         gb[l] = dZ.sum(axis=1)
         if l > 0:
-            dZ = (Ws[l].T @ dZ) * (acts[l] > 0)
+            dZ = rnd((Ws[l].T @ dZ) * (acts[l] > 0))
     return gW, gb

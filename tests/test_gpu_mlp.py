@@ -109,9 +109,10 @@ def test_loss_and_adam_against_numpy():
 
 
 def test_backward_against_fp64_oracle():
-    """Full network gradients on the GPU against the FP64 oracle on the same BF16 weights and inputs; the GPU rounds
-    activations and back-propagated errors to BF16, so agreement is within a few BF16 ulps relative to the
-    gradient scale."""
+    """Full network gradients on the GPU against the FP64 oracle on the same BF16 weights and inputs. The GPU
+    stores activations and back-propagated errors in BF16, so its error against exact arithmetic is bounded by
+    that of the same network with BF16 rounding at those points (the oracle run with a BF16 rounding), which
+    grows with depth through cancellation in the first layer's gradient (about 5% there, 0.02% at the last)."""
     import torch
     from chaos_amd.surrogate import SurrogateMLP
     say("backward vs oracle")
@@ -128,17 +129,27 @@ def test_backward_against_fp64_oracle():
     gW, gb = net.backward(Xd, B, acts, dout)
     Ws = [host_bf16(W) for W in net.Wb]
     bs = [b.cpu().numpy().astype(np.float64) for b in net.b]
-    ref_acts = MO.forward(Ws, bs, from_bf16_bits(to_bf16_bits(X)).astype(np.float64))
+    Xq = from_bf16_bits(to_bf16_bits(X)).astype(np.float64)
+    ref_acts = MO.forward(Ws, bs, Xq)
     o = out.cpu().numpy()
     assert np.max(np.abs(o - ref_acts[-1])) < 0.05 * (np.max(np.abs(ref_acts[-1])) + 1)
     _, _, d = MO.loss_and_dout(ref_acts[-1], y, t, w, 0.5)
     rW, rb = MO.backward(Ws, ref_acts, d)
+
+    def q(a):
+        return from_bf16_bits(to_bf16_bits(a)).astype(np.float64)
+    em_acts = MO.forward(Ws, bs, Xq, rnd=q)
+    _, _, de = MO.loss_and_dout(em_acts[-1].astype(np.float32).astype(np.float64), y, t, w, 0.5)
+    eW, eb = MO.backward(Ws, em_acts, de, rnd=q)
     for l in range(4):
-        g = gW[l].cpu().numpy()
-        rel = np.linalg.norm(g - rW[l]) / np.linalg.norm(rW[l])
-        assert rel < 0.05, (l, rel)
-        relb = np.linalg.norm(gb[l].cpu().numpy() - rb[l]) / np.linalg.norm(rb[l])
-        assert relb < 0.05, (l, relb)
+        n = np.linalg.norm(rW[l])
+        budget = np.linalg.norm(eW[l] - rW[l]) / n
+        rel = np.linalg.norm(gW[l].cpu().numpy() - rW[l]) / n
+        assert rel < 1.5 * budget + 5e-3, (l, rel, budget)
+        nb = np.linalg.norm(rb[l])
+        budget_b = np.linalg.norm(eb[l] - rb[l]) / nb
+        relb = np.linalg.norm(gb[l].cpu().numpy() - rb[l]) / nb
+        assert relb < 1.5 * budget_b + 5e-3, (l, relb, budget_b)
 
 
 def test_training_reduces_loss_on_separable_data():
