@@ -37,6 +37,12 @@ FP64_PEAK_TFLOPS = 78.6
 #      + OFLI on the map displacement (~145) + SALI (~30) + Householder QR spectrum (~150), about 500;
 # c1 = map step and one frame product (~57) + MGS, G_w, orthogonality loss and two logs (~117), about 174.
 FLOPS_PER_ORBIT_STEP = {"c2": 500.0, "c1": 174.0}
+# HBM bytes per orbit-kernel launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch), scaled
+# by what the counters read for a 1 GiB device copy (FETCH x 2.0, WRITE x 1.0); summaries under profiles/.
+TRAFFIC_BYTES_PER_LAUNCH = {"c2": (2 * 41214.06 + 730176.31) * 1024}
+# Mean Gauss fixed-point iterations per step on the c4 workload, measured with the oracle on its sample
+# (used when the CPU leg is skipped).
+C4_ITERATIONS_DEFAULT = 6.3
 
 
 def log(*a):
@@ -44,33 +50,41 @@ def log(*a):
 
 
 def workload(name):
-    from chaos_amd import MEGNO, FLI, OFLI, SALI, QRSpectrum, HouseholderQR, MGS
-    from chaos_amd.inputs import grid_centres, torus_uniform
+    from chaos_amd import MEGNO, FLI, OFLI, SALI, GALI, QRSpectrum, HouseholderQR, MGS, GivensQR
+    from chaos_amd import StandardMap, HenonHeilesGauss
+    from chaos_amd.inputs import grid_centres, torus_uniform, hh_section_starts
     if name == "c2":
-        return dict(name="c2", Ks=[0.97, 2.0, 6.0], x0=grid_centres(1024), N=10_000, tau=1, seed=1,
-                    red=(MEGNO(replace=False), FLI(replace=False), OFLI(replace=False), SALI()),
+        return dict(name="c2", systems=[StandardMap(K) for K in (0.97, 2.0, 6.0)], x0=grid_centres(1024), N=10_000,
+                    tau=1, seed=1, d=2, k=2, red=(MEGNO(replace=False), FLI(replace=False), OFLI(replace=False), SALI()),
                     red2=(QRSpectrum(HouseholderQR()),),
                     desc="standard map K in {0.97,2,6}, 1024^2 grid centres per K, 1e4 steps, k=2, tau=1, "
                          "MEGNO+FLI+OFLI+SALI on frame 1 + QR(Householder) on frame 2, FP64")
     if name == "c1":
-        return dict(name="c1", Ks=[6.0], x0=torus_uniform(1, 4096), N=100_000, tau=1, seed=1,
-                    red=(QRSpectrum(MGS()),), red2=(),
+        return dict(name="c1", systems=[StandardMap(6.0)], x0=torus_uniform(1, 4096), N=100_000, tau=1, seed=1,
+                    d=2, k=2, red=(QRSpectrum(MGS()),), red2=(),
                     desc="standard map K=6, 4096 starts uniform on the torus (seed 1), 1e5 steps, k=2, tau=1, "
                          "QRSpectrum(MGS()), FP64")
+    if name == "c4":
+        m = 65536
+        x0 = np.concatenate([hh_section_starts(1, m, E, orbit0=1 + j * m) for j, E in enumerate((1 / 12, 1 / 8, 1 / 6))])
+        return dict(name="c4", systems=[HenonHeilesGauss(h=0.01, tol=1e-15)], x0=x0, N=100_000, tau=10, seed=1,
+                    d=4, k=4, red=(MEGNO(replace=False), GALI()), red2=(QRSpectrum(GivensQR()),), N_warm=100,
+                    desc="Henon-Heiles, Gauss s=3, h=0.01, tol 1e-15; 65536 starts on the q2=0 section (p2>0) at each "
+                         "E in {1/12,1/8,1/6}, one launch of 196608 orbits; 1e5 steps, k=4, tau=10, "
+                         "MEGNO(columns 1-2)+GALI_2..4 on frame 1 + QR(Givens) on frame 2, FP64")
     raise SystemExit(f"unknown workload {name}")
 
 
 def gpu_run(wl, steps, warmup):
     import torch
-    from chaos_amd import StandardMap, _lib as L
     from chaos_amd.tangent import make_chain, _context
     lib, ctx = _context()
     x0 = wl["x0"]
-    m = len(x0)
-    d = k = 2
+    m, s = x0.shape
+    d, k = wl["d"], wl["k"]
     chain, allr = make_chain(wl["red"], wl["red2"])
     nf = 2 if wl["red2"] else 1
-    sd0 = StandardMap(1.0).descriptor()
+    sd0 = wl["systems"][0].descriptor()
     words = lib.cl_state_bytes(ctypes.byref(sd0), ctypes.byref(chain), 1, d, k) // 8
     dev = torch.device("cuda")
     x_master = torch.from_numpy(np.ascontiguousarray(x0.T)).to(dev)
@@ -86,8 +100,8 @@ def gpu_run(wl, steps, warmup):
     kernel_ms = []
 
     def one_step(i, timed):
-        K = wl["Ks"][i % len(wl["Ks"])]
-        sd = StandardMap(K).descriptor()
+        sd = wl["systems"][i % len(wl["systems"])].descriptor()
+        n = N if timed else wl.get("N_warm", N)
         dx.copy_(x_master)
         rc = lib.cl_initial_deviations_device(ctx, wl["seed"], 1, m, d, k, dW.data_ptr(), sh)
         assert rc == 0
@@ -95,7 +109,7 @@ def gpu_run(wl, steps, warmup):
             dW2.copy_(dW)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        rc = lib.cl_propagate_device(ctx, ctypes.byref(sd), ctypes.byref(chain), 1, m, 2, d, k, 0, N, N, tau,
+        rc = lib.cl_propagate_device(ctx, ctypes.byref(sd), ctypes.byref(chain), 1, m, s, d, k, 0, n, n, tau,
                                      dx.data_ptr(), dW.data_ptr(), dW2.data_ptr() if dW2 is not None else None,
                                      dst.data_ptr(), dcode.data_ptr(), dh.data_ptr(), sh)
         e1.record(stream)
@@ -121,27 +135,44 @@ def gpu_run(wl, steps, warmup):
 
 def cpu_baseline(wl, budget_orbits):
     """The oracle (glibc sin/cos/log) on the host's cores over a bounded sample of the workload."""
-    from chaos_amd import StandardMap
     from chaos_amd.tangent import make_chain
     from tests import oracle_lib as O
     x0 = wl["x0"]
     stride = max(1, len(x0) // budget_orbits)
     xs = x0[::stride]
     m = len(xs)
-    W0 = O.frames(wl["seed"], 1, len(x0), 2, 2)[::stride]
+    W0 = O.frames(wl["seed"], 1, len(x0), wl["d"], wl["k"])[::stride]
     chain, _ = make_chain(wl["red"], wl["red2"])
     nthreads = min(16, os.cpu_count() or 1)
     total, t = 0.0, 0.0
-    for K in wl["Ks"]:
+    O.gauss_iterations()
+    for sysm in wl["systems"]:
         t0 = time.perf_counter()
-        O.run(StandardMap(K).descriptor(), chain, xs, W0, wl["N"], wl["tau"], 2,
+        O.run(sysm.descriptor(), chain, xs, W0, wl["N"], wl["tau"], wl["k"],
               W2=W0 if wl["red2"] else None, math=O.MATH_LIBM, nthreads=nthreads)
         t += time.perf_counter() - t0
         total += float(m) * wl["N"]
-        log(f"cpu baseline K={K}: {m} orbits done")
-    return dict(value=total / t, unit="orbit-steps/s", cores=nthreads, kind="port",
-                sample=f"every {stride}th orbit of each K's set ({m} orbits x {wl['N']} steps x {len(wl['Ks'])} K), "
-                       f"oracle/oracle.cpp with glibc sin/cos/log, {nthreads} threads, {t:.1f} s")
+        log(f"cpu baseline {sysm!r}: {m} orbits done")
+    it, st = O.gauss_iterations()
+    ns = len(wl["systems"])
+    out = dict(value=total / t, unit="orbit-steps/s", cores=nthreads, kind="port",
+               sample=f"every {stride}th orbit ({m} orbits x {wl['N']} steps x {ns} system(s)), "
+                      f"oracle/oracle.cpp with glibc sin/cos/log, {nthreads} threads, {t:.1f} s")
+    if st:
+        out["gauss_iterations_per_step"] = it / st
+    return out
+
+
+def flops_per_orbit_step(wl, cpu):
+    """Algorithmic FP64 operations per orbit-step (fma = 2, add/mul/div/sqrt = 1, sin/cos/log = 20)."""
+    if wl["name"] != "c4":
+        return FLOPS_PER_ORBIT_STEP[wl["name"]]
+    # Gauss s = 3 on Henon-Heiles: per fixed-point iteration 3 vector fields (~10 each) and 12 compensated
+    # 3-term dot products (~62 each), ~780; the stage solve of the tangent ~2300 (12 x 12 LU); the handoff
+    # (Givens 4 x 4, MEGNO, GALI ~750) over tau = 10. Iterations per step from the oracle sample when it ran.
+    it = cpu.get("gauss_iterations_per_step") if cpu else None
+    it = it if it else C4_ITERATIONS_DEFAULT
+    return 780.0 * it + 2300.0 + 75.0
 
 
 def c5_main(a):
@@ -226,7 +257,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c4", "c5"])
     ap.add_argument("--c5-sizes", default="1048576,262144,4194304,100000",
                     help="c5: training orbits, held-out orbits, triaged orbits, steps N")
     ap.add_argument("--c5-epochs", type=int, default=8)
@@ -254,18 +285,19 @@ def main():
     g = gpu_run(wl, a.steps, a.warmup)
     value = g["orbit_steps"] * a.steps / g["wall"]
     avg_ms = float(np.mean(g["kernel_ms"]))
-    flops = FLOPS_PER_ORBIT_STEP[wl["name"]] * g["orbit_steps"]
-    achieved = flops / (avg_ms * 1e-3) / 1e12
-    cpu = None if a.no_cpu else cpu_baseline(wl, a.cpu_orbits if wl["name"] == "c2" else 512)
+    budget = {"c2": a.cpu_orbits, "c1": 512, "c4": 2048}[wl["name"]]
+    cpu = None if a.no_cpu else cpu_baseline(wl, budget)
+    fpos = flops_per_orbit_step(wl, cpu)
+    achieved = fpos * g["orbit_steps"] / (avg_ms * 1e-3) / 1e12
     out = {
         "metric": METRIC, "value": value, "unit": "orbit-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": g["wall"] * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": wl["desc"], "orbits_per_step": g["m"], "steps_per_orbit": wl["N"],
-                   "tau": wl["tau"], "k": 2, "nonfinite_orbits": g["bad"]},
+                   "tau": wl["tau"], "k": wl["k"], "nonfinite_orbits": g["bad"]},
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel_ms": avg_ms, "flops_per_orbit_step": FLOPS_PER_ORBIT_STEP[wl["name"]]},
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": TRAFFIC_BYTES_PER_LAUNCH.get(wl["name"]),
+                     "kernel_ms": avg_ms, "flops_per_orbit_step": fpos},
         "cpu_baseline": cpu,
     }
     if cpu:

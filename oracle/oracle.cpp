@@ -881,6 +881,11 @@ static bool red_failed(const cl_reducer& r, const RState<T>& s) {
 // ------------------------------------------------------------------ systems (d = 2 maps)
 #define OR_SYS_LINEAR 100  // test-only linear map x' = A x, tangent A (test/tangent-core/tc_helpers.jl:8-10,26-28)
 
+// Fixed-point iterations of the Gauss stage equations, counted per worker thread and summed at the end of
+// a run (the work census of the Henon-Heiles workload is proportional to them).
+static thread_local int64_t tl_gauss_iters = 0, tl_gauss_steps = 0;
+static std::atomic<int64_t> g_gauss_iters(0), g_gauss_steps(0);
+
 template <class P, class T>
 struct MapSys {
   int kind;
@@ -1029,6 +1034,8 @@ struct HHSys {
       stopped = delta == 0 || (it >= 2 && delta >= prev);
       prev = delta;
     }
+    tl_gauss_iters += it;
+    tl_gauss_steps += 1;
     T bound = tol;
     if (P::nan(tol)) {
       T mx = P::abs(Y[0][0]);
@@ -1244,6 +1251,9 @@ static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, in
           for (int r = 0; r < ch->n; ++r) write_state(ch->r[r], st[r], d, k, states + i * sb + off[r]);
       }
     }
+    g_gauss_iters += tl_gauss_iters;
+    g_gauss_steps += tl_gauss_steps;
+    tl_gauss_iters = tl_gauss_steps = 0;
   };
   if (nthreads <= 1) {
     worker();
@@ -1263,6 +1273,12 @@ static bool or_dims_ok(const cl_system* sys, int d, int k) {
 }
 
 extern "C" {
+
+// Gauss fixed-point iterations and steps summed over the runs since the last call; resets both.
+void or_gauss_iterations(int64_t* iters, int64_t* steps) {
+  *iters = g_gauss_iters.exchange(0);
+  *steps = g_gauss_steps.exchange(0);
+}
 
 // Bytes per orbit of the chain's state tuple (Julia layout), -1 for an unknown reducer.
 int64_t or_state_bytes(const cl_chain* ch, int d, int k) {

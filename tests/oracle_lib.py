@@ -33,6 +33,8 @@ def load():
         o.or_propagate_linear.argtypes = [P, ctypes.POINTER(L.cl_chain), I64, I, I64, I, P, P, P, P, P, I]
         o.or_initial_deviations.restype = I
         o.or_initial_deviations.argtypes = [ctypes.c_uint64, I64, I64, I, I, P]
+        o.or_gauss_iterations.restype = None
+        o.or_gauss_iterations.argtypes = [P, P]
         o.or_csum.restype = None
         o.or_csum.argtypes = [I64, ctypes.c_double, P, P]
         o.or_csum_list.restype = None
@@ -69,6 +71,13 @@ def run(sysd, chain, x0, W0, N, tau, k, W2=None, math=MATH_CL, nthreads=None, qu
     if W2a is not None:
         out["W2"] = W2a.transpose(0, 2, 1).copy()
     return out
+
+
+def gauss_iterations():
+    """(iterations, steps) of the Gauss stage solver summed over oracle runs since the last call."""
+    it, st = ctypes.c_int64(0), ctypes.c_int64(0)
+    load().or_gauss_iterations(ctypes.byref(it), ctypes.byref(st))
+    return it.value, st.value
 
 
 def run_linear(A, chain, x0, W0, N, tau, k, quad=False):
