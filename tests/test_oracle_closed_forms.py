@@ -288,3 +288,23 @@ def test_per_orbit_k_map_equals_fixed_k_runs():
         b = O.run(StandardMap(K).descriptor(), ch, x2[i:i + 1], W0[i:i + 1], 500, 1, 2)
         assert np.array_equal(a["states"][i], b["states"][0])
         assert np.array_equal(a["x"][i, :2], b["x"][0]) and a["x"][i, 2] == K
+
+
+def test_readme_single_orbit_anchor():
+    """README.md:19-21: K = 6, x0 = (0.1, 0.2), N = 1e5, k = 2, tau = 1, seed 1, MGS gives lambda = +-1.1348588476487835
+    with Julia's libm and StableRNG frames. Bitwise agreement needs both (unpinnable here), so this is a statistical
+    anchor: the finite-time exponent of a chaotic orbit at N = 1e5 scatters by about 3e-3 between orbit
+    realisations; the pair sums to zero (symplectic pairing)."""
+    ch, allr = make_chain((QRSpectrum(MGS()),))
+    x0 = np.array([[0.1, 0.2]])
+    W0 = O.frames(1, 1, 1, 2, 2)
+    for math_mode in (O.MATH_CL, O.MATH_LIBM):
+        r = O.run(L_std_map(6.0), ch, x0, W0, 100_000, 1, 2, math=math_mode)
+        lam = finals_of(decode_states(r["states"], allr, 2, 2), allr, 2, 2)[0]["lambda"][0]
+        assert abs(lam[0] - 1.1348588476487835) < 0.02
+        assert abs(lam[0] + lam[1]) < 1e-12
+
+
+def L_std_map(K):
+    from chaos_amd import StandardMap
+    return StandardMap(K).descriptor()
