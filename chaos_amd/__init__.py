@@ -3,11 +3,11 @@
 Hand-written HIP kernels for gfx950 behind a C ABI (include/chaoslab_hip.h); this package is the
 Python host side of that ABI, mirroring the reference's `propagate` API and reducer types.
 """
-from .systems import StandardMap, StandardMapK, WrappedStandardMap, HenonHeilesGauss, torus_state, hh_energy
+from .systems import StandardMap, StandardMapK, StandardMapChain, WrappedStandardMap, HenonHeilesGauss, torus_state, hh_energy
 from .reducers import (QRSpectrum, CGS, MGS, RGS, HouseholderQR, AugmentedQR, GivensQR, Benettin, MEGNO, SALI, FLI,
                        OFLI, GALI)
 from .tangent import propagate, PropagationResult, NumericalHealthError, initial_deviations
 
-__all__ = ["StandardMap", "StandardMapK", "WrappedStandardMap", "HenonHeilesGauss", "hh_energy", "torus_state", "QRSpectrum", "CGS", "MGS", "RGS", "HouseholderQR",
+__all__ = ["StandardMap", "StandardMapK", "StandardMapChain", "WrappedStandardMap", "HenonHeilesGauss", "hh_energy", "torus_state", "QRSpectrum", "CGS", "MGS", "RGS", "HouseholderQR",
            "AugmentedQR", "GivensQR", "Benettin", "MEGNO", "SALI", "FLI", "OFLI", "GALI", "propagate",
            "PropagationResult", "NumericalHealthError", "initial_deviations"]

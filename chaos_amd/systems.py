@@ -24,6 +24,37 @@ class StandardMap:
         return f"StandardMap({self.K!r})"
 
 
+class StandardMapChain:
+    """StandardMapChain(N, K, eps; boundary): N coupled standard maps, state (q_1..q_N, p_1..p_N) on lifted coordinates,
+    p_j' = p_j + K sin q_j + eps sum_nbr sin(q_j - q_k), q_j' = q_j + p_j' (src/Systems/standard_map_chain.jl:1-53).
+    `boundary` is "open" or "ring" (ring needs N >= 3). The reference bounds N <= 100; this build accepts N <= 128
+    and its device kernels take N in {8, 16, 32, 64, 128}."""
+
+    def __init__(self, N, K, eps, boundary="open"):
+        N = int(N)
+        if boundary not in ("open", "ring"):
+            raise ValueError(f"boundary must be 'open' or 'ring', got {boundary!r}")
+        if not 2 <= N <= 128:
+            raise ValueError(f"number of sites N must be in 2:128, got {N}")
+        if boundary == "ring" and N < 3:
+            raise ValueError(f"a ring needs N >= 3, got {N}")
+        self.N, self.K, self.eps, self.boundary = N, float(K), float(eps), boundary
+        self.dimension = 2 * N
+        self.state_length = 2 * N
+
+    def descriptor(self):
+        s = L.cl_system()
+        s.kind = L.CL_SYS_STANDARD_MAP_CHAIN
+        s.N = self.N
+        s.ring = 1 if self.boundary == "ring" else 0
+        s.K = self.K
+        s.eps = self.eps
+        return s
+
+    def __repr__(self):
+        return f"StandardMapChain({self.N}, {self.K!r}, {self.eps!r}; boundary={self.boundary})"
+
+
 class WrappedStandardMap:
     """WrappedStandardMap(K): the standard map on [0, 2pi)^2 with exact winding counts, state (x, p, m, n)
     (src/Systems/wrapped_standard_map.jl:75-123)."""

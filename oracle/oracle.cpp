@@ -14,6 +14,7 @@
 //   OFLI                  src/Indicators/ofli.jl:30-124 (direction field: the flow's, or the map displacement)
 //   GALI                  src/Indicators/gali.jl:193-372
 //   LogNorm (test reducer) test/tangent-core/tc_helpers.jl:56-70
+//   StandardMapChain, ChainTangent and the Householder QR spectrum on d = 2N rows: oracle_chain.hpp
 //
 // Independence from the product: the reducer chain is interpreted at run time over byte buffers
 // (the product compiles one kernel per chain), and every reducer is written out again here. The
@@ -558,7 +559,7 @@ static void qr_givens(const T* W, int d, int k, Fact<P, T>& f) {
 
 template <class P, class T>
 static T window_gw(const T* R0, int k) {
-  T R[64], Ri[64];
+  T R[256], Ri[256];
   T m = P::abs(R0[0]);
   for (int i = 1; i < k * k; ++i) m = jmax<P>(m, P::abs(R0[i]));
   bool sc = (m > (T)1e100 || m < (T)1e-100) && P::finite(m) && m > 0;
@@ -1272,7 +1273,27 @@ static bool or_dims_ok(const cl_system* sys, int d, int k) {
   return d == want && k >= 1 && k <= d;
 }
 
+#include "oracle_chain.hpp"
+
 extern "C" {
+
+// The coupled standard-map chain (kind CL_SYS_STANDARD_MAP_CHAIN) with no reducer or QRSpectrum(HouseholderQR()),
+// k <= 16, full runs from fresh states. Buffers are orbit-major: x[i][2N], W[i][k][2N] (column-major frames),
+// states[i][16k + 72]. Ns (optional) gives each orbit its own step count (a multiple of tau), else N for all.
+// mode RED_ROWS (0) is the reference's row-order reductions, RED_SITES (1) the device kernel's.
+int or_propagate_chain(const cl_system* sys, const cl_chain* ch, int64_t m, int k, const int64_t* Ns, int64_t N,
+                       int tau, double* x, double* W, void* states, int8_t* code, int64_t* hstep, int math, int mode,
+                       int nthreads) {
+  if (sys->kind != CL_SYS_STANDARD_MAP_CHAIN || sys->N < 2 || sys->N > 128 || (sys->ring && sys->N < 3)) return 10;
+  if (k < 1 || k > 16 || k > 2 * sys->N || tau < 1) return 10;
+  if (mode == RED_SITES && (sys->N & (sys->N - 1))) return 10;  // the butterfly needs a power-of-two site count
+  if (ch->n > 1 || (ch->n == 1 && !(ch->r[0].kind == CL_RED_QR && ch->r[0].alg == CL_QR_HOUSEHOLDER))) return 3;
+  for (int64_t i = 0; i < m; ++i)
+    if ((Ns ? Ns[i] : N) % tau) return 10;
+  if (math == MATH_CL)
+    return chain_all<F64Cl>(sys, ch, m, k, Ns, N, tau, x, W, (uint8_t*)states, code, hstep, mode, nthreads);
+  return chain_all<F64Libm>(sys, ch, m, k, Ns, N, tau, x, W, (uint8_t*)states, code, hstep, mode, nthreads);
+}
 
 // Gauss fixed-point iterations and steps summed over the runs since the last call; resets both.
 void or_gauss_iterations(int64_t* iters, int64_t* steps) {
@@ -1351,10 +1372,11 @@ static uint64_t mixr(uint64_t z) {
   return z ^ (z >> 31);
 }
 int or_initial_deviations(uint64_t seed, int64_t orbit0, int64_t m, int d, int k, double* W) {
-  if (d > 8 || k > d || k < 1) return 10;
+  if (d > 256 || k > d || k < 1 || k > 16) return 10;
+  std::vector<double> Gv((size_t)d * k);
   for (int64_t o = 0; o < m; ++o) {
     uint64_t key = mixr(mixr(seed) + (uint64_t)(orbit0 + o));
-    double G[64];
+    double* G = Gv.data();
     for (int j = 0; j < k; ++j)
       for (int i = 0; i < d; ++i) {
         uint64_t t = (uint64_t)(j * d + i), p = t >> 1;

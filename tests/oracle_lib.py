@@ -33,6 +33,9 @@ def load():
         o.or_propagate_linear.argtypes = [P, ctypes.POINTER(L.cl_chain), I64, I, I64, I, P, P, P, P, P, I]
         o.or_initial_deviations.restype = I
         o.or_initial_deviations.argtypes = [ctypes.c_uint64, I64, I64, I, I, P]
+        o.or_propagate_chain.restype = I
+        o.or_propagate_chain.argtypes = [ctypes.POINTER(L.cl_system), ctypes.POINTER(L.cl_chain), I64, I, P, I64, I,
+                                         P, P, P, P, P, I, I, I]
         o.or_gauss_iterations.restype = None
         o.or_gauss_iterations.argtypes = [P, P]
         o.or_csum.restype = None
@@ -71,6 +74,27 @@ def run(sysd, chain, x0, W0, N, tau, k, W2=None, math=MATH_CL, nthreads=None, qu
     if W2a is not None:
         out["W2"] = W2a.transpose(0, 2, 1).copy()
     return out
+
+
+RED_ROWS, RED_SITES = 0, 1
+
+
+def run_chain(sysd, chain, x0, W0, N, tau, k, Ns=None, math=MATH_CL, mode=RED_SITES, nthreads=None):
+    """Chain runs from fresh states: x0 (m, 2N), W0 (m, 2N, k); Ns optional per-orbit step counts."""
+    o = load()
+    x = np.ascontiguousarray(np.array(x0, dtype=np.float64))
+    m, d = x.shape
+    W = np.array(np.asarray(W0, dtype=np.float64).reshape(m, d, k).transpose(0, 2, 1), order="C", copy=True)
+    nb = 16 * k + 72 if chain.n else 0
+    states = np.zeros((m, max(nb, 1)), dtype=np.uint8)
+    code = np.zeros(m, dtype=np.int8)
+    hstep = np.zeros(m, dtype=np.int64)
+    Na = None if Ns is None else np.ascontiguousarray(np.asarray(Ns, dtype=np.int64))
+    nt = nthreads or min(16, os.cpu_count() or 1)
+    rc = o.or_propagate_chain(ctypes.byref(sysd), ctypes.byref(chain), m, k, _ptr(Na), int(N), tau, _ptr(x), _ptr(W),
+                              _ptr(states), _ptr(code), _ptr(hstep), math, mode, nt)
+    assert rc == 0, rc
+    return dict(x=x, W=W.transpose(0, 2, 1).copy(), states=states[:, :nb], code=code, hstep=hstep)
 
 
 def gauss_iterations():
