@@ -64,6 +64,8 @@ SIGNATURES = {
                                 I64, I, P, P, P, P, P, P, P]),
     "cl_propagate_device_traced": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I64, I, I, I, I64,
                                        I64, I64, I, P, P, P, P, P, P, P, I64, I, P]),
+    "cl_propagate_ragged": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I64, I, I, I, P, P, I,
+                                P, P, P, P, P, P]),
     "cl_initial_deviations": (I, [ctypes.c_uint64, I64, I64, I, I, P]),
     "cl_initial_deviations_device": (I, [P, ctypes.c_uint64, I64, I64, I, I, P, P]),
     "cl_math_host": (None, [I64, P, P, P, P]),

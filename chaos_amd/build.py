@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchaoslab_hip.so")
-SOURCES = ["propagate.hip", "inst_maps.hip", "inst_flow.hip", "mlp.hip"]
+SOURCES = ["chain_kernels.hip", "propagate.hip", "inst_maps.hip", "inst_flow.hip", "mlp.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CL_OFFLOAD_ARCH", "gfx950")
 

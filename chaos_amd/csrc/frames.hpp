@@ -67,4 +67,34 @@ CL_HD void seeded_frame(uint64_t seed, int64_t orbit, Frame<D, K>& W) {
   W = G;
 }
 
+// The same rule for the chain sizes (d up to 256, k up to 16), with run-time sizes, in place in G
+// (column-major d x k): draws, two modified Gram-Schmidt passes per column, division by the scaled norm.
+CL_HD void seeded_frame_dyn(uint64_t seed, int64_t orbit, int d, int k, double* G) {
+  uint64_t key = orbit_key(seed, orbit);
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < d; ++i) G[j * d + i] = normal_draw(key, (uint64_t)(j * d + i));
+  for (int j = 0; j < k; ++j) {
+    double* g = G + j * d;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int i = 0; i < j; ++i) {
+        const double* gi = G + i * d;
+        double c = gi[0] * g[0];
+        for (int l = 1; l < d; ++l) c = c + gi[l] * g[l];
+        for (int l = 0; l < d; ++l) g[l] = g[l] - c * gi[l];
+      }
+    double mx = cl_fabs(g[0]);
+    for (int l = 1; l < d; ++l) mx = cl_jmax(mx, cl_fabs(g[l]));
+    double nr = mx;
+    if (cl_isfinite(mx) && mx > 0.0) {
+      double s = (g[0] / mx) * (g[0] / mx);
+      for (int l = 1; l < d; ++l) {
+        double b = g[l] / mx;
+        s = s + b * b;
+      }
+      nr = mx * cl_sqrt(s);
+    }
+    for (int l = 0; l < d; ++l) g[l] = g[l] / nr;
+  }
+}
+
 }  // namespace cl

@@ -6,8 +6,10 @@
 #include <stdlib.h>
 #include <string.h>
 #include <mutex>
+#include <vector>
 #include "registry.hpp"
 #include "frames.hpp"
+#include "chain_kernels.hpp"
 
 using namespace cl;
 
@@ -16,8 +18,8 @@ struct cl_ctx {
   hipStream_t stream;
   char err[256];
   // grown-on-demand device buffers of the host entry point
-  void* buf[12];
-  size_t cap[12];
+  void* buf[14];
+  size_t cap[14];
 };
 
 static char g_create_err[256] = "";
@@ -104,6 +106,53 @@ static int validate(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int e
   return CL_OK;
 }
 
+// The coupled chain (kind CL_SYS_STANDARD_MAP_CHAIN): state 2N, frames 2N x k, no reducer or
+// QRSpectrum(HouseholderQR()); kernels for N in {8, 16, 32, 64, 128} and k = 16 (chain_kernels.hip).
+// Returns the state words per orbit through *words, -1 when the chain has no kernel.
+static int validate_chain(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m, int s, int d,
+                          int k, int tau, int* words, bool* qr) {
+  const int N = sys->N;
+  if (N < 2 || N > 128 || (sys->ring && N < 3)) return set_err(ctx, CL_ERR_ARGUMENT, "chain sites out of range");
+  if (d != 2 * N || s != 2 * N) return set_err(ctx, CL_ERR_ARGUMENT, "state length and d must be 2N for the chain");
+  if (!(k >= 1 && k <= d)) return set_err(ctx, CL_ERR_ARGUMENT, "k must be in 1:d");
+  if (tau < 1) return set_err(ctx, CL_ERR_ARGUMENT, "tau must be positive");
+  if (m < 0) return set_err(ctx, CL_ERR_ARGUMENT, "negative orbit count");
+  if (elt != CL_ELT_F64) return set_err(ctx, CL_ERR_UNSUPPORTED, "element type has no kernel for this system");
+  if (!chain_supported(N, k)) return set_err(ctx, CL_ERR_UNSUPPORTED, "no chain kernel for this N and k");
+  if (ch->n == 0) {
+    *qr = false;
+    *words = 0;
+  } else if (ch->n == 1 && ch->r[0].kind == CL_RED_QR && ch->r[0].alg == CL_QR_HOUSEHOLDER && ch->r[0].frame == 0) {
+    *qr = true;
+    *words = 2 * k + 9;
+  } else {
+    return set_err(ctx, CL_ERR_UNSUPPORTED, "chain kernels run no reducer or QRSpectrum(HouseholderQR())");
+  }
+  return CL_OK;
+}
+
+static ChainArgs chain_args(const cl_system* sys, const cl_chain* ch, int64_t m, double* dx, double* dW,
+                            uint64_t* dst, int8_t* dcode, int64_t* dhstep, int64_t n0, int64_t steps, int64_t N,
+                            int tau) {
+  ChainArgs a;
+  a.x = dx;
+  a.W = dW;
+  a.st = dst;
+  a.code = dcode;
+  a.hstep = dhstep;
+  a.Ns = nullptr;
+  a.order = nullptr;
+  a.m = m;
+  a.n0 = n0;
+  a.steps = steps;
+  a.N = N;
+  a.tau = tau;
+  a.K = sys->K;
+  a.eps = sys->eps;
+  a.tol = ch->n > 0 ? ch->r[0].tol : 8.0;
+  return a;
+}
+
 // ------------------------------------------------------------------------------ helper kernels
 // AoS[i][w] <-> SoA[w][i] for 8-byte words.
 __global__ void k_aos_to_soa(const uint64_t* __restrict__ a, uint64_t* __restrict__ s, int64_t m, int w) {
@@ -187,7 +236,7 @@ extern "C" cl_ctx* cl_create(int device) {
 extern "C" void cl_destroy(cl_ctx* c) {
   if (!c) return;
   hipSetDevice(c->device);
-  for (int i = 0; i < 12; ++i)
+  for (int i = 0; i < 14; ++i)
     if (c->buf[i]) hipFree(c->buf[i]);
   hipStreamDestroy(c->stream);
   free(c);
@@ -201,6 +250,12 @@ extern "C" __attribute__((visibility("hidden"))) hipStream_t cl_ctx_stream(cl_ct
 extern "C" int64_t cl_state_bytes(const cl_system* sys, const cl_chain* ch, int elt, int d, int k) {
   if (!sys || !ch || ch->n < 0 || ch->n > CL_MAX_REDUCERS) return -CL_ERR_ARGUMENT;
   if (elt != CL_ELT_F64) return -CL_ERR_UNSUPPORTED;
+  if (sys->kind == CL_SYS_STANDARD_MAP_CHAIN) {
+    int words = 0;
+    bool qr = false;
+    int rc = validate_chain(nullptr, sys, ch, elt, 0, d, d, k, 1, &words, &qr);
+    return rc ? -(int64_t)rc : (int64_t)words * 8;
+  }
   const Instance* in = find_instance(sys, ch, d, k);
   if (!in) return -CL_ERR_UNSUPPORTED;
   return (int64_t)in->words * 8;
@@ -241,11 +296,31 @@ static int launch(const Instance* in, const cl_system* sys, const cl_chain* ch, 
   return in->launch(sys, ch, a, st);
 }
 
+static int chain_device(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m, int s, int d,
+                        int k, int64_t n0, int64_t steps, int64_t N, int tau, double* dx, double* dW, uint64_t* dst,
+                        int8_t* dcode, int64_t* dhstep, hipStream_t st) {
+  int words = 0;
+  bool qr = false;
+  int rc = validate_chain(ctx, sys, ch, elt, m, s, d, k, tau, &words, &qr);
+  if (rc) return rc;
+  if (!(N >= 1 && N % tau == 0 && steps >= 1 && steps % tau == 0 && n0 >= 0 && n0 % tau == 0 && n0 + steps <= N))
+    return set_err(ctx, CL_ERR_ARGUMENT, "launch steps must be a positive multiple of tau within N");
+  rc = chain_launch(chain_args(sys, ch, m, dx, dW, dst, dcode, dhstep, n0, steps, N, tau), sys->N, sys->ring != 0,
+                    qr, st);
+  return rc ? set_err(ctx, rc, "chain kernel launch failed") : CL_OK;
+}
+
 extern "C" int cl_propagate_device(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m,
                                    int s, int d, int k, int64_t n0, int64_t steps, int64_t N, int tau,
                                    double* dx, double* dW, double* dW2, uint64_t* dst, int8_t* dcode,
                                    int64_t* dhstep, void* stream) {
   if (!ctx) return CL_ERR_ARGUMENT;
+  if (sys && ch && sys->kind == CL_SYS_STANDARD_MAP_CHAIN) {
+    if (dW2) return set_err(ctx, CL_ERR_UNSUPPORTED, "chain kernels carry one frame");
+    hipSetDevice(ctx->device);
+    return chain_device(ctx, sys, ch, elt, m, s, d, k, n0, steps, N, tau, dx, dW, dst, dcode, dhstep,
+                        stream ? (hipStream_t)stream : ctx->stream);
+  }
   const Instance* in = nullptr;
   int rc = validate(ctx, sys, ch, elt, m, s, d, k, n0, steps, N, tau, &in);
   if (rc) return rc;
@@ -275,7 +350,92 @@ extern "C" int cl_propagate_device_traced(cl_ctx* ctx, const cl_system* sys, con
   return CL_OK;
 }
 
-enum { B_X, B_W, B_W2, B_ST, B_CODE, B_H, B_XA, B_WA, B_W2A, B_STA };
+enum { B_X, B_W, B_W2, B_ST, B_CODE, B_H, B_XA, B_WA, B_W2A, B_STA, B_NS, B_ORDER };
+
+extern "C" int cl_propagate_ragged(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m, int s,
+                                   int d, int k, const int64_t* Ns, const int64_t* order, int tau, double* dx,
+                                   double* dW, uint64_t* dst, int8_t* dcode, int64_t* dhstep, void* stream) {
+  if (!ctx || !sys || !ch) return CL_ERR_ARGUMENT;
+  if (sys->kind != CL_SYS_STANDARD_MAP_CHAIN)
+    return set_err(ctx, CL_ERR_UNSUPPORTED, "per-orbit step counts are built for the chain kernels");
+  int words = 0;
+  bool qr = false;
+  int rc = validate_chain(ctx, sys, ch, elt, m, s, d, k, tau, &words, &qr);
+  if (rc) return rc;
+  if (m == 0) return CL_OK;
+  if (!Ns) return set_err(ctx, CL_ERR_ARGUMENT, "null step counts");
+  int64_t Nmax = 0;
+  for (int64_t i = 0; i < m; ++i) {
+    if (!(Ns[i] >= 1 && Ns[i] % tau == 0)) return set_err(ctx, CL_ERR_ARGUMENT, "each N_i must be a positive multiple of tau");
+    Nmax = Ns[i] > Nmax ? Ns[i] : Nmax;
+  }
+  if (order) {
+    std::vector<char> seen((size_t)m, 0);
+    for (int64_t i = 0; i < m; ++i) {
+      if (order[i] < 0 || order[i] >= m || seen[(size_t)order[i]]) return set_err(ctx, CL_ERR_ARGUMENT, "order is not a permutation");
+      seen[(size_t)order[i]] = 1;
+    }
+  }
+  hipSetDevice(ctx->device);
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  if ((rc = grow(ctx, B_NS, (size_t)m * 8)) || (order && (rc = grow(ctx, B_ORDER, (size_t)m * 8))))
+    return set_err(ctx, rc, "device allocation failed");
+  hipError_t e = hipMemcpyAsync(ctx->buf[B_NS], Ns, (size_t)m * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && order) e = hipMemcpyAsync(ctx->buf[B_ORDER], order, (size_t)m * 8, hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return set_err(ctx, map_err(e), "copy of step counts failed");
+  ChainArgs a = chain_args(sys, ch, m, dx, dW, dst, dcode, dhstep, 0, Nmax, Nmax, tau);
+  a.Ns = (const int64_t*)ctx->buf[B_NS];
+  a.order = order ? (const int64_t*)ctx->buf[B_ORDER] : nullptr;
+  rc = chain_launch(a, sys->N, sys->ring != 0, qr, st);
+  if (rc) return set_err(ctx, rc, "chain kernel launch failed");
+  // the step counts live in the context until the kernel has read them
+  e = hipStreamSynchronize(st);
+  return e == hipSuccess ? CL_OK : set_err(ctx, map_err(e), "chain kernel failed");
+}
+
+// The chunk entry point for the chain: host and device buffers share the orbit-major layout.
+static int chain_chunk(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m, int s, int d, int k,
+                       int64_t n0, int64_t steps, int64_t N, int tau, double* x, double* W, void* states,
+                       int8_t* code, int64_t* hstep) {
+  int words = 0;
+  bool qr = false;
+  int rc = validate_chain(ctx, sys, ch, elt, m, s, d, k, tau, &words, &qr);
+  if (rc) return rc;
+  if (m == 0) return CL_OK;
+  if (!x || !W || !code || !hstep || (words > 0 && !states)) return set_err(ctx, CL_ERR_ARGUMENT, "null buffer");
+  hipSetDevice(ctx->device);
+  hipStream_t st = ctx->stream;
+  size_t bx = (size_t)m * s * 8, bw = (size_t)m * d * k * 8, bs = (size_t)m * words * 8;
+  if ((rc = grow(ctx, B_X, bx)) || (rc = grow(ctx, B_W, bw)) || (rc = grow(ctx, B_ST, bs)) ||
+      (rc = grow(ctx, B_CODE, (size_t)m)) || (rc = grow(ctx, B_H, (size_t)m * 8)))
+    return set_err(ctx, rc, "device allocation failed");
+  double *dx = (double*)ctx->buf[B_X], *dW = (double*)ctx->buf[B_W];
+  uint64_t* dst = (uint64_t*)ctx->buf[B_ST];
+  int8_t* dcode = (int8_t*)ctx->buf[B_CODE];
+  int64_t* dh = (int64_t*)ctx->buf[B_H];
+  hipError_t e = hipSuccess;
+#define CK(call)                                      \
+  if ((e = (call)) != hipSuccess) {                   \
+    return set_err(ctx, map_err(e), #call " failed"); \
+  }
+  CK(hipMemcpyAsync(dx, x, bx, hipMemcpyHostToDevice, st));
+  CK(hipMemcpyAsync(dW, W, bw, hipMemcpyHostToDevice, st));
+  if (n0 > 0) {
+    if (words > 0) CK(hipMemcpyAsync(dst, states, bs, hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(dcode, code, (size_t)m, hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(dh, hstep, (size_t)m * 8, hipMemcpyHostToDevice, st));
+  }
+  rc = chain_device(ctx, sys, ch, elt, m, s, d, k, n0, steps, N, tau, dx, dW, dst, dcode, dh, st);
+  if (rc) return rc;
+  CK(hipMemcpyAsync(x, dx, bx, hipMemcpyDeviceToHost, st));
+  CK(hipMemcpyAsync(W, dW, bw, hipMemcpyDeviceToHost, st));
+  if (words > 0) CK(hipMemcpyAsync(states, dst, bs, hipMemcpyDeviceToHost, st));
+  CK(hipMemcpyAsync(code, dcode, (size_t)m, hipMemcpyDeviceToHost, st));
+  CK(hipMemcpyAsync(hstep, dh, (size_t)m * 8, hipMemcpyDeviceToHost, st));
+  CK(hipStreamSynchronize(st));
+#undef CK
+  return CL_OK;
+}
 
 extern "C" int cl_propagate_chunk(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m,
                                   int s, int d, int k, int64_t n0, int64_t steps, int64_t N, int tau, double* x,
@@ -283,6 +443,10 @@ extern "C" int cl_propagate_chunk(cl_ctx* ctx, const cl_system* sys, const cl_ch
                                   uint32_t* ws) {
   (void)ws;
   if (!ctx) return CL_ERR_ARGUMENT;
+  if (sys && ch && sys->kind == CL_SYS_STANDARD_MAP_CHAIN) {
+    if (W2) return set_err(ctx, CL_ERR_UNSUPPORTED, "chain kernels carry one frame");
+    return chain_chunk(ctx, sys, ch, elt, m, s, d, k, n0, steps, N, tau, x, W, states, code, hstep);
+  }
   const Instance* in = nullptr;
   int rc = validate(ctx, sys, ch, elt, m, s, d, k, n0, steps, N, tau, &in);
   if (rc) return rc;
@@ -359,12 +523,25 @@ static void frames_host(uint64_t seed, int64_t orbit0, int64_t m, double* W) {
 
 #define FRAME_SIZES(X) X(2, 1) X(2, 2) X(4, 1) X(4, 2) X(4, 3) X(4, 4)
 
+// Frames of the chain sizes (d > 4): one thread per orbit, built in place in the orbit-major output.
+__global__ void k_frames_dyn(uint64_t seed, int64_t orbit0, int64_t m, int d, int k, double* __restrict__ W) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  seeded_frame_dyn(seed, orbit0 + i, d, k, W + i * (int64_t)d * k);
+}
+
+static bool dyn_frame_ok(int d, int k) { return d > 4 && d <= 256 && k >= 1 && k <= d && k <= 16; }
+
 extern "C" int cl_initial_deviations(uint64_t seed, int64_t orbit0, int64_t m, int d, int k, double* W) {
   if (m < 0 || !W) return CL_ERR_ARGUMENT;
 #define X(D, K) \
   if (d == D && k == K) { frames_host<D, K>(seed, orbit0, m, W); return CL_OK; }
   FRAME_SIZES(X)
 #undef X
+  if (dyn_frame_ok(d, k)) {
+    for (int64_t i = 0; i < m; ++i) seeded_frame_dyn(seed, orbit0 + i, d, k, W + i * (int64_t)d * k);
+    return CL_OK;
+  }
   return CL_ERR_UNSUPPORTED;
 }
 
@@ -381,6 +558,10 @@ extern "C" int cl_initial_deviations_device(cl_ctx* ctx, uint64_t seed, int64_t 
   }
   FRAME_SIZES(X)
 #undef X
+  if (dyn_frame_ok(d, k)) {
+    hipLaunchKernelGGL(k_frames_dyn, dim3(grid_for(m, 64)), dim3(64), 0, st, seed, orbit0, m, d, k, dW);
+    return map_err(hipGetLastError());
+  }
   return set_err(ctx, CL_ERR_UNSUPPORTED, "no frame kernel for this d, k");
 }
 

@@ -150,6 +150,11 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
     words = nbytes // 8
     nf = 2 if reducers2 else 1
     dev = torch.device("cuda")
+    if sysd.kind == L.CL_SYS_STANDARD_MAP_CHAIN:
+        if reducers2:
+            raise L.HIPError(3, "chain kernels carry one frame")
+        return _propagate_orbit_major(lib, ctx, sys, sysd, chain, allr, x0, N, k, tau, seed, W0, launch_steps,
+                                      words, check, keep_states)
     dx = torch.from_numpy(np.ascontiguousarray(x0.T)).to(dev)
     dW = torch.empty((d * k, m), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
@@ -193,6 +198,94 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
         prov.update(r.provenance())
     return PropagationResult(x=x, W=W, finals=finals_of(states, allr, d, k), provenance=prov, W2=W2, code=code,
                              hstep=hstep, states=raw if keep_states else None)
+
+
+def _propagate_orbit_major(lib, ctx, sys, sysd, chain, allr, x0, N, k, tau, seed, W0, launch_steps, words, check,
+                           keep_states, Ns=None, order=None):
+    """The chain system: device buffers in the orbit-major layout (x[i][s], W[i][k][d], states[i][words])."""
+    import torch
+    m, s = x0.shape
+    d = sys.dimension
+    dev = torch.device("cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    dx = torch.from_numpy(x0.copy()).to(dev)
+    dW = torch.empty((m, k, d), dtype=torch.float64, device=dev)
+    if W0 is None:
+        rc = lib.cl_initial_deviations_device(ctx, int(seed) & (2 ** 64 - 1), 1, m, d, k, dW.data_ptr(), stream)
+        if rc:
+            raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
+    else:
+        w = np.asarray(W0, dtype=np.float64).reshape(m, d, k)
+        dW.copy_(torch.from_numpy(np.ascontiguousarray(w.transpose(0, 2, 1))))
+    dst = torch.zeros((m, max(words, 1)), dtype=torch.int64, device=dev)
+    dcode = torch.zeros(m, dtype=torch.int8, device=dev)
+    dh = torch.zeros(m, dtype=torch.int64, device=dev)
+    launches = 0
+    if Ns is not None:
+        Na = np.ascontiguousarray(np.asarray(Ns, dtype=np.int64))
+        oa = None if order is None else np.ascontiguousarray(np.asarray(order, dtype=np.int64))
+        rc = lib.cl_propagate_ragged(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, m, s, d, k,
+                                     Na.ctypes.data, None if oa is None else oa.ctypes.data, tau, dx.data_ptr(),
+                                     dW.data_ptr(), dst.data_ptr(), dcode.data_ptr(), dh.data_ptr(), stream)
+        if rc:
+            raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
+        launches, L_ = 1, int(Na.max()) if m else 0
+    else:
+        L_ = N if launch_steps is None else min(int(launch_steps), N)
+        n0 = 0
+        while n0 < N:
+            steps = min(L_, N - n0)
+            rc = lib.cl_propagate_device(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, m, s, d, k, n0,
+                                         steps, N, tau, dx.data_ptr(), dW.data_ptr(), None, dst.data_ptr(),
+                                         dcode.data_ptr(), dh.data_ptr(), stream)
+            if rc:
+                raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
+            n0 += steps
+            launches += 1
+    torch.cuda.synchronize()
+    x = dx.cpu().numpy()
+    W = dW.cpu().numpy().transpose(0, 2, 1).copy()
+    raw = dst[:, :words].cpu().numpy().copy().view(np.uint8).reshape(m, words * 8)
+    code = dcode.cpu().numpy()
+    hstep = dh.cpu().numpy()
+    states = decode_states(raw, allr, d, k)
+    if check:
+        check_health(code, hstep, x, W, states, allr)
+    prov = dict(backend="gpu", device=device_name(), backend_fallback_reason="", precision="Float64",
+                seed=seed, tau=tau, k=k, chunk_size=m, gpu_chunks=1, gpu_chunk_size=m, launch_steps=L_,
+                launches_per_chunk=launches)
+    for r, _ in allr:
+        prov.update(r.provenance())
+    return PropagationResult(x=x, W=W, finals=finals_of(states, allr, d, k), provenance=prov, code=code,
+                             hstep=hstep, states=raw if keep_states else None)
+
+
+def propagate_ragged(sys, x0, Ns, *, k, tau=1, seed, reducers=(), W0=None, order=None, check=True,
+                     keep_states=False):
+    """Chain ensembles with one step count per orbit (Ns[i], each a positive multiple of tau): every orbit is
+    the same as a `propagate` call of its own length. `order` (a permutation) sets the launch order; by default
+    the longest orbits go first."""
+    tau, k = int(tau), int(k)
+    x0 = np.ascontiguousarray(np.asarray(x0, dtype=np.float64))
+    m, s = x0.shape
+    d = sys.dimension
+    Ns = np.asarray(Ns, dtype=np.int64)
+    if Ns.shape != (m,) or np.any(Ns < 1) or np.any(Ns % tau):
+        raise ValueError("Ns needs one positive multiple of tau per orbit")
+    if s != sys.state_length or not (1 <= k <= d):
+        raise ValueError("state length or k does not match the system")
+    for r in reducers:
+        r.check(d, k)
+    lib, ctx = _context()
+    sysd = sys.descriptor()
+    chain, allr = make_chain(reducers)
+    nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, d, k)
+    if nbytes < 0:
+        raise L.HIPError(-nbytes, "no kernel instance for this system, k and reducer chain")
+    if order is None:
+        order = np.argsort(-Ns, kind="stable")
+    return _propagate_orbit_major(lib, ctx, sys, sysd, chain, allr, x0, 0, k, tau, seed, W0, None, nbytes // 8,
+                                  check, keep_states, Ns=Ns, order=order)
 
 
 def device_name():

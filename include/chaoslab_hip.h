@@ -15,6 +15,9 @@
  *     code   int8[m], hstep int64[m]
  *   Device buffers (cl_propagate_device) are structure-of-arrays, one orbit per lane:
  *     x[c*m + i], W[(col*d + row)*m + i], states as 8-byte words st[w*m + i].
+ *   The coupled chain (CL_SYS_STANDARD_MAP_CHAIN) spreads one orbit over 2N-row frames held by N
+ *   threads; its device buffers keep the orbit-major host layout above (x[i][2N], W[i][k][2N],
+ *   states[i][words]), and so do seeded frames of d > 4.
  *
  * Return codes are the indices of the reference's FALLBACK_REASONS (src/Foundation/Foundation.jl:71-75):
  *   0 ok, 2 no functional device, 3 kernel not built for this system/chain/element type,
@@ -150,12 +153,23 @@ int cl_propagate_device_traced(cl_ctx *ctx, const cl_system *sys, const cl_chain
                                double *dx, double *dW, double *dW2, uint64_t *dstates, int8_t *dcode,
                                int64_t *dhstep, float *dtrace, int64_t trace_n0, int trace_len, void *stream);
 
+/* Full runs of the coupled chain with one step count per orbit (Ns[i], host array, each a positive
+ * multiple of tau) on device buffers in the orbit-major layout, fresh states. `order` (host, optional)
+ * is a permutation giving the launch order of the orbits: longest first balances uneven step counts,
+ * as orbit groups are dispatched to the compute units in order. Synchronous.
+ * No reference counterpart (the reference runs one N per call): the ensemble over
+ * StandardMapChain(N, K, eps; boundary) (src/Systems/standard_map_chain.jl:11-53) with uneven lengths. */
+int cl_propagate_ragged(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain, int elt,
+                        int64_t m, int s, int d, int k, const int64_t *Ns, const int64_t *order, int tau,
+                        double *dx, double *dW, uint64_t *dstates, int8_t *dcode, int64_t *dhstep, void *stream);
+
 /* Seeded orthonormal frames, replacing initial_deviations(seed, orbit, Val(d), Val(k), T)
  * (src/Tangent/initial.jl:27-34): orbit ids orbit0 .. orbit0+m-1 (1-based ids as the reference),
  * key mix(mix(seed) + id) with the reference's splitmix64 finaliser, Gaussian d x k matrix from a
  * counter stream, two modified Gram-Schmidt passes per column and unit columns (R_jj > 0).
  * Column j does not depend on k.
- * Host version: AoS output W[m][k][d]. Device version: SoA output, asynchronous. */
+ * Host version: AoS output W[m][k][d]. Device version: SoA output (orbit-major for d > 4), asynchronous.
+ * Sizes: d in {2, 4} with k <= d, or 4 < d <= 256 with k <= 16. */
 int cl_initial_deviations(uint64_t seed, int64_t orbit0, int64_t m, int d, int k, double *W);
 int cl_initial_deviations_device(cl_ctx *ctx, uint64_t seed, int64_t orbit0, int64_t m, int d, int k,
                                  double *dW, void *stream);

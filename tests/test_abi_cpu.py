@@ -141,3 +141,26 @@ def test_propagate_validates_before_device():
         propagate(sm, x0, 10, k=1, seed=1, reducers=(MEGNO(),))
     with pytest.raises(ValueError, match="state length"):
         propagate(sm, np.zeros((3, 3)), 10, k=2, seed=1)
+
+
+def test_chain_state_bytes_and_frames():
+    """The chain's QR state is QRState{Float64,16} (16 k + 72 bytes); kernels exist for N in {8..128} powers of two
+    and k = 16; host frames of the chain sizes equal the oracle's."""
+    from chaos_amd import StandardMapChain, QRSpectrum, HouseholderQR, MGS
+    from chaos_amd.tangent import make_chain
+    lib = L.load()
+    qr = make_chain((QRSpectrum(HouseholderQR()),))[0]
+    for N in (8, 16, 32, 64, 128):
+        sd = StandardMapChain(N, 1.5, 0.1, "ring").descriptor()
+        assert lib.cl_state_bytes(ctypes.byref(sd), ctypes.byref(qr), 1, 2 * N, 16) == 16 * 16 + 72
+        assert lib.cl_state_bytes(ctypes.byref(sd), ctypes.byref(make_chain(())[0]), 1, 2 * N, 16) == 0
+        assert lib.cl_state_bytes(ctypes.byref(sd), ctypes.byref(qr), 1, 2 * N, 4) == -3
+    sd = StandardMapChain(100, 1.5, 0.1, "ring").descriptor()
+    assert lib.cl_state_bytes(ctypes.byref(sd), ctypes.byref(qr), 1, 200, 16) == -3
+    sd = StandardMapChain(16, 1.5, 0.1, "ring").descriptor()
+    mgs = make_chain((QRSpectrum(MGS()),))[0]
+    assert lib.cl_state_bytes(ctypes.byref(sd), ctypes.byref(mgs), 1, 32, 16) == -3
+    for d in (16, 256):
+        W = np.empty((3, 16, d))
+        assert lib.cl_initial_deviations(9, 1, 3, d, 16, W.ctypes.data) == 0
+        assert np.array_equal(W.transpose(0, 2, 1), O.frames(9, 1, 3, d, 16))

@@ -1,0 +1,119 @@
+"""The coupled standard-map chain kernels on the GPU against the oracle, bit for bit (the oracle's site-butterfly
+reduction order, which the kernels use), for every built N, both boundaries, launch splits, per-orbit step counts
+and the host chunk entry point."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from chaos_amd import StandardMapChain, QRSpectrum, HouseholderQR, propagate, propagate_ragged, _lib as L
+from chaos_amd.tangent import make_chain, _context
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+K = 16
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def starts(seed, m, N):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0, 2 * np.pi, (m, 2 * N))
+
+
+@pytest.mark.parametrize("ring", [True, False], ids=["ring", "open"])
+@pytest.mark.parametrize("N", [8, 16, 32, 64, 128])
+def test_chain_qr_bitwise(N, ring):
+    m = 13 if N <= 64 else 3
+    say("chain qr", N, ring, m)
+    sysm = StandardMapChain(N, 1.5, 0.1, "ring" if ring else "open")
+    x0 = starts(N + ring, m, N)
+    W0 = O.frames(5, 1, m, 2 * N, K)
+    chain, _ = make_chain((QRSpectrum(HouseholderQR()),))
+    g = propagate(sysm, x0, 300, k=K, tau=10, seed=5, reducers=(QRSpectrum(HouseholderQR()),), W0=W0,
+                  keep_states=True)
+    o = O.run_chain(sysm.descriptor(), chain, x0, W0, 300, 10, K, mode=O.RED_SITES)
+    assert np.array_equal(g.x, o["x"])
+    assert np.array_equal(g.W, o["W"])
+    assert np.array_equal(g.states, o["states"])
+    assert np.array_equal(g.code, o["code"]) and np.array_equal(g.hstep, o["hstep"])
+    lam = g.finals[0]["lambda"]
+    assert np.all(lam[:, 0] > 0.05)
+
+
+@pytest.mark.parametrize("N", [8, 128])
+def test_chain_no_reducer_and_launch_split(N):
+    say("chain split", N)
+    sysm = StandardMapChain(N, 1.5, 0.1, "ring")
+    x0 = starts(3, 5, N)
+    W0 = O.frames(3, 1, 5, 2 * N, K)
+    g = propagate(sysm, x0, 200, k=K, tau=10, seed=3, W0=W0)
+    o = O.run_chain(sysm.descriptor(), make_chain(())[0], x0, W0, 200, 10, K)
+    assert np.array_equal(g.x, o["x"]) and np.array_equal(g.W, o["W"])
+    r = (QRSpectrum(HouseholderQR()),)
+    a = propagate(sysm, x0, 200, k=K, tau=10, seed=3, reducers=r, W0=W0, keep_states=True)
+    b = propagate(sysm, x0, 200, k=K, tau=10, seed=3, reducers=r, W0=W0, keep_states=True, launch_steps=70)
+    assert np.array_equal(a.states, b.states) and np.array_equal(a.W, b.W) and np.array_equal(a.x, b.x)
+
+
+@pytest.mark.parametrize("N", [16, 128])
+def test_chain_ragged_bitwise(N):
+    say("chain ragged", N)
+    m = 9
+    sysm = StandardMapChain(N, 1.5, 0.1, "ring")
+    x0 = starts(11, m, N)
+    W0 = O.frames(11, 1, m, 2 * N, K)
+    Ns = 10 * np.random.default_rng(1).integers(3, 40, m)
+    r = (QRSpectrum(HouseholderQR()),)
+    g = propagate_ragged(sysm, x0, Ns, k=K, tau=10, seed=11, reducers=r, W0=W0, keep_states=True)
+    o = O.run_chain(sysm.descriptor(), make_chain(r)[0], x0, W0, 0, 10, K, Ns=Ns)
+    assert np.array_equal(g.x, o["x"]) and np.array_equal(g.W, o["W"])
+    assert np.array_equal(g.states, o["states"]) and np.array_equal(g.hstep, Ns)
+
+
+def test_chain_origin_spectrum():
+    """At the origin the chain's spectrum is +-ln mu_j of its modes (see tests/test_oracle_chain.py)."""
+    from tests.test_oracle_chain import origin_exponents
+    N = 32
+    say("chain origin", N)
+    sysm = StandardMapChain(N, 1.5, 0.1, "ring")
+    g = propagate(sysm, np.zeros((2, 2 * N)), 4000, k=K, tau=10, seed=2, reducers=(QRSpectrum(HouseholderQR()),))
+    assert np.all(g.x == 0)
+    want = origin_exponents(N, 1.5, 0.1, True)[:K]
+    assert np.max(np.abs(g.finals[0]["lambda"] - want)) < 10.0 / 4000
+
+
+def test_chain_host_chunk_and_frames():
+    import torch
+    N, m = 64, 4
+    say("chain chunk", N)
+    lib, ctx = _context()
+    sysm = StandardMapChain(N, 1.5, 0.1, "open")
+    sd = sysm.descriptor()
+    chain, _ = make_chain((QRSpectrum(HouseholderQR()),))
+    d = 2 * N
+    # device frames equal the host rule
+    dW = torch.empty((m, K, d), dtype=torch.float64, device="cuda")
+    assert lib.cl_initial_deviations_device(ctx, 7, 1, m, d, K, dW.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    hW = np.empty((m, K, d))
+    assert lib.cl_initial_deviations(7, 1, m, d, K, hW.ctypes.data) == 0
+    assert np.array_equal(dW.cpu().numpy(), hW)
+    x0 = starts(4, m, N)
+    x = x0.copy()
+    W = hW.copy()
+    words = 2 * K + 9
+    st = np.zeros((m, words), dtype=np.uint64)
+    code = np.zeros(m, dtype=np.int8)
+    h = np.zeros(m, dtype=np.int64)
+    rc = lib.cl_propagate_chunk(ctx, ctypes.byref(sd), ctypes.byref(chain), 1, m, d, d, K, 0, 100, 100, 10,
+                                x.ctypes.data, W.ctypes.data, None, st.ctypes.data, code.ctypes.data, h.ctypes.data,
+                                None)
+    assert rc == 0
+    o = O.run_chain(sd, chain, x0, hW.transpose(0, 2, 1), 100, 10, K)
+    assert np.array_equal(x, o["x"]) and np.array_equal(W.transpose(0, 2, 1), o["W"])
+    assert np.array_equal(st.view(np.uint8).reshape(m, -1), o["states"])
