@@ -8,6 +8,10 @@ cycle through the three K values. Inputs are resident in HBM before the timed re
 
 `--workload c1`: the reference's CPU-runnable case (K = 6, 4096 starts, 10^5 steps, QRSpectrum(MGS())).
 
+`--workload c3`: the coupled standard-map chain, N in {8, 16, 32, 64, 128} sites x 2048 orbits each, ring,
+K = 1.5, eps = 0.1, per-orbit step counts uniform on [2000, 20000] (multiples of tau = 10), k = 16,
+QRSpectrum(HouseholderQR()); one timed step propagates all five ensembles (longest orbits launched first).
+
 `--workload c5`: surrogate triage on the standard map with per-orbit K ~ U[0.5, 6]. The network is trained on
 1M orbits x 10^5 steps (prefix features + full-method labels), checked on 262144 held-out orbits, and then
 4M orbits are labelled twice: by the plain method (QR spectrum to N/10 and N for every orbit) and by the
@@ -175,6 +179,116 @@ def flops_per_orbit_step(wl, cpu):
     return 780.0 * it + 2300.0 + 75.0
 
 
+C3_NS = (8, 16, 32, 64, 128)
+
+
+def c3_flops(N, k=16, tau=10):
+    """Algorithmic FP64 operations per orbit-step of the chain: 6N sin/cos (force and tangent, 20 each), the banded
+    product ~8Nk, and per handoff (/tau) the Householder sweep, the Q formation and orth_loss, 2(2N)k^2 each, and
+    the k x k window inverse ~k^3/3."""
+    return 6 * N * 20 + 8 * N * k + (3 * 2 * (2 * N) * k * k + k ** 3 / 3) / tau
+
+
+def c3_main(a):
+    import torch
+    from chaos_amd import StandardMapChain, QRSpectrum, HouseholderQR, _lib as L
+    from chaos_amd.inputs import chain_starts, chain_step_counts
+    from chaos_amd.tangent import make_chain, _context
+    lib, ctx = _context()
+    m, k, tau = 2048, 16, 10
+    desc = (f"coupled standard-map chain, ring, K=1.5, eps=0.1, N in {list(C3_NS)} x {m} orbits, steps U[2000,20000] "
+            f"per orbit (multiples of tau={tau}), k={k}, QRSpectrum(HouseholderQR()), FP64")
+    log("workload", desc)
+    chain, _ = make_chain((QRSpectrum(HouseholderQR()),))
+    sets = []
+    for j, N in enumerate(C3_NS):
+        sysm = StandardMapChain(N, 1.5, 0.1, "ring")
+        x0 = chain_starts(3 + j, m, N)
+        Ns = chain_step_counts(3 + j, m, 2000, 20000, tau)
+        order = np.argsort(-Ns, kind="stable")
+        dx = torch.from_numpy(x0).cuda()
+        dW = torch.empty((m, k, 2 * N), dtype=torch.float64, device="cuda")
+        dst = torch.zeros((m, 2 * k + 9), dtype=torch.int64, device="cuda")
+        dc = torch.zeros(m, dtype=torch.int8, device="cuda")
+        dh = torch.zeros(m, dtype=torch.int64, device="cuda")
+        sets.append(dict(N=N, sys=sysm, sd=sysm.descriptor(), x0=x0, Ns=Ns, order=order, dx=dx, dW=dW, dst=dst,
+                         dc=dc, dh=dh, xm=dx.clone()))
+    for S in sets:
+        S["stream"] = torch.cuda.Stream()  # the five ensembles run concurrently, one stream each (default priority)
+
+    def run(S, Ns):
+        sh = S["stream"].cuda_stream
+        with torch.cuda.stream(S["stream"]):
+            S["dx"].copy_(S["xm"])
+        assert lib.cl_initial_deviations_device(ctx, 3, 1, m, 2 * S["N"], k, S["dW"].data_ptr(), sh) == 0
+        rc = lib.cl_propagate_ragged(ctx, ctypes.byref(S["sd"]), ctypes.byref(chain), 1, m, 2 * S["N"], 2 * S["N"], k,
+                                     Ns.ctypes.data, S["order"].ctypes.data, tau, S["dx"].data_ptr(),
+                                     S["dW"].data_ptr(), S["dst"].data_ptr(), S["dc"].data_ptr(), S["dh"].data_ptr(),
+                                     sh)
+        assert rc == 0, lib.cl_last_error(ctx)
+
+    for S in sets:  # warmup: 20 steps per orbit
+        run(S, np.full(m, 20, dtype=np.int64))
+    torch.cuda.synchronize()
+    log("warmup done")
+    per_n = {}
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for S in sets:
+            S["stream"].wait_event(ev0)
+        for S in sets:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(S["stream"])
+            run(S, S["Ns"])
+            e1.record(S["stream"])
+            per_n.setdefault(S["N"], []).append((ev0, e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = {N: float(np.mean([x.elapsed_time(z) for _, x, z in v])) for N, v in per_n.items()}
+    # the five launches overlap: the span from the common start event to the last one's end
+    span = float(np.mean([max(v[i][0].elapsed_time(v[i][2]) for v in per_n.values()) for i in range(a.steps)]))
+    steps_n = {S["N"]: float(S["Ns"].sum()) for S in sets}
+    total = sum(steps_n.values())
+    flops = sum(c3_flops(N) * steps_n[N] for N in steps_n)
+    kern_s = span * 1e-3
+    achieved = flops / kern_s / 1e12
+    bad = int(sum(int((S["dc"] != 0).sum().item()) for S in sets))
+    log("per-N ms", ms)
+    cpu = None
+    if not a.no_cpu:
+        from tests import oracle_lib as O
+        nthreads = min(16, os.cpu_count() or 1)
+        tc, done = 0.0, 0.0
+        for j, S in enumerate(sets):
+            sel = np.arange(0, m, 32)
+            W0 = O.frames(3, 1, m, 2 * S["N"], k)[sel]
+            t1 = time.perf_counter()
+            O.run_chain(S["sd"], chain, S["x0"][sel], W0, 0, tau, k, Ns=S["Ns"][sel], math=O.MATH_LIBM,
+                        mode=O.RED_ROWS, nthreads=nthreads)
+            tc += time.perf_counter() - t1
+            done += float(S["Ns"][sel].sum())
+            log(f"cpu baseline N={S['N']}: {len(sel)} orbits done")
+        cpu = dict(value=done / tc, unit="orbit-steps/s", cores=nthreads, kind="port",
+                   sample=f"every 32nd orbit of each N's set at its own step count ({int(done)} orbit-steps), "
+                          f"oracle/oracle_chain.hpp in the reference's row order with glibc sin/cos/log, "
+                          f"{nthreads} threads, {tc:.1f} s")
+    out = {
+        "metric": METRIC, "value": total * a.steps / wall, "unit": "orbit-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": 1, "ms_per_step": wall * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": desc, "orbit_steps_per_step": total, "kernel_ms_per_N": ms, "nonfinite_orbits": bad},
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel_ms": span,
+                     "flops_per_orbit_step": {N: c3_flops(N) for N in C3_NS}},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = out["value"] / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
 def c5_main(a):
     import torch
     from chaos_amd import surrogate as S
@@ -257,7 +371,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "c5"])
     ap.add_argument("--c5-sizes", default="1048576,262144,4194304,100000",
                     help="c5: training orbits, held-out orbits, triaged orbits, steps N")
     ap.add_argument("--c5-epochs", type=int, default=8)
@@ -272,6 +386,8 @@ def main():
         sys.exit(2)
     if a.workload == "c5":
         return c5_main(a)
+    if a.workload == "c3":
+        return c3_main(a)
     wl = workload(a.workload)
     log("workload", wl["desc"])
     if a.copy_probe:

@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchaoslab_hip.so")
-SOURCES = ["chain_kernels.hip", "propagate.hip", "inst_maps.hip", "inst_flow.hip", "mlp.hip"]
+SOURCES = ["chain_n128.hip", "chain_n64.hip", "chain_n32.hip", "chain_n16.hip", "chain_n8.hip", "chain_kernels.hip",
+           "propagate.hip", "inst_maps.hip", "inst_flow.hip", "mlp.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CL_OFFLOAD_ARCH", "gfx950")
 
@@ -36,7 +37,7 @@ def _compile(src):
 
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+    with ThreadPoolExecutor(max_workers=min(int(os.environ.get('MAX_JOBS', '8')), len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]

@@ -1,586 +1,15 @@
-// chain_kernels.hip - the coupled standard-map chain (StandardMapChain, src/Systems/standard_map_chain.jl:1-53)
-// with its banded tangent (ChainTangent, src/Systems/chain_tangent.jl:21-97) and the QR spectrum with
-// Householder reflectors (src/Lyapunov/householder.jl:17-65, src/Lyapunov/qr_spectrum.jl:56-78) on frames of
-// d = 2N rows and k = 16 columns.
-//
-// One thread per chain site: thread t of an orbit holds q_t, p_t and rows t (q part) and N+t (p part) of
-// the frame, 2k doubles. An orbit spans N lanes of a wave (N <= 64; 64/N orbits per 64-lane group) or a
-// 128-thread group (N = 128). Per step each thread evaluates sin/cos of its position and of the bond to its
-// right neighbour; the left bond's values come from the left neighbour, exact because q_j - q_l is exactly
-// -(q_l - q_j) and the portable sin is odd and cos even bit for bit. The banded product needs the
-// neighbours' q rows of the frame (cross-lane moves, LDS for N = 128).
-//
-// Column reductions (norms and dot products over the 2N rows) take a partial per site over its q row (when
-// in range) then its p row, combined by an xor butterfly over the N sites, largest offset first; every
-// thread ends with the same value because IEEE addition commutes. The CPU oracle restates this order
-// (oracle/oracle_chain.hpp, RED_SITES) and is checked against the reference's row order.
-//
-// HBM layout (orbit-major, the reference's host layout): x[i][2N], W[i][k][2N], states[i][2k+9 words].
-// Before each QR the thread stores its frame rows to W (a failed factorisation leaves the frame it was
-// given, qr_spectrum.jl:64-66). Per-orbit step counts and an orbit order (longest first) come from the
-// ragged entry point; groups are dispatched in order, which balances uneven orbits over the CUs.
-#include <hip/hip_runtime.h>
-#include "cl_core.hpp"
+// chain_kernels.hip - dispatch of the coupled standard-map chain kernels (chain_impl.hpp); the instances
+// for each site count are compiled in their own translation units (chain_n*.hip).
 #include "chain_kernels.hpp"
 #include "../../include/chaoslab_hip.h"
 
 namespace cl {
 
-namespace {
-
-// Register-resident frame rows: a vector type, so that a uniform runtime column index (the reflector loops)
-// is an indexed register move rather than a private-memory array.
-template <int K>
-using dvec = double __attribute__((ext_vector_type(K)));
-
-template <int NS>
-struct Geo {
-  static constexpr int BLOCK = NS > 64 ? NS : 64;
-  static constexpr int OPB = BLOCK / NS;
-};
-
-// ---------------------------------------------------------------- cross-lane primitives within one orbit
-template <int NS>
-struct Orb {
-  double* xb;  // LDS exchange buffer (N = 128 only): 128 * 16 doubles
-  int t;       // site
-  int base;    // first lane of this orbit inside its wave (N <= 64)
-
-  __device__ double bcast(double v, int owner) const {
-    if constexpr (NS > 64) {
-      __syncthreads();
-      if (t == owner) xb[0] = v;
-      __syncthreads();
-      return xb[0];
-    } else {
-      return __shfl(v, base + owner, 64);
-    }
-  }
-  __device__ int any(int v) const {
-    if constexpr (NS > 64) {
-      __syncthreads();
-      if (t == 0) reinterpret_cast<int*>(xb)[0] = 0;
-      __syncthreads();
-      if (v) reinterpret_cast<int*>(xb)[0] = 1;
-      __syncthreads();
-      return reinterpret_cast<int*>(xb)[0];
-    } else {
-#pragma unroll
-      for (int o = NS / 2; o >= 1; o >>= 1) v |= __shfl_xor(v, o, 64);
-      return v;
-    }
-  }
-  // butterfly sums of v[lo..hi] (the other entries are left untouched)
-  template <int M>
-  __device__ void sum(double (&v)[M], int lo = 0, int hi = M - 1) const {
-    if constexpr (NS > 64) {
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < M; ++j)
-        if (j >= lo && j <= hi) xb[j * 128 + t] = v[j];
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < M; ++j)
-        if (j >= lo && j <= hi) v[j] = v[j] + xb[j * 128 + (t ^ 64)];
-    }
-    constexpr int O0 = NS > 64 ? 32 : NS / 2;
-#pragma unroll
-    for (int o = O0; o >= 1; o >>= 1)
-#pragma unroll
-      for (int j = 0; j < M; ++j)
-        if (j >= lo && j <= hi) v[j] = v[j] + __shfl_xor(v[j], o, 64);
-  }
-  __device__ double sum1(double v) const {
-    double a[1] = {v};
-    sum<1>(a);
-    return a[0];
-  }
-  __device__ double max1(double v) const {
-    if constexpr (NS > 64) {
-      __syncthreads();
-      xb[t] = v;
-      __syncthreads();
-      v = cl_jmax(v, xb[t ^ 64]);
-    }
-    constexpr int O0 = NS > 64 ? 32 : NS / 2;
-#pragma unroll
-    for (int o = O0; o >= 1; o >>= 1) v = cl_jmax(v, __shfl_xor(v, o, 64));
-    return v;
-  }
-  // values of v at the left and right neighbour sites (ring order inside the orbit)
-  template <int M, class V>
-  __device__ void nbr(const V& v, V& l, V& r) const {
-    if constexpr (NS > 64) {
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < M; ++j) xb[j * 128 + t] = v[j];
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < M; ++j) {
-        l[j] = xb[j * 128 + ((t + NS - 1) & (NS - 1))];
-        r[j] = xb[j * 128 + ((t + 1) & (NS - 1))];
-      }
-    } else {
-      const int sl = base + ((t + NS - 1) & (NS - 1)), sr = base + ((t + 1) & (NS - 1));
-#pragma unroll
-      for (int j = 0; j < M; ++j) {
-        l[j] = __shfl(v[j], sl, 64);
-        r[j] = __shfl(v[j], sr, 64);
-      }
-    }
-  }
-};
-
-// Safe norm (src/Foundation/norm.jl:8-12) of one column over the rows >= j: the largest |entry|, then
-// m sqrt(sum (a/m)^2), the sum per site (q row then p row) and by the butterfly.
-template <int NS>
-__device__ double col_norm(const Orb<NS>& o, double aq, double ap, bool qin, bool pin) {
-  double mx = qin ? cl_fabs(aq) : 0.0;
-  if (pin) mx = qin ? cl_jmax(mx, cl_fabs(ap)) : cl_fabs(ap);
-  mx = o.max1(mx);
-  if (!(cl_isfinite(mx) && mx > 0.0)) return mx;
-  double part = 0.0;
-  if (qin) {
-    double b = aq / mx, c = ap / mx;
-    part = b * b + c * c;
-  } else if (pin) {
-    double c = ap / mx;
-    part = c * c;
-  }
-  return mx * cl_sqrt(o.sum1(part));
-}
-
-struct QRChainState {
-  // sums of the columns this thread owns (c = t, t + N, ...), the rest replicated
-  int64_t n, count_max, column, step;
-  double G_max, orth_max, value, threshold;
-  int8_t code;
-};
-
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// LDS visibility between the threads of one orbit: the whole group for N = 128, the wave otherwise.
-template <int NS>
-__device__ inline void osync() {
-  if constexpr (NS > 64) __syncthreads();
-  else wave_sync();
-}
-
-
-// One handoff of QRSpectrum(HouseholderQR()) (qr_spectrum.jl:56-78) on the orbit's frame (Wq, Wp rows of this
-// thread): reflector sweep (householder.jl:17-44), Q formation (:47-53), sign fix (:55-65), window_Gw
-// (orthonormalize.jl:112-127), orth_loss (qr_spectrum.jl:4) and the log sums. Reflectors live in LDS (Us, one
-// row pair per thread and reflector), the triangle R and its inverse in LDS per orbit (Ro, Rio).
-template <int NS, int KC>
-__device__ __attribute__((always_inline)) inline void qr_handoff(const Orb<NS>& o, const ChainArgs& a, int64_t n,
-                                                                 dvec<KC>& Wq, dvec<KC>& Wp, double* Wo,
-                                                                 CSum (&sums)[(KC + NS - 1) / NS], QRChainState& qs,
-                                                                 double* Us, double* Ro, double* Rio) {
-  constexpr int D = 2 * NS;
-  constexpr int NREF = KC < D ? KC : KC - 1;
-  constexpr int BLOCK = Geo<NS>::BLOCK;
-  constexpr int SPT = (KC + NS - 1) / NS;
-  const int t = o.t, tid = (int)threadIdx.x;
-  // the frame this factorisation starts from: what a failed handoff leaves in place
-#pragma unroll
-  for (int c = 0; c < KC; ++c) {
-    Wo[c * D + t] = Wq[c];
-    Wo[c * D + NS + t] = Wp[c];
-  }
-  int count = 0;
-#pragma unroll 1
-  for (int j = 0; j < NREF; ++j) {
-    const bool qin = t >= j, pin = NS + t >= j;
-    const double aq = Wq[j], ap = Wp[j];
-    const double c = col_norm<NS>(o, aq, ap, qin, pin);
-    double rho, uq = 0.0, up = 0.0;
-    if (cl_isfinite(c) && c > 0.0) {
-      const double ajv = o.bcast(j < NS ? aq : ap, j % NS);
-      const double sg = ajv < 0.0 ? -1.0 : 1.0;
-      double vq = aq, vp = ap;
-      if (j < NS) {
-        if (t == j) vq = ajv + sg * c;
-      } else {
-        if (t == j - NS) vp = ajv + sg * c;
-      }
-      const double nv = col_norm<NS>(o, vq, vp, qin, pin);
-      uq = qin ? vq / nv : 0.0;
-      up = pin ? vp / nv : 0.0;
-      double w[KC];
-#pragma unroll
-      for (int l = 0; l < KC; ++l) w[l] = qin ? uq * Wq[l] + up * Wp[l] : (pin ? up * Wp[l] : 0.0);
-      o.template sum<KC>(w, j + 1, KC - 1);
-#pragma unroll
-      for (int l = 0; l < KC; ++l)
-        if (l > j) {
-          if (qin) Wq[l] = Wq[l] - (2.0 * uq) * w[l];
-          if (pin) Wp[l] = Wp[l] - (2.0 * up) * w[l];
-        }
-      ++count;
-      rho = -sg * c;
-    } else {
-      rho = c;
-    }
-    Us[(2 * j) * BLOCK + tid] = uq;
-    Us[(2 * j + 1) * BLOCK + tid] = up;
-    if (j < NS) {
-      if (t == j) Wq[j] = rho;
-    } else {
-      if (t == j - NS) Wp[j] = rho;
-    }
-  }
-  // the upper triangle of the swept matrix (rows < k), unsigned, column-major with zeros below
-  if (t < KC) {
-#pragma unroll
-    for (int l = 0; l < KC; ++l) Ro[l * KC + t] = l >= t ? Wq[l] : 0.0;
-  }
-  if (NS < KC && NS + t < KC) {
-#pragma unroll
-    for (int l = 0; l < KC; ++l) Ro[l * KC + NS + t] = l >= NS + t ? Wp[l] : 0.0;
-  }
-  osync<NS>();
-  // sign fix and pivot check (householder.jl:55-65); R(r, l) = sgn_r A(r, l)
-  uint32_t neg = 0;
-  int piv = 0;
-#pragma unroll
-  for (int j = 0; j < KC; ++j) {
-    const double dj = Ro[j * KC + j];
-    if (dj < 0.0) neg |= 1u << j;
-    const double pv = (dj < 0.0 ? -1.0 : 1.0) * dj;
-    if (piv == 0 && !(cl_isfinite(pv) && pv > 0.0)) piv = j + 1;
-  }
-  auto sgn = [&](int r) -> double { return (neg >> r) & 1u ? -1.0 : 1.0; };
-  double gw_val = 0.0;
-  if (piv == 0) {
-    double mR = cl_fabs(Ro[0]);
-#pragma unroll 1
-    for (int l = 0; l < KC; ++l)
-      for (int r = 0; r <= l; ++r) mR = cl_jmax(mR, cl_fabs(Ro[l * KC + r]));
-    const bool sc = (mR > 1e100 || mR < 1e-100) && cl_isfinite(mR) && mR > 0.0;
-    auto getR = [&](int r, int l) -> double {
-      const double v = sgn(r) * Ro[l * KC + r];
-      return sc ? v / mR : v;
-    };
-    // R^-1 column cc by its owner thread(s), rows from the bottom up (orthonormalize.jl:116-124)
-#pragma unroll
-    for (int s2 = 0; s2 < SPT; ++s2) {
-      const int cc = t + s2 * NS;
-      if (cc < KC) {
-        double Ri[KC];
-#pragma unroll
-        for (int r = 0; r < KC; ++r) Ri[r] = 0.0;
-#pragma unroll 1
-        for (int r = KC - 1; r >= 0; --r) {
-          double v = 0.0;
-          if (r == cc) {
-            v = 1.0 / getR(r, r);
-          } else if (r < cc) {
-            double acc = 0.0;
-#pragma unroll
-            for (int l = 1; l < KC; ++l)
-              if (l > r && l <= cc) acc += getR(r, l) * Ri[l];
-            v = -acc / getR(r, r);
-          }
-#pragma unroll
-          for (int l = 0; l < KC; ++l)
-            if (l == r) Ri[l] = v;
-        }
-#pragma unroll
-        for (int r = 0; r < KC; ++r) Rio[cc * KC + r] = Ri[r];
-      }
-    }
-    osync<NS>();
-    // Frobenius norms in column-major order; the zeros below the diagonal add nothing
-    double sa, sb;
-    {
-      double v = getR(0, 0);
-      sa = v * v;
-      v = Rio[0];
-      sb = v * v;
-    }
-#pragma unroll 1
-    for (int l = 1; l < KC; ++l)
-      for (int r = 0; r <= l; ++r) {
-        const double u = getR(r, l), ui = Rio[l * KC + r];
-        sa = sa + u * u;
-        sb = sb + ui * ui;
-      }
-    gw_val = cl_sqrt(sa) * cl_sqrt(sb);
-  }
-  // Q = H_1 ... H_nref I[:, 1:k], reflectors in reverse order, into the frame registers
-#pragma unroll
-  for (int l = 0; l < KC; ++l) {
-    Wq[l] = t == l ? 1.0 : 0.0;
-    Wp[l] = NS + t == l ? 1.0 : 0.0;
-  }
-#pragma unroll 1
-  for (int j = NREF - 1; j >= 0; --j) {
-    const bool qin = t >= j, pin = NS + t >= j;
-    const double uq = Us[(2 * j) * BLOCK + tid], up = Us[(2 * j + 1) * BLOCK + tid];
-    double w[KC];
-#pragma unroll
-    for (int l = 0; l < KC; ++l) w[l] = qin ? uq * Wq[l] + up * Wp[l] : (pin ? up * Wp[l] : 0.0);
-    o.template sum<KC>(w, j, KC - 1);
-#pragma unroll
-    for (int l = 0; l < KC; ++l)
-      if (l >= j) {
-        if (qin) Wq[l] = Wq[l] - (2.0 * uq) * w[l];
-        if (pin) Wp[l] = Wp[l] - (2.0 * up) * w[l];
-      }
-  }
-#pragma unroll
-  for (int l = 0; l < KC; ++l) {
-    Wq[l] = Wq[l] * sgn(l);
-    Wp[l] = Wp[l] * sgn(l);
-  }
-  if (piv != 0) {
-    const double pv = sgn(piv - 1) * Ro[(piv - 1) * KC + piv - 1];
-    qs.code = 1;
-    qs.column = piv;
-    qs.value = pv;
-    qs.threshold = cl_isfinite(pv) ? 0.0 : __builtin_inf();
-    qs.step = n;
-  } else {
-    // orth_loss: largest |Q'Q - I| over the entries a <= b (the matrix is symmetric entry for entry)
-    double orth = 0.0;
-#pragma unroll 1
-    for (int b = 0; b < KC; ++b) {
-      const double qb = Wq[b], pb = Wp[b];
-      double v[KC];
-#pragma unroll
-      for (int l = 0; l < KC; ++l) v[l] = cl_fma(Wp[l], pb, Wq[l] * qb);
-      o.template sum<KC>(v, 0, b);
-#pragma unroll
-      for (int l = 0; l < KC; ++l)
-        if (l <= b) {
-          const double er = cl_fabs(l == b ? v[l] - 1.0 : v[l] - 0.0);
-          orth = (b == 0 && l == 0) ? er : cl_jmax(orth, er);
-        }
-    }
-    const double thr = a.tol * CL_EPS64 * (double)KC * 1.0;
-    if (!(orth <= thr)) {
-      qs.code = 4;
-      qs.column = 0;
-      qs.value = orth;
-      qs.threshold = thr;
-      qs.step = n;
-    } else {
-#pragma unroll
-      for (int s2 = 0; s2 < SPT; ++s2) {
-        const int cc = t + s2 * NS;
-        if (cc < KC) sums[s2] = csum_add(sums[s2], cl_log(sgn(cc) * Ro[cc * KC + cc]));
-      }
-      qs.n = n;
-      qs.G_max = cl_jmax(qs.G_max, gw_val);
-      qs.orth_max = cl_jmax(qs.orth_max, orth);
-      if ((int64_t)count > qs.count_max) qs.count_max = count;
-      qs.column = 0;
-    }
-  }
-  osync<NS>();  // Ro / Rio / Us are rewritten at the next handoff
-}
-
-template <int NS, int KC, bool RING, bool HASQR>
-__global__ void __launch_bounds__(Geo<NS>::BLOCK) chain_kernel(ChainArgs a) {
-  constexpr int D = 2 * NS;
-  constexpr int NREF = KC < D ? KC : KC - 1;
-  constexpr int SPT = (KC + NS - 1) / NS;  // QR sums owned per thread
-  constexpr int OPB = Geo<NS>::OPB;
-  constexpr int WORDS = 2 * KC + 9;
-  constexpr int BLOCK = Geo<NS>::BLOCK;
-  __shared__ double xb[NS > 64 ? 128 * KC : 1];
-  __shared__ double Us[HASQR ? 2 * NREF * BLOCK : 1];
-  __shared__ double Rs[HASQR ? OPB * KC * KC : 1];
-  __shared__ double Ris[HASQR ? OPB * KC * KC : 1];
-
-  Orb<NS> o;
-  o.xb = xb;
-  o.t = (int)threadIdx.x % NS;
-  o.base = ((int)threadIdx.x & 63) - ((int)threadIdx.x & 63) % NS;
-  const int t = o.t;
-  const int slot = (int)threadIdx.x / NS;
-  double* Ro = Rs + (HASQR ? slot * KC * KC : 0);
-  double* Rio = Ris + (HASQR ? slot * KC * KC : 0);
-  const int64_t g = (int64_t)blockIdx.x * OPB + slot;
-  const bool exists = g < a.m;
-  const int64_t i = exists ? (a.order ? a.order[g] : g) : 0;
-  bool active = exists && (a.n0 == 0 || a.code[i] == STATUS_OK);
-  const int64_t Ni = exists ? (a.Ns ? a.Ns[i] : a.N) : 0;
-  const int64_t Hi = active ? (a.Ns ? Ni : a.steps) / a.tau : 0;
-  // handoffs of the group: every lane of a 64-lane group (or the 128-thread group) loops the same count
-  int64_t Hmax = Hi;
-  if constexpr (NS <= 64) {
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) {
-      int64_t v = __shfl_xor(Hmax, s, 64);
-      Hmax = v > Hmax ? v : Hmax;
-    }
-  }
-
-  double* xo = a.x + i * D;
-  double* Wo = a.W + i * (int64_t)D * KC;
-  uint64_t* so = a.st + i * (int64_t)WORDS;
-  double q = 0.0, p = 0.0;
-  dvec<KC> Wq, Wp;
-#pragma unroll
-  for (int c = 0; c < KC; ++c) Wq[c] = Wp[c] = 0.0;
-  CSum sums[SPT];
-  QRChainState qs;
-  qs.n = qs.count_max = qs.column = qs.step = 0;
-  qs.G_max = qs.orth_max = qs.value = qs.threshold = 0.0;
-  qs.code = 0;
-#pragma unroll
-  for (int s = 0; s < SPT; ++s) sums[s] = csum_zero();
-  if (active) {
-    q = xo[t];
-    p = xo[NS + t];
-#pragma unroll
-    for (int c = 0; c < KC; ++c) {
-      Wq[c] = Wo[c * D + t];
-      Wp[c] = Wo[c * D + NS + t];
-    }
-    if (HASQR && a.n0 > 0) {
-#pragma unroll
-      for (int s = 0; s < SPT; ++s) {
-        int c = t + s * NS;
-        if (c < KC) sums[s] = CSum{__longlong_as_double(so[2 * c]), __longlong_as_double(so[2 * c + 1])};
-      }
-      qs.n = (int64_t)so[2 * KC];
-      qs.G_max = __longlong_as_double(so[2 * KC + 1]);
-      qs.orth_max = __longlong_as_double(so[2 * KC + 2]);
-      qs.count_max = (int64_t)so[2 * KC + 3];
-      qs.code = (int8_t)(so[2 * KC + 4] & 0xff);
-      qs.column = (int64_t)so[2 * KC + 5];
-      qs.value = __longlong_as_double(so[2 * KC + 6]);
-      qs.threshold = __longlong_as_double(so[2 * KC + 7]);
-      qs.step = (int64_t)so[2 * KC + 8];
-    }
-  }
-  const bool hasL = RING || t > 0, hasR = RING || t < NS - 1;
-  int64_t n = a.n0;
-  int8_t code = STATUS_OK;
-  bool done = !active, qr_failed = false;
-
-  for (int64_t h = 0; h < Hmax; ++h) {
-    const bool run = !done && h < Hi;
-    if (NS > 64 && !run) break;  // the 128-thread group is one orbit: uniform
-    if (run) {
-      for (int s = 0; s < a.tau; ++s) {
-        // ---- tangent at the old point, then the step (orbit.jl:52-53)
-        double qv[1] = {q}, ql[1], qr[1];
-        o.template nbr<1, double[1]>(qv, ql, qr);
-        double sq, cq, sb, cb;
-        cl_sincos(q, &sq, &cq);
-        cl_sincos(q - qr[0], &sb, &cb);  // bond to the right neighbour
-        double bv[2] = {sb, cb}, bl[2], br[2];
-        o.template nbr<2, double[2]>(bv, bl, br);  // bl = the left neighbour's right bond: sin(q - q_l) = -bl[0], cos = bl[1]
-        double gq = a.K * cq;
-        double cL = hasL ? a.eps * bl[1] : 0.0;
-        double cR = hasR ? a.eps * cb : 0.0;
-        double dg = (gq + cL) + cR, gl = -cL, gr = -cR;
-        dvec<KC> wl, wr;
-        o.template nbr<KC, dvec<KC>>(Wq, wl, wr);
-#pragma unroll
-        for (int c = 0; c < KC; ++c) {
-          double gw = dg * Wq[c];
-          if (hasL) gw = gw + gl * wl[c];
-          if (hasR) gw = gw + gr * wr[c];
-          double np = gw + Wp[c];
-          Wq[c] = (Wq[c] + gw) + Wp[c];
-          Wp[c] = np;
-        }
-        double fs = 0.0;
-        if (hasL) fs = fs + (-bl[0]);
-        if (hasR) fs = fs + sb;
-        double pn = p + (a.K * sq + a.eps * fs);
-        q = q + pn;
-        p = pn;
-      }
-      n += a.tau;
-      // ---- status (orbit.jl:1-3,55-56)
-      int badx = !(cl_isfinite(q) && cl_isfinite(p)), badw = 0;
-#pragma unroll
-      for (int c = 0; c < KC; ++c) badw |= !(cl_isfinite(Wq[c]) && cl_isfinite(Wp[c]));
-      badx = o.any(badx);
-      badw = o.any(badw);
-      if (badx || badw) {
-        code = badx ? STATUS_STATE : STATUS_W;
-        done = true;
-      } else if (HASQR && qs.code == 0) {
-        qr_handoff<NS, KC>(o, a, n, Wq, Wp, Wo, sums, qs, Us, Ro, Rio);
-        if (qs.code != 0) {
-          code = STATUS_REDUCER;
-          done = true;
-          qr_failed = true;  // the frame before this handoff's factorisation is already in W
-        }
-      }
-    }
-  }
-  if (!active) return;
-  if (!done) {
-    int badx = !(cl_isfinite(q) && cl_isfinite(p)), badw = 0;
-#pragma unroll
-    for (int c = 0; c < KC; ++c) badw |= !(cl_isfinite(Wq[c]) && cl_isfinite(Wp[c]));
-    badx = o.any(badx);
-    badw = o.any(badw);
-    code = n == Ni ? (badx ? STATUS_STATE : (badw ? STATUS_W : STATUS_OK)) : STATUS_OK;
-  }
-  xo[t] = q;
-  xo[NS + t] = p;
-  if (!qr_failed) {
-#pragma unroll
-    for (int c = 0; c < KC; ++c) {
-      Wo[c * D + t] = Wq[c];
-      Wo[c * D + NS + t] = Wp[c];
-    }
-  }
-  if (HASQR) {
-#pragma unroll
-    for (int s = 0; s < SPT; ++s) {
-      int c = t + s * NS;
-      if (c < KC) {
-        so[2 * c] = (uint64_t)__double_as_longlong(sums[s].s);
-        so[2 * c + 1] = (uint64_t)__double_as_longlong(sums[s].c);
-      }
-    }
-    if (t == 0) {
-      so[2 * KC] = (uint64_t)qs.n;
-      so[2 * KC + 1] = (uint64_t)__double_as_longlong(qs.G_max);
-      so[2 * KC + 2] = (uint64_t)__double_as_longlong(qs.orth_max);
-      so[2 * KC + 3] = (uint64_t)qs.count_max;
-      so[2 * KC + 4] = (uint64_t)(uint8_t)qs.code;
-      so[2 * KC + 5] = (uint64_t)qs.column;
-      so[2 * KC + 6] = (uint64_t)__double_as_longlong(qs.value);
-      so[2 * KC + 7] = (uint64_t)__double_as_longlong(qs.threshold);
-      so[2 * KC + 8] = (uint64_t)qs.step;
-    }
-  }
-  if (t == 0) {
-    a.code[i] = code;
-    a.hstep[i] = n;
-  }
-}
-
-template <int NS, int KC, bool RING, bool HASQR>
-int launch_chain(const ChainArgs& a, hipStream_t st) {
-  constexpr int B = Geo<NS>::BLOCK, OPB = Geo<NS>::OPB;
-  int64_t grid = (a.m + OPB - 1) / OPB;
-  hipLaunchKernelGGL((chain_kernel<NS, KC, RING, HASQR>), dim3((unsigned)grid), dim3(B), 0, st, a);
-  hipError_t e = hipGetLastError();
-  return e == hipSuccess ? 0 : (e == hipErrorOutOfMemory ? CL_ERR_OOM : CL_ERR_DEVICE_LOST);
-}
-
-template <int NS>
-int launch_n(const ChainArgs& a, bool ring, bool qr, hipStream_t st) {
-  if (ring) return qr ? launch_chain<NS, 16, true, true>(a, st) : launch_chain<NS, 16, true, false>(a, st);
-  return qr ? launch_chain<NS, 16, false, true>(a, st) : launch_chain<NS, 16, false, false>(a, st);
-}
-
-}  // namespace
+int launch_chain_n8(const ChainArgs& a, bool ring, bool qr, hipStream_t st);
+int launch_chain_n16(const ChainArgs& a, bool ring, bool qr, hipStream_t st);
+int launch_chain_n32(const ChainArgs& a, bool ring, bool qr, hipStream_t st);
+int launch_chain_n64(const ChainArgs& a, bool ring, bool qr, hipStream_t st);
+int launch_chain_n128(const ChainArgs& a, bool ring, bool qr, hipStream_t st);
 
 int chain_supported(int N, int k) {
   return (N == 8 || N == 16 || N == 32 || N == 64 || N == 128) && k == 16;
@@ -589,11 +18,11 @@ int chain_supported(int N, int k) {
 int chain_launch(const ChainArgs& a, int N, bool ring, bool qr, hipStream_t st) {
   if (a.m == 0) return 0;
   switch (N) {
-    case 8: return launch_n<8>(a, ring, qr, st);
-    case 16: return launch_n<16>(a, ring, qr, st);
-    case 32: return launch_n<32>(a, ring, qr, st);
-    case 64: return launch_n<64>(a, ring, qr, st);
-    case 128: return launch_n<128>(a, ring, qr, st);
+    case 8: return launch_chain_n8(a, ring, qr, st);
+    case 16: return launch_chain_n16(a, ring, qr, st);
+    case 32: return launch_chain_n32(a, ring, qr, st);
+    case 64: return launch_chain_n64(a, ring, qr, st);
+    case 128: return launch_chain_n128(a, ring, qr, st);
   }
   return CL_ERR_UNSUPPORTED;
 }

@@ -1,0 +1,10 @@
+// chain_n128.hip - instances of the chain kernel for N = 128 sites (ring and open, with and without the QR spectrum).
+#include "chain_impl.hpp"
+
+namespace cl {
+
+int launch_chain_n128(const ChainArgs& a, bool ring, bool qr, hipStream_t st) {
+  return chain_detail::launch_n<128>(a, ring, qr, st);
+}
+
+}  // namespace cl

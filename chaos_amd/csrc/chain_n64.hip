@@ -1,0 +1,10 @@
+// chain_n64.hip - instances of the chain kernel for N = 64 sites (ring and open, with and without the QR spectrum).
+#include "chain_impl.hpp"
+
+namespace cl {
+
+int launch_chain_n64(const ChainArgs& a, bool ring, bool qr, hipStream_t st) {
+  return chain_detail::launch_n<64>(a, ring, qr, st);
+}
+
+}  // namespace cl

@@ -350,7 +350,7 @@ extern "C" int cl_propagate_device_traced(cl_ctx* ctx, const cl_system* sys, con
   return CL_OK;
 }
 
-enum { B_X, B_W, B_W2, B_ST, B_CODE, B_H, B_XA, B_WA, B_W2A, B_STA, B_NS, B_ORDER };
+enum { B_X, B_W, B_W2, B_ST, B_CODE, B_H, B_XA, B_WA, B_W2A, B_STA };
 
 extern "C" int cl_propagate_ragged(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m, int s,
                                    int d, int k, const int64_t* Ns, const int64_t* order, int tau, double* dx,
@@ -378,19 +378,25 @@ extern "C" int cl_propagate_ragged(cl_ctx* ctx, const cl_system* sys, const cl_c
   }
   hipSetDevice(ctx->device);
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-  if ((rc = grow(ctx, B_NS, (size_t)m * 8)) || (order && (rc = grow(ctx, B_ORDER, (size_t)m * 8))))
-    return set_err(ctx, rc, "device allocation failed");
-  hipError_t e = hipMemcpyAsync(ctx->buf[B_NS], Ns, (size_t)m * 8, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess && order) e = hipMemcpyAsync(ctx->buf[B_ORDER], order, (size_t)m * 8, hipMemcpyHostToDevice, st);
-  if (e != hipSuccess) return set_err(ctx, map_err(e), "copy of step counts failed");
+  // step counts and order in stream-ordered device memory, freed after the kernel on the same stream, so
+  // calls on different streams can run concurrently
+  int64_t* dbuf = nullptr;
+  const size_t nb = (size_t)m * 8 * (order ? 2 : 1);
+  hipError_t e = hipMallocAsync((void**)&dbuf, nb, st);
+  if (e != hipSuccess) return set_err(ctx, map_err(e), "device allocation failed");
+  e = hipMemcpyAsync(dbuf, Ns, (size_t)m * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && order) e = hipMemcpyAsync(dbuf + m, order, (size_t)m * 8, hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) {
+    hipFreeAsync(dbuf, st);
+    return set_err(ctx, map_err(e), "copy of step counts failed");
+  }
   ChainArgs a = chain_args(sys, ch, m, dx, dW, dst, dcode, dhstep, 0, Nmax, Nmax, tau);
-  a.Ns = (const int64_t*)ctx->buf[B_NS];
-  a.order = order ? (const int64_t*)ctx->buf[B_ORDER] : nullptr;
+  a.Ns = dbuf;
+  a.order = order ? dbuf + m : nullptr;
   rc = chain_launch(a, sys->N, sys->ring != 0, qr, st);
+  e = hipFreeAsync(dbuf, st);
   if (rc) return set_err(ctx, rc, "chain kernel launch failed");
-  // the step counts live in the context until the kernel has read them
-  e = hipStreamSynchronize(st);
-  return e == hipSuccess ? CL_OK : set_err(ctx, map_err(e), "chain kernel failed");
+  return e == hipSuccess ? CL_OK : set_err(ctx, map_err(e), "stream-ordered free failed");
 }
 
 // The chunk entry point for the chain: host and device buffers share the orbit-major layout.

@@ -86,3 +86,17 @@ def hh_section_starts(seed, m, E, section="q2", orbit0=1, max_tries=64):
     if todo.any():
         raise RuntimeError(f"{int(todo.sum())} starts found no admissible point in {max_tries} draws")
     return out
+
+
+def chain_starts(seed, m, N, orbit0=1):
+    """(m, 2N) chain states 2 pi u, u uniform on [0,1)^{2N}: positions and momenta of every site on the torus."""
+    keys = orbit_keys(seed, m, orbit0) ^ IC_SALT
+    return np.stack([TWO_PI * uniforms(keys, c) for c in range(2 * N)], axis=1)
+
+
+def chain_step_counts(seed, m, lo, hi, tau, orbit0=1):
+    """Per-orbit step counts uniform on the multiples of tau in [lo, hi], from draw 2N+... of the orbit's stream
+    (an index past any state draw: 1000)."""
+    keys = orbit_keys(seed, m, orbit0) ^ IC_SALT
+    nlo, nhi = -(-lo // tau), hi // tau
+    return tau * (nlo + np.floor(uniforms(keys, 1000) * (nhi - nlo + 1)).astype(np.int64))

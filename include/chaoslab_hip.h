@@ -156,7 +156,9 @@ int cl_propagate_device_traced(cl_ctx *ctx, const cl_system *sys, const cl_chain
 /* Full runs of the coupled chain with one step count per orbit (Ns[i], host array, each a positive
  * multiple of tau) on device buffers in the orbit-major layout, fresh states. `order` (host, optional)
  * is a permutation giving the launch order of the orbits: longest first balances uneven step counts,
- * as orbit groups are dispatched to the compute units in order. Synchronous.
+ * as orbit groups are dispatched to the compute units in order. Asynchronous on `stream` (NULL: the
+ * context's stream); the host arrays are copied before the call returns, so calls on different streams
+ * run concurrently.
  * No reference counterpart (the reference runs one N per call): the ensemble over
  * StandardMapChain(N, K, eps; boundary) (src/Systems/standard_map_chain.jl:11-53) with uneven lengths. */
 int cl_propagate_ragged(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain, int elt,
