@@ -28,6 +28,8 @@ struct KArgs {
   float* trace;
   int64_t trace_n0;
   int trace_len;
+  // optional orbit mask: orbits with mask[i] == 0 are left untouched
+  const uint8_t* mask;
 };
 
 // Per-orbit context the reducers read: the point, the carried sin/cos, the frames and a cache of
@@ -234,6 +236,7 @@ __global__ void __launch_bounds__(256) orbit_kernel(KArgs a, Sys sys, Chain chai
   constexpr int D = Sys::D;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.m) return;
+  if (a.mask && !a.mask[i]) return;
   if (!(a.n0 == 0 || a.code[i] == STATUS_OK)) return;
   OrbitCtx<Sys, K, NF> c;
   c.sys = sys;

@@ -276,9 +276,10 @@ static int grow(cl_ctx* c, int slot, size_t bytes) {
 static int launch(const Instance* in, const cl_system* sys, const cl_chain* ch, int64_t m, double* dx,
                   double* dW, double* dW2, uint64_t* dst, int8_t* dcode, int64_t* dhstep, int64_t n0,
                   int64_t steps, int64_t N, int tau, hipStream_t st, float* trace = nullptr,
-                  int64_t trace_n0 = 0, int trace_len = 0) {
+                  int64_t trace_n0 = 0, int trace_len = 0, const uint8_t* mask = nullptr) {
   if (m == 0) return CL_OK;
   KArgs a;
+  a.mask = mask;
   a.trace = trace;
   a.trace_n0 = trace_n0;
   a.trace_len = trace_len;
@@ -294,6 +295,118 @@ static int launch(const Instance* in, const cl_system* sys, const cl_chain* ch, 
   a.N = N;
   a.tau = tau;
   return in->launch(sys, ch, a, st);
+}
+
+// ------------------------------------------------------------------------------ split execution
+// A chain with reducers on both frames can run as two single-frame passes (frame-1 reducers relabelled
+// to frame 0): each pass holds fewer registers, so more waves run per SIMD, and the frames never
+// interact (the Jacobian applied to each is the same). The passes reproduce the fused kernel exactly
+// for every orbit that completes in both; an orbit that fails in either pass (a non-finite state or
+// frame, or a failed reducer) is restored from the launch's inputs and run again by the fused kernel,
+// whose failure rule (status of x and both frames first, then every reducer in chain order) is the
+// reference's. CL_SPLIT=0 in the environment disables the split.
+static bool split_chains(const cl_chain* ch, cl_chain* ca, cl_chain* cb) {
+  memset(ca, 0, sizeof(*ca));
+  memset(cb, 0, sizeof(*cb));
+  bool seen1 = false;
+  for (int r = 0; r < ch->n; ++r) {
+    cl_reducer q = ch->r[r];
+    if (q.frame == 0) {
+      if (seen1) return false;  // frame-0 reducers must come first (the chain order is kept)
+      ca->r[ca->n++] = q;
+    } else {
+      seen1 = true;
+      q.frame = 0;
+      cb->r[cb->n++] = q;
+    }
+  }
+  return ca->n > 0 && cb->n > 0;
+}
+
+static bool split_enabled() {
+  const char* e = getenv("CL_SPLIT");
+  return !(e && e[0] == '0');
+}
+
+// Orbits that failed in either pass (and were live at n0) are restored from the inputs and marked for
+// the fused rerun.
+__global__ void k_reconcile(int64_t m, int64_t n0, int S, int DK, int words, const int8_t* __restrict__ code_in,
+                            const int8_t* __restrict__ codeB, const int64_t* __restrict__ h_in,
+                            const double* __restrict__ x_in, const double* __restrict__ W0_in,
+                            const double* __restrict__ W1_in, const uint64_t* __restrict__ st_in, double* x,
+                            double* W0, double* W1, uint64_t* st, int8_t* code, int64_t* hstep, uint8_t* mask) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const bool live = n0 == 0 || code_in[i] == STATUS_OK;
+  const bool rerun = live && (code[i] != STATUS_OK || codeB[i] != STATUS_OK);
+  mask[i] = rerun ? 1 : 0;
+  if (!rerun) return;
+  for (int c = 0; c < S; ++c) x[(int64_t)c * m + i] = x_in[(int64_t)c * m + i];
+  for (int e = 0; e < DK; ++e) {
+    W0[(int64_t)e * m + i] = W0_in[(int64_t)e * m + i];
+    W1[(int64_t)e * m + i] = W1_in[(int64_t)e * m + i];
+  }
+  for (int w = 0; w < words; ++w) st[(int64_t)w * m + i] = st_in[(int64_t)w * m + i];
+  code[i] = code_in[i];
+  hstep[i] = h_in[i];
+}
+
+// Runs a two-frame launch as two passes plus the fused rerun of failed orbits. Returns CL_ERR_UNSUPPORTED
+// (without side effects) when the chain does not split or a sub-chain has no kernel instance.
+static int launch_split(const Instance* in, const cl_system* sys, const cl_chain* ch, int64_t m, int s, int d,
+                        int k, double* dx, double* dW, double* dW2, uint64_t* dst, int8_t* dcode, int64_t* dhstep,
+                        int64_t n0, int64_t steps, int64_t N, int tau, hipStream_t st) {
+  cl_chain ca, cb;
+  if (!split_chains(ch, &ca, &cb)) return CL_ERR_UNSUPPORTED;
+  const Instance* ia = find_instance(sys, &ca, d, k);
+  const Instance* ib = find_instance(sys, &cb, d, k);
+  if (!ia || !ib || ia->nf != 1 || ib->nf != 1 || ia->words + ib->words != in->words) return CL_ERR_UNSUPPORTED;
+  if (m == 0) return CL_OK;
+  const int DK = d * k, words = in->words;
+  const size_t bx = (size_t)m * s * 8, bw = (size_t)m * DK * 8, bs = (size_t)m * words * 8;
+  // inputs (for the rerun), and pass B's private copies of x, code and step
+  size_t off[12], tot = 0;
+  const size_t sz[12] = {bx, bw, bw, bs, (size_t)m, (size_t)m * 8, bx, (size_t)m, (size_t)m * 8, (size_t)m, 0, 0};
+  for (int j = 0; j < 10; ++j) {
+    off[j] = tot;
+    tot += (sz[j] + 255) & ~(size_t)255;
+  }
+  char* base = nullptr;
+  hipError_t e = hipMallocAsync((void**)&base, tot, st);
+  if (e != hipSuccess) return map_err(e);
+  double *x_in = (double*)(base + off[0]), *W0_in = (double*)(base + off[1]), *W1_in = (double*)(base + off[2]);
+  uint64_t* st_in = (uint64_t*)(base + off[3]);
+  int8_t* code_in = (int8_t*)(base + off[4]);
+  int64_t* h_in = (int64_t*)(base + off[5]);
+  double* xB = (double*)(base + off[6]);
+  int8_t* codeB = (int8_t*)(base + off[7]);
+  int64_t* hB = (int64_t*)(base + off[8]);
+  uint8_t* mask = (uint8_t*)(base + off[9]);
+  int rc = CL_OK;
+  if ((e = hipMemcpyAsync(x_in, dx, bx, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(W0_in, dW, bw, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(W1_in, dW2, bw, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+      (n0 > 0 && (e = hipMemcpyAsync(st_in, dst, bs, hipMemcpyDeviceToDevice, st)) != hipSuccess) ||
+      (e = hipMemcpyAsync(code_in, dcode, (size_t)m, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(h_in, dhstep, (size_t)m * 8, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(xB, dx, bx, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(codeB, dcode, (size_t)m, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(hB, dhstep, (size_t)m * 8, hipMemcpyDeviceToDevice, st)) != hipSuccess) {
+    rc = map_err(e);
+  }
+  if (!rc) rc = launch(ia, sys, &ca, m, dx, dW, nullptr, dst, dcode, dhstep, n0, steps, N, tau, st);
+  if (!rc)
+    rc = launch(ib, sys, &cb, m, xB, dW2, nullptr, dst + (size_t)ia->words * m, codeB, hB, n0, steps, N, tau, st);
+  if (!rc) {
+    hipLaunchKernelGGL(k_reconcile, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, m, n0, s, DK,
+                       n0 > 0 ? words : 0, code_in, codeB, h_in, x_in, W0_in, W1_in, st_in, dx, dW, dW2, dst, dcode,
+                       dhstep, mask);
+    e = hipGetLastError();
+    rc = map_err(e);
+  }
+  if (!rc) rc = launch(in, sys, ch, m, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st, nullptr, 0, 0, mask);
+  hipFreeAsync(base, st);
+  return rc;
 }
 
 static int chain_device(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m, int s, int d,
@@ -327,6 +440,10 @@ extern "C" int cl_propagate_device(cl_ctx* ctx, const cl_system* sys, const cl_c
   if (in->nf > 1 && !dW2) return set_err(ctx, CL_ERR_ARGUMENT, "the chain uses a second frame: W2 is required");
   hipSetDevice(ctx->device);
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  if (in->nf > 1 && split_enabled()) {
+    rc = launch_split(in, sys, ch, m, s, d, k, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st);
+    if (rc != CL_ERR_UNSUPPORTED) return rc ? set_err(ctx, rc, "split launch failed") : CL_OK;
+  }
   rc = launch(in, sys, ch, m, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st);
   if (rc) return set_err(ctx, rc, "kernel launch failed");
   return CL_OK;

@@ -251,3 +251,64 @@ def test_henon_heiles_gauss_bitwise(chains):
         W0 = O.frames(21, 1, m, 4, k)
         orc = O.run(sysm.descriptor(), ch, x0, W0, N, 10, k, W2=W0 if red2 else None, math=O.MATH_CL)
         compare(res, orc, allr, 4, k)
+
+
+def test_split_passes_with_failures_equal_fused_and_oracle():
+    """A two-frame chain runs as two single-frame passes; orbits that fail in either pass are rerun by the fused
+    kernel. With failures planted in one frame at a time (a zero column on either frame, overflow on either frame,
+    a non-finite start), the result equals the fused kernel's and the oracle's bit for bit, launch split or not."""
+    import ctypes
+    import os
+    import torch
+    from chaos_amd import _lib as L
+    from chaos_amd.tangent import _context
+    say("split with failures")
+    ind = (MEGNO(replace=False), FLI(replace=False), OFLI(replace=False), SALI())
+    qr = (QRSpectrum(HouseholderQR()),)
+    ch, allr = make_chain(ind, qr)
+    m, N = 6, 60
+    x0 = starts(StandardMap, m, 21)
+    x0[5] = [np.inf, 0.1]
+    W0 = O.frames(21, 1, m, 2, 2).copy()
+    W2 = W0.copy()
+    W0[1][:, 1] = 0.0
+    W2[2][:, 1] = 0.0
+    W2[3] *= 1.5e308
+    W0[4] *= 1.5e308
+    sd = StandardMap(2.0).descriptor()
+    lib, ctx = _context()
+    words = lib.cl_state_bytes(ctypes.byref(sd), ctypes.byref(ch), 1, 2, 2) // 8
+
+    def run(split, L_):
+        os.environ["CL_SPLIT"] = "1" if split else "0"
+        try:
+            dx = torch.from_numpy(np.ascontiguousarray(x0.T)).cuda()
+            dW = torch.from_numpy(np.ascontiguousarray(W0.transpose(2, 1, 0).reshape(4, m))).cuda()
+            dW2 = torch.from_numpy(np.ascontiguousarray(W2.transpose(2, 1, 0).reshape(4, m))).cuda()
+            dst = torch.zeros((words, m), dtype=torch.int64, device="cuda")
+            dc = torch.zeros(m, dtype=torch.int8, device="cuda")
+            dh = torch.zeros(m, dtype=torch.int64, device="cuda")
+            n0 = 0
+            while n0 < N:
+                st = min(L_, N - n0)
+                rc = lib.cl_propagate_device(ctx, ctypes.byref(sd), ctypes.byref(ch), 1, m, 2, 2, 2, n0, st, N, 1,
+                                             dx.data_ptr(), dW.data_ptr(), dW2.data_ptr(), dst.data_ptr(),
+                                             dc.data_ptr(), dh.data_ptr(), None)
+                assert rc == 0
+                n0 += st
+            torch.cuda.synchronize()
+            return dict(x=dx.cpu().numpy().T.copy(), W=dW.cpu().numpy().reshape(2, 2, m).transpose(2, 1, 0),
+                        W2=dW2.cpu().numpy().reshape(2, 2, m).transpose(2, 1, 0),
+                        states=dst.cpu().numpy().T.copy().view(np.uint8).reshape(m, -1), code=dc.cpu().numpy(),
+                        hstep=dh.cpu().numpy())
+        finally:
+            os.environ.pop("CL_SPLIT", None)
+
+    orc = O.run(sd, ch, x0, W0, N, 1, 2, W2=W2, math=O.MATH_CL)
+    assert sorted(set(orc["code"].tolist())) == [0, 1, 2, 3]
+    for split in (True, False):
+        for L_ in (N, 7):
+            g = run(split, L_)
+            for key in ("x", "W", "W2", "states", "code", "hstep"):
+                a, b = np.ascontiguousarray(g[key]), np.ascontiguousarray(orc[key])
+                assert a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8)), (split, L_, key)
