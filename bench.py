@@ -41,9 +41,10 @@ FP64_PEAK_TFLOPS = 78.6
 #      + OFLI on the map displacement (~145) + SALI (~30) + Householder QR spectrum (~150), about 500;
 # c1 = map step and one frame product (~57) + MGS, G_w, orthogonality loss and two logs (~117), about 174.
 FLOPS_PER_ORBIT_STEP = {"c2": 500.0, "c1": 174.0}
-# HBM bytes per orbit-kernel launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch), scaled
-# by what the counters read for a 1 GiB device copy (FETCH x 2.0, WRITE x 1.0); summaries under profiles/.
-TRAFFIC_BYTES_PER_LAUNCH = {"c2": (2 * 41214.06 + 730176.31) * 1024}
+# HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch), scaled by what the
+# counters read for a 1 GiB device copy (FETCH x 2.0, WRITE x 1.0): for c2 the two passes of the split chain
+# (indicator pass 24792.9 / 590963.3 KiB, QR pass 24693.1 / 164881.8 KiB); summaries under profiles/.
+TRAFFIC_BYTES_PER_LAUNCH = {"c2": (2 * (24792.88 + 24693.06) + 590963.25 + 164881.81) * 1024}
 # Mean Gauss fixed-point iterations per step on the c4 workload, measured with the oracle on its sample
 # (used when the CPU leg is skipped).
 C4_ITERATIONS_DEFAULT = 6.3

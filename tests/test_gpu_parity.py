@@ -312,3 +312,25 @@ def test_split_passes_with_failures_equal_fused_and_oracle():
             for key in ("x", "W", "W2", "states", "code", "hstep"):
                 a, b = np.ascontiguousarray(g[key]), np.ascontiguousarray(orc[key])
                 assert a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8)), (split, L_, key)
+
+
+def test_split_and_fused_equal_for_the_gauss_flow():
+    """The Henon-Heiles two-frame chain runs fused by default (its step dominates); forcing the split gives the
+    same bits."""
+    import os
+    from chaos_amd import HenonHeilesGauss, GALI, GivensQR
+    from chaos_amd.inputs import hh_section_starts
+    say("gauss split vs fused")
+    x0 = hh_section_starts(3, 64, 1 / 8)
+    out = []
+    for flag in ("0", "1"):
+        os.environ["CL_SPLIT"] = flag
+        try:
+            out.append(propagate(HenonHeilesGauss(h=0.01, tol=1e-15), x0, 200, k=4, tau=10, seed=3,
+                                 reducers=(MEGNO(replace=False), GALI()), reducers2=(QRSpectrum(GivensQR()),),
+                                 check=False, keep_states=True))
+        finally:
+            os.environ.pop("CL_SPLIT", None)
+    a, b = out
+    assert np.array_equal(a.states, b.states) and np.array_equal(a.W, b.W) and np.array_equal(a.W2, b.W2)
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.code, b.code)
