@@ -82,6 +82,7 @@ def workload(name):
 
 def gpu_run(wl, steps, warmup):
     import torch
+    from chaos_amd import QRSpectrum
     from chaos_amd.tangent import make_chain, _context
     lib, ctx = _context()
     x0 = wl["x0"]
@@ -135,7 +136,17 @@ def gpu_run(wl, steps, warmup):
     log(f"timed {steps} steps in {wall:.3f} s")
     kms = [a.elapsed_time(b) for a, b in kernel_ms]
     bad = int((dcode != 0).sum().item())
-    return dict(wall=wall, kernel_ms=kms, orbit_steps=float(m) * N, m=m, bad=bad)
+    # final QR spectra of a fixed sample of orbits of the last timed launch (for the lambda check)
+    lam = None
+    last = (warmup + steps - 1) % len(wl["systems"])
+    if any(isinstance(r, QRSpectrum) for r in list(wl["red"]) + list(wl["red2"])) and wl["d"] == 2:
+        from chaos_amd.tangent import decode_states, finals_of
+        idx = np.arange(0, m, max(1, m // 64))[:64]
+        raw = dst[:, torch.from_numpy(idx).to(dev)].cpu().numpy().T.copy().view(np.uint8).reshape(len(idx), -1)
+        fin = finals_of(decode_states(raw, allr, d, k), allr, d, k)
+        qi = [j for j, (r, _) in enumerate(allr) if isinstance(r, QRSpectrum)][0]
+        lam = dict(idx=idx, lam=fin[qi]["lambda"], system=wl["systems"][last])
+    return dict(wall=wall, kernel_ms=kms, orbit_steps=float(m) * N, m=m, bad=bad, lam=lam)
 
 
 def cpu_baseline(wl, budget_orbits):
@@ -166,6 +177,29 @@ def cpu_baseline(wl, budget_orbits):
     if st:
         out["gauss_iterations_per_step"] = it / st
     return out
+
+
+def lambda_check(wl, g):
+    """Same-orbit check of the GPU's FP64 Lyapunov spectra on a sample of the last launch's orbits: the oracle
+    recomputes the Jacobians, frame products and QR spectrum in __float128 along the same FP64 orbit (the AC4
+    method, test/tangent-core/tc_ac01_04.jl:66-88) and the largest |lambda_gpu - lambda_quad| is reported."""
+    from chaos_amd import QRSpectrum
+    from chaos_amd.tangent import make_chain
+    from tests import oracle_lib as O
+    lam = g.get("lam")
+    if lam is None:
+        return None
+    qr = [r for r in list(wl["red"]) + list(wl["red2"]) if isinstance(r, QRSpectrum)][0]
+    chain, _ = make_chain((qr,))
+    idx = lam["idx"]
+    x0 = wl["x0"][idx]
+    W0 = O.frames(wl["seed"], 1, len(wl["x0"]), 2, 2)[idx]
+    t0 = time.perf_counter()
+    lq = O.same_orbit_quad(lam["system"].K, chain, x0, W0, wl["N"], wl["tau"], 2)
+    d = np.abs(lam["lam"] - lq)
+    return dict(mode="same orbit: FP64 orbit, __float128 Jacobians, frame products and QR (oracle)",
+                orbits=int(len(idx)), system=repr(lam["system"]), max_abs=float(d.max()),
+                median_abs=float(np.median(d)), seconds=round(time.perf_counter() - t0, 1))
 
 
 def flops_per_orbit_step(wl, cpu):
@@ -404,6 +438,8 @@ def main():
     avg_ms = float(np.mean(g["kernel_ms"]))
     budget = {"c2": a.cpu_orbits, "c1": 512, "c4": 2048}[wl["name"]]
     cpu = None if a.no_cpu else cpu_baseline(wl, budget)
+    if cpu is not None:
+        cpu["lambda_check"] = lambda_check(wl, g)
     fpos = flops_per_orbit_step(wl, cpu)
     achieved = fpos * g["orbit_steps"] / (avg_ms * 1e-3) / 1e12
     out = {

@@ -1295,6 +1295,49 @@ int or_propagate_chain(const cl_system* sys, const cl_chain* ch, int64_t m, int 
   return chain_all<F64Libm>(sys, ch, m, k, Ns, N, tau, x, W, (uint8_t*)states, code, hstep, mode, nthreads);
 }
 
+// Same-orbit check of the standard map's QR spectrum (the AC4 method, test/tangent-core/tc_ac01_04.jl:66-88):
+// the orbit steps in FP64 exactly as the kernels step it (portable sin), while the Jacobian at each FP64
+// point, the frame products and QRSpectrum (chain ch: one QR reducer) run in __float128. Writes
+// lambda[i][j] = total(sums_j) / n. x0 [m][2], W0 [m][k][2] (column-major frames).
+int or_same_orbit_quad(double K, const cl_chain* ch, int64_t m, int k, int64_t N, int tau, const double* x0,
+                       const double* W0, double* lambda, int nthreads) {
+  if (ch->n != 1 || ch->r[0].kind != CL_RED_QR || k < 1 || k > 2 || N % tau) return 10;
+  std::atomic<int64_t> next(0);
+  auto worker = [&]() {
+    while (true) {
+      int64_t i = next.fetch_add(1);
+      if (i >= m) break;
+      double xd = x0[2 * i], pd = x0[2 * i + 1];
+      quad W[4], O[4];
+      for (int e = 0; e < 2 * k; ++e) W[e] = (quad)W0[i * 2 * k + e];
+      QRSt<quad> st;
+      memset(&st, 0, sizeof(st));
+      for (int s2 = 0; s2 < 8; ++s2) st.sums[s2].s = st.sums[s2].c = 0;
+      st.G_max = st.orth_max = st.value = st.threshold = 0;
+      for (int64_t n = 1; n <= N; ++n) {
+        quad c = (quad)K * cosq((quad)xd);
+        quad a11 = (quad)1 + c;
+        for (int j = 0; j < k; ++j) {
+          quad w0 = W[2 * j], w1 = W[2 * j + 1];
+          O[2 * j] = fmaq((quad)1, w1, a11 * w0);
+          O[2 * j + 1] = fmaq((quad)1, w1, c * w0);
+        }
+        for (int e = 0; e < 2 * k; ++e) W[e] = O[e];
+        double p2 = pd + K * cl_sin(xd);
+        xd = xd + p2;
+        pd = p2;
+        if (n % tau == 0) red_qr<Quad>(ch->r[0], st, n, W, 2, k);
+      }
+      for (int j = 0; j < k; ++j)
+        lambda[i * k + j] = st.n ? dbl((st.sums[j].s + st.sums[j].c) / (quad)st.n) : 0.0;
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 0; t < (nthreads > 1 ? nthreads : 1); ++t) th.emplace_back(worker);
+  for (auto& t : th) t.join();
+  return 0;
+}
+
 // Gauss fixed-point iterations and steps summed over the runs since the last call; resets both.
 void or_gauss_iterations(int64_t* iters, int64_t* steps) {
   *iters = g_gauss_iters.exchange(0);

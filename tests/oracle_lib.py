@@ -36,6 +36,8 @@ def load():
         o.or_propagate_chain.restype = I
         o.or_propagate_chain.argtypes = [ctypes.POINTER(L.cl_system), ctypes.POINTER(L.cl_chain), I64, I, P, I64, I,
                                          P, P, P, P, P, I, I, I]
+        o.or_same_orbit_quad.restype = I
+        o.or_same_orbit_quad.argtypes = [ctypes.c_double, ctypes.POINTER(L.cl_chain), I64, I, I64, I, P, P, P, I]
         o.or_gauss_iterations.restype = None
         o.or_gauss_iterations.argtypes = [P, P]
         o.or_csum.restype = None
@@ -95,6 +97,18 @@ def run_chain(sysd, chain, x0, W0, N, tau, k, Ns=None, math=MATH_CL, mode=RED_SI
                               _ptr(states), _ptr(code), _ptr(hstep), math, mode, nt)
     assert rc == 0, rc
     return dict(x=x, W=W.transpose(0, 2, 1).copy(), states=states[:, :nb], code=code, hstep=hstep)
+
+
+def same_orbit_quad(K, chain, x0, W0, N, tau, k, nthreads=None):
+    """lambda (m, k) of the standard map's QR spectrum with the FP64 orbit and __float128 tangents and QR."""
+    o = load()
+    x = np.ascontiguousarray(np.asarray(x0, dtype=np.float64))
+    m = x.shape[0]
+    W = np.array(np.asarray(W0, dtype=np.float64).reshape(m, 2, k).transpose(0, 2, 1), order="C", copy=True)
+    lam = np.zeros((m, k))
+    nt = nthreads or min(16, os.cpu_count() or 1)
+    assert o.or_same_orbit_quad(float(K), ctypes.byref(chain), m, k, int(N), tau, _ptr(x), _ptr(W), _ptr(lam), nt) == 0
+    return lam
 
 
 def gauss_iterations():
