@@ -5,9 +5,9 @@ Python host side of that ABI, mirroring the reference's `propagate` API and redu
 """
 from .systems import StandardMap, StandardMapK, StandardMapChain, WrappedStandardMap, HenonHeilesGauss, torus_state, hh_energy
 from .reducers import (QRSpectrum, CGS, MGS, RGS, HouseholderQR, AugmentedQR, GivensQR, Benettin, MEGNO, SALI, FLI,
-                       OFLI, GALI)
+                       OFLI, GALI, ResidualMonitor)
 from .tangent import propagate, propagate_ragged, PropagationResult, NumericalHealthError, initial_deviations
 
 __all__ = ["StandardMap", "StandardMapK", "StandardMapChain", "WrappedStandardMap", "HenonHeilesGauss", "hh_energy", "torus_state", "QRSpectrum", "CGS", "MGS", "RGS", "HouseholderQR",
-           "AugmentedQR", "GivensQR", "Benettin", "MEGNO", "SALI", "FLI", "OFLI", "GALI", "propagate", "propagate_ragged",
+           "AugmentedQR", "GivensQR", "Benettin", "MEGNO", "SALI", "FLI", "OFLI", "GALI", "ResidualMonitor", "propagate", "propagate_ragged",
            "PropagationResult", "NumericalHealthError", "initial_deviations"]

@@ -14,6 +14,7 @@ CL_ELT_F64, CL_ELT_F64_F32TAN, CL_ELT_DD = 1, 2, 3
 CL_SYS_STANDARD_MAP, CL_SYS_WRAPPED_STANDARD_MAP, CL_SYS_STANDARD_MAP_CHAIN, CL_SYS_HENON_HEILES_GAUSS = 1, 2, 3, 4
 CL_SYS_STANDARD_MAP_K = 5
 CL_RED_QR, CL_RED_BENETTIN, CL_RED_MEGNO, CL_RED_SALI, CL_RED_FLI, CL_RED_OFLI, CL_RED_GALI = 1, 2, 3, 4, 5, 6, 7
+CL_RED_MONITOR = 9
 CL_QR_CGS, CL_QR_MGS, CL_QR_RGS, CL_QR_HOUSEHOLDER, CL_QR_AUGMENTED, CL_QR_GIVENS = 1, 2, 3, 4, 5, 6
 CL_MAX_REDUCERS = 8
 
@@ -33,7 +34,7 @@ class cl_system(ctypes.Structure):
 
 class cl_reducer(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("alg", ctypes.c_int32), ("replace", ctypes.c_int32),
-                ("frame", ctypes.c_int32), ("tol", ctypes.c_double), ("pad_", ctypes.c_double)]
+                ("frame", ctypes.c_int32), ("tol", ctypes.c_double), ("tol2", ctypes.c_double)]
 
 
 class cl_chain(ctypes.Structure):

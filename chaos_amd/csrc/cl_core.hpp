@@ -126,6 +126,14 @@ struct StdMap {
       W.at(1, j) = cl_fma(1.0, w1, c * w0);
     }
   }
+  // the one-step tangent at the current point, column-major (the ResidualMonitor reads it)
+  CL_HD void jacobian(const Aux& a, double* J) const {
+    double c = K * a.cs;
+    J[0] = 1.0 + c;
+    J[1] = c;
+    J[2] = 1.0;
+    J[3] = 1.0;
+  }
   CL_HD void forward(const Aux& a, double* x) const {
     double p = x[1] + K * a.sn;
     x[0] = x[0] + p;
@@ -164,6 +172,14 @@ struct StdMapK {
       W.at(1, j) = cl_fma(1.0, w1, c * w0);
     }
   }
+  // the one-step tangent at the current point, column-major (the ResidualMonitor reads it)
+  CL_HD void jacobian(const Aux& a, double* J) const {
+    double c = a.K * a.cs;
+    J[0] = 1.0 + c;
+    J[1] = c;
+    J[2] = 1.0;
+    J[3] = 1.0;
+  }
   CL_HD void forward(const Aux& a, double* x) const {
     double p = x[1] + a.K * a.sn;
     x[0] = x[0] + p;
@@ -195,6 +211,14 @@ struct WrappedStdMap {
       W.at(0, j) = cl_fma(1.0, w1, m11 * w0);
       W.at(1, j) = cl_fma(1.0, w1, c * w0);
     }
+  }
+  // the one-step tangent at the current point, column-major (the ResidualMonitor reads it)
+  CL_HD void jacobian(const Aux& a, double* J) const {
+    double c = K * a.cs;
+    J[0] = 1.0 + c;
+    J[1] = c;
+    J[2] = 1.0;
+    J[3] = 1.0;
   }
   static CL_HD void reduce(double u, double tp, double* r, double* a) {
     double q = cl_floor(u / tp);
@@ -1024,6 +1048,100 @@ struct GALI {
       }
     }
     st.n = n;
+  }
+};
+
+// ------------------------------------------------------------------------------ ResidualMonitor
+// ResidualMonitor(sys, det_tol, orth_tol) (src/Diagnostics/monitor.jl:44-54): the det and symplecticity
+// residuals of the one-step tangent at the current point, |det M - 1| and max |M' Omega M - Omega|
+// (src/Systems/residuals.jl:19-33), against det_tol eps (1 + max|M|) and det_tol eps N^2 (1 + max|M|)^2, and for
+// k >= 2 the frame's orthogonality residual against orth_tol (src/Diagnostics/residuals.jl:10,25-33); running
+// maxima. det is the StaticArrays 2 x 2 rule A[1]A[4] - A[3]A[2] for d = 2 and an expansion along the first row
+// for d = 4; the matrix products sum left to right.
+CL_HD double det3(double a00, double a01, double a02, double a10, double a11, double a12, double a20, double a21,
+                  double a22) {
+  return (a00 * (a11 * a22 - a12 * a21) - a01 * (a10 * a22 - a12 * a20)) + a02 * (a10 * a21 - a11 * a20);
+}
+template <int D>
+CL_HD double det_of(const double* J) {
+  if constexpr (D == 2) {
+    return J[0] * J[3] - J[2] * J[1];
+  } else {
+#define M_(r, c) J[(c) * 4 + (r)]
+    double t0 = M_(0, 0) * det3(M_(1, 1), M_(1, 2), M_(1, 3), M_(2, 1), M_(2, 2), M_(2, 3), M_(3, 1), M_(3, 2), M_(3, 3));
+    double t1 = M_(0, 1) * det3(M_(1, 0), M_(1, 2), M_(1, 3), M_(2, 0), M_(2, 2), M_(2, 3), M_(3, 0), M_(3, 2), M_(3, 3));
+    double t2 = M_(0, 2) * det3(M_(1, 0), M_(1, 1), M_(1, 3), M_(2, 0), M_(2, 1), M_(2, 3), M_(3, 0), M_(3, 1), M_(3, 3));
+    double t3 = M_(0, 3) * det3(M_(1, 0), M_(1, 1), M_(1, 2), M_(2, 0), M_(2, 1), M_(2, 2), M_(3, 0), M_(3, 1), M_(3, 2));
+#undef M_
+    return ((t0 - t1) + t2) - t3;
+  }
+}
+template <int D>
+CL_HD double omega_entry(int r, int c) {
+  constexpr int H = D / 2;
+  return (r < H && c == r + H) ? 1.0 : ((r >= H && c == r - H) ? -1.0 : 0.0);
+}
+template <int D>
+CL_HD double symp_residual(const double* J) {
+  double A[D * D];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      double v = J[i * D + 0] * omega_entry<D>(0, j);
+#pragma unroll
+      for (int l = 1; l < D; ++l) v = v + J[i * D + l] * omega_entry<D>(l, j);
+      A[i * D + j] = v;
+    }
+  double mx = 0.0;
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      double v = A[i * D + 0] * J[j * D + 0];
+#pragma unroll
+      for (int l = 1; l < D; ++l) v = v + A[i * D + l] * J[j * D + l];
+      double e = cl_fabs(v - omega_entry<D>(i, j));
+      mx = (i == 0 && j == 0) ? e : cl_jmax(mx, e);
+    }
+  return mx;
+}
+
+template <int D, int K>
+struct Monitor {
+  typedef MonitorState State;
+  double det_tol, orth_tol;
+  CL_HD void init(State& st) const {
+    st.det_max = st.symp_max = st.orth_max = st.sum_max = st.pair_max = st.G_max = st.M_max = 0.0;
+    st.code = 0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) st.pad_[i] = 0;
+    st.value = st.threshold = 0.0;
+    st.step = 0;
+  }
+  CL_HD void step(State& st, int64_t n, const double* J, const Frame<D, K>& W) const {
+    if (st.code != 0) return;
+    double mx = cl_fabs(J[0]);
+#pragma unroll
+    for (int e = 1; e < D * D; ++e) mx = cl_jmax(mx, cl_fabs(J[e]));
+    const double dv = cl_fabs(det_of<D>(J) - 1.0);
+    const double sv = symp_residual<D>(J);
+    const double ov = K >= 2 ? orth_loss<D, K>(W) : 0.0;
+    const double om = 1.0 + mx;
+    const double thr_d = (det_tol * CL_EPS64) * om;
+    const double thr_s = ((det_tol * CL_EPS64) * (double)((D / 2) * (D / 2))) * (om * om);
+    const double thr_o = orth_tol;
+    const bool bd = !(cl_isfinite(dv) && dv <= thr_d), bs = !(cl_isfinite(sv) && sv <= thr_s),
+               bo = !(cl_isfinite(ov) && ov <= thr_o);
+    const int code = bd ? 1 : (bs ? 2 : (bo ? 3 : 0));
+    st.det_max = cl_jmax(st.det_max, dv);
+    st.symp_max = cl_jmax(st.symp_max, sv);
+    st.orth_max = cl_jmax(st.orth_max, ov);
+    st.M_max = cl_jmax(st.M_max, mx);
+    st.code = (int8_t)code;
+    st.value = code == 1 ? dv : (code == 2 ? sv : (code == 3 ? ov : 0.0));
+    st.threshold = code == 1 ? thr_d : (code == 2 ? thr_s : (code == 3 ? thr_o : 0.0));
+    st.step = n;
   }
 };
 

@@ -275,6 +275,14 @@ struct HHGauss {
       }
     W = O;
   }
+  // the one-step tangent at the current point, column-major (NaN when the stage solve failed)
+  CL_HD void jacobian(const Aux& a, double* J) const {
+    step_jacobian(a, J);
+    if (!a.ok) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) J[e] = __builtin_nan("");
+    }
+  }
   CL_HD void forward(const Aux& a, double* z) const {
 #pragma unroll
     for (int c = 0; c < 4; ++c) z[c] = a.ok ? z[c] + a.inc[c] : __builtin_nan("");

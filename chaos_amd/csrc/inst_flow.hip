@@ -25,6 +25,8 @@ typedef Cons<S_<GALI<4, 2>>, Nil> F_gali2;
 typedef Cons<S_<QRSpectrum<QR_GIVENS, 4, 2>>, Nil> F_giv2;
 typedef Cons<S_<MEGNO<4>>, Cons<S_<FLI<4>>, Cons<S_<OFLI<4, 2>>, Cons<S_<SALI<4>>, Nil>>>> F_mfos2;
 typedef Cons<S_<Benettin<4>>, Nil> F_ben1;
+typedef Cons<S_<Monitor<4, 4>>, Nil> F_mon4;
+typedef Cons<S_<QRSpectrum<QR_GIVENS, 4, 4>>, Cons<S_<Monitor<4, 4>>, Nil>> F_giv_mon4;
 typedef Cons<S_<OFLI<4, 1>>, Nil> F_ofli1;
 
 #define ADDF(CH, K, NF) if (*n < cap) out[(*n)++] = make_instance<CL_SYS_HENON_HEILES_GAUSS, HHGauss, K, NF, CH>();
@@ -32,6 +34,7 @@ typedef Cons<S_<OFLI<4, 1>>, Nil> F_ofli1;
 void register_instances_flow(Instance* out, int* n, int cap) {
   ADDF(F_none, 4, 1) ADDF(F_giv4, 4, 1) ADDF(F_hh4, 4, 1) ADDF(F_mgs4, 4, 1) ADDF(F_gali4, 4, 1)
   ADDF(F_megno4, 4, 1) ADDF(F_ofli4, 4, 1) ADDF(F_megno_gali4, 4, 1) ADDF(F_c4, 4, 2)
+  ADDF(F_mon4, 4, 1) ADDF(F_giv_mon4, 4, 1)
   ADDF(F_none, 2, 1) ADDF(F_megno2, 2, 1) ADDF(F_sali2, 2, 1) ADDF(F_fli2, 2, 1) ADDF(F_gali2, 2, 1)
   ADDF(F_giv2, 2, 1) ADDF(F_mfos2, 2, 1)
   ADDF(F_none, 1, 1) ADDF(F_ben1, 1, 1) ADDF(F_ofli1, 1, 1)

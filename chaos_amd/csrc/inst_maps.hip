@@ -38,6 +38,10 @@ typedef Cons<QR1(QR_HOUSEHOLDER), Nil> C1_hh;
 typedef Cons<QR1(QR_AUGMENTED), Nil> C1_aug;
 typedef Cons<QR1(QR_GIVENS), Nil> C1_giv;
 typedef Cons<S_<OFLI<2, 1>>, Nil> C1_ofli;
+// ResidualMonitor alone and after the QR spectrum (it then checks the orthonormalised frame)
+typedef Cons<S_<Monitor<2, 2>>, Nil> C_mon;
+typedef Cons<QR2(QR_MGS, 0), Cons<S_<Monitor<2, 2>>, Nil>> C_mgs_mon;
+typedef Cons<S_<Monitor<2, 1>>, Nil> C1_mon;
 
 #define ADD2(SK, SYS, CH, NF) if (*n < cap) out[(*n)++] = make_instance<SK, SYS, 2, NF, CH>();
 #define ADD1(SK, SYS, CH) if (*n < cap) out[(*n)++] = make_instance<SK, SYS, 1, 1, CH>();
@@ -47,10 +51,10 @@ typedef Cons<S_<OFLI<2, 1>>, Nil> C1_ofli;
   ADD2(SK, SYS, C_sali, 1) ADD2(SK, SYS, C_fli, 1) ADD2(SK, SYS, C_ofli, 1) ADD2(SK, SYS, C_gali, 1) \
   ADD2(SK, SYS, C_megno_sali, 1) ADD2(SK, SYS, C_fli_sali, 1) ADD2(SK, SYS, C_mfs, 1)           \
   ADD2(SK, SYS, C_mfos, 1) ADD2(SK, SYS, C_mfos_hh, 2) ADD2(SK, SYS, C_mfos_mgs, 2)            \
-  ADD2(SK, SYS, C_mfs_hh, 2)                                                                   \
+  ADD2(SK, SYS, C_mfs_hh, 2) ADD2(SK, SYS, C_mon, 1) ADD2(SK, SYS, C_mgs_mon, 1)             \
   ADD1(SK, SYS, C_none) ADD1(SK, SYS, C_ben) ADD1(SK, SYS, C1_cgs) ADD1(SK, SYS, C1_mgs)        \
   ADD1(SK, SYS, C1_rgs) ADD1(SK, SYS, C1_hh) ADD1(SK, SYS, C1_aug) ADD1(SK, SYS, C1_giv)        \
-  ADD1(SK, SYS, C1_ofli)
+  ADD1(SK, SYS, C1_ofli) ADD1(SK, SYS, C1_mon)
 
 // the surrogate workload on the per-orbit-K map: the traced prefix (MEGNO + FLI on frame 1, the QR spectrum
 // on frame 2), the continuation and the plain method (the QR spectrum alone)

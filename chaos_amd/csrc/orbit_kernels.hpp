@@ -119,6 +119,13 @@ CL_HD void slot_step(Slot<GALI<D, K>, FR>& s, int64_t n, Ctx& c) {
   c.logs_valid[FR] = false;
 }
 
+template <int D, int K, int FR, class Ctx>
+CL_HD void slot_step(Slot<Monitor<D, K>, FR>& s, int64_t n, Ctx& c) {
+  double J[D * D];
+  c.sys.jacobian(c.aux, J);
+  s.r.step(s.st, n, J, c.W[FR]);
+}
+
 // Reducer parameters from the descriptor.
 template <int ALG, int D, int K>
 CL_HD void slot_setup(QRSpectrum<ALG, D, K>& r, const cl_reducer& d) { r.tol = d.tol; }
@@ -134,6 +141,11 @@ template <int D, int K>
 CL_HD void slot_setup(OFLI<D, K>& r, const cl_reducer& d) { r.replace = d.replace != 0; }
 template <int D, int K>
 CL_HD void slot_setup(GALI<D, K>&, const cl_reducer&) {}
+template <int D, int K>
+CL_HD void slot_setup(Monitor<D, K>& r, const cl_reducer& d) {
+  r.det_tol = d.tol;
+  r.orth_tol = d.tol2;
+}
 
 // ------------------------------------------------------------------------------ chain recursion
 CL_HD void chain_setup(Nil&, const cl_chain&, int) {}

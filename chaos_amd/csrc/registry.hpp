@@ -91,6 +91,10 @@ template <int D, int K>
 struct Kind<GALI<D, K>> {
   static constexpr int kind = CL_RED_GALI, alg = 0;
 };
+template <int D, int K>
+struct Kind<Monitor<D, K>> {
+  static constexpr int kind = CL_RED_MONITOR, alg = 0;
+};
 
 inline void fill_key(Instance&, const Nil&, int) {}
 template <class H, class T>

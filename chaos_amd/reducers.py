@@ -293,3 +293,39 @@ class GALI(Reducer):
 
     def provenance(self):
         return dict(indicator="gali")
+
+
+class ResidualMonitor(Reducer):
+    """ResidualMonitor(sys, det_tol, orth_tol) (src/Diagnostics/monitor.jl:9-54): at each handoff the det and
+    symplecticity residuals of the one-step tangent at the current point (det_tol times eps, scaled by
+    1 + max|M| and N^2 (1 + max|M|)^2, src/Diagnostics/residuals.jl:25-33) and, for k >= 2, the orthogonality
+    residual of the frame against orth_tol; running maxima. The system is the one being propagated."""
+    kind = L.CL_RED_MONITOR
+    NAMES = ("det", "symplecticity", "orthogonality")
+
+    def __init__(self, sys, det_tol, orth_tol):
+        self.sys, self.det_tol, self.orth_tol = sys, float(det_tol), float(orth_tol)
+
+    def descriptor(self, frame):
+        r = super().descriptor(frame)
+        r.tol = self.det_tol
+        r.tol2 = self.orth_tol
+        return r
+
+    def dtype(self, d, k):
+        return np.dtype([("det_max", "<f8"), ("symp_max", "<f8"), ("orth_max", "<f8"), ("sum_max", "<f8"),
+                         ("pair_max", "<f8"), ("G_max", "<f8"), ("M_max", "<f8"), ("code", "i1"), ("pad", "u1", 7),
+                         ("value", "<f8"), ("threshold", "<f8"), ("step", "<i8")])
+
+    def final(self, st):
+        return dict(det_max=st["det_max"].copy(), symp_max=st["symp_max"].copy(), orth_max=st["orth_max"].copy(),
+                    sum_max=st["sum_max"].copy(), pair_max=st["pair_max"].copy(), M_max=st["M_max"].copy(),
+                    G_max=st["G_max"].copy())
+
+    def error(self, row):
+        v = float(row["value"])
+        thr = float(row["threshold"]) if np.isfinite(v) else float("inf")
+        return self.NAMES[int(row["code"]) - 1], v, thr, int(row["step"]), 0
+
+    def provenance(self):
+        return dict(monitor_det_tol=self.det_tol, monitor_orth_tol=self.orth_tol)

@@ -143,6 +143,9 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
         r.check(d, k)
     lib, ctx = _context()
     sysd = sys.descriptor()
+    for r in list(reducers) + list(reducers2):
+        if getattr(r, "sys", None) is not None and bytes(r.sys.descriptor()) != bytes(sysd):
+            raise ValueError(f"{type(r).__name__} monitors {r.sys!r}, not the propagated {sys!r}")
     chain, allr = make_chain(reducers, reducers2)
     nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, d, k)
     if nbytes < 0:

@@ -76,6 +76,7 @@ typedef struct {
 #define CL_RED_FLI 5       /* FLI(; replace), src/Indicators/fli.jl:292-362 */
 #define CL_RED_OFLI 6      /* OFLI(flow; replace), src/Indicators/ofli.jl:23-126 */
 #define CL_RED_GALI 7      /* GALI(), src/Indicators/gali.jl:190-374 */
+#define CL_RED_MONITOR 9   /* ResidualMonitor(sys, det_tol, orth_tol), src/Diagnostics/monitor.jl:9-54 */
 
 #define CL_QR_CGS 1
 #define CL_QR_MGS 2
@@ -90,8 +91,8 @@ typedef struct {
   int32_t replace;  /* MEGNO / FLI / OFLI chain-position flag */
   int32_t frame;    /* 0: the frame W; 1: the second frame W2 (a second tangent frame on the same
                        orbit, so a replacing indicator chain and a QR spectrum can share one pass) */
-  double tol;       /* QRSpectrum orthogonality_tol */
-  double pad_;
+  double tol;       /* QRSpectrum orthogonality_tol; ResidualMonitor det_tol */
+  double tol2;      /* ResidualMonitor orth_tol */
 } cl_reducer;
 
 #define CL_MAX_REDUCERS 8

@@ -14,6 +14,8 @@
 //   OFLI                  src/Indicators/ofli.jl:30-124 (direction field: the flow's, or the map displacement)
 //   GALI                  src/Indicators/gali.jl:193-372
 //   LogNorm (test reducer) test/tangent-core/tc_helpers.jl:56-70
+//   ResidualMonitor       src/Diagnostics/monitor.jl:44-54; residuals src/Diagnostics/residuals.jl:4-36,
+//                         src/Systems/residuals.jl:19-33
 //   StandardMapChain, ChainTangent and the Householder QR spectrum on d = 2N rows: oracle_chain.hpp
 //
 // Independence from the product: the reducer chain is interpreted at run time over byte buffers
@@ -162,6 +164,7 @@ static int red_bytes(const cl_reducer& r, int d, int k) {
     case CL_RED_OFLI: return 16 * k + 120;
     case CL_RED_GALI: return 24 * k + 32;
     case 8: return 16 * k;  // LogNorm
+    case CL_RED_MONITOR: return 88;
   }
   return -1;
 }
@@ -225,7 +228,15 @@ struct LogNormSt {
 };
 
 template <class T>
+struct MonSt {
+  T det_max, symp_max, orth_max, sum_max, pair_max, G_max, M_max;
+  int8_t code;
+  T value, threshold;
+  int64_t step;
+};
+template <class T>
 struct RState {  // one of the above, by kind
+  MonSt<T> mon;
   QRSt<T> qr;
   BenSt<T> ben;
   MegSt<T> meg;
@@ -244,6 +255,22 @@ template <class T>
 static void write_state(const cl_reducer& r, const RState<T>& s, int d, int k, uint8_t* out) {
   Buf b{out, 0};
   switch (r.kind) {
+    case CL_RED_MONITOR: {
+      const MonSt<T>& q = s.mon;
+      b.put(dbl(q.det_max));
+      b.put(dbl(q.symp_max));
+      b.put(dbl(q.orth_max));
+      b.put(dbl(q.sum_max));
+      b.put(dbl(q.pair_max));
+      b.put(dbl(q.G_max));
+      b.put(dbl(q.M_max));
+      b.put(q.code);
+      b.pad(7);
+      b.put(dbl(q.value));
+      b.put(dbl(q.threshold));
+      b.put(q.step);
+      break;
+    }
     case CL_RED_QR: {
       const QRSt<T>& q = s.qr;
       for (int i = 0; i < k; ++i) {
@@ -868,6 +895,7 @@ static void red_lognorm(LogNormSt<T>& st, T* W, int d, int k) {
 template <class T>
 static bool red_failed(const cl_reducer& r, const RState<T>& s) {
   switch (r.kind) {
+    case CL_RED_MONITOR: return s.mon.code != 0;
     case CL_RED_QR: return s.qr.code != 0;
     case CL_RED_BENETTIN: return s.ben.code != 0;
     case CL_RED_MEGNO: return s.meg.code != 0;
@@ -951,6 +979,15 @@ struct MapSys {
       }
     }
   }
+  // the one-step tangent at x, column-major: [[1 + K cos x, 1], [K cos x, 1]] (standard_map.jl:30-34)
+  void jac(const T* x, T* J) const {
+    const T Kx = kind == CL_SYS_STANDARD_MAP_K ? x[2] : K;
+    T c = Kx * P::cos(x[0]);
+    J[0] = (T)1 + c;
+    J[1] = c;
+    J[2] = (T)1;
+    J[3] = (T)1;
+  }
   // map displacement field at x (OFLI direction on a map)
   void field(const T* x, T* f) const {
     T ks = (kind == CL_SYS_STANDARD_MAP_K ? x[2] : K) * P::sin(x[0]);
@@ -1002,7 +1039,9 @@ struct HHSys {
       }
     return s + c;
   }
-  void step(T* x, T* const* W, int nf, int k) const {
+  // The stage solve of one step at x: the increment and the one-step Jacobian J (row-major), NaN when the
+  // solve failed.
+  void solve(const T* x, T (&J)[4][4], T (&inc)[4], bool& ok_out) const {
     T f0[4], Y[3][4], F[3][4];
     vfield(x, f0);
     for (int i = 0; i < 3; ++i) {
@@ -1045,7 +1084,7 @@ struct HHSys {
       bound = (T)8 * P::eps() * jmax<P>((T)1, mx);
     }
     bool ok = stopped && P::finite(delta) && delta <= bound;
-    T inc[4];
+    ok_out = ok;
     for (int c = 0; c < 4; ++c) {
       T Fc[3] = {F[0][c], F[1][c], F[2][c]};
       inc[c] = h * cdot(OR_GB_HI, OR_GB_LO, Fc);
@@ -1101,7 +1140,6 @@ struct HHSys {
         for (int t = r + 1; t < 6; ++t) v = v - B[r][t] * Q[t][c];
         Q[r][c] = v / B[r][r];
       }
-    T J[4][4];
     for (int r = 0; r < 4; ++r)
       for (int c = 0; c < 4; ++c) J[r][c] = r == c ? (T)1 : (T)0;
     for (int i = 0; i < 3; ++i) {
@@ -1122,6 +1160,19 @@ struct HHSys {
     if (!ok)
       for (int r = 0; r < 4; ++r)
         for (int c = 0; c < 4; ++c) J[r][c] = (T)NAN;
+  }
+  // The one-step tangent at x, column-major (the ResidualMonitor reads it at each handoff).
+  void jac(const T* x, T* Jc) const {
+    T J[4][4], inc[4];
+    bool ok;
+    solve(x, J, inc, ok);
+    for (int c = 0; c < 4; ++c)
+      for (int r = 0; r < 4; ++r) Jc[c * 4 + r] = J[r][c];
+  }
+  void step(T* x, T* const* W, int nf, int k) const {
+    T J[4][4], inc[4];
+    bool ok;
+    solve(x, J, inc, ok);
     for (int f = 0; f < nf; ++f) {
       T O[16];
       for (int j = 0; j < k; ++j)
@@ -1136,6 +1187,71 @@ struct HHSys {
   }
   void field(const T* x, T* f) const { vfield(x, f); }
 };
+
+// ------------------------------------------------------------------ ResidualMonitor (src/Diagnostics/monitor.jl:44-54)
+// det of a column-major d x d tangent (d = 2: A[1]A[4] - A[3]A[2], the StaticArrays 2 x 2 rule; d = 4: expansion
+// along the first row with 3 x 3 minors, left to right)
+template <class T>
+static T det3(T a00, T a01, T a02, T a10, T a11, T a12, T a20, T a21, T a22) {
+  return (a00 * (a11 * a22 - a12 * a21) - a01 * (a10 * a22 - a12 * a20)) + a02 * (a10 * a21 - a11 * a20);
+}
+template <class T>
+static T detn(const T* J, int d) {
+  if (d == 2) return J[0] * J[3] - J[2] * J[1];
+  auto M = [&](int r, int c) { return J[c * 4 + r]; };
+  T t0 = M(0, 0) * det3(M(1, 1), M(1, 2), M(1, 3), M(2, 1), M(2, 2), M(2, 3), M(3, 1), M(3, 2), M(3, 3));
+  T t1 = M(0, 1) * det3(M(1, 0), M(1, 2), M(1, 3), M(2, 0), M(2, 2), M(2, 3), M(3, 0), M(3, 2), M(3, 3));
+  T t2 = M(0, 2) * det3(M(1, 0), M(1, 1), M(1, 3), M(2, 0), M(2, 1), M(2, 3), M(3, 0), M(3, 1), M(3, 3));
+  T t3 = M(0, 3) * det3(M(1, 0), M(1, 1), M(1, 2), M(2, 0), M(2, 1), M(2, 2), M(3, 0), M(3, 1), M(3, 2));
+  return ((t0 - t1) + t2) - t3;
+}
+// max |M' Omega M - Omega|, Omega = [0 I; -I 0]; products with sums left to right (symplectic_residual)
+template <class P, class T>
+static T symp_res(const T* J, int d) {
+  const int h = d / 2;
+  auto Om = [&](int r, int c) -> T { return (r < h && c == r + h) ? (T)1 : ((r >= h && c == r - h) ? (T)-1 : (T)0); };
+  T A[16], mx = 0;
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) {
+      T v = J[i * d + 0] * Om(0, j);
+      for (int l = 1; l < d; ++l) v = v + J[i * d + l] * Om(l, j);
+      A[i * d + j] = v;  // (M' Omega)[i][j], with M'[i][l] = M[l][i] = J[i * d + l]
+    }
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) {
+      T v = A[i * d + 0] * J[j * d + 0];
+      for (int l = 1; l < d; ++l) v = v + A[i * d + l] * J[j * d + l];
+      T e = P::abs(v - Om(i, j));
+      mx = (i == 0 && j == 0) ? e : jmax<P>(mx, e);
+    }
+  return mx;
+}
+template <class P, class T, class SYS>
+static void red_monitor(const cl_reducer& r, MonSt<T>& st, int64_t n, const SYS& sys, const T* x, const T* W, int d,
+                        int k) {
+  if (st.code) return;
+  T J[16];
+  sys.jac(x, J);
+  T mx = P::abs(J[0]);
+  for (int e = 1; e < d * d; ++e) mx = jmax<P>(mx, P::abs(J[e]));
+  const T dv = P::abs(detn(J, d) - (T)1);
+  const T sv = symp_res<P>(J, d);
+  const T ov = k >= 2 ? orth_loss<P>(W, d, k) : (T)0;
+  const T one_m = (T)1 + mx;
+  const T thr_d = ((T)r.tol * P::eps()) * one_m;
+  const T thr_s = (((T)r.tol * P::eps()) * (T)((d / 2) * (d / 2))) * (one_m * one_m);
+  const T thr_o = (T)r.tol2;
+  auto bad = [](T v, T t) { return !(P::finite(v) && v <= t); };
+  int code = bad(dv, thr_d) ? 1 : (bad(sv, thr_s) ? 2 : (bad(ov, thr_o) ? 3 : 0));
+  st.det_max = jmax<P>(st.det_max, dv);
+  st.symp_max = jmax<P>(st.symp_max, sv);
+  st.orth_max = jmax<P>(st.orth_max, ov);
+  st.M_max = jmax<P>(st.M_max, mx);
+  st.code = (int8_t)code;
+  st.value = code == 1 ? dv : (code == 2 ? sv : (code == 3 ? ov : (T)0));
+  st.threshold = code == 1 ? thr_d : (code == 2 ? thr_s : (code == 3 ? thr_o : (T)0));
+  st.step = n;
+}
 
 // ------------------------------------------------------------------ the per-orbit body
 template <class P, class T, class SYS>
@@ -1194,6 +1310,7 @@ static int8_t run_orbit(const SYS& sys, const cl_chain& ch, int d, int k, int64_
         }
         case CL_RED_GALI: red_gali<P>(st[r].gali, n, w, d, k); break;
         case 8: red_lognorm<P>(st[r].ln, w, d, k); break;
+        case CL_RED_MONITOR: red_monitor<P>(rd, st[r].mon, n, sys, x, w, d, k); break;
       }
     }
     for (int r = 0; r < ch.n; ++r)

@@ -11,7 +11,8 @@ import numpy as np
 import pytest
 
 from chaos_amd import (StandardMap, WrappedStandardMap, QRSpectrum, CGS, MGS, RGS, HouseholderQR, AugmentedQR,
-                       GivensQR, Benettin, MEGNO, SALI, FLI, OFLI, GALI, propagate, NumericalHealthError, torus_state)
+                       GivensQR, Benettin, MEGNO, SALI, FLI, OFLI, GALI, ResidualMonitor, propagate, NumericalHealthError,
+                       torus_state)
 from chaos_amd import _lib as L
 from chaos_amd.inputs import torus_uniform
 from chaos_amd.tangent import make_chain, decode_states, finals_of, _context
@@ -334,3 +335,45 @@ def test_split_and_fused_equal_for_the_gauss_flow():
     a, b = out
     assert np.array_equal(a.states, b.states) and np.array_equal(a.W, b.W) and np.array_equal(a.W2, b.W2)
     assert np.array_equal(a.x, b.x) and np.array_equal(a.code, b.code)
+
+
+
+MON_MAP_CASES = [(2, False, lambda K: (64.0, 1e-12)), (2, True, lambda K: (64.0, 1.75e-14 if K > 1 else 1.3e-15)),
+                 (2, True, lambda K: (0.3, 1.0)), (1, False, lambda K: (64.0, 0.0))]
+
+
+@pytest.mark.parametrize("k,qr,tols", MON_MAP_CASES, ids=["mon-k2", "mgs+mon-orth", "mgs+mon-det", "mon-k1"])
+def test_residual_monitor_bitwise_maps(k, qr, tols):
+    """ResidualMonitor on the standard map: the residuals of the one-step tangent, the frame orthogonality and the
+    first exceedance equal the oracle's bit for bit. The tolerances put the first orthogonality (about half the
+    orbits) or det (K = 6, every orbit) exceedance at varied steps; alone on a k = 2 frame the monitor trips at the
+    first handoff, and with k = 1 the frame overflows at K = 6 (status 2)."""
+    m, N = 384, 1200
+    for K in (0.97, 6.0):
+        s = StandardMap(K)
+        dt, ot = tols(K)
+        chain = ((QRSpectrum(MGS()),) if qr else ()) + (ResidualMonitor(s, dt, ot),)
+        say("monitor", cname(chain), "K", K, "k", k, "tols", dt, ot, "m", m, "N", N)
+        x0 = starts(StandardMap, m, 23)
+        res = propagate(s, x0, N, k=k, tau=1, seed=7, reducers=chain, check=False, keep_states=True)
+        ch, allr = make_chain(chain)
+        orc = O.run(s.descriptor(), ch, x0, O.frames(7, 1, m, 2, k), N, 1, k, math=O.MATH_CL)
+        compare(res, orc, allr, 2, k)
+
+
+def test_residual_monitor_bitwise_gauss():
+    """ResidualMonitor on Henon-Heiles under the Gauss method, alone and after a Givens QR spectrum: the 4x4 step
+    Jacobian's determinant and symplectic residuals and the first exceedance (orthogonality or det for part of the
+    orbits at these tolerances) equal the oracle's bit for bit."""
+    from chaos_amd import HenonHeilesGauss
+    from chaos_amd.inputs import hh_section_starts
+    m, N = 256, 400
+    hh = HenonHeilesGauss(h=0.01, tol=1e-15)
+    x0 = hh_section_starts(5, m, 1 / 8)
+    for qr, dt, ot in ((False, 64.0, 1e-12), (True, 64.0, 8.8e-16), (True, 1.0, 1.0)):
+        chain = ((QRSpectrum(GivensQR()),) if qr else ()) + (ResidualMonitor(hh, dt, ot),)
+        say("monitor gauss", cname(chain), "tols", dt, ot, "m", m, "N", N)
+        res = propagate(hh, x0, N, k=4, tau=10, seed=21, reducers=chain, check=False, keep_states=True)
+        ch, allr = make_chain(chain)
+        orc = O.run(hh.descriptor(), ch, x0, O.frames(21, 1, m, 4, 4), N, 10, 4, math=O.MATH_CL)
+        compare(res, orc, allr, 4, 4)
