@@ -329,3 +329,86 @@ class ResidualMonitor(Reducer):
 
     def provenance(self):
         return dict(monitor_det_tol=self.det_tol, monitor_orth_tol=self.orth_tol)
+
+
+# ---------------------------------------------------------------- residual definitions and thresholds
+# (src/Diagnostics/residuals.jl:4-44), host-side for the end-of-run checks of a monitor's state.
+
+def det_threshold(tol, M):
+    """tol * eps * (1 + max|M|) (residuals.jl:29)."""
+    return (float(tol) * EPS64) * (1.0 + float(np.max(np.abs(M))))
+
+
+def symp_threshold(tol, M):
+    """tol * eps * N^2 * (1 + max|M|)^2 with 2N = size(M, 1) (residuals.jl:32-35)."""
+    n = np.shape(M)[0] // 2
+    a = 1.0 + float(np.max(np.abs(M)))
+    return ((float(tol) * EPS64) * float(n * n)) * (a * a)
+
+
+def sum_threshold(tol, G_max, tau, M_max):
+    """tol * eps * (G_max / tau + M_max), independent of the run length (residuals.jl:38); vectorised."""
+    return (float(tol) * EPS64) * (np.asarray(G_max, dtype=np.float64) / tau + np.asarray(M_max, dtype=np.float64))
+
+
+def eps_sum(lam):
+    """|sum_i lambda_i| over the last axis, summed left to right (residuals.jl:13)."""
+    lam = np.asarray(lam, dtype=np.float64)
+    s = lam[..., 0].copy()
+    for i in range(1, lam.shape[-1]):
+        s = s + lam[..., i]
+    return np.abs(s)
+
+
+def eps_pair(lam):
+    """max_i |lambda_i + lambda_{M+1-i}| in estimator order, no sorting (residuals.jl:16-18)."""
+    lam = np.asarray(lam, dtype=np.float64)
+    M = lam.shape[-1]
+    out = np.abs(lam[..., 0] + lam[..., M - 1])
+    for i in range(1, M // 2):
+        out = np.maximum(out, np.abs(lam[..., i] + lam[..., M - 1 - i]))
+    return out
+
+
+def check_residual(what, value, threshold, *, step, orbit, backend="cpu"):
+    """Raise NumericalHealthError when `value` is not finite or exceeds `threshold`, reporting an infinite
+    threshold for a non-finite value (residuals.jl:46-50)."""
+    from .tangent import NumericalHealthError
+    value, threshold = float(value), float(threshold)
+    if np.isfinite(value) and value <= threshold:
+        return None
+    raise NumericalHealthError(what, value, threshold if np.isfinite(value) else float("inf"), int(step), int(orbit),
+                               backend)
+
+
+def monitor_exponents(st, lam, G, n, *, pairing_tol, tau, M_max, step, orbit0=1):
+    """Feed each orbit's exponents `lam` (m, 2N, estimator order) and window growth `G` (m,) to its monitor
+    state `st` (the decoded ResidualMonitor states, one row per orbit) and return the new states
+    (src/Diagnostics/monitor.jl:66-87): a recorded det, symplecticity or orthogonality failure is raised first;
+    then G_max and M_max take the running maxima, the sum residual is checked against sum_threshold
+    (raising for the first orbit above it) and sum_max and pair_max take the running maxima. `n` is the run
+    length the exponents are quoted at (not used by the checks)."""
+    from .tangent import NumericalHealthError
+    st = np.array(st, copy=True)
+    lam = np.asarray(lam, dtype=np.float64)
+    if lam.ndim == 1:
+        lam = lam[None, :]
+    m = len(st)
+    bad = np.flatnonzero(st["code"] != 0)
+    if len(bad):
+        i = int(bad[0])
+        v = float(st["value"][i])
+        raise NumericalHealthError(ResidualMonitor.NAMES[int(st["code"][i]) - 1], v,
+                                   float(st["threshold"][i]) if np.isfinite(v) else float("inf"), int(st["step"][i]),
+                                   orbit0 + i, "cpu")
+    G_max = np.maximum(st["G_max"], np.broadcast_to(np.asarray(G, dtype=np.float64), (m,)))
+    M_hi = np.maximum(st["M_max"], np.broadcast_to(np.asarray(M_max, dtype=np.float64), (m,)))
+    s = eps_sum(lam)
+    thr = sum_threshold(pairing_tol, G_max, tau, M_hi)
+    for i in range(m):
+        check_residual("sum", s[i], thr[i], step=step, orbit=orbit0 + i)
+    st["sum_max"] = np.maximum(st["sum_max"], s)
+    st["pair_max"] = np.maximum(st["pair_max"], eps_pair(lam))
+    st["G_max"] = G_max
+    st["M_max"] = M_hi
+    return st

@@ -981,6 +981,10 @@ struct MapSys {
   }
   // the one-step tangent at x, column-major: [[1 + K cos x, 1], [K cos x, 1]] (standard_map.jl:30-34)
   void jac(const T* x, T* J) const {
+    if (kind == OR_SYS_LINEAR) {
+      for (int i = 0; i < 4; ++i) J[i] = A[i];
+      return;
+    }
     const T Kx = kind == CL_SYS_STANDARD_MAP_K ? x[2] : K;
     T c = Kx * P::cos(x[0]);
     J[0] = (T)1 + c;
