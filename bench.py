@@ -172,7 +172,8 @@ def cpu_baseline(wl, budget_orbits):
     it, st = O.gauss_iterations()
     ns = len(wl["systems"])
     out = dict(value=total / t, unit="orbit-steps/s", cores=nthreads, kind="port",
-               sample=f"every {stride}th orbit ({m} orbits x {wl['N']} steps x {ns} system(s)), "
+               sample=(f"every {stride}th orbit" if stride > 1 else "every orbit") +
+                      f" ({m} orbits x {wl['N']} steps x {ns} system(s)), "
                       f"oracle/oracle.cpp with glibc sin/cos/log, {nthreads} threads, {t:.1f} s")
     if st:
         out["gauss_iterations_per_step"] = it / st
@@ -436,7 +437,7 @@ def main():
     g = gpu_run(wl, a.steps, a.warmup)
     value = g["orbit_steps"] * a.steps / g["wall"]
     avg_ms = float(np.mean(g["kernel_ms"]))
-    budget = {"c2": a.cpu_orbits, "c1": 512, "c4": 2048}[wl["name"]]
+    budget = {"c2": a.cpu_orbits, "c1": 4096, "c4": 2048}[wl["name"]]
     cpu = None if a.no_cpu else cpu_baseline(wl, budget)
     if cpu is not None:
         cpu["lambda_check"] = lambda_check(wl, g)

@@ -103,6 +103,19 @@ CL_HD bool frame_finite(const Frame<D, K>& W) {
 }
 
 // ------------------------------------------------------------------------------ systems
+// W <- M W for M = [[1 + c, 1], [c, 1]], the one-step tangent of the standard-map family, in the
+// fused_product order (product.jl:9-20): C[i,j] = M[i,1] W[1,j], then fma(M[i,2], W[2,j], acc).
+template <int K2>
+CL_HD void map_tangent_apply(double c, Frame<2, K2>& W) {
+  double m11 = 1.0 + c;
+#pragma unroll
+  for (int j = 0; j < K2; ++j) {
+    double w0 = W.at(0, j), w1 = W.at(1, j);
+    W.at(0, j) = cl_fma(1.0, w1, m11 * w0);
+    W.at(1, j) = cl_fma(1.0, w1, c * w0);
+  }
+}
+
 // Standard map p' = p + K sin x, x' = x + p', Jacobian [[1 + c, 1], [c, 1]], c = K cos x.
 // The kernel carries sin and cos of the current point, so one reduction serves the Jacobian,
 // the step and a map displacement field read at the handoff.
@@ -114,18 +127,11 @@ struct StdMap {
     double sn, cs;
   };
   CL_HD void aux(const double* x, Aux& a) const { cl_sincos(x[0], &a.sn, &a.cs); }
+  // the one parameter c = K cos x of the tangent at the current point
+  CL_HD double tan_c(const Aux& a) const { return K * a.cs; }
   // W <- M W with fused_product order: C[i,j] = M[i,1] W[1,j], then fma(M[i,2], W[2,j], acc).
   template <int K2>
-  CL_HD void tangent_apply(const Aux& a, Frame<2, K2>& W) const {
-    double c = K * a.cs;
-    double m11 = 1.0 + c;
-#pragma unroll
-    for (int j = 0; j < K2; ++j) {
-      double w0 = W.at(0, j), w1 = W.at(1, j);
-      W.at(0, j) = cl_fma(1.0, w1, m11 * w0);
-      W.at(1, j) = cl_fma(1.0, w1, c * w0);
-    }
-  }
+  CL_HD void tangent_apply(const Aux& a, Frame<2, K2>& W) const { map_tangent_apply(tan_c(a), W); }
   // the one-step tangent at the current point, column-major (the ResidualMonitor reads it)
   CL_HD void jacobian(const Aux& a, double* J) const {
     double c = K * a.cs;
@@ -161,17 +167,9 @@ struct StdMapK {
     cl_sincos(x[0], &a.sn, &a.cs);
     a.K = x[2];
   }
+  CL_HD double tan_c(const Aux& a) const { return a.K * a.cs; }
   template <int K2>
-  CL_HD void tangent_apply(const Aux& a, Frame<2, K2>& W) const {
-    double c = a.K * a.cs;
-    double m11 = 1.0 + c;
-#pragma unroll
-    for (int j = 0; j < K2; ++j) {
-      double w0 = W.at(0, j), w1 = W.at(1, j);
-      W.at(0, j) = cl_fma(1.0, w1, m11 * w0);
-      W.at(1, j) = cl_fma(1.0, w1, c * w0);
-    }
-  }
+  CL_HD void tangent_apply(const Aux& a, Frame<2, K2>& W) const { map_tangent_apply(tan_c(a), W); }
   // the one-step tangent at the current point, column-major (the ResidualMonitor reads it)
   CL_HD void jacobian(const Aux& a, double* J) const {
     double c = a.K * a.cs;
@@ -201,17 +199,9 @@ struct WrappedStdMap {
     double sn, cs;
   };
   CL_HD void aux(const double* x, Aux& a) const { cl_sincos(x[0], &a.sn, &a.cs); }
+  CL_HD double tan_c(const Aux& a) const { return K * a.cs; }
   template <int K2>
-  CL_HD void tangent_apply(const Aux& a, Frame<2, K2>& W) const {
-    double c = K * a.cs;
-    double m11 = 1.0 + c;
-#pragma unroll
-    for (int j = 0; j < K2; ++j) {
-      double w0 = W.at(0, j), w1 = W.at(1, j);
-      W.at(0, j) = cl_fma(1.0, w1, m11 * w0);
-      W.at(1, j) = cl_fma(1.0, w1, c * w0);
-    }
-  }
+  CL_HD void tangent_apply(const Aux& a, Frame<2, K2>& W) const { map_tangent_apply(tan_c(a), W); }
   // the one-step tangent at the current point, column-major (the ResidualMonitor reads it)
   CL_HD void jacobian(const Aux& a, double* J) const {
     double c = K * a.cs;

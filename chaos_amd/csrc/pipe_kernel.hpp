@@ -1,0 +1,181 @@
+// pipe_kernel.hpp - a wave-specialised pipeline for small map ensembles whose chain is one QR spectrum.
+//
+// A small ensemble (a few thousand orbits) fills a few dozen waves on a 1024-SIMD device, so the fused
+// kernel runs at the issue rate of one wave per SIMD: its time is the length of the per-step instruction
+// stream (sin/cos of the point, the frame product, the factorisation, its checks and two logs), step after
+// step. The work of one orbit-step splits into four parts with one-way data flow:
+//
+//   A  the orbit: sin/cos of x_n, the step, the parameter c_n = K cos x_n of the tangent (standard_map.jl:30-34)
+//   B  the frame: W <- M(c) W in the fused_product order, then the factorisation W = QR (carries W = Q)
+//   C  the checks of the factorisation: window_Gw conditioning and orth_loss (qr_spectrum.jl:54-73),
+//      and the running maxima G_max, orth_max
+//   D  the exponent sums: add(sums[i], log R_ii)
+//
+// Each part runs on its own wave of a 256-lane group (64 orbits, one per lane of every wave), so one
+// orbit-step occupies four SIMDs. The parts hand data forward through double-buffered LDS rings: A writes
+// c for a chunk of steps while B consumes the previous chunk's c and writes R and Q, which C and D read one
+// chunk later. One barrier per chunk. The arithmetic of every part is the fused kernel's, in the same order,
+// so the results are bitwise the same.
+//
+// A failure (non-finite point or frame at a handoff, a failed factorisation or check) is not handled here:
+// the group marks the orbit, writes nothing back for it, and the caller reruns the marked orbits with the
+// fused kernel from the unchanged inputs, which reproduces the reference's stop rule exactly
+// (orbit.jl:48-58; qr_spectrum.jl:54-96).
+#pragma once
+#include "registry.hpp"
+
+namespace cl {
+
+constexpr int PIPE_LANES = 64;   // orbits per group
+constexpr int PIPE_STEPS = 16;   // map steps per chunk at most (tau <= 16)
+constexpr int PIPE_HAND = 8;     // handoffs per chunk at most
+
+// handoffs per chunk for a given tau: chunks of about 8 steps, at least one handoff
+CL_HD int pipe_handoffs(int tau) { return tau <= PIPE_HAND ? PIPE_HAND / tau : 1; }
+
+template <class Sys, int ALG, int K>
+__global__ void __launch_bounds__(256) pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H,
+                                                   uint8_t* rerun) {
+  constexpr int D = 2;
+  typedef QRState<K> St;
+  __shared__ double ringC[2][PIPE_STEPS][PIPE_LANES];
+  __shared__ double ringR[2][PIPE_HAND][K * K][PIPE_LANES];
+  __shared__ double ringQ[2][PIPE_HAND][D * K][PIPE_LANES];
+  __shared__ int bad_by[4][PIPE_LANES];
+  __shared__ double outG[PIPE_LANES], outO[PIPE_LANES];
+  __shared__ int64_t outCount[PIPE_LANES];
+
+  const int role = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * PIPE_LANES + lane;
+  const bool in_range = i < a.m;
+  const bool live = in_range && (a.n0 == 0 || a.code[i] == STATUS_OK);
+  const int tau = a.tau;
+
+  St st;
+  qr.init(st);
+  if (live && a.n0 != 0) state_load(st, a.st, a.m, i, 0);
+
+  double x[Sys::S];
+  typename Sys::Aux aux;
+  Frame<D, K> W;
+#pragma unroll
+  for (int j = 0; j < Sys::S; ++j) x[j] = 0.0;
+#pragma unroll
+  for (int e = 0; e < D * K; ++e) W.w[e] = 0.0;
+  if (role == 0 && in_range) {
+#pragma unroll
+    for (int j = 0; j < Sys::S; ++j) x[j] = a.x[(int64_t)j * a.m + i];
+  }
+  if (role == 1 && in_range) {
+#pragma unroll
+    for (int e = 0; e < D * K; ++e) W.w[e] = a.W0[(int64_t)e * a.m + i];
+  }
+  if (role == 0) sys.aux(x, aux);
+
+  const int64_t nh = a.steps / tau;
+  const int64_t nc = (nh + H - 1) / H;
+  bool bad = false;
+  for (int64_t p = 0; p < nc + 2; ++p) {
+    if (role == 0) {
+      if (p < nc) {
+        const int slot = (int)(p & 1);
+        const int64_t h0 = p * H, h1 = (h0 + H < nh) ? h0 + H : nh;
+        for (int64_t h = h0; h < h1; ++h) {
+          for (int t = 0; t < tau; ++t) {
+            ringC[slot][(int)(h - h0) * tau + t][lane] = sys.tan_c(aux);
+            sys.forward(aux, x);
+            sys.aux(x, aux);
+          }
+          bool xf = true;
+#pragma unroll
+          for (int j = 0; j < Sys::S; ++j) xf = xf && cl_isfinite(x[j]);
+          bad = bad || !xf;
+        }
+      }
+    } else if (role == 1) {
+      const int64_t q = p - 1;
+      if (q >= 0 && q < nc) {
+        const int slot = (int)(q & 1);
+        const int64_t h0 = q * H, h1 = (h0 + H < nh) ? h0 + H : nh;
+        for (int64_t h = h0; h < h1; ++h) {
+          const int r = (int)(h - h0);
+          for (int t = 0; t < tau; ++t) map_tangent_apply(ringC[slot][r * tau + t][lane], W);
+          bad = bad || !frame_finite(W);
+          QRFact<D, K> f;
+          orthonormalize<ALG, D, K>(W, f);
+          bad = bad || f.status != QR_OK;
+#pragma unroll
+          for (int e = 0; e < K * K; ++e) ringR[slot][r][e][lane] = f.R[e];
+#pragma unroll
+          for (int e = 0; e < D * K; ++e) ringQ[slot][r][e][lane] = f.Q.w[e];
+          W = f.Q;
+          st.count_max = st.count_max > f.count ? st.count_max : f.count;
+        }
+      }
+    } else {
+      const int64_t q = p - 2;
+      if (q >= 0 && q < nc) {
+        const int slot = (int)(q & 1);
+        const int64_t h0 = q * H, h1 = (h0 + H < nh) ? h0 + H : nh;
+        for (int64_t h = h0; h < h1; ++h) {
+          const int r = (int)(h - h0);
+          double R[K * K];
+#pragma unroll
+          for (int e = 0; e < K * K; ++e) R[e] = ringR[slot][r][e][lane];
+          if (role == 2) {
+            Frame<D, K> Q;
+#pragma unroll
+            for (int e = 0; e < D * K; ++e) Q.w[e] = ringQ[slot][r][e][lane];
+            const double Gw = window_Gw<K>(R);
+            if (QRSpectrum<ALG, D, K>::checks_conditioning && !(Gw * CL_EPS64 < 0.5)) bad = true;
+            const double orth = K >= 2 ? orth_loss<D, K>(Q) : 0.0;
+            const double g = (ALG == QR_CGS || ALG == QR_MGS) ? Gw : 1.0;
+            const double thr = qr.tol * CL_EPS64 * (double)K * g;
+            if (K >= 2 && !(orth <= thr)) bad = true;
+            st.G_max = cl_jmax(st.G_max, G_from_Gw<K>(Gw));
+            st.orth_max = cl_jmax(st.orth_max, orth);
+          } else {
+#pragma unroll
+            for (int e = 0; e < K; ++e) st.sums[e] = csum_add(st.sums[e], cl_log(R[e * K + e]));
+            st.n = a.n0 + (h + 1) * tau;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  bad_by[role][lane] = bad ? 1 : 0;
+  if (role == 1) outCount[lane] = st.count_max;
+  if (role == 2) {
+    outG[lane] = st.G_max;
+    outO[lane] = st.orth_max;
+  }
+  __syncthreads();
+  const bool failed = (bad_by[0][lane] | bad_by[1][lane] | bad_by[2][lane] | bad_by[3][lane]) != 0;
+  if (role == 0 && in_range) rerun[i] = (live && failed) ? 1 : 0;
+  if (!live) return;
+  const bool keep = !failed && nh > 0;
+  if (role == 0) {
+    if (keep) {
+#pragma unroll
+      for (int j = 0; j < Sys::S; ++j) a.x[(int64_t)j * a.m + i] = x[j];
+    }
+  } else if (role == 1) {
+    if (keep) {
+#pragma unroll
+      for (int e = 0; e < D * K; ++e) a.W0[(int64_t)e * a.m + i] = W.w[e];
+    }
+  } else if (role == 3) {
+    if (keep) {
+      st.G_max = outG[lane];
+      st.orth_max = outO[lane];
+      st.count_max = outCount[lane];
+      st.column = 0;
+      state_store(st, a.st, a.m, i, 0);
+      a.code[i] = STATUS_OK;
+      a.hstep[i] = a.n0 + nh * tau;
+    }
+  }
+}
+
+}  // namespace cl

@@ -29,11 +29,24 @@ static std::once_flag g_once;
 static Instance g_inst[256];
 static int g_ninst = 0;
 
+static PipeInstance g_pipe[64];
+static int g_npipe = 0;
+
 static void load_table() {
   std::call_once(g_once, [] {
     register_instances_map(g_inst, &g_ninst, 256);
     register_instances_flow(g_inst, &g_ninst, 256);
+    register_pipes(g_pipe, &g_npipe, 64);
   });
+}
+
+// The pipeline kernel for a one-frame chain that is a single QR spectrum on a two-dimensional map.
+static const PipeInstance* find_pipe(const cl_system* sys, const cl_chain* ch, int d, int k) {
+  load_table();
+  if (d != 2 || ch->n != 1 || ch->r[0].kind != CL_RED_QR || ch->r[0].frame != 0) return nullptr;
+  for (int i = 0; i < g_npipe; ++i)
+    if (g_pipe[i].sys_kind == sys->kind && g_pipe[i].k == k && g_pipe[i].alg == ch->r[0].alg) return &g_pipe[i];
+  return nullptr;
 }
 
 static const Instance* find_instance(const cl_system* sys, const cl_chain* ch, int d, int k) {
@@ -415,6 +428,62 @@ static int launch_split(const Instance* in, const cl_system* sys, const cl_chain
   return rc;
 }
 
+// ------------------------------------------------------------------------------ pipelined small ensembles
+// Below a few tens of thousands of orbits the fused kernel leaves most SIMDs idle and runs at the issue rate of
+// one wave; the pipeline kernel spreads each orbit-step over four waves (pipe_kernel.hpp). Orbits it marks
+// (any failure) are rerun by the fused kernel from their unchanged inputs. CL_PIPE=0 disables it, CL_PIPE=1
+// uses it for every eligible launch; by default it serves launches of at most PIPE_MAX_ORBITS orbits.
+static const int64_t PIPE_MAX_ORBITS = 16384;
+
+static bool pipe_enabled(int64_t m) {
+  const char* e = getenv("CL_PIPE");
+  if (e && e[0] == '0') return false;
+  if (e && e[0] == '1') return true;
+  return m <= PIPE_MAX_ORBITS;
+}
+
+static int launch_pipe(const Instance* in, const cl_system* sys, const cl_chain* ch, int64_t m, int d, int k,
+                       double* dx, double* dW, uint64_t* dst, int8_t* dcode, int64_t* dhstep, int64_t n0,
+                       int64_t steps, int64_t N, int tau, hipStream_t st) {
+  const PipeInstance* pi = find_pipe(sys, ch, d, k);
+  if (!pi || in->nf != 1 || tau > PIPE_TAU_MAX || !pipe_enabled(m)) return CL_ERR_UNSUPPORTED;
+  if (m == 0) return CL_OK;
+  uint8_t* rerun = nullptr;
+  hipError_t e = hipMallocAsync((void**)&rerun, (size_t)m, st);
+  if (e != hipSuccess) return map_err(e);
+  KArgs a;
+  memset(&a, 0, sizeof(a));
+  a.x = dx;
+  a.W0 = dW;
+  a.st = dst;
+  a.code = dcode;
+  a.hstep = dhstep;
+  a.m = m;
+  a.n0 = n0;
+  a.steps = steps;
+  a.N = N;
+  a.tau = tau;
+  int rc = pi->launch(sys, ch, a, rerun, st);
+  if (!rc) rc = launch(in, sys, ch, m, dx, dW, nullptr, dst, dcode, dhstep, n0, steps, N, tau, st, nullptr, 0, 0, rerun);
+  hipFreeAsync(rerun, st);
+  return rc;
+}
+
+// One launch by the fastest applicable route: two single-frame passes for a two-frame map chain, the pipeline
+// for a small single-QR ensemble, else the fused kernel. Every route gives the fused kernel's bits.
+static int launch_best(const Instance* in, const cl_system* sys, const cl_chain* ch, int64_t m, int s, int d, int k,
+                       double* dx, double* dW, double* dW2, uint64_t* dst, int8_t* dcode, int64_t* dhstep,
+                       int64_t n0, int64_t steps, int64_t N, int tau, hipStream_t st) {
+  int rc;
+  if (in->nf > 1 && split_enabled(sys->kind)) {
+    rc = launch_split(in, sys, ch, m, s, d, k, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st);
+    if (rc != CL_ERR_UNSUPPORTED) return rc;
+  }
+  rc = launch_pipe(in, sys, ch, m, d, k, dx, dW, dst, dcode, dhstep, n0, steps, N, tau, st);
+  if (rc != CL_ERR_UNSUPPORTED) return rc;
+  return launch(in, sys, ch, m, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st);
+}
+
 static int chain_device(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m, int s, int d,
                         int k, int64_t n0, int64_t steps, int64_t N, int tau, double* dx, double* dW, uint64_t* dst,
                         int8_t* dcode, int64_t* dhstep, hipStream_t st) {
@@ -446,11 +515,7 @@ extern "C" int cl_propagate_device(cl_ctx* ctx, const cl_system* sys, const cl_c
   if (in->nf > 1 && !dW2) return set_err(ctx, CL_ERR_ARGUMENT, "the chain uses a second frame: W2 is required");
   hipSetDevice(ctx->device);
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-  if (in->nf > 1 && split_enabled(sys->kind)) {
-    rc = launch_split(in, sys, ch, m, s, d, k, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st);
-    if (rc != CL_ERR_UNSUPPORTED) return rc ? set_err(ctx, rc, "split launch failed") : CL_OK;
-  }
-  rc = launch(in, sys, ch, m, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st);
+  rc = launch_best(in, sys, ch, m, s, d, k, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st);
   if (rc) return set_err(ctx, rc, "kernel launch failed");
   return CL_OK;
 }
@@ -619,7 +684,8 @@ extern "C" int cl_propagate_chunk(cl_ctx* ctx, const cl_system* sys, const cl_ch
     CK(hipMemcpyAsync(dcode, code, (size_t)m, hipMemcpyHostToDevice, st));
     CK(hipMemcpyAsync(dh, hstep, (size_t)m * 8, hipMemcpyHostToDevice, st));
   }
-  rc = launch(in, sys, ch, m, (double*)dx, (double*)dW, (double*)dW2, dst, dcode, dh, n0, steps, N, tau, st);
+  rc = launch_best(in, sys, ch, m, s, d, k, (double*)dx, (double*)dW, (double*)dW2, dst, dcode, dh, n0, steps, N,
+                   tau, st);
   if (rc) return set_err(ctx, rc, "kernel launch failed");
   hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(m * s, B)), dim3(B), 0, st, dx, xa, m, s);
   hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(m * dk, B)), dim3(B), 0, st, dW, wa, m, dk);
