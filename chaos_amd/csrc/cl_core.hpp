@@ -590,7 +590,14 @@ CL_HD double window_Gw(const double* R0) {
   for (int i = 1; i < K * K; ++i) m = cl_jmax(m, cl_fabs(R0[i]));
   bool scale = (m > 1e100 || m < 1e-100) && cl_isfinite(m) && m > 0.0;
 #pragma unroll
-  for (int i = 0; i < K * K; ++i) R[i] = scale ? R0[i] / m : R0[i];
+  for (int i = 0; i < K * K; ++i) R[i] = R0[i];
+  if (scale) {
+    // rare: the empty volatile asm keeps the compiler from computing the K^2 divisions for every lane and
+    // selecting afterwards
+    __asm__ volatile("");
+#pragma unroll
+    for (int i = 0; i < K * K; ++i) R[i] = R0[i] / m;
+  }
   double Ri[K * K];
 #pragma unroll
   for (int i = 0; i < K * K; ++i) Ri[i] = 0.0;

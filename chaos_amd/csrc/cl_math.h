@@ -13,11 +13,17 @@
  *    product made exact with fma (the reduced value is exact by Sterbenz' lemma).
  *    |x| >= 2^50: Payne and Hanek (1983): a 192-bit window of the bits of 2/pi selected by the
  *    exponent of x, an exact 53x192-bit integer product, the quadrant from its integer bits.
- *    Then the Taylor series of sin and cos (coefficients 1/n! rounded to double) on
- *    |r| <= 1, with the first-order correction for the low part of r.
- *  - log: x = 2^k m, m in [sqrt(2)/2, sqrt(2)), f = m - 1, s = f/(2+f), and the series
- *    log(1+f) = 2 atanh(s) = f - f^2/2 + s (f^2/2 + R(s^2)), R(z) = sum_i 2 z^i/(2i+1),
+ *    Then sin(r) = r + r^3 S(r^2) and cos(r) = 1 - r^2/2 + r^4 C(r^2) on |r| <= 1 (near 2^50 the
+ *    quadrant from the rounded x * 2/pi can be off by one, so |r| reaches about 0.9), with the
+ *    first-order correction for the low part of r.
+ *  - log: x = 2^k m, m in [sqrt(2)/2, sqrt(2)), f = m - 1, s = f/(2+f), and
+ *    log(1+f) = 2 atanh(s) = f - f^2/2 + s (f^2/2 + R(s^2)), R(z) = z P(z) ~ sum_i 2 z^i/(2i+1),
  *    with ln 2 split so that k*LN2_HI is exact.
+ *  S, C and P (degree 6) are minimax polynomials for relative error on the reduced intervals, fitted
+ *  by the Remez exchange algorithm in 200-bit arithmetic (chaos_amd/csrc/minimax.py; relative errors
+ *  after rounding to double 2^-53.8, 2^-54.0, 2^-50.9) and evaluated by Horner's rule with separate
+ *  multiplies and adds (a constant addend then stays a scalar operand instead of taking a vector register
+ *  across the kernels' step loops, as the tied accumulator of a fused multiply-add would).
  *  - TwoSum (Knuth), TwoProd with fma (Dekker's product made exact by the fused multiply-add).
  * The constants come from chaos_amd/csrc/gen_math_constants.py.
  */
@@ -236,20 +242,18 @@ CL_HD int cl_rem_pio2(double x, double *rh, double *rl) {
 /* ---------------------------------------------------------------- sin / cos kernels, |r| <= 1 */
 CL_HD double cl_sin_kernel(double rh, double rl) {
   double z = rh * rh;
-  double p = -0x1.ae64567f544e4p-26 + z * (0x1.6124613a86d09p-33 + z * (-0x1.ae7f3e733b81fp-41 +
-             z * (0x1.952c77030ad4ap-49 + z * (-0x1.2f49b46814157p-57 + z * 0x1.71b8ef6dcf572p-66))));
-  p = -0x1.5555555555555p-3 + z * (0x1.1111111111111p-7 + z * (-0x1.a01a01a01a01ap-13 +
-      z * (0x1.71de3a556c734p-19 + z * p)));
+  double p = -0x1.5555555555555p-3 + z * (0x1.11111111110fep-7 + z * (-0x1.a01a01a015250p-13 +
+             z * (0x1.71de3a4e27e9ep-19 + z * (-0x1.ae644c23fee72p-26 + z * (0x1.611ccbda97a69p-33 +
+             z * -0x1.a8fb6995f6ab8p-41)))));
   double v = z * rh;
   return rh + (rl * (1.0 - 0.5 * z) + v * p);
 }
 
 CL_HD double cl_cos_kernel(double rh, double rl) {
   double z = rh * rh;
-  double p = 0x1.1eed8eff8d898p-29 + z * (-0x1.93974a8c07c9dp-37 + z * (0x1.ae7f3e733b81fp-45 +
-             z * (-0x1.6827863b97d97p-53 + z * (0x1.e542ba4020225p-62 + z * -0x1.0ce396db7f853p-70))));
-  p = 0x1.5555555555555p-5 + z * (-0x1.6c16c16c16c17p-10 + z * (0x1.a01a01a01a01ap-16 +
-      z * (-0x1.27e4fb7789f5cp-22 + z * p)));
+  double p = 0x1.5555555555555p-5 + z * (-0x1.6c16c16c16c0ep-10 + z * (0x1.a01a01a017da5p-16 +
+             z * (-0x1.27e4fb7450d4fp-22 + z * (0x1.1eed8a6694be0p-29 + z * (-0x1.93908da96df55p-37 +
+             z * 0x1.a9983c88bc5b3p-45)))));
   double hz = 0.5 * z;
   double w = 1.0 - hz;
   return w + (((1.0 - w) - hz) + (z * z * p - rh * rl));
@@ -305,10 +309,9 @@ CL_HD double cl_log(double x) {
   double f = m - 1.0;
   double s = f / (2.0 + f);
   double z = s * s;
-  double R = z * (0x1.5555555555555p-1 + z * (0x1.999999999999ap-2 + z * (0x1.2492492492492p-2 +
-             z * (0x1.c71c71c71c71cp-3 + z * (0x1.745d1745d1746p-3 + z * (0x1.3b13b13b13b14p-3 +
-             z * (0x1.1111111111111p-3 + z * (0x1.e1e1e1e1e1e1ep-4 + z * (0x1.af286bca1af28p-4 +
-             z * (0x1.8618618618618p-4 + z * (0x1.642c8590b2164p-4 + z * 0x1.47ae147ae147bp-4)))))))))));
+  double R = z * (0x1.5555555555558p-1 + z * (0x1.999999999529cp-2 + z * (0x1.2492492df8420p-2 +
+             z * (0x1.c71c62dd6bbfcp-3 + z * (0x1.7462b6c858cc6p-3 + z * (0x1.39fe1f2d19f7ap-3 +
+             z * 0x1.2b5b3cd07f765p-3))))));
   double hfsq = 0.5 * f * f;
   double dk = (double)k;
   double lo = dk * CL_LN2_LO + s * (hfsq + R);

@@ -11,11 +11,14 @@
 //      and the running maxima G_max, orth_max
 //   D  the exponent sums: add(sums[i], log R_ii)
 //
-// Each part runs on its own wave of a 256-lane group (64 orbits, one per lane of every wave), so one
-// orbit-step occupies four SIMDs. The parts hand data forward through double-buffered LDS rings: A writes
-// c for a chunk of steps while B consumes the previous chunk's c and writes R and Q, which C and D read one
-// chunk later. One barrier per chunk. The arithmetic of every part is the fused kernel's, in the same order,
-// so the results are bitwise the same.
+// Each part runs on its own waves of a 384-lane group (64 orbits, one per lane of every wave): A and B one
+// wave each; C two waves, for the even and the odd handoffs (a maximum does not depend on the order of its
+// terms, and both start from the same value); D one wave per column (each column's compensated sum stays in
+// step order). One orbit-step occupies six SIMDs, and B, whose factorisation carries the frame from step to
+// step, sets the pace. The parts hand data forward through double-buffered LDS rings: A writes c for a chunk
+// of steps while B consumes the previous chunk's c and writes R and Q, which C and D read one chunk later.
+// One barrier per chunk. The arithmetic of every part is the fused kernel's, in the same order, so the
+// results are bitwise the same.
 //
 // A failure (non-finite point or frame at a handoff, a failed factorisation or check) is not handled here:
 // the group marks the orbit, writes nothing back for it, and the caller reruns the marked orbits with the
@@ -27,6 +30,7 @@
 namespace cl {
 
 constexpr int PIPE_LANES = 64;   // orbits per group
+constexpr int PIPE_WAVES = 6;    // A, B, C even, C odd, D column 1, D column 2
 constexpr int PIPE_STEPS = 16;   // map steps per chunk at most (tau <= 16)
 constexpr int PIPE_HAND = 8;     // handoffs per chunk at most
 
@@ -34,18 +38,23 @@ constexpr int PIPE_HAND = 8;     // handoffs per chunk at most
 CL_HD int pipe_handoffs(int tau) { return tau <= PIPE_HAND ? PIPE_HAND / tau : 1; }
 
 template <class Sys, int ALG, int K>
-__global__ void __launch_bounds__(256) pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H,
+__global__ void __launch_bounds__(PIPE_WAVES * PIPE_LANES) pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H,
                                                    uint8_t* rerun) {
   constexpr int D = 2;
   typedef QRState<K> St;
   __shared__ double ringC[2][PIPE_STEPS][PIPE_LANES];
   __shared__ double ringR[2][PIPE_HAND][K * K][PIPE_LANES];
   __shared__ double ringQ[2][PIPE_HAND][D * K][PIPE_LANES];
-  __shared__ int bad_by[4][PIPE_LANES];
-  __shared__ double outG[PIPE_LANES], outO[PIPE_LANES];
+  __shared__ int bad_by[PIPE_WAVES][PIPE_LANES];
+  __shared__ double outG[2][PIPE_LANES], outO[2][PIPE_LANES];
+  __shared__ CSum outS[2][PIPE_LANES];  // [1]: column 2's sum, from the wave that formed it
   __shared__ int64_t outCount[PIPE_LANES];
 
-  const int role = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // Waves w and w + 4 of a group share a SIMD (round-robin placement over the CU's four SIMDs). Measured:
+  // the step time is set by B whatever it shares with (placements differ by under 3%); the two C waves get a
+  // SIMD each, A shares with D of column 2 and B with D of column 1.
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int role = wave < 4 ? wave : (wave == 4 ? 5 : 4);
   const int64_t i = (int64_t)blockIdx.x * PIPE_LANES + lane;
   const bool in_range = i < a.m;
   const bool live = in_range && (a.n0 == 0 || a.code[i] == STATUS_OK);
@@ -97,9 +106,19 @@ __global__ void __launch_bounds__(256) pipe_kernel(KArgs a, Sys sys, QRSpectrum<
       if (q >= 0 && q < nc) {
         const int slot = (int)(q & 1);
         const int64_t h0 = q * H, h1 = (h0 + H < nh) ? h0 + H : nh;
+        // the next step's c is read one step ahead, so the LDS latency overlaps this step's work instead of
+        // sitting on the frame's step-to-step chain (the index is clamped to the chunk's last step)
+        const int ns = (int)(h1 - h0) * tau;
+        int si = 0;
+        double cn = ringC[slot][0][lane];
         for (int64_t h = h0; h < h1; ++h) {
           const int r = (int)(h - h0);
-          for (int t = 0; t < tau; ++t) map_tangent_apply(ringC[slot][r * tau + t][lane], W);
+          for (int t = 0; t < tau; ++t) {
+            const double c = cn;
+            ++si;
+            cn = ringC[slot][si < ns ? si : ns - 1][lane];
+            map_tangent_apply(c, W);
+          }
           bad = bad || !frame_finite(W);
           QRFact<D, K> f;
           orthonormalize<ALG, D, K>(W, f);
@@ -112,17 +131,17 @@ __global__ void __launch_bounds__(256) pipe_kernel(KArgs a, Sys sys, QRSpectrum<
           st.count_max = st.count_max > f.count ? st.count_max : f.count;
         }
       }
-    } else {
+    } else if (role <= 3) {
       const int64_t q = p - 2;
       if (q >= 0 && q < nc) {
         const int slot = (int)(q & 1);
         const int64_t h0 = q * H, h1 = (h0 + H < nh) ? h0 + H : nh;
-        for (int64_t h = h0; h < h1; ++h) {
+        for (int64_t h = h0 + ((h0 + role) & 1); h < h1; h += 2) {  // role 2: even h, role 3: odd h
           const int r = (int)(h - h0);
           double R[K * K];
 #pragma unroll
           for (int e = 0; e < K * K; ++e) R[e] = ringR[slot][r][e][lane];
-          if (role == 2) {
+          {
             Frame<D, K> Q;
 #pragma unroll
             for (int e = 0; e < D * K; ++e) Q.w[e] = ringQ[slot][r][e][lane];
@@ -134,24 +153,39 @@ __global__ void __launch_bounds__(256) pipe_kernel(KArgs a, Sys sys, QRSpectrum<
             if (K >= 2 && !(orth <= thr)) bad = true;
             st.G_max = cl_jmax(st.G_max, G_from_Gw<K>(Gw));
             st.orth_max = cl_jmax(st.orth_max, orth);
-          } else {
-#pragma unroll
-            for (int e = 0; e < K; ++e) st.sums[e] = csum_add(st.sums[e], cl_log(R[e * K + e]));
-            st.n = a.n0 + (h + 1) * tau;
           }
         }
+      }
+    } else {
+      const int col = role - 4;
+      const int64_t q = p - 2;
+      if (col < K && q >= 0 && q < nc) {
+        const int slot = (int)(q & 1);
+        const int64_t h0 = q * H, h1 = (h0 + H < nh) ? h0 + H : nh;
+        // the column is uniform over the wave; selects keep the state out of indexed (scratch) memory
+        CSum sm = col == 0 ? st.sums[0] : st.sums[K - 1];
+        for (int64_t h = h0; h < h1; ++h) {
+          const int r = (int)(h - h0);
+          sm = csum_add(sm, cl_log(ringR[slot][r][col * K + col][lane]));
+        }
+        if (col == 0) st.sums[0] = sm;
+        else st.sums[K - 1] = sm;
+        st.n = a.n0 + h1 * tau;
       }
     }
     __syncthreads();
   }
   bad_by[role][lane] = bad ? 1 : 0;
   if (role == 1) outCount[lane] = st.count_max;
-  if (role == 2) {
-    outG[lane] = st.G_max;
-    outO[lane] = st.orth_max;
+  if (role == 2 || role == 3) {
+    outG[role - 2][lane] = st.G_max;
+    outO[role - 2][lane] = st.orth_max;
   }
+  if (role == 5 && K > 1) outS[1][lane] = st.sums[K - 1];
   __syncthreads();
-  const bool failed = (bad_by[0][lane] | bad_by[1][lane] | bad_by[2][lane] | bad_by[3][lane]) != 0;
+  bool failed = false;
+#pragma unroll
+  for (int w = 0; w < PIPE_WAVES; ++w) failed = failed || bad_by[w][lane] != 0;
   if (role == 0 && in_range) rerun[i] = (live && failed) ? 1 : 0;
   if (!live) return;
   const bool keep = !failed && nh > 0;
@@ -165,10 +199,12 @@ __global__ void __launch_bounds__(256) pipe_kernel(KArgs a, Sys sys, QRSpectrum<
 #pragma unroll
       for (int e = 0; e < D * K; ++e) a.W0[(int64_t)e * a.m + i] = W.w[e];
     }
-  } else if (role == 3) {
+  } else if (role == 4) {
     if (keep) {
-      st.G_max = outG[lane];
-      st.orth_max = outO[lane];
+      // the maxima over the even and the odd handoffs, each started from the launch's input value
+      st.G_max = cl_jmax(outG[0][lane], outG[1][lane]);
+      st.orth_max = cl_jmax(outO[0][lane], outO[1][lane]);
+      if (K > 1) st.sums[K - 1] = outS[1][lane];
       st.count_max = outCount[lane];
       st.column = 0;
       state_store(st, a.st, a.m, i, 0);
