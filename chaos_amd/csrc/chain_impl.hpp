@@ -11,9 +11,11 @@
 // neighbours' q rows of the frame (cross-lane moves, LDS for N = 128).
 //
 // Column reductions (norms and dot products over the 2N rows) take a partial per site over its q row (when
-// in range) then its p row, combined by an xor butterfly over the N sites, largest offset first; every
-// thread ends with the same value because IEEE addition commutes. The CPU oracle restates this order
-// (oracle/oracle_chain.hpp, RED_SITES) and is checked against the reference's row order.
+// in range) then its p row, combined by a butterfly over the N sites, smallest level first (pairs of
+// sites, then pairs of pairs, ...); every thread ends with the same value because IEEE addition commutes.
+// The in-wave levels use DPP lane moves and the gfx950 permlane swaps (no LDS); at N = 128 the two waves'
+// sums are added last through LDS. The CPU oracle restates this order (oracle/oracle_chain.hpp, RED_SITES)
+// and is checked against the reference's row order.
 //
 // HBM layout (orbit-major, the reference's host layout): x[i][2N], W[i][k][2N], states[i][2k+9 words].
 // Before each QR the thread stores its frame rows to W (a failed factorisation leaves the frame it was
@@ -41,8 +43,56 @@ struct Geo {
 };
 
 // ---------------------------------------------------------------- cross-lane primitives within one orbit
+// In-wave butterfly level L (smallest first). L = 0, 1: the lane ^ 1, ^ 2 (DPP quad_perm); L = 2, 3: the lane
+// mirrored in its 8-lane half-row, 16-lane row (DPP row_half_mirror, row_mirror): after the lower levels every
+// lane of a 4-lane (8-lane) group holds the group's value, so the mirrored lane pairs the same two groups as
+// lane ^ 4 (^ 8) would; L = 4, 5: v_permlane16_swap, v_permlane32_swap exchange the 16-lane rows and the
+// 32-lane halves and leave both values of a pair in every lane. `pair2` returns the two values whose sum
+// (maximum, or) the level forms, in an order that may differ between the lanes of a pair (the operations
+// used commute).
+__device__ __attribute__((always_inline)) inline double dbl_of(unsigned lo, unsigned hi) {
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int L>
+__device__ __attribute__((always_inline)) inline void pair2(double v, double& a, double& b) {
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)bits, hi = (unsigned)(bits >> 32);
+  if constexpr (L < 4) {
+    constexpr int ctrl = L == 0 ? 0xB1 : (L == 1 ? 0x4E : (L == 2 ? 0x141 : 0x140));
+    a = v;
+    b = dbl_of((unsigned)__builtin_amdgcn_mov_dpp((int)lo, ctrl, 0xf, 0xf, false),
+               (unsigned)__builtin_amdgcn_mov_dpp((int)hi, ctrl, 0xf, 0xf, false));
+  } else if constexpr (L == 4) {
+    const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    a = dbl_of(l[0], h[0]);
+    b = dbl_of(l[1], h[1]);
+  } else {
+    const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    a = dbl_of(l[0], h[0]);
+    b = dbl_of(l[1], h[1]);
+  }
+}
+template <int L>
+__device__ __attribute__((always_inline)) inline int or_level(int v) {
+  if constexpr (L < 4) {
+    constexpr int ctrl = L == 0 ? 0xB1 : (L == 1 ? 0x4E : (L == 2 ? 0x141 : 0x140));
+    return v | __builtin_amdgcn_mov_dpp(v, ctrl, 0xf, 0xf, false);
+  } else if constexpr (L == 4) {
+    const auto x = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
+    return (int)(x[0] | x[1]);
+  } else {
+    const auto x = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+    return (int)(x[0] | x[1]);
+  }
+}
+
 template <int NS>
 struct Orb {
+  // in-wave butterfly levels of an orbit: log2 of its lanes in one wave
+  static constexpr int LV = NS >= 64 ? 6 : (NS == 32 ? 5 : (NS == 16 ? 4 : (NS == 8 ? 3 : (NS == 4 ? 2 : 1))));
+
   double* xb;  // LDS exchange buffer (N = 128 only): 128 * 16 doubles
   int t;       // site
   int base;    // first lane of this orbit inside its wave (N <= 64)
@@ -71,14 +121,34 @@ struct Orb {
       __syncthreads();
       return reinterpret_cast<int*>(xb)[0];
     } else {
-#pragma unroll
-      for (int o = NS / 2; o >= 1; o >>= 1) v |= __shfl_xor(v, o, 64);
+      v = or_level<0>(v);
+      if constexpr (LV > 1) v = or_level<1>(v);
+      if constexpr (LV > 2) v = or_level<2>(v);
+      if constexpr (LV > 3) v = or_level<3>(v);
+      if constexpr (LV > 4) v = or_level<4>(v);
+      if constexpr (LV > 5) v = or_level<5>(v);
       return v;
     }
+  }
+  template <int L, int M>
+  __device__ __attribute__((always_inline)) static void sum_level(double (&v)[M], int lo, int hi) {
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+      if (j >= lo && j <= hi) {
+        double a, b;
+        pair2<L>(v[j], a, b);
+        v[j] = a + b;
+      }
   }
   // butterfly sums of v[lo..hi] (the other entries are left untouched)
   template <int M>
   __device__ __attribute__((always_inline)) void sum(double (&v)[M], int lo = 0, int hi = M - 1) const {
+    sum_level<0>(v, lo, hi);
+    if constexpr (LV > 1) sum_level<1>(v, lo, hi);
+    if constexpr (LV > 2) sum_level<2>(v, lo, hi);
+    if constexpr (LV > 3) sum_level<3>(v, lo, hi);
+    if constexpr (LV > 4) sum_level<4>(v, lo, hi);
+    if constexpr (LV > 5) sum_level<5>(v, lo, hi);
     if constexpr (NS > 64) {
       __syncthreads();
 #pragma unroll
@@ -89,31 +159,40 @@ struct Orb {
       for (int j = 0; j < M; ++j)
         if (j >= lo && j <= hi) v[j] = v[j] + xb[j * 128 + (t ^ 64)];
     }
-    constexpr int O0 = NS > 64 ? 32 : NS / 2;
-#pragma unroll
-    for (int o = O0; o >= 1; o >>= 1)
-#pragma unroll
-      for (int j = 0; j < M; ++j)
-        if (j >= lo && j <= hi) v[j] = v[j] + __shfl_xor(v[j], o, 64);
   }
   __device__ __attribute__((always_inline)) double sum1(double v) const {
     double a[1] = {v};
     sum<1>(a);
     return a[0];
   }
+  template <int L>
+  __device__ __attribute__((always_inline)) static double max_level(double v) {
+    double a, b;
+    pair2<L>(v, a, b);
+    return fmax(a, b);
+  }
+  __device__ __attribute__((always_inline)) static double max_wave(double v) {
+    v = max_level<0>(v);
+    if constexpr (LV > 1) v = max_level<1>(v);
+    if constexpr (LV > 2) v = max_level<2>(v);
+    if constexpr (LV > 3) v = max_level<3>(v);
+    if constexpr (LV > 4) v = max_level<4>(v);
+    if constexpr (LV > 5) v = max_level<5>(v);
+    return v;
+  }
   __device__ __attribute__((always_inline)) double max1(double v) const {
+    v = max_wave(v);
     if constexpr (NS > 64) {
       __syncthreads();
       xb[t] = v;
       __syncthreads();
       v = fmax(v, xb[t ^ 64]);
     }
-    constexpr int O0 = NS > 64 ? 32 : NS / 2;
-#pragma unroll
-    for (int o = O0; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     return v;
   }
   __device__ __attribute__((always_inline)) void max2(double& u, double& v) const {
+    u = max_wave(u);
+    v = max_wave(v);
     if constexpr (NS > 64) {
       __syncthreads();
       xb[t] = u;
@@ -121,13 +200,6 @@ struct Orb {
       __syncthreads();
       u = fmax(u, xb[t ^ 64]);
       v = fmax(v, xb[128 + (t ^ 64)]);
-    }
-    constexpr int O0 = NS > 64 ? 32 : NS / 2;
-#pragma unroll
-    for (int o = O0; o >= 1; o >>= 1) {
-      const double a = __shfl_xor(u, o, 64), b = __shfl_xor(v, o, 64);
-      u = fmax(u, a);
-      v = fmax(v, b);
     }
   }
   __device__ __attribute__((always_inline)) double right1(double v) const {

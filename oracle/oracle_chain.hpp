@@ -13,9 +13,9 @@
 //   RED_ROWS  (0): the reference's left fold over the rows in row order (q rows 1..N, then p rows), the dot and sum
 //                  of a StaticArrays vector;
 //   RED_SITES (1): the order of the device kernel, which gives each chain site one thread: a partial per site over its
-//                  q row (when that row is in range) then its p row, then an xor butterfly over the N sites,
-//                  v[t] <- v[t] + v[t ^ o] for o = N/2, N/4, ..., 1, whose result every site holds (IEEE addition
-//                  commutes). In this order rows that hold structural zeros (rows above the reflector's pivot, and
+//                  q row (when that row is in range) then its p row, then a butterfly over the N sites,
+//                  v[t] <- v[t] + v[t ^ o] for o = 1, 2, 4, ..., N/2 (pairs of sites first), whose result every site
+//                  holds (IEEE addition commutes). In this order rows that hold structural zeros (rows above the reflector's pivot, and
 //                  columns a reflector leaves unchanged) are skipped; the results differ from RED_ROWS by rounding only.
 // The reference's left fold is the restatement; RED_SITES is what the device kernel computes bit for bit.
 
@@ -25,7 +25,7 @@ template <class T>
 static T butterfly(std::vector<T>& v) {
   int N = (int)v.size();
   std::vector<T> w(N);
-  for (int o = N / 2; o >= 1; o /= 2) {
+  for (int o = 1; o < N; o *= 2) {
     for (int t = 0; t < N; ++t) w[t] = v[t] + v[t ^ o];
     v.swap(w);
   }
