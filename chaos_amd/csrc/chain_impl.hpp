@@ -516,8 +516,12 @@ __device__ __attribute__((always_inline)) inline void qr_handoff(const Orb<NS>& 
   osync<NS>();  // Ro / Rio / Us are rewritten at the next handoff
 }
 
+// N >= 64: ask the allocator for two waves per SIMD (256 registers, a few spilled): measured on the C3 workload,
+// N = 128 925 ms against 1001 ms and N = 64 552 against 573; at N = 32 the spills cost more than the second
+// wave (484 against 453), so smaller N keep the whole register file.
 template <int NS, int KC, bool RING, bool HASQR>
-__global__ void __launch_bounds__(Geo<NS>::BLOCK) chain_kernel(ChainArgs a) {
+__global__ void __launch_bounds__(Geo<NS>::BLOCK) __attribute__((amdgpu_waves_per_eu(NS >= 64 ? 2 : 1)))
+chain_kernel(ChainArgs a) {
   constexpr int D = 2 * NS;
   constexpr int NREF = KC < D ? KC : KC - 1;
   constexpr int SPT = (KC + NS - 1) / NS;  // QR sums owned per thread
