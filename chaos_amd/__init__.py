@@ -5,11 +5,11 @@ Python host side of that ABI, mirroring the reference's `propagate` API and redu
 """
 from .systems import StandardMap, StandardMapK, StandardMapChain, WrappedStandardMap, HenonHeilesGauss, torus_state, hh_energy
 from .reducers import (QRSpectrum, CGS, MGS, RGS, HouseholderQR, AugmentedQR, GivensQR, Benettin, MEGNO, SALI, FLI,
-                       OFLI, GALI, ResidualMonitor, monitor_exponents, det_threshold, symp_threshold, sum_threshold,
+                       OFLI, GALI, ResidualMonitor, RunMonitor, monitor_exponents, det_threshold, symp_threshold, sum_threshold,
                        eps_sum, eps_pair, check_residual)
 from .tangent import propagate, propagate_ragged, PropagationResult, NumericalHealthError, initial_deviations
 
 __all__ = ["StandardMap", "StandardMapK", "StandardMapChain", "WrappedStandardMap", "HenonHeilesGauss", "hh_energy", "torus_state", "QRSpectrum", "CGS", "MGS", "RGS", "HouseholderQR",
-           "AugmentedQR", "GivensQR", "Benettin", "MEGNO", "SALI", "FLI", "OFLI", "GALI", "ResidualMonitor", "monitor_exponents", "det_threshold",
+           "AugmentedQR", "GivensQR", "Benettin", "MEGNO", "SALI", "FLI", "OFLI", "GALI", "ResidualMonitor", "RunMonitor", "monitor_exponents", "det_threshold",
            "symp_threshold", "sum_threshold", "eps_sum", "eps_pair", "check_residual", "propagate", "propagate_ragged",
            "PropagationResult", "NumericalHealthError", "initial_deviations"]

@@ -15,6 +15,7 @@ CL_SYS_STANDARD_MAP, CL_SYS_WRAPPED_STANDARD_MAP, CL_SYS_STANDARD_MAP_CHAIN, CL_
 CL_SYS_STANDARD_MAP_K = 5
 CL_RED_QR, CL_RED_BENETTIN, CL_RED_MEGNO, CL_RED_SALI, CL_RED_FLI, CL_RED_OFLI, CL_RED_GALI = 1, 2, 3, 4, 5, 6, 7
 CL_RED_MONITOR = 9
+CL_RED_RUNMONITOR = 10
 CL_QR_CGS, CL_QR_MGS, CL_QR_RGS, CL_QR_HOUSEHOLDER, CL_QR_AUGMENTED, CL_QR_GIVENS = 1, 2, 3, 4, 5, 6
 CL_MAX_REDUCERS = 8
 
@@ -67,6 +68,8 @@ SIGNATURES = {
                                        I64, I64, I, P, P, P, P, P, P, P, I64, I, P]),
     "cl_propagate_ragged": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I64, I, I, I, P, P, I,
                                 P, P, P, P, P, P]),
+    "cl_record_rows": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I64, I, I, I, I, P, P, P, P, P,
+                           P]),
     "cl_initial_deviations": (I, [ctypes.c_uint64, I64, I64, I, I, P]),
     "cl_initial_deviations_device": (I, [P, ctypes.c_uint64, I64, I64, I, I, P, P]),
     "cl_math_host": (None, [I64, P, P, P, P]),

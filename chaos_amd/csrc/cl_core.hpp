@@ -1142,4 +1142,23 @@ struct Monitor {
   }
 };
 
+// RunMonitor(sys) (src/Experiments/run_record.jl:25-43): the recorder's non-raising monitor, running maxima of
+// |det M - 1|, max |M' Omega M - Omega| and max |M| of the one-step tangent at each handoff; it never fails.
+struct RunMonState {
+  double det_max, symp_max, M_max;
+};
+template <int D>
+struct RunMon {
+  typedef RunMonState State;
+  CL_HD void init(State& st) const { st.det_max = st.symp_max = st.M_max = 0.0; }
+  CL_HD void step(State& st, int64_t, const double* J) const {
+    double mx = cl_fabs(J[0]);
+#pragma unroll
+    for (int e = 1; e < D * D; ++e) mx = cl_jmax(mx, cl_fabs(J[e]));
+    st.det_max = cl_jmax(st.det_max, cl_fabs(det_of<D>(J) - 1.0));
+    st.symp_max = cl_jmax(st.symp_max, symp_residual<D>(J));
+    st.M_max = cl_jmax(st.M_max, mx);
+  }
+};
+
 }  // namespace cl

@@ -42,6 +42,14 @@ typedef Cons<S_<OFLI<2, 1>>, Nil> C1_ofli;
 typedef Cons<S_<Monitor<2, 2>>, Nil> C_mon;
 typedef Cons<QR2(QR_MGS, 0), Cons<S_<Monitor<2, 2>>, Nil>> C_mgs_mon;
 typedef Cons<S_<Monitor<2, 1>>, Nil> C1_mon;
+// the recorder's chains (an estimator, then the non-raising RunMonitor): methods L3, L4, L4b, L5, L5b (k = 2) and
+// L1 (Benettin, k = 1) (src/Experiments/run_record.jl:49-53)
+typedef Cons<QR2(QR_CGS, 0), Cons<S_<RunMon<2>>, Nil>> C_rec_cgs;
+typedef Cons<QR2(QR_MGS, 0), Cons<S_<RunMon<2>>, Nil>> C_rec_mgs;
+typedef Cons<QR2(QR_RGS, 0), Cons<S_<RunMon<2>>, Nil>> C_rec_rgs;
+typedef Cons<QR2(QR_HOUSEHOLDER, 0), Cons<S_<RunMon<2>>, Nil>> C_rec_hh;
+typedef Cons<QR2(QR_AUGMENTED, 0), Cons<S_<RunMon<2>>, Nil>> C_rec_aug;
+typedef Cons<S_<Benettin<2>>, Cons<S_<RunMon<2>>, Nil>> C1_rec_ben;
 
 #define ADD2(SK, SYS, CH, NF) if (*n < cap) out[(*n)++] = make_instance<SK, SYS, 2, NF, CH>();
 #define ADD1(SK, SYS, CH) if (*n < cap) out[(*n)++] = make_instance<SK, SYS, 1, 1, CH>();
@@ -54,7 +62,8 @@ typedef Cons<S_<Monitor<2, 1>>, Nil> C1_mon;
   ADD2(SK, SYS, C_mfs_hh, 2) ADD2(SK, SYS, C_mon, 1) ADD2(SK, SYS, C_mgs_mon, 1)             \
   ADD1(SK, SYS, C_none) ADD1(SK, SYS, C_ben) ADD1(SK, SYS, C1_cgs) ADD1(SK, SYS, C1_mgs)        \
   ADD1(SK, SYS, C1_rgs) ADD1(SK, SYS, C1_hh) ADD1(SK, SYS, C1_aug) ADD1(SK, SYS, C1_giv)        \
-  ADD1(SK, SYS, C1_ofli) ADD1(SK, SYS, C1_mon)
+  ADD1(SK, SYS, C1_ofli) ADD1(SK, SYS, C1_mon) ADD2(SK, SYS, C_rec_cgs, 1) ADD2(SK, SYS, C_rec_mgs, 1)          \
+  ADD2(SK, SYS, C_rec_rgs, 1) ADD2(SK, SYS, C_rec_hh, 1) ADD2(SK, SYS, C_rec_aug, 1) ADD1(SK, SYS, C1_rec_ben)
 
 // the surrogate workload on the per-orbit-K map: the traced prefix (MEGNO + FLI on frame 1, the QR spectrum
 // on frame 2), the continuation and the plain method (the QR spectrum alone)

@@ -126,6 +126,13 @@ CL_HD void slot_step(Slot<Monitor<D, K>, FR>& s, int64_t n, Ctx& c) {
   s.r.step(s.st, n, J, c.W[FR]);
 }
 
+template <int D, int FR, class Ctx>
+CL_HD void slot_step(Slot<RunMon<D>, FR>& s, int64_t n, Ctx& c) {
+  double J[D * D];
+  c.sys.jacobian(c.aux, J);
+  s.r.step(s.st, n, J);
+}
+
 // Reducer parameters from the descriptor.
 template <int ALG, int D, int K>
 CL_HD void slot_setup(QRSpectrum<ALG, D, K>& r, const cl_reducer& d) { r.tol = d.tol; }
@@ -141,6 +148,8 @@ template <int D, int K>
 CL_HD void slot_setup(OFLI<D, K>& r, const cl_reducer& d) { r.replace = d.replace != 0; }
 template <int D, int K>
 CL_HD void slot_setup(GALI<D, K>&, const cl_reducer&) {}
+template <int D>
+CL_HD void slot_setup(RunMon<D>&, const cl_reducer&) {}
 template <int D, int K>
 CL_HD void slot_setup(Monitor<D, K>& r, const cl_reducer& d) {
   r.det_tol = d.tol;
@@ -167,10 +176,15 @@ CL_HD void chain_step(Cons<H, T>& c, int64_t n, Ctx& ctx) {
   slot_step(c.head, n, ctx);
   chain_step(c.tail, n, ctx);
 }
+template <class S>
+CL_HD bool state_failed(const S& s) {
+  return s.code != 0;
+}
+CL_HD bool state_failed(const RunMonState&) { return false; }
 CL_HD bool chain_failed(const Nil&) { return false; }
 template <class H, class T>
 CL_HD bool chain_failed(const Cons<H, T>& c) {
-  return c.head.st.code != 0 || chain_failed(c.tail);
+  return state_failed(c.head.st) || chain_failed(c.tail);
 }
 template <class C>
 struct ChainWords;

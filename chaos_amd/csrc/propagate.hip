@@ -484,6 +484,30 @@ static int launch_best(const Instance* in, const cl_system* sys, const cl_chain*
   return launch(in, sys, ch, m, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st);
 }
 
+namespace cl {
+int record_rows_launch(int64_t m, int s, int k, int benettin, int g, int G, const double* x, const uint64_t* st,
+                       const int8_t* code, int64_t* fail, double* R, hipStream_t stream);
+}
+
+// Row g of the recorder's array after the launch that ends at grid step g (src/Experiments/run_device.jl:9-37).
+extern "C" int cl_record_rows(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int64_t m, int s, int k, int g,
+                              int G, const double* dx, const uint64_t* dst, const int8_t* dcode, int64_t* dfail,
+                              double* dR, void* stream) {
+  if (!ctx || !sys || !ch) return CL_ERR_ARGUMENT;
+  const bool est_ok = ch->n == 2 && ch->r[1].kind == CL_RED_RUNMONITOR && ch->r[0].frame == 0 &&
+                      ch->r[1].frame == 0 &&
+                      ((ch->r[0].kind == CL_RED_QR && (k == 1 || k == 2)) || (ch->r[0].kind == CL_RED_BENETTIN && k == 1));
+  if (!est_ok) return set_err(ctx, CL_ERR_UNSUPPORTED, "the recorder's chain is (QRSpectrum or Benettin, RunMonitor)");
+  const bool sys_ok = (sys->kind == CL_SYS_STANDARD_MAP && s == 2) || (sys->kind == CL_SYS_WRAPPED_STANDARD_MAP && s == 4);
+  if (!sys_ok) return set_err(ctx, CL_ERR_UNSUPPORTED, "the recorder runs the standard map, lifted or wrapped");
+  if (m < 0 || G < 1 || g < 0 || g >= G) return set_err(ctx, CL_ERR_ARGUMENT, "grid row out of range");
+  if (m > 0 && (!dx || !dst || !dcode || !dfail || !dR)) return set_err(ctx, CL_ERR_ARGUMENT, "null buffer");
+  hipSetDevice(ctx->device);
+  int rc = record_rows_launch(m, s, k, ch->r[0].kind == CL_RED_BENETTIN, g, G, dx, dst, dcode, dfail, dR,
+                              stream ? (hipStream_t)stream : ctx->stream);
+  return rc ? set_err(ctx, rc, "record kernel launch failed") : CL_OK;
+}
+
 static int chain_device(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m, int s, int d,
                         int k, int64_t n0, int64_t steps, int64_t N, int tau, double* dx, double* dW, uint64_t* dst,
                         int8_t* dcode, int64_t* dhstep, hipStream_t st) {

@@ -95,6 +95,10 @@ template <int D, int K>
 struct Kind<Monitor<D, K>> {
   static constexpr int kind = CL_RED_MONITOR, alg = 0;
 };
+template <int D>
+struct Kind<RunMon<D>> {
+  static constexpr int kind = CL_RED_RUNMONITOR, alg = 0;
+};
 
 inline void fill_key(Instance&, const Nil&, int) {}
 template <class H, class T>

@@ -1,0 +1,207 @@
+"""The recorder's device path (src/Experiments/run_device.jl, src/Experiments/run_record.jl): grid records of an
+ensemble of standard-map orbits.
+
+Each orbit runs with the chain (estimator, RunMonitor) in segments that end at the grid steps; after each segment
+the row of the grid step holds the estimator's exponents, the monitor's det and symplecticity maxima, the
+orthogonality maximum, the running maxima of the sum and pairing residuals (k = 2), G_max, M_max and the point
+(`_dev_row!`, run_device.jl:9-37). The segments are launches of the propagate kernels, so a row equals the final
+values of the same estimator run directly to that step, bit for bit; the rows are written on the device by
+`cl_record_rows`. An orbit whose segment fails records the 1-based failing row and no further rows.
+"""
+import ast
+import ctypes
+import math
+from dataclasses import dataclass
+from fractions import Fraction
+
+import mpmath as mp
+import numpy as np
+
+from . import _lib as L
+from .reducers import QRSpectrum, Benettin, CGS, MGS, RGS, HouseholderQR, AugmentedQR, RunMonitor
+from .systems import StandardMap, WrappedStandardMap
+
+# method codes with a device path (run_record.jl:47-53); L2 (two trajectories), L6 (LAPACK QR) and L9 (SVD) run on
+# the reference's CPU path
+QR_METHODS = {"L3": CGS, "L4": MGS, "L4b": RGS, "L5": HouseholderQR, "L5b": AugmentedQR}
+METHODS = ("L1",) + tuple(QR_METHODS)
+
+
+def estimator(method, orthogonality_tol=8.0):
+    """The estimator of a method code (_estimator, run_record.jl:49-53)."""
+    if method == "L1":
+        return Benettin()
+    if method in QR_METHODS:
+        return QRSpectrum(QR_METHODS[method](), orthogonality_tol)
+    raise ValueError(f"method {method!r} has no device path (two-trajectory L2, LAPACK QR L6 and SVD L9 do not)")
+
+
+def vectors(method):
+    """Deviation vectors of a method: 1 for Benettin, 2 for the QR spectra."""
+    return 1 if method == "L1" else 2
+
+
+def record_grid(N, tau, *, points_per_decade, first_steps, extra_fractions):
+    """Steps at which an item of horizon N is recorded (record_grid, src/Experiments/run_grid.jl:18-38): every step
+    1..first_steps, round(10^(j / points_per_decade)) evaluated in 256 bits and rounded half up, round(f N) for each
+    extra fraction read as its exact decimal, and N; each moved up to the handoff tau ceil(n / tau); sorted, unique."""
+    N, tau = int(N), int(tau)
+    targets = set(range(1, min(int(first_steps), N) + 1))
+    with mp.workprec(256):
+        j = 0
+        while True:
+            v = mp.mpf(10) ** (mp.mpf(j) / int(points_per_decade))
+            if v > N:
+                break
+            targets.add(int(mp.floor(v + mp.mpf(1) / 2)))
+            j += 1
+    for f in extra_fractions:
+        targets.add(math.floor(Fraction(str(f)) * N + Fraction(1, 2)))
+    targets.add(N)
+    return sorted({tau * -(-t // tau) for t in targets})
+
+
+def _eval_angle(text):
+    """A decimal or closed-form angle text ("pi/4", "0.3 + pi/5", "2pi") in 256-bit arithmetic: numerals read at
+    that precision, `pi`, + - * / and parentheses (_eval_exact, run_grid.jl:57-77)."""
+    import re
+    src = re.sub(r"(\d)pi", r"\1*pi", str(text))
+
+    def ev(node):
+        if isinstance(node, ast.Expression):
+            return ev(node.body)
+        if isinstance(node, ast.Constant) and isinstance(node.value, (int, float)):
+            seg = ast.get_source_segment(src, node)
+            return mp.mpf(seg)
+        if isinstance(node, ast.Name) and node.id == "pi":
+            return +mp.pi
+        if isinstance(node, ast.UnaryOp) and isinstance(node.op, (ast.USub, ast.UAdd)):
+            v = ev(node.operand)
+            return -v if isinstance(node.op, ast.USub) else v
+        if isinstance(node, ast.BinOp) and isinstance(node.op, (ast.Add, ast.Sub, ast.Mult, ast.Div)):
+            a, b = ev(node.left), ev(node.right)
+            return {ast.Add: a + b, ast.Sub: a - b, ast.Mult: a * b, ast.Div: a / b}[type(node.op)]
+        raise ValueError(f"unsupported angle text {text!r}")
+    return ev(ast.parse(src, mode="eval"))
+
+
+def deviation_frame(direction, k):
+    """The 2 x k start frame of a direction (deviation_frame, run_grid.jl:90-104): first column (cos a, sin a) with
+    a evaluated in 256 bits, second column (k = 2) its counterclockwise quarter turn (-sin a, cos a)."""
+    if k not in (1, 2):
+        raise ValueError(f"deviation_frame: k must be 1 or 2, got {k}")
+    with mp.workprec(256):
+        a = _eval_angle(direction)
+        c, s = float(mp.cos(a)), float(mp.sin(a))
+    return np.array([[c], [s]]) if k == 1 else np.array([[c, -s], [s, c]])
+
+
+@dataclass
+class Records:
+    """Record array R[orbit, grid row, column] (columns: lambda_1..k, det, symp, orth, sum, pair, G_max, M_max, then
+    the point), the 1-based failing row per orbit (0: none) and the final carried data."""
+    method: str
+    grid: np.ndarray
+    R: np.ndarray
+    fail: np.ndarray
+    code: np.ndarray
+    hstep: np.ndarray
+    x: np.ndarray
+    W: np.ndarray
+    states: np.ndarray
+    k: int
+    s: int
+
+    def unpack(self, j):
+        """The record buffer of orbit j (_unpack! and _blank_failed!, run_device.jl:123-139, run_record.jl:251-261):
+        rows before the failing row, NaN from it on; the G_max and M_max of the last good row (1.0 when none)."""
+        k, s, G = self.k, self.s, len(self.grid)
+        f = int(self.fail[j])
+        last = G if f == 0 else f - 1
+        Rj = self.R[j]
+        nan = float("nan")
+        out = dict(t=np.zeros(G, np.int64), lyap=np.full((G, k), nan), state=np.full((G, 2), nan),
+                   det=np.full(G, nan), symp=np.full(G, nan), orth=np.full(G, nan), sum=np.full(G, nan),
+                   pair=np.full(G, nan), gmax=1.0, mmax=1.0, failrow=f,
+                   winding=np.zeros((G, 2), np.int64) if s == 4 else None)
+        for g in range(last):
+            out["t"][g] = self.grid[g]
+            out["lyap"][g] = Rj[g, :k]
+            out["det"][g], out["symp"][g], out["orth"][g] = Rj[g, k], Rj[g, k + 1], Rj[g, k + 2]
+            out["sum"][g], out["pair"][g] = Rj[g, k + 3], Rj[g, k + 4]
+            out["state"][g] = Rj[g, k + 7:k + 9]
+            if s == 4:
+                out["winding"][g] = Rj[g, k + 9:k + 11].astype(np.int64)
+        if last >= 1:
+            out["gmax"], out["mmax"] = float(Rj[last - 1, k + 5]), float(Rj[last - 1, k + 6])
+        return out
+
+
+def record(sys, x0, N, *, method, tau=1, grid=None, W0=None, direction="pi/4", orthogonality_tol=8.0,
+           launch_steps=None):
+    """Record an ensemble on the GPU: `sys` a StandardMap (lifted, x0 of shape (m, 2)) or WrappedStandardMap
+    (x0 of shape (m, 4), torus_state form); `grid` the record steps (default: the study grid with 20 points per
+    decade, every step to 100 and the fractions 0.1 and 0.5); the start frame `W0` (2 x k, or one per orbit) or the
+    `direction` text of deviation_frame. Launches are split at `launch_steps` (a multiple of tau) when given."""
+    import torch
+    from .tangent import make_chain, _context
+    if not isinstance(sys, (StandardMap, WrappedStandardMap)):
+        raise ValueError("the recorder's device path runs the standard map, lifted or wrapped")
+    N, tau = int(N), int(tau)
+    k = vectors(method)
+    if grid is None:
+        grid = record_grid(N, tau, points_per_decade=20, first_steps=100, extra_fractions=(0.1, 0.5))
+    grid = np.asarray(grid, dtype=np.int64)
+    G = len(grid)
+    if G == 0 or grid[-1] != N or np.any(grid % tau) or np.any(np.diff(grid) <= 0) or grid[0] <= 0:
+        raise ValueError("grid must be increasing positive multiples of tau ending at N")
+    if launch_steps is not None and (launch_steps < 1 or launch_steps % tau):
+        raise ValueError("launch_steps must be a positive multiple of tau")
+    x0 = np.ascontiguousarray(np.asarray(x0, dtype=np.float64))
+    m, s = x0.shape
+    if s != sys.state_length:
+        raise ValueError(f"state length {s} does not match {sys!r}")
+    if W0 is None:
+        W0 = deviation_frame(direction, k)
+    W0 = np.asarray(W0, dtype=np.float64)
+    if W0.shape == (2, k):
+        W0 = np.broadcast_to(W0, (m, 2, k))
+    if W0.shape != (m, 2, k):
+        raise ValueError("W0 must be 2 x k or one 2 x k frame per orbit")
+    chain, allr = make_chain((estimator(method, orthogonality_tol), RunMonitor(sys)))
+    lib, ctx = _context()
+    sysd = sys.descriptor()
+    nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, 2, k)
+    if nbytes < 0:
+        raise L.HIPError(-nbytes, "no kernel instance for this method")
+    words = nbytes // 8
+    ncol = k + 7 + s
+    dev = torch.device("cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    dx = torch.from_numpy(np.ascontiguousarray(x0.T)).to(dev)
+    dW = torch.from_numpy(np.ascontiguousarray(W0.transpose(2, 1, 0).reshape(2 * k, m))).to(dev)
+    dst = torch.zeros((words, m), dtype=torch.int64, device=dev)
+    dcode = torch.zeros(m, dtype=torch.int8, device=dev)
+    dh = torch.zeros(m, dtype=torch.int64, device=dev)
+    dfail = torch.zeros(m, dtype=torch.int64, device=dev)
+    dR = torch.zeros((m, G, ncol), dtype=torch.float64, device=dev)
+    n = 0
+    for g in range(G):
+        target = int(grid[g])
+        while n < target:
+            steps = target - n if launch_steps is None else min(int(launch_steps), target - n)
+            rc = lib.cl_propagate_device(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, m, s, 2, k, n,
+                                         steps, N, tau, dx.data_ptr(), dW.data_ptr(), None, dst.data_ptr(),
+                                         dcode.data_ptr(), dh.data_ptr(), stream)
+            if rc:
+                raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
+            n += steps
+        rc = lib.cl_record_rows(ctx, ctypes.byref(sysd), ctypes.byref(chain), m, s, k, g, G, dx.data_ptr(),
+                                dst.data_ptr(), dcode.data_ptr(), dfail.data_ptr(), dR.data_ptr(), stream)
+        if rc:
+            raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
+    torch.cuda.synchronize()
+    raw = dst.cpu().numpy().T.copy().view(np.uint8).reshape(m, words * 8)
+    return Records(method=method, grid=grid, R=dR.cpu().numpy(), fail=dfail.cpu().numpy(), code=dcode.cpu().numpy(),
+                   hstep=dh.cpu().numpy(), x=dx.cpu().numpy().T.copy(),
+                   W=dW.cpu().numpy().reshape(k, 2, m).transpose(2, 1, 0).copy(), states=raw, k=k, s=s)

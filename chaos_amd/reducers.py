@@ -412,3 +412,19 @@ def monitor_exponents(st, lam, G, n, *, pairing_tol, tau, M_max, step, orbit0=1)
     st["G_max"] = G_max
     st["M_max"] = M_hi
     return st
+
+
+class RunMonitor(Reducer):
+    """RunMonitor(sys) (src/Experiments/run_record.jl:25-43): the recorder's non-raising monitor, the running
+    maxima of |det M - 1|, max |M' Omega M - Omega| and max |M| of the one-step tangent at each handoff. It never
+    fails."""
+    kind = L.CL_RED_RUNMONITOR
+
+    def __init__(self, sys):
+        self.sys = sys
+
+    def dtype(self, d, k):
+        return np.dtype([("det_max", "<f8"), ("symp_max", "<f8"), ("M_max", "<f8")])
+
+    def final(self, st):
+        return dict(det_max=st["det_max"].copy(), symp_max=st["symp_max"].copy(), M_max=st["M_max"].copy())

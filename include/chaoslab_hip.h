@@ -77,6 +77,7 @@ typedef struct {
 #define CL_RED_OFLI 6      /* OFLI(flow; replace), src/Indicators/ofli.jl:23-126 */
 #define CL_RED_GALI 7      /* GALI(), src/Indicators/gali.jl:190-374 */
 #define CL_RED_MONITOR 9   /* ResidualMonitor(sys, det_tol, orth_tol), src/Diagnostics/monitor.jl:9-54 */
+#define CL_RED_RUNMONITOR 10 /* RunMonitor(sys), the recorder's non-raising monitor, src/Experiments/run_record.jl:25-43 */
 
 #define CL_QR_CGS 1
 #define CL_QR_MGS 2
@@ -165,6 +166,16 @@ int cl_propagate_device_traced(cl_ctx *ctx, const cl_system *sys, const cl_chain
 int cl_propagate_ragged(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain, int elt,
                         int64_t m, int s, int d, int k, const int64_t *Ns, const int64_t *order, int tau,
                         double *dx, double *dW, uint64_t *dstates, int8_t *dcode, int64_t *dhstep, void *stream);
+
+/* The recorder's rows (replaces the row writes of _record_kernel!, src/Experiments/run_device.jl:9-37,41-72):
+ * after the launch (cl_propagate_device) that ends at grid step g of G, row g of the orbit-major record array
+ * dR[m][G][k + 7 + s] is written for every orbit still running: lambda (k), det_max, symp_max, orth, the running
+ * maxima of the sum and pairing residuals (k = 2; zero columns for k = 1), G_max, M_max, the point (s = 2 lifted,
+ * 4 wrapped). An orbit whose launch failed gets dfail[i] = g + 1 and no further rows; dfail and dR start zeroed.
+ * The chain is (QRSpectrum or Benettin, RunMonitor) on a standard map; device buffers; asynchronous. */
+int cl_record_rows(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain, int64_t m, int s, int k, int g,
+                   int G, const double *dx, const uint64_t *dstates, const int8_t *dcode, int64_t *dfail,
+                   double *dR, void *stream);
 
 /* Seeded orthonormal frames, replacing initial_deviations(seed, orbit, Val(d), Val(k), T)
  * (src/Tangent/initial.jl:27-34): orbit ids orbit0 .. orbit0+m-1 (1-based ids as the reference),

@@ -165,6 +165,7 @@ static int red_bytes(const cl_reducer& r, int d, int k) {
     case CL_RED_GALI: return 24 * k + 32;
     case 8: return 16 * k;  // LogNorm
     case CL_RED_MONITOR: return 88;
+    case CL_RED_RUNMONITOR: return 24;
   }
   return -1;
 }
@@ -269,6 +270,12 @@ static void write_state(const cl_reducer& r, const RState<T>& s, int d, int k, u
       b.put(dbl(q.value));
       b.put(dbl(q.threshold));
       b.put(q.step);
+      break;
+    }
+    case CL_RED_RUNMONITOR: {
+      b.put(dbl(s.mon.det_max));
+      b.put(dbl(s.mon.symp_max));
+      b.put(dbl(s.mon.M_max));
       break;
     }
     case CL_RED_QR: {
@@ -896,6 +903,7 @@ template <class T>
 static bool red_failed(const cl_reducer& r, const RState<T>& s) {
   switch (r.kind) {
     case CL_RED_MONITOR: return s.mon.code != 0;
+    case CL_RED_RUNMONITOR: return false;
     case CL_RED_QR: return s.qr.code != 0;
     case CL_RED_BENETTIN: return s.ben.code != 0;
     case CL_RED_MEGNO: return s.meg.code != 0;
@@ -1257,11 +1265,37 @@ static void red_monitor(const cl_reducer& r, MonSt<T>& st, int64_t n, const SYS&
   st.step = n;
 }
 
+// RunMonitor (src/Experiments/run_record.jl:37-40): running maxima of |det M - 1|, the symplectic residual and
+// max |M| of the one-step tangent; never fails
+template <class P, class T, class SYS>
+static void red_runmonitor(MonSt<T>& st, const SYS& sys, const T* x, int d) {
+  T J[16];
+  sys.jac(x, J);
+  T mx = P::abs(J[0]);
+  for (int e = 1; e < d * d; ++e) mx = jmax<P>(mx, P::abs(J[e]));
+  st.det_max = jmax<P>(st.det_max, P::abs(detn(J, d) - (T)1));
+  st.symp_max = jmax<P>(st.symp_max, symp_res<P>(J, d));
+  st.M_max = jmax<P>(st.M_max, mx);
+}
+
 // ------------------------------------------------------------------ the per-orbit body
+template <class P, class T, class SYS>
+static int8_t run_orbit_seg(const SYS& sys, const cl_chain& ch, int d, int k, int64_t n0, int64_t steps, int64_t N,
+                            int tau, T* x, T* W0, T* W1, RState<T>* st, int64_t* hstep);
+
+// N steps from fresh reducer states
 template <class P, class T, class SYS>
 static int8_t run_orbit(const SYS& sys, const cl_chain& ch, int d, int k, int64_t N, int tau, T* x,
                         T* W0, T* W1, RState<T>* st, int64_t* hstep) {
   for (int r = 0; r < ch.n; ++r) init_state(ch.r[r], st[r], k);
+  return run_orbit_seg<P>(sys, ch, d, k, 0, N, N, tau, x, W0, W1, st, hstep);
+}
+
+// One launch (orbit.jl:44-60): `steps` steps from global step n0 with the carried point, frames and reducer
+// states; the final status check runs only at step N.
+template <class P, class T, class SYS>
+static int8_t run_orbit_seg(const SYS& sys, const cl_chain& ch, int d, int k, int64_t n0, int64_t steps, int64_t N,
+                            int tau, T* x, T* W0, T* W1, RState<T>* st, int64_t* hstep) {
   T* W[2] = {W0, W1};
   int nf = 1;
   for (int r = 0; r < ch.n; ++r)
@@ -1274,8 +1308,8 @@ static int8_t run_orbit(const SYS& sys, const cl_chain& ch, int d, int k, int64_
         if (!P::finite(W[f][i])) return 2;
     return 0;
   };
-  int64_t n = 0;
-  for (int64_t h = 0; h < N / tau; ++h) {
+  int64_t n = n0;
+  for (int64_t h = 0; h < steps / tau; ++h) {
     for (int t = 0; t < tau; ++t) sys.step(x, W, nf, k);
     n += tau;
     int8_t code = status();
@@ -1315,6 +1349,7 @@ static int8_t run_orbit(const SYS& sys, const cl_chain& ch, int d, int k, int64_
         case CL_RED_GALI: red_gali<P>(st[r].gali, n, w, d, k); break;
         case 8: red_lognorm<P>(st[r].ln, w, d, k); break;
         case CL_RED_MONITOR: red_monitor<P>(rd, st[r].mon, n, sys, x, w, d, k); break;
+        case CL_RED_RUNMONITOR: red_runmonitor<P>(st[r].mon, sys, x, d); break;
       }
     }
     for (int r = 0; r < ch.n; ++r)
@@ -1324,7 +1359,7 @@ static int8_t run_orbit(const SYS& sys, const cl_chain& ch, int d, int k, int64_
       }
   }
   *hstep = n;
-  return status();
+  return n == N ? status() : (int8_t)0;
 }
 
 template <class P>
@@ -1376,6 +1411,100 @@ static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, in
     g_gauss_iters += tl_gauss_iters;
     g_gauss_steps += tl_gauss_steps;
     tl_gauss_iters = tl_gauss_steps = 0;
+  };
+  if (nthreads <= 1) {
+    worker();
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ the recorder (src/Experiments/run_record.jl)
+// Each orbit runs in segments of the per-orbit body to the grid steps (_record_loop!, run_record.jl:160-172); after
+// a successful segment, row g of R[orbit][g][col] is the row of _dev_row! (run_device.jl:9-37): lambda (k), the
+// RunMonitor's det and symp maxima, orth (the QR state's orth_max, 0 for Benettin), for k = 2 the running maxima of
+// |lambda_1 + lambda_2| (eps_sum and eps_pair coincide at k = 2), G_max (1 for Benettin), M_max, then the point. A
+// failing segment sets fail = g + 1 (1-based grid row) and ends the orbit's rows. The chain is (estimator,
+// RunMonitor) with the estimator a QR spectrum (k = 1, 2) or Benettin (k = 1).
+template <class P>
+static void record_row(const cl_chain* ch, int k, int s, int g, const RState<double>* st, const double* x, double* row,
+                       const double* prev) {
+  double lam[2] = {0.0, 0.0}, orth = 0.0, gmax = 1.0;
+  if (ch->r[0].kind == CL_RED_QR) {
+    const QRSt<double>& q = st[0].qr;
+    for (int j = 0; j < k; ++j) lam[j] = q.n == 0 ? 0.0 : (q.sums[j].s + q.sums[j].c) / (double)q.n;
+    orth = q.orth_max;
+    gmax = q.G_max;
+  } else {
+    const BenSt<double>& b = st[0].ben;
+    lam[0] = b.n == 0 ? 0.0 : (b.sum.s + b.sum.c) / (double)b.n;
+  }
+  const MonSt<double>& mon = st[1].mon;
+  for (int j = 0; j < k; ++j) row[j] = lam[j];
+  row[k] = mon.det_max;
+  row[k + 1] = mon.symp_max;
+  row[k + 2] = orth;
+  if (k == 2) {
+    const double e = P::abs(lam[0] + lam[1]);
+    row[k + 3] = g > 0 ? jmax<P>(prev[k + 3], e) : e;
+    row[k + 4] = g > 0 ? jmax<P>(prev[k + 4], e) : e;
+  }
+  row[k + 5] = gmax;
+  row[k + 6] = mon.M_max;
+  for (int j = 0; j < s; ++j) row[k + 7 + j] = x[j];
+}
+
+template <class P>
+static int record_all(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int k, int64_t N, int tau,
+                      const int64_t* grid, int G, double* x, double* W, double* R, int64_t* fail, int8_t* code,
+                      int64_t* hstep, uint8_t* states, int nthreads) {
+  const int d = 2, ncol = k + 7 + s;
+  MapSys<P, double> ms;
+  ms.kind = sys->kind;
+  ms.K = sys->K;
+  ms.s = s;
+  for (int i = 0; i < 4; ++i) ms.A[i] = 0.0;
+  int sb = 0, off[CL_MAX_REDUCERS];
+  for (int r = 0; r < ch->n; ++r) {
+    off[r] = sb;
+    sb += red_bytes(ch->r[r], d, k);
+  }
+  std::atomic<int64_t> next(0);
+  auto worker = [&]() {
+    RState<double> st[2];
+    while (true) {
+      int64_t lo = next.fetch_add(16);
+      if (lo >= m) break;
+      int64_t hi = lo + 16 < m ? lo + 16 : m;
+      for (int64_t i = lo; i < hi; ++i) {
+        double xx[4], w0[4], w1[4] = {0, 0, 0, 0};
+        for (int c = 0; c < s; ++c) xx[c] = x[i * s + c];
+        for (int e = 0; e < d * k; ++e) w0[e] = W[i * d * k + e];
+        for (int r = 0; r < ch->n; ++r) init_state(ch->r[r], st[r], k);
+        int64_t n = 0, h = 0;
+        int8_t c = 0;
+        fail[i] = 0;
+        for (int g = 0; g < G; ++g) {
+          c = run_orbit_seg<P>(ms, *ch, d, k, n, grid[g] - n, N, tau, xx, w0, w1, st, &h);
+          if (c != 0) {
+            fail[i] = g + 1;
+            break;
+          }
+          n = grid[g];
+          double* row = R + ((size_t)i * G + g) * ncol;
+          record_row<P>(ch, k, s, g, st, xx, row, g > 0 ? row - ncol : nullptr);
+        }
+        code[i] = c;
+        hstep[i] = h;
+        for (int cc = 0; cc < s; ++cc) x[i * s + cc] = xx[cc];
+        for (int e = 0; e < d * k; ++e) W[i * d * k + e] = w0[e];
+        if (states)
+          for (int r = 0; r < ch->n; ++r) write_state(ch->r[r], st[r], d, k, states + i * sb + off[r]);
+      }
+    }
   };
   if (nthreads <= 1) {
     worker();
@@ -1485,6 +1614,23 @@ int or_propagate(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int
   if (math == MATH_CL)
     return propagate_all<F64Cl>(sys, ch, m, s, d, k, N, tau, x, W, W2, (uint8_t*)states, code, hstep, nthreads);
   return propagate_all<F64Libm>(sys, ch, m, s, d, k, N, tau, x, W, W2, (uint8_t*)states, code, hstep, nthreads);
+}
+
+// The recorder on a standard map (lifted, s = 2) or a wrapped standard map (s = 4) with the chain (estimator,
+// RunMonitor): grid[0..G) increasing multiples of tau ending at N; R (m x G x (k + 7 + s), zeroed by the caller) gets
+// the rows, fail[i] the 1-based failing grid row or 0; x, W, code, hstep and states as in or_propagate.
+int or_record(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int k, int64_t N, int tau,
+              const int64_t* grid, int G, double* x, double* W, double* R, int64_t* fail, int8_t* code,
+              int64_t* hstep, void* states, int math, int nthreads) {
+  const bool est_ok = ch->n == 2 && ch->r[1].kind == CL_RED_RUNMONITOR &&
+                      ((ch->r[0].kind == CL_RED_QR && (k == 1 || k == 2)) || (ch->r[0].kind == CL_RED_BENETTIN && k == 1));
+  const bool sys_ok = (sys->kind == CL_SYS_STANDARD_MAP && s == 2) || (sys->kind == CL_SYS_WRAPPED_STANDARD_MAP && s == 4);
+  if (!est_ok || !sys_ok || N % tau || G < 1 || grid[G - 1] != N) return 10;
+  for (int g = 0; g < G; ++g)
+    if (grid[g] % tau || grid[g] <= (g ? grid[g - 1] : 0)) return 10;
+  if (math == MATH_CL)
+    return record_all<F64Cl>(sys, ch, m, s, k, N, tau, grid, G, x, W, R, fail, code, hstep, (uint8_t*)states, nthreads);
+  return record_all<F64Libm>(sys, ch, m, s, k, N, tau, grid, G, x, W, R, fail, code, hstep, (uint8_t*)states, nthreads);
 }
 
 // A linear map x' = A x (A column-major 2 x 2): the closed-form fixtures of the reference's suites
