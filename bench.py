@@ -19,6 +19,9 @@ triage (256-step prefix, network, full method only where |logit| is below a marg
 set). `value` is the triaged rate in orbit-steps/s (4M x 10^5 over the triage wall time); the line also
 carries the speedup and the misclassified fraction against the plain labels.
 
+`--workload rec`: the recorder's device path (grid records) on the reference's cost ensemble: 2048 standard-map
+orbits x 2e5 steps per method (L1, L3, L4, L4b, L5, L5b), lifted, tau = 1, the study grid.
+
 Prints one JSON line with the metric, the roofline of the orbit kernel (HIP events on the launch stream)
 and the CPU baseline (the oracle, glibc math, on the host's cores, on a bounded sample of the workload).
 """
@@ -402,12 +405,89 @@ def c5_main(a):
     print(json.dumps(out), flush=True)
 
 
+# The recorder's cost as the reference measures it (verify/experiments-device.jl:37-58): 2048 orbits x 2e5 steps per
+# method, lifted, tau 1, the study grid; ns per orbit-step on its CPU threads and its GPU
+# (workspaces/module-build/stages/04-verify/output/gpu-rtx5060-d53-verify-log.txt:168-175).
+REC_REFERENCE_NS = {"L1": (4.173, 12.454), "L3": (7.906, 25.252), "L4": (8.594, 25.120), "L4b": (8.732, 35.920),
+                    "L5": (9.171, 27.082), "L5b": (9.804, 27.927)}
+REC_MONITOR_FLOPS = 47  # RunMonitor per handoff: det (5), M' Omega M - Omega and its maximum (35), max |M| (7)
+
+
+def rec_main(a):
+    """--workload rec: the recorder's device path (src/Experiments/run_device.jl) on the reference's cost ensemble:
+    each method's grid records of 2048 standard-map orbits (K = 6, starts uniform on the torus) over 2e5 steps, lifted,
+    tau = 1, the study grid (20 points per decade, every step to 100, fractions 0.1 and 0.5), frame direction pi/4.
+    One step = one method's full recording; value = orbit-steps over all methods / wall time (the rows copied back to
+    the host included)."""
+    import torch
+    from chaos_amd import StandardMap, RunMonitor
+    from chaos_amd.recorder import record, record_grid, deviation_frame, estimator, vectors, METHODS
+    from chaos_amd.inputs import torus_uniform
+    m, N = [int(v) for v in a.rec_sizes.split(",")]
+    sysm = StandardMap(6.0)
+    x0 = torus_uniform(1, m)
+    grid = record_grid(N, 1, points_per_decade=20, first_steps=100, extra_fractions=(0.1, 0.5))
+    log(f"recorder: {m} orbits x {N} steps, {len(grid)} grid rows, methods {', '.join(METHODS)}")
+    record(sysm, x0[:64], 200, method="L4", tau=1)  # warm-up: loads the kernels
+    per, wall, dev_ms, bad = {}, 0.0, {}, 0
+    for meth in METHODS:
+        tm = {}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rec = record(sysm, x0, N, method=meth, tau=1, grid=grid, timing=tm)
+        dt = time.perf_counter() - t0
+        wall += dt
+        per[meth] = dt / (m * N) * 1e9
+        dev_ms[meth] = tm["device_ms"]
+        bad += int(np.sum(rec.fail > 0))
+        log(f"recorder {meth}: {dt:.3f} s, {per[meth]:.3f} ns per orbit-step, {int(np.sum(rec.fail > 0))} failed")
+    total = len(METHODS) * m * N
+    value = total / wall
+    cpu = None
+    if not a.no_cpu:
+        from chaos_amd.tangent import make_chain
+        from tests import oracle_lib as O
+        ms_ = max(1, m // 8)
+        nt = min(16, os.cpu_count() or 1)
+        tc, steps_c = 0.0, 0
+        for meth in METHODS:
+            k = vectors(meth)
+            chain, _ = make_chain((estimator(meth), RunMonitor(sysm)))
+            W0 = np.broadcast_to(deviation_frame("pi/4", k), (ms_, 2, k))
+            t0 = time.perf_counter()
+            O.record(sysm.descriptor(), chain, x0[:ms_], W0, N, 1, k, grid, math=O.MATH_LIBM, nthreads=nt)
+            tc += time.perf_counter() - t0
+            steps_c += ms_ * N
+        cpu = dict(value=steps_c / tc, unit="orbit-steps/s", cores=nt, kind="port",
+                   sample=f"the first {ms_} orbits of each method's ensemble x {N} steps, the oracle's recording "
+                          f"(oracle/oracle.cpp, or_record) with glibc sin/cos/log, {nt} threads, {tc:.1f} s")
+    fpos = 174 + REC_MONITOR_FLOPS
+    achieved = fpos * m * N / (dev_ms["L4"] * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": value, "unit": "orbit-steps/s", "n_gpus": 1, "steps": len(METHODS), "warmup": 1,
+        "ms_per_step": wall * 1e3 / len(METHODS), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"recorder (grid records), standard map K=6, {m} orbits x {N} steps per method, "
+                               f"lifted, tau=1, study grid ({len(grid)} rows), methods {'/'.join(METHODS)}",
+                   "ns_per_orbit_step": per, "device_ms": dev_ms, "failed_orbits": bad,
+                   "reference_ns_per_orbit_step_cpu23_gpu_rtx5060": REC_REFERENCE_NS},
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel_ms": dev_ms["L4"],
+                     "flops_per_orbit_step": fpos, "kernel": "method L4 (MGS + RunMonitor)"},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "rec"])
+    ap.add_argument("--rec-sizes", default="2048,200000", help="rec: orbits, steps")
     ap.add_argument("--c5-sizes", default="1048576,262144,4194304,100000",
                     help="c5: training orbits, held-out orbits, triaged orbits, steps N")
     ap.add_argument("--c5-epochs", type=int, default=8)
@@ -424,6 +504,8 @@ def main():
         return c5_main(a)
     if a.workload == "c3":
         return c3_main(a)
+    if a.workload == "rec":
+        return rec_main(a)
     wl = workload(a.workload)
     log("workload", wl["desc"])
     if a.copy_probe:

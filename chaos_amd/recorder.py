@@ -138,11 +138,12 @@ class Records:
 
 
 def record(sys, x0, N, *, method, tau=1, grid=None, W0=None, direction="pi/4", orthogonality_tol=8.0,
-           launch_steps=None):
+           launch_steps=None, timing=None):
     """Record an ensemble on the GPU: `sys` a StandardMap (lifted, x0 of shape (m, 2)) or WrappedStandardMap
     (x0 of shape (m, 4), torus_state form); `grid` the record steps (default: the study grid with 20 points per
     decade, every step to 100 and the fractions 0.1 and 0.5); the start frame `W0` (2 x k, or one per orbit) or the
-    `direction` text of deviation_frame. Launches are split at `launch_steps` (a multiple of tau) when given."""
+    `direction` text of deviation_frame. Launches are split at `launch_steps` (a multiple of tau) when given. A dict
+    passed as `timing` receives the device time of the launches (HIP events on the launch stream) in "device_ms"."""
     import torch
     from .tangent import make_chain, _context
     if not isinstance(sys, (StandardMap, WrappedStandardMap)):
@@ -185,6 +186,9 @@ def record(sys, x0, N, *, method, tau=1, grid=None, W0=None, direction="pi/4", o
     dh = torch.zeros(m, dtype=torch.int64, device=dev)
     dfail = torch.zeros(m, dtype=torch.int64, device=dev)
     dR = torch.zeros((m, G, ncol), dtype=torch.float64, device=dev)
+    if timing is not None:
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
     n = 0
     for g in range(G):
         target = int(grid[g])
@@ -200,7 +204,11 @@ def record(sys, x0, N, *, method, tau=1, grid=None, W0=None, direction="pi/4", o
                                 dst.data_ptr(), dcode.data_ptr(), dfail.data_ptr(), dR.data_ptr(), stream)
         if rc:
             raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
+    if timing is not None:
+        ev1.record()
     torch.cuda.synchronize()
+    if timing is not None:
+        timing["device_ms"] = ev0.elapsed_time(ev1)
     raw = dst.cpu().numpy().T.copy().view(np.uint8).reshape(m, words * 8)
     return Records(method=method, grid=grid, R=dR.cpu().numpy(), fail=dfail.cpu().numpy(), code=dcode.cpu().numpy(),
                    hstep=dh.cpu().numpy(), x=dx.cpu().numpy().T.copy(),
