@@ -107,8 +107,9 @@ def test_oracle_record_equals_direct_runs(method, wrapped):
 
 
 def test_oracle_record_failures():
-    """An orbit whose frame has a zero column fails at the first handoff (row 1, no rows); one whose frame overflows
-    fails at its first handoff after the overflow; the others record every row."""
+    """An orbit whose frame has a zero column fails at the first handoff (row 1, no rows); under Benettin, an orbit
+    whose state overflows fails at the first grid row at or after the overflowing handoff; the others record every
+    row."""
     sys = StandardMap(6.0)
     m, N, tau = 5, 400, 1
     grid = record_grid(N, tau, points_per_decade=5, first_steps=4, extra_fractions=[0.5])
@@ -144,3 +145,21 @@ def test_unpack_blanks_from_the_failing_row():
     assert np.all(np.isnan(b["lyap"][2:])) and np.array_equal(b["lyap"][:2], R[1, :2, :2])
     assert b["gmax"] == R[1, 1, k + 5] and b["mmax"] == R[1, 1, k + 6] and np.array_equal(b["t"], [1, 2, 0, 0])
     assert np.array_equal(b["winding"][:2], R[1, :2, k + 9:k + 11].astype(np.int64))
+
+
+def test_run_monitor_equals_residual_monitor_maxima():
+    """RunMonitor (run_record.jl:37-40) keeps the same running maxima of the det and symplecticity residuals and
+    of max |M| as ResidualMonitor (monitor.jl:44-54, pinned by the reference's D34 checks) when the latter's
+    tolerances never trip; it never fails."""
+    from chaos_amd import ResidualMonitor, QRSpectrum, MGS
+    sys = StandardMap(3.5)
+    m, N = 16, 500
+    x0 = starts(m, False, seed=21)
+    W0 = np.broadcast_to(deviation_frame("pi/3", 2), (m, 2, 2))
+    ch, allr = make_chain((QRSpectrum(MGS()), RunMonitor(sys), ResidualMonitor(sys, 1e30, math.inf)))
+    r = O.run(sys.descriptor(), ch, x0, W0, N, 1, 2)
+    assert np.all(r["code"] == 0)
+    fin = finals_of(decode_states(r["states"], allr, 2, 2), allr, 2, 2)
+    for key in ("det_max", "symp_max", "M_max"):
+        assert np.array_equal(fin[1][key], fin[2][key]), key
+    assert np.all(fin[1]["M_max"] <= 1 + 3.5) and np.all(fin[1]["det_max"] < 1e-14)
