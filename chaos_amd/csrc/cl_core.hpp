@@ -116,6 +116,14 @@ CL_HD void map_tangent_apply(double c, Frame<2, K2>& W) {
   }
 }
 
+// The one-step tangent [[1 + c, 1], [c, 1]] of the standard-map family, column-major.
+CL_HD void map_jacobian(double c, double* J) {
+  J[0] = 1.0 + c;
+  J[1] = c;
+  J[2] = 1.0;
+  J[3] = 1.0;
+}
+
 // Standard map p' = p + K sin x, x' = x + p', Jacobian [[1 + c, 1], [c, 1]], c = K cos x.
 // The kernel carries sin and cos of the current point, so one reduction serves the Jacobian,
 // the step and a map displacement field read at the handoff.
@@ -133,13 +141,7 @@ struct StdMap {
   template <int K2>
   CL_HD void tangent_apply(const Aux& a, Frame<2, K2>& W) const { map_tangent_apply(tan_c(a), W); }
   // the one-step tangent at the current point, column-major (the ResidualMonitor reads it)
-  CL_HD void jacobian(const Aux& a, double* J) const {
-    double c = K * a.cs;
-    J[0] = 1.0 + c;
-    J[1] = c;
-    J[2] = 1.0;
-    J[3] = 1.0;
-  }
+  CL_HD void jacobian(const Aux& a, double* J) const { map_jacobian(tan_c(a), J); }
   CL_HD void forward(const Aux& a, double* x) const {
     double p = x[1] + K * a.sn;
     x[0] = x[0] + p;
@@ -171,13 +173,7 @@ struct StdMapK {
   template <int K2>
   CL_HD void tangent_apply(const Aux& a, Frame<2, K2>& W) const { map_tangent_apply(tan_c(a), W); }
   // the one-step tangent at the current point, column-major (the ResidualMonitor reads it)
-  CL_HD void jacobian(const Aux& a, double* J) const {
-    double c = a.K * a.cs;
-    J[0] = 1.0 + c;
-    J[1] = c;
-    J[2] = 1.0;
-    J[3] = 1.0;
-  }
+  CL_HD void jacobian(const Aux& a, double* J) const { map_jacobian(tan_c(a), J); }
   CL_HD void forward(const Aux& a, double* x) const {
     double p = x[1] + a.K * a.sn;
     x[0] = x[0] + p;
@@ -203,13 +199,7 @@ struct WrappedStdMap {
   template <int K2>
   CL_HD void tangent_apply(const Aux& a, Frame<2, K2>& W) const { map_tangent_apply(tan_c(a), W); }
   // the one-step tangent at the current point, column-major (the ResidualMonitor reads it)
-  CL_HD void jacobian(const Aux& a, double* J) const {
-    double c = K * a.cs;
-    J[0] = 1.0 + c;
-    J[1] = c;
-    J[2] = 1.0;
-    J[3] = 1.0;
-  }
+  CL_HD void jacobian(const Aux& a, double* J) const { map_jacobian(tan_c(a), J); }
   static CL_HD void reduce(double u, double tp, double* r, double* a) {
     double q = cl_floor(u / tp);
     double rr = u - tp * q;

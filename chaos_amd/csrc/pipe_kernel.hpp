@@ -10,6 +10,8 @@
 //   C  the checks of the factorisation: window_Gw conditioning and orth_loss (qr_spectrum.jl:54-73),
 //      and the running maxima G_max, orth_max
 //   D  the exponent sums: add(sums[i], log R_ii)
+//   E  (chains QRSpectrum + RunMonitor, the recorder's) the monitor's running maxima of the one-step tangent's
+//      det and symplecticity residuals and of max |M| at each handoff point, from c at that point
 //
 // Each part runs on its own waves of a 384-lane group (64 orbits, one per lane of every wave): A and B one
 // wave each; C two waves, for the even and the odd handoffs (a maximum does not depend on the order of its
@@ -30,31 +32,35 @@
 namespace cl {
 
 constexpr int PIPE_LANES = 64;   // orbits per group
-constexpr int PIPE_WAVES = 6;    // A, B, C even, C odd, D column 1, D column 2
+constexpr int PIPE_WAVES = 6;    // A, B, C even, C odd, D column 1, D column 2 (and E for a monitored chain)
 constexpr int PIPE_STEPS = 16;   // map steps per chunk at most (tau <= 16)
 constexpr int PIPE_HAND = 8;     // handoffs per chunk at most
 
 // handoffs per chunk for a given tau: chunks of about 8 steps, at least one handoff
 CL_HD int pipe_handoffs(int tau) { return tau <= PIPE_HAND ? PIPE_HAND / tau : 1; }
 
-template <class Sys, int ALG, int K>
-__global__ void __launch_bounds__(PIPE_WAVES * PIPE_LANES) pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H,
-                                                   uint8_t* rerun) {
+// MON: the chain is QRSpectrum then RunMonitor (its state words follow the QR state's)
+template <class Sys, int ALG, int K, bool MON>
+__global__ void __launch_bounds__((MON ? 7 : 6) * PIPE_LANES)
+pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
   constexpr int D = 2;
+  constexpr int NW = MON ? 7 : 6;
+  constexpr int WMON = (int)(sizeof(QRState<K>) / 8);  // the monitor's first state word
   typedef QRState<K> St;
   __shared__ double ringC[2][PIPE_STEPS][PIPE_LANES];
   __shared__ double ringR[2][PIPE_HAND][K * K][PIPE_LANES];
   __shared__ double ringQ[2][PIPE_HAND][D * K][PIPE_LANES];
-  __shared__ int bad_by[PIPE_WAVES][PIPE_LANES];
+  __shared__ double ringM[MON ? 2 : 1][MON ? PIPE_HAND : 1][PIPE_LANES];  // c at each handoff point
+  __shared__ int bad_by[NW][PIPE_LANES];
   __shared__ double outG[2][PIPE_LANES], outO[2][PIPE_LANES];
   __shared__ CSum outS[2][PIPE_LANES];  // [1]: column 2's sum, from the wave that formed it
   __shared__ int64_t outCount[PIPE_LANES];
 
   // Waves w and w + 4 of a group share a SIMD (round-robin placement over the CU's four SIMDs). Measured:
   // the step time is set by B whatever it shares with (placements differ by under 3%); the two C waves get a
-  // SIMD each, A shares with D of column 2 and B with D of column 1.
+  // SIMD each, A shares with D of column 2 and B with D of column 1 (E, when present, with C even).
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int role = wave < 4 ? wave : (wave == 4 ? 5 : 4);
+  const int role = wave < 4 ? wave : (wave == 4 ? 5 : (wave == 5 ? 4 : 6));
   const int64_t i = (int64_t)blockIdx.x * PIPE_LANES + lane;
   const bool in_range = i < a.m;
   const bool live = in_range && (a.n0 == 0 || a.code[i] == STATUS_OK);
@@ -63,6 +69,10 @@ __global__ void __launch_bounds__(PIPE_WAVES * PIPE_LANES) pipe_kernel(KArgs a, 
   St st;
   qr.init(st);
   if (live && a.n0 != 0) state_load(st, a.st, a.m, i, 0);
+  RunMonState mst;
+  const RunMon<2> mon = {};
+  mon.init(mst);
+  if (MON && role == 6 && live && a.n0 != 0) state_load(mst, a.st, a.m, i, WMON);
 
   double x[Sys::S];
   typename Sys::Aux aux;
@@ -99,6 +109,7 @@ __global__ void __launch_bounds__(PIPE_WAVES * PIPE_LANES) pipe_kernel(KArgs a, 
 #pragma unroll
           for (int j = 0; j < Sys::S; ++j) xf = xf && cl_isfinite(x[j]);
           bad = bad || !xf;
+          if constexpr (MON) ringM[slot][(int)(h - h0)][lane] = sys.tan_c(aux);
         }
       }
     } else if (role == 1) {
@@ -156,7 +167,7 @@ __global__ void __launch_bounds__(PIPE_WAVES * PIPE_LANES) pipe_kernel(KArgs a, 
           }
         }
       }
-    } else {
+    } else if (role <= 5) {
       const int col = role - 4;
       const int64_t q = p - 2;
       if (col < K && q >= 0 && q < nc) {
@@ -172,6 +183,17 @@ __global__ void __launch_bounds__(PIPE_WAVES * PIPE_LANES) pipe_kernel(KArgs a, 
         else st.sums[K - 1] = sm;
         st.n = a.n0 + h1 * tau;
       }
+    } else if constexpr (MON) {
+      const int64_t q = p - 1;
+      if (q >= 0 && q < nc) {
+        const int slot = (int)(q & 1);
+        const int64_t h0 = q * H, h1 = (h0 + H < nh) ? h0 + H : nh;
+        for (int64_t h = h0; h < h1; ++h) {
+          double J[4];
+          map_jacobian(ringM[slot][(int)(h - h0)][lane], J);
+          mon.step(mst, a.n0 + (h + 1) * tau, J);
+        }
+      }
     }
     __syncthreads();
   }
@@ -185,7 +207,7 @@ __global__ void __launch_bounds__(PIPE_WAVES * PIPE_LANES) pipe_kernel(KArgs a, 
   __syncthreads();
   bool failed = false;
 #pragma unroll
-  for (int w = 0; w < PIPE_WAVES; ++w) failed = failed || bad_by[w][lane] != 0;
+  for (int w = 0; w < NW; ++w) failed = failed || bad_by[w][lane] != 0;
   if (role == 0 && in_range) rerun[i] = (live && failed) ? 1 : 0;
   if (!live) return;
   const bool keep = !failed && nh > 0;
@@ -211,6 +233,8 @@ __global__ void __launch_bounds__(PIPE_WAVES * PIPE_LANES) pipe_kernel(KArgs a, 
       a.code[i] = STATUS_OK;
       a.hstep[i] = a.n0 + nh * tau;
     }
+  } else if (MON && role == 6) {
+    if (keep) state_store(mst, a.st, a.m, i, WMON);
   }
 }
 

@@ -40,12 +40,16 @@ static void load_table() {
   });
 }
 
-// The pipeline kernel for a one-frame chain that is a single QR spectrum on a two-dimensional map.
+// The pipeline kernel for a one-frame chain on a two-dimensional map that is a single QR spectrum, or a QR
+// spectrum followed by RunMonitor.
 static const PipeInstance* find_pipe(const cl_system* sys, const cl_chain* ch, int d, int k) {
   load_table();
-  if (d != 2 || ch->n != 1 || ch->r[0].kind != CL_RED_QR || ch->r[0].frame != 0) return nullptr;
+  if (d != 2 || ch->n < 1 || ch->n > 2 || ch->r[0].kind != CL_RED_QR || ch->r[0].frame != 0) return nullptr;
+  const int mon = ch->n == 2;
+  if (mon && (ch->r[1].kind != CL_RED_RUNMONITOR || ch->r[1].frame != 0)) return nullptr;
   for (int i = 0; i < g_npipe; ++i)
-    if (g_pipe[i].sys_kind == sys->kind && g_pipe[i].k == k && g_pipe[i].alg == ch->r[0].alg) return &g_pipe[i];
+    if (g_pipe[i].sys_kind == sys->kind && g_pipe[i].k == k && g_pipe[i].alg == ch->r[0].alg && g_pipe[i].mon == mon)
+      return &g_pipe[i];
   return nullptr;
 }
 

@@ -130,11 +130,11 @@ Instance make_instance() {
 void register_instances_map(Instance* out, int* n, int cap);
 void register_instances_flow(Instance* out, int* n, int cap);
 
-// Wave-specialised pipeline kernels for small map ensembles whose chain is one QRSpectrum on frame 1
-// (pipe_kernel.hpp). `launch` marks in `rerun` the orbits the caller must rerun with the fused kernel.
+// Wave-specialised pipeline kernels for small map ensembles whose chain is one QRSpectrum on frame 1, alone or
+// followed by RunMonitor (pipe_kernel.hpp). `launch` marks in `rerun` the orbits the caller must rerun with the fused kernel.
 constexpr int PIPE_TAU_MAX = 16;  // largest tau the pipeline's LDS rings hold
 struct PipeInstance {
-  int sys_kind, k, alg;
+  int sys_kind, k, alg, mon;  // mon: the chain is the QR spectrum followed by RunMonitor
   int (*launch)(const cl_system*, const cl_chain*, const KArgs&, uint8_t* rerun, hipStream_t);
 };
 void register_pipes(PipeInstance* out, int* n, int cap);

@@ -132,3 +132,27 @@ def test_pipeline_tau_above_the_ring_falls_back():
     run = lambda: propagate(StandardMap(2.0), x0, N, k=2, tau=tau, seed=3, reducers=(QRSpectrum(MGS()),),
                             check=False, keep_states=True)
     same(with_pipe("1", run), with_pipe("0", run))
+
+
+@pytest.mark.parametrize("alg", [CGS, MGS, RGS, HouseholderQR, AugmentedQR], ids=lambda a: a.__name__)
+def test_pipeline_monitored_chain(alg):
+    """The recorder's chains (a QR spectrum, then RunMonitor) through the pipeline's seventh wave: equal to the
+    fused kernel and the oracle, lifted and wrapped, with failures and a launch split."""
+    from chaos_amd import RunMonitor
+    m, N, tau = 400, 600, 2
+    for system in (StandardMap, WrappedStandardMap):
+        x0 = torus_uniform(14, m)
+        x0[: m // 5] *= 25.0
+        if system is WrappedStandardMap:
+            x0 = np.array([torus_state(z) for z in x0])
+        x0[3, :2] = [np.inf, 0.1]
+        s = system(3.0)
+        chain = (QRSpectrum(alg(), orthogonality_tol=0.6), RunMonitor(s))
+        for L_ in (None, 10):
+            say("pipeline monitored", system.__name__, alg.__name__, "launch", L_)
+            run = lambda: propagate(s, x0, N, k=2, tau=tau, seed=6, reducers=chain, check=False, keep_states=True,
+                                    launch_steps=L_)
+            p = with_pipe("1", run)
+            same(p, with_pipe("0", run))
+            ch, allr = make_chain(chain)
+            same_oracle(p, O.run(s.descriptor(), ch, x0, O.frames(6, 1, m, 2, 2), N, tau, 2, math=O.MATH_CL))
