@@ -408,33 +408,39 @@ def c5_main(a):
 # The recorder's cost as the reference measures it (verify/experiments-device.jl:37-58): 2048 orbits x 2e5 steps per
 # method, lifted, tau 1, the study grid; ns per orbit-step on its CPU threads and its GPU
 # (workspaces/module-build/stages/04-verify/output/gpu-rtx5060-d53-verify-log.txt:168-175).
-REC_REFERENCE_NS = {"L1": (4.173, 12.454), "L3": (7.906, 25.252), "L4": (8.594, 25.120), "L4b": (8.732, 35.920),
-                    "L5": (9.171, 27.082), "L5b": (9.804, 27.927)}
+REC_REFERENCE_NS = {"L1": (4.173, 12.454), "L2": (2.937, 11.187), "L3": (7.906, 25.252), "L4": (8.594, 25.120),
+                    "L4b": (8.732, 35.920), "L5": (9.171, 27.082), "L5b": (9.804, 27.927)}
+REC_L2_EPS = 1e-8       # the study's primary two-trajectory distance; L2 records the wrapped map (its main_l2 arm)
 REC_MONITOR_FLOPS = 47  # RunMonitor per handoff: det (5), M' Omega M - Omega and its maximum (35), max |M| (7)
 
 
 def rec_main(a):
     """--workload rec: the recorder's device path (src/Experiments/run_device.jl) on the reference's cost ensemble:
-    each method's grid records of 2048 standard-map orbits (K = 6, starts uniform on the torus) over 2e5 steps, lifted,
-    tau = 1, the study grid (20 points per decade, every step to 100, fractions 0.1 and 0.5), frame direction pi/4.
-    One step = one method's full recording; value = orbit-steps over all methods / wall time (the rows copied back to
-    the host included)."""
+    each method's grid records of 2048 standard-map orbits (K = 6, starts uniform on the torus) over 2e5 steps, lifted
+    (L2: wrapped, eps = 1e-8, as the reference's cost run takes it), tau = 1, the study grid (20 points per decade,
+    every step to 100, fractions 0.1 and 0.5), frame direction pi/4. One step = one method's full recording; value =
+    orbit-steps over all methods / wall time (the rows copied back to the host included)."""
     import torch
-    from chaos_amd import StandardMap, RunMonitor
+    from chaos_amd import StandardMap, WrappedStandardMap, RunMonitor, torus_state
     from chaos_amd.recorder import record, record_grid, deviation_frame, estimator, vectors, METHODS
     from chaos_amd.inputs import torus_uniform
     m, N = [int(v) for v in a.rec_sizes.split(",")]
-    sysm = StandardMap(6.0)
+    sysm, sysw = StandardMap(6.0), WrappedStandardMap(6.0)
     x0 = torus_uniform(1, m)
+    xw = np.array([torus_state(z) for z in x0])
     grid = record_grid(N, 1, points_per_decade=20, first_steps=100, extra_fractions=(0.1, 0.5))
     log(f"recorder: {m} orbits x {N} steps, {len(grid)} grid rows, methods {', '.join(METHODS)}")
     record(sysm, x0[:64], 200, method="L4", tau=1)  # warm-up: loads the kernels
+    record(sysw, xw[:64], 200, method="L2", tau=1, eps=REC_L2_EPS)
     per, wall, dev_ms, bad = {}, 0.0, {}, 0
     for meth in METHODS:
         tm = {}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        rec = record(sysm, x0, N, method=meth, tau=1, grid=grid, timing=tm)
+        if meth == "L2":
+            rec = record(sysw, xw, N, method=meth, tau=1, grid=grid, eps=REC_L2_EPS, timing=tm)
+        else:
+            rec = record(sysm, x0, N, method=meth, tau=1, grid=grid, timing=tm)
         dt = time.perf_counter() - t0
         wall += dt
         per[meth] = dt / (m * N) * 1e9
@@ -452,15 +458,20 @@ def rec_main(a):
         tc, steps_c = 0.0, 0
         for meth in METHODS:
             k = vectors(meth)
-            chain, _ = make_chain((estimator(meth), RunMonitor(sysm)))
             W0 = np.broadcast_to(deviation_frame("pi/4", k), (ms_, 2, k))
             t0 = time.perf_counter()
-            O.record(sysm.descriptor(), chain, x0[:ms_], W0, N, 1, k, grid, math=O.MATH_LIBM, nthreads=nt)
+            if meth == "L2":
+                O.shadow_record(sysw.descriptor(), xw[:ms_], np.ascontiguousarray(W0[:, :, 0]), N, 1, REC_L2_EPS,
+                                grid, math=O.MATH_LIBM, nthreads=nt)
+            else:
+                chain, _ = make_chain((estimator(meth), RunMonitor(sysm)))
+                O.record(sysm.descriptor(), chain, x0[:ms_], W0, N, 1, k, grid, math=O.MATH_LIBM, nthreads=nt)
             tc += time.perf_counter() - t0
             steps_c += ms_ * N
         cpu = dict(value=steps_c / tc, unit="orbit-steps/s", cores=nt, kind="port",
                    sample=f"the first {ms_} orbits of each method's ensemble x {N} steps, the oracle's recording "
-                          f"(oracle/oracle.cpp, or_record) with glibc sin/cos/log, {nt} threads, {tc:.1f} s")
+                          f"(oracle/oracle.cpp, or_record and or_shadow_record) with glibc sin/cos/log, {nt} threads, "
+                          f"{tc:.1f} s")
     fpos = 174 + REC_MONITOR_FLOPS
     achieved = fpos * m * N / (dev_ms["L4"] * 1e-3) / 1e12
     out = {
@@ -468,7 +479,8 @@ def rec_main(a):
         "ms_per_step": wall * 1e3 / len(METHODS), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"recorder (grid records), standard map K=6, {m} orbits x {N} steps per method, "
-                               f"lifted, tau=1, study grid ({len(grid)} rows), methods {'/'.join(METHODS)}",
+                               f"lifted (L2 wrapped, eps={REC_L2_EPS}), tau=1, study grid ({len(grid)} rows), "
+                               f"methods {'/'.join(METHODS)}",
                    "ns_per_orbit_step": per, "device_ms": dev_ms, "failed_orbits": bad,
                    "reference_ns_per_orbit_step_cpu23_gpu_rtx5060": REC_REFERENCE_NS},
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -8,8 +8,9 @@ from .reducers import (QRSpectrum, CGS, MGS, RGS, HouseholderQR, AugmentedQR, Gi
                        OFLI, GALI, ResidualMonitor, RunMonitor, monitor_exponents, det_threshold, symp_threshold, sum_threshold,
                        eps_sum, eps_pair, check_residual)
 from .tangent import propagate, propagate_ragged, PropagationResult, NumericalHealthError, initial_deviations
+from .two_trajectory import propagate_shadow
 
 __all__ = ["StandardMap", "StandardMapK", "StandardMapChain", "WrappedStandardMap", "HenonHeilesGauss", "hh_energy", "torus_state", "QRSpectrum", "CGS", "MGS", "RGS", "HouseholderQR",
            "AugmentedQR", "GivensQR", "Benettin", "MEGNO", "SALI", "FLI", "OFLI", "GALI", "ResidualMonitor", "RunMonitor", "monitor_exponents", "det_threshold",
            "symp_threshold", "sum_threshold", "eps_sum", "eps_pair", "check_residual", "propagate", "propagate_ragged",
-           "PropagationResult", "NumericalHealthError", "initial_deviations"]
+           "PropagationResult", "NumericalHealthError", "initial_deviations", "propagate_shadow"]

@@ -70,6 +70,10 @@ SIGNATURES = {
                                 P, P, P, P, P, P]),
     "cl_record_rows": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I64, I, I, I, I, P, P, P, P, P,
                            P]),
+    "cl_shadow_device": (I, [P, ctypes.POINTER(cl_system), I64, I, I64, I64, I64, I, ctypes.c_double, P, P, P, P]),
+    "cl_propagate_shadow_chunk": (I, [P, ctypes.POINTER(cl_system), I64, I, I64, I64, I64, I, ctypes.c_double, P, P,
+                                      P]),
+    "cl_shadow_record_rows": (I, [P, I64, I, I, I, I64, P, P, P, P]),
     "cl_initial_deviations": (I, [ctypes.c_uint64, I64, I64, I, I, P]),
     "cl_initial_deviations_device": (I, [P, ctypes.c_uint64, I64, I64, I, I, P, P]),
     "cl_math_host": (None, [I64, P, P, P, P]),

@@ -106,8 +106,8 @@ static int validate(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int e
   if (ch->n < 0 || ch->n > CL_MAX_REDUCERS) return set_err(ctx, CL_ERR_ARGUMENT, "chain length out of range");
   if (!(tau >= 1 && N >= 1 && N % tau == 0))
     return set_err(ctx, CL_ERR_ARGUMENT, "N must be a positive multiple of tau");
-  if (!(steps >= 1 && steps % tau == 0 && n0 >= 0 && n0 % tau == 0 && n0 + steps <= N))
-    return set_err(ctx, CL_ERR_ARGUMENT, "launch steps must be a positive multiple of tau within N");
+  if (!(steps >= 0 && steps % tau == 0 && n0 >= 0 && n0 % tau == 0 && n0 + steps <= N))
+    return set_err(ctx, CL_ERR_ARGUMENT, "launch steps must be a multiple of tau within N");
   if (m < 0) return set_err(ctx, CL_ERR_ARGUMENT, "negative orbit count");
   int sd = sys_dim(sys->kind);
   if (sd < 0) return set_err(ctx, CL_ERR_UNSUPPORTED, "system kind has no kernel");
@@ -519,8 +519,8 @@ static int chain_device(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, i
   bool qr = false;
   int rc = validate_chain(ctx, sys, ch, elt, m, s, d, k, tau, &words, &qr);
   if (rc) return rc;
-  if (!(N >= 1 && N % tau == 0 && steps >= 1 && steps % tau == 0 && n0 >= 0 && n0 % tau == 0 && n0 + steps <= N))
-    return set_err(ctx, CL_ERR_ARGUMENT, "launch steps must be a positive multiple of tau within N");
+  if (!(N >= 1 && N % tau == 0 && steps >= 0 && steps % tau == 0 && n0 >= 0 && n0 % tau == 0 && n0 + steps <= N))
+    return set_err(ctx, CL_ERR_ARGUMENT, "launch steps must be a multiple of tau within N");
   rc = chain_launch(chain_args(sys, ch, m, dx, dW, dst, dcode, dhstep, n0, steps, N, tau), sys->N, sys->ring != 0,
                     qr, st);
   return rc ? set_err(ctx, rc, "chain kernel launch failed") : CL_OK;
@@ -657,6 +657,87 @@ static int chain_chunk(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, in
   CK(hipStreamSynchronize(st));
 #undef CK
   return CL_OK;
+}
+
+// ------------------------------------------------------------------------------ two-trajectory estimator
+namespace cl {
+int shadow_launch(const cl_system* s, const double* x0, const double* u0, uint64_t* e, int64_t m, int64_t n0,
+                  int64_t steps, int64_t N, int tau, double d0, hipStream_t st);
+int shadow_rows_launch(int64_t m, int s, int g, int G, int64_t target, const uint64_t* e, int64_t* fail, double* R,
+                       hipStream_t st);
+}
+
+static int validate_shadow(cl_ctx* ctx, const cl_system* sys, int64_t m, int s, int64_t n0, int64_t steps, int64_t N,
+                           int tau, double d0) {
+  if (!ctx || !sys) return CL_ERR_ARGUMENT;
+  const bool sys_ok = (sys->kind == CL_SYS_STANDARD_MAP && s == 2) || (sys->kind == CL_SYS_WRAPPED_STANDARD_MAP && s == 4);
+  if (!sys_ok) return set_err(ctx, CL_ERR_UNSUPPORTED, "the two-trajectory kernel runs the standard map, lifted or wrapped");
+  if (m < 0 || !(tau >= 1 && N >= 1 && N % tau == 0 && steps >= 0 && steps % tau == 0 && n0 >= 0 && n0 % tau == 0 &&
+                 n0 + steps <= N))
+    return set_err(ctx, CL_ERR_ARGUMENT, "launch steps must be a multiple of tau within N");
+  if (!(d0 > 0.0 && d0 < __builtin_inf())) return set_err(ctx, CL_ERR_ARGUMENT, "d0 must be finite and positive");
+  return CL_OK;
+}
+
+extern "C" int cl_shadow_device(cl_ctx* ctx, const cl_system* sys, int64_t m, int s, int64_t n0, int64_t steps,
+                                int64_t N, int tau, double d0, const double* dx0, const double* du0, uint64_t* dends,
+                                void* stream) {
+  int rc = validate_shadow(ctx, sys, m, s, n0, steps, N, tau, d0);
+  if (rc) return rc;
+  if (m > 0 && (!dends || (n0 == 0 && (!dx0 || !du0)))) return set_err(ctx, CL_ERR_ARGUMENT, "null buffer");
+  hipSetDevice(ctx->device);
+  rc = shadow_launch(sys, dx0, du0, dends, m, n0, steps, N, tau, d0, stream ? (hipStream_t)stream : ctx->stream);
+  return rc ? set_err(ctx, rc, "two-trajectory kernel launch failed") : CL_OK;
+}
+
+extern "C" int cl_propagate_shadow_chunk(cl_ctx* ctx, const cl_system* sys, int64_t m, int s, int64_t n0,
+                                         int64_t steps, int64_t N, int tau, double d0, const double* x0,
+                                         const double* u0, void* ends) {
+  int rc = validate_shadow(ctx, sys, m, s, n0, steps, N, tau, d0);
+  if (rc) return rc;
+  if (m == 0) return CL_OK;
+  if (!x0 || !u0 || !ends) return set_err(ctx, CL_ERR_ARGUMENT, "null buffer");
+  hipSetDevice(ctx->device);
+  hipStream_t st = ctx->stream;
+  const int ew = 2 * s + 6;
+  const size_t bx = (size_t)m * s * 8, bu = (size_t)m * 16, be = (size_t)m * ew * 8;
+  char* base = nullptr;
+  const size_t tot = 2 * (bx + bu + be) + 1024;
+  hipError_t e = hipMallocAsync((void**)&base, tot, st);
+  if (e != hipSuccess) return set_err(ctx, map_err(e), "device allocation failed");
+  uint64_t *xa = (uint64_t*)base, *ua = (uint64_t*)(base + bx), *ea = (uint64_t*)(base + bx + bu);
+  uint64_t *xs = (uint64_t*)(base + bx + bu + be), *us = (uint64_t*)((char*)xs + bx), *es = (uint64_t*)((char*)us + bu);
+  const int B = 256;
+  if ((e = hipMemcpyAsync(xa, x0, bx, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(ua, u0, bu, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(ea, ends, be, hipMemcpyHostToDevice, st)) != hipSuccess) {
+    hipFreeAsync(base, st);
+    return set_err(ctx, map_err(e), "copy to the device failed");
+  }
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(m * s, B)), dim3(B), 0, st, xa, xs, m, s);
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(m * 2, B)), dim3(B), 0, st, ua, us, m, 2);
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(m * ew, B)), dim3(B), 0, st, ea, es, m, ew);
+  rc = shadow_launch(sys, (double*)xs, (double*)us, es, m, n0, steps, N, tau, d0, st);
+  if (!rc) {
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(m * ew, B)), dim3(B), 0, st, es, ea, m, ew);
+    e = hipMemcpyAsync(ends, ea, be, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = map_err(e);
+  }
+  hipFreeAsync(base, st);
+  hipStreamSynchronize(st);
+  return rc ? set_err(ctx, rc, "two-trajectory chunk failed") : CL_OK;
+}
+
+extern "C" int cl_shadow_record_rows(cl_ctx* ctx, int64_t m, int s, int g, int G, int64_t target,
+                                     const uint64_t* dends, int64_t* dfail, double* dR, void* stream) {
+  if (!ctx) return CL_ERR_ARGUMENT;
+  if (m < 0 || (s != 2 && s != 4) || G < 1 || g < 0 || g >= G || target < 1)
+    return set_err(ctx, CL_ERR_ARGUMENT, "grid row out of range");
+  if (m > 0 && (!dends || !dfail || !dR)) return set_err(ctx, CL_ERR_ARGUMENT, "null buffer");
+  hipSetDevice(ctx->device);
+  int rc = shadow_rows_launch(m, s, g, G, target, dends, dfail, dR, stream ? (hipStream_t)stream : ctx->stream);
+  return rc ? set_err(ctx, rc, "record kernel launch failed") : CL_OK;
 }
 
 extern "C" int cl_propagate_chunk(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m,

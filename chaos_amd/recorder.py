@@ -21,10 +21,11 @@ from . import _lib as L
 from .reducers import QRSpectrum, Benettin, CGS, MGS, RGS, HouseholderQR, AugmentedQR, RunMonitor
 from .systems import StandardMap, WrappedStandardMap
 
-# method codes with a device path (run_record.jl:47-53); L2 (two trajectories), L6 (LAPACK QR) and L9 (SVD) run on
-# the reference's CPU path
+# method codes with a device path (run_record.jl:47-53, run_device.jl:230); L6 (LAPACK QR) and L9 (SVD) run on the
+# reference's CPU path only
 QR_METHODS = {"L3": CGS, "L4": MGS, "L4b": RGS, "L5": HouseholderQR, "L5b": AugmentedQR}
-METHODS = ("L1",) + tuple(QR_METHODS)
+VARIATIONAL = ("L1",) + tuple(QR_METHODS)   # the methods with a reducer chain on the tangent frame
+METHODS = ("L1", "L2") + tuple(QR_METHODS)
 
 
 def estimator(method, orthogonality_tol=8.0):
@@ -33,12 +34,14 @@ def estimator(method, orthogonality_tol=8.0):
         return Benettin()
     if method in QR_METHODS:
         return QRSpectrum(QR_METHODS[method](), orthogonality_tol)
-    raise ValueError(f"method {method!r} has no device path (two-trajectory L2, LAPACK QR L6 and SVD L9 do not)")
+    if method == "L2":
+        raise ValueError("L2 (two trajectories) carries a shadow orbit, not a reducer chain")
+    raise ValueError(f"method {method!r} has no device path (LAPACK QR L6 and SVD L9 do not)")
 
 
 def vectors(method):
-    """Deviation vectors of a method: 1 for Benettin, 2 for the QR spectra."""
-    return 1 if method == "L1" else 2
+    """Deviation vectors of a method: 1 for Benettin and the two-trajectory estimator, 2 for the QR spectra."""
+    return 1 if method in ("L1", "L2") else 2
 
 
 def record_grid(N, tau, *, points_per_decade, first_steps, extra_fractions):
@@ -137,13 +140,15 @@ class Records:
         return out
 
 
-def record(sys, x0, N, *, method, tau=1, grid=None, W0=None, direction="pi/4", orthogonality_tol=8.0,
+def record(sys, x0, N, *, method, tau=1, grid=None, W0=None, direction="pi/4", orthogonality_tol=8.0, eps=None,
            launch_steps=None, timing=None):
     """Record an ensemble on the GPU: `sys` a StandardMap (lifted, x0 of shape (m, 2)) or WrappedStandardMap
     (x0 of shape (m, 4), torus_state form); `grid` the record steps (default: the study grid with 20 points per
     decade, every step to 100 and the fractions 0.1 and 0.5); the start frame `W0` (2 x k, or one per orbit) or the
-    `direction` text of deviation_frame. Launches are split at `launch_steps` (a multiple of tau) when given. A dict
-    passed as `timing` receives the device time of the launches (HIP events on the launch stream) in "device_ms"."""
+    `direction` text of deviation_frame. L2 (two trajectories) takes the shadow distance `eps` (a decimal text or a
+    float, read as the nearest Float64) and the first column of the frame as its direction. Launches are split at
+    `launch_steps` (a multiple of tau) when given. A dict passed as `timing` receives the device time of the
+    launches (HIP events on the launch stream) in "device_ms"."""
     import torch
     from .tangent import make_chain, _context
     if not isinstance(sys, (StandardMap, WrappedStandardMap)):
@@ -169,6 +174,11 @@ def record(sys, x0, N, *, method, tau=1, grid=None, W0=None, direction="pi/4", o
         W0 = np.broadcast_to(W0, (m, 2, k))
     if W0.shape != (m, 2, k):
         raise ValueError("W0 must be 2 x k or one 2 x k frame per orbit")
+    if method == "L2":
+        if eps is None:
+            raise ValueError("L2 needs the shadow distance eps")
+        return _record_shadow(sys, x0, N, tau, grid, np.ascontiguousarray(W0[:, :, 0]), float(str(eps)),
+                              launch_steps, timing)
     chain, allr = make_chain((estimator(method, orthogonality_tol), RunMonitor(sys)))
     lib, ctx = _context()
     sysd = sys.descriptor()
@@ -213,3 +223,50 @@ def record(sys, x0, N, *, method, tau=1, grid=None, W0=None, direction="pi/4", o
     return Records(method=method, grid=grid, R=dR.cpu().numpy(), fail=dfail.cpu().numpy(), code=dcode.cpu().numpy(),
                    hstep=dh.cpu().numpy(), x=dx.cpu().numpy().T.copy(),
                    W=dW.cpu().numpy().reshape(k, 2, m).transpose(2, 1, 0).copy(), states=raw, k=k, s=s)
+
+
+def _record_shadow(sys, x0, N, tau, grid, u0, d0, launch_steps, timing):
+    """L2 (_shadow_batch!, run_device.jl:175-197; _shadow_record_kernel!, :74-98): the end records carried across
+    segments ending at the grid steps (`cl_shadow_device`), the row of each grid step written on the device
+    (`cl_shadow_record_rows`): total(sum) / step in column 0 and the point in columns 8..; no start check, a start
+    out of range fails at row 1."""
+    import torch
+    from .tangent import _context
+    from .two_trajectory import check_d0, decode_ends
+    d0 = check_d0(d0)
+    m, s = x0.shape
+    G = len(grid)
+    lib, ctx = _context()
+    sysd = sys.descriptor()
+    dev = torch.device("cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    dx = torch.from_numpy(np.ascontiguousarray(x0.T)).to(dev)
+    du = torch.from_numpy(np.ascontiguousarray(u0.T)).to(dev)
+    de = torch.zeros((2 * s + 6, m), dtype=torch.int64, device=dev)
+    dfail = torch.zeros(m, dtype=torch.int64, device=dev)
+    dR = torch.zeros((m, G, 1 + 7 + s), dtype=torch.float64, device=dev)
+    if timing is not None:
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+    n = 0
+    for g in range(G):
+        target = int(grid[g])
+        while n < target:
+            steps = target - n if launch_steps is None else min(int(launch_steps), target - n)
+            rc = lib.cl_shadow_device(ctx, ctypes.byref(sysd), m, s, n, steps, N, tau, d0, dx.data_ptr(),
+                                      du.data_ptr(), de.data_ptr(), stream)
+            if rc:
+                raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
+            n += steps
+        rc = lib.cl_shadow_record_rows(ctx, m, s, g, G, target, de.data_ptr(), dfail.data_ptr(), dR.data_ptr(), stream)
+        if rc:
+            raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
+    if timing is not None:
+        ev1.record()
+    torch.cuda.synchronize()
+    if timing is not None:
+        timing["device_ms"] = ev0.elapsed_time(ev1)
+    raw = de.cpu().numpy().T.copy().view(np.uint64)
+    e = decode_ends(raw, s)
+    return Records(method="L2", grid=grid, R=dR.cpu().numpy(), fail=dfail.cpu().numpy(), code=e["code"],
+                   hstep=e["step"], x=e["x"], W=e["y"], states=raw, k=1, s=s)

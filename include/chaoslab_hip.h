@@ -177,6 +177,26 @@ int cl_record_rows(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain, int
                    int G, const double *dx, const uint64_t *dstates, const int8_t *dcode, int64_t *dfail,
                    double *dR, void *stream);
 
+/* The two-trajectory estimator (replaces _shadow_kernel!, src/Lyapunov/two_trajectory.jl:113-121, and its body
+ * _shadow_body, :45-82): the reference orbit and a shadow at distance d0, placed back at d0 along the separation
+ * every tau steps; the end record of each orbit (ShadowEnd{s, Float64}: x[s], y[s], sum (s, c), dprev, code + 7 pad
+ * bytes, step, value; 2s + 6 words) is carried across launches. At n0 = 0 the shadow starts at x0 + d0 u0 (u0 unit,
+ * 2 per orbit); later launches continue orbits whose code is 0. Standard map, lifted (s = 2, plain separation) or
+ * wrapped (s = 4, minimal image, src/Lyapunov/separation.jl).
+ * Device version: SoA buffers (x0[c*m + i], u0[c*m + i], ends[w*m + i]), asynchronous.
+ * Host chunk version (replaces one launch of _shadow_chunk!, :124-143): AoS host buffers (x0[i][s], u0[i][2],
+ * ends[i][2s + 6] words in the Julia byte layout), synchronous. */
+int cl_shadow_device(cl_ctx *ctx, const cl_system *sys, int64_t m, int s, int64_t n0, int64_t steps, int64_t N,
+                     int tau, double d0, const double *dx0, const double *du0, uint64_t *dends, void *stream);
+int cl_propagate_shadow_chunk(cl_ctx *ctx, const cl_system *sys, int64_t m, int s, int64_t n0, int64_t steps,
+                              int64_t N, int tau, double d0, const double *x0, const double *u0, void *ends);
+/* The recorder's rows for the two-trajectory method (replaces _shadow_record_kernel!'s row writes,
+ * src/Experiments/run_device.jl:74-98): after the launch that ends at grid step g (global step `target`), row g of
+ * dR[m][G][1 + 7 + s] gets lambda = total(sum) / target in column 0 and the point in columns 8..; an orbit whose
+ * record has a nonzero code gets dfail[i] = g + 1. Device buffers, asynchronous. */
+int cl_shadow_record_rows(cl_ctx *ctx, int64_t m, int s, int g, int G, int64_t target, const uint64_t *dends,
+                          int64_t *dfail, double *dR, void *stream);
+
 /* Seeded orthonormal frames, replacing initial_deviations(seed, orbit, Val(d), Val(k), T)
  * (src/Tangent/initial.jl:27-34): orbit ids orbit0 .. orbit0+m-1 (1-based ids as the reference),
  * key mix(mix(seed) + id) with the reference's splitmix64 finaliser, Gaussian d x k matrix from a

@@ -1516,6 +1516,233 @@ static int record_all(const cl_system* sys, const cl_chain* ch, int64_t m, int s
   return 0;
 }
 
+// ------------------------------------------------------------------ two-trajectory estimator
+// The reference orbit and a shadow at distance d0, stepped together and placed back at d0 along the separation
+// every tau steps (_shadow_body, src/Lyapunov/two_trajectory.jl:45-82). Separation and placement
+// (src/Lyapunov/separation.jl:9-40): the plain difference and ref + delta on the lifted map and the linear map; on
+// the wrapped map the minimal image of each coordinate in (-pi, pi] and the wrapped placement whose reduction counts
+// are added to the reference's.
+struct ShadowRec {  // ShadowEnd{s, Float64} (two_trajectory.jl:11-19)
+  double x[4], y[4];
+  Acc<double> sum;
+  double dprev;
+  int8_t code;
+  int64_t step;
+  double value;
+};
+
+template <class P>
+static void wrap_reduce(double u, double tp, double& r, double& a) {  // _reduce, wrapped_standard_map.jl:15-27
+  a = P::floor(u / tp);
+  r = u - tp * a;
+  if (r < 0) {
+    r += tp;
+    a -= 1;
+  }
+  if (r >= tp) {
+    r -= tp;
+    a += 1;
+  }
+}
+
+template <class P>
+static void sh_sep(const MapSys<P, double>& ms, const double* x, const double* y, double* d) {
+  if (ms.kind == CL_SYS_WRAPPED_STANDARD_MAP) {
+    const double tp = 2.0 * P::pi(), h = tp / 2;
+    for (int j = 0; j < 2; ++j) {
+      const double u = y[j] - x[j];
+      d[j] = u > h ? u - tp : (u <= -h ? u + tp : u);
+    }
+  } else {
+    d[0] = y[0] - x[0];
+    d[1] = y[1] - x[1];
+  }
+}
+
+template <class P>
+static void sh_place(const MapSys<P, double>& ms, const double* x, const double* d, double* y) {
+  if (ms.kind == CL_SYS_WRAPPED_STANDARD_MAP) {
+    const double tp = 2.0 * P::pi();
+    double a, b;
+    wrap_reduce<P>(x[0] + d[0], tp, y[0], a);
+    wrap_reduce<P>(x[1] + d[1], tp, y[1], b);
+    y[2] = x[2] + a;
+    y[3] = x[3] + b;
+  } else {
+    y[0] = x[0] + d[0];
+    y[1] = x[1] + d[1];
+  }
+}
+
+template <class P>
+static ShadowRec sh_end(const double* x, const double* y, int s, Acc<double> acc, double dprev, int8_t code,
+                        int64_t step, double value) {
+  ShadowRec e;
+  memset(&e, 0, sizeof(e));
+  for (int j = 0; j < s; ++j) {
+    e.x[j] = x[j];
+    e.y[j] = y[j];
+  }
+  e.sum = acc;
+  e.dprev = dprev;
+  e.code = code;
+  e.step = step;
+  e.value = value;
+  return e;
+}
+
+// One launch of `steps` steps from global step n0; at n0 = 0 the shadow starts at x0 + d0 u0 (_shadow_start,
+// two_trajectory.jl:31-36), later launches continue from e.
+template <class P>
+static ShadowRec shadow_body(const MapSys<P, double>& ms, const double* x0, const double* u0, const ShadowRec& e,
+                             int64_t n0, int64_t steps, int tau, double d0) {
+  const int s = ms.s;
+  double x[4], y[4], dprev;
+  Acc<double> acc;
+  if (n0 == 0) {
+    for (int j = 0; j < s; ++j) x[j] = y[j] = x0[j];
+    const double dl[2] = {d0 * u0[0], d0 * u0[1]};
+    double sp[2];
+    sh_place(ms, x, dl, y);
+    sh_sep(ms, x, y, sp);
+    dprev = snorm<P>(sp, 2);
+    acc.s = acc.c = 0.0;
+    const int8_t code = !(P::finite(dprev) && dprev > 0) ? 2 : ((dprev < d0 / 2 || dprev > 2 * d0) ? 3 : 0);
+    if (code) return sh_end<P>(x, y, s, acc, dprev, code, 0, dprev);
+  } else {
+    for (int j = 0; j < s; ++j) {
+      x[j] = e.x[j];
+      y[j] = e.y[j];
+    }
+    acc = e.sum;
+    dprev = e.dprev;
+  }
+  int64_t n = n0;
+  for (int64_t h = 0; h < steps / tau; ++h) {
+    for (int t = 0; t < tau; ++t) {
+      ms.step(x, nullptr, 0, 0);
+      ms.step(y, nullptr, 0, 0);
+    }
+    n += tau;
+    bool bad = false;
+    double v = 0.0;
+    for (int j = 0; j < s && !bad; ++j)
+      if (!P::finite(x[j])) bad = true, v = x[j];
+    for (int j = 0; j < s && !bad; ++j)
+      if (!P::finite(y[j])) bad = true, v = y[j];
+    if (bad) return sh_end<P>(x, y, s, acc, dprev, 1, n, v);
+    double dl[2];
+    sh_sep(ms, x, y, dl);
+    const double d = snorm<P>(dl, 2);
+    if (!(P::finite(d) && d > 0)) return sh_end<P>(x, y, s, acc, dprev, 2, n, d);
+    acc = acc_add<P>(acc, P::log(d / dprev));
+    const double r = d0 / d;
+    const double nd[2] = {dl[0] * r, dl[1] * r};
+    double sp[2];
+    sh_place(ms, x, nd, y);
+    sh_sep(ms, x, y, sp);
+    dprev = snorm<P>(sp, 2);
+    if (!(dprev >= d0 / 2 && dprev <= 2 * d0))
+      return sh_end<P>(x, y, s, acc, dprev, (P::finite(dprev) && dprev > 0) ? 3 : 2, n, dprev);
+  }
+  return sh_end<P>(x, y, s, acc, dprev, 0, n, dprev);
+}
+
+// end record <-> the Julia byte layout of ShadowEnd{s, Float64}: 2s + 6 words
+static void sh_write(const ShadowRec& e, int s, uint64_t* w) {
+  memcpy(w, e.x, 8 * s);
+  memcpy(w + s, e.y, 8 * s);
+  memcpy(w + 2 * s, &e.sum.s, 8);
+  memcpy(w + 2 * s + 1, &e.sum.c, 8);
+  memcpy(w + 2 * s + 2, &e.dprev, 8);
+  w[2 * s + 3] = (uint8_t)e.code;
+  memcpy(w + 2 * s + 4, &e.step, 8);
+  memcpy(w + 2 * s + 5, &e.value, 8);
+}
+static ShadowRec sh_read(const uint64_t* w, int s) {
+  ShadowRec e;
+  memset(&e, 0, sizeof(e));
+  memcpy(e.x, w, 8 * s);
+  memcpy(e.y, w + s, 8 * s);
+  memcpy(&e.sum.s, w + 2 * s, 8);
+  memcpy(&e.sum.c, w + 2 * s + 1, 8);
+  memcpy(&e.dprev, w + 2 * s + 2, 8);
+  e.code = (int8_t)(w[2 * s + 3] & 0xff);
+  memcpy(&e.step, w + 2 * s + 4, 8);
+  memcpy(&e.value, w + 2 * s + 5, 8);
+  return e;
+}
+
+template <class F>
+static void for_orbits(int64_t m, int nthreads, F f) {
+  std::atomic<int64_t> next(0);
+  auto worker = [&]() {
+    while (true) {
+      int64_t lo = next.fetch_add(64);
+      if (lo >= m) break;
+      for (int64_t i = lo; i < (lo + 64 < m ? lo + 64 : m); ++i) f(i);
+    }
+  };
+  if (nthreads <= 1) {
+    worker();
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthreads; ++t) th.emplace_back(worker);
+  for (auto& t : th) t.join();
+}
+
+template <class P>
+static MapSys<P, double> shadow_sys(const cl_system* sys, int s, const double* A) {
+  MapSys<P, double> ms;
+  ms.kind = A ? OR_SYS_LINEAR : sys->kind;
+  ms.K = A ? 0.0 : sys->K;
+  ms.s = s;
+  for (int i = 0; i < 4; ++i) ms.A[i] = A ? A[i] : 0.0;
+  return ms;
+}
+
+template <class P>
+static void shadow_all(const cl_system* sys, const double* A, int64_t m, int s, int64_t n0, int64_t steps, int tau,
+                       double d0, const double* x0, const double* u0, uint64_t* ends, int nthreads) {
+  const MapSys<P, double> ms = shadow_sys<P>(sys, s, A);
+  const int ew = 2 * s + 6;
+  for_orbits(m, nthreads, [&](int64_t i) {
+    ShadowRec e = sh_read(ends + i * ew, s);
+    if (n0 > 0 && e.code != 0) return;  // failed in an earlier launch: kept, not advanced (two_trajectory.jl:113-121)
+    e = shadow_body(ms, x0 + i * s, u0 + 2 * i, e, n0, steps, tau, d0);
+    sh_write(e, s, ends + i * ew);
+  });
+}
+
+// The recorder's L2 path (_record_shadow!, src/Experiments/run_record.jl:152-172): segments to each grid step;
+// row g gets total(sum) / grid[g] in column 0 and the point in columns 8..8+s; fail = 1-based failing row.
+template <class P>
+static void shadow_record_all(const cl_system* sys, int64_t m, int s, int tau, double d0, const int64_t* grid, int G,
+                              const double* x0, const double* u0, double* R, int64_t* fail, uint64_t* ends,
+                              int nthreads) {
+  const MapSys<P, double> ms = shadow_sys<P>(sys, s, nullptr);
+  const int ew = 2 * s + 6, ncol = 1 + 7 + s;
+  for_orbits(m, nthreads, [&](int64_t i) {
+    ShadowRec e;
+    memset(&e, 0, sizeof(e));
+    int64_t n = 0;
+    fail[i] = 0;
+    for (int g = 0; g < G; ++g) {
+      e = shadow_body(ms, x0 + i * s, u0 + 2 * i, e, n, grid[g] - n, tau, d0);
+      if (e.code != 0) {
+        fail[i] = g + 1;
+        break;
+      }
+      n = grid[g];
+      double* row = R + ((size_t)i * G + g) * ncol;
+      row[0] = acc_total(e.sum) / (double)grid[g];
+      for (int j = 0; j < s; ++j) row[8 + j] = e.x[j];
+    }
+    sh_write(e, s, ends + i * ew);
+  });
+}
+
 static bool or_dims_ok(const cl_system* sys, int d, int k) {
   int want = sys->kind == CL_SYS_HENON_HEILES_GAUSS ? 4
            : (sys->kind == CL_SYS_STANDARD_MAP || sys->kind == CL_SYS_WRAPPED_STANDARD_MAP ||
@@ -1631,6 +1858,38 @@ int or_record(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int k,
   if (math == MATH_CL)
     return record_all<F64Cl>(sys, ch, m, s, k, N, tau, grid, G, x, W, R, fail, code, hstep, (uint8_t*)states, nthreads);
   return record_all<F64Libm>(sys, ch, m, s, k, N, tau, grid, G, x, W, R, fail, code, hstep, (uint8_t*)states, nthreads);
+}
+
+// The two-trajectory estimator on a standard map (lifted s = 2, wrapped s = 4), or on the linear map x' = A x
+// (A column-major 2 x 2, s = 2) when A is given: one launch of `steps` steps from global step n0 over host AoS
+// buffers x0[m][s], u0[m][2] (unit directions) and the end records ends[m][2s + 6] (Julia byte layout), in place.
+int or_shadow(const cl_system* sys, const double* A, int64_t m, int s, int64_t n0, int64_t steps, int64_t N, int tau,
+              double d0, const double* x0, const double* u0, void* ends, int math, int nthreads) {
+  const bool sys_ok = A ? s == 2
+                        : ((sys->kind == CL_SYS_STANDARD_MAP && s == 2) || (sys->kind == CL_SYS_WRAPPED_STANDARD_MAP && s == 4));
+  if (!sys_ok || tau < 1 || steps < 0 || steps % tau || n0 < 0 || n0 % tau || n0 + steps > N || N % tau) return 10;
+  if (!(d0 > 0 && cl_isfinite(d0))) return 10;
+  if (math == MATH_CL)
+    shadow_all<F64Cl>(sys, A, m, s, n0, steps, tau, d0, x0, u0, (uint64_t*)ends, nthreads);
+  else
+    shadow_all<F64Libm>(sys, A, m, s, n0, steps, tau, d0, x0, u0, (uint64_t*)ends, nthreads);
+  return 0;
+}
+
+// The recorder's two-trajectory rows: R (m x G x (8 + s), zeroed by the caller), fail[i] the 1-based failing grid row
+// or 0, ends the end records after the last segment run.
+int or_shadow_record(const cl_system* sys, int64_t m, int s, int64_t N, int tau, double d0, const int64_t* grid,
+                     int G, const double* x0, const double* u0, double* R, int64_t* fail, void* ends, int math,
+                     int nthreads) {
+  const bool sys_ok = (sys->kind == CL_SYS_STANDARD_MAP && s == 2) || (sys->kind == CL_SYS_WRAPPED_STANDARD_MAP && s == 4);
+  if (!sys_ok || tau < 1 || N % tau || G < 1 || grid[G - 1] != N || !(d0 > 0 && cl_isfinite(d0))) return 10;
+  for (int g = 0; g < G; ++g)
+    if (grid[g] % tau || grid[g] <= (g ? grid[g - 1] : 0)) return 10;
+  if (math == MATH_CL)
+    shadow_record_all<F64Cl>(sys, m, s, tau, d0, grid, G, x0, u0, R, fail, (uint64_t*)ends, nthreads);
+  else
+    shadow_record_all<F64Libm>(sys, m, s, tau, d0, grid, G, x0, u0, R, fail, (uint64_t*)ends, nthreads);
+  return 0;
 }
 
 // A linear map x' = A x (A column-major 2 x 2): the closed-form fixtures of the reference's suites

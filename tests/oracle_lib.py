@@ -43,6 +43,11 @@ def load():
         o.or_record.restype = I
         o.or_record.argtypes = [ctypes.POINTER(L.cl_system), ctypes.POINTER(L.cl_chain), I64, I, I, I64, I, P, I, P, P,
                                 P, P, P, P, P, I, I]
+        o.or_shadow.restype = I
+        o.or_shadow.argtypes = [ctypes.POINTER(L.cl_system), P, I64, I, I64, I64, I64, I, ctypes.c_double, P, P, P, I, I]
+        o.or_shadow_record.restype = I
+        o.or_shadow_record.argtypes = [ctypes.POINTER(L.cl_system), I64, I, I64, I, ctypes.c_double, P, I, P, P, P, P,
+                                       P, I, I]
         o.or_csum.restype = None
         o.or_csum.argtypes = [I64, ctypes.c_double, P, P]
         o.or_csum_list.restype = None
@@ -164,3 +169,47 @@ def record(sysd, chain, x0, W0, N, tau, k, grid, math=MATH_CL, nthreads=None):
                      _ptr(R), _ptr(fail), _ptr(code), _ptr(hstep), _ptr(states), math, nt)
     assert rc == 0, rc
     return dict(R=R, fail=fail, x=x, W=W.transpose(0, 2, 1).copy(), states=states[:, :nb], code=code, hstep=hstep)
+
+
+def shadow_ends(raw, s):
+    """End records (m, 2s + 6) uint64 -> dict of x, y, sum, comp, dprev, code, step, value."""
+    f = raw.view(np.float64)
+    return dict(x=f[:, :s].copy(), y=f[:, s:2 * s].copy(), sum=f[:, 2 * s].copy(), comp=f[:, 2 * s + 1].copy(),
+                dprev=f[:, 2 * s + 2].copy(), code=(raw[:, 2 * s + 3] & 0xff).astype(np.uint8).view(np.int8),
+                step=raw[:, 2 * s + 4].view(np.int64).copy(), value=f[:, 2 * s + 5].copy(), raw=raw)
+
+
+def shadow(sysd, x0, u0, N, tau, d0, *, n0=0, steps=None, ends=None, A=None, math=MATH_CL, nthreads=None):
+    """The two-trajectory estimator (oracle/oracle.cpp, or_shadow): one launch of `steps` (default N - n0) from
+    global step n0 over x0 (m, s) and unit directions u0 (m, 2), continuing `ends` (m, 2s + 6 words) when n0 > 0;
+    with A (2 x 2) the linear map x' = A x. Returns the decoded end records."""
+    o = load()
+    x = np.ascontiguousarray(np.asarray(x0, dtype=np.float64))
+    m, s = x.shape
+    u = np.ascontiguousarray(np.asarray(u0, dtype=np.float64))
+    e = np.zeros((m, 2 * s + 6), dtype=np.uint64) if ends is None else np.array(ends, dtype=np.uint64, copy=True)
+    Ac = None if A is None else np.ascontiguousarray(np.asarray(A, dtype=np.float64).T.ravel())
+    st = int(N) - n0 if steps is None else int(steps)
+    nt = nthreads or min(16, os.cpu_count() or 1)
+    rc = o.or_shadow(ctypes.byref(sysd) if sysd is not None else None, _ptr(Ac), m, s, n0, st, int(N), tau, float(d0),
+                     _ptr(x), _ptr(u), _ptr(e), math, nt)
+    assert rc == 0, rc
+    return shadow_ends(e, s)
+
+
+def shadow_record(sysd, x0, u0, N, tau, d0, grid, math=MATH_CL, nthreads=None):
+    """The recorder's two-trajectory rows (or_shadow_record): R (m, G, 8 + s), fail and the end records."""
+    o = load()
+    x = np.ascontiguousarray(np.asarray(x0, dtype=np.float64))
+    m, s = x.shape
+    u = np.ascontiguousarray(np.asarray(u0, dtype=np.float64))
+    g = np.ascontiguousarray(np.asarray(grid, dtype=np.int64))
+    G = len(g)
+    R = np.zeros((m, G, 8 + s))
+    fail = np.zeros(m, dtype=np.int64)
+    e = np.zeros((m, 2 * s + 6), dtype=np.uint64)
+    nt = nthreads or min(16, os.cpu_count() or 1)
+    rc = o.or_shadow_record(ctypes.byref(sysd), m, s, int(N), tau, float(d0), _ptr(g), G, _ptr(x), _ptr(u), _ptr(R),
+                            _ptr(fail), _ptr(e), math, nt)
+    assert rc == 0, rc
+    return dict(R=R, fail=fail, ends=shadow_ends(e, s))

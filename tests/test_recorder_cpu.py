@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from chaos_amd import StandardMap, WrappedStandardMap, RunMonitor, torus_state
-from chaos_amd.recorder import record_grid, deviation_frame, estimator, vectors, Records, METHODS
+from chaos_amd.recorder import record_grid, deviation_frame, estimator, vectors, Records, METHODS, VARIATIONAL
 from chaos_amd.tangent import make_chain, decode_states, finals_of
 from tests import oracle_lib as O
 
@@ -56,10 +56,11 @@ def test_deviation_frame_ex_chunks():
 
 
 def test_estimators_and_vectors():
-    assert set(METHODS) == {"L1", "L3", "L4", "L4b", "L5", "L5b"}
-    assert vectors("L1") == 1 and vectors("L4") == 2
-    with pytest.raises(ValueError):
-        estimator("L9")
+    assert set(METHODS) == {"L1", "L2", "L3", "L4", "L4b", "L5", "L5b"} and set(VARIATIONAL) == set(METHODS) - {"L2"}
+    assert vectors("L1") == 1 and vectors("L2") == 1 and vectors("L4") == 2
+    for bad in ("L9", "L2"):
+        with pytest.raises(ValueError):
+            estimator(bad)
 
 
 def starts(m, wrapped, seed=3):
