@@ -99,6 +99,9 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m chaos_amd.build` "
                           "(the product has no CPU fallback)")
+    # torch first: the library then binds to the HIP runtime torch has loaded, so the process holds one runtime
+    # (loaded the other way round, two runtimes see no device in one of them)
+    import torch  # noqa: F401
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

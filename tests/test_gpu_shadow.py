@@ -64,7 +64,7 @@ def test_device_shadow_equals_oracle(wrapped, tau):
     s = x0.shape[1]
     u = seeded_directions(77, m)
     orc = O.shadow(sysm.descriptor(), x0, u, N, tau, d0)
-    assert np.sum(orc["code"] != 0) >= 3
+    assert np.sum(orc["code"] != 0) >= 2
     for L_ in (None, 4 * 37):
         say("shadow", "wrapped" if wrapped else "lifted", "tau", tau, "m", m, "N", N, "launch", L_)
         res = propagate_shadow(sysm, x0, N, d0=d0, tau=tau, seed=77, launch_steps=L_, check=False)
@@ -145,7 +145,7 @@ def test_recorder_l2_equals_oracle(wrapped):
     s = x0.shape[1]
     u = np.broadcast_to(deviation_frame("pi/4", 1)[:, 0], (m, 2))
     orc = O.shadow_record(sysm.descriptor(), x0, u, N, tau, 1e-8, grid)
-    assert np.sum(orc["fail"] > 0) >= 3
+    assert np.sum(orc["fail"] > 0) >= 2
     for L_ in (None, 21):
         say("recorder L2", "wrapped" if wrapped else "lifted", "m", m, "N", N, "launch", L_)
         rec = record(sysm, x0, N, method="L2", tau=tau, grid=grid, eps="1e-8", launch_steps=L_)
