@@ -9,8 +9,10 @@ from .reducers import (QRSpectrum, CGS, MGS, RGS, HouseholderQR, AugmentedQR, Gi
                        eps_sum, eps_pair, check_residual)
 from .tangent import propagate, propagate_ragged, PropagationResult, NumericalHealthError, initial_deviations
 from .two_trajectory import propagate_shadow
+from .reducers import StretchingNumbers, stretching_range, spectral_distance, spectrum_density, spectrum_cell
 
 __all__ = ["StandardMap", "StandardMapK", "StandardMapChain", "WrappedStandardMap", "HenonHeilesGauss", "hh_energy", "torus_state", "QRSpectrum", "CGS", "MGS", "RGS", "HouseholderQR",
            "AugmentedQR", "GivensQR", "Benettin", "MEGNO", "SALI", "FLI", "OFLI", "GALI", "ResidualMonitor", "RunMonitor", "monitor_exponents", "det_threshold",
            "symp_threshold", "sum_threshold", "eps_sum", "eps_pair", "check_residual", "propagate", "propagate_ragged",
-           "PropagationResult", "NumericalHealthError", "initial_deviations", "propagate_shadow"]
+           "PropagationResult", "NumericalHealthError", "initial_deviations", "propagate_shadow",
+           "StretchingNumbers", "stretching_range", "spectral_distance", "spectrum_density", "spectrum_cell"]

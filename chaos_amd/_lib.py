@@ -16,6 +16,7 @@ CL_SYS_STANDARD_MAP_K = 5
 CL_RED_QR, CL_RED_BENETTIN, CL_RED_MEGNO, CL_RED_SALI, CL_RED_FLI, CL_RED_OFLI, CL_RED_GALI = 1, 2, 3, 4, 5, 6, 7
 CL_RED_MONITOR = 9
 CL_RED_RUNMONITOR = 10
+CL_RED_STRETCHING = 11
 CL_QR_CGS, CL_QR_MGS, CL_QR_RGS, CL_QR_HOUSEHOLDER, CL_QR_AUGMENTED, CL_QR_GIVENS = 1, 2, 3, 4, 5, 6
 CL_MAX_REDUCERS = 8
 
@@ -35,7 +36,8 @@ class cl_system(ctypes.Structure):
 
 class cl_reducer(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("alg", ctypes.c_int32), ("replace", ctypes.c_int32),
-                ("frame", ctypes.c_int32), ("tol", ctypes.c_double), ("tol2", ctypes.c_double)]
+                ("frame", ctypes.c_int32), ("tol", ctypes.c_double), ("tol2", ctypes.c_double),
+                ("lo", ctypes.c_double), ("hi", ctypes.c_double), ("bins", ctypes.c_int32), ("pad2_", ctypes.c_int32)]
 
 
 class cl_chain(ctypes.Structure):
@@ -64,6 +66,9 @@ SIGNATURES = {
                                I64, I, P, P, P, P, P, P, P]),
     "cl_propagate_device": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I64, I, I, I, I64, I64,
                                 I64, I, P, P, P, P, P, P, P]),
+    "cl_propagate_device_ws": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I64, I, I, I, I64, I64,
+                                   I64, I, P, P, P, P, P, P, P, P]),
+    "cl_workspace_entries": (I64, [ctypes.POINTER(cl_chain), I, I]),
     "cl_propagate_device_traced": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I64, I, I, I, I64,
                                        I64, I64, I, P, P, P, P, P, P, P, I64, I, P]),
     "cl_propagate_ragged": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I64, I, I, I, P, P, I,

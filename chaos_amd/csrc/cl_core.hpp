@@ -1151,4 +1151,77 @@ struct RunMon {
   }
 };
 
+// ------------------------------------------------------------------------------ StretchingNumbers (k = 2)
+// StretchingNumbers (src/Indicators/stretching.jl:34-147): at each handoff n = K tau the two arriving columns give
+// a_j = log(alpha_j) (the column logs MEGNO, FLI and SALI share, so the sums are bitwise theirs); a_j and a_j^2 go into
+// compensated sums and the stretching number z_j = a_j / tau is counted in a cell of a cumulative histogram in the
+// orbit's UInt32 workspace: bins cells of width h = (hi - lo) / bins over [lo, hi], cell 0 below, bins + 1 above
+// (spectrum_cell, :104-108). Column j's cells are the workspace entries (j - 1)(bins + 2) + cell.
+struct StretchState {
+  CSum s1, s2, q1, q2;
+  double value;
+  int64_t n, K, step;
+  int8_t code, column;
+  int8_t pad_[6];
+};
+constexpr int64_t SN_MAX_HANDOFFS = 1000000000;  // 4-byte counts (stretching.jl:91)
+
+template <int D>
+struct Stretch {
+  bool replace;
+  double lo, hi;
+  int bins;
+  typedef StretchState State;
+  CL_HD void init(State& st) const {
+    st.s1 = csum_zero(); st.s2 = csum_zero(); st.q1 = csum_zero(); st.q2 = csum_zero();
+    st.value = 0.0;
+    st.n = 0; st.K = 0; st.step = 0;
+    st.code = 0; st.column = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) st.pad_[i] = 0;
+  }
+  CL_HD int64_t cell(double z) const {
+    if (z < lo) return 0;
+    if (z > hi) return (int64_t)bins + 1;
+    const int64_t c = (int64_t)cl_floor((z - lo) / ((hi - lo) / (double)bins)) + 1;
+    return c < (int64_t)bins ? c : (int64_t)bins;
+  }
+  // ws: the orbit's first workspace entry, entries `stride` apart
+  template <int K>
+  CL_HD void step_logs(State& st, int64_t n, Frame<D, K>& W, ColLogs& c, uint32_t* ws, int64_t stride) const {
+    if (st.code != 0) return;
+    if (!(c.ok1 && c.ok2)) {
+      st.column = c.ok1 ? 2 : 1;
+      st.value = c.ok1 ? c.a2 : c.a1;
+      st.step = n;
+      st.code = 1;
+      return;
+    }
+    const int64_t K1 = st.K + 1;
+    if (K1 > SN_MAX_HANDOFFS) {
+      st.value = (double)K1;
+      st.step = n;
+      st.code = 2;
+      st.column = 0;
+      return;
+    }
+    const double tau = (double)(n / K1);
+    ws[cell(c.l1 / tau) * stride] += 1u;
+    ws[((int64_t)bins + 2 + cell(c.l2 / tau)) * stride] += 1u;
+    st.s1 = csum_add(st.s1, c.l1);
+    st.s2 = csum_add(st.s2, c.l2);
+    st.q1 = csum_add(st.q1, c.l1 * c.l1);
+    st.q2 = csum_add(st.q2, c.l2 * c.l2);
+    st.n = n;
+    st.K = K1;
+    if (replace) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        W.at(i, 0) = W.at(i, 0) / c.a1;
+        W.at(i, 1) = W.at(i, 1) / c.a2;
+      }
+    }
+  }
+};
+
 }  // namespace cl

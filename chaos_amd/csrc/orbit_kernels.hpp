@@ -30,6 +30,8 @@ struct KArgs {
   int trace_len;
   // optional orbit mask: orbits with mask[i] == 0 are left untouched
   const uint8_t* mask;
+  // reducer workspace [entries][m] (UInt32), or null when the chain requests none
+  uint32_t* ws;
 };
 
 // Per-orbit context the reducers read: the point, the carried sin/cos, the frames and a cache of
@@ -42,6 +44,8 @@ struct OrbitCtx {
   Frame<Sys::D, K> W[NF];
   bool logs_valid[NF];
   ColLogs logs[NF];
+  uint32_t* ws;  // the orbit's first workspace entry (entries m apart)
+  int64_t wstride;
 };
 
 // ------------------------------------------------------------------------------ chain slots
@@ -127,6 +131,29 @@ CL_HD void slot_step(Slot<Monitor<D, K>, FR>& s, int64_t n, Ctx& c) {
 }
 
 template <int D, int FR, class Ctx>
+CL_HD void slot_step(Slot<Stretch<D>, FR>& s, int64_t n, Ctx& c) {
+  ensure_logs<D, 2, 0>(c, FR);
+  s.r.step_logs(s.st, n, c.W[FR], c.logs[FR], c.ws, c.wstride);
+  if (s.r.replace) c.logs_valid[FR] = false;
+}
+
+// The workspace of a fresh run is zeroed by the first launch (src/Tangent/propagate.jl:29-31).
+template <class R, int FR, class Ctx>
+CL_HD void slot_ws_init(const Slot<R, FR>&, Ctx&) {}
+template <int D, int FR, class Ctx>
+CL_HD void slot_ws_init(const Slot<Stretch<D>, FR>& s, Ctx& c) {
+  const int64_t e = 2 * ((int64_t)s.r.bins + 2);
+  for (int64_t j = 0; j < e; ++j) c.ws[j * c.wstride] = 0u;
+}
+template <class Ctx>
+CL_HD void chain_ws_init(const Nil&, Ctx&) {}
+template <class H, class T, class Ctx>
+CL_HD void chain_ws_init(const Cons<H, T>& ch, Ctx& c) {
+  slot_ws_init(ch.head, c);
+  chain_ws_init(ch.tail, c);
+}
+
+template <int D, int FR, class Ctx>
 CL_HD void slot_step(Slot<RunMon<D>, FR>& s, int64_t n, Ctx& c) {
   double J[D * D];
   c.sys.jacobian(c.aux, J);
@@ -150,6 +177,13 @@ template <int D, int K>
 CL_HD void slot_setup(GALI<D, K>&, const cl_reducer&) {}
 template <int D>
 CL_HD void slot_setup(RunMon<D>&, const cl_reducer&) {}
+template <int D>
+CL_HD void slot_setup(Stretch<D>& r, const cl_reducer& d) {
+  r.replace = d.replace != 0;
+  r.lo = d.lo;
+  r.hi = d.hi;
+  r.bins = d.bins;
+}
 template <int D, int K>
 CL_HD void slot_setup(Monitor<D, K>& r, const cl_reducer& d) {
   r.det_tol = d.tol;
@@ -275,8 +309,14 @@ __global__ void __launch_bounds__(256) orbit_kernel(KArgs a, Sys sys, Chain chai
     for (int e = 0; e < D * K; ++e) c.W[NF - 1].w[e] = a.W1[(int64_t)e * a.m + i];
   }
   Chain ch = chain0;
-  if (a.n0 == 0) chain_init(ch);
-  else chain_load(ch, a.st, a.m, i, 0);
+  c.ws = a.ws ? a.ws + i : nullptr;
+  c.wstride = a.m;
+  if (a.n0 == 0) {
+    chain_init(ch);
+    chain_ws_init(ch, c);
+  } else {
+    chain_load(ch, a.st, a.m, i, 0);
+  }
   int64_t n = a.n0;
   int8_t code = STATUS_OK;
   bool early = false;

@@ -117,10 +117,42 @@ static int validate(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int e
   if (s != state_dim(sys->kind)) return set_err(ctx, CL_ERR_ARGUMENT, "state length does not match the system");
   if (!(k >= 1 && k <= d)) return set_err(ctx, CL_ERR_ARGUMENT, "k must be in 1:d");
   if (elt != CL_ELT_F64) return set_err(ctx, CL_ERR_UNSUPPORTED, "element type has no kernel for this system");
+  int nws = 0;
+  for (int r = 0; r < ch->n; ++r) {
+    const cl_reducer& q = ch->r[r];
+    if (q.kind != CL_RED_STRETCHING) continue;
+    ++nws;
+    if (k != 2) return set_err(ctx, CL_ERR_ARGUMENT, "StretchingNumbers needs k = 2");
+    if (!(q.bins >= 1 && q.lo < q.hi && q.hi - q.lo < __builtin_inf() && q.lo > -__builtin_inf()))
+      return set_err(ctx, CL_ERR_ARGUMENT, "StretchingNumbers needs bins >= 1 and a finite range lo < hi");
+    if ((n0 + steps) / tau > 1000000000) return set_err(ctx, CL_ERR_ARGUMENT, "more than 10^9 handoffs");
+  }
+  if (nws > 1) return set_err(ctx, CL_ERR_ARGUMENT, "at most one reducer of a chain may request a workspace");
   const Instance* in = find_instance(sys, ch, d, k);
   if (!in) return set_err(ctx, CL_ERR_UNSUPPORTED, "no kernel instance for this system/chain/k");
   *out = in;
   return CL_OK;
+}
+
+// UInt32 workspace entries per orbit (_chain_workspace, src/Tangent/orbit.jl:8-18): 2 (bins + 2) for
+// StretchingNumbers (stretching.jl:93-96), 0 when no reducer requests one, -1 when two do.
+static int64_t ws_entries(const cl_chain* ch) {
+  int64_t e = 0;
+  int c = 0;
+  for (int r = 0; r < ch->n && r < CL_MAX_REDUCERS; ++r)
+    if (ch->r[r].kind == CL_RED_STRETCHING) {
+      e = 2 * ((int64_t)ch->r[r].bins + 2);
+      ++c;
+    }
+  return c > 1 ? -1 : e;
+}
+
+extern "C" int64_t cl_workspace_entries(const cl_chain* ch, int d, int k) {
+  (void)d;
+  (void)k;
+  if (!ch || ch->n < 0 || ch->n > CL_MAX_REDUCERS) return -CL_ERR_ARGUMENT;
+  const int64_t e = ws_entries(ch);
+  return e < 0 ? -CL_ERR_ARGUMENT : e;
 }
 
 // The coupled chain (kind CL_SYS_STANDARD_MAP_CHAIN): state 2N, frames 2N x k, no reducer or
@@ -179,6 +211,20 @@ __global__ void k_aos_to_soa(const uint64_t* __restrict__ a, uint64_t* __restric
   s[t] = a[i * w + j];
 }
 __global__ void k_soa_to_aos(const uint64_t* __restrict__ s, uint64_t* __restrict__ a, int64_t m, int w) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m * w) return;
+  int64_t j = t / m, i = t - j * m;
+  a[i * w + j] = s[t];
+}
+
+// the same for 4-byte words (the reducer workspace)
+__global__ void k_aos_to_soa32(const uint32_t* __restrict__ a, uint32_t* __restrict__ s, int64_t m, int64_t w) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m * w) return;
+  int64_t j = t / m, i = t - j * m;
+  s[t] = a[i * w + j];
+}
+__global__ void k_soa_to_aos32(const uint32_t* __restrict__ s, uint32_t* __restrict__ a, int64_t m, int64_t w) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= m * w) return;
   int64_t j = t / m, i = t - j * m;
@@ -293,9 +339,12 @@ static int grow(cl_ctx* c, int slot, size_t bytes) {
 static int launch(const Instance* in, const cl_system* sys, const cl_chain* ch, int64_t m, double* dx,
                   double* dW, double* dW2, uint64_t* dst, int8_t* dcode, int64_t* dhstep, int64_t n0,
                   int64_t steps, int64_t N, int tau, hipStream_t st, float* trace = nullptr,
-                  int64_t trace_n0 = 0, int trace_len = 0, const uint8_t* mask = nullptr) {
+                  int64_t trace_n0 = 0, int trace_len = 0, const uint8_t* mask = nullptr,
+                  uint32_t* ws = nullptr) {
   if (m == 0) return CL_OK;
   KArgs a;
+  memset(&a, 0, sizeof(a));
+  a.ws = ws;
   a.mask = mask;
   a.trace = trace;
   a.trace_n0 = trace_n0;
@@ -477,7 +526,9 @@ static int launch_pipe(const Instance* in, const cl_system* sys, const cl_chain*
 // for a small single-QR ensemble, else the fused kernel. Every route gives the fused kernel's bits.
 static int launch_best(const Instance* in, const cl_system* sys, const cl_chain* ch, int64_t m, int s, int d, int k,
                        double* dx, double* dW, double* dW2, uint64_t* dst, int8_t* dcode, int64_t* dhstep,
-                       int64_t n0, int64_t steps, int64_t N, int tau, hipStream_t st) {
+                       int64_t n0, int64_t steps, int64_t N, int tau, hipStream_t st, uint32_t* ws = nullptr) {
+  if (ws)  // a workspace chain runs the fused kernel
+    return launch(in, sys, ch, m, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st, nullptr, 0, 0, nullptr, ws);
   int rc;
   if (in->nf > 1 && split_enabled(sys->kind)) {
     rc = launch_split(in, sys, ch, m, s, d, k, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st);
@@ -541,9 +592,32 @@ extern "C" int cl_propagate_device(cl_ctx* ctx, const cl_system* sys, const cl_c
   int rc = validate(ctx, sys, ch, elt, m, s, d, k, n0, steps, N, tau, &in);
   if (rc) return rc;
   if (in->nf > 1 && !dW2) return set_err(ctx, CL_ERR_ARGUMENT, "the chain uses a second frame: W2 is required");
+  if (ws_entries(ch) != 0)
+    return set_err(ctx, CL_ERR_ARGUMENT, "the chain requests a workspace: call cl_propagate_device_ws");
   hipSetDevice(ctx->device);
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
   rc = launch_best(in, sys, ch, m, s, d, k, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st);
+  if (rc) return set_err(ctx, rc, "kernel launch failed");
+  return CL_OK;
+}
+
+extern "C" int cl_propagate_device_ws(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m,
+                                      int s, int d, int k, int64_t n0, int64_t steps, int64_t N, int tau,
+                                      double* dx, double* dW, double* dW2, uint64_t* dst, int8_t* dcode,
+                                      int64_t* dhstep, uint32_t* dws, void* stream) {
+  if (!ctx) return CL_ERR_ARGUMENT;
+  if (sys && sys->kind == CL_SYS_STANDARD_MAP_CHAIN)
+    return set_err(ctx, CL_ERR_UNSUPPORTED, "chain kernels run no workspace reducer");
+  const Instance* in = nullptr;
+  int rc = validate(ctx, sys, ch, elt, m, s, d, k, n0, steps, N, tau, &in);
+  if (rc) return rc;
+  if (in->nf > 1 && !dW2) return set_err(ctx, CL_ERR_ARGUMENT, "the chain uses a second frame: W2 is required");
+  const int64_t e = ws_entries(ch);
+  if (e == 0) return set_err(ctx, CL_ERR_ARGUMENT, "the chain requests no workspace: call cl_propagate_device");
+  if (m > 0 && !dws) return set_err(ctx, CL_ERR_ARGUMENT, "null workspace");
+  hipSetDevice(ctx->device);
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  rc = launch_best(in, sys, ch, m, s, d, k, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st, dws);
   if (rc) return set_err(ctx, rc, "kernel launch failed");
   return CL_OK;
 }
@@ -566,7 +640,7 @@ extern "C" int cl_propagate_device_traced(cl_ctx* ctx, const cl_system* sys, con
   return CL_OK;
 }
 
-enum { B_X, B_W, B_W2, B_ST, B_CODE, B_H, B_XA, B_WA, B_W2A, B_STA };
+enum { B_X, B_W, B_W2, B_ST, B_CODE, B_H, B_XA, B_WA, B_W2A, B_STA, B_WS, B_WSA };
 
 extern "C" int cl_propagate_ragged(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m, int s,
                                    int d, int k, const int64_t* Ns, const int64_t* order, int tau, double* dx,
@@ -744,7 +818,6 @@ extern "C" int cl_propagate_chunk(cl_ctx* ctx, const cl_system* sys, const cl_ch
                                   int s, int d, int k, int64_t n0, int64_t steps, int64_t N, int tau, double* x,
                                   double* W, double* W2, void* states, int8_t* code, int64_t* hstep,
                                   uint32_t* ws) {
-  (void)ws;
   if (!ctx) return CL_ERR_ARGUMENT;
   if (sys && ch && sys->kind == CL_SYS_STANDARD_MAP_CHAIN) {
     if (W2) return set_err(ctx, CL_ERR_UNSUPPORTED, "chain kernels carry one frame");
@@ -754,11 +827,19 @@ extern "C" int cl_propagate_chunk(cl_ctx* ctx, const cl_system* sys, const cl_ch
   int rc = validate(ctx, sys, ch, elt, m, s, d, k, n0, steps, N, tau, &in);
   if (rc) return rc;
   if (m == 0) return CL_OK;
-  if (!x || !W || !code || !hstep || (in->words > 0 && !states) || (in->nf > 1 && !W2))
+  const int64_t nws = ws_entries(ch);
+  if (!x || !W || !code || !hstep || (in->words > 0 && !states) || (in->nf > 1 && !W2) || (nws > 0 && !ws))
     return set_err(ctx, CL_ERR_ARGUMENT, "null buffer");
   hipSetDevice(ctx->device);
   hipStream_t st = ctx->stream;
   const int dk = d * k, nw = in->words;
+  uint32_t *dws = nullptr, *wsa = nullptr;
+  const size_t bws = (size_t)m * (size_t)(nws > 0 ? nws : 0) * 4;
+  if (nws > 0) {
+    if ((rc = grow(ctx, B_WS, bws)) || (rc = grow(ctx, B_WSA, bws))) return set_err(ctx, rc, "device allocation failed");
+    dws = (uint32_t*)ctx->buf[B_WS];
+    wsa = (uint32_t*)ctx->buf[B_WSA];
+  }
   size_t bx = (size_t)m * s * 8, bw = (size_t)m * dk * 8, bs = (size_t)m * nw * 8;
   if ((rc = grow(ctx, B_X, bx)) || (rc = grow(ctx, B_W, bw)) || (rc = grow(ctx, B_ST, bs)) ||
       (rc = grow(ctx, B_CODE, (size_t)m)) || (rc = grow(ctx, B_H, (size_t)m * 8)) || (rc = grow(ctx, B_XA, bx)) ||
@@ -792,10 +873,18 @@ extern "C" int cl_propagate_chunk(cl_ctx* ctx, const cl_system* sys, const cl_ch
     }
     CK(hipMemcpyAsync(dcode, code, (size_t)m, hipMemcpyHostToDevice, st));
     CK(hipMemcpyAsync(dh, hstep, (size_t)m * 8, hipMemcpyHostToDevice, st));
+    if (nws > 0) {
+      CK(hipMemcpyAsync(wsa, ws, bws, hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_aos_to_soa32, dim3(grid_for(m * nws, B)), dim3(B), 0, st, wsa, dws, m, nws);
+    }
   }
   rc = launch_best(in, sys, ch, m, s, d, k, (double*)dx, (double*)dW, (double*)dW2, dst, dcode, dh, n0, steps, N,
-                   tau, st);
+                   tau, st, dws);
   if (rc) return set_err(ctx, rc, "kernel launch failed");
+  if (nws > 0) {
+    hipLaunchKernelGGL(k_soa_to_aos32, dim3(grid_for(m * nws, B)), dim3(B), 0, st, dws, wsa, m, nws);
+    CK(hipMemcpyAsync(ws, wsa, bws, hipMemcpyDeviceToHost, st));
+  }
   hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(m * s, B)), dim3(B), 0, st, dx, xa, m, s);
   hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(m * dk, B)), dim3(B), 0, st, dW, wa, m, dk);
   CK(hipMemcpyAsync(x, xa, bx, hipMemcpyDeviceToHost, st));

@@ -99,6 +99,10 @@ template <int D>
 struct Kind<RunMon<D>> {
   static constexpr int kind = CL_RED_RUNMONITOR, alg = 0;
 };
+template <int D>
+struct Kind<Stretch<D>> {
+  static constexpr int kind = CL_RED_STRETCHING, alg = 0;
+};
 
 inline void fill_key(Instance&, const Nil&, int) {}
 template <class H, class T>

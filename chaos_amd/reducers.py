@@ -428,3 +428,182 @@ class RunMonitor(Reducer):
 
     def final(self, st):
         return dict(det_max=st["det_max"].copy(), symp_max=st["symp_max"].copy(), M_max=st["M_max"].copy())
+
+
+# ------------------------------------------------------------------------------ StretchingNumbers
+SN_MAX_HANDOFFS = 10 ** 9
+SN_CONVENTION = ("cumulative spectra, D at every interval, columns renormalised separately, no orthogonalisation, "
+                 "overflow cells of width h")
+
+
+def spectrum_cell(z, lo, hi, B):
+    """The cell of a stretching number (spectrum_cell, src/Indicators/stretching.jl:104-108): 0 below lo, B + 1 above
+    hi, else min(floor((z - lo) / h) + 1, B) with h = (hi - lo) / B, in float64."""
+    z, lo, hi = float(z), float(lo), float(hi)
+    if z < lo:
+        return 0
+    if z > hi:
+        return int(B) + 1
+    return min(int(np.floor((z - lo) / ((hi - lo) / float(B)))) + 1, int(B))
+
+
+def spectral_distance(N1, N2, h):
+    """Spectral distance D = sqrt(S) / (k sqrt(h)) of two count vectors (spectral_distance, stretching.jl:189-204):
+    S the exact integer sum of squared differences, k the common total; 0 for total 0."""
+    a, b = np.asarray(N1, dtype=np.int64), np.asarray(N2, dtype=np.int64)
+    if a.shape != b.shape:
+        raise ValueError(f"count vectors of lengths {a.size} and {b.size}")
+    k1, k2 = int(a.sum()), int(b.sum())
+    if k1 != k2:
+        raise ValueError(f"count vectors must have equal totals, got {k1} and {k2}")
+    if k1 == 0:
+        return 0.0
+    S = int(((a - b) ** 2).sum())
+    return float(np.sqrt(np.float64(S))) / (k1 * float(np.sqrt(np.float64(h))))
+
+
+def spectrum_density(counts, h):
+    """Normalised spectrum N_c / (k h) of cells 1..B of a count column of length B + 2 (stretching.jl:212-215)."""
+    c = np.asarray(counts, dtype=np.int64)
+    k = int(c.sum())
+    return np.zeros(c.size - 2) if k == 0 else c[1:-1].astype(np.float64) / (k * h)
+
+
+def stretching_range(sys):
+    """Derived symmetric range (-hi, hi) for the standard maps (stretching_range, stretching.jl:235-241): hi just above
+    L_sigma = ln sigma(K), sigma(K) = (sqrt(S2) + sqrt(S2 - 4)) / 2 with S2 = 2 K^2 + 2 |K| + 5, padded by 8 eps
+    relative, computed at 256 bits and rounded once."""
+    import mpmath as mp
+    with mp.workprec(256):
+        K = mp.mpf(float(sys.K))
+        S2 = 2 * K * K + 2 * abs(K) + 5
+        L = mp.log((mp.sqrt(S2) + mp.sqrt(S2 - 4)) / 2)
+        hi = float(L * (1 + 8 * mp.mpf(2) ** -52))
+    return (-hi, hi)
+
+
+class StretchingNumbers(Reducer):
+    """StretchingNumbers(; range, bins=1000, interval=100, min_per_cell=100, replace=True) for k = 2
+    (src/Indicators/stretching.jl:34-147): cumulative spectra of the per-handoff log growth of the two columns in a
+    UInt32 workspace of 2 (bins + 2) counts per orbit, and the compensated sums of a and a^2."""
+    kind = L.CL_RED_STRETCHING
+
+    def __init__(self, *, range, bins=1000, interval=100, min_per_cell=100, replace=True):
+        lo, hi = (float(v) for v in range)
+        if not (np.isfinite(lo) and np.isfinite(hi) and lo < hi):
+            raise ValueError(f"range must be finite with s_lo < s_hi, got ({lo}, {hi})")
+        if int(bins) < 1:
+            raise ValueError(f"bins must be at least 1, got {bins}")
+        if int(bins) > 2 ** 30:
+            raise ValueError(f"bins must fit the descriptor (at most 2^30), got {bins}")
+        if int(interval) < 1:
+            raise ValueError(f"interval must be at least 1 map step, got {interval}")
+        if int(min_per_cell) < 0:
+            raise ValueError(f"min_per_cell must be nonnegative, got {min_per_cell}")
+        self.lo, self.hi, self.bins = lo, hi, int(bins)
+        self.interval, self.min_per_cell, self.replace = int(interval), int(min_per_cell), bool(replace)
+
+    def descriptor(self, frame):
+        r = super().descriptor(frame)
+        r.lo, r.hi, r.bins = self.lo, self.hi, self.bins
+        return r
+
+    def check(self, d, k):
+        _k2("StretchingNumbers", k)
+
+    def workspace(self, d, k):
+        """reducer_workspace (stretching.jl:93-96): 2 (bins + 2) UInt32 entries per orbit."""
+        return 2 * (self.bins + 2)
+
+    def check_run(self, tau, N):
+        """check_run (stretching.jl:66-70): interval a multiple of tau, at most 10^9 handoffs."""
+        if self.interval % int(tau):
+            raise ValueError(f"interval {self.interval} must be a multiple of tau = {tau}")
+        if int(N) // int(tau) > SN_MAX_HANDOFFS:
+            raise ValueError(f"a run of {int(N) // int(tau)} handoffs exceeds the limit of 10^9 handoffs")
+
+    def check_frame(self, W0, orbit=0):
+        """check_frame (stretching.jl:258-275): both columns unit within 8 eps, then not parallel; returns the larger
+        norm defect, else raises NumericalHealthError("initial frame") at step 0 naming the column."""
+        from .tangent import NumericalHealthError
+        W0 = np.asarray(W0, dtype=np.float64)
+        if W0.ndim != 2 or W0.shape[1] != 2:
+            raise ValueError("StretchingNumbers needs a frame with 2 columns")
+        thr = 8 * EPS64
+        defect = 0.0
+        for i in range(2):
+            dv = abs(_snorm(W0[:, i]) - 1)
+            if not (np.isfinite(dv) and dv <= thr):
+                raise NumericalHealthError("initial frame", float(dv), thr, 0, orbit, "cpu", i + 1)
+            defect = max(defect, dv)
+        para = min(_snorm(W0[:, 0] - W0[:, 1]), _snorm(W0[:, 0] + W0[:, 1]))
+        if not para > thr:
+            raise NumericalHealthError("initial frame", float(para), thr, 0, orbit, "cpu", 2)
+        return defect
+
+    def dtype(self, d, k):
+        return np.dtype([("s1", "<f8", 2), ("s2", "<f8", 2), ("q1", "<f8", 2), ("q2", "<f8", 2), ("value", "<f8"),
+                         ("n", "<i8"), ("K", "<i8"), ("step", "<i8"), ("code", "i1"), ("column", "i1"),
+                         ("pad", "u1", 6)])
+
+    def final(self, st, ws=None):
+        """reducer_final (stretching.jl:165-185): (n, handoffs, sum, comp, sum2, comp2, mean, std) and, with the
+        workspace rows ws (m, 2 (bins + 2)), D, undersampled, counts ((bins + 2) x 2 per orbit), below, above."""
+        n, K = st["n"], st["K"]
+        zero = K == 0
+        nn = np.where(zero, 1, n).astype(np.float64)
+        tau = (n // np.where(zero, 1, K)).astype(np.float64)
+        out = dict(n=n.copy(), handoffs=K.copy(),
+                   sum=np.stack([st["s1"][:, 0], st["s2"][:, 0]], 1), comp=np.stack([st["s1"][:, 1], st["s2"][:, 1]], 1),
+                   sum2=np.stack([st["q1"][:, 0], st["q2"][:, 0]], 1),
+                   comp2=np.stack([st["q1"][:, 1], st["q2"][:, 1]], 1))
+        mean, std = [], []
+        for a, q in (("s1", "q1"), ("s2", "q2")):
+            m_ = np.where(zero, 0.0, _acc(st[a]) / nn)
+            v = _acc(st[q]) / (np.where(zero, 1.0, tau) * nn) - m_ * m_
+            mean.append(m_)
+            std.append(np.where(zero, 0.0, np.sqrt(np.maximum(v, 0.0))))
+        out["mean"], out["std"] = np.stack(mean, 1), np.stack(std, 1)
+        if ws is not None:
+            B = self.bins
+            ws = np.asarray(ws, dtype=np.uint32).reshape(len(n), 2, B + 2)
+            counts = ws.transpose(0, 2, 1).copy()
+            h = (self.hi - self.lo) / B
+            out["counts"] = counts
+            # spectral_distance per orbit, vectorised: both columns count one entry per handoff, so the totals agree
+            c64 = counts.astype(np.int64)
+            k1 = c64[:, :, 0].sum(axis=1)
+            S = ((c64[:, :, 0] - c64[:, :, 1]) ** 2).sum(axis=1)
+            with np.errstate(invalid="ignore", divide="ignore"):
+                D = np.sqrt(S.astype(np.float64)) / (k1.astype(np.float64) * float(np.sqrt(np.float64(h))))
+            out["D"] = np.where(k1 == 0, 0.0, D)
+            out["undersampled"] = K < self.min_per_cell * B
+            out["below"] = counts[:, 0, :].astype(np.int64)
+            out["above"] = counts[:, B + 1, :].astype(np.int64)
+        return out
+
+    def error(self, row):
+        v = float(row["value"])
+        if int(row["code"]) == 2:
+            return "handoffs", v, float(SN_MAX_HANDOFFS), int(row["step"]), 0
+        return "deviation matrix", v, 0.0 if np.isfinite(v) else np.inf, int(row["step"]), int(row["column"])
+
+    def provenance(self):
+        return dict(indicator="stretching", convention=SN_CONVENTION, spectrum_bins=self.bins, spectrum_lo=self.lo,
+                    spectrum_hi=self.hi, spectrum_cell_width=(self.hi - self.lo) / self.bins,
+                    spectrum_interval=self.interval, spectrum_min_per_cell=self.min_per_cell, replace=self.replace)
+
+
+def _snorm(v):
+    """safe_norm of a float64 vector (src/Foundation/norm.jl:8-12), NaN propagating."""
+    v = np.asarray(v, dtype=np.float64)
+    a = np.abs(v)
+    if np.any(np.isnan(a)):
+        return float("nan")
+    m = float(a.max())
+    if not (np.isfinite(m) and m > 0):
+        return m
+    s = 0.0
+    for x in v:
+        s += (x / m) * (x / m)
+    return m * float(np.sqrt(s))

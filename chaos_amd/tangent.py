@@ -42,6 +42,7 @@ class PropagationResult:
     code: np.ndarray = None
     hstep: np.ndarray = None
     states: np.ndarray = field(default=None, repr=False)
+    workspace: np.ndarray = field(default=None, repr=False)   # (m, entries) UInt32 of a workspace reducer
 
 
 _ctx = None
@@ -90,11 +91,25 @@ def decode_states(raw, allr, d, k):
     return [np.ascontiguousarray(raw[:, o:o + dt.itemsize]).view(dt)[:, 0] for o, dt in lay]
 
 
-def finals_of(states, allr, d, k):
+def finals_of(states, allr, d, k, ws=None):
+    """reducer_final of each reducer; the workspace reducer gets the orbit rows of the workspace `ws`."""
     out = []
     for (r, _), st in zip(allr, states):
-        out.append(r.final(st, d) if r.kind == L.CL_RED_GALI else r.final(st))
+        if r.kind == L.CL_RED_GALI:
+            out.append(r.final(st, d))
+        elif hasattr(r, "workspace"):
+            out.append(r.final(st, ws))
+        else:
+            out.append(r.final(st))
     return out
+
+
+def workspace_entries(allr, d, k):
+    """_chain_workspace (src/Tangent/orbit.jl:8-18): the entries of the one workspace reducer, 0 without one."""
+    ws = [r.workspace(d, k) for r, _ in allr if hasattr(r, "workspace")]
+    if len(ws) > 1:
+        raise ValueError(f"at most one reducer of a chain may request a workspace, got {len(ws)}")
+    return ws[0] if ws else 0
 
 
 def check_health(code, hstep, x, W, states, allr, orbit0=1):
@@ -147,12 +162,15 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
         if getattr(r, "sys", None) is not None and bytes(r.sys.descriptor()) != bytes(sysd):
             raise ValueError(f"{type(r).__name__} monitors {r.sys!r}, not the propagated {sys!r}")
     chain, allr = make_chain(reducers, reducers2)
+    nws = workspace_entries(allr, d, k)
     nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, d, k)
     if nbytes < 0:
         raise L.HIPError(-nbytes, "no kernel instance for this system, k and reducer chain")
     words = nbytes // 8
     nf = 2 if reducers2 else 1
     dev = torch.device("cuda")
+    if sysd.kind == L.CL_SYS_STANDARD_MAP_CHAIN and nws:
+        raise L.HIPError(3, "chain kernels run no workspace reducer")
     if sysd.kind == L.CL_SYS_STANDARD_MAP_CHAIN:
         if reducers2:
             raise L.HIPError(3, "chain kernels carry one frame")
@@ -172,14 +190,22 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
     dst = torch.zeros((max(words, 1), m), dtype=torch.int64, device=dev)
     dcode = torch.zeros(m, dtype=torch.int8, device=dev)
     dh = torch.zeros(m, dtype=torch.int64, device=dev)
+    # the workspace [entries][m] UInt32 (held as int32: torch has no uint32 arithmetic; the bits are the same)
+    dws = torch.empty((nws, m), dtype=torch.int32, device=dev) if nws else None
     L_ = N if launch_steps is None else min(int(launch_steps), N)
     n0, launches = 0, 0
     while n0 < N:
         steps = min(L_, N - n0)
-        rc = lib.cl_propagate_device(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, m, s, d, k, n0,
-                                     steps, N, tau, dx.data_ptr(), dW.data_ptr(),
-                                     dW2.data_ptr() if dW2 is not None else None, dst.data_ptr(), dcode.data_ptr(),
-                                     dh.data_ptr(), stream)
+        if dws is not None:
+            rc = lib.cl_propagate_device_ws(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, m, s, d, k,
+                                            n0, steps, N, tau, dx.data_ptr(), dW.data_ptr(),
+                                            dW2.data_ptr() if dW2 is not None else None, dst.data_ptr(),
+                                            dcode.data_ptr(), dh.data_ptr(), dws.data_ptr(), stream)
+        else:
+            rc = lib.cl_propagate_device(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, m, s, d, k, n0,
+                                         steps, N, tau, dx.data_ptr(), dW.data_ptr(),
+                                         dW2.data_ptr() if dW2 is not None else None, dst.data_ptr(),
+                                         dcode.data_ptr(), dh.data_ptr(), stream)
         if rc:
             raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
         n0 += steps
@@ -192,6 +218,7 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
     code = dcode.cpu().numpy()
     hstep = dh.cpu().numpy()
     states = decode_states(raw, allr, d, k)
+    ws = dws.cpu().numpy().view(np.uint32).T.copy() if dws is not None else None
     if check:
         check_health(code, hstep, x, W, states, allr)
     prov = dict(backend="gpu", device=device_name(), backend_fallback_reason="", precision="Float64",
@@ -199,8 +226,8 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
                 launches_per_chunk=launches)
     for r, _ in allr:
         prov.update(r.provenance())
-    return PropagationResult(x=x, W=W, finals=finals_of(states, allr, d, k), provenance=prov, W2=W2, code=code,
-                             hstep=hstep, states=raw if keep_states else None)
+    return PropagationResult(x=x, W=W, finals=finals_of(states, allr, d, k, ws), provenance=prov, W2=W2, code=code,
+                             hstep=hstep, states=raw if keep_states else None, workspace=ws)
 
 
 def _propagate_orbit_major(lib, ctx, sys, sysd, chain, allr, x0, N, k, tau, seed, W0, launch_steps, words, check,

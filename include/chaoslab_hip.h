@@ -78,6 +78,8 @@ typedef struct {
 #define CL_RED_GALI 7      /* GALI(), src/Indicators/gali.jl:190-374 */
 #define CL_RED_MONITOR 9   /* ResidualMonitor(sys, det_tol, orth_tol), src/Diagnostics/monitor.jl:9-54 */
 #define CL_RED_RUNMONITOR 10 /* RunMonitor(sys), the recorder's non-raising monitor, src/Experiments/run_record.jl:25-43 */
+#define CL_RED_STRETCHING 11 /* StretchingNumbers(; range, bins, interval, min_per_cell, replace), k = 2, with a
+                                UInt32 workspace of 2 (bins + 2) counts per orbit, src/Indicators/stretching.jl:34-147 */
 
 #define CL_QR_CGS 1
 #define CL_QR_MGS 2
@@ -94,6 +96,9 @@ typedef struct {
                        orbit, so a replacing indicator chain and a QR spectrum can share one pass) */
   double tol;       /* QRSpectrum orthogonality_tol; ResidualMonitor det_tol */
   double tol2;      /* ResidualMonitor orth_tol */
+  double lo, hi;    /* StretchingNumbers range (s_lo, s_hi) */
+  int32_t bins;     /* StretchingNumbers cells B (the workspace holds 2 (B + 2) counts per orbit) */
+  int32_t pad2_;
 } cl_reducer;
 
 #define CL_MAX_REDUCERS 8
@@ -133,7 +138,8 @@ int64_t cl_state_bytes(const cl_system *sys, const cl_chain *chain, int elt, int
  * including the host<->device copies of `_run_chunk!` (src/Tangent/propagate.jl:95-110).
  * n0 == 0 initialises the reducer states (the input states are ignored); an orbit whose code is not
  * 0 at n0 > 0 is left untouched. W2 is the second frame (NULL when no reducer uses frame 1).
- * ws is the reducer workspace (no reducer here requests one: pass NULL). Synchronous. */
+ * ws is the reducer workspace, ws[i][cl_workspace_entries(...)] UInt32 counts per orbit (the reference's
+ * ws[:, i]), zeroed by the launch at n0 == 0; NULL when the chain requests none. Synchronous. */
 int cl_propagate_chunk(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain, int elt,
                        int64_t m, int s, int d, int k, int64_t n0, int64_t steps, int64_t N, int tau,
                        double *x, double *W, double *W2, void *states, int8_t *code, int64_t *hstep,
@@ -145,6 +151,17 @@ int cl_propagate_device(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain
                         int64_t m, int s, int d, int k, int64_t n0, int64_t steps, int64_t N, int tau,
                         double *dx, double *dW, double *dW2, uint64_t *dstates, int8_t *dcode,
                         int64_t *dhstep, void *stream);
+
+/* cl_propagate_device for a chain with a workspace reducer (StretchingNumbers): dws[e][m], e < cl_workspace_entries,
+ * UInt32 (the reference's ws[:, i], word-major), zeroed by the launch at n0 == 0 and carried across launches like the
+ * states (src/Tangent/propagate.jl:29-50). cl_propagate_device refuses such a chain. */
+int cl_propagate_device_ws(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain, int elt,
+                           int64_t m, int s, int d, int k, int64_t n0, int64_t steps, int64_t N, int tau,
+                           double *dx, double *dW, double *dW2, uint64_t *dstates, int8_t *dcode,
+                           int64_t *dhstep, uint32_t *dws, void *stream);
+/* UInt32 workspace entries per orbit of the chain (_chain_workspace, src/Tangent/orbit.jl:8-18): 2 (bins + 2) for
+ * StretchingNumbers, 0 when no reducer requests one; -10 when two reducers request one. */
+int64_t cl_workspace_entries(const cl_chain *chain, int d, int k);
 
 /* cl_propagate_device plus a feature trace of the first `trace_len` handoffs after global step trace_n0:
  * float trace[3 * trace_len][m], rows 3(t-1)..3(t-1)+2 of handoff t = MEGNO's Y of column 1, FLI_B and

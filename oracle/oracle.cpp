@@ -166,8 +166,16 @@ static int red_bytes(const cl_reducer& r, int d, int k) {
     case 8: return 16 * k;  // LogNorm
     case CL_RED_MONITOR: return 88;
     case CL_RED_RUNMONITOR: return 24;
+    case CL_RED_STRETCHING: return 104;
   }
   return -1;
+}
+
+// UInt32 workspace entries of a chain: 2 (bins + 2) for StretchingNumbers (stretching.jl:93-96), else 0
+static int64_t chain_ws(const cl_chain& ch) {
+  for (int r = 0; r < ch.n; ++r)
+    if (ch.r[r].kind == CL_RED_STRETCHING) return 2 * ((int64_t)ch.r[r].bins + 2);
+  return 0;
 }
 
 // ------------------------------------------------------------------ reducer states in T
@@ -236,7 +244,15 @@ struct MonSt {
   int64_t step;
 };
 template <class T>
+struct StretchSt {  // StretchState{T} (stretching.jl:77-88)
+  Acc<T> s1, s2, q1, q2;
+  T value;
+  int64_t n, K, step;
+  int8_t code, column;
+};
+template <class T>
 struct RState {  // one of the above, by kind
+  StretchSt<T> sn;
   MonSt<T> mon;
   QRSt<T> qr;
   BenSt<T> ben;
@@ -276,6 +292,22 @@ static void write_state(const cl_reducer& r, const RState<T>& s, int d, int k, u
       b.put(dbl(s.mon.det_max));
       b.put(dbl(s.mon.symp_max));
       b.put(dbl(s.mon.M_max));
+      break;
+    }
+    case CL_RED_STRETCHING: {
+      const StretchSt<T>& q = s.sn;
+      const Acc<T>* a4[4] = {&q.s1, &q.s2, &q.q1, &q.q2};
+      for (int j = 0; j < 4; ++j) {
+        b.put(dbl(a4[j]->s));
+        b.put(dbl(a4[j]->c));
+      }
+      b.put(dbl(q.value));
+      b.put(q.n);
+      b.put(q.K);
+      b.put(q.step);
+      b.put(q.code);
+      b.put(q.column);
+      b.pad(6);
       break;
     }
     case CL_RED_QR: {
@@ -721,6 +753,50 @@ static void red_megno(const cl_reducer& r, MegSt<T>& st, int64_t n, T* W, int d)
     }
 }
 
+// StretchingNumbers (stretching.jl:119-138): the column logs, the cell of z = a / tau in the orbit's workspace
+// (spectrum_cell, :104-108; column j at offset (j - 1)(bins + 2)), the sums of a and a^2, the replacement.
+template <class P, class T>
+static int64_t sn_cell(T z, T lo, T hi, int64_t B) {
+  if (z < lo) return 0;
+  if (z > hi) return B + 1;
+  const int64_t c = (int64_t)P::floor((z - lo) / ((hi - lo) / (T)B)) + 1;
+  return c < B ? c : B;
+}
+template <class P, class T>
+static void red_stretch(const cl_reducer& r, StretchSt<T>& st, int64_t n, T* W, int d, uint32_t* ws) {
+  if (st.code) return;
+  T a1, a2;
+  if (!two_norms<P>(W, d, a1, a2, st.column, st.value)) {
+    st.step = n;
+    st.code = 1;
+    return;
+  }
+  const int64_t K = st.K + 1;
+  if (K > 1000000000) {
+    st.value = (T)K;
+    st.step = n;
+    st.code = 2;
+    st.column = 0;
+    return;
+  }
+  const T tau = (T)(n / K), lo = (T)r.lo, hi = (T)r.hi;
+  const int64_t B = r.bins;
+  const T l1 = P::log(a1), l2 = P::log(a2);
+  ws[sn_cell<P>(l1 / tau, lo, hi, B)] += 1u;
+  ws[B + 2 + sn_cell<P>(l2 / tau, lo, hi, B)] += 1u;
+  st.s1 = acc_add<P>(st.s1, l1);
+  st.s2 = acc_add<P>(st.s2, l2);
+  st.q1 = acc_add<P>(st.q1, l1 * l1);
+  st.q2 = acc_add<P>(st.q2, l2 * l2);
+  st.n = n;
+  st.K = K;
+  if (r.replace)
+    for (int i = 0; i < d; ++i) {
+      W[i] = W[i] / a1;
+      W[d + i] = W[d + i] / a2;
+    }
+}
+
 template <class P, class T>
 static void red_sali(SaliSt<T>& st, int64_t n, T* W, int d) {
   if (st.code) return;
@@ -904,6 +980,7 @@ static bool red_failed(const cl_reducer& r, const RState<T>& s) {
   switch (r.kind) {
     case CL_RED_MONITOR: return s.mon.code != 0;
     case CL_RED_RUNMONITOR: return false;
+    case CL_RED_STRETCHING: return s.sn.code != 0;
     case CL_RED_QR: return s.qr.code != 0;
     case CL_RED_BENETTIN: return s.ben.code != 0;
     case CL_RED_MEGNO: return s.meg.code != 0;
@@ -1281,21 +1358,22 @@ static void red_runmonitor(MonSt<T>& st, const SYS& sys, const T* x, int d) {
 // ------------------------------------------------------------------ the per-orbit body
 template <class P, class T, class SYS>
 static int8_t run_orbit_seg(const SYS& sys, const cl_chain& ch, int d, int k, int64_t n0, int64_t steps, int64_t N,
-                            int tau, T* x, T* W0, T* W1, RState<T>* st, int64_t* hstep);
+                            int tau, T* x, T* W0, T* W1, RState<T>* st, int64_t* hstep, uint32_t* ws = nullptr);
 
 // N steps from fresh reducer states
 template <class P, class T, class SYS>
 static int8_t run_orbit(const SYS& sys, const cl_chain& ch, int d, int k, int64_t N, int tau, T* x,
-                        T* W0, T* W1, RState<T>* st, int64_t* hstep) {
+                        T* W0, T* W1, RState<T>* st, int64_t* hstep, uint32_t* ws = nullptr) {
   for (int r = 0; r < ch.n; ++r) init_state(ch.r[r], st[r], k);
-  return run_orbit_seg<P>(sys, ch, d, k, 0, N, N, tau, x, W0, W1, st, hstep);
+  if (ws) memset(ws, 0, 4 * (size_t)chain_ws(ch));
+  return run_orbit_seg<P>(sys, ch, d, k, 0, N, N, tau, x, W0, W1, st, hstep, ws);
 }
 
 // One launch (orbit.jl:44-60): `steps` steps from global step n0 with the carried point, frames and reducer
 // states; the final status check runs only at step N.
 template <class P, class T, class SYS>
 static int8_t run_orbit_seg(const SYS& sys, const cl_chain& ch, int d, int k, int64_t n0, int64_t steps, int64_t N,
-                            int tau, T* x, T* W0, T* W1, RState<T>* st, int64_t* hstep) {
+                            int tau, T* x, T* W0, T* W1, RState<T>* st, int64_t* hstep, uint32_t* ws) {
   T* W[2] = {W0, W1};
   int nf = 1;
   for (int r = 0; r < ch.n; ++r)
@@ -1350,6 +1428,7 @@ static int8_t run_orbit_seg(const SYS& sys, const cl_chain& ch, int d, int k, in
         case 8: red_lognorm<P>(st[r].ln, w, d, k); break;
         case CL_RED_MONITOR: red_monitor<P>(rd, st[r].mon, n, sys, x, w, d, k); break;
         case CL_RED_RUNMONITOR: red_runmonitor<P>(st[r].mon, sys, x, d); break;
+        case CL_RED_STRETCHING: red_stretch<P>(rd, st[r].sn, n, w, d, ws); break;
       }
     }
     for (int r = 0; r < ch.n; ++r)
@@ -1365,8 +1444,9 @@ static int8_t run_orbit_seg(const SYS& sys, const cl_chain& ch, int d, int k, in
 template <class P>
 static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int d, int k, int64_t N,
                          int tau, double* x, double* W, double* W2, uint8_t* states, int8_t* code, int64_t* hstep,
-                         int nthreads, const double* A = nullptr) {
+                         int nthreads, const double* A = nullptr, uint32_t* ws = nullptr) {
   typedef typename P::T T;
+  const int64_t nws = chain_ws(*ch);
   MapSys<P, T> ms;
   ms.kind = sys->kind;
   ms.K = (T)sys->K;
@@ -1396,9 +1476,10 @@ static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, in
           w0[e] = (T)W[i * d * k + e];
           w1[e] = W2 ? (T)W2[i * d * k + e] : (T)0;
         }
+        uint32_t* wsi = (ws && nws > 0) ? ws + i * nws : nullptr;
         code[i] = sys->kind == CL_SYS_HENON_HEILES_GAUSS
-                      ? run_orbit<P, T>(hs, *ch, d, k, N, tau, xx, w0, w1, st.data(), &hstep[i])
-                      : run_orbit<P, T>(ms, *ch, d, k, N, tau, xx, w0, w1, st.data(), &hstep[i]);
+                      ? run_orbit<P, T>(hs, *ch, d, k, N, tau, xx, w0, w1, st.data(), &hstep[i], wsi)
+                      : run_orbit<P, T>(ms, *ch, d, k, N, tau, xx, w0, w1, st.data(), &hstep[i], wsi);
         for (int c = 0; c < s; ++c) x[i * s + c] = dbl(xx[c]);
         for (int e = 0; e < d * k; ++e) {
           W[i * d * k + e] = dbl(w0[e]);
@@ -1890,6 +1971,31 @@ int or_shadow_record(const cl_system* sys, int64_t m, int s, int64_t N, int tau,
   else
     shadow_record_all<F64Libm>(sys, m, s, tau, d0, grid, G, x0, u0, R, fail, (uint64_t*)ends, nthreads);
   return 0;
+}
+
+// or_propagate for a chain with a workspace reducer (StretchingNumbers): ws[m][chain_ws] UInt32 counts per orbit
+// (the reference's ws[:, i]), zeroed at the start of each orbit. With A (column-major 2 x 2) the linear map
+// x' = A x runs instead of sys (then s = d = 2).
+int or_propagate_ws(const cl_system* sys, const double* A, const cl_chain* ch, int64_t m, int s, int d, int k,
+                    int64_t N, int tau, double* x, double* W, void* states, int8_t* code, int64_t* hstep,
+                    uint32_t* ws, int math, int nthreads) {
+  cl_system lin;
+  if (A) {
+    memset(&lin, 0, sizeof(lin));
+    lin.kind = OR_SYS_LINEAR;
+    sys = &lin;
+    if (s != 2 || d != 2) return 10;
+  } else if (!or_dims_ok(sys, d, k)) {
+    return 10;
+  }
+  if (N % tau || k < 1 || k > 2 || !ws) return 10;
+  for (int r = 0; r < ch->n; ++r)
+    if (ch->r[r].kind == CL_RED_STRETCHING && (k != 2 || ch->r[r].bins < 1 || !(ch->r[r].lo < ch->r[r].hi))) return 10;
+  if (math == MATH_CL)
+    return propagate_all<F64Cl>(sys, ch, m, s, d, k, N, tau, x, W, nullptr, (uint8_t*)states, code, hstep, nthreads,
+                                A, ws);
+  return propagate_all<F64Libm>(sys, ch, m, s, d, k, N, tau, x, W, nullptr, (uint8_t*)states, code, hstep, nthreads,
+                                A, ws);
 }
 
 // A linear map x' = A x (A column-major 2 x 2): the closed-form fixtures of the reference's suites

@@ -48,6 +48,9 @@ def load():
         o.or_shadow_record.restype = I
         o.or_shadow_record.argtypes = [ctypes.POINTER(L.cl_system), I64, I, I64, I, ctypes.c_double, P, I, P, P, P, P,
                                        P, I, I]
+        o.or_propagate_ws.restype = I
+        o.or_propagate_ws.argtypes = [ctypes.POINTER(L.cl_system), P, ctypes.POINTER(L.cl_chain), I64, I, I, I, I64, I,
+                                      P, P, P, P, P, P, I, I]
         o.or_csum.restype = None
         o.or_csum.argtypes = [I64, ctypes.c_double, P, P]
         o.or_csum_list.restype = None
@@ -213,3 +216,24 @@ def shadow_record(sysd, x0, u0, N, tau, d0, grid, math=MATH_CL, nthreads=None):
                             _ptr(fail), _ptr(e), math, nt)
     assert rc == 0, rc
     return dict(R=R, fail=fail, ends=shadow_ends(e, s))
+
+
+def run_ws(sysd, chain, nws, x0, W0, N, tau, k, A=None, math=MATH_CL, nthreads=None):
+    """Full N-step runs of a chain with a workspace reducer (or_propagate_ws): x0 (m, s), W0 (m, d, k); with A (2 x 2)
+    the linear map x' = A x. Returns x, W, raw states, code, hstep and the workspace rows ws (m, nws) UInt32."""
+    o = load()
+    x = np.ascontiguousarray(np.array(x0, dtype=np.float64))
+    m, s = x.shape
+    d = 2
+    W = np.array(np.asarray(W0, dtype=np.float64).reshape(m, d, k).transpose(0, 2, 1), order="C", copy=True)
+    nb = o.or_state_bytes(ctypes.byref(chain), d, k)
+    states = np.zeros((m, max(nb, 1)), dtype=np.uint8)
+    code = np.zeros(m, dtype=np.int8)
+    hstep = np.zeros(m, dtype=np.int64)
+    ws = np.full((m, nws), 0xDEADBEEF, dtype=np.uint32)
+    Ac = None if A is None else np.ascontiguousarray(np.asarray(A, dtype=np.float64).T.ravel())
+    nt = nthreads or min(16, os.cpu_count() or 1)
+    rc = o.or_propagate_ws(None if A is not None else ctypes.byref(sysd), _ptr(Ac), ctypes.byref(chain), m, s, d, k,
+                           int(N), tau, _ptr(x), _ptr(W), _ptr(states), _ptr(code), _ptr(hstep), _ptr(ws), math, nt)
+    assert rc == 0, rc
+    return dict(x=x, W=W.transpose(0, 2, 1).copy(), states=states[:, :nb], code=code, hstep=hstep, ws=ws)
