@@ -10,9 +10,13 @@ from .reducers import (QRSpectrum, CGS, MGS, RGS, HouseholderQR, AugmentedQR, Gi
 from .tangent import propagate, propagate_ragged, PropagationResult, NumericalHealthError, initial_deviations
 from .two_trajectory import propagate_shadow
 from .reducers import StretchingNumbers, stretching_range, spectral_distance, spectrum_density, spectrum_cell
+from .birkhoff import (WeightedBirkhoff, CosObservable, SinObservable, LiftedObservable, weighted_birkhoff,
+                       birkhoff_index)
 
 __all__ = ["StandardMap", "StandardMapK", "StandardMapChain", "WrappedStandardMap", "HenonHeilesGauss", "hh_energy", "torus_state", "QRSpectrum", "CGS", "MGS", "RGS", "HouseholderQR",
            "AugmentedQR", "GivensQR", "Benettin", "MEGNO", "SALI", "FLI", "OFLI", "GALI", "ResidualMonitor", "RunMonitor", "monitor_exponents", "det_threshold",
            "symp_threshold", "sum_threshold", "eps_sum", "eps_pair", "check_residual", "propagate", "propagate_ragged",
            "PropagationResult", "NumericalHealthError", "initial_deviations", "propagate_shadow",
-           "StretchingNumbers", "stretching_range", "spectral_distance", "spectrum_density", "spectrum_cell"]
+           "StretchingNumbers", "stretching_range", "spectral_distance", "spectrum_density", "spectrum_cell",
+           "WeightedBirkhoff", "CosObservable", "SinObservable", "LiftedObservable", "weighted_birkhoff",
+           "birkhoff_index"]

@@ -34,10 +34,22 @@ class cl_system(ctypes.Structure):
                 ("K", ctypes.c_double), ("eps", ctypes.c_double), ("h", ctypes.c_double), ("tol", ctypes.c_double)]
 
 
+CL_RED_BIRKHOFF = 12
+CL_OBS_COS, CL_OBS_SIN, CL_OBS_LIFTED = 1, 2, 3
+CL_WB_MAX_WINDOWS, CL_WB_MAX_OBSERVABLES = 16, 8
+
+
+class cl_birkhoff(ctypes.Structure):
+    _fields_ = [("H", ctypes.c_int32), ("K", ctypes.c_int32), ("order", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                ("window", ctypes.c_int64 * CL_WB_MAX_WINDOWS), ("kind", ctypes.c_int32 * CL_WB_MAX_OBSERVABLES),
+                ("a", ctypes.c_int32 * CL_WB_MAX_OBSERVABLES), ("b", ctypes.c_int32 * CL_WB_MAX_OBSERVABLES)]
+
+
 class cl_reducer(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("alg", ctypes.c_int32), ("replace", ctypes.c_int32),
                 ("frame", ctypes.c_int32), ("tol", ctypes.c_double), ("tol2", ctypes.c_double),
-                ("lo", ctypes.c_double), ("hi", ctypes.c_double), ("bins", ctypes.c_int32), ("pad2_", ctypes.c_int32)]
+                ("lo", ctypes.c_double), ("hi", ctypes.c_double), ("bins", ctypes.c_int32), ("pad2_", ctypes.c_int32),
+                ("birkhoff", ctypes.POINTER(cl_birkhoff))]
 
 
 class cl_chain(ctypes.Structure):
@@ -69,6 +81,7 @@ SIGNATURES = {
     "cl_propagate_device_ws": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I64, I, I, I, I64, I64,
                                    I64, I, P, P, P, P, P, P, P, P]),
     "cl_workspace_entries": (I64, [ctypes.POINTER(cl_chain), I, I]),
+    "cl_birkhoff_weights": (I, [I64, I, P]),
     "cl_propagate_device_traced": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I64, I, I, I, I64,
                                        I64, I64, I, P, P, P, P, P, P, P, I64, I, P]),
     "cl_propagate_ragged": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I64, I, I, I, P, P, I,

@@ -19,6 +19,7 @@
 #pragma once
 #include <stdint.h>
 #include "cl_math.h"
+#include "../../include/chaoslab_hip.h"
 
 #if !defined(__HIPCC__)
 #define __device__
@@ -1221,6 +1222,109 @@ struct Stretch {
         W.at(i, 1) = W.at(i, 1) / c.a2;
       }
     }
+  }
+};
+
+// ------------------------------------------------------------------------------ WeightedBirkhoff
+// WeightedBirkhoff (src/Indicators/wbirkhoff_reducer.jl:4-116, wbirkhoff.jl): K observables of the orbit point over H
+// horizons in one pass. Handoff j (0-based) is sample j; horizon N takes samples 0..2N-1, window 1 the first N with
+// the weight w(j), window 2 the next N with w(j - N) (_wb_advance, :42-48): B += w (window 1 only, a zero weight adds
+// nothing), A_i += w f (Neumaier), S = max(S, |f|). The horizons' sums (H (2 + 5K) words: B, then per observable A1,
+// A2, S) are too many for registers: they stay in HBM, word-major like the other states, and each handoff reads and
+// writes only the horizons that have not yet taken their 2N samples. The weights w(jj) = exp(-1 / (t^p (1-t)^p)),
+// t = min(jj, N - jj) / N (birkhoff_weight, naff_reducer.jl:13-17), are the same for every orbit: a table per
+// horizon, formed once per launch. The register state is the tail of WBState: n, handoffs, code.
+struct WBHead {
+  int64_t n;
+  int32_t handoffs;
+  int16_t code;
+  int16_t pad_;
+};
+struct WBParams {
+  int H, K, order;
+  int64_t win[CL_WB_MAX_WINDOWS];
+  int64_t woff[CL_WB_MAX_WINDOWS];  // offset of horizon h's weights in the table
+  int kind[CL_WB_MAX_OBSERVABLES], a[CL_WB_MAX_OBSERVABLES], b[CL_WB_MAX_OBSERVABLES];
+  int64_t nmax2;  // 2 max(window): handoffs of a complete run
+  int hw;         // words of the horizons' sums
+};
+CL_HD double wb_pow(double u, int p) {
+  const double u2 = u * u;
+  return p == 1 ? u : (p == 2 ? u2 : (p == 3 ? u2 * u : u2 * u2));
+}
+// birkhoff_weight(T(min(jj, N - jj)) / T(N), p)
+CL_HD double wb_weight(int64_t jj, int64_t N, int p) {
+  const int64_t mn = jj < N - jj ? jj : N - jj;
+  const double t = (double)mn / (double)N;
+  if (t <= 0.0 || t >= 1.0) return 0.0;
+  return cl_exp(-1.0 / (wb_pow(t, p) * wb_pow(1.0 - t, p)));
+}
+
+struct WBirk {
+  WBParams p;
+  typedef WBHead State;
+  CL_HD void init(State& st) const {
+    st.n = 0;
+    st.handoffs = 0;
+    st.code = 0;
+    st.pad_ = 0;
+  }
+  template <class Sys>
+  CL_HD double observable(int k, const double* x) const {
+    const int a = p.a[k], b = p.b[k];
+    if (p.kind[k] == CL_OBS_COS) return cl_cos((double)b * x[a]);
+    if (p.kind[k] == CL_OBS_SIN) return cl_sin((double)b * x[a]);
+    return x[a] + 2.0 * (3.141592653589793 + 1.2246467991473532e-16) * x[b];
+  }
+  // mem: the orbit's first horizon word (words `stride` apart); wtab: the weight tables
+  template <class Sys>
+  CL_HD void step(State& st, int64_t n, const double* x, double* mem, int64_t stride, const double* wtab) const {
+    if (st.code != 0) return;
+    if ((int64_t)st.handoffs >= p.nmax2) {  // every horizon is complete: only the counters advance
+      st.n = n;
+      if (st.handoffs < 2147483647) st.handoffs += 1;
+      return;
+    }
+    double f[CL_WB_MAX_OBSERVABLES];
+#pragma unroll
+    for (int k = 0; k < CL_WB_MAX_OBSERVABLES; ++k) f[k] = k < p.K ? observable<Sys>(k, x) : 0.0;
+#pragma unroll
+    for (int k = 0; k < CL_WB_MAX_OBSERVABLES; ++k)
+      if (k < p.K && !cl_isfinite(f[k])) {
+        st.code = (int16_t)(k + 1);
+        st.n = n;
+        return;
+      }
+    const int64_t j = st.handoffs;
+    const int cw = 2 + 5 * p.K;
+    for (int h = 0; h < p.H; ++h) {
+      const int64_t N = p.win[h];
+      if (j >= 2 * N) continue;
+      const int i = j >= N ? 1 : 0;
+      const double w = wtab[p.woff[h] + (j - i * N)];
+      double* hb = mem + (int64_t)h * cw * stride;
+      if (i == 0 && w != 0.0) {
+        CSum B{hb[0], hb[stride]};
+        B = csum_add(B, w);
+        hb[0] = B.s;
+        hb[stride] = B.c;
+      }
+#pragma unroll
+      for (int k = 0; k < CL_WB_MAX_OBSERVABLES; ++k) {
+        if (k >= p.K) break;
+        double* c = hb + (int64_t)(2 + 5 * k) * stride;
+        c[4 * stride] = cl_jmax(c[4 * stride], cl_fabs(f[k]));
+        if (w != 0.0) {
+          double* A = c + 2 * i * stride;
+          CSum a{A[0], A[stride]};
+          a = csum_add(a, w * f[k]);
+          A[0] = a.s;
+          A[stride] = a.c;
+        }
+      }
+    }
+    st.n = n;
+    if (j < 2147483647) st.handoffs = (int32_t)(j + 1);
   }
 };
 

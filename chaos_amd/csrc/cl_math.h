@@ -24,6 +24,7 @@
  *  after rounding to double 2^-53.8, 2^-54.0, 2^-50.9) and evaluated by Horner's rule with separate
  *  multiplies and adds (a constant addend then stays a scalar operand instead of taking a vector register
  *  across the kernels' step loops, as the tied accumulator of a fused multiply-add would).
+ *  - exp: Cody and Waite reduction by ln 2 and the degree-13 Taylor polynomial of exp(r) - 1 - r (see cl_exp).
  *  - TwoSum (Knuth), TwoProd with fma (Dekker's product made exact by the fused multiply-add).
  * The constants come from chaos_amd/csrc/gen_math_constants.py.
  */
@@ -316,6 +317,36 @@ CL_HD double cl_log(double x) {
   double dk = (double)k;
   double lo = dk * CL_LN2_LO + s * (hfsq + R);
   return dk * CL_LN2_HI + (f - (hfsq - lo));
+}
+
+/* exp(x): k = rint(x / ln 2), r = (x - k LN2_HI) - k LN2_LO (k LN2_HI is exact: |k| < 2^11 and LN2_HI has 42
+ * significant bits), |r| <= ln 2 / 2 plus rounding; exp(r) = 1 + (r + r^2 E(r)) with E the Taylor polynomial
+ * sum_{n=2..13} r^(n-2) / n! (the truncation r^12 / 14! is below 2^-60 relative on the interval), the coefficients
+ * 1/n! rounded to double. The power 2^k is applied by one multiply; below 2^-1022 by two (an exact one into the normal
+ * range, then one rounding into the subnormals), above 2^1020 by two so the exponent field never overflows. */
+CL_HD double cl_exp(double x) {
+  if (cl_isnan(x)) return x;
+  if (x > 710.0) return __builtin_inf();
+  if (x < -746.0) return 0.0;
+  double k = cl_rint(x * 0x1.71547652b82fep+0);
+  double r = (x - k * CL_LN2_HI) - k * CL_LN2_LO;
+  double e = 0x1.6124613a86d09p-33;
+  e = e * r + 0x1.1eed8eff8d898p-29;
+  e = e * r + 0x1.ae64567f544e4p-26;
+  e = e * r + 0x1.27e4fb7789f5cp-22;
+  e = e * r + 0x1.71de3a556c734p-19;
+  e = e * r + 0x1.a01a01a01a01ap-16;
+  e = e * r + 0x1.a01a01a01a01ap-13;
+  e = e * r + 0x1.6c16c16c16c17p-10;
+  e = e * r + 0x1.1111111111111p-7;
+  e = e * r + 0x1.5555555555555p-5;
+  e = e * r + 0x1.5555555555555p-3;
+  e = e * r + 0.5;
+  double p = 1.0 + (r + (r * r) * e);
+  int ki = (int)k;
+  if (ki > 1020) return (p * cl_pow2i(ki - 4)) * 16.0;
+  if (ki < -1022) return (p * cl_pow2i(ki + 54)) * 0x1p-54;
+  return p * cl_pow2i(ki);
 }
 
 #endif /* CL_MATH_H */

@@ -53,6 +53,9 @@ typedef Cons<S_<Benettin<2>>, Cons<S_<RunMon<2>>, Nil>> C1_rec_ben;
 // StretchingNumbers (a workspace reducer, k = 2) alone and as the indicator recorder's chain with RunMonitor
 typedef Cons<S_<Stretch<2>>, Nil> C_st;
 typedef Cons<S_<Stretch<2>>, Cons<S_<RunMon<2>>, Nil>> C_st_mon;
+// WeightedBirkhoff (k = 1) alone and after RunMonitor (the indicator recorder's window-kind order)
+typedef Cons<S_<WBirk>, Nil> C1_wb;
+typedef Cons<S_<RunMon<2>>, Cons<S_<WBirk>, Nil>> C1_mon_wb;
 
 #define ADD2(SK, SYS, CH, NF) if (*n < cap) out[(*n)++] = make_instance<SK, SYS, 2, NF, CH>();
 #define ADD1(SK, SYS, CH) if (*n < cap) out[(*n)++] = make_instance<SK, SYS, 1, 1, CH>();
@@ -67,7 +70,7 @@ typedef Cons<S_<Stretch<2>>, Cons<S_<RunMon<2>>, Nil>> C_st_mon;
   ADD1(SK, SYS, C1_rgs) ADD1(SK, SYS, C1_hh) ADD1(SK, SYS, C1_aug) ADD1(SK, SYS, C1_giv)        \
   ADD1(SK, SYS, C1_ofli) ADD1(SK, SYS, C1_mon) ADD2(SK, SYS, C_rec_cgs, 1) ADD2(SK, SYS, C_rec_mgs, 1)          \
   ADD2(SK, SYS, C_rec_rgs, 1) ADD2(SK, SYS, C_rec_hh, 1) ADD2(SK, SYS, C_rec_aug, 1) ADD1(SK, SYS, C1_rec_ben) \
-  ADD2(SK, SYS, C_st, 1) ADD2(SK, SYS, C_st_mon, 1)
+  ADD2(SK, SYS, C_st, 1) ADD2(SK, SYS, C_st_mon, 1) ADD1(SK, SYS, C1_wb) ADD1(SK, SYS, C1_mon_wb)
 
 // the surrogate workload on the per-orbit-K map: the traced prefix (MEGNO + FLI on frame 1, the QR spectrum
 // on frame 2), the continuation and the plain method (the QR spectrum alone)

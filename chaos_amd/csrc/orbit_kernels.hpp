@@ -32,6 +32,8 @@ struct KArgs {
   const uint8_t* mask;
   // reducer workspace [entries][m] (UInt32), or null when the chain requests none
   uint32_t* ws;
+  // WeightedBirkhoff weight tables (one per horizon), or null
+  const double* wtab;
 };
 
 // Per-orbit context the reducers read: the point, the carried sin/cos, the frames and a cache of
@@ -46,6 +48,8 @@ struct OrbitCtx {
   ColLogs logs[NF];
   uint32_t* ws;  // the orbit's first workspace entry (entries m apart)
   int64_t wstride;
+  double* mem;   // the orbit's first WeightedBirkhoff horizon word in the state array (words m apart)
+  const double* wtab;
 };
 
 // ------------------------------------------------------------------------------ chain slots
@@ -137,6 +141,11 @@ CL_HD void slot_step(Slot<Stretch<D>, FR>& s, int64_t n, Ctx& c) {
   if (s.r.replace) c.logs_valid[FR] = false;
 }
 
+template <int FR, class Ctx>
+CL_HD void slot_step(Slot<WBirk, FR>& s, int64_t n, Ctx& c) {
+  s.r.template step<decltype(c.sys)>(s.st, n, c.x, c.mem, c.wstride, c.wtab);
+}
+
 // The workspace of a fresh run is zeroed by the first launch (src/Tangent/propagate.jl:29-31).
 template <class R, int FR, class Ctx>
 CL_HD void slot_ws_init(const Slot<R, FR>&, Ctx&) {}
@@ -144,6 +153,11 @@ template <int D, int FR, class Ctx>
 CL_HD void slot_ws_init(const Slot<Stretch<D>, FR>& s, Ctx& c) {
   const int64_t e = 2 * ((int64_t)s.r.bins + 2);
   for (int64_t j = 0; j < e; ++j) c.ws[j * c.wstride] = 0u;
+}
+// WeightedBirkhoff's horizon sums start at zero (CompensatedSum() and S = 0 are all-zero words).
+template <int FR, class Ctx>
+CL_HD void slot_ws_init(const Slot<WBirk, FR>& s, Ctx& c) {
+  for (int j = 0; j < s.r.p.hw; ++j) c.mem[(int64_t)j * c.wstride] = 0.0;
 }
 template <class Ctx>
 CL_HD void chain_ws_init(const Nil&, Ctx&) {}
@@ -177,6 +191,26 @@ template <int D, int K>
 CL_HD void slot_setup(GALI<D, K>&, const cl_reducer&) {}
 template <int D>
 CL_HD void slot_setup(RunMon<D>&, const cl_reducer&) {}
+inline void slot_setup(WBirk& r, const cl_reducer& d) {
+  const cl_birkhoff& b = *d.birkhoff;
+  r.p.H = b.H;
+  r.p.K = b.K;
+  r.p.order = b.order;
+  int64_t off = 0, mx = 0;
+  for (int h = 0; h < CL_WB_MAX_WINDOWS; ++h) {
+    r.p.win[h] = h < b.H ? b.window[h] : 0;
+    r.p.woff[h] = off;
+    off += r.p.win[h];
+    mx = r.p.win[h] > mx ? r.p.win[h] : mx;
+  }
+  for (int k = 0; k < CL_WB_MAX_OBSERVABLES; ++k) {
+    r.p.kind[k] = k < b.K ? b.kind[k] : 0;
+    r.p.a[k] = k < b.K ? b.a[k] : 0;
+    r.p.b[k] = k < b.K ? b.b[k] : 0;
+  }
+  r.p.nmax2 = 2 * mx;
+  r.p.hw = b.H * (2 + 5 * b.K);
+}
 template <int D>
 CL_HD void slot_setup(Stretch<D>& r, const cl_reducer& d) {
   r.replace = d.replace != 0;
@@ -253,7 +287,31 @@ CL_HD int chain_load(Cons<H, T>& c, const uint64_t* st, int64_t m, int64_t i, in
   state_load(c.head.st, st, m, i, w);
   return chain_load(c.tail, st, m, i, w + (int)(sizeof(c.head.st) / 8));
 }
+// WeightedBirkhoff, the last reducer of its chain: the register state (n, handoffs, code) follows the horizon sums
+// in the Julia layout of WBState.
+template <int FR>
+CL_HD int chain_load(Cons<Slot<WBirk, FR>, Nil>& c, const uint64_t* st, int64_t m, int64_t i, int w) {
+  state_load(c.head.st, st, m, i, w + c.head.r.p.hw);
+  return w + c.head.r.p.hw + 2;
+}
+template <class C>
+struct WBOffset {  // state words before a trailing WeightedBirkhoff
+  static constexpr int value = 0;
+};
+template <class H, class T>
+struct WBOffset<Cons<H, T>> {
+  static constexpr int value = (int)(sizeof(decltype(H::st)) / 8) + WBOffset<T>::value;
+};
+template <int FR>
+struct WBOffset<Cons<Slot<WBirk, FR>, Nil>> {
+  static constexpr int value = 0;
+};
 CL_HD int chain_store(const Nil&, uint64_t*, int64_t, int64_t, int w) { return w; }
+template <int FR>
+CL_HD int chain_store(const Cons<Slot<WBirk, FR>, Nil>& c, uint64_t* st, int64_t m, int64_t i, int w) {
+  state_store(c.head.st, st, m, i, w + c.head.r.p.hw);
+  return w + c.head.r.p.hw + 2;
+}
 template <class H, class T>
 CL_HD int chain_store(const Cons<H, T>& c, uint64_t* st, int64_t m, int64_t i, int w) {
   state_store(c.head.st, st, m, i, w);
@@ -311,6 +369,8 @@ __global__ void __launch_bounds__(256) orbit_kernel(KArgs a, Sys sys, Chain chai
   Chain ch = chain0;
   c.ws = a.ws ? a.ws + i : nullptr;
   c.wstride = a.m;
+  c.mem = (double*)(a.st + (int64_t)WBOffset<Chain>::value * a.m + i);
+  c.wtab = a.wtab;
   if (a.n0 == 0) {
     chain_init(ch);
     chain_ws_init(ch, c);

@@ -100,6 +100,55 @@ static int set_err(cl_ctx* ctx, int code, const char* msg) {
 }
 
 // Validation shared by both launch entry points (src/Tangent/propagate.jl:279-289, orbit.jl:187-191).
+// WeightedBirkhoff (the last reducer of its chain): the horizon sums held in the state array, H (2 + 5K) words.
+static int wb_index(const cl_chain* ch) {
+  for (int r = 0; r < ch->n && r < CL_MAX_REDUCERS; ++r)
+    if (ch->r[r].kind == CL_RED_BIRKHOFF) return r;
+  return -1;
+}
+static int64_t wb_mem_words(const cl_chain* ch) {
+  const int r = wb_index(ch);
+  if (r < 0 || !ch->r[r].birkhoff) return 0;
+  const cl_birkhoff& b = *ch->r[r].birkhoff;
+  return (int64_t)b.H * (2 + 5 * b.K);
+}
+// The descriptor's own checks (WeightedBirkhoff's constructor, wbirkhoff_reducer.jl:28-34): 1 to 16 horizons of at
+// least 16 samples, 1 to 8 observables (the reference allows 31: more than 8 has no kernel), order 1 to 4, the
+// observables' coordinates inside the state; the reducer must be the last of its chain.
+static const char* wb_check(const cl_chain* ch, int s) {
+  const int r = wb_index(ch);
+  if (r < 0) return nullptr;
+  if (r != ch->n - 1) return "WeightedBirkhoff must be the last reducer of its chain";
+  const cl_birkhoff* b = ch->r[r].birkhoff;
+  if (!b) return "WeightedBirkhoff needs its parameters";
+  if (b->H < 1 || b->H > CL_WB_MAX_WINDOWS) return "WeightedBirkhoff: 1 to 16 windows";
+  if (b->K < 1 || b->K > CL_WB_MAX_OBSERVABLES) return "WeightedBirkhoff: 1 to 8 observables";
+  if (b->order < 1 || b->order > 4) return "WeightedBirkhoff: birkhoff_order must be 1 to 4";
+  for (int h = 0; h < b->H; ++h)
+    if (b->window[h] < 16 || b->window[h] > ((int64_t)1 << 40)) return "WeightedBirkhoff: each window at least 16";
+  for (int k = 0; k < b->K; ++k) {
+    const bool trig = b->kind[k] == CL_OBS_COS || b->kind[k] == CL_OBS_SIN;
+    if (!(trig || b->kind[k] == CL_OBS_LIFTED)) return "WeightedBirkhoff: unknown observable kind";
+    if (b->a[k] < 0 || b->a[k] >= s || (!trig && (b->b[k] < 0 || b->b[k] >= s)))
+      return "WeightedBirkhoff: observable coordinate outside the state";
+  }
+  return nullptr;
+}
+
+// The weight tables: horizon h's w(jj), jj < N_h, at offset sum_{h' < h} N_h'.
+__global__ void k_wb_weights(int H, cl_birkhoff b, double* wtab) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t off = 0;
+  for (int h = 0; h < H; ++h) {
+    const int64_t N = b.window[h];
+    if (t >= off && t < off + N) {
+      wtab[t] = wb_weight(t - off, N, b.order);
+      return;
+    }
+    off += N;
+  }
+}
+
 static int validate(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m, int s, int d,
                     int k, int64_t n0, int64_t steps, int64_t N, int tau, const Instance** out) {
   if (!sys || !ch) return set_err(ctx, CL_ERR_ARGUMENT, "null system or chain descriptor");
@@ -128,6 +177,7 @@ static int validate(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int e
     if ((n0 + steps) / tau > 1000000000) return set_err(ctx, CL_ERR_ARGUMENT, "more than 10^9 handoffs");
   }
   if (nws > 1) return set_err(ctx, CL_ERR_ARGUMENT, "at most one reducer of a chain may request a workspace");
+  if (const char* why = wb_check(ch, s)) return set_err(ctx, CL_ERR_ARGUMENT, why);
   const Instance* in = find_instance(sys, ch, d, k);
   if (!in) return set_err(ctx, CL_ERR_UNSUPPORTED, "no kernel instance for this system/chain/k");
   *out = in;
@@ -310,6 +360,12 @@ extern "C" const char* cl_last_error(cl_ctx* c) { return c ? c->err : g_create_e
 // The context's stream, for the other translation units of the library (not part of the C ABI).
 extern "C" __attribute__((visibility("hidden"))) hipStream_t cl_ctx_stream(cl_ctx* c) { return c->stream; }
 
+extern "C" int cl_birkhoff_weights(int64_t N, int order, double* w) {
+  if (N < 1 || order < 1 || order > 4 || !w) return CL_ERR_ARGUMENT;
+  for (int64_t jj = 0; jj < N; ++jj) w[jj] = wb_weight(jj, N, order);
+  return CL_OK;
+}
+
 extern "C" int64_t cl_state_bytes(const cl_system* sys, const cl_chain* ch, int elt, int d, int k) {
   if (!sys || !ch || ch->n < 0 || ch->n > CL_MAX_REDUCERS) return -CL_ERR_ARGUMENT;
   if (elt != CL_ELT_F64) return -CL_ERR_UNSUPPORTED;
@@ -321,7 +377,8 @@ extern "C" int64_t cl_state_bytes(const cl_system* sys, const cl_chain* ch, int 
   }
   const Instance* in = find_instance(sys, ch, d, k);
   if (!in) return -CL_ERR_UNSUPPORTED;
-  return (int64_t)in->words * 8;
+  if (wb_check(ch, state_dim(sys->kind))) return -CL_ERR_ARGUMENT;
+  return ((int64_t)in->words + wb_mem_words(ch)) * 8;
 }
 
 static int grow(cl_ctx* c, int slot, size_t bytes) {
@@ -346,6 +403,17 @@ static int launch(const Instance* in, const cl_system* sys, const cl_chain* ch, 
   memset(&a, 0, sizeof(a));
   a.ws = ws;
   a.mask = mask;
+  const int wbr = wb_index(ch);
+  double* wtab = nullptr;
+  if (wbr >= 0) {  // the weight tables, formed on the device for this launch
+    const cl_birkhoff& b = *ch->r[wbr].birkhoff;
+    int64_t tot = 0;
+    for (int h = 0; h < b.H; ++h) tot += b.window[h];
+    hipError_t e = hipMallocAsync((void**)&wtab, (size_t)tot * 8, st);
+    if (e != hipSuccess) return map_err(e);
+    hipLaunchKernelGGL(k_wb_weights, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, b.H, b, wtab);
+    a.wtab = wtab;
+  }
   a.trace = trace;
   a.trace_n0 = trace_n0;
   a.trace_len = trace_len;
@@ -360,7 +428,9 @@ static int launch(const Instance* in, const cl_system* sys, const cl_chain* ch, 
   a.steps = steps;
   a.N = N;
   a.tau = tau;
-  return in->launch(sys, ch, a, st);
+  int rc = in->launch(sys, ch, a, st);
+  if (wtab) hipFreeAsync(wtab, st);
+  return rc;
 }
 
 // ------------------------------------------------------------------------------ split execution
@@ -828,11 +898,12 @@ extern "C" int cl_propagate_chunk(cl_ctx* ctx, const cl_system* sys, const cl_ch
   if (rc) return rc;
   if (m == 0) return CL_OK;
   const int64_t nws = ws_entries(ch);
-  if (!x || !W || !code || !hstep || (in->words > 0 && !states) || (in->nf > 1 && !W2) || (nws > 0 && !ws))
+  if (!x || !W || !code || !hstep || (in->words + wb_mem_words(ch) > 0 && !states) || (in->nf > 1 && !W2) ||
+      (nws > 0 && !ws))
     return set_err(ctx, CL_ERR_ARGUMENT, "null buffer");
   hipSetDevice(ctx->device);
   hipStream_t st = ctx->stream;
-  const int dk = d * k, nw = in->words;
+  const int dk = d * k, nw = in->words + (int)wb_mem_words(ch);
   uint32_t *dws = nullptr, *wsa = nullptr;
   const size_t bws = (size_t)m * (size_t)(nws > 0 ? nws : 0) * 4;
   if (nws > 0) {

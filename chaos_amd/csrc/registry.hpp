@@ -103,6 +103,10 @@ template <int D>
 struct Kind<Stretch<D>> {
   static constexpr int kind = CL_RED_STRETCHING, alg = 0;
 };
+template <>
+struct Kind<WBirk> {
+  static constexpr int kind = CL_RED_BIRKHOFF, alg = 0;
+};
 
 inline void fill_key(Instance&, const Nil&, int) {}
 template <class H, class T>

@@ -80,6 +80,26 @@ typedef struct {
 #define CL_RED_RUNMONITOR 10 /* RunMonitor(sys), the recorder's non-raising monitor, src/Experiments/run_record.jl:25-43 */
 #define CL_RED_STRETCHING 11 /* StretchingNumbers(; range, bins, interval, min_per_cell, replace), k = 2, with a
                                 UInt32 workspace of 2 (bins + 2) counts per orbit, src/Indicators/stretching.jl:34-147 */
+#define CL_RED_BIRKHOFF 12   /* WeightedBirkhoff(; window, observables, birkhoff_order), any k, the last reducer of its
+                                chain, src/Indicators/wbirkhoff_reducer.jl:4-116; parameters in cl_reducer.birkhoff */
+
+/* Observables of WeightedBirkhoff (src/Indicators/wbirkhoff.jl:17-77), coordinates 0-based:
+ * CL_OBS_COS cos(b x[a]), CL_OBS_SIN sin(b x[a]) (b the integer frequency), CL_OBS_LIFTED x[a] + 2 pi x[b]. */
+#define CL_OBS_COS 1
+#define CL_OBS_SIN 2
+#define CL_OBS_LIFTED 3
+#define CL_WB_MAX_WINDOWS 16
+#define CL_WB_MAX_OBSERVABLES 8
+typedef struct {
+  int32_t H;        /* horizons, 1..CL_WB_MAX_WINDOWS */
+  int32_t K;        /* observables, 1..CL_WB_MAX_OBSERVABLES */
+  int32_t order;    /* birkhoff_order p, 1..4 */
+  int32_t pad_;
+  int64_t window[CL_WB_MAX_WINDOWS];   /* N_h >= 16; a run has 2 max(N_h) handoffs */
+  int32_t kind[CL_WB_MAX_OBSERVABLES]; /* CL_OBS_* */
+  int32_t a[CL_WB_MAX_OBSERVABLES];
+  int32_t b[CL_WB_MAX_OBSERVABLES];
+} cl_birkhoff;
 
 #define CL_QR_CGS 1
 #define CL_QR_MGS 2
@@ -99,6 +119,7 @@ typedef struct {
   double lo, hi;    /* StretchingNumbers range (s_lo, s_hi) */
   int32_t bins;     /* StretchingNumbers cells B (the workspace holds 2 (B + 2) counts per orbit) */
   int32_t pad2_;
+  const cl_birkhoff *birkhoff; /* WeightedBirkhoff parameters (host memory, read at each call) */
 } cl_reducer;
 
 #define CL_MAX_REDUCERS 8
@@ -159,6 +180,11 @@ int cl_propagate_device_ws(cl_ctx *ctx, const cl_system *sys, const cl_chain *ch
                            int64_t m, int s, int d, int k, int64_t n0, int64_t steps, int64_t N, int tau,
                            double *dx, double *dW, double *dW2, uint64_t *dstates, int8_t *dcode,
                            int64_t *dhstep, uint32_t *dws, void *stream);
+/* The unnormalised weights w(jj) = exp(-1 / (t^p (1 - t)^p)), t = min(jj, N - jj) / N, jj = 0..N-1, of a horizon N
+ * (_wb and birkhoff_weight, src/Indicators/naff_reducer.jl:13-17), computed on the host with the kernels' exp, so the
+ * pure function on a stored series (weighted_birkhoff, src/Indicators/wbirkhoff.jl:139-162) uses the device's weights. */
+int cl_birkhoff_weights(int64_t N, int order, double *w);
+
 /* UInt32 workspace entries per orbit of the chain (_chain_workspace, src/Tangent/orbit.jl:8-18): 2 (bins + 2) for
  * StretchingNumbers, 0 when no reducer requests one; -10 when two reducers request one. */
 int64_t cl_workspace_entries(const cl_chain *chain, int d, int k);

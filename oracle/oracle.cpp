@@ -47,6 +47,7 @@ struct F64Libm {
   static double sin(double x) { return cl_isfinite(x) ? ::sin(x) : x - x; }
   static double cos(double x) { return cl_isfinite(x) ? ::cos(x) : x - x; }
   static double log(double x) { return ::log(x); }
+  static double exp(double x) { return ::exp(x); }
   static double sqrt(double x) { return ::sqrt(x); }
   static double fma(double a, double b, double c) { return ::fma(a, b, c); }
   static double abs(double x) { return fabs(x); }
@@ -62,12 +63,14 @@ struct F64Cl : F64Libm {
   static double sin(double x) { return cl_sin(x); }
   static double cos(double x) { return cl_cos(x); }
   static double log(double x) { return cl_log(x); }
+  static double exp(double x) { return cl_exp(x); }
 };
 struct Quad {
   typedef quad T;
   static quad sin(quad x) { return finite(x) ? sinq(x) : x - x; }
   static quad cos(quad x) { return finite(x) ? cosq(x) : x - x; }
   static quad log(quad x) { return logq(x); }
+  static quad exp(quad x) { return expq(x); }
   static quad sqrt(quad x) { return sqrtq(x); }
   static quad fma(quad a, quad b, quad c) { return fmaq(a, b, c); }
   static quad abs(quad x) { return fabsq(x); }
@@ -167,6 +170,7 @@ static int red_bytes(const cl_reducer& r, int d, int k) {
     case CL_RED_MONITOR: return 88;
     case CL_RED_RUNMONITOR: return 24;
     case CL_RED_STRETCHING: return 104;
+    case CL_RED_BIRKHOFF: return r.birkhoff ? 8 * r.birkhoff->H * (2 + 5 * r.birkhoff->K) + 16 : -1;
   }
   return -1;
 }
@@ -251,7 +255,17 @@ struct StretchSt {  // StretchState{T} (stretching.jl:77-88)
   int8_t code, column;
 };
 template <class T>
+struct WBSt {  // WBState{T, H, K} (wbirkhoff_reducer.jl:36-49), up to 16 horizons and 8 observables
+  Acc<T> B[CL_WB_MAX_WINDOWS];
+  Acc<T> A1[CL_WB_MAX_WINDOWS][CL_WB_MAX_OBSERVABLES], A2[CL_WB_MAX_WINDOWS][CL_WB_MAX_OBSERVABLES];
+  T S[CL_WB_MAX_WINDOWS][CL_WB_MAX_OBSERVABLES];
+  int64_t n;
+  int32_t handoffs;
+  int16_t code;
+};
+template <class T>
 struct RState {  // one of the above, by kind
+  WBSt<T> wb;
   StretchSt<T> sn;
   MonSt<T> mon;
   QRSt<T> qr;
@@ -292,6 +306,26 @@ static void write_state(const cl_reducer& r, const RState<T>& s, int d, int k, u
       b.put(dbl(s.mon.det_max));
       b.put(dbl(s.mon.symp_max));
       b.put(dbl(s.mon.M_max));
+      break;
+    }
+    case CL_RED_BIRKHOFF: {
+      const WBSt<T>& q = s.wb;
+      const cl_birkhoff& pb = *r.birkhoff;
+      for (int h = 0; h < pb.H; ++h) {
+        b.put(dbl(q.B[h].s));
+        b.put(dbl(q.B[h].c));
+        for (int c = 0; c < pb.K; ++c) {
+          b.put(dbl(q.A1[h][c].s));
+          b.put(dbl(q.A1[h][c].c));
+          b.put(dbl(q.A2[h][c].s));
+          b.put(dbl(q.A2[h][c].c));
+          b.put(dbl(q.S[h][c]));
+        }
+      }
+      b.put(q.n);
+      b.put(q.handoffs);
+      b.put(q.code);
+      b.pad(2);
       break;
     }
     case CL_RED_STRETCHING: {
@@ -797,6 +831,65 @@ static void red_stretch(const cl_reducer& r, StretchSt<T>& st, int64_t n, T* W, 
     }
 }
 
+// WeightedBirkhoff (wbirkhoff_reducer.jl:51-66, the update _wb_sample of wbirkhoff.jl:118-123): the observables of
+// the point, the first non-finite one fails the orbit; sample j = handoffs goes into window 1 or 2 of every horizon
+// that has not yet taken its 2N samples, with the weight exp(-1 / (t^p (1 - t)^p)), t = min(jj, N - jj) / N
+// (birkhoff_weight, naff_reducer.jl:13-17; the powers by plain multiplication).
+template <class P, class T>
+static T wb_obs(const cl_birkhoff& b, int k, const T* x) {
+  if (b.kind[k] == CL_OBS_COS) return P::cos((T)b.b[k] * x[b.a[k]]);
+  if (b.kind[k] == CL_OBS_SIN) return P::sin((T)b.b[k] * x[b.a[k]]);
+  return x[b.a[k]] + (T)2 * P::pi() * x[b.b[k]];
+}
+template <class P, class T>
+static T wb_w(int64_t jj, int64_t N, int p) {
+  const int64_t mn = jj < N - jj ? jj : N - jj;
+  const T t = (T)mn / (T)N;
+  if (t <= 0 || t >= 1) return (T)0;
+  auto pw = [&](T u) {
+    const T u2 = u * u;
+    return p == 1 ? u : (p == 2 ? u2 : (p == 3 ? u2 * u : u2 * u2));
+  };
+  return P::exp((T)-1 / (pw(t) * pw((T)1 - t)));
+}
+template <class P, class T>
+static void red_wbirk(const cl_reducer& r, WBSt<T>& st, int64_t n, const T* x) {
+  if (st.code) return;
+  const cl_birkhoff& b = *r.birkhoff;
+  int64_t mx = 0;
+  for (int h = 0; h < b.H; ++h) mx = b.window[h] > mx ? b.window[h] : mx;
+  if ((int64_t)st.handoffs >= 2 * mx) {
+    st.n = n;
+    if (st.handoffs < 2147483647) st.handoffs += 1;
+    return;
+  }
+  T f[CL_WB_MAX_OBSERVABLES];
+  for (int k = 0; k < b.K; ++k) f[k] = wb_obs<P>(b, k, x);
+  for (int k = 0; k < b.K; ++k)
+    if (!P::finite(f[k])) {
+      st.code = (int16_t)(k + 1);
+      st.n = n;
+      return;
+    }
+  const int64_t j = st.handoffs;
+  for (int h = 0; h < b.H; ++h) {
+    const int64_t N = b.window[h];
+    if (j >= 2 * N) continue;
+    const int i = j >= N ? 1 : 0;
+    const T w = wb_w<P, T>(j - i * N, N, b.order);
+    if (i == 0 && w != 0) st.B[h] = acc_add<P>(st.B[h], w);
+    for (int k = 0; k < b.K; ++k) {
+      st.S[h][k] = jmax<P>(st.S[h][k], P::abs(f[k]));
+      if (w != 0) {
+        if (i == 0) st.A1[h][k] = acc_add<P>(st.A1[h][k], w * f[k]);
+        else st.A2[h][k] = acc_add<P>(st.A2[h][k], w * f[k]);
+      }
+    }
+  }
+  st.n = n;
+  if (j < 2147483647) st.handoffs = (int32_t)(j + 1);
+}
+
 template <class P, class T>
 static void red_sali(SaliSt<T>& st, int64_t n, T* W, int d) {
   if (st.code) return;
@@ -981,6 +1074,7 @@ static bool red_failed(const cl_reducer& r, const RState<T>& s) {
     case CL_RED_MONITOR: return s.mon.code != 0;
     case CL_RED_RUNMONITOR: return false;
     case CL_RED_STRETCHING: return s.sn.code != 0;
+    case CL_RED_BIRKHOFF: return s.wb.code != 0;
     case CL_RED_QR: return s.qr.code != 0;
     case CL_RED_BENETTIN: return s.ben.code != 0;
     case CL_RED_MEGNO: return s.meg.code != 0;
@@ -1429,6 +1523,7 @@ static int8_t run_orbit_seg(const SYS& sys, const cl_chain& ch, int d, int k, in
         case CL_RED_MONITOR: red_monitor<P>(rd, st[r].mon, n, sys, x, w, d, k); break;
         case CL_RED_RUNMONITOR: red_runmonitor<P>(st[r].mon, sys, x, d); break;
         case CL_RED_STRETCHING: red_stretch<P>(rd, st[r].sn, n, w, d, ws); break;
+        case CL_RED_BIRKHOFF: red_wbirk<P>(rd, st[r].wb, n, x); break;
       }
     }
     for (int r = 0; r < ch.n; ++r)
@@ -2009,6 +2104,17 @@ int or_propagate_linear(const double* A, const cl_chain* ch, int64_t m, int k, i
   if (quad_)
     return propagate_all<Quad>(&sys, ch, m, 2, 2, k, N, tau, x, W, nullptr, (uint8_t*)states, code, hstep, 1, A);
   return propagate_all<F64Cl>(&sys, ch, m, 2, 2, k, N, tau, x, W, nullptr, (uint8_t*)states, code, hstep, 1, A);
+}
+
+// The weights w(jj), jj < N, of WeightedBirkhoff's horizon N (math as in or_propagate).
+void or_wb_weights(int64_t N, int order, double* w, int math) {
+  for (int64_t jj = 0; jj < N; ++jj)
+    w[jj] = math == MATH_CL ? wb_w<F64Cl, double>(jj, N, order) : wb_w<F64Libm, double>(jj, N, order);
+}
+
+// The portable exp of the kernels (cl_math.h), for its accuracy test against libm.
+void or_exp(int64_t n, const double* x, double* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = cl_exp(x[i]);
 }
 
 // n additions of x into a compensated sum (Foundation.jl:48-58), and the plain sum, for the AC9 fixture.

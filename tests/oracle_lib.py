@@ -51,6 +51,10 @@ def load():
         o.or_propagate_ws.restype = I
         o.or_propagate_ws.argtypes = [ctypes.POINTER(L.cl_system), P, ctypes.POINTER(L.cl_chain), I64, I, I, I, I64, I,
                                       P, P, P, P, P, P, I, I]
+        o.or_wb_weights.restype = None
+        o.or_wb_weights.argtypes = [I64, I, P, I]
+        o.or_exp.restype = None
+        o.or_exp.argtypes = [I64, P, P]
         o.or_csum.restype = None
         o.or_csum.argtypes = [I64, ctypes.c_double, P, P]
         o.or_csum_list.restype = None
@@ -237,3 +241,16 @@ def run_ws(sysd, chain, nws, x0, W0, N, tau, k, A=None, math=MATH_CL, nthreads=N
                            int(N), tau, _ptr(x), _ptr(W), _ptr(states), _ptr(code), _ptr(hstep), _ptr(ws), math, nt)
     assert rc == 0, rc
     return dict(x=x, W=W.transpose(0, 2, 1).copy(), states=states[:, :nb], code=code, hstep=hstep, ws=ws)
+
+
+def wb_weights(N, order, math=MATH_CL):
+    w = np.empty(int(N))
+    load().or_wb_weights(int(N), int(order), w.ctypes.data, math)
+    return w
+
+
+def exp(x):
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    out = np.empty_like(x)
+    load().or_exp(x.size, x.ctypes.data, out.ctypes.data)
+    return out
