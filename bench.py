@@ -26,6 +26,7 @@ Prints one JSON line with the metric, the roofline of the orbit kernel (HIP even
 and the CPU baseline (the oracle, glibc math, on the host's cores, on a bounded sample of the workload).
 """
 import argparse
+import math
 import ctypes
 import json
 import os
@@ -493,12 +494,143 @@ def rec_main(a):
     print(json.dumps(out), flush=True)
 
 
+# The reference's per-spec cost records of its indicator recorder (verify/experiments-indicators-device.jl:151-176, specs
+# st and bk of verify/experiments-indicators-block.toml:123-184; workspaces/module-build/stages/04-verify/output/
+# gpu-rtx5060-d86-verify-log.txt:24-29): 1024 wrapped orbits, ns per orbit-step on 23 CPU threads and its GPU (st is
+# CPU-only there).
+IND_REFERENCE_NS = {"st": (15.34, None), "bk": (29.29, 63.1)}
+
+
+def ind_case(lib, ctx, name, m, K=2 * math.pi, seed=5):
+    """One indicator workload on device buffers: st = (StretchingNumbers(bins 100, interval 4), RunMonitor), k = 2,
+    4000 steps; bk = (RunMonitor, WeightedBirkhoff(windows 250 and 1000, order 2, cos q, sin q, cos 2p, lifted p)),
+    k = 1 with the window kinds' constant deviation column, 2000 steps. Wrapped map, tau = 1, starts uniform on the
+    torus. Returns the launch's device time (HIP events on the launch stream) and the device buffers."""
+    import torch
+    from chaos_amd import (WrappedStandardMap, StretchingNumbers, RunMonitor, WeightedBirkhoff, CosObservable,
+                           SinObservable, LiftedObservable, stretching_range, torus_state)
+    from chaos_amd.tangent import make_chain
+    from chaos_amd.inputs import torus_uniform
+    sysm = WrappedStandardMap(K)
+    z = torus_uniform(seed, m)
+    x0 = np.array([torus_state(v) for v in z])
+    if name == "st":
+        reds = (StretchingNumbers(range=stretching_range(sysm), bins=100, interval=4, min_per_cell=2), RunMonitor(sysm))
+        k, N, reset = 2, 4000, False
+    else:
+        reds = (RunMonitor(sysm), WeightedBirkhoff(window=(250, 1000), birkhoff_order=2, observables=(
+            CosObservable(1), SinObservable(1), CosObservable(2, 2), LiftedObservable(2, 4))))
+        k, N, reset = 1, 2000, True
+    chain, allr = make_chain(reds, reset_frame=reset)
+    sd = sysm.descriptor()
+    words = lib.cl_state_bytes(ctypes.byref(sd), ctypes.byref(chain), 1, 2, k) // 8
+    nws = lib.cl_workspace_entries(ctypes.byref(chain), 2, k)
+    dev = torch.device("cuda")
+    dx = torch.from_numpy(np.ascontiguousarray(x0.T)).to(dev)
+    W = np.eye(2)[:, :k]
+    dW = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(W.T.reshape(2 * k, 1), (2 * k, m)))).to(dev)
+    dst = torch.empty((words, m), dtype=torch.int64, device=dev)
+    dcode = torch.zeros(m, dtype=torch.int8, device=dev)
+    dh = torch.zeros(m, dtype=torch.int64, device=dev)
+    dws = torch.empty((max(nws, 1), m), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    if nws:
+        rc = lib.cl_propagate_device_ws(ctx, ctypes.byref(sd), ctypes.byref(chain), 1, m, 4, 2, k, 0, N, N, 1,
+                                        dx.data_ptr(), dW.data_ptr(), None, dst.data_ptr(), dcode.data_ptr(),
+                                        dh.data_ptr(), dws.data_ptr(), stream.cuda_stream)
+    else:
+        rc = lib.cl_propagate_device(ctx, ctypes.byref(sd), ctypes.byref(chain), 1, m, 4, 2, k, 0, N, N, 1,
+                                     dx.data_ptr(), dW.data_ptr(), None, dst.data_ptr(), dcode.data_ptr(),
+                                     dh.data_ptr(), stream.cuda_stream)
+    e1.record(stream)
+    assert rc == 0, lib.cl_last_error(ctx)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return dict(name=name, m=m, N=N, wall=wall, kernel_ms=e0.elapsed_time(e1), bad=int((dcode != 0).sum().item()),
+                x0=x0, chain=chain, allr=allr, k=k, nws=nws, sysm=sysm)
+
+
+# algorithmic HBM bytes per orbit-step of the bk launch: the horizon sums a handoff reads and writes (window 1: B and,
+# per observable, S and A1, 2 + 3K words; window 2: S and A2, 3K words), summed over the two horizons' 2N handoffs
+# (250: 250 x 14 + 250 x 12 words; 1000: 1000 x 14 + 1000 x 12), 8 bytes each way, over 2000 steps
+IND_BK_BYTES = (250 * 14 + 250 * 12 + 1000 * 14 + 1000 * 12) * 16 / 2000
+HBM_PEAK_GBS = 8000.0
+
+
+def ind_main(a):
+    """--workload ind: the two indicator reducers of the path with no reference device path or a slow one, at the
+    reference's own ensemble (1024 orbits) and at 2^20 orbits. value = orbit-steps of the 2^20 runs / their wall time
+    (inputs resident, one launch each)."""
+    from chaos_amd.tangent import _context
+    lib, ctx = _context()
+    big = int(a.ind_orbits)
+    for name in ("st", "bk"):
+        ind_case(lib, ctx, name, 256)   # warm-up: loads the kernels
+    res = {}
+    for name in ("st", "bk"):
+        for m in (1024, big):
+            r = ind_case(lib, ctx, name, m)
+            res[(name, m)] = r
+            log(f"ind {name}: {m} orbits x {r['N']} steps: {r['kernel_ms']:.2f} ms device, "
+                f"{r['kernel_ms'] * 1e6 / (m * r['N']):.4f} ns per orbit-step, {r['bad']} failed")
+    total = sum(res[(n, big)]["m"] * res[(n, big)]["N"] for n in ("st", "bk"))
+    wall = sum(res[(n, big)]["wall"] for n in ("st", "bk"))
+    value = total / wall
+    per = {f"{n}@{m}": res[(n, m)]["kernel_ms"] * 1e6 / (m * res[(n, m)]["N"]) for n in ("st", "bk") for m in (1024, big)}
+    cpu = None
+    if not a.no_cpu:
+        from tests import oracle_lib as O
+        nt = min(16, os.cpu_count() or 1)
+        tc, steps_c = 0.0, 0
+        for n in ("st", "bk"):
+            r = res[(n, 1024)]
+            ms_ = 256
+            W0 = np.broadcast_to(np.eye(2)[:, :r["k"]], (ms_, 2, r["k"]))
+            t0 = time.perf_counter()
+            if r["nws"]:
+                O.run_ws(r["sysm"].descriptor(), r["chain"], r["nws"], r["x0"][:ms_], W0, r["N"], 1, r["k"],
+                         math=O.MATH_LIBM, nthreads=nt)
+            else:
+                O.run(r["sysm"].descriptor(), r["chain"], r["x0"][:ms_], W0, r["N"], 1, r["k"], math=O.MATH_LIBM,
+                      nthreads=nt)
+            tc += time.perf_counter() - t0
+            steps_c += ms_ * r["N"]
+        cpu = dict(value=steps_c / tc, unit="orbit-steps/s", cores=nt, kind="port",
+                   sample=f"the first 256 orbits of each 1024-orbit case, the oracle (or_propagate_ws, or_propagate) "
+                          f"with glibc sin/cos/log/exp, {nt} threads, {tc:.2f} s")
+    bk = res[("bk", big)]
+    achieved = IND_BK_BYTES * bk["m"] * bk["N"] / (bk["kernel_ms"] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC, "value": value, "unit": "orbit-steps/s", "n_gpus": 1, "steps": 2, "warmup": 1,
+        "ms_per_step": wall * 1e3 / 2, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"indicator reducers on the wrapped standard map K=2pi, tau=1: st = StretchingNumbers "
+                               f"(100 cells, interval 4) + RunMonitor, k=2, 4000 steps; bk = RunMonitor + "
+                               f"WeightedBirkhoff (windows 250/1000, order 2, 4 observables), k=1 constant column, "
+                               f"2000 steps; {big} orbits each (and the reference's 1024)",
+                   "ns_per_orbit_step": per, "reference_ns_per_orbit_step_cpu23_gpu_rtx5060": IND_REFERENCE_NS,
+                   "failed_orbits": {f"{n}@{m}": res[(n, m)]["bad"] for n in ("st", "bk") for m in (1024, big)}},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_ms": bk["kernel_ms"],
+                     "bytes_per_orbit_step": IND_BK_BYTES, "kernel": "bk (WeightedBirkhoff horizon sums in HBM)"},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "rec"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "rec", "ind"])
+    ap.add_argument("--ind-orbits", default=1 << 20, type=int, help="ind: orbits of the large cases")
     ap.add_argument("--rec-sizes", default="2048,200000", help="rec: orbits, steps")
     ap.add_argument("--c5-sizes", default="1048576,262144,4194304,100000",
                     help="c5: training orbits, held-out orbits, triaged orbits, steps N")
@@ -518,6 +650,8 @@ def main():
         return c3_main(a)
     if a.workload == "rec":
         return rec_main(a)
+    if a.workload == "ind":
+        return ind_main(a)
     wl = workload(a.workload)
     log("workload", wl["desc"])
     if a.copy_probe:

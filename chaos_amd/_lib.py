@@ -19,6 +19,7 @@ CL_RED_RUNMONITOR = 10
 CL_RED_STRETCHING = 11
 CL_QR_CGS, CL_QR_MGS, CL_QR_RGS, CL_QR_HOUSEHOLDER, CL_QR_AUGMENTED, CL_QR_GIVENS = 1, 2, 3, 4, 5, 6
 CL_MAX_REDUCERS = 8
+CL_CHAIN_RESET_FRAME = 1
 
 # FALLBACK_REASONS of the reference (src/Foundation/Foundation.jl:71-75), indexed from 1.
 FALLBACK_REASONS = [
@@ -53,7 +54,7 @@ class cl_reducer(ctypes.Structure):
 
 
 class cl_chain(ctypes.Structure):
-    _fields_ = [("n", ctypes.c_int32), ("pad_", ctypes.c_int32), ("r", cl_reducer * CL_MAX_REDUCERS)]
+    _fields_ = [("n", ctypes.c_int32), ("flags", ctypes.c_int32), ("r", cl_reducer * CL_MAX_REDUCERS)]
 
 
 class cl_probe_t(ctypes.Structure):

@@ -34,6 +34,8 @@ struct KArgs {
   uint32_t* ws;
   // WeightedBirkhoff weight tables (one per horizon), or null
   const double* wtab;
+  // restart the frame from the unit columns after every handoff (CL_CHAIN_RESET_FRAME)
+  int reset_frame;
 };
 
 // Per-orbit context the reducers read: the point, the carried sin/cos, the frames and a cache of
@@ -403,6 +405,12 @@ __global__ void __launch_bounds__(256) orbit_kernel(KArgs a, Sys sys, Chain chai
       code = STATUS_REDUCER;
       early = true;
       break;
+    }
+    if (a.reset_frame) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int e = 0; e < D * K; ++e) c.W[f].w[e] = (e % (D + 1) == 0) ? 1.0 : 0.0;
     }
   }
   if (!early) code = (n == a.N) ? frame_status(c) : STATUS_OK;

@@ -403,6 +403,7 @@ static int launch(const Instance* in, const cl_system* sys, const cl_chain* ch, 
   memset(&a, 0, sizeof(a));
   a.ws = ws;
   a.mask = mask;
+  a.reset_frame = (ch->flags & CL_CHAIN_RESET_FRAME) ? 1 : 0;
   const int wbr = wb_index(ch);
   double* wtab = nullptr;
   if (wbr >= 0) {  // the weight tables, formed on the device for this launch

@@ -63,8 +63,9 @@ def _context():
     return lib, _ctx
 
 
-def make_chain(reducers, reducers2=()):
+def make_chain(reducers, reducers2=(), reset_frame=False):
     ch = L.cl_chain()
+    ch.flags = L.CL_CHAIN_RESET_FRAME if reset_frame else 0
     allr = [(r, 0) for r in reducers] + [(r, 1) for r in reducers2]
     if len(allr) > L.CL_MAX_REDUCERS:
         raise ValueError(f"at most {L.CL_MAX_REDUCERS} reducers per run")
@@ -133,8 +134,10 @@ def check_health(code, hstep, x, W, states, allr, orbit0=1):
 
 
 def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None, launch_steps=None,
-              check=True, keep_states=False):
-    """Propagate every orbit of `x0` (shape (m, s)) for N steps with k deviation vectors on the GPU."""
+              check=True, keep_states=False, reset_frame=False):
+    """Propagate every orbit of `x0` (shape (m, s)) for N steps with k deviation vectors on the GPU.
+    reset_frame=True restarts the frame from the unit columns after every handoff (the indicator recorder's window
+    kinds, src/Experiments/run_indicator_device.jl:66-80); W0 is then the unit columns."""
     import torch
     N, tau, k = int(N), int(tau), int(k)
     if not (tau >= 1 and N >= 1 and N % tau == 0):
@@ -161,7 +164,9 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
     for r in list(reducers) + list(reducers2):
         if getattr(r, "sys", None) is not None and bytes(r.sys.descriptor()) != bytes(sysd):
             raise ValueError(f"{type(r).__name__} monitors {r.sys!r}, not the propagated {sys!r}")
-    chain, allr = make_chain(reducers, reducers2)
+    chain, allr = make_chain(reducers, reducers2, reset_frame)
+    if reset_frame:
+        W0 = np.broadcast_to(np.eye(d)[:, :k], (m, d, k))
     nws = workspace_entries(allr, d, k)
     nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, d, k)
     if nbytes < 0:

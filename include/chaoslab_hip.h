@@ -123,9 +123,14 @@ typedef struct {
 } cl_reducer;
 
 #define CL_MAX_REDUCERS 8
+/* cl_chain.flags: CL_CHAIN_RESET_FRAME restarts the deviation frame from the unit columns e_1..e_k after every
+ * handoff, as the indicator recorder's window kinds run their readers ("one handoff at a time with the constant
+ * deviation column", _ind_window_kernel!, src/Experiments/run_indicator_device.jl:66-80); the caller passes the
+ * unit columns as W. */
+#define CL_CHAIN_RESET_FRAME 1
 typedef struct {
   int32_t n;
-  int32_t pad_;
+  int32_t flags;
   cl_reducer r[CL_MAX_REDUCERS];
 } cl_chain;
 

@@ -1531,6 +1531,9 @@ static int8_t run_orbit_seg(const SYS& sys, const cl_chain& ch, int d, int k, in
         *hstep = n;
         return 3;
       }
+    if (ch.flags & CL_CHAIN_RESET_FRAME)  // the window kinds' constant deviation column (run_indicator_device.jl:66-80)
+      for (int f = 0; f < nf; ++f)
+        for (int e = 0; e < d * k; ++e) W[f][e] = (e % (d + 1) == 0) ? (T)1 : (T)0;
   }
   *hstep = n;
   return n == N ? status() : (int8_t)0;

@@ -121,3 +121,23 @@ def test_full_size_sample():
     assert np.array_equal(res.states[idx], orc["states"]) and np.array_equal(res.code[idx], orc["code"])
     dig = res.finals[0]["dig"][res.code == 0][:, :, -1]
     assert np.median(dig) > 8
+
+
+@pytest.mark.parametrize("wrapped", [False, True], ids=["lifted", "wrapped"])
+def test_window_mode_equals_oracle(wrapped):
+    """The window kinds' constant deviation column (CL_CHAIN_RESET_FRAME): (RunMonitor, WeightedBirkhoff) on chaotic
+    orbits at K = 2 pi over windows 250 and 1000 (order 2, the reference's bk spec) equals the oracle bit for bit."""
+    m = 3000
+    sysm = (WrappedStandardMap if wrapped else StandardMap)(2 * math.pi)
+    x0 = starts(m, wrapped, 31)
+    obs = (CosObservable(1), SinObservable(1), CosObservable(2, 2),
+           LiftedObservable(2, 4) if wrapped else LiftedObservable(1, 2))
+    red = WeightedBirkhoff(window=(250, 1000), observables=obs, birkhoff_order=2)
+    chain, _ = make_chain((RunMonitor(sysm), red), reset_frame=True)
+    W0 = np.broadcast_to(np.array([[1.0], [0.0]]), (m, 2, 1))
+    orc = O.run(sysm.descriptor(), chain, x0, W0, 2000, 1, 1)
+    say("birkhoff window mode", "wrapped" if wrapped else "lifted", "m", m)
+    res = propagate(sysm, x0, 2000, k=1, tau=1, seed=1, reducers=(RunMonitor(sysm), red), check=False,
+                    keep_states=True, reset_frame=True, launch_steps=700)
+    assert np.array_equal(res.code, orc["code"]) and np.array_equal(res.states, orc["states"])
+    assert np.mean(orc["code"] == 0) > 0.99

@@ -24,8 +24,8 @@ def cp(p):
     return 4.0 ** p   # c_p of the reference suite (wb_helpers.jl:21)
 
 
-def run_wb(red, sysm, x0, N, tau=1, extra_first=(), k=1):
-    chain, allr = make_chain(tuple(extra_first) + (red,))
+def run_wb(red, sysm, x0, N, tau=1, extra_first=(), k=1, reset=False):
+    chain, allr = make_chain(tuple(extra_first) + (red,), reset_frame=reset)
     W0 = np.tile(np.array([[1.0], [0.0]]) if k == 1 else np.eye(2), (len(x0), 1, 1))
     r = O.run(sysm.descriptor(), chain, x0, W0, N, tau, k)
     fin = finals_of(decode_states(r["states"], allr, 2, k), allr, 2, k)
@@ -148,7 +148,7 @@ def sampled_series(sysm, x0, nsamp, tau, obs):
     """The observables at the handoffs 1..nsamp of one orbit: its point after j tau steps from the oracle's runs,
     the values with the kernels' sin/cos (cl_math_host)."""
     lib = L.load()
-    chain, _ = make_chain(())
+    chain, _ = make_chain((), reset_frame=True)   # the deviation column restarts each handoff: no overflow stops the orbit
     pts = []
     for j in range(1, nsamp + 1):
         pts.append(O.run(sysm.descriptor(), chain, x0[None], np.array([[[1.0], [0.0]]]), j * tau, tau, 1)["x"][0])
@@ -241,3 +241,24 @@ def test_index_rules():
         red.check_run(1, 100)
     with pytest.raises(ValueError):
         red.check_run(0, 128)
+
+
+@pytest.mark.parametrize("K", [2 * math.pi, 10.0])
+def test_window_mode_on_chaotic_orbits(K):
+    """The window kinds' constant deviation column (reset after every handoff, run_indicator_device.jl:66-80): on
+    chaotic orbits the column never overflows, the frame is e_1 after each handoff, RunMonitor's maxima are unchanged,
+    and the reducer still equals the pure function on the sampled series bit for bit (AC6 on the chaotic classes)."""
+    sysm = WrappedStandardMap(K)
+    Ns = (250, 1000)
+    obs = (CosObservable(1), SinObservable(1), CosObservable(2, 2), LiftedObservable(2, 4))
+    red = WeightedBirkhoff(window=Ns, observables=obs, birkhoff_order=2)
+    x0 = np.array(torus_state([0.7, 2.9]))
+    r, f = run_wb(red, sysm, x0[None], 2000, 1, extra_first=(RunMonitor(sysm),), reset=True)
+    assert r["code"][0] == 0 and np.array_equal(r["W"][0], [[1.0], [0.0]])
+    r0, _ = run_wb(red, sysm, x0[None], 2000, 1, extra_first=(RunMonitor(sysm),))
+    assert r0["code"][0] == 2   # without the reset the unnormalised column overflows
+    assert np.array_equal(r["states"][0, :24], r0["states"][0, :24]) or r0["hstep"][0] < 2000
+    ser = sampled_series(sysm, x0, 2000, 1, obs)
+    g = weighted_birkhoff(ser, Ns, order=2)
+    assert np.array_equal(f["avg1"][0], g["avg1"]) and np.array_equal(f["avg2"][0], g["avg2"])
+    assert np.array_equal(f["S"][0], g["S"])
