@@ -48,7 +48,9 @@ FLOPS_PER_ORBIT_STEP = {"c2": 500.0, "c1": 174.0}
 # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch), scaled by what the
 # counters read for a 1 GiB device copy (FETCH x 2.0, WRITE x 1.0): for c2 the two passes of the split chain
 # (indicator pass 24792.9 / 590963.3 KiB, QR pass 24693.1 / 164881.8 KiB); summaries under profiles/.
-TRAFFIC_BYTES_PER_LAUNCH = {"c2": (2 * (24792.88 + 24693.06) + 590963.25 + 164881.81) * 1024}
+TRAFFIC_BYTES_PER_LAUNCH = {"c2": (2 * (24792.88 + 24693.06) + 590963.25 + 164881.81) * 1024,
+                            # the ind workload's bk launch at 2^20 orbits (58419373.6 / 226179433.0 KiB)
+                            "ind_bk": (2 * 58419373.56 + 226179433.0) * 1024}
 # Mean Gauss fixed-point iterations per step on the c4 workload, measured with the oracle on its sample
 # (used when the CPU leg is skipped).
 C4_ITERATIONS_DEFAULT = 6.3
@@ -615,7 +617,9 @@ def ind_main(a):
                    "ns_per_orbit_step": per, "reference_ns_per_orbit_step_cpu23_gpu_rtx5060": IND_REFERENCE_NS,
                    "failed_orbits": {f"{n}@{m}": res[(n, m)]["bad"] for n in ("st", "bk") for m in (1024, big)}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_ms": bk["kernel_ms"],
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC_BYTES_PER_LAUNCH["ind_bk"] if bk["m"] == 1 << 20 else None,
+                     "kernel_ms": bk["kernel_ms"],
                      "bytes_per_orbit_step": IND_BK_BYTES, "kernel": "bk (WeightedBirkhoff horizon sums in HBM)"},
         "cpu_baseline": cpu,
     }

@@ -1240,14 +1240,29 @@ struct WBHead {
   int16_t code;
   int16_t pad_;
 };
+// The per-launch table (device memory): words 0..15 the windows N_h, 16..31 the offsets of their weights, then the
+// weights (so the horizon loop reads its bounds with uniform loads instead of indexing a kernel argument).
+constexpr int WB_TABLE_HEADER = 2 * CL_WB_MAX_WINDOWS;
 struct WBParams {
   int H, K, order;
-  int64_t win[CL_WB_MAX_WINDOWS];
-  int64_t woff[CL_WB_MAX_WINDOWS];  // offset of horizon h's weights in the table
   int kind[CL_WB_MAX_OBSERVABLES], a[CL_WB_MAX_OBSERVABLES], b[CL_WB_MAX_OBSERVABLES];
   int64_t nmax2;  // 2 max(window): handoffs of a complete run
   int hw;         // words of the horizons' sums
 };
+// x[a] of a point of S coordinates without indexing the register array at run time
+template <int S>
+CL_HD double wb_pick(const double* x, int a) {
+  double v = x[0];
+#pragma unroll
+  for (int j = 1; j < S; ++j) {
+    double xj = x[j];
+#if defined(__HIP_DEVICE_COMPILE__)
+    __asm__ volatile("" : "+v"(xj));  // keeps the selects on registers (else they fold into a scratch load of x[a])
+#endif
+    v = (a == j) ? xj : v;
+  }
+  return v;
+}
 CL_HD double wb_pow(double u, int p) {
   const double u2 = u * u;
   return p == 1 ? u : (p == 2 ? u2 : (p == 3 ? u2 * u : u2 * u2));
@@ -1272,9 +1287,10 @@ struct WBirk {
   template <class Sys>
   CL_HD double observable(int k, const double* x) const {
     const int a = p.a[k], b = p.b[k];
-    if (p.kind[k] == CL_OBS_COS) return cl_cos((double)b * x[a]);
-    if (p.kind[k] == CL_OBS_SIN) return cl_sin((double)b * x[a]);
-    return x[a] + 2.0 * (3.141592653589793 + 1.2246467991473532e-16) * x[b];
+    const double xa = wb_pick<Sys::S>(x, a);
+    if (p.kind[k] == CL_OBS_COS) return cl_cos((double)b * xa);
+    if (p.kind[k] == CL_OBS_SIN) return cl_sin((double)b * xa);
+    return xa + 2.0 * (3.141592653589793 + 1.2246467991473532e-16) * wb_pick<Sys::S>(x, b);
   }
   // mem: the orbit's first horizon word (words `stride` apart); wtab: the weight tables
   template <class Sys>
@@ -1297,11 +1313,12 @@ struct WBirk {
       }
     const int64_t j = st.handoffs;
     const int cw = 2 + 5 * p.K;
+    const int64_t* hdr = reinterpret_cast<const int64_t*>(wtab);
     for (int h = 0; h < p.H; ++h) {
-      const int64_t N = p.win[h];
+      const int64_t N = hdr[h];
       if (j >= 2 * N) continue;
       const int i = j >= N ? 1 : 0;
-      const double w = wtab[p.woff[h] + (j - i * N)];
+      const double w = wtab[WB_TABLE_HEADER + hdr[CL_WB_MAX_WINDOWS + h] + (j - i * N)];
       double* hb = mem + (int64_t)h * cw * stride;
       if (i == 0 && w != 0.0) {
         CSum B{hb[0], hb[stride]};
@@ -1311,7 +1328,7 @@ struct WBirk {
       }
 #pragma unroll
       for (int k = 0; k < CL_WB_MAX_OBSERVABLES; ++k) {
-        if (k >= p.K) break;
+        if (k >= p.K) continue;
         double* c = hb + (int64_t)(2 + 5 * k) * stride;
         c[4 * stride] = cl_jmax(c[4 * stride], cl_fabs(f[k]));
         if (w != 0.0) {

@@ -198,13 +198,8 @@ inline void slot_setup(WBirk& r, const cl_reducer& d) {
   r.p.H = b.H;
   r.p.K = b.K;
   r.p.order = b.order;
-  int64_t off = 0, mx = 0;
-  for (int h = 0; h < CL_WB_MAX_WINDOWS; ++h) {
-    r.p.win[h] = h < b.H ? b.window[h] : 0;
-    r.p.woff[h] = off;
-    off += r.p.win[h];
-    mx = r.p.win[h] > mx ? r.p.win[h] : mx;
-  }
+  int64_t mx = 0;
+  for (int h = 0; h < b.H; ++h) mx = b.window[h] > mx ? b.window[h] : mx;
   for (int k = 0; k < CL_WB_MAX_OBSERVABLES; ++k) {
     r.p.kind[k] = k < b.K ? b.kind[k] : 0;
     r.p.a[k] = k < b.K ? b.a[k] : 0;

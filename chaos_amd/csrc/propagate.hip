@@ -135,14 +135,22 @@ static const char* wb_check(const cl_chain* ch, int s) {
   return nullptr;
 }
 
-// The weight tables: horizon h's w(jj), jj < N_h, at offset sum_{h' < h} N_h'.
+// The per-launch table: the header (windows, offsets) and horizon h's w(jj), jj < N_h, at offset
+// sum_{h' < h} N_h' after it.
 __global__ void k_wb_weights(int H, cl_birkhoff b, double* wtab) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t* hdr = reinterpret_cast<int64_t*>(wtab);
+  if (t < CL_WB_MAX_WINDOWS) {
+    int64_t o = 0;
+    for (int h = 0; h < t && h < H; ++h) o += b.window[h];
+    hdr[t] = t < H ? b.window[t] : 0;
+    hdr[CL_WB_MAX_WINDOWS + t] = o;
+  }
   int64_t off = 0;
   for (int h = 0; h < H; ++h) {
     const int64_t N = b.window[h];
     if (t >= off && t < off + N) {
-      wtab[t] = wb_weight(t - off, N, b.order);
+      wtab[WB_TABLE_HEADER + t] = wb_weight(t - off, N, b.order);
       return;
     }
     off += N;
@@ -410,9 +418,10 @@ static int launch(const Instance* in, const cl_system* sys, const cl_chain* ch, 
     const cl_birkhoff& b = *ch->r[wbr].birkhoff;
     int64_t tot = 0;
     for (int h = 0; h < b.H; ++h) tot += b.window[h];
-    hipError_t e = hipMallocAsync((void**)&wtab, (size_t)tot * 8, st);
+    hipError_t e = hipMallocAsync((void**)&wtab, (size_t)(tot + WB_TABLE_HEADER) * 8, st);
     if (e != hipSuccess) return map_err(e);
-    hipLaunchKernelGGL(k_wb_weights, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, b.H, b, wtab);
+    const int64_t nt = tot > CL_WB_MAX_WINDOWS ? tot : CL_WB_MAX_WINDOWS;
+    hipLaunchKernelGGL(k_wb_weights, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, b.H, b, wtab);
     a.wtab = wtab;
   }
   a.trace = trace;
