@@ -13,8 +13,8 @@ int pipe_launch(const cl_system* s, const cl_chain* d, const KArgs& a, uint8_t* 
   slot_setup(qr, d->r[0]);
   const int H = pipe_handoffs(a.tau);
   const int64_t grid = (a.m + PIPE_LANES - 1) / PIPE_LANES;
-  hipLaunchKernelGGL((pipe_kernel<Sys, ALG, K, MON>), dim3((unsigned)grid), dim3((MON ? 7 : 6) * PIPE_LANES), 0, st, a,
-                     sys, qr, H, rerun);
+  hipLaunchKernelGGL((pipe_kernel<Sys, ALG, K, MON>), dim3((unsigned)grid), dim3(pipe_waves<ALG, K, MON>() * PIPE_LANES),
+                     0, st, a, sys, qr, H, rerun);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (e == hipErrorOutOfMemory ? CL_ERR_OOM : CL_ERR_DEVICE_LOST);
 }

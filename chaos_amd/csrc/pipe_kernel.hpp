@@ -3,24 +3,28 @@
 // A small ensemble (a few thousand orbits) fills a few dozen waves on a 1024-SIMD device, so the fused
 // kernel runs at the issue rate of one wave per SIMD: its time is the length of the per-step instruction
 // stream (sin/cos of the point, the frame product, the factorisation, its checks and two logs), step after
-// step. The work of one orbit-step splits into four parts with one-way data flow:
+// step. The work of one orbit-step splits into parts with one-way data flow:
 //
 //   A  the orbit: sin/cos of x_n, the step, the parameter c_n = K cos x_n of the tangent (standard_map.jl:30-34)
-//   B  the frame: W <- M(c) W in the fused_product order, then the factorisation W = QR (carries W = Q)
+//   B  the frame: W <- M(c) W in the fused_product order, then the factorisation W = QR (carries W = Q). For the
+//      Gram-Schmidt family at k = 2 the first column's recurrence (normalise M q_1) does not read the second
+//      column, so B splits into B1, column 1, and B2, column 2, which projects on the q_1 that B1 formed one
+//      chunk earlier: two step-to-step chains of half the length instead of one.
 //   C  the checks of the factorisation: window_Gw conditioning and orth_loss (qr_spectrum.jl:54-73),
 //      and the running maxima G_max, orth_max
 //   D  the exponent sums: add(sums[i], log R_ii)
 //   E  (chains QRSpectrum + RunMonitor, the recorder's) the monitor's running maxima of the one-step tangent's
 //      det and symplecticity residuals and of max |M| at each handoff point, from c at that point
 //
-// Each part runs on its own waves of a 384-lane group (64 orbits, one per lane of every wave): A and B one
-// wave each; C two waves, for the even and the odd handoffs (a maximum does not depend on the order of its
+// Each part runs on its own waves of a group of 64 orbits (one per lane of every wave): A, B (or B1 and B2) and
+// E one wave each; C two waves, for the even and the odd handoffs (a maximum does not depend on the order of its
 // terms, and both start from the same value); D one wave per column (each column's compensated sum stays in
-// step order). One orbit-step occupies six SIMDs, and B, whose factorisation carries the frame from step to
-// step, sets the pace. The parts hand data forward through double-buffered LDS rings: A writes c for a chunk
-// of steps while B consumes the previous chunk's c and writes R and Q, which C and D read one chunk later.
-// One barrier per chunk. The arithmetic of every part is the fused kernel's, in the same order, so the
-// results are bitwise the same.
+// step order). The parts hand data forward through triple-buffered LDS rings, one chunk of steps per stage:
+// at iteration p, A writes c for chunk p, B (B1) factorises chunk p - 1, B2 chunk p - 2, C and D read the
+// factors of the chunk the last B stage finished one iteration earlier. One barrier per iteration. C and D
+// form their per-handoff values for the whole chunk before folding them in order, so the handoffs' logs and
+// checks overlap instead of queuing on the running sum. The arithmetic of every part is the fused kernel's,
+// in the same order, so the results are bitwise the same.
 //
 // A failure (non-finite point or frame at a handoff, a failed factorisation or check) is not handled here:
 // the group marks the orbit, writes nothing back for it, and the caller reruns the marked orbits with the
@@ -32,35 +36,99 @@
 namespace cl {
 
 constexpr int PIPE_LANES = 64;   // orbits per group
-constexpr int PIPE_WAVES = 6;    // A, B, C even, C odd, D column 1, D column 2 (and E for a monitored chain)
 constexpr int PIPE_STEPS = 16;   // map steps per chunk at most (tau <= 16)
 constexpr int PIPE_HAND = 8;     // handoffs per chunk at most
+constexpr int PIPE_SLOTS = 3;    // ring buffers: the chunk an iteration writes, the one it reads, the one in flight
 
 // handoffs per chunk for a given tau: chunks of about 8 steps, at least one handoff
 CL_HD int pipe_handoffs(int tau) { return tau <= PIPE_HAND ? PIPE_HAND / tau : 1; }
 
+// B splits into two column waves for the Gram-Schmidt family at k = 2
+template <int ALG, int K>
+constexpr bool pipe_split() {
+  return K == 2 && (ALG == QR_CGS || ALG == QR_MGS || ALG == QR_RGS);
+}
+template <int ALG, int K, bool MON>
+constexpr int pipe_waves() {
+  return 6 + (pipe_split<ALG, K>() ? 1 : 0) + (MON ? 1 : 0);
+}
+
+// roles, and their waves: wave w runs on SIMD w mod 4, so A and the B waves get a SIMD each, shared with a light
+// part (C, D or E)
+enum : int { PR_A = 0, PR_B = 1, PR_C0 = 2, PR_C1 = 3, PR_D0 = 4, PR_D1 = 5, PR_E = 6, PR_B2 = 7 };
+template <bool SPLIT>
+CL_HD int pipe_role(int wave) {
+  if (SPLIT)  // waves 0..7: A, B1, B2, C odd, C even, D col 1, D col 2, E
+    return wave == 0 ? PR_A : wave == 1 ? PR_B : wave == 2 ? PR_B2 : wave == 3 ? PR_C1 : wave == 4 ? PR_C0
+         : wave == 5 ? PR_D0 : wave == 6 ? PR_D1 : PR_E;
+  // waves 0..6: A, B, C even, C odd, D col 2, D col 1, E
+  return wave == 0 ? PR_A : wave == 1 ? PR_B : wave == 2 ? PR_C0 : wave == 3 ? PR_C1 : wave == 4 ? PR_D1
+       : wave == 5 ? PR_D0 : PR_E;
+}
+
+// Column m of the Gram-Schmidt factorisation of a 2 x 2 frame (orthonormalize_gs, m = 0 and m = 1, the same
+// operations in the same order): u is the column after the tangent, q1 the first column of Q (m = 1).
+// Returns the norm gamma; r01 gets the projection coefficient (m = 1), passes the reorthogonalisation passes.
+template <int ALG>
+CL_HD double pipe_gs_column(int m, double* u, const double* q1, double* r01, int* passes, int8_t* status) {
+  *passes = 1;
+  *r01 = 0.0;
+  if (m == 1) {
+    if (ALG == QR_CGS || ALG == QR_MGS) {
+      const double c = dot<2>(q1, u);
+      *r01 = c;
+#pragma unroll
+      for (int l = 0; l < 2; ++l) u[l] = u[l] - c * q1[l];
+    } else {  // RGS
+      const double eta = 1.0 / cl_sqrt(2.0);
+      double nold = safe_norm<2>(u);
+      int ps = 0;
+      double r = 0.0;
+      for (;;) {
+        ps += 1;
+        double vold[2] = {u[0], u[1]};
+        const double c = dot<2>(q1, vold);
+        r = r + c;
+#pragma unroll
+        for (int l = 0; l < 2; ++l) u[l] = u[l] - c * q1[l];
+        double nnew = safe_norm<2>(u);
+        if (nnew >= nold * eta || !(cl_isfinite(nnew) && nnew > 0.0)) break;
+        if (ps == MAX_PASSES) {
+          ps = MAX_PASSES + 1;
+          break;
+        }
+        nold = nnew;
+      }
+      *r01 = r;
+      *passes = ps;
+    }
+  }
+  const double gamma = safe_norm<2>(u);
+  *status = *passes > MAX_PASSES ? QR_PASSES : (!(cl_isfinite(gamma) && gamma > 0.0) ? QR_PIVOT : QR_OK);
+  return gamma;
+}
+
 // MON: the chain is QRSpectrum then RunMonitor (its state words follow the QR state's)
 template <class Sys, int ALG, int K, bool MON>
-__global__ void __launch_bounds__((MON ? 7 : 6) * PIPE_LANES)
+__global__ void __launch_bounds__((pipe_waves<ALG, K, MON>() * PIPE_LANES))
 pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
   constexpr int D = 2;
-  constexpr int NW = MON ? 7 : 6;
+  constexpr bool SPLIT = pipe_split<ALG, K>();
+  constexpr int NW = pipe_waves<ALG, K, MON>();
   constexpr int WMON = (int)(sizeof(QRState<K>) / 8);  // the monitor's first state word
+  constexpr int LAG = SPLIT ? 3 : 2;                    // iterations from A's chunk to C's and D's
   typedef QRState<K> St;
-  __shared__ double ringC[2][PIPE_STEPS][PIPE_LANES];
-  __shared__ double ringR[2][PIPE_HAND][K * K][PIPE_LANES];
-  __shared__ double ringQ[2][PIPE_HAND][D * K][PIPE_LANES];
+  __shared__ double ringC[PIPE_SLOTS][PIPE_STEPS][PIPE_LANES];
+  __shared__ double ringR[PIPE_SLOTS][PIPE_HAND][K * K][PIPE_LANES];
+  __shared__ double ringQ[PIPE_SLOTS][PIPE_HAND][D * K][PIPE_LANES];
   __shared__ double ringM[MON ? 2 : 1][MON ? PIPE_HAND : 1][PIPE_LANES];  // c at each handoff point
-  __shared__ int bad_by[NW][PIPE_LANES];
+  __shared__ int bad_by[8][PIPE_LANES];
   __shared__ double outG[2][PIPE_LANES], outO[2][PIPE_LANES];
   __shared__ CSum outS[2][PIPE_LANES];  // [1]: column 2's sum, from the wave that formed it
-  __shared__ int64_t outCount[PIPE_LANES];
+  __shared__ int64_t outCount[2][PIPE_LANES];  // [1]: B2's (the reorthogonalisation passes of column 2)
 
-  // Waves w and w + 4 of a group share a SIMD (round-robin placement over the CU's four SIMDs). Measured:
-  // the step time is set by B whatever it shares with (placements differ by under 3%); the two C waves get a
-  // SIMD each, A shares with D of column 2 and B with D of column 1 (E, when present, with C even).
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int role = wave < 4 ? wave : (wave == 4 ? 5 : (wave == 5 ? 4 : 6));
+  const int role = pipe_role<SPLIT>(wave);
   const int64_t i = (int64_t)blockIdx.x * PIPE_LANES + lane;
   const bool in_range = i < a.m;
   const bool live = in_range && (a.n0 == 0 || a.code[i] == STATUS_OK);
@@ -72,7 +140,7 @@ pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
   RunMonState mst;
   const RunMon<2> mon = {};
   mon.init(mst);
-  if (MON && role == 6 && live && a.n0 != 0) state_load(mst, a.st, a.m, i, WMON);
+  if (MON && role == PR_E && live && a.n0 != 0) state_load(mst, a.st, a.m, i, WMON);
 
   double x[Sys::S];
   typename Sys::Aux aux;
@@ -81,23 +149,24 @@ pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
   for (int j = 0; j < Sys::S; ++j) x[j] = 0.0;
 #pragma unroll
   for (int e = 0; e < D * K; ++e) W.w[e] = 0.0;
-  if (role == 0 && in_range) {
+  if (role == PR_A && in_range) {
 #pragma unroll
     for (int j = 0; j < Sys::S; ++j) x[j] = a.x[(int64_t)j * a.m + i];
   }
-  if (role == 1 && in_range) {
+  if ((role == PR_B || role == PR_B2) && in_range) {
 #pragma unroll
     for (int e = 0; e < D * K; ++e) W.w[e] = a.W0[(int64_t)e * a.m + i];
   }
-  if (role == 0) sys.aux(x, aux);
+  if (role == PR_A) sys.aux(x, aux);
 
   const int64_t nh = a.steps / tau;
   const int64_t nc = (nh + H - 1) / H;
   bool bad = false;
-  for (int64_t p = 0; p < nc + 2; ++p) {
-    if (role == 0) {
+  int64_t cmax = st.count_max;
+  for (int64_t p = 0; p < nc + LAG; ++p) {
+    if (role == PR_A) {
       if (p < nc) {
-        const int slot = (int)(p & 1);
+        const int slot = (int)(p % PIPE_SLOTS);
         const int64_t h0 = p * H, h1 = (h0 + H < nh) ? h0 + H : nh;
         for (int64_t h = h0; h < h1; ++h) {
           for (int t = 0; t < tau; ++t) {
@@ -109,13 +178,14 @@ pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
 #pragma unroll
           for (int j = 0; j < Sys::S; ++j) xf = xf && cl_isfinite(x[j]);
           bad = bad || !xf;
-          if constexpr (MON) ringM[slot][(int)(h - h0)][lane] = sys.tan_c(aux);
+          if constexpr (MON) ringM[(int)(p & 1)][(int)(h - h0)][lane] = sys.tan_c(aux);
         }
       }
-    } else if (role == 1) {
-      const int64_t q = p - 1;
+    } else if (role == PR_B || role == PR_B2) {
+      // B (or B1, column 1) works on chunk p - 1; B2 (column 2) on chunk p - 2, with B1's q_1 of that chunk
+      const int64_t q = p - (role == PR_B ? 1 : 2);
       if (q >= 0 && q < nc) {
-        const int slot = (int)(q & 1);
+        const int slot = (int)(q % PIPE_SLOTS);
         const int64_t h0 = q * H, h1 = (h0 + H < nh) ? h0 + H : nh;
         // the next step's c is read one step ahead, so the LDS latency overlaps this step's work instead of
         // sitting on the frame's step-to-step chain (the index is clamped to the chunk's last step)
@@ -124,41 +194,86 @@ pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
         double cn = ringC[slot][0][lane];
         for (int64_t h = h0; h < h1; ++h) {
           const int r = (int)(h - h0);
-          for (int t = 0; t < tau; ++t) {
-            const double c = cn;
-            ++si;
-            cn = ringC[slot][si < ns ? si : ns - 1][lane];
-            map_tangent_apply(c, W);
+          if constexpr (SPLIT) {
+            const int col = role == PR_B ? 0 : 1;
+            Frame<D, 1> w1;
+            w1.w[0] = W.w[2 * col];
+            w1.w[1] = W.w[2 * col + 1];
+            for (int t = 0; t < tau; ++t) {
+              const double c = cn;
+              ++si;
+              cn = ringC[slot][si < ns ? si : ns - 1][lane];
+              map_tangent_apply(c, w1);
+            }
+            bad = bad || !frame_finite(w1);
+            double u[2] = {w1.w[0], w1.w[1]};
+            double q1[2] = {0.0, 0.0};
+            if (role == PR_B2) {
+              q1[0] = ringQ[slot][r][0][lane];
+              q1[1] = ringQ[slot][r][1][lane];
+            }
+            double r01;
+            int passes;
+            int8_t status;
+            const double gamma = pipe_gs_column<ALG>(col, u, q1, &r01, &passes, &status);
+            bad = bad || status != QR_OK;
+            const double qa = u[0] / gamma, qb = u[1] / gamma;
+            ringQ[slot][r][2 * col][lane] = qa;
+            ringQ[slot][r][2 * col + 1][lane] = qb;
+            // R column-major: column 1 is (gamma, 0), column 2 (r01, gamma)
+            ringR[slot][r][2 * col][lane] = col == 0 ? gamma : r01;
+            ringR[slot][r][2 * col + 1][lane] = col == 0 ? 0.0 : gamma;
+            W.w[2 * col] = qa;
+            W.w[2 * col + 1] = qb;
+            cmax = cmax > passes ? cmax : (int64_t)passes;
+          } else {
+            for (int t = 0; t < tau; ++t) {
+              const double c = cn;
+              ++si;
+              cn = ringC[slot][si < ns ? si : ns - 1][lane];
+              map_tangent_apply(c, W);
+            }
+            bad = bad || !frame_finite(W);
+            QRFact<D, K> f;
+            orthonormalize<ALG, D, K>(W, f);
+            bad = bad || f.status != QR_OK;
+#pragma unroll
+            for (int e = 0; e < K * K; ++e) ringR[slot][r][e][lane] = f.R[e];
+#pragma unroll
+            for (int e = 0; e < D * K; ++e) ringQ[slot][r][e][lane] = f.Q.w[e];
+            W = f.Q;
+            cmax = cmax > f.count ? cmax : (int64_t)f.count;
           }
-          bad = bad || !frame_finite(W);
-          QRFact<D, K> f;
-          orthonormalize<ALG, D, K>(W, f);
-          bad = bad || f.status != QR_OK;
-#pragma unroll
-          for (int e = 0; e < K * K; ++e) ringR[slot][r][e][lane] = f.R[e];
-#pragma unroll
-          for (int e = 0; e < D * K; ++e) ringQ[slot][r][e][lane] = f.Q.w[e];
-          W = f.Q;
-          st.count_max = st.count_max > f.count ? st.count_max : f.count;
         }
       }
-    } else if (role <= 3) {
-      const int64_t q = p - 2;
+    } else if (role == PR_C0 || role == PR_C1) {
+      const int64_t q = p - LAG;
       if (q >= 0 && q < nc) {
-        const int slot = (int)(q & 1);
+        const int slot = (int)(q % PIPE_SLOTS);
         const int64_t h0 = q * H, h1 = (h0 + H < nh) ? h0 + H : nh;
-        for (int64_t h = h0 + ((h0 + role) & 1); h < h1; h += 2) {  // role 2: even h, role 3: odd h
-          const int r = (int)(h - h0);
-          double R[K * K];
+        const int par = (int)((h0 + (role == PR_C0 ? 0 : 1)) & 1);  // C0: even h, C1: odd h
+        double gw[PIPE_HAND / 2], ol[PIPE_HAND / 2];
 #pragma unroll
-          for (int e = 0; e < K * K; ++e) R[e] = ringR[slot][r][e][lane];
-          {
+        for (int u = 0; u < PIPE_HAND / 2; ++u) {
+          const int r = par + 2 * u;
+          gw[u] = 0.0;
+          ol[u] = 0.0;
+          if (h0 + r < h1) {
+            double R[K * K];
+#pragma unroll
+            for (int e = 0; e < K * K; ++e) R[e] = ringR[slot][r][e][lane];
             Frame<D, K> Q;
 #pragma unroll
             for (int e = 0; e < D * K; ++e) Q.w[e] = ringQ[slot][r][e][lane];
-            const double Gw = window_Gw<K>(R);
+            gw[u] = window_Gw<K>(R);
+            ol[u] = K >= 2 ? orth_loss<D, K>(Q) : 0.0;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < PIPE_HAND / 2; ++u) {
+          if (h0 + par + 2 * u < h1) {
+            const double Gw = gw[u], orth = ol[u];
             if (QRSpectrum<ALG, D, K>::checks_conditioning && !(Gw * CL_EPS64 < 0.5)) bad = true;
-            const double orth = K >= 2 ? orth_loss<D, K>(Q) : 0.0;
             const double g = (ALG == QR_CGS || ALG == QR_MGS) ? Gw : 1.0;
             const double thr = qr.tol * CL_EPS64 * (double)K * g;
             if (K >= 2 && !(orth <= thr)) bad = true;
@@ -167,18 +282,22 @@ pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
           }
         }
       }
-    } else if (role <= 5) {
-      const int col = role - 4;
-      const int64_t q = p - 2;
+    } else if (role == PR_D0 || role == PR_D1) {
+      const int col = role - PR_D0;
+      const int64_t q = p - LAG;
       if (col < K && q >= 0 && q < nc) {
-        const int slot = (int)(q & 1);
+        const int slot = (int)(q % PIPE_SLOTS);
         const int64_t h0 = q * H, h1 = (h0 + H < nh) ? h0 + H : nh;
+        const int nr = (int)(h1 - h0);
+        // the logs of the chunk's handoffs do not depend on each other: form them all, then fold in step order
+        double lg[PIPE_HAND];
+#pragma unroll
+        for (int r = 0; r < PIPE_HAND; ++r) lg[r] = r < nr ? cl_log(ringR[slot][r][col * K + col][lane]) : 0.0;
         // the column is uniform over the wave; selects keep the state out of indexed (scratch) memory
         CSum sm = col == 0 ? st.sums[0] : st.sums[K - 1];
-        for (int64_t h = h0; h < h1; ++h) {
-          const int r = (int)(h - h0);
-          sm = csum_add(sm, cl_log(ringR[slot][r][col * K + col][lane]));
-        }
+#pragma unroll
+        for (int r = 0; r < PIPE_HAND; ++r)
+          if (r < nr) sm = csum_add(sm, lg[r]);
         if (col == 0) st.sums[0] = sm;
         else st.sums[K - 1] = sm;
         st.n = a.n0 + h1 * tau;
@@ -186,54 +305,64 @@ pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
     } else if constexpr (MON) {
       const int64_t q = p - 1;
       if (q >= 0 && q < nc) {
-        const int slot = (int)(q & 1);
         const int64_t h0 = q * H, h1 = (h0 + H < nh) ? h0 + H : nh;
         for (int64_t h = h0; h < h1; ++h) {
           double J[4];
-          map_jacobian(ringM[slot][(int)(h - h0)][lane], J);
+          map_jacobian(ringM[(int)(q & 1)][(int)(h - h0)][lane], J);
           mon.step(mst, a.n0 + (h + 1) * tau, J);
         }
       }
     }
     __syncthreads();
   }
-  bad_by[role][lane] = bad ? 1 : 0;
-  if (role == 1) outCount[lane] = st.count_max;
-  if (role == 2 || role == 3) {
-    outG[role - 2][lane] = st.G_max;
-    outO[role - 2][lane] = st.orth_max;
+  bad_by[wave][lane] = bad ? 1 : 0;
+  if (role == PR_B) outCount[0][lane] = cmax;
+  if (role == PR_B2) outCount[1][lane] = cmax;
+  if (role == PR_C0 || role == PR_C1) {
+    outG[role - PR_C0][lane] = st.G_max;
+    outO[role - PR_C0][lane] = st.orth_max;
   }
-  if (role == 5 && K > 1) outS[1][lane] = st.sums[K - 1];
+  if (role == PR_D1 && K > 1) outS[1][lane] = st.sums[K - 1];
   __syncthreads();
   bool failed = false;
 #pragma unroll
   for (int w = 0; w < NW; ++w) failed = failed || bad_by[w][lane] != 0;
-  if (role == 0 && in_range) rerun[i] = (live && failed) ? 1 : 0;
+  // the count is the maximum over both columns (B2 holds column 2's reorthogonalisation passes)
+  int64_t cnt = outCount[0][lane];
+  if (SPLIT) cnt = cnt > outCount[1][lane] ? cnt : outCount[1][lane];
+  if (role == PR_A && in_range) rerun[i] = (live && failed) ? 1 : 0;
   if (!live) return;
   const bool keep = !failed && nh > 0;
-  if (role == 0) {
+  if (role == PR_A) {
     if (keep) {
 #pragma unroll
       for (int j = 0; j < Sys::S; ++j) a.x[(int64_t)j * a.m + i] = x[j];
     }
-  } else if (role == 1) {
+  } else if (role == PR_B2) {
     if (keep) {
-#pragma unroll
-      for (int e = 0; e < D * K; ++e) a.W0[(int64_t)e * a.m + i] = W.w[e];
+      a.W0[(int64_t)2 * a.m + i] = W.w[2];
+      a.W0[(int64_t)3 * a.m + i] = W.w[3];
     }
-  } else if (role == 4) {
+  } else if (role == PR_B) {
+    if (keep) {
+      const int ne = SPLIT ? 2 : D * K;
+#pragma unroll
+      for (int e = 0; e < D * K; ++e)
+        if (e < ne) a.W0[(int64_t)e * a.m + i] = W.w[e];
+    }
+  } else if (role == PR_D0) {
     if (keep) {
       // the maxima over the even and the odd handoffs, each started from the launch's input value
       st.G_max = cl_jmax(outG[0][lane], outG[1][lane]);
       st.orth_max = cl_jmax(outO[0][lane], outO[1][lane]);
       if (K > 1) st.sums[K - 1] = outS[1][lane];
-      st.count_max = outCount[lane];
+      st.count_max = cnt;
       st.column = 0;
       state_store(st, a.st, a.m, i, 0);
       a.code[i] = STATUS_OK;
       a.hstep[i] = a.n0 + nh * tau;
     }
-  } else if (MON && role == 6) {
+  } else if (MON && role == PR_E) {
     if (keep) state_store(mst, a.st, a.m, i, WMON);
   }
 }
