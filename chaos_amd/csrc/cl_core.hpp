@@ -612,6 +612,23 @@ CL_HD double window_Gw(const double* R0) {
   return cl_sqrt(a) * cl_sqrt(b);
 }
 
+// A one-division estimate of window_Gw<2> for a 2 x 2 upper-triangular R: ||R^-1||_F = ||R||_F / |R11 R22|, so
+// G_w = ||R||_F^2 / |R11 R22| in exact arithmetic. With the entries in [1e-100, 1e100] (no rescaling, no
+// overflow and no underflow that moves either result by more than a unit in the last place) window_Gw's
+// rounding errors stay below 10 units in the last place and this one's below 5, so the two differ by less than
+// GW_QUICK_REL times the estimate. Returns -1 outside that range (and for non-finite entries).
+constexpr double GW_QUICK_REL = 1e-13;
+// an estimate below this settles the conditioning check (window_Gw * eps < 0.5, i.e. G_w < 2^51 = 2.25e15)
+constexpr double GW_QUICK_GATE = 2.0e15;
+CL_HD double gw_quick2(const double* R) {
+  const double r0 = cl_fabs(R[0]), r2 = cl_fabs(R[2]), r3 = cl_fabs(R[3]);
+  const bool ok = r0 >= 1e-100 && r0 <= 1e100 && r3 >= 1e-100 && r3 <= 1e100 && r2 <= 1e100 && R[1] == 0.0;
+  const double a = (R[0] * R[0] + R[2] * R[2]) + R[3] * R[3];
+  return ok ? a / (r0 * r3) : -1.0;
+}
+
+// G_from_Gw is non-decreasing in G_w >= 0 (every operation is, and rounding keeps order), so the largest G over
+// a run of handoffs is G of the largest G_w, bit for bit.
 template <int K>
 CL_HD double G_from_Gw(double Gw) {
   if (K == 1) return 1.0;

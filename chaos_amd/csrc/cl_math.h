@@ -289,16 +289,10 @@ CL_HD double cl_cos(double x) {
 }
 
 /* ---------------------------------------------------------------- log */
-CL_HD double cl_log(double x) {
-  if (cl_isnan(x)) return x;
-  if (x < 0.0) return __builtin_nan("");
-  if (x == 0.0) return -__builtin_inf();
-  if (!cl_isfinite(x)) return x;
-  int k = 0;
-  if (x < 0x1p-1022) { /* subnormal */
-    x = x * 0x1p54;
-    k = -54;
-  }
+/* log(x 2^k0) for a positive normal finite x: the arithmetic of cl_log after its special cases, with no branch, so
+ * that independent logs interleave. */
+CL_HD double cl_log_normal(double x, int k0) {
+  int k = k0;
   uint64_t b = cl_bits(x);
   k += (int)(b >> 52) - 1023;
   uint64_t mant = b & 0x000fffffffffffffull;
@@ -318,6 +312,19 @@ CL_HD double cl_log(double x) {
   double lo = dk * CL_LN2_LO + s * (hfsq + R);
   return dk * CL_LN2_HI + (f - (hfsq - lo));
 }
+
+CL_HD double cl_log(double x) {
+  if (cl_isnan(x)) return x;
+  if (x < 0.0) return __builtin_nan("");
+  if (x == 0.0) return -__builtin_inf();
+  if (!cl_isfinite(x)) return x;
+  if (x < 0x1p-1022) /* subnormal */
+    return cl_log_normal(x * 0x1p54, -54);
+  return cl_log_normal(x, 0);
+}
+
+/* x is positive, normal and finite (cl_log_normal's domain) */
+CL_HD int cl_is_pos_normal(double x) { return x >= 0x1p-1022 && x <= 0x1.fffffffffffffp+1023; }
 
 /* exp(x): k = rint(x / ln 2), r = (x - k LN2_HI) - k LN2_LO (k LN2_HI is exact: |k| < 2^11 and LN2_HI has 42
  * significant bits), |r| <= ln 2 / 2 plus rounding; exp(r) = 1 + (r + r^2 E(r)) with E the Taylor polynomial

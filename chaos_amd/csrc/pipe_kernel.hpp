@@ -11,8 +11,10 @@
 //      column, so B splits into B1, column 1, and B2, column 2, which projects on the q_1 that B1 formed one
 //      chunk earlier: two step-to-step chains of half the length instead of one.
 //   C  the checks of the factorisation: window_Gw conditioning and orth_loss (qr_spectrum.jl:54-73),
-//      and the running maxima G_max, orth_max
-//   D  the exponent sums: add(sums[i], log R_ii)
+//      and the running maxima G_max, orth_max. At k = 2 a one-division estimate of G_w (gw_quick2) settles the
+//      checks and the maximum for almost every handoff; window_Gw runs only where the estimate's error bound
+//      leaves the outcome open, so every decision and every stored value is window_Gw's
+//   D  the exponent sums: add(sums[i], log R_ii), for diagonals that are positive normal numbers
 //   E  (chains QRSpectrum + RunMonitor, the recorder's) the monitor's running maxima of the one-step tangent's
 //      det and symplecticity residuals and of max |M| at each handoff point, from c at that point
 //
@@ -163,6 +165,9 @@ pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
   const int64_t nc = (nh + H - 1) / H;
   bool bad = false;
   int64_t cmax = st.count_max;
+  constexpr bool GS_THR = ALG == QR_CGS || ALG == QR_MGS;  // orth_loss's threshold scales with G_w
+  const double c0 = qr.tol * CL_EPS64 * (double)K;
+  double gw_max = 0.0;  // C: the largest G_w of its handoffs (G_w >= 0)
   for (int64_t p = 0; p < nc + LAG; ++p) {
     if (role == PR_A) {
       if (p < nc) {
@@ -252,6 +257,15 @@ pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
         const int slot = (int)(q % PIPE_SLOTS);
         const int64_t h0 = q * H, h1 = (h0 + H < nh) ? h0 + H : nh;
         const int par = (int)((h0 + (role == PR_C0 ? 0 : 1)) & 1);  // C0: even h, C1: odd h
+        // the checks as QRSpectrum::step makes them, from window_Gw; gw_max collects G_w (the largest G is
+        // G_from_Gw of the largest G_w)
+        auto exact = [&](double Gw, double orth) {
+          if (QRSpectrum<ALG, D, K>::checks_conditioning && !(Gw * CL_EPS64 < 0.5)) bad = true;
+          const double g = GS_THR ? Gw : 1.0;
+          const double thr = qr.tol * CL_EPS64 * (double)K * g;
+          if (K >= 2 && !(orth <= thr)) bad = true;
+          gw_max = cl_jmax(gw_max, Gw);
+        };
         double gw[PIPE_HAND / 2], ol[PIPE_HAND / 2];
 #pragma unroll
         for (int u = 0; u < PIPE_HAND / 2; ++u) {
@@ -265,20 +279,31 @@ pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
             Frame<D, K> Q;
 #pragma unroll
             for (int e = 0; e < D * K; ++e) Q.w[e] = ringQ[slot][r][e][lane];
-            gw[u] = window_Gw<K>(R);
+            gw[u] = K == 2 ? gw_quick2(R) : window_Gw<K>(R);
             ol[u] = K >= 2 ? orth_loss<D, K>(Q) : 0.0;
           }
         }
 #pragma unroll
         for (int u = 0; u < PIPE_HAND / 2; ++u) {
-          if (h0 + par + 2 * u < h1) {
-            const double Gw = gw[u], orth = ol[u];
-            if (QRSpectrum<ALG, D, K>::checks_conditioning && !(Gw * CL_EPS64 < 0.5)) bad = true;
-            const double g = (ALG == QR_CGS || ALG == QR_MGS) ? Gw : 1.0;
-            const double thr = qr.tol * CL_EPS64 * (double)K * g;
-            if (K >= 2 && !(orth <= thr)) bad = true;
-            st.G_max = cl_jmax(st.G_max, G_from_Gw<K>(Gw));
-            st.orth_max = cl_jmax(st.orth_max, orth);
+          const int r = par + 2 * u;
+          if (h0 + r < h1) {
+            if constexpr (K == 2) {
+              // window_Gw only where the estimate (within GW_QUICK_REL of it) cannot settle a check or might
+              // raise the running maximum: a new maximum, G_w near 2^52, orth_loss near its threshold
+              const double g = gw[u];
+              bool sure = g >= 0.0 && g < GW_QUICK_GATE && g * (1.0 + GW_QUICK_REL) < gw_max;
+              sure = sure && ol[u] <= (GS_THR ? c0 * g * (1.0 - 2.0 * GW_QUICK_REL) : c0 * 1.0);
+              if (!sure) {
+                __asm__ volatile("");  // keeps the rare branch a branch
+                double R[K * K];
+#pragma unroll
+                for (int e = 0; e < K * K; ++e) R[e] = ringR[slot][r][e][lane];
+                exact(window_Gw<K>(R), ol[u]);
+              }
+            } else {
+              exact(gw[u], ol[u]);
+            }
+            st.orth_max = cl_jmax(st.orth_max, ol[u]);
           }
         }
       }
@@ -289,10 +314,17 @@ pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
         const int slot = (int)(q % PIPE_SLOTS);
         const int64_t h0 = q * H, h1 = (h0 + H < nh) ? h0 + H : nh;
         const int nr = (int)(h1 - h0);
-        // the logs of the chunk's handoffs do not depend on each other: form them all, then fold in step order
+        // the logs of the chunk's handoffs do not depend on each other: form them all, then fold in step order.
+        // cl_log_normal is cl_log's arithmetic without its branches, so the logs interleave; a diagonal outside
+        // its domain (zero, negative, subnormal, not finite) sends the orbit to the rerun
         double lg[PIPE_HAND];
 #pragma unroll
-        for (int r = 0; r < PIPE_HAND; ++r) lg[r] = r < nr ? cl_log(ringR[slot][r][col * K + col][lane]) : 0.0;
+        for (int r = 0; r < PIPE_HAND; ++r) {
+          const double v = ringR[slot][r][col * K + col][lane];
+          const bool ok = r >= nr || cl_is_pos_normal(v);
+          bad = bad || !ok;
+          lg[r] = cl_log_normal(r < nr && ok ? v : 1.0, 0);
+        }
         // the column is uniform over the wave; selects keep the state out of indexed (scratch) memory
         CSum sm = col == 0 ? st.sums[0] : st.sums[K - 1];
 #pragma unroll
@@ -319,7 +351,7 @@ pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
   if (role == PR_B) outCount[0][lane] = cmax;
   if (role == PR_B2) outCount[1][lane] = cmax;
   if (role == PR_C0 || role == PR_C1) {
-    outG[role - PR_C0][lane] = st.G_max;
+    outG[role - PR_C0][lane] = gw_max;
     outO[role - PR_C0][lane] = st.orth_max;
   }
   if (role == PR_D1 && K > 1) outS[1][lane] = st.sums[K - 1];
@@ -352,8 +384,9 @@ pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
     }
   } else if (role == PR_D0) {
     if (keep) {
-      // the maxima over the even and the odd handoffs, each started from the launch's input value
-      st.G_max = cl_jmax(outG[0][lane], outG[1][lane]);
+      // the maxima over the even and the odd handoffs, each started from the launch's input value; G_max from
+      // the largest G_w (nh > 0 handoffs, each with G_w > 0)
+      st.G_max = cl_jmax(st.G_max, G_from_Gw<K>(cl_jmax(outG[0][lane], outG[1][lane])));
       st.orth_max = cl_jmax(outO[0][lane], outO[1][lane]);
       if (K > 1) st.sums[K - 1] = outS[1][lane];
       st.count_max = cnt;

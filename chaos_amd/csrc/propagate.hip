@@ -1047,6 +1047,13 @@ extern "C" void cl_math_host(int64_t n, const double* x, double* sn, double* cs,
   }
 }
 
+extern "C" void cl_gw_host(int64_t n, const double* R, double* quick, double* exact) {
+  for (int64_t i = 0; i < n; ++i) {
+    quick[i] = gw_quick2(R + 4 * i);
+    exact[i] = window_Gw<2>(R + 4 * i);
+  }
+}
+
 extern "C" int cl_math_device(cl_ctx* ctx, int64_t n, const double* dx, double* dsn, double* dcs, double* dlg,
                               void* stream) {
   if (!ctx || n < 0) return CL_ERR_ARGUMENT;

@@ -261,6 +261,10 @@ int cl_initial_deviations_device(cl_ctx *ctx, uint64_t seed, int64_t orbit0, int
 void cl_math_host(int64_t n, const double *x, double *sn, double *cs, double *lg);
 int cl_math_device(cl_ctx *ctx, int64_t n, const double *dx, double *dsn, double *dcs, double *dlg,
                    void *stream);
+/* G_w = ||R||_F ||R^-1||_F of n 2 x 2 upper-triangular factors R (column-major, 4 entries each) as the kernels
+ * form it on the host: window_Gw (orthonormalize.jl:112-127) and the one-division estimate the pipeline's
+ * checks use to skip it (-1 outside its range), for the test of the estimate's error bound. */
+void cl_gw_host(int64_t n, const double *R, double *quick, double *exact);
 
 /* ------------------------------------------------------------------ surrogate network kernels
  * No reference counterpart: the learned triage of the ensemble (a dense network on the prefix trace).

@@ -2115,6 +2115,12 @@ void or_wb_weights(int64_t N, int order, double* w, int math) {
     w[jj] = math == MATH_CL ? wb_w<F64Cl, double>(jj, N, order) : wb_w<F64Libm, double>(jj, N, order);
 }
 
+// window_gw of n 2 x 2 upper-triangular factors (column-major, 4 entries each), for the test of the kernels' G_w
+// estimate.
+void or_window_gw2(int64_t n, const double* R, double* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = window_gw<F64Cl, double>(R + 4 * i, 2);
+}
+
 // The portable exp of the kernels (cl_math.h), for its accuracy test against libm.
 void or_exp(int64_t n, const double* x, double* out) {
   for (int64_t i = 0; i < n; ++i) out[i] = cl_exp(x[i]);

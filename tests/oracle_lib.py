@@ -55,6 +55,8 @@ def load():
         o.or_wb_weights.argtypes = [I64, I, P, I]
         o.or_exp.restype = None
         o.or_exp.argtypes = [I64, P, P]
+        o.or_window_gw2.restype = None
+        o.or_window_gw2.argtypes = [I64, P, P]
         o.or_csum.restype = None
         o.or_csum.argtypes = [I64, ctypes.c_double, P, P]
         o.or_csum_list.restype = None
@@ -253,4 +255,12 @@ def exp(x):
     x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
     out = np.empty_like(x)
     load().or_exp(x.size, x.ctypes.data, out.ctypes.data)
+    return out
+
+
+def window_gw2(R):
+    """window_gw of 2 x 2 upper-triangular factors R[n, 4] (column-major)."""
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    out = np.empty(R.shape[0])
+    load().or_window_gw2(R.shape[0], R.ctypes.data, out.ctypes.data)
     return out
