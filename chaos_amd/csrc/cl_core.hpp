@@ -1266,6 +1266,16 @@ struct WBParams {
   int64_t nmax2;  // 2 max(window): handoffs of a complete run
   int hw;         // words of the horizons' sums
 };
+// Makes v computed at this point: the compiler then cannot sink its computation into the conditional block that
+// uses it, where independent computations would run one after another instead of interleaving.
+CL_HD void cl_pin(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __asm__ volatile("" ::"v"(v));
+#else
+  (void)v;
+#endif
+}
+
 // x[a] of a point of S coordinates without indexing the register array at run time
 template <int S>
 CL_HD double wb_pick(const double* x, int a) {

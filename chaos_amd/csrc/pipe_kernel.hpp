@@ -55,13 +55,13 @@ constexpr int pipe_waves() {
   return 6 + (pipe_split<ALG, K>() ? 1 : 0) + (MON ? 1 : 0);
 }
 
-// roles, and their waves: wave w runs on SIMD w mod 4, so A and the B waves get a SIMD each, shared with a light
-// part (C, D or E)
+// roles, and their waves: wave w runs on SIMD w mod 4. Split: A has a SIMD of its own, the two C waves share
+// one, B1 and B2 each share theirs with a D wave (and A with E in the recorder's chain).
 enum : int { PR_A = 0, PR_B = 1, PR_C0 = 2, PR_C1 = 3, PR_D0 = 4, PR_D1 = 5, PR_E = 6, PR_B2 = 7 };
 template <bool SPLIT>
 CL_HD int pipe_role(int wave) {
-  if (SPLIT)  // waves 0..7: A, B1, B2, C odd, C even, D col 1, D col 2, E
-    return wave == 0 ? PR_A : wave == 1 ? PR_B : wave == 2 ? PR_B2 : wave == 3 ? PR_C1 : wave == 4 ? PR_C0
+  if (SPLIT)  // waves 0..7: C even, B1, B2, A, C odd, D col 1, D col 2, E
+    return wave == 0 ? PR_C0 : wave == 1 ? PR_B : wave == 2 ? PR_B2 : wave == 3 ? PR_A : wave == 4 ? PR_C1
          : wave == 5 ? PR_D0 : wave == 6 ? PR_D1 : PR_E;
   // waves 0..6: A, B, C even, C odd, D col 2, D col 1, E
   return wave == 0 ? PR_A : wave == 1 ? PR_B : wave == 2 ? PR_C0 : wave == 3 ? PR_C1 : wave == 4 ? PR_D1
@@ -268,20 +268,23 @@ pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
         };
         double gw[PIPE_HAND / 2], ol[PIPE_HAND / 2];
 #pragma unroll
+        // for every entry of the slot (entries past the chunk's end hold an earlier chunk's factors and go unused),
+        // as straight-line code
         for (int u = 0; u < PIPE_HAND / 2; ++u) {
           const int r = par + 2 * u;
-          gw[u] = 0.0;
-          ol[u] = 0.0;
-          if (h0 + r < h1) {
-            double R[K * K];
+          double R[K * K];
 #pragma unroll
-            for (int e = 0; e < K * K; ++e) R[e] = ringR[slot][r][e][lane];
-            Frame<D, K> Q;
+          for (int e = 0; e < K * K; ++e) R[e] = ringR[slot][r][e][lane];
+          Frame<D, K> Q;
 #pragma unroll
-            for (int e = 0; e < D * K; ++e) Q.w[e] = ringQ[slot][r][e][lane];
-            gw[u] = K == 2 ? gw_quick2(R) : window_Gw<K>(R);
-            ol[u] = K >= 2 ? orth_loss<D, K>(Q) : 0.0;
-          }
+          for (int e = 0; e < D * K; ++e) Q.w[e] = ringQ[slot][r][e][lane];
+          gw[u] = K == 2 ? gw_quick2(R) : window_Gw<K>(R);
+          ol[u] = K >= 2 ? orth_loss<D, K>(Q) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < PIPE_HAND / 2; ++u) {
+          cl_pin(gw[u]);
+          cl_pin(ol[u]);
         }
 #pragma unroll
         for (int u = 0; u < PIPE_HAND / 2; ++u) {
@@ -325,6 +328,9 @@ pipe_kernel(KArgs a, Sys sys, QRSpectrum<ALG, 2, K> qr, int H, uint8_t* rerun) {
           bad = bad || !ok;
           lg[r] = cl_log_normal(r < nr && ok ? v : 1.0, 0);
         }
+        // computed here, not sunk into the conditional folds below (where they would run one after another)
+#pragma unroll
+        for (int r = 0; r < PIPE_HAND; ++r) cl_pin(lg[r]);
         // the column is uniform over the wave; selects keep the state out of indexed (scratch) memory
         CSum sm = col == 0 ? st.sums[0] : st.sums[K - 1];
 #pragma unroll
