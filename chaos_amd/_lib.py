@@ -96,7 +96,7 @@ SIGNATURES = {
     "cl_initial_deviations": (I, [ctypes.c_uint64, I64, I64, I, I, P]),
     "cl_initial_deviations_device": (I, [P, ctypes.c_uint64, I64, I64, I, I, P, P]),
     "cl_math_host": (None, [I64, P, P, P, P]),
-    "cl_gw_host": (None, [I64, P, P, P]),
+    "cl_gw_host": (None, [I64, P, P, P, P, P]),
     "cl_math_device": (I, [P, I64, P, P, P, P, P]),
     "cl_mlp_gemm": (I, [P, I, I, I, I, I, P, I, P, I, P, I, I, P, I, P, I, P]),
     "cl_mlp_rowsum": (I, [P, I, I, P, I, P, P]),

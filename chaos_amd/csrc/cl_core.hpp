@@ -682,6 +682,26 @@ struct QRSpectrum {
     st.threshold = threshold;
     st.step = n;
   }
+  // At k = 2, true when the one-division estimate of G_w (gw_quick2, within GW_QUICK_REL of window_Gw) settles
+  // the handoff without window_Gw: the conditioning check passes (estimate below GW_QUICK_GATE), orth_loss is
+  // below its G_w-scaled threshold by more than the estimate's error (CGS, MGS; the other algorithms' threshold
+  // does not use G_w), and G of this handoff cannot exceed G_max: in exact arithmetic G(G_w) <= G_max exactly
+  // when G_w <= G_max + 1/G_max, tested as G_w G_max < G_max^2 + 1 with a 2e-12 margin (G_max in [2, 1e150], so
+  // G_max^2 stays finite). G is concave with slope G / sqrt(G_w^2 - 4), so that margin keeps G(G_w) at least
+  // 2e-12 G_max below G_max; G's rounding is below 5 units in the last place from G_w = 2.5 on, and below G_w
+  // it cannot reach G_max >= 2. The running maximum then keeps its value, as jmax would.
+  CL_HD bool settled_by_estimate(const State& st, const QRFact<D, K>& f, double orth) const {
+    if constexpr (K != 2) {
+      return false;
+    } else {
+      const double q = gw_quick2(f.R);
+      const double Gm = st.G_max;
+      bool ok = q >= 0.0 && Gm >= 2.0 && Gm < 1e150 && q * Gm * (1.0 + 2e-12) < Gm * Gm + 1.0;
+      if (checks_conditioning) ok = ok && q < GW_QUICK_GATE;
+      if (ALG == QR_CGS || ALG == QR_MGS) ok = ok && orth <= tol * CL_EPS64 * 2.0 * q * (1.0 - 2.0 * GW_QUICK_REL);
+      return ok;
+    }
+  }
   CL_HD void step(State& st, int64_t n, Frame<D, K>& W) const {
     if (st.code != 0) return;
     QRFact<D, K> f;
@@ -698,23 +718,31 @@ struct QRSpectrum {
       }
       return;
     }
-    double Gw = window_Gw<K>(f.R);
-    if (checks_conditioning && !(Gw * CL_EPS64 < 0.5)) {
-      fail(st, 3, 0, Gw * CL_EPS64, 0.5, n);
-      return;
-    }
-    double orth = K >= 2 ? orth_loss<D, K>(f.Q) : 0.0;
-    double g = (ALG == QR_CGS || ALG == QR_MGS) ? Gw : 1.0;
-    double thr = tol * CL_EPS64 * (double)K * g;
-    if (K >= 2 && !(orth <= thr)) {
-      fail(st, 4, 0, orth, thr, n);
-      return;
+    constexpr bool GS_THR = ALG == QR_CGS || ALG == QR_MGS;  // orth_loss's threshold scales with G_w
+    const double orth = K >= 2 ? orth_loss<D, K>(f.Q) : 0.0;
+    if (!settled_by_estimate(st, f, orth)) {
+      double Gw = window_Gw<K>(f.R);
+      if (checks_conditioning && !(Gw * CL_EPS64 < 0.5)) {
+        fail(st, 3, 0, Gw * CL_EPS64, 0.5, n);
+        return;
+      }
+      double g = GS_THR ? Gw : 1.0;
+      double thr = tol * CL_EPS64 * (double)K * g;
+      if (K >= 2 && !(orth <= thr)) {
+        fail(st, 4, 0, orth, thr, n);
+        return;
+      }
+      st.G_max = cl_jmax(st.G_max, G_from_Gw<K>(Gw));
+    } else if (!GS_THR) {
+      const double thr = tol * CL_EPS64 * (double)K * 1.0;  // the threshold without G_w
+      if (!(orth <= thr)) {
+        fail(st, 4, 0, orth, thr, n);
+        return;
+      }
     }
 #pragma unroll
     for (int i = 0; i < K; ++i) st.sums[i] = csum_add(st.sums[i], cl_log(f.R[i * K + i]));
-    double G = G_from_Gw<K>(Gw);
     st.n = n;
-    st.G_max = cl_jmax(st.G_max, G);
     st.orth_max = cl_jmax(st.orth_max, orth);
     st.count_max = st.count_max > f.count ? st.count_max : f.count;
     st.column = 0;

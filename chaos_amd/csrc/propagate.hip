@@ -1047,10 +1047,19 @@ extern "C" void cl_math_host(int64_t n, const double* x, double* sn, double* cs,
   }
 }
 
-extern "C" void cl_gw_host(int64_t n, const double* R, double* quick, double* exact) {
+extern "C" void cl_gw_host(int64_t n, const double* R, const double* G_max, double* quick, double* exact,
+                           uint8_t* settled) {
+  QRSpectrum<QR_HOUSEHOLDER, 2, 2> qr;
+  qr.tol = 8.0;
   for (int64_t i = 0; i < n; ++i) {
     quick[i] = gw_quick2(R + 4 * i);
     exact[i] = window_Gw<2>(R + 4 * i);
+    QRState<2> st;
+    qr.init(st);
+    st.G_max = G_max[i];
+    QRFact<2, 2> f;
+    for (int e = 0; e < 4; ++e) f.R[e] = R[4 * i + e];
+    settled[i] = qr.settled_by_estimate(st, f, 0.0) ? 1 : 0;
   }
 }
 

@@ -262,9 +262,11 @@ void cl_math_host(int64_t n, const double *x, double *sn, double *cs, double *lg
 int cl_math_device(cl_ctx *ctx, int64_t n, const double *dx, double *dsn, double *dcs, double *dlg,
                    void *stream);
 /* G_w = ||R||_F ||R^-1||_F of n 2 x 2 upper-triangular factors R (column-major, 4 entries each) as the kernels
- * form it on the host: window_Gw (orthonormalize.jl:112-127) and the one-division estimate the pipeline's
- * checks use to skip it (-1 outside its range), for the test of the estimate's error bound. */
-void cl_gw_host(int64_t n, const double *R, double *quick, double *exact);
+ * form it on the host: window_Gw (orthonormalize.jl:112-127) and the one-division estimate the kernels' checks
+ * use to skip it (-1 outside its range); settled[i] is the fused QR step's decision (Householder, orth_loss 0)
+ * that the estimate settles the handoff with G_max[i] as the running maximum. For the tests of the estimate's
+ * error bound and of that decision. */
+void cl_gw_host(int64_t n, const double *R, const double *G_max, double *quick, double *exact, uint8_t *settled);
 
 /* ------------------------------------------------------------------ surrogate network kernels
  * No reference counterpart: the learned triage of the ensemble (a dense network on the prefix trace).

@@ -13,12 +13,20 @@ U = 2.0 ** -53
 REL = 1e-13
 
 
-def gw_host(R):
+def gw_host(R, G_max=None):
     R = np.ascontiguousarray(R, dtype=np.float64)
-    q = np.empty(R.shape[0])
-    e = np.empty(R.shape[0])
-    L.load().cl_gw_host(R.shape[0], R.ctypes.data, q.ctypes.data, e.ctypes.data)
-    return q, e
+    n = R.shape[0]
+    Gm = np.zeros(n) if G_max is None else np.ascontiguousarray(np.broadcast_to(G_max, (n,)), dtype=np.float64)
+    q = np.empty(n)
+    e = np.empty(n)
+    st = np.empty(n, dtype=np.uint8)
+    L.load().cl_gw_host(n, R.ctypes.data, Gm.ctypes.data, q.ctypes.data, e.ctypes.data, st.ctypes.data)
+    return (q, e) if G_max is None else (q, e, st.astype(bool))
+
+
+def G_of(gw):
+    """G from G_w at k = 2 in IEEE double (orthonormalize.jl:112-127: (G_w + sqrt(max(G_w^2 - 4, 0))) / 2)."""
+    return (gw + np.sqrt(np.maximum(gw * gw - 4.0, 0.0))) / 2.0
 
 
 def logu(rng, lo, hi, n):
@@ -88,3 +96,31 @@ def test_estimate_undefined_outside_range():
     # the range's ends are inside
     q, e = gw_host(np.array([[1e-100, 0.0, 1e100, 1e100], [1e100, 0.0, -1e100, 1e-100]]))
     assert np.all(q > 0.0) and np.all(np.abs(e - q) <= 64 * U * q)
+
+
+def test_fused_step_skip_never_raises_the_maximum():
+    """The fused QR step skips window_Gw when the estimate shows that G of the handoff cannot exceed the running
+    G_max. Adversarial maxima around G of the exact G_w (equal, one ulp either side, relative offsets from 1e-15
+    to 1e-9): wherever the step skips, G(window_Gw) <= G_max, so jmax would have kept G_max. Skips happen once
+    G_max clears G by the margin (2e-12), and never below G_max = 2 or above 1e150."""
+    rng = np.random.default_rng(5)
+    R = factors(3, 40_000)
+    _, e = gw_host(R)
+    keep = np.isfinite(e) & (e > 2.0) & (e < 1e140)
+    R = R[keep]
+    G = G_of(e[keep])
+    offs = np.array([0.0, -1e-15, 1e-15, -1e-13, 1e-13, 1e-12, 2e-12, 4e-12, 1e-11, 1e-9])
+    skipped_total = 0
+    for o in offs:
+        for Gm in (G * (1.0 + o), np.nextafter(G, np.inf), np.nextafter(G, -np.inf)):
+            q, e2, st = gw_host(R, Gm)
+            assert np.array_equal(e2, e[keep])
+            bad = st & (G > Gm)
+            assert not bad.any(), (o, np.flatnonzero(bad)[:5])
+            assert not (st & ((Gm < 2.0) | (Gm >= 1e150))).any()
+            skipped_total += int(st.sum())
+    # well clear of the margin the estimate settles every factor whose G_max is in range
+    Gm = G * (1.0 + 1e-10)
+    q, _, st = gw_host(R, Gm)
+    inrange = (q >= 0.0) & (Gm >= 2.0) & (Gm < 1e150)
+    assert st[inrange].all() and skipped_total > 0
