@@ -89,6 +89,9 @@ SIGNATURES = {
                                 P, P, P, P, P, P]),
     "cl_record_rows": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I64, I, I, I, I, P, P, P, P, P,
                            P]),
+    "cl_ind_record_rows": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I64, I, I, I, P, P, P, P, P,
+                               P]),
+    "cl_ind_row_width": (I, [ctypes.POINTER(cl_chain), I]),
     "cl_shadow_device": (I, [P, ctypes.POINTER(cl_system), I64, I, I64, I64, I64, I, ctypes.c_double, P, P, P, P]),
     "cl_propagate_shadow_chunk": (I, [P, ctypes.POINTER(cl_system), I64, I, I64, I64, I64, I, ctypes.c_double, P, P,
                                       P]),

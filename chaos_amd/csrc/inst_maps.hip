@@ -53,6 +53,13 @@ typedef Cons<S_<Benettin<2>>, Cons<S_<RunMon<2>>, Nil>> C1_rec_ben;
 // StretchingNumbers (a workspace reducer, k = 2) alone and as the indicator recorder's chain with RunMonitor
 typedef Cons<S_<Stretch<2>>, Nil> C_st;
 typedef Cons<S_<Stretch<2>>, Cons<S_<RunMon<2>>, Nil>> C_st_mon;
+// the indicator recorder's tangent kind: members then RunMonitor (src/Experiments/run_indicator_chain.jl:27-31; the
+// study's spec is FLI, MEGNO, SALI), and each member alone
+typedef Cons<S_<FLI<2>>, Cons<S_<MEGNO<2>>, Cons<S_<SALI<2>>, Cons<S_<RunMon<2>>, Nil>>>> C_ind_fms;
+typedef Cons<S_<MEGNO<2>>, Cons<S_<FLI<2>>, Cons<S_<SALI<2>>, Cons<S_<RunMon<2>>, Nil>>>> C_ind_mfs;
+typedef Cons<S_<FLI<2>>, Cons<S_<RunMon<2>>, Nil>> C_ind_f;
+typedef Cons<S_<MEGNO<2>>, Cons<S_<RunMon<2>>, Nil>> C_ind_m;
+typedef Cons<S_<SALI<2>>, Cons<S_<RunMon<2>>, Nil>> C_ind_s;
 // WeightedBirkhoff (k = 1) alone and after RunMonitor (the indicator recorder's window-kind order)
 typedef Cons<S_<WBirk>, Nil> C1_wb;
 typedef Cons<S_<RunMon<2>>, Cons<S_<WBirk>, Nil>> C1_mon_wb;
@@ -70,7 +77,9 @@ typedef Cons<S_<RunMon<2>>, Cons<S_<WBirk>, Nil>> C1_mon_wb;
   ADD1(SK, SYS, C1_rgs) ADD1(SK, SYS, C1_hh) ADD1(SK, SYS, C1_aug) ADD1(SK, SYS, C1_giv)        \
   ADD1(SK, SYS, C1_ofli) ADD1(SK, SYS, C1_mon) ADD2(SK, SYS, C_rec_cgs, 1) ADD2(SK, SYS, C_rec_mgs, 1)          \
   ADD2(SK, SYS, C_rec_rgs, 1) ADD2(SK, SYS, C_rec_hh, 1) ADD2(SK, SYS, C_rec_aug, 1) ADD1(SK, SYS, C1_rec_ben) \
-  ADD2(SK, SYS, C_st, 1) ADD2(SK, SYS, C_st_mon, 1) ADD1(SK, SYS, C1_wb) ADD1(SK, SYS, C1_mon_wb)
+  ADD2(SK, SYS, C_st, 1) ADD2(SK, SYS, C_st_mon, 1) ADD1(SK, SYS, C1_wb) ADD1(SK, SYS, C1_mon_wb)               \
+  ADD2(SK, SYS, C_ind_fms, 1) ADD2(SK, SYS, C_ind_mfs, 1) ADD2(SK, SYS, C_ind_f, 1) ADD2(SK, SYS, C_ind_m, 1)   \
+  ADD2(SK, SYS, C_ind_s, 1)
 
 // the surrogate workload on the per-orbit-K map: the traced prefix (MEGNO + FLI on frame 1, the QR spectrum
 // on frame 2), the continuation and the plain method (the QR spectrum alone)

@@ -619,10 +619,7 @@ static int launch_best(const Instance* in, const cl_system* sys, const cl_chain*
   return launch(in, sys, ch, m, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st);
 }
 
-namespace cl {
-int record_rows_launch(int64_t m, int s, int k, int benettin, int g, int G, const double* x, const uint64_t* st,
-                       const int8_t* code, int64_t* fail, double* R, hipStream_t stream);
-}
+#include "record.hpp"
 
 // Row g of the recorder's array after the launch that ends at grid step g (src/Experiments/run_device.jl:9-37).
 extern "C" int cl_record_rows(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int64_t m, int s, int k, int g,
@@ -641,6 +638,58 @@ extern "C" int cl_record_rows(cl_ctx* ctx, const cl_system* sys, const cl_chain*
   int rc = record_rows_launch(m, s, k, ch->r[0].kind == CL_RED_BENETTIN, g, G, dx, dst, dcode, dfail, dR,
                               stream ? (hipStream_t)stream : ctx->stream);
   return rc ? set_err(ctx, rc, "record kernel launch failed") : CL_OK;
+}
+
+// Row g of the indicator recorder's array (src/Experiments/run_indicator_device.jl:13-29,31-60,62-91): a series
+// kind's chain is its members (FLI, MEGNO, SALI on frame 1) then RunMonitor; a window kind's is RunMonitor then
+// one reader, whose row holds the monitor and the point only.
+extern "C" int cl_ind_record_rows(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int64_t m, int s, int g,
+                                  int G, const double* dx, const uint64_t* dst, const int8_t* dcode, int64_t* dfail,
+                                  double* dR, void* stream) {
+  if (!ctx || !sys || !ch) return CL_ERR_ARGUMENT;
+  const bool sys_ok = (sys->kind == CL_SYS_STANDARD_MAP && s == 2) || (sys->kind == CL_SYS_WRAPPED_STANDARD_MAP && s == 4);
+  if (!sys_ok) return set_err(ctx, CL_ERR_UNSUPPORTED, "the indicator recorder runs the standard map, lifted or wrapped");
+  IndRowDesc d;
+  memset(&d, 0, sizeof(d));
+  const bool window = ch->n >= 2 && ch->r[0].kind == CL_RED_RUNMONITOR;
+  if (window) {
+    d.n = 0;
+    d.wmon = 0;
+  } else {
+    if (ch->n < 2 || ch->n - 1 > IND_MAX_MEMBERS || ch->r[ch->n - 1].kind != CL_RED_RUNMONITOR)
+      return set_err(ctx, CL_ERR_UNSUPPORTED, "a series kind's chain is 1 to 4 members then RunMonitor");
+    int w = 0, cols = 0;
+    for (int j = 0; j + 1 < ch->n; ++j) {
+      const int kd = ch->r[j].kind;
+      if (ch->r[j].frame != 0) return set_err(ctx, CL_ERR_UNSUPPORTED, "members read frame 1");
+      d.kind[j] = kd;
+      d.off[j] = w;
+      if (kd == CL_RED_FLI) { w += (int)(sizeof(FLIState) / 8); cols += 2; }
+      else if (kd == CL_RED_MEGNO) { w += (int)(sizeof(MEGNOState) / 8); cols += 6; }
+      else if (kd == CL_RED_SALI) { w += (int)(sizeof(SALIState) / 8); cols += 2; }
+      else return set_err(ctx, CL_ERR_UNSUPPORTED, "series members are FLI, MEGNO and SALI");
+    }
+    d.n = ch->n - 1;
+    d.wmon = w;
+    d.ncol = cols;
+  }
+  d.ncol += 3 + s;
+  if (m < 0 || G < 1 || g < 0 || g >= G) return set_err(ctx, CL_ERR_ARGUMENT, "grid row out of range");
+  if (m > 0 && (!dx || !dst || !dcode || !dfail || !dR)) return set_err(ctx, CL_ERR_ARGUMENT, "null buffer");
+  hipSetDevice(ctx->device);
+  int rc = ind_rows_launch(m, s, d, g, G, dx, dst, dcode, dfail, dR, stream ? (hipStream_t)stream : ctx->stream);
+  return rc ? set_err(ctx, rc, "indicator row kernel launch failed") : CL_OK;
+}
+
+extern "C" int cl_ind_row_width(const cl_chain* ch, int s) {
+  if (!ch || ch->n < 1) return CL_ERR_ARGUMENT;
+  if (ch->r[0].kind == CL_RED_RUNMONITOR) return 3 + s;
+  int cols = 0;
+  for (int j = 0; j + 1 < ch->n; ++j) {
+    const int kd = ch->r[j].kind;
+    cols += kd == CL_RED_MEGNO ? 6 : (kd == CL_RED_FLI || kd == CL_RED_SALI) ? 2 : 0;
+  }
+  return cols + 3 + s;
 }
 
 static int chain_device(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m, int s, int d,

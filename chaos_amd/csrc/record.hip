@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include "cl_core.hpp"
 #include "../../include/chaoslab_hip.h"
+#include "record.hpp"
 
 namespace cl {
 
@@ -55,6 +56,65 @@ int record_rows_launch(int64_t m, int s, int k, int benettin, int g, int G, cons
   if (m == 0) return 0;
   hipLaunchKernelGGL(k_record_rows, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, m, s, k, benettin, g,
                      G, x, st, code, fail, R);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (e == hipErrorOutOfMemory ? CL_ERR_OOM : CL_ERR_DEVICE_LOST);
+}
+
+// The indicator recorder's rows (src/Experiments/run_indicator_device.jl:13-29 and _put!, run_indicator_record.jl:39-70):
+// the columns of each member's final (FLI: fli, fli1; MEGNO: Y, Ybar, Ybar20 of column 1 then 2; SALI: sali, L), then
+// det_max, symp_max and M_max of RunMonitor, then the point. A window kind has no members (monitor and point only).
+// R is orbit-major, R[(i G + g) ncol + col]; states SoA by word. `off` is each member's first state word.
+__global__ void k_ind_rows(int64_t m, int s, IndRowDesc d, int g, int G, const double* __restrict__ x,
+                           const uint64_t* __restrict__ st, const int8_t* __restrict__ code, int64_t* fail,
+                           double* R) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m || fail[i] != 0) return;
+  if (code[i] != STATUS_OK) {
+    fail[i] = g + 1;
+    return;
+  }
+  double* row = R + ((size_t)i * G + g) * d.ncol;
+  auto dword = [&](int w) { return __longlong_as_double((long long)st[(int64_t)w * m + i]); };
+  auto iword = [&](int w) { return (int64_t)st[(int64_t)w * m + i]; };
+  int col = 0;
+#pragma unroll
+  for (int j = 0; j < IND_MAX_MEMBERS; ++j) {
+    if (j >= d.n) break;
+    const int o = d.off[j];
+    if (d.kind[j] == CL_RED_FLI) {  // FLIState: s1, s2, f1, fB, value, n, ...  (fli.jl:91-94)
+      row[col++] = dword(o + 5);
+      row[col++] = dword(o + 4);
+    } else if (d.kind[j] == CL_RED_MEGNO) {  // MEGNOState: c1, c2 (s, R1, R2, Q, Q2), value, n, K (megno.jl:97-111)
+      const int64_t n = iword(o + 21), K = iword(o + 22);
+      for (int c = 0; c < 2; ++c) {
+        const int b = o + 10 * c;
+        double Y = 0.0, Yb = 0.0, Y20 = 0.0;
+        if (K != 0) {
+          const double tn = (double)n;
+          Y = 2.0 * (dword(b + 2) + dword(b + 3)) / tn;
+          Yb = (dword(b + 6) + dword(b + 7)) / (double)K;
+          Y20 = (double)(n / K) * (dword(b + 8) + dword(b + 9)) / (tn * tn * tn);
+        }
+        row[col++] = Y;
+        row[col++] = Yb;
+        row[col++] = Y20;
+      }
+    } else {  // SALIState: s1, s2, sali, value, n  (sali.jl:62-66)
+      row[col++] = dword(o + 4);
+      row[col++] = iword(o + 6) == 0 ? 0.0 : -((dword(o + 0) + dword(o + 2)) + (dword(o + 1) + dword(o + 3)));
+    }
+  }
+  row[col] = dword(d.wmon);          // det_max
+  row[col + 1] = dword(d.wmon + 1);  // symp_max
+  row[col + 2] = dword(d.wmon + 2);  // M_max
+  for (int j = 0; j < s; ++j) row[col + 3 + j] = x[(int64_t)j * m + i];
+}
+
+int ind_rows_launch(int64_t m, int s, const IndRowDesc& d, int g, int G, const double* x, const uint64_t* st,
+                    const int8_t* code, int64_t* fail, double* R, hipStream_t stream) {
+  if (m == 0) return 0;
+  hipLaunchKernelGGL(k_ind_rows, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, m, s, d, g, G, x, st,
+                     code, fail, R);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (e == hipErrorOutOfMemory ? CL_ERR_OOM : CL_ERR_DEVICE_LOST);
 }

@@ -66,7 +66,7 @@ def hh_section_starts(seed, m, E, section="q2", orbit0=1, max_tries=64):
     r = np.sqrt(2 * E)
     if section == "q1":
         from scipy.optimize import brentq
-        u = lambda q: q * q / 2 - q ** 3 / 3 - E
+        u = lambda q: q * q / 2 - q * q * q / 3 - E  # q^3 as the reference's literal power: (q q) q
         a, b = (brentq(u, -1.0, 0.0), brentq(u, 0.0, 1.0)) if E < 1 / 6 else (-0.5, 1.0)
     for t in range(max_tries):
         if not todo.any():
@@ -79,7 +79,7 @@ def hh_section_starts(seed, m, E, section="q2", orbit0=1, max_tries=64):
             out[ok] = np.stack([q1, np.zeros(m), p1, np.sqrt(np.where(rem > 0, rem, 0))], axis=1)[ok]
         else:
             q2, p2 = a + (b - a) * u1, -r + 2 * r * u2
-            rem = 2 * (E - p2 * p2 / 2 - (q2 * q2 / 2 - q2 ** 3 / 3))
+            rem = 2 * (E - p2 * p2 / 2 - (q2 * q2 / 2 - q2 * q2 * q2 / 3))
             ok = todo & (rem > 0) & (np.abs(q2) <= 1)
             out[ok] = np.stack([np.zeros(m), q2, np.sqrt(np.where(rem > 0, rem, 0)), p2], axis=1)[ok]
         todo &= ~ok

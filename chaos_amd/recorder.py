@@ -270,3 +270,166 @@ def _record_shadow(sys, x0, N, tau, grid, u0, d0, launch_steps, timing):
     e = decode_ends(raw, s)
     return Records(method="L2", grid=grid, R=dR.cpu().numpy(), fail=dfail.cpu().numpy(), code=e["code"],
                    hstep=e["step"], x=e["x"], W=e["y"], states=raw, k=1, s=s)
+
+
+# ---------------------------------------------------------------- the indicator recorder (device kinds)
+# Columns of E24 per member (src/Experiments/run_indicator_chain.jl:16-17); RLI belongs to the pair kind.
+IND_MEMBER_COLUMNS = {"SALI": ("SALI.sali", "SALI.L"), "FLI": ("FLI.fli", "FLI.fli1"),
+                      "MEGNO": ("MEGNO.Y1", "MEGNO.Ybar1", "MEGNO.Ybar20_1", "MEGNO.Y2", "MEGNO.Ybar2",
+                                "MEGNO.Ybar20_2")}
+
+
+def ind_members(names):
+    """The members of a tangent chain in order (chain_members, run_indicator_chain.jl:27-31): every member but the
+    last non-replacing, the last replacing; SALI has one form (it always replaces)."""
+    from .reducers import MEGNO, FLI, SALI
+    names = tuple(names)
+    if not 1 <= len(names) <= 4:
+        raise ValueError("a tangent chain has 1 to 4 members")
+    out = []
+    for i, nm in enumerate(names):
+        last = i == len(names) - 1
+        if nm == "SALI":
+            out.append(SALI())
+        elif nm == "FLI":
+            out.append(FLI(replace=last))
+        elif nm == "MEGNO":
+            out.append(MEGNO(replace=last))
+        else:
+            raise ValueError(f"the device tangent kind runs FLI, MEGNO and SALI, not {nm!r}")
+    return tuple(out)
+
+
+@dataclass
+class IndRecords:
+    """Record array R[orbit, grid row, column] of an indicator item (columns: the members' series columns, det_max,
+    symp_max, M_max, the point), the 1-based failing row per orbit (0: none) and the final carried data; `reducers`
+    is the chain, for the final tables of a window kind's reader (finals_of)."""
+    kind: str
+    columns: tuple
+    grid: np.ndarray
+    R: np.ndarray
+    fail: np.ndarray
+    code: np.ndarray
+    hstep: np.ndarray
+    x: np.ndarray
+    W: np.ndarray
+    states: np.ndarray
+    reducers: tuple
+    s: int
+
+    def unpack(self, j):
+        """The buffer of orbit j (_ind_unpack! and _blank_ind!, run_indicator_device.jl:111-125,
+        run_indicator_record.jl:134-143): rows before the failing row, NaN from it on; M_max of the last good row
+        (1.0 when none); t is the grid step of each recorded row and 0 after."""
+        nser, G = len(self.columns), len(self.grid)
+        f = int(self.fail[j])
+        last = G if f == 0 else f - 1
+        Rj = self.R[j]
+        nan = float("nan")
+        out = dict(t=np.zeros(G, np.int64), series=np.full((G, nser), nan), det=np.full(G, nan),
+                   symp=np.full(G, nan), state=np.full((G, 2), nan), mmax=1.0, failrow=f,
+                   winding=np.zeros((G, 2), np.int64) if self.s == 4 else None)
+        for g in range(last):
+            out["t"][g] = self.grid[g]
+            out["series"][g] = Rj[g, :nser]
+            out["det"][g], out["symp"][g] = Rj[g, nser], Rj[g, nser + 1]
+            out["state"][g] = Rj[g, nser + 3:nser + 5]
+            if self.s == 4:
+                out["winding"][g] = Rj[g, nser + 5:nser + 7].astype(np.int64)
+        if last >= 1:
+            out["mmax"] = float(Rj[last - 1, nser + 2])
+        return out
+
+
+def record_indicators(sys, x0, N, *, members=("FLI", "MEGNO", "SALI"), reader=None, tau=1, grid=None, W0=None,
+                      direction="pi/4", launch_steps=None, timing=None):
+    """Record an indicator item on the GPU (ind_device_records, run_indicator_device.jl:164-197).
+
+    The tangent kind (`reader` None): the chain is `members` (ind_members) then RunMonitor, on the frame `W0` (2 x 2,
+    or one per orbit; default deviation_frame(direction, 2)); each row holds the members' series columns. A window
+    kind (`reader` a WeightedBirkhoff): the chain is RunMonitor then the reader, stepped with the constant deviation
+    column (1, 0) reset at every handoff; its rows hold the monitor and the point, and its tables come from the
+    reader's final state (finals_of on `states`). `sys` is a StandardMap (x0 of shape (m, 2)) or a
+    WrappedStandardMap (x0 of shape (m, 4)); `grid` the record steps (default: the study grid). Launches end at the
+    grid steps and are split at `launch_steps` (a multiple of tau) when given."""
+    import torch
+    from .tangent import make_chain, _context
+    if not isinstance(sys, (StandardMap, WrappedStandardMap)):
+        raise ValueError("the indicator recorder's device path runs the standard map, lifted or wrapped")
+    N, tau = int(N), int(tau)
+    window = reader is not None
+    k = 1 if window else 2
+    if grid is None:
+        grid = record_grid(N, tau, points_per_decade=20, first_steps=100, extra_fractions=(0.1, 0.5))
+    grid = np.asarray(grid, dtype=np.int64)
+    G = len(grid)
+    if G == 0 or grid[-1] != N or np.any(grid % tau) or np.any(np.diff(grid) <= 0) or grid[0] <= 0:
+        raise ValueError("grid must be increasing positive multiples of tau ending at N")
+    if launch_steps is not None and (launch_steps < 1 or launch_steps % tau):
+        raise ValueError("launch_steps must be a positive multiple of tau")
+    x0 = np.ascontiguousarray(np.asarray(x0, dtype=np.float64))
+    m, s = x0.shape
+    if s != sys.state_length:
+        raise ValueError(f"state length {s} does not match {sys!r}")
+    if window:
+        reds = (RunMonitor(sys), reader)
+        reader.check_run(tau, N)
+        W0 = np.broadcast_to(np.array([[1.0], [0.0]]), (m, 2, 1))
+        columns = ()
+    else:
+        reds = ind_members(members) + (RunMonitor(sys),)
+        columns = tuple(c for nm in members for c in IND_MEMBER_COLUMNS[nm])
+        if W0 is None:
+            W0 = deviation_frame(direction, 2)
+        W0 = np.asarray(W0, dtype=np.float64)
+        if W0.shape == (2, 2):
+            W0 = np.broadcast_to(W0, (m, 2, 2))
+        if W0.shape != (m, 2, 2):
+            raise ValueError("W0 must be 2 x 2 or one 2 x 2 frame per orbit")
+    chain, allr = make_chain(reds, reset_frame=window)
+    lib, ctx = _context()
+    sysd = sys.descriptor()
+    nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, 2, k)
+    if nbytes < 0:
+        raise L.HIPError(-nbytes, "no kernel instance for this chain")
+    words = nbytes // 8
+    ncol = lib.cl_ind_row_width(ctypes.byref(chain), s)
+    assert ncol == len(columns) + 3 + s
+    dev = torch.device("cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    dx = torch.from_numpy(np.ascontiguousarray(x0.T)).to(dev)
+    dW = torch.from_numpy(np.ascontiguousarray(W0.transpose(2, 1, 0).reshape(2 * k, m))).to(dev)
+    dst = torch.zeros((words, m), dtype=torch.int64, device=dev)
+    dcode = torch.zeros(m, dtype=torch.int8, device=dev)
+    dh = torch.zeros(m, dtype=torch.int64, device=dev)
+    dfail = torch.zeros(m, dtype=torch.int64, device=dev)
+    dR = torch.zeros((m, G, ncol), dtype=torch.float64, device=dev)
+    if timing is not None:
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+    n = 0
+    for g in range(G):
+        target = int(grid[g])
+        while n < target:
+            steps = target - n if launch_steps is None else min(int(launch_steps), target - n)
+            rc = lib.cl_propagate_device(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, m, s, 2, k, n,
+                                         steps, N, tau, dx.data_ptr(), dW.data_ptr(), None, dst.data_ptr(),
+                                         dcode.data_ptr(), dh.data_ptr(), stream)
+            if rc:
+                raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
+            n += steps
+        rc = lib.cl_ind_record_rows(ctx, ctypes.byref(sysd), ctypes.byref(chain), m, s, g, G, dx.data_ptr(),
+                                    dst.data_ptr(), dcode.data_ptr(), dfail.data_ptr(), dR.data_ptr(), stream)
+        if rc:
+            raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
+    if timing is not None:
+        ev1.record()
+    torch.cuda.synchronize()
+    if timing is not None:
+        timing["device_ms"] = ev0.elapsed_time(ev1)
+    raw = dst.cpu().numpy().T.copy().view(np.uint8).reshape(m, words * 8)
+    return IndRecords(kind="birkhoff" if window else "tangent", columns=columns, grid=grid, R=dR.cpu().numpy(),
+                      fail=dfail.cpu().numpy(), code=dcode.cpu().numpy(), hstep=dh.cpu().numpy(),
+                      x=dx.cpu().numpy().T.copy(), W=dW.cpu().numpy().reshape(k, 2, m).transpose(2, 1, 0).copy(),
+                      states=raw, reducers=allr, s=s)

@@ -168,7 +168,7 @@ class MEGNO(Reducer):
         tau = (st["n"][:, None] // np.where(zero, 1, K)).astype(np.float64)
         Y = np.where(zero, 0.0, 2 * R1 / nn)
         Ybar = np.where(zero, 0.0, Q / np.where(zero, 1, K))
-        Ybar20 = np.where(zero, 0.0, tau * Q2 / nn ** 3)
+        Ybar20 = np.where(zero, 0.0, tau * Q2 / (nn * nn * nn))  # tn^3 is (tn tn) tn, not pow
         return dict(n=st["n"].copy(), handoffs=st["K"].copy(), Y=Y, Ybar=Ybar, Ybar20=Ybar20,
                     sum=c[:, :, 0, 0].copy(), comp=c[:, :, 0, 1].copy(), R1=R1, R2=R2, Q=Q, Q2=Q2)
 

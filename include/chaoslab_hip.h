@@ -225,6 +225,21 @@ int cl_record_rows(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain, int
                    int G, const double *dx, const uint64_t *dstates, const int8_t *dcode, int64_t *dfail,
                    double *dR, void *stream);
 
+/* The indicator recorder's rows (replaces the row writes of _ind_series_kernel! and _ind_window_kernel!,
+ * src/Experiments/run_indicator_device.jl:13-29,31-60,62-91): after the launch that ends at grid step g of G, row g
+ * of dR[m][G][cl_ind_row_width] is written for every orbit still running. A series kind's chain is its members
+ * (FLI, MEGNO, SALI in any order, at most 4, on frame 1) then RunMonitor; the row is each member's final columns
+ * (run_indicator_record.jl:39-70: FLI fli, fli1; MEGNO Y, Ybar, Ybar20 of column 1 then column 2; SALI sali, L),
+ * then det_max, symp_max, M_max, then the point (s = 2 lifted, 4 wrapped). A window kind's chain is RunMonitor
+ * then its reader; its row is the monitor's three maxima and the point. An orbit whose launch failed gets
+ * dfail[i] = g + 1 (its 1-based grid row) and no further rows; dfail and dR start zeroed. Device buffers;
+ * asynchronous. */
+int cl_ind_record_rows(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain, int64_t m, int s, int g, int G,
+                       const double *dx, const uint64_t *dstates, const int8_t *dcode, int64_t *dfail, double *dR,
+                       void *stream);
+/* Columns of a row of cl_ind_record_rows for this chain and state length s (negative: bad argument). */
+int cl_ind_row_width(const cl_chain *chain, int s);
+
 /* The two-trajectory estimator (replaces _shadow_kernel!, src/Lyapunov/two_trajectory.jl:113-121, and its body
  * _shadow_body, :45-82): the reference orbit and a shadow at distance d0, placed back at d0 along the separation
  * every tau steps; the end record of each orbit (ShadowEnd{s, Float64}: x[s], y[s], sum (s, c), dprev, code + 7 pad

@@ -1636,11 +1636,62 @@ static void record_row(const cl_chain* ch, int k, int s, int g, const RState<dou
   for (int j = 0; j < s; ++j) row[k + 7 + j] = x[j];
 }
 
+// The indicator recorder's row (src/Experiments/run_indicator_record.jl:39-70 `_put!`, run_indicator_device.jl:13-29):
+// a series kind's chain is its members then RunMonitor, and the row holds each member's final columns (FLI: fli,
+// fli1, fli.jl:91-94; MEGNO: megno_series of column 1 then 2, megno.jl:97-102, with tn^3 the literal power
+// (tn tn) tn; SALI: sali and L, sali.jl:62-66), then det_max, symp_max, M_max, then the point. A window kind's chain
+// is RunMonitor then its reader, and the row holds the monitor and the point.
+static void ind_row(const cl_chain* ch, int s, const RState<double>* st, const double* x, double* row) {
+  const bool window = ch->r[0].kind == CL_RED_RUNMONITOR;
+  const int nmem = window ? 0 : ch->n - 1;
+  int col = 0;
+  for (int r = 0; r < nmem; ++r) {
+    if (ch->r[r].kind == CL_RED_FLI) {
+      row[col++] = st[r].fli.fB;
+      row[col++] = st[r].fli.f1;
+    } else if (ch->r[r].kind == CL_RED_MEGNO) {
+      const MegSt<double>& q = st[r].meg;
+      for (int c = 0; c < 2; ++c) {
+        double y = 0.0, yb = 0.0, y20 = 0.0;
+        if (q.K != 0) {
+          const double tn = (double)q.n;
+          y = 2.0 * (q.c[c].R1.s + q.c[c].R1.c) / tn;
+          yb = (q.c[c].Q.s + q.c[c].Q.c) / (double)q.K;
+          y20 = (double)(q.n / q.K) * (q.c[c].Q2.s + q.c[c].Q2.c) / (tn * tn * tn);
+        }
+        row[col++] = y;
+        row[col++] = yb;
+        row[col++] = y20;
+      }
+    } else {
+      const SaliSt<double>& q = st[r].sali;
+      row[col++] = q.sali;
+      row[col++] = q.n == 0 ? 0.0 : -((q.s1.s + q.s2.s) + (q.s1.c + q.s2.c));
+    }
+  }
+  const MonSt<double>& mon = st[window ? 0 : ch->n - 1].mon;
+  row[col] = mon.det_max;
+  row[col + 1] = mon.symp_max;
+  row[col + 2] = mon.M_max;
+  for (int j = 0; j < s; ++j) row[col + 3 + j] = x[j];
+}
+
+static int ind_row_width(const cl_chain* ch, int s) {
+  if (ch->r[0].kind == CL_RED_RUNMONITOR) return 3 + s;
+  int c = 0;
+  for (int r = 0; r + 1 < ch->n; ++r) c += ch->r[r].kind == CL_RED_MEGNO ? 6 : 2;
+  return c + 3 + s;
+}
+
+// The recorder loops (_record_batch, run_record.jl; _ind_series_loop! and _ind_window_loop!,
+// run_indicator_record.jl:99-132): segments of the shared body end at the grid steps, the row of each grid step is
+// written from the carried states; a failing segment records its 1-based grid row and ends the orbit's rows.
+// ind = false: the exponent methods' row (record_row); true: the indicator row (ind_row).
 template <class P>
 static int record_all(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int k, int64_t N, int tau,
                       const int64_t* grid, int G, double* x, double* W, double* R, int64_t* fail, int8_t* code,
-                      int64_t* hstep, uint8_t* states, int nthreads) {
-  const int d = 2, ncol = k + 7 + s;
+                      int64_t* hstep, uint8_t* states, int nthreads, bool ind = false) {
+  const int d = 2, ncol = ind ? ind_row_width(ch, s) : k + 7 + s;
   MapSys<P, double> ms;
   ms.kind = sys->kind;
   ms.K = sys->K;
@@ -1653,7 +1704,7 @@ static int record_all(const cl_system* sys, const cl_chain* ch, int64_t m, int s
   }
   std::atomic<int64_t> next(0);
   auto worker = [&]() {
-    RState<double> st[2];
+    RState<double> st[CL_MAX_REDUCERS];
     while (true) {
       int64_t lo = next.fetch_add(16);
       if (lo >= m) break;
@@ -1674,7 +1725,8 @@ static int record_all(const cl_system* sys, const cl_chain* ch, int64_t m, int s
           }
           n = grid[g];
           double* row = R + ((size_t)i * G + g) * ncol;
-          record_row<P>(ch, k, s, g, st, xx, row, g > 0 ? row - ncol : nullptr);
+          if (ind) ind_row(ch, s, st, xx, row);
+          else record_row<P>(ch, k, s, g, st, xx, row, g > 0 ? row - ncol : nullptr);
         }
         code[i] = c;
         hstep[i] = h;
@@ -2037,6 +2089,31 @@ int or_record(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int k,
   if (math == MATH_CL)
     return record_all<F64Cl>(sys, ch, m, s, k, N, tau, grid, G, x, W, R, fail, code, hstep, (uint8_t*)states, nthreads);
   return record_all<F64Libm>(sys, ch, m, s, k, N, tau, grid, G, x, W, R, fail, code, hstep, (uint8_t*)states, nthreads);
+}
+
+// The indicator recorder on a standard map (lifted s = 2, wrapped s = 4): a series kind's chain (FLI, MEGNO, SALI
+// members, then RunMonitor; k = 2) or a window kind's (RunMonitor then WeightedBirkhoff, k = 1, with the constant
+// deviation column flag). R[m][G][row width], fail, x, W (m x k x 2), code, hstep and the states as or_record.
+int or_ind_record(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int k, int64_t N, int tau,
+                  const int64_t* grid, int G, double* x, double* W, double* R, int64_t* fail, int8_t* code,
+                  int64_t* hstep, void* states, int math, int nthreads) {
+  const bool sys_ok = (sys->kind == CL_SYS_STANDARD_MAP && s == 2) || (sys->kind == CL_SYS_WRAPPED_STANDARD_MAP && s == 4);
+  bool ch_ok = ch->n >= 2;
+  if (ch_ok && ch->r[0].kind == CL_RED_RUNMONITOR) {
+    ch_ok = ch->n == 2 && ch->r[1].kind == CL_RED_BIRKHOFF && k == 1 && (ch->flags & CL_CHAIN_RESET_FRAME);
+  } else if (ch_ok) {
+    ch_ok = k == 2 && ch->n <= 5 && ch->r[ch->n - 1].kind == CL_RED_RUNMONITOR;
+    for (int r = 0; ch_ok && r + 1 < ch->n; ++r)
+      ch_ok = ch->r[r].kind == CL_RED_FLI || ch->r[r].kind == CL_RED_MEGNO || ch->r[r].kind == CL_RED_SALI;
+  }
+  if (!ch_ok || !sys_ok || N % tau || G < 1 || grid[G - 1] != N) return 10;
+  for (int g = 0; g < G; ++g)
+    if (grid[g] % tau || grid[g] <= (g ? grid[g - 1] : 0)) return 10;
+  if (math == MATH_CL)
+    return record_all<F64Cl>(sys, ch, m, s, k, N, tau, grid, G, x, W, R, fail, code, hstep, (uint8_t*)states,
+                             nthreads, true);
+  return record_all<F64Libm>(sys, ch, m, s, k, N, tau, grid, G, x, W, R, fail, code, hstep, (uint8_t*)states,
+                             nthreads, true);
 }
 
 // The two-trajectory estimator on a standard map (lifted s = 2, wrapped s = 4), or on the linear map x' = A x

@@ -43,6 +43,9 @@ def load():
         o.or_record.restype = I
         o.or_record.argtypes = [ctypes.POINTER(L.cl_system), ctypes.POINTER(L.cl_chain), I64, I, I, I64, I, P, I, P, P,
                                 P, P, P, P, P, I, I]
+        o.or_ind_record.restype = I
+        o.or_ind_record.argtypes = [ctypes.POINTER(L.cl_system), ctypes.POINTER(L.cl_chain), I64, I, I, I64, I, P, I,
+                                    P, P, P, P, P, P, P, I, I]
         o.or_shadow.restype = I
         o.or_shadow.argtypes = [ctypes.POINTER(L.cl_system), P, I64, I, I64, I64, I64, I, ctypes.c_double, P, P, P, I, I]
         o.or_shadow_record.restype = I
@@ -156,6 +159,29 @@ def frames(seed, orbit0, m, d, k):
     out = np.zeros((m, k, d), dtype=np.float64)
     assert o.or_initial_deviations(seed & (2 ** 64 - 1), orbit0, m, d, k, out.ctypes.data) == 0
     return out.transpose(0, 2, 1).copy()
+
+
+def ind_record(sysd, chain, x0, W0, N, tau, k, grid, ncol, math=MATH_CL, nthreads=None):
+    """The indicator recorder (oracle/oracle.cpp, or_ind_record): a series kind's chain (members, RunMonitor; k = 2)
+    or a window kind's (RunMonitor, WeightedBirkhoff; k = 1, reset-frame flag); returns R (m, G, ncol), fail, x, W,
+    raw states, code, hstep."""
+    o = load()
+    x = np.ascontiguousarray(np.array(x0, dtype=np.float64))
+    m, s = x.shape
+    W = np.array(np.asarray(W0, dtype=np.float64).reshape(m, 2, k).transpose(0, 2, 1), order="C", copy=True)
+    g = np.ascontiguousarray(np.asarray(grid, dtype=np.int64))
+    G = len(g)
+    R = np.zeros((m, G, ncol))
+    fail = np.zeros(m, dtype=np.int64)
+    nb = o.or_state_bytes(ctypes.byref(chain), 2, k)
+    states = np.zeros((m, max(nb, 1)), dtype=np.uint8)
+    code = np.zeros(m, dtype=np.int8)
+    hstep = np.zeros(m, dtype=np.int64)
+    nt = nthreads or min(16, os.cpu_count() or 1)
+    rc = o.or_ind_record(ctypes.byref(sysd), ctypes.byref(chain), m, s, k, int(N), tau, _ptr(g), G, _ptr(x),
+                         _ptr(W), _ptr(R), _ptr(fail), _ptr(code), _ptr(hstep), _ptr(states), math, nt)
+    assert rc == 0, rc
+    return dict(R=R, fail=fail, x=x, W=W.transpose(0, 2, 1).copy(), states=states[:, :nb], code=code, hstep=hstep)
 
 
 def record(sysd, chain, x0, W0, N, tau, k, grid, math=MATH_CL, nthreads=None):

@@ -1,0 +1,84 @@
+"""The indicator recorder's device path (record_indicators: the propagate launches ending at the grid steps and
+cl_ind_record_rows) against the oracle's recorder (or_ind_record) bit for bit: rows, failing rows, points, frames,
+states, codes (src/Experiments/run_indicator_device.jl:31-91). Every test prints its case before touching the GPU."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from chaos_amd import (StandardMap, WrappedStandardMap, RunMonitor, WeightedBirkhoff, CosObservable, SinObservable,
+                       LiftedObservable, torus_state)
+from chaos_amd.recorder import IND_MEMBER_COLUMNS, ind_members, record_grid, record_indicators
+from chaos_amd.tangent import make_chain
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def starts(m, wrapped, seed):
+    rng = np.random.default_rng(seed)
+    z = 2 * math.pi * rng.random((m, 2))
+    z[1] = [np.nan, 0.5]               # fails in the first segment
+    if wrapped:
+        return np.array([torus_state(v) if np.all(np.isfinite(v)) else [v[0], v[1], 0.0, 0.0] for v in z])
+    z[2] = [0.5, 1e306]                # overflows after some 180 steps: a failing row inside the grid
+    return z
+
+
+@pytest.mark.parametrize("members", [("FLI", "MEGNO", "SALI"), ("MEGNO", "FLI", "SALI"), ("MEGNO",), ("FLI",),
+                                     ("SALI",)])
+@pytest.mark.parametrize("wrapped", [False, True], ids=["lifted", "wrapped"])
+@pytest.mark.parametrize("tau,launch", [(1, None), (2, 38)])
+def test_tangent_kind_equals_oracle(members, wrapped, tau, launch):
+    m, N = 1000, 400
+    sysm = (WrappedStandardMap if wrapped else StandardMap)(6.0)
+    x0 = starts(m, wrapped, 11 + tau)
+    s = x0.shape[1]
+    grid = record_grid(N, tau, points_per_decade=8, first_steps=20, extra_fractions=(0.1, 0.5))
+    W0 = O.frames(6, 0, m, 2, 2)
+    chain, _ = make_chain(ind_members(members) + (RunMonitor(sysm),))
+    ncol = sum(len(IND_MEMBER_COLUMNS[n]) for n in members) + 3 + s
+    orc = O.ind_record(sysm.descriptor(), chain, x0, W0, N, tau, 2, grid, ncol)
+    say("ind tangent", members, "wrapped" if wrapped else "lifted", "tau", tau, "launch", launch, "m", m, "N", N,
+        "rows", len(grid))
+    rec = record_indicators(sysm, x0, N, members=members, tau=tau, grid=grid, W0=W0, launch_steps=launch)
+    assert rec.R.shape == orc["R"].shape
+    assert np.array_equal(rec.fail, orc["fail"]) and np.array_equal(rec.code, orc["code"])
+    assert np.array_equal(rec.hstep, orc["hstep"])
+    assert np.array_equal(rec.R.view(np.uint64), orc["R"].view(np.uint64))
+    assert np.array_equal(rec.states, orc["states"])
+    ok = rec.code == 0
+    assert np.array_equal(rec.x[ok].view(np.uint64), orc["x"][ok].view(np.uint64))
+    assert np.array_equal(rec.W[ok].view(np.uint64), orc["W"][ok].view(np.uint64))
+    assert rec.fail[1] == 1 and (wrapped or 1 < rec.fail[2] < len(grid))
+    u = rec.unpack(2)
+    assert np.all(np.isnan(u["series"][max(u["failrow"] - 1, 0):])) if u["failrow"] else True
+
+
+@pytest.mark.parametrize("wrapped", [False, True], ids=["lifted", "wrapped"])
+def test_birkhoff_window_kind_equals_oracle(wrapped):
+    m, N, tau = 1500, 2000, 1
+    sysm = (WrappedStandardMap if wrapped else StandardMap)(2 * math.pi)
+    rng = np.random.default_rng(21)
+    z = 2 * math.pi * rng.random((m, 2))
+    x0 = np.array([torus_state(v) for v in z]) if wrapped else z
+    x0[4] = [np.nan, 0.1, 0.0, 0.0] if wrapped else [np.nan, 0.1]
+    lift = LiftedObservable(2, 4) if wrapped else LiftedObservable(1, 2)
+    wb = WeightedBirkhoff(window=(250, 1000), observables=(CosObservable(1), SinObservable(2), lift),
+                          birkhoff_order=2)
+    grid = record_grid(N, tau, points_per_decade=10, first_steps=10, extra_fractions=(0.5,))
+    chain, _ = make_chain((RunMonitor(sysm), wb), reset_frame=True)
+    W1 = np.broadcast_to(np.array([[1.0], [0.0]]), (m, 2, 1))
+    orc = O.ind_record(sysm.descriptor(), chain, x0, W1, N, tau, 1, grid, 3 + x0.shape[1])
+    say("ind birkhoff window", "wrapped" if wrapped else "lifted", "m", m, "N", N, "rows", len(grid))
+    rec = record_indicators(sysm, x0, N, reader=wb, tau=tau, grid=grid, launch_steps=300)
+    assert np.array_equal(rec.fail, orc["fail"]) and np.array_equal(rec.code, orc["code"])
+    assert np.array_equal(rec.R.view(np.uint64), orc["R"].view(np.uint64))
+    assert np.array_equal(rec.states, orc["states"])
+    assert rec.fail[4] == 1 and np.mean(rec.fail == 0) > 0.99
