@@ -500,7 +500,23 @@ def rec_main(a):
 # st and bk of verify/experiments-indicators-block.toml:123-184; workspaces/module-build/stages/04-verify/output/
 # gpu-rtx5060-d86-verify-log.txt:24-29): 1024 wrapped orbits, ns per orbit-step on 23 CPU threads and its GPU (st is
 # CPU-only there).
-IND_REFERENCE_NS = {"st": (15.34, None), "bk": (29.29, 63.1)}
+IND_REFERENCE_NS = {"st": (15.34, None), "bk": (29.29, 63.1), "tg": (45.14, 64.88)}
+
+
+def ind_tg_case(m, N=4000, K=2 * math.pi, seed=5):
+    """The indicator recorder's tangent kind as the reference times it (gpu-rtx5060-d86-verify-log.txt:27, spec tg of
+    experiments-indicators-block.toml:97-103): members FLI, MEGNO, SALI then RunMonitor, k = 2, the study grid of
+    4000 steps, rows written at every grid step; wrapped map, tau = 1. Returns the device time of the recording."""
+    from chaos_amd import WrappedStandardMap, torus_state
+    from chaos_amd.inputs import torus_uniform
+    from chaos_amd.recorder import record_indicators
+    sysm = WrappedStandardMap(K)
+    x0 = np.array([torus_state(v) for v in torus_uniform(seed, m)])
+    t = {}
+    t0 = time.perf_counter()
+    rec = record_indicators(sysm, x0, N, members=("FLI", "MEGNO", "SALI"), tau=1, timing=t)
+    return dict(m=m, N=N, wall=time.perf_counter() - t0, kernel_ms=t["device_ms"], rows=len(rec.grid),
+                bad=int((rec.code != 0).sum()))
 
 
 def ind_case(lib, ctx, name, m, K=2 * math.pi, seed=5):
@@ -579,10 +595,17 @@ def ind_main(a):
             res[(name, m)] = r
             log(f"ind {name}: {m} orbits x {r['N']} steps: {r['kernel_ms']:.2f} ms device, "
                 f"{r['kernel_ms'] * 1e6 / (m * r['N']):.4f} ns per orbit-step, {r['bad']} failed")
+    ind_tg_case(256, N=400)   # warm-up
+    for m in (1024, 65536):
+        r = ind_tg_case(m)
+        res[("tg", m)] = r
+        log(f"ind tg (recorded, {r['rows']} rows): {m} orbits x {r['N']} steps: {r['kernel_ms']:.2f} ms device, "
+            f"{r['kernel_ms'] * 1e6 / (m * r['N']):.4f} ns per orbit-step, {r['bad']} failed")
     total = sum(res[(n, big)]["m"] * res[(n, big)]["N"] for n in ("st", "bk"))
     wall = sum(res[(n, big)]["wall"] for n in ("st", "bk"))
     value = total / wall
     per = {f"{n}@{m}": res[(n, m)]["kernel_ms"] * 1e6 / (m * res[(n, m)]["N"]) for n in ("st", "bk") for m in (1024, big)}
+    per.update({f"tg@{m}": res[("tg", m)]["kernel_ms"] * 1e6 / (m * res[("tg", m)]["N"]) for m in (1024, 65536)})
     cpu = None
     if not a.no_cpu:
         from tests import oracle_lib as O
@@ -613,7 +636,9 @@ def ind_main(a):
         "config": {"workload": f"indicator reducers on the wrapped standard map K=2pi, tau=1: st = StretchingNumbers "
                                f"(100 cells, interval 4) + RunMonitor, k=2, 4000 steps; bk = RunMonitor + "
                                f"WeightedBirkhoff (windows 250/1000, order 2, 4 observables), k=1 constant column, "
-                               f"2000 steps; {big} orbits each (and the reference's 1024)",
+                               f"2000 steps; {big} orbits each (and the reference's 1024); tg = the indicator "
+                               f"recorder's tangent kind (FLI, MEGNO, SALI + RunMonitor, study grid rows), 4000 steps, "
+                               f"1024 and 65536 orbits (outside value)",
                    "ns_per_orbit_step": per, "reference_ns_per_orbit_step_cpu23_gpu_rtx5060": IND_REFERENCE_NS,
                    "failed_orbits": {f"{n}@{m}": res[(n, m)]["bad"] for n in ("st", "bk") for m in (1024, big)}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
