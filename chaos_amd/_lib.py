@@ -13,6 +13,8 @@ CL_OK, CL_ERR_NO_DEVICE, CL_ERR_UNSUPPORTED, CL_ERR_OOM, CL_ERR_DEVICE_LOST, CL_
 CL_ELT_F64, CL_ELT_F64_F32TAN, CL_ELT_DD = 1, 2, 3
 CL_SYS_STANDARD_MAP, CL_SYS_WRAPPED_STANDARD_MAP, CL_SYS_STANDARD_MAP_CHAIN, CL_SYS_HENON_HEILES_GAUSS = 1, 2, 3, 4
 CL_SYS_STANDARD_MAP_K = 5
+CL_SYS_PAIR_STANDARD_MAP = 6
+CL_SYS_PAIR_WRAPPED_STANDARD_MAP = 7
 CL_RED_QR, CL_RED_BENETTIN, CL_RED_MEGNO, CL_RED_SALI, CL_RED_FLI, CL_RED_OFLI, CL_RED_GALI = 1, 2, 3, 4, 5, 6, 7
 CL_RED_MONITOR = 9
 CL_RED_RUNMONITOR = 10
@@ -36,6 +38,7 @@ class cl_system(ctypes.Structure):
 
 
 CL_RED_BIRKHOFF = 12
+CL_RED_RLI = 13
 CL_OBS_COS, CL_OBS_SIN, CL_OBS_LIFTED = 1, 2, 3
 CL_WB_MAX_WINDOWS, CL_WB_MAX_OBSERVABLES = 16, 8
 

@@ -239,6 +239,50 @@ struct WrappedStdMap {
   }
 };
 
+// The product of a map with itself (src/Systems/pair.jl:1-77): the state is (state A; state B), each half steps by the
+// base map alone, and the tangent acts block by block on the frame's rows 1..d and d+1..2d (the zero blocks are never
+// multiplied, so each block's product is the single-orbit one). The monitor reads orbit A's one-step tangent
+// (run_indicator_record.jl:6-10).
+template <class B>
+struct PairMap {
+  static constexpr int D = 2 * B::D;
+  static constexpr int S = 2 * B::S;
+  B base;
+  struct Aux {
+    typename B::Aux a, b;
+  };
+  CL_HD void aux(const double* x, Aux& au) const {
+    base.aux(x, au.a);
+    base.aux(x + B::S, au.b);
+  }
+  template <int K2>
+  CL_HD void tangent_apply(const Aux& au, Frame<D, K2>& W) const {
+    Frame<B::D, K2> wa, wb;
+#pragma unroll
+    for (int j = 0; j < K2; ++j)
+#pragma unroll
+      for (int r = 0; r < B::D; ++r) {
+        wa.at(r, j) = W.at(r, j);
+        wb.at(r, j) = W.at(B::D + r, j);
+      }
+    base.tangent_apply(au.a, wa);
+    base.tangent_apply(au.b, wb);
+#pragma unroll
+    for (int j = 0; j < K2; ++j)
+#pragma unroll
+      for (int r = 0; r < B::D; ++r) {
+        W.at(r, j) = wa.at(r, j);
+        W.at(B::D + r, j) = wb.at(r, j);
+      }
+  }
+  CL_HD void jacobian(const Aux& au, double* J) const { base.jacobian(au.a, J); }
+  CL_HD void forward(const Aux& au, double* z) const {
+    base.forward(au.a, z);
+    base.forward(au.b, z + B::S);
+  }
+  CL_HD void field(const double* x, const Aux& au, double* f) const { base.field(x, au.a, f); }
+};
+
 // ------------------------------------------------------------------------------ reducer states
 // Byte layouts of the reference's isbits states (Julia lays struct fields out with C alignment),
 // so a host can rebuild them and call the reference's reducer_final unchanged.
@@ -269,6 +313,14 @@ struct MegnoSums {
 };
 struct MEGNOState {
   MegnoSums c1, c2;
+  double value;
+  int64_t n, K, step;
+  int8_t code, column;
+  int8_t pad_[6];
+};
+
+struct RLIState {
+  CSum sA, sB, Q;
   double value;
   int64_t n, K, step;
   int8_t code, column;
@@ -892,6 +944,60 @@ struct SALI {
     for (int i = 0; i < D; ++i) {
       W.at(i, 0) = u1[i];
       W.at(i, 1) = u2[i];
+    }
+  }
+};
+
+// ------------------------------------------------------------------------------ RLI (k = 2, pair systems)
+// rli.jl:77-110: the two arriving columns' norms and logs (the shared ColLogs, the same operations as Benettin's),
+// the off-block check (both blocks' cross entries exactly 0), DL = |(sB.s - sA.s) + (sB.c - sA.c)| / n added to Q.
+template <int D>
+struct RLI {
+  bool replace;
+  typedef RLIState State;
+  CL_HD void init(State& st) const {
+    st.sA = csum_zero(); st.sB = csum_zero(); st.Q = csum_zero();
+    st.value = 0.0;
+    st.n = 0; st.K = 0; st.step = 0;
+    st.code = 0; st.column = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) st.pad_[i] = 0;
+  }
+  CL_HD void step_logs(State& st, int64_t n, Frame<D, 2>& W, ColLogs& c) const {
+    if (st.code != 0) return;
+    if (!(c.ok1 && c.ok2)) {
+      st.column = c.ok1 ? 2 : 1;
+      st.value = c.ok1 ? c.a2 : c.a1;
+      st.step = n;
+      st.code = 1;
+      return;
+    }
+    constexpr int h = D / 2;
+    double m1 = 0.0, m2 = 0.0;  // the frame is finite here, so the hardware max is Julia's
+#pragma unroll
+    for (int i = 0; i < h; ++i) {
+      m1 = __builtin_fmax(m1, cl_fabs(W.at(h + i, 0)));
+      m2 = __builtin_fmax(m2, cl_fabs(W.at(i, 1)));
+    }
+    if (!(m1 == 0.0 && m2 == 0.0)) {
+      st.column = m1 == 0.0 ? 2 : 1;
+      st.value = m1 == 0.0 ? m2 : m1;
+      st.step = n;
+      st.code = 2;
+      return;
+    }
+    st.sA = csum_add(st.sA, c.l1);
+    st.sB = csum_add(st.sB, c.l2);
+    const double dl = cl_fabs((st.sB.s - st.sA.s) + (st.sB.c - st.sA.c)) / (double)n;
+    st.Q = csum_add(st.Q, dl);
+    st.n = n;
+    st.K = st.K + 1;
+    if (replace) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        W.at(i, 0) = W.at(i, 0) / c.a1;
+        W.at(i, 1) = W.at(i, 1) / c.a2;
+      }
     }
   }
 };

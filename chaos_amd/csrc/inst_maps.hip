@@ -86,7 +86,17 @@ typedef Cons<S_<RunMon<2>>, Cons<S_<WBirk>, Nil>> C1_mon_wb;
 typedef Cons<S_<MEGNO<2>>, Cons<S_<FLI<2>>, Cons<QR2(QR_MGS, 1), Nil>>> C_prefix_mgs;
 typedef Cons<S_<MEGNO<2>>, Cons<S_<FLI<2>>, Nil>> C_mf;
 
+// the pair kind of the indicator recorder (run_indicator_chain.jl:27-31, the study's spec RLI, MEGNO) on the
+// product map, and RLI alone; RunMonitor reads orbit A's 2 x 2 tangent
+typedef Cons<S_<RLI<4>>, Cons<S_<MEGNO<4>>, Cons<S_<RunMon<2>>, Nil>>> C_pair_rm_mon;
+typedef Cons<S_<RLI<4>>, Cons<S_<RunMon<2>>, Nil>> C_pair_r_mon;
+typedef Cons<S_<RLI<4>>, Nil> C_pair_r;
+#define PAIR_FOR(SK, SYS) \
+  ADD2(SK, SYS, C_pair_rm_mon, 1) ADD2(SK, SYS, C_pair_r_mon, 1) ADD2(SK, SYS, C_pair_r, 1)
+
 void register_instances_map(Instance* out, int* n, int cap) {
+  PAIR_FOR(CL_SYS_PAIR_STANDARD_MAP, PairMap<StdMap>)
+  PAIR_FOR(CL_SYS_PAIR_WRAPPED_STANDARD_MAP, PairMap<WrappedStdMap>)
   ALL_FOR(CL_SYS_STANDARD_MAP, StdMap)
   ALL_FOR(CL_SYS_WRAPPED_STANDARD_MAP, WrappedStdMap)
   ADD2(CL_SYS_STANDARD_MAP_K, StdMapK, C_none, 1) ADD2(CL_SYS_STANDARD_MAP_K, StdMapK, C_mgs, 1)

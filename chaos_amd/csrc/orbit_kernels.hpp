@@ -106,6 +106,12 @@ CL_HD void slot_step(Slot<FLI<D>, FR>& s, int64_t n, Ctx& c) {
   if (s.r.replace) c.logs_valid[FR] = false;
 }
 template <int D, int FR, class Ctx>
+CL_HD void slot_step(Slot<RLI<D>, FR>& s, int64_t n, Ctx& c) {
+  ensure_logs<D, 2, 0>(c, FR);
+  s.r.step_logs(s.st, n, c.W[FR], c.logs[FR]);
+  if (s.r.replace) c.logs_valid[FR] = false;
+}
+template <int D, int FR, class Ctx>
 CL_HD void slot_step(Slot<SALI<D>, FR>& s, int64_t n, Ctx& c) {
   ensure_logs<D, 2, 0>(c, FR);
   s.r.step_logs(s.st, n, c.W[FR], c.logs[FR]);
@@ -187,6 +193,8 @@ template <int D>
 CL_HD void slot_setup(FLI<D>& r, const cl_reducer& d) { r.replace = d.replace != 0; }
 template <int D>
 CL_HD void slot_setup(SALI<D>&, const cl_reducer&) {}
+template <int D>
+CL_HD void slot_setup(RLI<D>& r, const cl_reducer& d) { r.replace = d.replace != 0; }
 template <int D, int K>
 CL_HD void slot_setup(OFLI<D, K>& r, const cl_reducer& d) { r.replace = d.replace != 0; }
 template <int D, int K>

@@ -75,6 +75,8 @@ static int state_dim(int kind) {
     case CL_SYS_WRAPPED_STANDARD_MAP: return 4;
     case CL_SYS_HENON_HEILES_GAUSS: return 4;
     case CL_SYS_STANDARD_MAP_K: return 3;
+    case CL_SYS_PAIR_STANDARD_MAP: return 4;
+    case CL_SYS_PAIR_WRAPPED_STANDARD_MAP: return 8;
     default: return -1;
   }
 }
@@ -83,7 +85,9 @@ static int sys_dim(int kind) {
     case CL_SYS_STANDARD_MAP:
     case CL_SYS_WRAPPED_STANDARD_MAP:
     case CL_SYS_STANDARD_MAP_K: return 2;
-    case CL_SYS_HENON_HEILES_GAUSS: return 4;
+    case CL_SYS_HENON_HEILES_GAUSS:
+    case CL_SYS_PAIR_STANDARD_MAP:
+    case CL_SYS_PAIR_WRAPPED_STANDARD_MAP: return 4;
     default: return -1;
   }
 }
@@ -647,8 +651,13 @@ extern "C" int cl_ind_record_rows(cl_ctx* ctx, const cl_system* sys, const cl_ch
                                   int G, const double* dx, const uint64_t* dst, const int8_t* dcode, int64_t* dfail,
                                   double* dR, void* stream) {
   if (!ctx || !sys || !ch) return CL_ERR_ARGUMENT;
-  const bool sys_ok = (sys->kind == CL_SYS_STANDARD_MAP && s == 2) || (sys->kind == CL_SYS_WRAPPED_STANDARD_MAP && s == 4);
-  if (!sys_ok) return set_err(ctx, CL_ERR_UNSUPPORTED, "the indicator recorder runs the standard map, lifted or wrapped");
+  const bool pair = sys->kind == CL_SYS_PAIR_STANDARD_MAP || sys->kind == CL_SYS_PAIR_WRAPPED_STANDARD_MAP;
+  const bool sys_ok = ((sys->kind == CL_SYS_STANDARD_MAP || sys->kind == CL_SYS_PAIR_STANDARD_MAP) ||
+                       (sys->kind == CL_SYS_WRAPPED_STANDARD_MAP || sys->kind == CL_SYS_PAIR_WRAPPED_STANDARD_MAP)) &&
+                      s == state_dim(sys->kind);
+  if (!sys_ok)
+    return set_err(ctx, CL_ERR_UNSUPPORTED, "the indicator recorder runs the standard map or its pair, lifted or wrapped");
+  const int ns = pair ? s / 2 : s;  // the point columns hold orbit A's state (the first ns components)
   IndRowDesc d;
   memset(&d, 0, sizeof(d));
   const bool window = ch->n >= 2 && ch->r[0].kind == CL_RED_RUNMONITOR;
@@ -667,17 +676,18 @@ extern "C" int cl_ind_record_rows(cl_ctx* ctx, const cl_system* sys, const cl_ch
       if (kd == CL_RED_FLI) { w += (int)(sizeof(FLIState) / 8); cols += 2; }
       else if (kd == CL_RED_MEGNO) { w += (int)(sizeof(MEGNOState) / 8); cols += 6; }
       else if (kd == CL_RED_SALI) { w += (int)(sizeof(SALIState) / 8); cols += 2; }
-      else return set_err(ctx, CL_ERR_UNSUPPORTED, "series members are FLI, MEGNO and SALI");
+      else if (kd == CL_RED_RLI && pair) { w += (int)(sizeof(RLIState) / 8); cols += 2; }
+      else return set_err(ctx, CL_ERR_UNSUPPORTED, "series members are FLI, MEGNO and SALI (and RLI on a pair)");
     }
     d.n = ch->n - 1;
     d.wmon = w;
     d.ncol = cols;
   }
-  d.ncol += 3 + s;
+  d.ncol += 3 + ns;
   if (m < 0 || G < 1 || g < 0 || g >= G) return set_err(ctx, CL_ERR_ARGUMENT, "grid row out of range");
   if (m > 0 && (!dx || !dst || !dcode || !dfail || !dR)) return set_err(ctx, CL_ERR_ARGUMENT, "null buffer");
   hipSetDevice(ctx->device);
-  int rc = ind_rows_launch(m, s, d, g, G, dx, dst, dcode, dfail, dR, stream ? (hipStream_t)stream : ctx->stream);
+  int rc = ind_rows_launch(m, ns, d, g, G, dx, dst, dcode, dfail, dR, stream ? (hipStream_t)stream : ctx->stream);
   return rc ? set_err(ctx, rc, "indicator row kernel launch failed") : CL_OK;
 }
 
@@ -687,7 +697,7 @@ extern "C" int cl_ind_row_width(const cl_chain* ch, int s) {
   int cols = 0;
   for (int j = 0; j + 1 < ch->n; ++j) {
     const int kd = ch->r[j].kind;
-    cols += kd == CL_RED_MEGNO ? 6 : (kd == CL_RED_FLI || kd == CL_RED_SALI) ? 2 : 0;
+    cols += kd == CL_RED_MEGNO ? 6 : (kd == CL_RED_FLI || kd == CL_RED_SALI || kd == CL_RED_RLI) ? 2 : 0;
   }
   return cols + 3 + s;
 }

@@ -61,8 +61,10 @@ int record_rows_launch(int64_t m, int s, int k, int benettin, int g, int G, cons
 }
 
 // The indicator recorder's rows (src/Experiments/run_indicator_device.jl:13-29 and _put!, run_indicator_record.jl:39-70):
-// the columns of each member's final (FLI: fli, fli1; MEGNO: Y, Ybar, Ybar20 of column 1 then 2; SALI: sali, L), then
-// det_max, symp_max and M_max of RunMonitor, then the point. A window kind has no members (monitor and point only).
+// the columns of each member's final (FLI: fli, fli1; MEGNO: Y, Ybar, Ybar20 of column 1 then 2; SALI: sali, L; RLI:
+// DL, RLI), then
+// det_max, symp_max and M_max of RunMonitor, then the point (orbit A's state on a pair). A window kind has no members
+// (monitor and point only).
 // R is orbit-major, R[(i G + g) ncol + col]; states SoA by word. `off` is each member's first state word.
 __global__ void k_ind_rows(int64_t m, int s, IndRowDesc d, int g, int G, const double* __restrict__ x,
                            const uint64_t* __restrict__ st, const int8_t* __restrict__ code, int64_t* fail,
@@ -99,6 +101,15 @@ __global__ void k_ind_rows(int64_t m, int s, IndRowDesc d, int g, int G, const d
         row[col++] = Yb;
         row[col++] = Y20;
       }
+    } else if (d.kind[j] == CL_RED_RLI) {  // RLIState: sA, sB, Q, value, n, K  (rli.jl:116-119)
+      const int64_t n = iword(o + 7), K = iword(o + 8);
+      double dl = 0.0, rli = 0.0;
+      if (K != 0) {
+        dl = cl_fabs((dword(o + 2) - dword(o + 0)) + (dword(o + 3) - dword(o + 1))) / (double)n;
+        rli = (dword(o + 4) + dword(o + 5)) / (double)K;
+      }
+      row[col++] = dl;
+      row[col++] = rli;
     } else {  // SALIState: s1, s2, sali, value, n  (sali.jl:62-66)
       row[col++] = dword(o + 4);
       row[col++] = iword(o + 6) == 0 ? 0.0 : -((dword(o + 0) + dword(o + 2)) + (dword(o + 1) + dword(o + 3)));

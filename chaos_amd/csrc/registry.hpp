@@ -32,6 +32,18 @@ inline WrappedStdMap make_sys<WrappedStdMap>(const cl_system& s) {
 }
 
 template <>
+inline PairMap<StdMap> make_sys<PairMap<StdMap>>(const cl_system& s) {
+  PairMap<StdMap> m;
+  m.base.K = s.K;
+  return m;
+}
+template <>
+inline PairMap<WrappedStdMap> make_sys<PairMap<WrappedStdMap>>(const cl_system& s) {
+  PairMap<WrappedStdMap> m;
+  m.base.K = s.K;
+  return m;
+}
+template <>
 inline StdMapK make_sys<StdMapK>(const cl_system&) {
   return StdMapK();
 }
@@ -78,6 +90,10 @@ struct Kind<MEGNO<D>> {
 template <int D>
 struct Kind<SALI<D>> {
   static constexpr int kind = CL_RED_SALI, alg = 0;
+};
+template <int D>
+struct Kind<RLI<D>> {
+  static constexpr int kind = CL_RED_RLI, alg = 0;
 };
 template <int D>
 struct Kind<FLI<D>> {

@@ -274,15 +274,16 @@ def _record_shadow(sys, x0, N, tau, grid, u0, d0, launch_steps, timing):
 
 # ---------------------------------------------------------------- the indicator recorder (device kinds)
 # Columns of E24 per member (src/Experiments/run_indicator_chain.jl:16-17); RLI belongs to the pair kind.
-IND_MEMBER_COLUMNS = {"SALI": ("SALI.sali", "SALI.L"), "FLI": ("FLI.fli", "FLI.fli1"),
+IND_MEMBER_COLUMNS = {"SALI": ("SALI.sali", "SALI.L"), "FLI": ("FLI.fli", "FLI.fli1"), "RLI": ("RLI.DL", "RLI.RLI"),
                       "MEGNO": ("MEGNO.Y1", "MEGNO.Ybar1", "MEGNO.Ybar20_1", "MEGNO.Y2", "MEGNO.Ybar2",
                                 "MEGNO.Ybar20_2")}
 
 
-def ind_members(names):
-    """The members of a tangent chain in order (chain_members, run_indicator_chain.jl:27-31): every member but the
-    last non-replacing, the last replacing; SALI has one form (it always replaces)."""
-    from .reducers import MEGNO, FLI, SALI
+def ind_members(names, pair=False):
+    """The members of a tangent (or, with `pair`, a pair) chain in order (chain_members, run_indicator_chain.jl:27-31):
+    every member but the last non-replacing, the last replacing; SALI has one form (it always replaces); RLI runs on
+    the pair kind only."""
+    from .reducers import MEGNO, FLI, SALI, RLI
     names = tuple(names)
     if not 1 <= len(names) <= 4:
         raise ValueError("a tangent chain has 1 to 4 members")
@@ -295,8 +296,10 @@ def ind_members(names):
             out.append(FLI(replace=last))
         elif nm == "MEGNO":
             out.append(MEGNO(replace=last))
+        elif nm == "RLI" and pair:
+            out.append(RLI(replace=last))
         else:
-            raise ValueError(f"the device tangent kind runs FLI, MEGNO and SALI, not {nm!r}")
+            raise ValueError(f"the device tangent kind runs FLI, MEGNO and SALI (and RLI on a pair), not {nm!r}")
     return tuple(out)
 
 
@@ -347,7 +350,9 @@ def record_indicators(sys, x0, N, *, members=("FLI", "MEGNO", "SALI"), reader=No
     """Record an indicator item on the GPU (ind_device_records, run_indicator_device.jl:164-197).
 
     The tangent kind (`reader` None): the chain is `members` (ind_members) then RunMonitor, on the frame `W0` (2 x 2,
-    or one per orbit; default deviation_frame(direction, 2)); each row holds the members' series columns. A window
+    or one per orbit; default deviation_frame(direction, 2)); each row holds the members' series columns. The pair
+    kind: `sys` a PairMap, x0 the pair states (pair_start), members with RLI (the study's spec RLI, MEGNO), W0 the
+    block frame (4 x 2; default pair_frame of the direction); the point columns hold orbit A's state. A window
     kind (`reader` a WeightedBirkhoff): the chain is RunMonitor then the reader, stepped with the constant deviation
     column (1, 0) reset at every handoff; its rows hold the monitor and the point, and its tables come from the
     reader's final state (finals_of on `states`). `sys` is a StandardMap (x0 of shape (m, 2)) or a
@@ -355,8 +360,12 @@ def record_indicators(sys, x0, N, *, members=("FLI", "MEGNO", "SALI"), reader=No
     grid steps and are split at `launch_steps` (a multiple of tau) when given."""
     import torch
     from .tangent import make_chain, _context
-    if not isinstance(sys, (StandardMap, WrappedStandardMap)):
-        raise ValueError("the indicator recorder's device path runs the standard map, lifted or wrapped")
+    from .systems import PairMap
+    pair = isinstance(sys, PairMap)
+    if not isinstance(sys, (StandardMap, WrappedStandardMap, PairMap)):
+        raise ValueError("the indicator recorder's device path runs the standard map or its pair, lifted or wrapped")
+    if pair and reader is not None:
+        raise ValueError("a window kind runs on the map itself, not a pair")
     N, tau = int(N), int(tau)
     window = reader is not None
     k = 1 if window else 2
@@ -378,28 +387,32 @@ def record_indicators(sys, x0, N, *, members=("FLI", "MEGNO", "SALI"), reader=No
         W0 = np.broadcast_to(np.array([[1.0], [0.0]]), (m, 2, 1))
         columns = ()
     else:
-        reds = ind_members(members) + (RunMonitor(sys),)
+        reds = ind_members(members, pair=pair) + (RunMonitor(sys),)
         columns = tuple(c for nm in members for c in IND_MEMBER_COLUMNS[nm])
+        d = sys.dimension
         if W0 is None:
-            W0 = deviation_frame(direction, 2)
+            from .systems import pair_frame
+            W0 = pair_frame(deviation_frame(direction, 1)[:, 0]) if pair else deviation_frame(direction, 2)
         W0 = np.asarray(W0, dtype=np.float64)
-        if W0.shape == (2, 2):
-            W0 = np.broadcast_to(W0, (m, 2, 2))
-        if W0.shape != (m, 2, 2):
-            raise ValueError("W0 must be 2 x 2 or one 2 x 2 frame per orbit")
+        if W0.shape == (d, 2):
+            W0 = np.broadcast_to(W0, (m, d, 2))
+        if W0.shape != (m, d, 2):
+            raise ValueError(f"W0 must be {d} x 2 or one {d} x 2 frame per orbit")
+    d = W0.shape[1]
+    ns = s // 2 if pair else s   # the point columns: orbit A's state on a pair
     chain, allr = make_chain(reds, reset_frame=window)
     lib, ctx = _context()
     sysd = sys.descriptor()
-    nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, 2, k)
+    nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, d, k)
     if nbytes < 0:
         raise L.HIPError(-nbytes, "no kernel instance for this chain")
     words = nbytes // 8
-    ncol = lib.cl_ind_row_width(ctypes.byref(chain), s)
-    assert ncol == len(columns) + 3 + s
+    ncol = lib.cl_ind_row_width(ctypes.byref(chain), ns)
+    assert ncol == len(columns) + 3 + ns
     dev = torch.device("cuda")
     stream = torch.cuda.current_stream().cuda_stream
     dx = torch.from_numpy(np.ascontiguousarray(x0.T)).to(dev)
-    dW = torch.from_numpy(np.ascontiguousarray(W0.transpose(2, 1, 0).reshape(2 * k, m))).to(dev)
+    dW = torch.from_numpy(np.ascontiguousarray(W0.transpose(2, 1, 0).reshape(d * k, m))).to(dev)
     dst = torch.zeros((words, m), dtype=torch.int64, device=dev)
     dcode = torch.zeros(m, dtype=torch.int8, device=dev)
     dh = torch.zeros(m, dtype=torch.int64, device=dev)
@@ -413,7 +426,7 @@ def record_indicators(sys, x0, N, *, members=("FLI", "MEGNO", "SALI"), reader=No
         target = int(grid[g])
         while n < target:
             steps = target - n if launch_steps is None else min(int(launch_steps), target - n)
-            rc = lib.cl_propagate_device(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, m, s, 2, k, n,
+            rc = lib.cl_propagate_device(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, m, s, d, k, n,
                                          steps, N, tau, dx.data_ptr(), dW.data_ptr(), None, dst.data_ptr(),
                                          dcode.data_ptr(), dh.data_ptr(), stream)
             if rc:
@@ -429,7 +442,7 @@ def record_indicators(sys, x0, N, *, members=("FLI", "MEGNO", "SALI"), reader=No
     if timing is not None:
         timing["device_ms"] = ev0.elapsed_time(ev1)
     raw = dst.cpu().numpy().T.copy().view(np.uint8).reshape(m, words * 8)
-    return IndRecords(kind="birkhoff" if window else "tangent", columns=columns, grid=grid, R=dR.cpu().numpy(),
-                      fail=dfail.cpu().numpy(), code=dcode.cpu().numpy(), hstep=dh.cpu().numpy(),
-                      x=dx.cpu().numpy().T.copy(), W=dW.cpu().numpy().reshape(k, 2, m).transpose(2, 1, 0).copy(),
-                      states=raw, reducers=allr, s=s)
+    return IndRecords(kind="birkhoff" if window else ("pair" if pair else "tangent"), columns=columns, grid=grid,
+                      R=dR.cpu().numpy(), fail=dfail.cpu().numpy(), code=dcode.cpu().numpy(), hstep=dh.cpu().numpy(),
+                      x=dx.cpu().numpy().T.copy(), W=dW.cpu().numpy().reshape(k, d, m).transpose(2, 1, 0).copy(),
+                      states=raw, reducers=allr, s=ns)

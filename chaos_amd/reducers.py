@@ -180,6 +180,43 @@ class MEGNO(Reducer):
         return dict(indicator="megno", convention="weights at the end of the window, mean over the handoffs 1..K")
 
 
+class RLI(Reducer):
+    """RLI(; replace=True) for k = 2 on a pair system (src/Indicators/rli.jl:1-140): the two columns' log sums (A, B),
+    DL = |S_B - S_A| / n and the compensated sum Q of the DL series; RLI = Q / K."""
+    kind = L.CL_RED_RLI
+
+    def __init__(self, replace=True):
+        self.replace = bool(replace)
+
+    def check(self, d, k):
+        if k != 2 or d % 2:
+            raise ValueError(f"RLI needs k = 2 on an even dimension, got d = {d}, k = {k}")
+
+    def dtype(self, d, k):
+        return np.dtype([("sA", "<f8", 2), ("sB", "<f8", 2), ("Q", "<f8", 2), ("value", "<f8"), ("n", "<i8"),
+                         ("K", "<i8"), ("step", "<i8"), ("code", "i1"), ("column", "i1"), ("pad", "u1", 6)])
+
+    def final(self, st):
+        """(n, handoffs, DL, RLI, sum, comp, Q), rli.jl:116-119."""
+        sA, sB = st["sA"], st["sB"]
+        K = st["K"]
+        zero = K == 0
+        n = np.where(zero, 1, st["n"]).astype(np.float64)
+        dl = np.where(zero, 0.0, np.abs((sB[:, 0] - sA[:, 0]) + (sB[:, 1] - sA[:, 1])) / n)
+        Q = st["Q"][:, 0] + st["Q"][:, 1]
+        rli = np.where(zero, 0.0, Q / np.where(zero, 1, K))
+        return dict(n=st["n"].copy(), handoffs=K.copy(), DL=dl, RLI=rli,
+                    sum=np.stack([sA[:, 0], sB[:, 0]], axis=1), comp=np.stack([sA[:, 1], sB[:, 1]], axis=1), Q=Q)
+
+    def error(self, row):
+        v = float(row["value"])
+        what = "pairing" if int(row["code"]) == 2 else "deviation matrix"
+        return what, v, 0.0 if np.isfinite(v) else np.inf, int(row["step"]), int(row["column"])
+
+    def provenance(self):
+        return dict(indicator="rli", convention="same xi_0, shadow along xi_0, mean over the handoffs 1..K")
+
+
 class SALI(Reducer):
     """SALI() for k = 2, replaces W by unit columns (src/Indicators/sali.jl:147-201)."""
     kind = L.CL_RED_SALI

@@ -149,3 +149,88 @@ def torus_state(z):
     x, m = _reduce(float(z[0]), TWO_PI)
     p, n = _reduce(float(z[1]), TWO_PI)
     return (x, p, m, n)
+
+
+class PairMap:
+    """PairMap(sys): the product of a standard map with itself (src/Systems/pair.jl:1-77), state (state A; state B),
+    dimension 4; each half steps by the base map and the tangent acts block by block. RunMonitor reads orbit A."""
+    dimension = 4
+
+    def __init__(self, sys):
+        if not isinstance(sys, (StandardMap, WrappedStandardMap)):
+            raise ValueError("the pair kernels run the standard map, lifted or wrapped")
+        self.sys = sys
+        self.K = sys.K
+        self.state_length = 2 * sys.state_length
+
+    def descriptor(self):
+        s = L.cl_system()
+        s.kind = L.CL_SYS_PAIR_WRAPPED_STANDARD_MAP if isinstance(self.sys, WrappedStandardMap) \
+            else L.CL_SYS_PAIR_STANDARD_MAP
+        s.K = self.K
+        return s
+
+    def __repr__(self):
+        return f"PairMap({self.sys!r})"
+
+
+def _wrap_diff(u, tp):
+    """Minimal image of a coordinate difference (rli.jl:145): u - tp round(u / tp), ties to even."""
+    return u - tp * round(u / tp)
+
+
+def _safe_norm2(a, b):
+    m = max(abs(a), abs(b))
+    if not (math.isfinite(m) and m > 0):
+        return m
+    o = b if abs(a) >= abs(b) else a
+    r = o / m
+    return m * math.sqrt(1.0 + r * r)
+
+
+def pair_distance(sys, z):
+    """safe_norm of the minimal-image separation of the two halves of a pair state (rli.jl:156-166)."""
+    h = len(z) // 2
+    return _safe_norm2(_wrap_diff(float(z[h]) - float(z[0]), TWO_PI), _wrap_diff(float(z[h + 1]) - float(z[1]), TWO_PI))
+
+
+class SeparationError(ValueError):
+    """NumericalHealthError("separation", value, threshold, 0, orbit) of pair_start (rli.jl:189-202)."""
+
+    def __init__(self, value, threshold, orbit=0):
+        self.value, self.threshold, self.orbit = value, threshold, orbit
+        super().__init__(f"separation: value {value!r}, threshold {threshold!r}, step 0, orbit {orbit}")
+
+
+def pair_start(sys, x0, u, Dx, orbit=0):
+    """Start of a pair on the base map `sys` (pair_start, rli.jl:180-202): state A = x0 (torus_state(x0) for the
+    wrapped map), B = A + Dx u (reduced with A's counts when wrapped); the realized separation must be finite,
+    positive and within [Dx/2, 2 Dx]. Returns (z, Dx, Dx_r, mode)."""
+    dx = float(Dx)
+    if not (math.isfinite(dx) and dx > 0):
+        raise SeparationError(dx, 0.0 if math.isfinite(dx) else math.inf, orbit)
+    wrapped = isinstance(sys, WrappedStandardMap)
+    zA = torus_state(x0) if wrapped else (float(x0[0]), float(x0[1]))
+    d0, d1 = dx * float(u[0]), dx * float(u[1])
+    if wrapped:
+        x, a = _reduce(zA[0] + d0, TWO_PI)
+        p, b = _reduce(zA[1] + d1, TWO_PI)
+        zB = (x, p, zA[2] + a, zA[3] + b)
+    else:
+        zB = (zA[0] + d0, zA[1] + d1)
+    z = np.array(tuple(zA) + tuple(zB), dtype=np.float64)
+    dr = pair_distance(sys, z)
+    if not (math.isfinite(dr) and dr > 0):
+        raise SeparationError(dr, 0.0 if math.isfinite(dr) else math.inf, orbit)
+    if not (dx / 2 <= dr <= 2 * dx):
+        raise SeparationError(dr, dx, orbit)
+    return z, dx, dr, "wrapped" if wrapped else "lifted"
+
+
+def pair_frame(xi):
+    """The block frame of RLI (rli.jl:12-14): column 1 (xi; 0) for orbit A, column 2 (0; xi) for orbit B."""
+    W = np.zeros((4, 2))
+    W[0:2, 0] = xi
+    W[2:4, 1] = xi
+    return W
+

@@ -54,6 +54,8 @@ extern "C" {
 #define CL_SYS_HENON_HEILES_GAUSS 4    /* IntegratorMap(Gauss(Float64, s; tol), HenonHeilesFlow(), h), src/Integrators/integrator_map.jl:12-23 */
 #define CL_SYS_STANDARD_MAP_K 5        /* StandardMap with its K carried per orbit: state (x, p, K); no reference
                                           counterpart (the reference holds one K per system, standard_map.jl:8-10) */
+#define CL_SYS_PAIR_STANDARD_MAP 6     /* PairMap(StandardMap(K)), src/Systems/pair.jl:1-77: state (A; B), s = 4, d = 4 */
+#define CL_SYS_PAIR_WRAPPED_STANDARD_MAP 7 /* PairMap(WrappedStandardMap(K)): s = 8, d = 4 */
 
 typedef struct {
   int32_t kind;     /* CL_SYS_* */
@@ -82,6 +84,7 @@ typedef struct {
                                 UInt32 workspace of 2 (bins + 2) counts per orbit, src/Indicators/stretching.jl:34-147 */
 #define CL_RED_BIRKHOFF 12   /* WeightedBirkhoff(; window, observables, birkhoff_order), any k, the last reducer of its
                                 chain, src/Indicators/wbirkhoff_reducer.jl:4-116; parameters in cl_reducer.birkhoff */
+#define CL_RED_RLI 13        /* RLI(; replace), k = 2 on a pair system, src/Indicators/rli.jl:1-140 */
 
 /* Observables of WeightedBirkhoff (src/Indicators/wbirkhoff.jl:17-77), coordinates 0-based:
  * CL_OBS_COS cos(b x[a]), CL_OBS_SIN sin(b x[a]) (b the integer frequency), CL_OBS_LIFTED x[a] + 2 pi x[b]. */

@@ -163,6 +163,7 @@ static int red_bytes(const cl_reducer& r, int d, int k) {
     case CL_RED_BENETTIN: return 48;
     case CL_RED_MEGNO: return 200;
     case CL_RED_SALI: return 72;
+    case CL_RED_RLI: return 88;
     case CL_RED_FLI: return 96;
     case CL_RED_OFLI: return 16 * k + 120;
     case CL_RED_GALI: return 24 * k + 32;
@@ -204,6 +205,13 @@ struct MSums {
 template <class T>
 struct MegSt {
   MSums<T> c[2];
+  T value;
+  int64_t n, K, step;
+  int8_t code, column;
+};
+template <class T>
+struct RliSt {  // RLIState (rli.jl:53-63)
+  Acc<T> sA, sB, Q;
   T value;
   int64_t n, K, step;
   int8_t code, column;
@@ -272,6 +280,7 @@ struct RState {  // one of the above, by kind
   BenSt<T> ben;
   MegSt<T> meg;
   SaliSt<T> sali;
+  RliSt<T> rli;
   FliSt<T> fli;
   OfliSt<T> ofli;
   GaliSt<T> gali;
@@ -383,6 +392,23 @@ static void write_state(const cl_reducer& r, const RState<T>& s, int d, int k, u
           b.put(dbl(a[j]->c));
         }
       }
+      b.put(dbl(q.value));
+      b.put(q.n);
+      b.put(q.K);
+      b.put(q.step);
+      b.put(q.code);
+      b.put(q.column);
+      b.pad(6);
+      break;
+    }
+    case CL_RED_RLI: {
+      const RliSt<T>& q = s.rli;
+      b.put(dbl(q.sA.s));
+      b.put(dbl(q.sA.c));
+      b.put(dbl(q.sB.s));
+      b.put(dbl(q.sB.c));
+      b.put(dbl(q.Q.s));
+      b.put(dbl(q.Q.c));
       b.put(dbl(q.value));
       b.put(q.n);
       b.put(q.K);
@@ -787,6 +813,45 @@ static void red_megno(const cl_reducer& r, MegSt<T>& st, int64_t n, T* W, int d)
     }
 }
 
+// RLI on a pair (rli.jl:77-110): both columns' scaled norms and logs (Benettin's operations), the off-block check,
+// DL = |(sB.s - sA.s) + (sB.c - sA.c)| / n from the new sums into the compensated Q, the replacement.
+template <class P, class T>
+static void red_rli(const cl_reducer& r, RliSt<T>& st, int64_t n, T* W, int d) {
+  if (st.code) return;
+  T a1 = snorm<P>(W, d), a2 = snorm<P>(W + d, d);
+  const bool ok1 = P::finite(a1) && a1 > 0, ok2 = P::finite(a2) && a2 > 0;
+  if (!(ok1 && ok2)) {
+    st.column = ok1 ? 2 : 1;
+    st.value = ok1 ? a2 : a1;
+    st.step = n;
+    st.code = 1;
+    return;
+  }
+  const int h = d / 2;
+  T m1 = 0, m2 = 0;
+  for (int i = 0; i < h; ++i) {
+    m1 = jmax<P>(m1, P::abs(W[h + i]));
+    m2 = jmax<P>(m2, P::abs(W[d + i]));
+  }
+  if (!(m1 == 0 && m2 == 0)) {
+    st.column = m1 == 0 ? 2 : 1;
+    st.value = m1 == 0 ? m2 : m1;
+    st.step = n;
+    st.code = 2;
+    return;
+  }
+  st.sA = acc_add<P>(st.sA, P::log(a1));
+  st.sB = acc_add<P>(st.sB, P::log(a2));
+  st.Q = acc_add<P>(st.Q, P::abs((st.sB.s - st.sA.s) + (st.sB.c - st.sA.c)) / (T)n);
+  st.n = n;
+  st.K += 1;
+  if (r.replace)
+    for (int i = 0; i < d; ++i) {
+      W[i] = W[i] / a1;
+      W[d + i] = W[d + i] / a2;
+    }
+}
+
 // StretchingNumbers (stretching.jl:119-138): the column logs, the cell of z = a / tau in the orbit's workspace
 // (spectrum_cell, :104-108; column j at offset (j - 1)(bins + 2)), the sums of a and a^2, the replacement.
 template <class P, class T>
@@ -1079,6 +1144,7 @@ static bool red_failed(const cl_reducer& r, const RState<T>& s) {
     case CL_RED_BENETTIN: return s.ben.code != 0;
     case CL_RED_MEGNO: return s.meg.code != 0;
     case CL_RED_SALI: return s.sali.code != 0;
+    case CL_RED_RLI: return s.rli.code != 0;
     case CL_RED_FLI: return s.fli.code != 0;
     case CL_RED_OFLI: return s.ofli.code != 0;
     case CL_RED_GALI: return s.gali.code != 0;
@@ -1158,6 +1224,7 @@ struct MapSys {
       }
     }
   }
+  int jd() const { return 2; }  // the size of the one-step tangent the monitors read
   // the one-step tangent at x, column-major: [[1 + K cos x, 1], [K cos x, 1]] (standard_map.jl:30-34)
   void jac(const T* x, T* J) const {
     if (kind == OR_SYS_LINEAR) {
@@ -1180,6 +1247,36 @@ struct MapSys {
 };
 
 
+// The product of a map with itself (src/Systems/pair.jl:1-77): state (A; B), each half stepped by the base map with
+// its own block of the frame's rows (1..2 for A, 3..4 for B), the monitor reading orbit A (run_indicator_record.jl:6-10).
+template <class P, class T>
+struct PairSys {
+  MapSys<P, T> base;
+  int s;  // the pair's state length, twice the base's
+  int jd() const { return 2; }
+  void step(T* x, T* const* W, int nf, int k) const {
+    T wa[2][32], wb[2][32];
+    T* pa[2] = {wa[0], wa[1]};
+    T* pb[2] = {wb[0], wb[1]};
+    for (int f = 0; f < nf; ++f)
+      for (int j = 0; j < k; ++j)
+        for (int r = 0; r < 2; ++r) {
+          wa[f][2 * j + r] = W[f][4 * j + r];
+          wb[f][2 * j + r] = W[f][4 * j + 2 + r];
+        }
+    base.step(x, pa, nf, k);
+    base.step(x + base.s, pb, nf, k);
+    for (int f = 0; f < nf; ++f)
+      for (int j = 0; j < k; ++j)
+        for (int r = 0; r < 2; ++r) {
+          W[f][4 * j + r] = wa[f][2 * j + r];
+          W[f][4 * j + 2 + r] = wb[f][2 * j + r];
+        }
+  }
+  void jac(const T* x, T* J) const { base.jac(x, J); }
+  void field(const T* x, T* f) const { base.field(x, f); }
+};
+
 // ------------------------------------------------------------------ Henon-Heiles under Gauss s = 3
 // IntegratorMap(Gauss(Float64, 3; tol), HenonHeilesFlow(), h) (src/Integrators/integrator_map.jl:12-23):
 // fixed-point stage iteration with compensated sums (src/Integrators/gauss.jl:90-141), increment
@@ -1199,6 +1296,7 @@ static const double OR_GB_LO[3] = {-0x1.c71c71c71c71cp-57, 0x1.c71c71c71c71cp-56
 template <class P, class T>
 struct HHSys {
   int s = 4, d = 4;
+  int jd() const { return 4; }
   T h, tol;  // tol NaN: 8 eps max(1, |Y|)
   int maxiter;
   static void vfield(const T* z, T* f) {
@@ -1511,6 +1609,7 @@ static int8_t run_orbit_seg(const SYS& sys, const cl_chain& ch, int d, int k, in
         }
         case CL_RED_MEGNO: red_megno<P>(rd, st[r].meg, n, w, d); break;
         case CL_RED_SALI: red_sali<P>(st[r].sali, n, w, d); break;
+        case CL_RED_RLI: red_rli<P>(rd, st[r].rli, n, w, d); break;
         case CL_RED_FLI: red_fli<P>(rd, st[r].fli, n, w, d); break;
         case CL_RED_OFLI: {
           T f[8];
@@ -1521,7 +1620,7 @@ static int8_t run_orbit_seg(const SYS& sys, const cl_chain& ch, int d, int k, in
         case CL_RED_GALI: red_gali<P>(st[r].gali, n, w, d, k); break;
         case 8: red_lognorm<P>(st[r].ln, w, d, k); break;
         case CL_RED_MONITOR: red_monitor<P>(rd, st[r].mon, n, sys, x, w, d, k); break;
-        case CL_RED_RUNMONITOR: red_runmonitor<P>(st[r].mon, sys, x, d); break;
+        case CL_RED_RUNMONITOR: red_runmonitor<P>(st[r].mon, sys, x, sys.jd()); break;
         case CL_RED_STRETCHING: red_stretch<P>(rd, st[r].sn, n, w, d, ws); break;
         case CL_RED_BIRKHOFF: red_wbirk<P>(rd, st[r].wb, n, x); break;
       }
@@ -1550,6 +1649,13 @@ static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, in
   ms.K = (T)sys->K;
   ms.s = s;
   for (int i = 0; i < 4; ++i) ms.A[i] = A ? (T)A[i] : (T)0;
+  const bool pair = sys->kind == CL_SYS_PAIR_STANDARD_MAP || sys->kind == CL_SYS_PAIR_WRAPPED_STANDARD_MAP;
+  PairSys<P, T> ps;
+  ps.base.kind = sys->kind == CL_SYS_PAIR_STANDARD_MAP ? CL_SYS_STANDARD_MAP : CL_SYS_WRAPPED_STANDARD_MAP;
+  ps.base.K = (T)sys->K;
+  ps.base.s = s / 2;
+  for (int i = 0; i < 4; ++i) ps.base.A[i] = (T)0;
+  ps.s = s;
   HHSys<P, T> hs;
   hs.h = (T)sys->h;
   hs.tol = (T)sys->tol;
@@ -1568,7 +1674,7 @@ static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, in
       if (lo >= m) break;
       int64_t hi = lo + 16 < m ? lo + 16 : m;
       for (int64_t i = lo; i < hi; ++i) {
-        T xx[4], w0[64], w1[64];
+        T xx[8], w0[64], w1[64];
         for (int c = 0; c < s; ++c) xx[c] = (T)x[i * s + c];
         for (int e = 0; e < d * k; ++e) {
           w0[e] = (T)W[i * d * k + e];
@@ -1577,7 +1683,8 @@ static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, in
         uint32_t* wsi = (ws && nws > 0) ? ws + i * nws : nullptr;
         code[i] = sys->kind == CL_SYS_HENON_HEILES_GAUSS
                       ? run_orbit<P, T>(hs, *ch, d, k, N, tau, xx, w0, w1, st.data(), &hstep[i], wsi)
-                      : run_orbit<P, T>(ms, *ch, d, k, N, tau, xx, w0, w1, st.data(), &hstep[i], wsi);
+                  : pair ? run_orbit<P, T>(ps, *ch, d, k, N, tau, xx, w0, w1, st.data(), &hstep[i], wsi)
+                         : run_orbit<P, T>(ms, *ch, d, k, N, tau, xx, w0, w1, st.data(), &hstep[i], wsi);
         for (int c = 0; c < s; ++c) x[i * s + c] = dbl(xx[c]);
         for (int e = 0; e < d * k; ++e) {
           W[i * d * k + e] = dbl(w0[e]);
@@ -1641,6 +1748,8 @@ static void record_row(const cl_chain* ch, int k, int s, int g, const RState<dou
 // fli1, fli.jl:91-94; MEGNO: megno_series of column 1 then 2, megno.jl:97-102, with tn^3 the literal power
 // (tn tn) tn; SALI: sali and L, sali.jl:62-66), then det_max, symp_max, M_max, then the point. A window kind's chain
 // is RunMonitor then its reader, and the row holds the monitor and the point.
+static bool pair_kind(int kind) { return kind == CL_SYS_PAIR_STANDARD_MAP || kind == CL_SYS_PAIR_WRAPPED_STANDARD_MAP; }
+
 static void ind_row(const cl_chain* ch, int s, const RState<double>* st, const double* x, double* row) {
   const bool window = ch->r[0].kind == CL_RED_RUNMONITOR;
   const int nmem = window ? 0 : ch->n - 1;
@@ -1663,6 +1772,15 @@ static void ind_row(const cl_chain* ch, int s, const RState<double>* st, const d
         row[col++] = yb;
         row[col++] = y20;
       }
+    } else if (ch->r[r].kind == CL_RED_RLI) {  // rli.jl:116-119
+      const RliSt<double>& q = st[r].rli;
+      double dl = 0.0, rli = 0.0;
+      if (q.K != 0) {
+        dl = F64Libm::abs((q.sB.s - q.sA.s) + (q.sB.c - q.sA.c)) / (double)q.n;
+        rli = (q.Q.s + q.Q.c) / (double)q.K;
+      }
+      row[col++] = dl;
+      row[col++] = rli;
     } else {
       const SaliSt<double>& q = st[r].sali;
       row[col++] = q.sali;
@@ -1691,12 +1809,17 @@ template <class P>
 static int record_all(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int k, int64_t N, int tau,
                       const int64_t* grid, int G, double* x, double* W, double* R, int64_t* fail, int8_t* code,
                       int64_t* hstep, uint8_t* states, int nthreads, bool ind = false) {
-  const int d = 2, ncol = ind ? ind_row_width(ch, s) : k + 7 + s;
+  const int ncol = ind ? ind_row_width(ch, pair_kind(sys->kind) ? s / 2 : s) : k + 7 + s;
+  const bool pair = sys->kind == CL_SYS_PAIR_STANDARD_MAP || sys->kind == CL_SYS_PAIR_WRAPPED_STANDARD_MAP;
   MapSys<P, double> ms;
-  ms.kind = sys->kind;
+  ms.kind = pair ? (sys->kind == CL_SYS_PAIR_STANDARD_MAP ? CL_SYS_STANDARD_MAP : CL_SYS_WRAPPED_STANDARD_MAP) : sys->kind;
   ms.K = sys->K;
-  ms.s = s;
+  ms.s = pair ? s / 2 : s;
   for (int i = 0; i < 4; ++i) ms.A[i] = 0.0;
+  PairSys<P, double> ps;
+  ps.base = ms;
+  ps.s = s;
+  const int d = pair ? 4 : 2;
   int sb = 0, off[CL_MAX_REDUCERS];
   for (int r = 0; r < ch->n; ++r) {
     off[r] = sb;
@@ -1710,7 +1833,7 @@ static int record_all(const cl_system* sys, const cl_chain* ch, int64_t m, int s
       if (lo >= m) break;
       int64_t hi = lo + 16 < m ? lo + 16 : m;
       for (int64_t i = lo; i < hi; ++i) {
-        double xx[4], w0[4], w1[4] = {0, 0, 0, 0};
+        double xx[8], w0[8], w1[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int c = 0; c < s; ++c) xx[c] = x[i * s + c];
         for (int e = 0; e < d * k; ++e) w0[e] = W[i * d * k + e];
         for (int r = 0; r < ch->n; ++r) init_state(ch->r[r], st[r], k);
@@ -1718,14 +1841,15 @@ static int record_all(const cl_system* sys, const cl_chain* ch, int64_t m, int s
         int8_t c = 0;
         fail[i] = 0;
         for (int g = 0; g < G; ++g) {
-          c = run_orbit_seg<P>(ms, *ch, d, k, n, grid[g] - n, N, tau, xx, w0, w1, st, &h);
+          c = pair ? run_orbit_seg<P>(ps, *ch, d, k, n, grid[g] - n, N, tau, xx, w0, w1, st, &h, nullptr)
+                   : run_orbit_seg<P>(ms, *ch, d, k, n, grid[g] - n, N, tau, xx, w0, w1, st, &h, nullptr);
           if (c != 0) {
             fail[i] = g + 1;
             break;
           }
           n = grid[g];
           double* row = R + ((size_t)i * G + g) * ncol;
-          if (ind) ind_row(ch, s, st, xx, row);
+          if (ind) ind_row(ch, pair ? s / 2 : s, st, xx, row);
           else record_row<P>(ch, k, s, g, st, xx, row, g > 0 ? row - ncol : nullptr);
         }
         code[i] = c;
@@ -1977,7 +2101,8 @@ static void shadow_record_all(const cl_system* sys, int64_t m, int s, int tau, d
 static bool or_dims_ok(const cl_system* sys, int d, int k) {
   int want = sys->kind == CL_SYS_HENON_HEILES_GAUSS ? 4
            : (sys->kind == CL_SYS_STANDARD_MAP || sys->kind == CL_SYS_WRAPPED_STANDARD_MAP ||
-              sys->kind == CL_SYS_STANDARD_MAP_K) ? 2 : -1;
+              sys->kind == CL_SYS_STANDARD_MAP_K) ? 2
+           : (sys->kind == CL_SYS_PAIR_STANDARD_MAP || sys->kind == CL_SYS_PAIR_WRAPPED_STANDARD_MAP) ? 4 : -1;
   return d == want && k >= 1 && k <= d;
 }
 
@@ -2097,14 +2222,17 @@ int or_record(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int k,
 int or_ind_record(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int k, int64_t N, int tau,
                   const int64_t* grid, int G, double* x, double* W, double* R, int64_t* fail, int8_t* code,
                   int64_t* hstep, void* states, int math, int nthreads) {
-  const bool sys_ok = (sys->kind == CL_SYS_STANDARD_MAP && s == 2) || (sys->kind == CL_SYS_WRAPPED_STANDARD_MAP && s == 4);
+  const bool sys_ok = (sys->kind == CL_SYS_STANDARD_MAP && s == 2) || (sys->kind == CL_SYS_WRAPPED_STANDARD_MAP && s == 4) ||
+                      (sys->kind == CL_SYS_PAIR_STANDARD_MAP && s == 4) ||
+                      (sys->kind == CL_SYS_PAIR_WRAPPED_STANDARD_MAP && s == 8);
   bool ch_ok = ch->n >= 2;
   if (ch_ok && ch->r[0].kind == CL_RED_RUNMONITOR) {
     ch_ok = ch->n == 2 && ch->r[1].kind == CL_RED_BIRKHOFF && k == 1 && (ch->flags & CL_CHAIN_RESET_FRAME);
   } else if (ch_ok) {
     ch_ok = k == 2 && ch->n <= 5 && ch->r[ch->n - 1].kind == CL_RED_RUNMONITOR;
     for (int r = 0; ch_ok && r + 1 < ch->n; ++r)
-      ch_ok = ch->r[r].kind == CL_RED_FLI || ch->r[r].kind == CL_RED_MEGNO || ch->r[r].kind == CL_RED_SALI;
+      ch_ok = ch->r[r].kind == CL_RED_FLI || ch->r[r].kind == CL_RED_MEGNO || ch->r[r].kind == CL_RED_SALI ||
+              (ch->r[r].kind == CL_RED_RLI && pair_kind(sys->kind));
   }
   if (!ch_ok || !sys_ok || N % tau || G < 1 || grid[G - 1] != N) return 10;
   for (int g = 0; g < G; ++g)

@@ -77,7 +77,8 @@ def run(sysd, chain, x0, W0, N, tau, k, W2=None, math=MATH_CL, nthreads=None, qu
     o = load()
     x = np.ascontiguousarray(np.array(x0, dtype=np.float64))
     m, s = x.shape
-    d = 4 if sysd.kind == L.CL_SYS_HENON_HEILES_GAUSS else 2
+    d = 4 if sysd.kind in (L.CL_SYS_HENON_HEILES_GAUSS, L.CL_SYS_PAIR_STANDARD_MAP,
+                           L.CL_SYS_PAIR_WRAPPED_STANDARD_MAP) else 2
     W = np.array(np.asarray(W0, dtype=np.float64).reshape(m, d, k).transpose(0, 2, 1), order="C", copy=True)
     W2a = None if W2 is None else np.array(np.asarray(W2, dtype=np.float64).reshape(m, d, k).transpose(0, 2, 1), order="C", copy=True)
     nb = o.or_state_bytes(ctypes.byref(chain), d, k)
@@ -168,12 +169,13 @@ def ind_record(sysd, chain, x0, W0, N, tau, k, grid, ncol, math=MATH_CL, nthread
     o = load()
     x = np.ascontiguousarray(np.array(x0, dtype=np.float64))
     m, s = x.shape
-    W = np.array(np.asarray(W0, dtype=np.float64).reshape(m, 2, k).transpose(0, 2, 1), order="C", copy=True)
+    d = 4 if sysd.kind in (L.CL_SYS_PAIR_STANDARD_MAP, L.CL_SYS_PAIR_WRAPPED_STANDARD_MAP) else 2
+    W = np.array(np.asarray(W0, dtype=np.float64).reshape(m, d, k).transpose(0, 2, 1), order="C", copy=True)
     g = np.ascontiguousarray(np.asarray(grid, dtype=np.int64))
     G = len(g)
     R = np.zeros((m, G, ncol))
     fail = np.zeros(m, dtype=np.int64)
-    nb = o.or_state_bytes(ctypes.byref(chain), 2, k)
+    nb = o.or_state_bytes(ctypes.byref(chain), d, k)
     states = np.zeros((m, max(nb, 1)), dtype=np.uint8)
     code = np.zeros(m, dtype=np.int8)
     hstep = np.zeros(m, dtype=np.int64)

@@ -82,3 +82,37 @@ def test_birkhoff_window_kind_equals_oracle(wrapped):
     assert np.array_equal(rec.R.view(np.uint64), orc["R"].view(np.uint64))
     assert np.array_equal(rec.states, orc["states"])
     assert rec.fail[4] == 1 and np.mean(rec.fail == 0) > 0.99
+
+
+@pytest.mark.parametrize("members", [("RLI", "MEGNO"), ("RLI",)])
+@pytest.mark.parametrize("wrapped", [False, True], ids=["lifted", "wrapped"])
+@pytest.mark.parametrize("tau,launch", [(1, None), (2, 50)])
+def test_pair_kind_equals_oracle(members, wrapped, tau, launch):
+    """The pair kind (PairMap + RLI, the study's spec RLI, MEGNO): the block frame stepped block by block, RLI's sums,
+    the monitor on orbit A, rows of orbit A's point; planted failures (a non-finite half, a broken block frame)."""
+    from chaos_amd import PairMap, pair_start, pair_frame
+    m, N = 800, 300
+    base = (WrappedStandardMap if wrapped else StandardMap)(6.0)
+    rng = np.random.default_rng(31 + tau)
+    u = np.array([math.cos(0.3), math.sin(0.3)])
+    z = np.array([pair_start(base, 2 * math.pi * rng.random(2), u, 1e-10)[0] for _ in range(m)])
+    s = base.state_length
+    z[3, s] = np.nan                              # orbit B not finite: fails at the first handoff
+    W = np.broadcast_to(pair_frame(u), (m, 4, 2)).copy()
+    W[5, 2, 0] = 1e-200                           # off-block entry: "pairing" at the first handoff
+    psys = PairMap(base)
+    grid = record_grid(N, tau, points_per_decade=8, first_steps=20, extra_fractions=(0.5,))
+    chain, _ = make_chain(ind_members(members, pair=True) + (RunMonitor(psys),))
+    ncol = sum(len(IND_MEMBER_COLUMNS[n]) for n in members) + 3 + s
+    orc = O.ind_record(psys.descriptor(), chain, z, W, N, tau, 2, grid, ncol)
+    say("ind pair", members, "wrapped" if wrapped else "lifted", "tau", tau, "launch", launch, "m", m, "N", N)
+    rec = record_indicators(psys, z, N, members=members, tau=tau, grid=grid, W0=W, launch_steps=launch)
+    assert rec.R.shape == orc["R"].shape
+    assert np.array_equal(rec.fail, orc["fail"]) and np.array_equal(rec.code, orc["code"])
+    assert np.array_equal(rec.hstep, orc["hstep"])
+    assert np.array_equal(rec.R.view(np.uint64), orc["R"].view(np.uint64))
+    assert np.array_equal(rec.states, orc["states"])
+    ok = rec.code == 0
+    assert np.array_equal(rec.x[ok].view(np.uint64), orc["x"][ok].view(np.uint64))
+    assert np.array_equal(rec.W[ok].view(np.uint64), orc["W"][ok].view(np.uint64))
+    assert rec.fail[3] == 1 and rec.fail[5] == 1 and np.mean(ok) > 0.99

@@ -147,3 +147,107 @@ def test_megno_ybar20_uses_the_literal_cube():
     f = r.final(st)
     t = float(n)
     assert f["Ybar20"][0, 0] == 1.0 * 1.0 / ((t * t) * t)
+
+
+# ---------------------------------------------------------------- the pair kind (PairMap + RLI)
+def pair_starts(base, m, seed, dx=1e-9):
+    from chaos_amd import pair_start
+    rng = np.random.default_rng(seed)
+    u = np.array([math.cos(0.7), math.sin(0.7)])
+    z = [pair_start(base, 2 * math.pi * rng.random(2), u, dx)[0] for _ in range(m)]
+    return np.array(z), u
+
+
+@pytest.mark.parametrize("wrapped", [False, True], ids=["lifted", "wrapped"])
+def test_oracle_rli_sums_are_the_single_orbit_benettin_sums(wrapped):
+    """rli.jl:14-18: RLI's log sums take Benettin's operations, so s_A and s_B are bitwise the single-orbit sums of
+    orbits A and B with the start direction xi_0; DL and RLI follow from them (rli.jl:74, 116-119)."""
+    from chaos_amd import PairMap, RLI, Benettin, MEGNO, pair_frame
+    m, N, tau = 40, 300, 3
+    base = (WrappedStandardMap if wrapped else StandardMap)(1.3)
+    z, u = pair_starts(base, m, 12)
+    s = base.state_length
+    psys = PairMap(base)
+    W = np.broadcast_to(pair_frame(u), (m, 4, 2))
+    for reds in ((RLI(),), (RLI(replace=False), MEGNO())):
+        chain, allr = make_chain(reds)
+        r = O.run(psys.descriptor(), chain, z, W, N, tau, 2)
+        assert np.all(r["code"] == 0)
+        st = decode_states(r["states"], allr, 4, 2)[0]
+        bchain, ball = make_chain((Benettin(),))
+        u1 = np.broadcast_to(u.reshape(2, 1), (m, 2, 1))
+        for half, key in ((slice(0, s), "sA"), (slice(s, 2 * s), "sB")):
+            b = O.run(base.descriptor(), bchain, z[:, half], u1, N, tau, 1)
+            bst = decode_states(b["states"], ball, 2, 1)[0]
+            assert np.array_equal(st[key][:, 0], bst["sum"][:, 0]) and np.array_equal(st[key][:, 1], bst["sum"][:, 1])
+            assert np.array_equal(r["x"][:, half], b["x"])
+        f = RLI().final(st)
+        assert np.all(f["handoffs"] == N // tau) and np.all(f["DL"] >= 0) and np.all(np.isfinite(f["RLI"]))
+        # identical halves: DL and RLI exactly 0
+    zz = np.concatenate([z[:, :s], z[:, :s]], axis=1)
+    chain, allr = make_chain((RLI(),))
+    r = O.run(psys.descriptor(), chain, zz, W, N, tau, 2)
+    f = RLI().final(decode_states(r["states"], allr, 4, 2)[0])
+    assert np.all(f["DL"] == 0.0) and np.all(f["RLI"] == 0.0)
+
+
+def test_oracle_rli_pairing_and_norm_failures():
+    """A frame with a nonzero off-block entry fails the first handoff as "pairing" (code 2, the column, the largest
+    off-block magnitude); a zero column as "deviation matrix" (code 1) (rli.jl:84-95)."""
+    from chaos_amd import PairMap, RLI, pair_frame
+    base = StandardMap(1.0)
+    z, u = pair_starts(base, 3, 4)
+    W = np.broadcast_to(pair_frame(u), (3, 4, 2)).copy()
+    W[0, 2, 0] = 1e-300          # block B row of column 1
+    W[1, 1, 1] = -2.0            # block A row of column 2
+    W[2, :, 1] = 0.0             # column 2 vanishes
+    chain, allr = make_chain((RLI(),))
+    r = O.run(PairMap(base).descriptor(), chain, z, W, 10, 1, 2)
+    st = decode_states(r["states"], allr, 4, 2)[0]
+    assert list(r["code"]) == [3, 3, 3] and list(r["hstep"]) == [1, 1, 1]
+    assert list(st["code"]) == [2, 2, 1] and list(st["column"]) == [1, 2, 2]
+    assert st["value"][1] > 0 and st["value"][2] == 0.0
+
+
+def test_pair_start_separation_rule():
+    from chaos_amd import pair_start, SeparationError, pair_distance
+    base = WrappedStandardMap(1.0)
+    z, dx, dr, mode = pair_start(base, (6.2830, 0.1), (1.0, 0.0), 1e-3)
+    assert mode == "wrapped" and z[4] < 0.01 and z[6] == z[2] + 1   # B wrapped past 2 pi: its count steps
+    assert abs(dr - 1e-3) < 1e-12 and pair_distance(base, z) == dr
+    for bad in (0.0, -1e-9, float("nan"), float("inf")):
+        with pytest.raises(SeparationError):
+            pair_start(base, (1.0, 1.0), (1.0, 0.0), bad)
+    with pytest.raises(SeparationError):   # a lifted gap of 2 pi reads as 0 (the minimal image)
+        pair_start(StandardMap(1.0), (1.0, 1.0), (1.0, 0.0), 2 * math.pi)
+
+
+@pytest.mark.parametrize("wrapped", [False, True], ids=["lifted", "wrapped"])
+def test_oracle_pair_rows_equal_finals(wrapped):
+    from chaos_amd import PairMap, pair_frame
+    m, N, tau = 48, 200, 1
+    base = (WrappedStandardMap if wrapped else StandardMap)(6.0)
+    z, u = pair_starts(base, m, 9)
+    psys = PairMap(base)
+    members = ("RLI", "MEGNO")
+    reds = ind_members(members, pair=True) + (RunMonitor(psys),)
+    chain, allr = make_chain(reds)
+    grid = record_grid(N, tau, points_per_decade=6, first_steps=10, extra_fractions=(0.5,))
+    s = base.state_length
+    W = np.broadcast_to(pair_frame(u), (m, 4, 2))
+    rec = O.ind_record(psys.descriptor(), chain, z, W, N, tau, 2, grid, 2 + 6 + 3 + s)
+    assert np.all(rec["fail"] == 0)
+    for g, t in enumerate(grid):
+        r = O.run(psys.descriptor(), chain, z, W, int(t), tau, 2)
+        fin = finals_of(decode_states(r["states"], allr, 4, 2), allr, 4, 2)
+        rl, mg, mon = fin
+        cols = [rl["DL"], rl["RLI"]]
+        for c in range(2):
+            cols += [mg["Y"][:, c], mg["Ybar"][:, c], mg["Ybar20"][:, c]]
+        cols += [mon["det_max"], mon["symp_max"], mon["M_max"]] + [r["x"][:, j] for j in range(s)]
+        assert np.array_equal(rec["R"][:, g], np.stack(cols, axis=1))
+    # the monitor reads orbit A: its maxima equal a RunMonitor run on orbit A alone
+    mchain, mall = make_chain((RunMonitor(base),))
+    b = O.run(base.descriptor(), mchain, z[:, :s], np.broadcast_to(u.reshape(2, 1), (m, 2, 1)), N, tau, 1)
+    mon = decode_states(b["states"], mall, 2, 1)[0]
+    assert np.array_equal(rec["R"][:, -1, 8], mon["det_max"]) and np.array_equal(rec["R"][:, -1, 10], mon["M_max"])
