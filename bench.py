@@ -500,7 +500,24 @@ def rec_main(a):
 # st and bk of verify/experiments-indicators-block.toml:123-184; workspaces/module-build/stages/04-verify/output/
 # gpu-rtx5060-d86-verify-log.txt:24-29): 1024 wrapped orbits, ns per orbit-step on 23 CPU threads and its GPU (st is
 # CPU-only there).
-IND_REFERENCE_NS = {"st": (15.34, None), "bk": (29.29, 63.1), "tg": (45.14, 64.88)}
+IND_REFERENCE_NS = {"st": (15.34, None), "bk": (29.29, 63.1), "tg": (45.14, 64.88), "pr": (925.4, 267.6)}
+
+
+def ind_pr_case(m, N=300, K=2 * math.pi, seed=5):
+    """The indicator recorder's pair kind as the reference times it (gpu-rtx5060-d86-verify-log.txt:28, spec pr of
+    experiments-indicators-block.toml:109-115): PairMap of the wrapped map, members RLI, MEGNO then RunMonitor,
+    separation 1e-12 along the start direction, the study grid of 300 steps. Returns the recording's device time."""
+    from chaos_amd import WrappedStandardMap, PairMap, pair_start
+    from chaos_amd.inputs import torus_uniform
+    from chaos_amd.recorder import record_indicators
+    base = WrappedStandardMap(K)
+    u = np.array([math.cos(math.pi / 4), math.sin(math.pi / 4)])
+    z = np.array([pair_start(base, v, u, 1e-12)[0] for v in torus_uniform(seed, m)])
+    t = {}
+    t0 = time.perf_counter()
+    rec = record_indicators(PairMap(base), z, N, members=("RLI", "MEGNO"), tau=1, timing=t)
+    return dict(m=m, N=N, wall=time.perf_counter() - t0, kernel_ms=t["device_ms"], rows=len(rec.grid),
+                bad=int((rec.code != 0).sum()))
 
 
 def ind_tg_case(m, N=4000, K=2 * math.pi, seed=5):
@@ -601,11 +618,18 @@ def ind_main(a):
         res[("tg", m)] = r
         log(f"ind tg (recorded, {r['rows']} rows): {m} orbits x {r['N']} steps: {r['kernel_ms']:.2f} ms device, "
             f"{r['kernel_ms'] * 1e6 / (m * r['N']):.4f} ns per orbit-step, {r['bad']} failed")
+    ind_pr_case(256, N=60)   # warm-up
+    for m in (1024, 65536):
+        r = ind_pr_case(m)
+        res[("pr", m)] = r
+        log(f"ind pr (recorded, {r['rows']} rows): {m} orbits x {r['N']} steps: {r['kernel_ms']:.2f} ms device, "
+            f"{r['kernel_ms'] * 1e6 / (m * r['N']):.4f} ns per orbit-step, {r['bad']} failed")
     total = sum(res[(n, big)]["m"] * res[(n, big)]["N"] for n in ("st", "bk"))
     wall = sum(res[(n, big)]["wall"] for n in ("st", "bk"))
     value = total / wall
     per = {f"{n}@{m}": res[(n, m)]["kernel_ms"] * 1e6 / (m * res[(n, m)]["N"]) for n in ("st", "bk") for m in (1024, big)}
-    per.update({f"tg@{m}": res[("tg", m)]["kernel_ms"] * 1e6 / (m * res[("tg", m)]["N"]) for m in (1024, 65536)})
+    per.update({f"{n}@{m}": res[(n, m)]["kernel_ms"] * 1e6 / (m * res[(n, m)]["N"]) for n in ("tg", "pr")
+                for m in (1024, 65536)})
     cpu = None
     if not a.no_cpu:
         from tests import oracle_lib as O
@@ -638,7 +662,8 @@ def ind_main(a):
                                f"WeightedBirkhoff (windows 250/1000, order 2, 4 observables), k=1 constant column, "
                                f"2000 steps; {big} orbits each (and the reference's 1024); tg = the indicator "
                                f"recorder's tangent kind (FLI, MEGNO, SALI + RunMonitor, study grid rows), 4000 steps, "
-                               f"1024 and 65536 orbits (outside value)",
+                               f"1024 and 65536 orbits; pr = its pair kind (PairMap, RLI, MEGNO + RunMonitor), 300 "
+                               f"steps, 1024 and 65536 pairs (tg and pr outside value)",
                    "ns_per_orbit_step": per, "reference_ns_per_orbit_step_cpu23_gpu_rtx5060": IND_REFERENCE_NS,
                    "failed_orbits": {f"{n}@{m}": res[(n, m)]["bad"] for n in ("st", "bk") for m in (1024, big)}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
