@@ -318,9 +318,12 @@ CL_HD double cl_log(double x) {
   if (x < 0.0) return __builtin_nan("");
   if (x == 0.0) return -__builtin_inf();
   if (!cl_isfinite(x)) return x;
-  if (x < 0x1p-1022) /* subnormal */
-    return cl_log_normal(x * 0x1p54, -54);
-  return cl_log_normal(x, 0);
+  int k0 = 0;
+  if (x < 0x1p-1022) { /* subnormal */
+    x = x * 0x1p54;
+    k0 = -54;
+  }
+  return cl_log_normal(x, k0);
 }
 
 /* x is positive, normal and finite (cl_log_normal's domain) */
