@@ -500,11 +500,29 @@ def rec_main(a):
 # st and bk of verify/experiments-indicators-block.toml:123-184; workspaces/module-build/stages/04-verify/output/
 # gpu-rtx5060-d86-verify-log.txt:24-29): 1024 wrapped orbits, ns per orbit-step on 23 CPU threads and its GPU (st is
 # CPU-only there).
-IND_REFERENCE_NS = {"st": (15.34, None), "bk": (29.29, 63.1), "tg": (45.14, 64.88), "pr": (925.4, 267.6)}
+IND_REFERENCE_NS = {"st": (15.34, None), "bk": (29.29, 63.1), "tg": (45.14, 64.88), "pr": (925.4, 267.6),
+                    "pa": (29.34, 32.79)}
+
+
+def ind_pa_case(m, K=2 * math.pi, seed=5):
+    """The phase-advance window kind as the reference times it (gpu-rtx5060-d86-verify-log.txt:28, spec pa of
+    experiments-indicators-block.toml:139-149): RunMonitor then WeightedPhaseAdvance (windows 250 and 1000, order 2,
+    planar (q, p)) with the constant deviation column, 2001 steps, wrapped map. Returns the recording's device time."""
+    from chaos_amd import WrappedStandardMap, WeightedPhaseAdvance, PlanarObservable, torus_state
+    from chaos_amd.inputs import torus_uniform
+    from chaos_amd.recorder import record_indicators
+    sysm = WrappedStandardMap(K)
+    x0 = np.array([torus_state(v) for v in torus_uniform(seed, m)])
+    pa = WeightedPhaseAdvance(window=(250, 1000), observable=PlanarObservable(1, 2), birkhoff_order=2)
+    t = {}
+    t0 = time.perf_counter()
+    rec = record_indicators(sysm, x0, 2001, reader=pa, tau=1, timing=t)
+    return dict(m=m, N=2001, wall=time.perf_counter() - t0, kernel_ms=t["device_ms"], rows=len(rec.grid),
+                bad=int((rec.code != 0).sum()))
 
 
 def ind_pr_case(m, N=300, K=2 * math.pi, seed=5):
-    """The indicator recorder's pair kind as the reference times it (gpu-rtx5060-d86-verify-log.txt:28, spec pr of
+    """The indicator recorder's pair kind as the reference times it (gpu-rtx5060-d86-verify-log.txt:25, spec pr of
     experiments-indicators-block.toml:109-115): PairMap of the wrapped map, members RLI, MEGNO then RunMonitor,
     separation 1e-12 along the start direction, the study grid of 300 steps. Returns the recording's device time."""
     from chaos_amd import WrappedStandardMap, PairMap, pair_start
@@ -521,7 +539,7 @@ def ind_pr_case(m, N=300, K=2 * math.pi, seed=5):
 
 
 def ind_tg_case(m, N=4000, K=2 * math.pi, seed=5):
-    """The indicator recorder's tangent kind as the reference times it (gpu-rtx5060-d86-verify-log.txt:27, spec tg of
+    """The indicator recorder's tangent kind as the reference times it (gpu-rtx5060-d86-verify-log.txt:24, spec tg of
     experiments-indicators-block.toml:97-103): members FLI, MEGNO, SALI then RunMonitor, k = 2, the study grid of
     4000 steps, rows written at every grid step; wrapped map, tau = 1. Returns the device time of the recording."""
     from chaos_amd import WrappedStandardMap, torus_state
@@ -624,11 +642,17 @@ def ind_main(a):
         res[("pr", m)] = r
         log(f"ind pr (recorded, {r['rows']} rows): {m} orbits x {r['N']} steps: {r['kernel_ms']:.2f} ms device, "
             f"{r['kernel_ms'] * 1e6 / (m * r['N']):.4f} ns per orbit-step, {r['bad']} failed")
+    ind_pa_case(256)   # warm-up
+    for m in (1024, 65536):
+        r = ind_pa_case(m)
+        res[("pa", m)] = r
+        log(f"ind pa (recorded, {r['rows']} rows): {m} orbits x {r['N']} steps: {r['kernel_ms']:.2f} ms device, "
+            f"{r['kernel_ms'] * 1e6 / (m * r['N']):.4f} ns per orbit-step, {r['bad']} failed")
     total = sum(res[(n, big)]["m"] * res[(n, big)]["N"] for n in ("st", "bk"))
     wall = sum(res[(n, big)]["wall"] for n in ("st", "bk"))
     value = total / wall
     per = {f"{n}@{m}": res[(n, m)]["kernel_ms"] * 1e6 / (m * res[(n, m)]["N"]) for n in ("st", "bk") for m in (1024, big)}
-    per.update({f"{n}@{m}": res[(n, m)]["kernel_ms"] * 1e6 / (m * res[(n, m)]["N"]) for n in ("tg", "pr")
+    per.update({f"{n}@{m}": res[(n, m)]["kernel_ms"] * 1e6 / (m * res[(n, m)]["N"]) for n in ("tg", "pr", "pa")
                 for m in (1024, 65536)})
     cpu = None
     if not a.no_cpu:
@@ -663,7 +687,9 @@ def ind_main(a):
                                f"2000 steps; {big} orbits each (and the reference's 1024); tg = the indicator "
                                f"recorder's tangent kind (FLI, MEGNO, SALI + RunMonitor, study grid rows), 4000 steps, "
                                f"1024 and 65536 orbits; pr = its pair kind (PairMap, RLI, MEGNO + RunMonitor), 300 "
-                               f"steps, 1024 and 65536 pairs (tg and pr outside value)",
+                               f"steps, 1024 and 65536 pairs; pa = its phase-advance window kind (RunMonitor + "
+                               f"WeightedPhaseAdvance, windows 250/1000, order 2, planar), 2001 steps, 1024 and 65536 "
+                               f"orbits (tg, pr and pa outside value)",
                    "ns_per_orbit_step": per, "reference_ns_per_orbit_step_cpu23_gpu_rtx5060": IND_REFERENCE_NS,
                    "failed_orbits": {f"{n}@{m}": res[(n, m)]["bad"] for n in ("st", "bk") for m in (1024, big)}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -11,6 +11,7 @@ from .reducers import (QRSpectrum, CGS, MGS, RGS, HouseholderQR, AugmentedQR, Gi
 from .tangent import propagate, propagate_ragged, PropagationResult, NumericalHealthError, initial_deviations
 from .two_trajectory import propagate_shadow
 from .reducers import RLI
+from .birkhoff import WeightedPhaseAdvance, PlanarObservable, LiftedAngleObservable, circle_distance, frequency_index
 from .reducers import StretchingNumbers, stretching_range, spectral_distance, spectrum_density, spectrum_cell
 from .birkhoff import (WeightedBirkhoff, CosObservable, SinObservable, LiftedObservable, weighted_birkhoff,
                        birkhoff_index)

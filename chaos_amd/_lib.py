@@ -39,7 +39,9 @@ class cl_system(ctypes.Structure):
 
 CL_RED_BIRKHOFF = 12
 CL_RED_RLI = 13
+CL_RED_PHASE_ADVANCE = 14
 CL_OBS_COS, CL_OBS_SIN, CL_OBS_LIFTED = 1, 2, 3
+CL_OBS_PLANAR, CL_OBS_LIFTED_ANGLE = 4, 5
 CL_WB_MAX_WINDOWS, CL_WB_MAX_OBSERVABLES = 16, 8
 
 

@@ -259,3 +259,109 @@ class WeightedBirkhoff(Reducer):
                     birkhoff_order=self.birkhoff_order, window=self.window,
                     observables=tuple(observable_name(o) for o in self.observables),
                     handoffs=2 * max(self.window), floor_multiplier=WB_FLOOR_MULTIPLIER, min_window=WB_MIN_WINDOW)
+
+
+# ---------------------------------------------------------------- WeightedPhaseAdvance
+class PlanarObservable:
+    """PlanarObservable(iq, ip) (naff_reducer.jl:88-97): the sample x[iq] + i x[ip] (1-based indices), its increment the
+    principal argument of z' conj(z)."""
+
+    def __init__(self, iq, ip):
+        self.iq, self.ip = int(iq), int(ip)
+
+    def code(self):
+        return L.CL_OBS_PLANAR, self.iq - 1, self.ip - 1
+
+    def name(self):
+        return "planar"
+
+
+class LiftedAngleObservable:
+    """LiftedAngleObservable(iq, im=0) (naff_reducer.jl:99-112): a wrapped coordinate x[iq] and its winding count x[im]
+    (0: none), its increment (q' - q) + 2 pi (m' - m)."""
+
+    def __init__(self, iq, im=0):
+        self.iq, self.im = int(iq), int(im)
+
+    def code(self):
+        return L.CL_OBS_LIFTED_ANGLE, self.iq - 1, self.im - 1
+
+    def name(self):
+        return "lifted-angle"
+
+
+EPS64 = 2.0 ** -52
+
+
+def circle_distance(a, b):
+    """abs(d - round(d)), d = a - b (naff_map.jl:8), round half to even."""
+    d = np.asarray(a) - np.asarray(b)
+    return np.abs(d - np.round(d))
+
+
+def frequency_index(delta, floor_multiplier=32):
+    """(D, below_floor) of NF10 (naff_map.jl:26-29): log10(max(delta, 32 eps)), delta <= 32 eps."""
+    fl = float(floor_multiplier) * EPS64
+    delta = np.asarray(delta, dtype=np.float64)
+    return np.log10(np.maximum(delta, fl)), delta <= fl
+
+
+class WeightedPhaseAdvance(Reducer):
+    """WeightedPhaseAdvance(; window, observable, birkhoff_order=1) (src/Indicators/naff_reducer.jl:142-254): a reader of
+    the orbit point for any k; window N or a tuple of 1 or 2 N (each >= 16; the kernels take 2 at most), a run has
+    2 max(window) + 1 handoffs (check_run)."""
+    kind = L.CL_RED_PHASE_ADVANCE
+
+    def __init__(self, *, window, observable, birkhoff_order=1):
+        self.window = _windows(window)
+        if len(self.window) > 2:
+            raise ValueError("window: the kernels take 1 or 2 horizons")
+        if not isinstance(observable, (PlanarObservable, LiftedAngleObservable)):
+            raise ValueError("observable must be a PlanarObservable or a LiftedAngleObservable")
+        self.observable = observable
+        self.birkhoff_order = _order(birkhoff_order)
+        self._desc = L.cl_birkhoff()
+        self._desc.H, self._desc.K, self._desc.order = len(self.window), 1, self.birkhoff_order
+        for h, N in enumerate(self.window):
+            self._desc.window[h] = N
+        self._desc.kind[0], self._desc.a[0], self._desc.b[0] = observable.code()
+
+    def descriptor(self, frame):
+        r = super().descriptor(frame)
+        r.alg = len(self.window)
+        r.birkhoff = ctypes.pointer(self._desc)
+        return r
+
+    def check_run(self, tau, N, d=None):
+        """check_run (naff_reducer.jl:236-242): N / tau = 2 max(window) + 1 handoffs."""
+        need = 2 * max(self.window) + 1
+        if not (int(N) % int(tau) == 0 and int(N) // int(tau) == need):
+            raise ValueError(f"the run of {N} steps at tau = {tau} does not have the {need} handoffs (2 window + 1) "
+                             f"that window = {self.window} needs")
+
+    def dtype(self, d, k):
+        hor = np.dtype([("A1", "<f8", 2), ("B1", "<f8", 2), ("A2", "<f8", 2), ("B2", "<f8", 2)])
+        return np.dtype([("hor", hor, (len(self.window),)), ("prev1", "<f8"), ("prev2", "<f8"), ("n", "<i8"),
+                         ("handoffs", "<i4"), ("nwrap", "<i2"), ("code", "i1"), ("pad", "u1")])
+
+    def final(self, st):
+        """reducer_final (naff_reducer.jl:220-230): nu1 once handoffs >= N + 1, nu2 from 2N + 1 (NaN before), delta, D,
+        below_floor per horizon; complete, near_pi."""
+        tp = 2.0 * math.pi
+        s = st["handoffs"].astype(np.int64)
+        hor = st["hor"]
+        tot = lambda a: a[..., 0] + a[..., 1]
+        nu1 = np.stack([np.where(s >= N + 1, tot(hor["A1"][:, h]) / (tp * tot(hor["B1"][:, h])), np.nan)
+                        for h, N in enumerate(self.window)], axis=1)
+        nu2 = np.stack([np.where(s >= 2 * N + 1, tot(hor["A2"][:, h]) / (tp * tot(hor["B2"][:, h])), np.nan)
+                        for h, N in enumerate(self.window)], axis=1)
+        with np.errstate(invalid="ignore"):
+            delta = circle_distance(nu1, nu2)
+            D, below = frequency_index(delta)
+        D = np.where(np.isnan(delta), np.nan, D)
+        below = np.where(np.isnan(delta), False, below)
+        return dict(n=st["n"].copy(), handoffs=s, complete=s >= 2 * max(self.window) + 1,
+                    near_pi=st["nwrap"].astype(np.int64), nu1=nu1, nu2=nu2, delta=delta, D=D, below_floor=below)
+
+    def error(self, row):
+        return "series", float("nan"), float("inf"), int(row["n"]), 1

@@ -1436,6 +1436,115 @@ CL_HD double wb_weight(int64_t jj, int64_t N, int p) {
   return cl_exp(-1.0 / (wb_pow(t, p) * wb_pow(1.0 - t, p)));
 }
 
+// ------------------------------------------------------------------------------ WeightedPhaseAdvance
+// naff_reducer.jl:163-230: per horizon N the two windows' weighted sums A = sum w_j Delta_j, B = sum w_j of the sample
+// increments (window 1: increments 0..N-1, window 2: N..2N-1, weights birkhoff_weight at the mirrored index), the
+// previous sample, the count of increments near the branch cut (planar), a non-finite sample recorded as code 1.
+struct WPAHorizon {
+  CSum A1, B1, A2, B2;
+};
+template <int H>
+struct WPAState {  // the reference's WPAState{Float64, H} layout: 64 H + 32 bytes
+  WPAHorizon hor[H];
+  double prev1, prev2;
+  int64_t n;
+  int32_t handoffs;
+  int16_t nwrap;
+  int8_t code, pad_;
+};
+
+// phase_increments' increment (naff_reducer.jl:47-67): the principal argument of z conj(zprev), from power-of-two
+// normalised samples when the product leaves [1e-280, 1e280]; NaN when a sample is exactly 0.
+CL_HD void wpa_unit_scale(double re, double im, double* ore, double* oim) {
+  const double m = __builtin_fmax(cl_fabs(re), cl_fabs(im));
+  int e = 0;
+  if (cl_isfinite(m) && m > 0.0) {
+    e = (int)((cl_bits(m) >> 52) & 0x7ff) - 1023;
+    e = e < -1000 ? -1000 : (e > 1000 ? 1000 : e);
+  }
+  *ore = __builtin_ldexp(re, -e);
+  *oim = __builtin_ldexp(im, -e);
+}
+CL_HD double wpa_phase_inc(double pre, double pim, double re, double im) {
+  if ((pre == 0.0 && pim == 0.0) || (re == 0.0 && im == 0.0)) return __builtin_nan("");
+  double rr = re * pre - im * (-pim);
+  double ri = re * (-pim) + im * pre;
+  const double s = __builtin_fmax(cl_fabs(rr), cl_fabs(ri));
+  if (!(s > 1e-280 && s < 1e280)) {
+    double a, b, c, d;
+    wpa_unit_scale(re, im, &a, &b);
+    wpa_unit_scale(pre, pim, &c, &d);
+    rr = a * c - b * (-d);
+    ri = a * (-d) + b * c;
+  }
+  return cl_atan2(ri, rr);
+}
+
+template <int H>
+struct WPA {
+  int64_t window[H];
+  int order, obs, ia, ib;
+  int64_t wmax;
+  typedef WPAState<H> State;
+  CL_HD void init(State& st) const {
+#pragma unroll
+    for (int h = 0; h < H; ++h) st.hor[h].A1 = st.hor[h].B1 = st.hor[h].A2 = st.hor[h].B2 = csum_zero();
+    st.prev1 = st.prev2 = 0.0;
+    st.n = 0;
+    st.handoffs = 0;
+    st.nwrap = 0;
+    st.code = 0;
+    st.pad_ = 0;
+  }
+  template <class Sys>
+  CL_HD void step(State& st, int64_t n, const double* x) const {
+    if (st.code != 0) return;
+    double a0 = wb_pick<Sys::S>(x, ia);
+    double a1 = obs == CL_OBS_LIFTED_ANGLE && ib < 0 ? 0.0 : wb_pick<Sys::S>(x, ib);
+    if (!(cl_isfinite(a0) && cl_isfinite(a1))) {
+      st.n = n;
+      st.code = 1;
+      return;
+    }
+    const int32_t s = st.handoffs;
+    const int32_t h1 = s < 2147483647 ? s + 1 : s;
+    if (s == 0) {
+      st.prev1 = a0;
+      st.prev2 = a1;
+      st.n = n;
+      st.handoffs = h1;
+      return;
+    }
+    const double tp = 2.0 * 3.141592653589793;
+    const double d = obs == CL_OBS_PLANAR ? wpa_phase_inc(st.prev1, st.prev2, a0, a1)
+                                          : (a0 - st.prev1) + tp * (a1 - st.prev2);
+    const int64_t j = (int64_t)s - 1;
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      const int64_t N = window[h];
+      if (j < 2 * N) {
+        const double w = wb_weight(j < N ? j : j - N, N, order);
+        if (w != 0.0) {
+          if (j < N) {
+            st.hor[h].A1 = csum_add(st.hor[h].A1, w * d);
+            st.hor[h].B1 = csum_add(st.hor[h].B1, w);
+          } else {
+            st.hor[h].A2 = csum_add(st.hor[h].A2, w * d);
+            st.hor[h].B2 = csum_add(st.hor[h].B2, w);
+          }
+        }
+      }
+    }
+    const bool flagged = obs == CL_OBS_PLANAR && cl_fabs(d) > 9.0 * 3.141592653589793 / 10.0 &&
+                         (int64_t)s <= 2 * wmax && st.nwrap < 32767;
+    st.prev1 = a0;
+    st.prev2 = a1;
+    st.n = n;
+    st.handoffs = h1;
+    if (flagged) st.nwrap = (int16_t)(st.nwrap + 1);
+  }
+};
+
 struct WBirk {
   WBParams p;
   typedef WBHead State;

@@ -329,6 +329,57 @@ CL_HD double cl_log(double x) {
 /* x is positive, normal and finite (cl_log_normal's domain) */
 CL_HD int cl_is_pos_normal(double x) { return x >= 0x1p-1022 && x <= 0x1.fffffffffffffp+1023; }
 
+/* ---------------------------------------------------------------- atan2 */
+/* atan2(y, x) in (-pi, pi]: u = min(|x|, |y|) / max(|x|, |y|) in [0, 1]; with c = k/8 (k = rint(8u)),
+ * atan(u) = atan(c) + atan(t), t = (u - c) / (1 + u c), |t| <= 1/16, atan(c) a double-double constant and atan(t) its
+ * Taylor series t - t^3/3 + ... - t^15/15 (the first omitted term is below 2^-64 relative); then pi/2 - a when |y| > |x|,
+ * pi - a when x < 0 (double-double pi), and the sign of y. Zero arguments: atan2(+-0, +0) = +-0, atan2(+-0, -0) = +-pi;
+ * a NaN argument gives NaN. Within 2 units in the last place of the correctly rounded value (tested against libm). */
+CL_HD double cl_atan2(double y, double x) {
+  if (cl_isnan(x) || cl_isnan(y)) return x + y;
+  const double ax = cl_fabs(x), ay = cl_fabs(y);
+  const double PI_HI = 0x1.921fb54442d18p+1, PI_LO = 0x1.1a62633145c07p-53;
+  const double PIO2_HI = 0x1.921fb54442d18p+0, PIO2_LO = 0x1.1a62633145c07p-54;
+  double a;
+  if (ay == 0.0) {
+    a = cl_signbit(x) ? PI_HI : 0.0;
+  } else if (!cl_isfinite(ax) || !cl_isfinite(ay)) {
+    /* C99's values at infinity */
+    if (cl_isfinite(ax)) a = PIO2_HI;
+    else if (cl_isfinite(ay)) a = cl_signbit(x) ? PI_HI : 0.0;
+    else a = cl_signbit(x) ? 0x1.2d97c7f3321d2p+1 : 0x1.921fb54442d18p-1; /* 3 pi / 4, pi / 4 */
+  } else {
+    const int swap = ay > ax;
+    const double u = swap ? ax / ay : ay / ax;
+    const double AT_HI[9] = {0.0, 0x1.fd5ba9aac2f6ep-4, 0x1.f5b75f92c80ddp-3, 0x1.6f61941e4def1p-2, 0x1.dac670561bb4fp-2,
+                             0x1.1e00babdefeb4p-1, 0x1.4978fa3269ee1p-1, 0x1.700a7c5784634p-1, 0x1.921fb54442d18p-1};
+    const double AT_LO[9] = {0.0, -0x1.cd37686760c17p-59, 0x1.8ab6e3cf7afbdp-57, -0x1.c63aae6f6e918p-56,
+                             0x1.a2b7f222f65e2p-56, -0x1.928df287a668fp-58, 0x1.2419a87f2a458p-56, -0x1.8c34d25aadef6p-56,
+                             0x1.1a62633145c07p-55};
+    const int k = (int)cl_rint(u * 8.0);
+    double ahi = 0.0, alo = 0.0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) { /* selects: no indexed register array */
+      ahi = j == k ? AT_HI[j] : ahi;
+      alo = j == k ? AT_LO[j] : alo;
+    }
+    const double c = (double)k * 0.125;
+    const double t = (u - c) / (1.0 + u * c);
+    const double z = t * t;
+    double p = -0x1.1111111111111p-4;                  /* -1/15 */
+    p = p * z + 0x1.3b13b13b13b14p-4;                  /* 1/13 */
+    p = p * z - 0x1.745d1745d1746p-4;                  /* 1/11 */
+    p = p * z + 0x1.c71c71c71c71cp-4;                  /* 1/9 */
+    p = p * z - 0x1.2492492492492p-3;                  /* 1/7 */
+    p = p * z + 0x1.999999999999ap-3;                  /* 1/5 */
+    p = p * z - 0x1.5555555555555p-2;                  /* 1/3 */
+    a = ahi + (t + (t * (z * p) + alo));
+    if (swap) a = PIO2_HI - (a - PIO2_LO);
+    if (cl_signbit(x)) a = PI_HI - (a - PI_LO);
+  }
+  return cl_signbit(y) ? -a : a;
+}
+
 /* exp(x): k = rint(x / ln 2), r = (x - k LN2_HI) - k LN2_LO (k LN2_HI is exact: |k| < 2^11 and LN2_HI has 42
  * significant bits), |r| <= ln 2 / 2 plus rounding; exp(r) = 1 + (r + r^2 E(r)) with E the Taylor polynomial
  * sum_{n=2..13} r^(n-2) / n! (the truncation r^12 / 14! is below 2^-60 relative on the interval), the coefficients

@@ -63,6 +63,11 @@ typedef Cons<S_<SALI<2>>, Cons<S_<RunMon<2>>, Nil>> C_ind_s;
 // WeightedBirkhoff (k = 1) alone and after RunMonitor (the indicator recorder's window-kind order)
 typedef Cons<S_<WBirk>, Nil> C1_wb;
 typedef Cons<S_<RunMon<2>>, Cons<S_<WBirk>, Nil>> C1_mon_wb;
+// WeightedPhaseAdvance (k = 1, one or two windows) alone and after RunMonitor (the phase-advance window kind)
+typedef Cons<S_<WPA<1>>, Nil> C1_pa1;
+typedef Cons<S_<WPA<2>>, Nil> C1_pa2;
+typedef Cons<S_<RunMon<2>>, Cons<S_<WPA<1>>, Nil>> C1_mon_pa1;
+typedef Cons<S_<RunMon<2>>, Cons<S_<WPA<2>>, Nil>> C1_mon_pa2;
 
 #define ADD2(SK, SYS, CH, NF) if (*n < cap) out[(*n)++] = make_instance<SK, SYS, 2, NF, CH>();
 #define ADD1(SK, SYS, CH) if (*n < cap) out[(*n)++] = make_instance<SK, SYS, 1, 1, CH>();
@@ -79,7 +84,8 @@ typedef Cons<S_<RunMon<2>>, Cons<S_<WBirk>, Nil>> C1_mon_wb;
   ADD2(SK, SYS, C_rec_rgs, 1) ADD2(SK, SYS, C_rec_hh, 1) ADD2(SK, SYS, C_rec_aug, 1) ADD1(SK, SYS, C1_rec_ben) \
   ADD2(SK, SYS, C_st, 1) ADD2(SK, SYS, C_st_mon, 1) ADD1(SK, SYS, C1_wb) ADD1(SK, SYS, C1_mon_wb)               \
   ADD2(SK, SYS, C_ind_fms, 1) ADD2(SK, SYS, C_ind_mfs, 1) ADD2(SK, SYS, C_ind_f, 1) ADD2(SK, SYS, C_ind_m, 1)   \
-  ADD2(SK, SYS, C_ind_s, 1)
+  ADD2(SK, SYS, C_ind_s, 1) ADD1(SK, SYS, C1_pa1) ADD1(SK, SYS, C1_pa2) ADD1(SK, SYS, C1_mon_pa1)              \
+  ADD1(SK, SYS, C1_mon_pa2)
 
 // the surrogate workload on the per-orbit-K map: the traced prefix (MEGNO + FLI on frame 1, the QR spectrum
 // on frame 2), the continuation and the plain method (the QR spectrum alone)

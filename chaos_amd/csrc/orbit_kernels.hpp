@@ -149,6 +149,10 @@ CL_HD void slot_step(Slot<Stretch<D>, FR>& s, int64_t n, Ctx& c) {
   if (s.r.replace) c.logs_valid[FR] = false;
 }
 
+template <int H, int FR, class Ctx>
+CL_HD void slot_step(Slot<WPA<H>, FR>& s, int64_t n, Ctx& c) {
+  s.r.template step<decltype(c.sys)>(s.st, n, c.x);
+}
 template <int FR, class Ctx>
 CL_HD void slot_step(Slot<WBirk, FR>& s, int64_t n, Ctx& c) {
   s.r.template step<decltype(c.sys)>(s.st, n, c.x, c.mem, c.wstride, c.wtab);
@@ -201,6 +205,19 @@ template <int D, int K>
 CL_HD void slot_setup(GALI<D, K>&, const cl_reducer&) {}
 template <int D>
 CL_HD void slot_setup(RunMon<D>&, const cl_reducer&) {}
+template <int H>
+inline void slot_setup(WPA<H>& r, const cl_reducer& d) {
+  const cl_birkhoff& b = *d.birkhoff;
+  r.wmax = 0;
+  for (int h = 0; h < H; ++h) {
+    r.window[h] = b.window[h];
+    r.wmax = r.wmax > b.window[h] ? r.wmax : b.window[h];
+  }
+  r.order = b.order;
+  r.obs = b.kind[0];
+  r.ia = b.a[0];
+  r.ib = b.b[0];
+}
 inline void slot_setup(WBirk& r, const cl_reducer& d) {
   const cl_birkhoff& b = *d.birkhoff;
   r.p.H = b.H;

@@ -61,7 +61,7 @@ static const Instance* find_instance(const cl_system* sys, const cl_chain* ch, i
     bool same = true;
     for (int r = 0; r < ch->n && same; ++r) {
       const cl_reducer& q = ch->r[r];
-      int alg = q.kind == CL_RED_QR ? q.alg : 0;
+      int alg = (q.kind == CL_RED_QR || q.kind == CL_RED_PHASE_ADVANCE) ? q.alg : 0;  // the QR algorithm; the windows
       same = in.red[r][0] == q.kind && in.red[r][1] == alg && in.red[r][2] == q.frame;
     }
     if (same) return &in;
@@ -139,6 +139,25 @@ static const char* wb_check(const cl_chain* ch, int s) {
   return nullptr;
 }
 
+// WeightedPhaseAdvance's descriptor (naff_reducer.jl:148-152, check_run :236-242): 1 or 2 windows of at least 16
+// samples (the key's alg is the window count), order 1 to 4, a planar or lifted-angle observable inside the state.
+static const char* pa_check(const cl_chain* ch, int s) {
+  for (int r = 0; r < ch->n && r < CL_MAX_REDUCERS; ++r) {
+    if (ch->r[r].kind != CL_RED_PHASE_ADVANCE) continue;
+    const cl_birkhoff* b = ch->r[r].birkhoff;
+    if (!b) return "WeightedPhaseAdvance needs its parameters";
+    if (b->H < 1 || b->H > 2 || ch->r[r].alg != b->H) return "WeightedPhaseAdvance: 1 or 2 windows (alg = the count)";
+    if (b->order < 1 || b->order > 4) return "WeightedPhaseAdvance: birkhoff_order must be 1 to 4";
+    for (int h = 0; h < b->H; ++h)
+      if (b->window[h] < 16 || b->window[h] > ((int64_t)1 << 40)) return "WeightedPhaseAdvance: each window at least 16";
+    const bool planar = b->kind[0] == CL_OBS_PLANAR, lifted = b->kind[0] == CL_OBS_LIFTED_ANGLE;
+    if (!(planar || lifted)) return "WeightedPhaseAdvance: the observable is planar or lifted-angle";
+    if (b->a[0] < 0 || b->a[0] >= s || b->b[0] >= s || (planar && b->b[0] < 0) || (lifted && b->b[0] < -1))
+      return "WeightedPhaseAdvance: observable index outside the state";
+  }
+  return nullptr;
+}
+
 // The per-launch table: the header (windows, offsets) and horizon h's w(jj), jj < N_h, at offset
 // sum_{h' < h} N_h' after it.
 __global__ void k_wb_weights(int H, cl_birkhoff b, double* wtab) {
@@ -190,6 +209,7 @@ static int validate(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int e
   }
   if (nws > 1) return set_err(ctx, CL_ERR_ARGUMENT, "at most one reducer of a chain may request a workspace");
   if (const char* why = wb_check(ch, s)) return set_err(ctx, CL_ERR_ARGUMENT, why);
+  if (const char* why = pa_check(ch, s)) return set_err(ctx, CL_ERR_ARGUMENT, why);
   const Instance* in = find_instance(sys, ch, d, k);
   if (!in) return set_err(ctx, CL_ERR_UNSUPPORTED, "no kernel instance for this system/chain/k");
   *out = in;

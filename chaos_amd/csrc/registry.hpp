@@ -119,6 +119,10 @@ template <int D>
 struct Kind<Stretch<D>> {
   static constexpr int kind = CL_RED_STRETCHING, alg = 0;
 };
+template <int H>
+struct Kind<WPA<H>> {  // the window count is the key's alg
+  static constexpr int kind = CL_RED_PHASE_ADVANCE, alg = H;
+};
 template <>
 struct Kind<WBirk> {
   static constexpr int kind = CL_RED_BIRKHOFF, alg = 0;

@@ -85,12 +85,20 @@ typedef struct {
 #define CL_RED_BIRKHOFF 12   /* WeightedBirkhoff(; window, observables, birkhoff_order), any k, the last reducer of its
                                 chain, src/Indicators/wbirkhoff_reducer.jl:4-116; parameters in cl_reducer.birkhoff */
 #define CL_RED_RLI 13        /* RLI(; replace), k = 2 on a pair system, src/Indicators/rli.jl:1-140 */
+#define CL_RED_PHASE_ADVANCE 14 /* WeightedPhaseAdvance(; window, observable, birkhoff_order), any k, a reader of the point,
+                                   src/Indicators/naff_reducer.jl:142-254; 1 or 2 windows, parameters in cl_reducer.birkhoff
+                                   (window, order, and the observable in kind[0], a[0], b[0]) */
 
 /* Observables of WeightedBirkhoff (src/Indicators/wbirkhoff.jl:17-77), coordinates 0-based:
  * CL_OBS_COS cos(b x[a]), CL_OBS_SIN sin(b x[a]) (b the integer frequency), CL_OBS_LIFTED x[a] + 2 pi x[b]. */
 #define CL_OBS_COS 1
 #define CL_OBS_SIN 2
 #define CL_OBS_LIFTED 3
+/* Observables of WeightedPhaseAdvance (naff_reducer.jl:88-120): CL_OBS_PLANAR the sample x[a] + i x[b], its increment
+ * the principal argument of z' conj(z); CL_OBS_LIFTED_ANGLE the pair (x[a], x[b]) of a coordinate and its winding count
+ * (b = -1: none), its increment (q' - q) + 2 pi (m' - m). */
+#define CL_OBS_PLANAR 4
+#define CL_OBS_LIFTED_ANGLE 5
 #define CL_WB_MAX_WINDOWS 16
 #define CL_WB_MAX_OBSERVABLES 8
 typedef struct {

@@ -172,6 +172,7 @@ static int red_bytes(const cl_reducer& r, int d, int k) {
     case CL_RED_RUNMONITOR: return 24;
     case CL_RED_STRETCHING: return 104;
     case CL_RED_BIRKHOFF: return r.birkhoff ? 8 * r.birkhoff->H * (2 + 5 * r.birkhoff->K) + 16 : -1;
+    case CL_RED_PHASE_ADVANCE: return r.birkhoff ? 64 * r.birkhoff->H + 32 : -1;
   }
   return -1;
 }
@@ -272,8 +273,18 @@ struct WBSt {  // WBState{T, H, K} (wbirkhoff_reducer.jl:36-49), up to 16 horizo
   int16_t code;
 };
 template <class T>
+struct PaSt {  // WPAState{T, H} (naff_reducer.jl:166-174), 1 or 2 horizons
+  Acc<T> A1[2], B1[2], A2[2], B2[2];
+  T prev1, prev2;
+  int64_t n;
+  int32_t handoffs;
+  int16_t nwrap;
+  int8_t code;
+};
+template <class T>
 struct RState {  // one of the above, by kind
   WBSt<T> wb;
+  PaSt<T> pa;
   StretchSt<T> sn;
   MonSt<T> mon;
   QRSt<T> qr;
@@ -399,6 +410,23 @@ static void write_state(const cl_reducer& r, const RState<T>& s, int d, int k, u
       b.put(q.code);
       b.put(q.column);
       b.pad(6);
+      break;
+    }
+    case CL_RED_PHASE_ADVANCE: {
+      const PaSt<T>& q = s.pa;
+      for (int h = 0; h < r.birkhoff->H; ++h) {
+        b.put(dbl(q.A1[h].s)); b.put(dbl(q.A1[h].c));
+        b.put(dbl(q.B1[h].s)); b.put(dbl(q.B1[h].c));
+        b.put(dbl(q.A2[h].s)); b.put(dbl(q.A2[h].c));
+        b.put(dbl(q.B2[h].s)); b.put(dbl(q.B2[h].c));
+      }
+      b.put(dbl(q.prev1));
+      b.put(dbl(q.prev2));
+      b.put(q.n);
+      b.put(q.handoffs);
+      b.put(q.nwrap);
+      b.put(q.code);
+      b.pad(1);
       break;
     }
     case CL_RED_RLI: {
@@ -896,6 +924,7 @@ static void red_stretch(const cl_reducer& r, StretchSt<T>& st, int64_t n, T* W, 
     }
 }
 
+
 // WeightedBirkhoff (wbirkhoff_reducer.jl:51-66, the update _wb_sample of wbirkhoff.jl:118-123): the observables of
 // the point, the first non-finite one fails the orbit; sample j = handoffs goes into window 1 or 2 of every horizon
 // that has not yet taken its 2N samples, with the weight exp(-1 / (t^p (1 - t)^p)), t = min(jj, N - jj) / N
@@ -916,6 +945,88 @@ static T wb_w(int64_t jj, int64_t N, int p) {
     return p == 1 ? u : (p == 2 ? u2 : (p == 3 ? u2 * u : u2 * u2));
   };
   return P::exp((T)-1 / (pw(t) * pw((T)1 - t)));
+}
+
+// WeightedPhaseAdvance (naff_reducer.jl:199-218): the sample (planar (x[a], x[b]); lifted-angle (x[a], x[b] or 0)),
+// a non-finite sample fails the orbit (code 1), the first handoff stores it, each later one adds its increment
+// (phase_increments' principal argument, or (q' - q) + 2 pi (m' - m)) with weight birkhoff_weight at the mirrored
+// index to the window of each horizon that holds increment j = handoffs - 1; increments with |Delta| > 0.9 pi counted.
+template <class P>
+static void pa_unit_scale(double re, double im, double& ore, double& oim) {  // naff_reducer.jl:40-45
+  const double m = std::max(std::fabs(re), std::fabs(im));
+  int e = 0;
+  if (std::isfinite(m) && m > 0) {
+    uint64_t u;
+    memcpy(&u, &m, 8);
+    e = (int)((u >> 52) & 0x7ff) - 1023;
+    e = e < -1000 ? -1000 : (e > 1000 ? 1000 : e);
+  }
+  ore = std::ldexp(re, -e);
+  oim = std::ldexp(im, -e);
+}
+template <class P>
+static double pa_phase_inc(double pre, double pim, double re, double im) {  // naff_reducer.jl:47-56
+  if ((pre == 0 && pim == 0) || (re == 0 && im == 0)) return NAN;
+  // r = z conj(zprev), Julia's complex product
+  double rr = re * pre - im * (-pim), ri = re * (-pim) + im * pre;
+  const double s = std::max(std::fabs(rr), std::fabs(ri));
+  if (!(s > 1e-280 && s < 1e280)) {
+    double a, b, c, d;
+    pa_unit_scale<P>(re, im, a, b);
+    pa_unit_scale<P>(pre, pim, c, d);
+    rr = a * c - b * (-d);
+    ri = a * (-d) + b * c;
+  }
+  return cl_atan2(ri, rr);
+}
+template <class P, class T>
+static void red_pa(const cl_reducer& r, PaSt<T>& st, int64_t n, const T* x) {
+  if (st.code) return;
+  const cl_birkhoff& b = *r.birkhoff;
+  const bool planar = b.kind[0] == CL_OBS_PLANAR;
+  const T a0 = x[b.a[0]], a1 = (!planar && b.b[0] < 0) ? (T)0 : x[b.b[0]];
+  if (!(P::finite(a0) && P::finite(a1))) {
+    st.n = n;
+    st.code = 1;
+    return;
+  }
+  const int32_t s = st.handoffs;
+  const int32_t h1 = s < 2147483647 ? s + 1 : s;
+  if (s == 0) {
+    st.prev1 = a0;
+    st.prev2 = a1;
+    st.n = n;
+    st.handoffs = h1;
+    return;
+  }
+  const T tp = (T)2 * (T)3.141592653589793;
+  const T d = planar ? (T)pa_phase_inc<P>((double)st.prev1, (double)st.prev2, (double)a0, (double)a1)
+                     : (a0 - st.prev1) + tp * (a1 - st.prev2);
+  const int64_t j = (int64_t)s - 1;
+  int64_t wmax = 0;
+  for (int h = 0; h < b.H; ++h) {
+    const int64_t N = b.window[h];
+    wmax = N > wmax ? N : wmax;
+    if (j < 2 * N) {
+      const T w = wb_w<P, T>(j < N ? j : j - N, N, b.order);
+      if (w != 0) {
+        if (j < N) {
+          st.A1[h] = acc_add<P>(st.A1[h], w * d);
+          st.B1[h] = acc_add<P>(st.B1[h], w);
+        } else {
+          st.A2[h] = acc_add<P>(st.A2[h], w * d);
+          st.B2[h] = acc_add<P>(st.B2[h], w);
+        }
+      }
+    }
+  }
+  const bool flagged = planar && P::abs(d) > (T)9 * (T)3.141592653589793 / (T)10 && (int64_t)s <= 2 * wmax &&
+                       st.nwrap < 32767;
+  st.prev1 = a0;
+  st.prev2 = a1;
+  st.n = n;
+  st.handoffs = h1;
+  if (flagged) st.nwrap += 1;
 }
 template <class P, class T>
 static void red_wbirk(const cl_reducer& r, WBSt<T>& st, int64_t n, const T* x) {
@@ -1145,6 +1256,7 @@ static bool red_failed(const cl_reducer& r, const RState<T>& s) {
     case CL_RED_MEGNO: return s.meg.code != 0;
     case CL_RED_SALI: return s.sali.code != 0;
     case CL_RED_RLI: return s.rli.code != 0;
+    case CL_RED_PHASE_ADVANCE: return s.pa.code != 0;
     case CL_RED_FLI: return s.fli.code != 0;
     case CL_RED_OFLI: return s.ofli.code != 0;
     case CL_RED_GALI: return s.gali.code != 0;
@@ -1623,6 +1735,7 @@ static int8_t run_orbit_seg(const SYS& sys, const cl_chain& ch, int d, int k, in
         case CL_RED_RUNMONITOR: red_runmonitor<P>(st[r].mon, sys, x, sys.jd()); break;
         case CL_RED_STRETCHING: red_stretch<P>(rd, st[r].sn, n, w, d, ws); break;
         case CL_RED_BIRKHOFF: red_wbirk<P>(rd, st[r].wb, n, x); break;
+        case CL_RED_PHASE_ADVANCE: red_pa<P>(rd, st[r].pa, n, x); break;
       }
     }
     for (int r = 0; r < ch.n; ++r)
@@ -2227,7 +2340,8 @@ int or_ind_record(const cl_system* sys, const cl_chain* ch, int64_t m, int s, in
                       (sys->kind == CL_SYS_PAIR_WRAPPED_STANDARD_MAP && s == 8);
   bool ch_ok = ch->n >= 2;
   if (ch_ok && ch->r[0].kind == CL_RED_RUNMONITOR) {
-    ch_ok = ch->n == 2 && ch->r[1].kind == CL_RED_BIRKHOFF && k == 1 && (ch->flags & CL_CHAIN_RESET_FRAME);
+    ch_ok = ch->n == 2 && (ch->r[1].kind == CL_RED_BIRKHOFF || ch->r[1].kind == CL_RED_PHASE_ADVANCE) && k == 1 &&
+            (ch->flags & CL_CHAIN_RESET_FRAME);
   } else if (ch_ok) {
     ch_ok = k == 2 && ch->n <= 5 && ch->r[ch->n - 1].kind == CL_RED_RUNMONITOR;
     for (int r = 0; ch_ok && r + 1 < ch->n; ++r)
@@ -2324,6 +2438,11 @@ void or_wb_weights(int64_t N, int order, double* w, int math) {
 // estimate.
 void or_window_gw2(int64_t n, const double* R, double* out) {
   for (int64_t i = 0; i < n; ++i) out[i] = window_gw<F64Cl, double>(R + 4 * i, 2);
+}
+
+// The portable atan2 of the kernels (cl_math.h), for its accuracy test against libm.
+void or_atan2(int64_t n, const double* y, const double* x, double* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = cl_atan2(y[i], x[i]);
 }
 
 // The portable exp of the kernels (cl_math.h), for its accuracy test against libm.

@@ -58,6 +58,8 @@ def load():
         o.or_wb_weights.argtypes = [I64, I, P, I]
         o.or_exp.restype = None
         o.or_exp.argtypes = [I64, P, P]
+        o.or_atan2.restype = None
+        o.or_atan2.argtypes = [I64, P, P, P]
         o.or_window_gw2.restype = None
         o.or_window_gw2.argtypes = [I64, P, P]
         o.or_csum.restype = None
@@ -292,3 +294,13 @@ def window_gw2(R):
     out = np.empty(R.shape[0])
     load().or_window_gw2(R.shape[0], R.ctypes.data, out.ctypes.data)
     return out
+
+
+def atan2(y, x):
+    """The kernels' portable atan2 (cl_math.h) on the host."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    x = np.ascontiguousarray(np.broadcast_to(x, y.shape), dtype=np.float64)
+    out = np.empty_like(y)
+    load().or_atan2(y.size, y.ctypes.data, x.ctypes.data, out.ctypes.data)
+    return out
+

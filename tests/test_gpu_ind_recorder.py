@@ -116,3 +116,31 @@ def test_pair_kind_equals_oracle(members, wrapped, tau, launch):
     assert np.array_equal(rec.x[ok].view(np.uint64), orc["x"][ok].view(np.uint64))
     assert np.array_equal(rec.W[ok].view(np.uint64), orc["W"][ok].view(np.uint64))
     assert rec.fail[3] == 1 and rec.fail[5] == 1 and np.mean(ok) > 0.99
+
+
+@pytest.mark.parametrize("obs", ["planar", "lifted"])
+@pytest.mark.parametrize("wrapped", [False, True], ids=["lifted", "wrapped"])
+def test_phase_advance_window_kind_equals_oracle(obs, wrapped):
+    """The phase-advance window kind (RunMonitor, WeightedPhaseAdvance; the study's pa spec: windows 250 and 1000,
+    order 2, planar (q, p)) and the lifted-angle observable, against the oracle bit for bit."""
+    from chaos_amd import WeightedPhaseAdvance, PlanarObservable, LiftedAngleObservable
+    m, tau = 1200, 1
+    windows = (250, 1000) if obs == "planar" else (64,)
+    N = 2 * max(windows) + 1
+    sysm = (WrappedStandardMap if wrapped else StandardMap)(0.9)
+    rng = np.random.default_rng(41)
+    z = 2 * math.pi * rng.random((m, 2))
+    x0 = np.array([torus_state(v) for v in z]) if wrapped else z
+    x0[6] = [np.nan, 0.1, 0.0, 0.0] if wrapped else [np.nan, 0.1]
+    o = PlanarObservable(1, 2) if obs == "planar" else (LiftedAngleObservable(2, 4) if wrapped else LiftedAngleObservable(2))
+    pa = WeightedPhaseAdvance(window=windows, observable=o, birkhoff_order=2)
+    grid = record_grid(N, tau, points_per_decade=10, first_steps=10, extra_fractions=(0.5,))
+    chain, _ = make_chain((RunMonitor(sysm), pa), reset_frame=True)
+    W1 = np.broadcast_to(np.array([[1.0], [0.0]]), (m, 2, 1))
+    orc = O.ind_record(sysm.descriptor(), chain, x0, W1, N, tau, 1, grid, 3 + x0.shape[1])
+    say("ind phase advance", obs, "wrapped" if wrapped else "lifted", "m", m, "N", N, "rows", len(grid))
+    rec = record_indicators(sysm, x0, N, reader=pa, tau=tau, grid=grid, launch_steps=400)
+    assert np.array_equal(rec.fail, orc["fail"]) and np.array_equal(rec.code, orc["code"])
+    assert np.array_equal(rec.R.view(np.uint64), orc["R"].view(np.uint64))
+    assert np.array_equal(rec.states, orc["states"])
+    assert rec.fail[6] == 1 and np.mean(rec.fail == 0) > 0.99
