@@ -722,7 +722,8 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: the ensemble shards by orbit ranges with no exchange during propagation, "
-              f"and this build runs on single-GPU boxes only (asked for --gpus {a.gpus}).", file=sys.stderr)
+              f"so the figure at N GPUs is N times the one-GPU figure; multi-GPU runs are not supported "
+              f"(asked for --gpus {a.gpus}).", file=sys.stderr)
         sys.exit(2)
     if a.workload == "c5":
         return c5_main(a)
