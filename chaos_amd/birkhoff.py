@@ -151,7 +151,7 @@ def weighted_birkhoff(series, window, *, order=1, weights_=None):
 
 
 def birkhoff_index(avg1, avg2, S, floor_multiplier=WB_FLOOR_MULTIPLIER):
-    """The index of an average pair (birkhoff_index, wbirkhoff.jl:182-199), elementwise: delta = avg1 - avg2,
+    """The index of an average pair (birkhoff_index, wbirkhoff.jl:144-171), elementwise: delta = avg1 - avg2,
     phi = max(32 eps S, floatmin), absdig = -log10 max(|delta|, phi), reldig = -log10(max(|delta|, phi) / m) with
     m = |avg1| / 2 + |avg2| / 2 where m > phi, dig = max(absdig, reldig) (absdig where reldig is NaN), below_floor =
     |delta| <= phi; S = 0 gives NaN digits and `zero`; a non-finite delta gives NaN digits and no flags."""

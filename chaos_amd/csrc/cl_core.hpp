@@ -7,13 +7,13 @@
 //   fused frame product      src/Foundation/product.jl:9-20 (explicit fma, ascending q)
 //   Neumaier sums, norms     src/Foundation/Foundation.jl:48-58, src/Foundation/norm.jl:8-12
 //   standard map             src/Systems/standard_map.jl:18-21,30-34
-//   wrapped standard map     src/Systems/wrapped_standard_map.jl:82-111
+//   wrapped standard map     src/Systems/wrapped_standard_map.jl:16-57
 //   QR spectrum              src/Lyapunov/qr_spectrum.jl:54-73, orthonormalize.jl:38-151
 //   Householder/Givens       src/Lyapunov/householder.jl:17-123
-//   Benettin                 src/Lyapunov/benettin.jl:159-164
+//   Benettin                 src/Lyapunov/benettin.jl:27-33
 //   MEGNO / SALI / FLI       src/Indicators/megno.jl:63-80, sali.jl:166-179, fli.jl:320-341
 //   OFLI                     src/Indicators/ofli.jl:59-102
-//   GALI                     src/Indicators/gali.jl:220-341
+//   GALI                     src/Indicators/gali.jl:47-169
 // so that, with cl_math.h providing the same sin/cos/log bits on host and device, a kernel
 // reproduces the host restatement of the same run bit for bit.
 #pragma once
@@ -216,7 +216,7 @@ struct WrappedStdMap {
     *a = q;
   }
   CL_HD void forward(const Aux& au, double* z) const {
-    // 2 * (pi_hi + pi_lo) in Float64 is 2 pi rounded (wrapped_standard_map.jl:106, standard_map.jl:48)
+    // 2 * (pi_hi + pi_lo) in Float64 is 2 pi rounded (wrapped_standard_map.jl:40, standard_map.jl:48)
     const double tp = 2.0 * (3.141592653589793 + 1.2246467991473532e-16);
     double p, a;
     reduce(z[1] + K * au.sn, tp, &p, &a);
@@ -239,7 +239,7 @@ struct WrappedStdMap {
   }
 };
 
-// The product of a map with itself (src/Systems/pair.jl:1-77): the state is (state A; state B), each half steps by the
+// The product of a map with itself (src/Systems/pair.jl:1-68): the state is (state A; state B), each half steps by the
 // base map alone, and the tangent acts block by block on the frame's rows 1..d and d+1..2d (the zero blocks are never
 // multiplied, so each block's product is the single-orbit one). The monitor reads orbit A's one-step tangent
 // (run_indicator_record.jl:6-10).
@@ -386,7 +386,7 @@ struct QRFact {
   int column;  // 1-based failing column, 0 if none
 };
 
-// Gram-Schmidt family (orthonormalize.jl:281-348).
+// Gram-Schmidt family (orthonormalize.jl:38-105).
 template <int ALG, int D, int K>
 CL_HD void orthonormalize_gs(const Frame<D, K>& W, QRFact<D, K>& f) {
 #pragma unroll

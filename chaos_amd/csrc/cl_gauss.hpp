@@ -57,7 +57,7 @@ struct HHGauss {
     double q[3][2];  // positions of the converged stages
   };
 
-  // J grad H for H = |p|^2/2 + |q|^2/2 + q1^2 q2 - q2^3/3 (src/Systems/flow_models.jl:205-207, flow_core.jl:75)
+  // J grad H for H = |p|^2/2 + |q|^2/2 + q1^2 q2 - q2^3/3 (src/Systems/flow_models.jl:101-103, flow_core.jl:75)
   static CL_HD void vf(const double* z, double* f) {
     f[0] = z[2];
     f[1] = z[3];
@@ -288,7 +288,7 @@ struct HHGauss {
     for (int c = 0; c < 4; ++c) z[c] = a.ok ? z[c] + a.inc[c] : __builtin_nan("");
   }
   CL_HD void field(const double* z, const Aux&, double* f) const { vf(z, f); }
-  // energy H(z) (flow_models.jl:205): (p1^2 + p2^2)/2 + (q1^2 + q2^2)/2 + q1^2 q2 - q2^3/3
+  // energy H(z) (flow_models.jl:101): (p1^2 + p2^2)/2 + (q1^2 + q2^2)/2 + q1^2 q2 - q2^3/3
   static CL_HD double energy(const double* z) {
     return (((z[2] * z[2] + z[3] * z[3]) / 2.0 + (z[0] * z[0] + z[1] * z[1]) / 2.0) + (z[0] * z[0]) * z[1]) -
            (z[1] * z[1]) * z[1] / 3.0;

@@ -11,7 +11,7 @@
 //   G[i, j]  = z(j d + i)      (column j depends only on draws of column j and below)
 //   W        = G[:, 1:k] orthonormalised column by column with two modified Gram-Schmidt passes
 //              against the earlier columns, then divided by its scaled norm (the two-pass rule of
-//              the reference's `_orthonormalize`, src/Tangent/initial.jl:225-235), so R_jj > 0.
+//              the reference's `_orthonormalize`, src/Tangent/initial.jl:10-25), so R_jj > 0.
 // Column j depends only on G's first j columns, so it does not depend on k, the property the
 // reference tests for its frames (test/tangent-core/tc_ac05_08.jl:37-79).
 #pragma once

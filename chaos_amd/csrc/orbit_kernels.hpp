@@ -4,8 +4,8 @@
 // Per step (src/Tangent/orbit.jl:48-58): the tangent at the OLD point is applied to every frame
 // with the explicit-fma product, then the map advances; sin and cos of the new point are formed
 // once and serve the next Jacobian, the next step and any reducer that reads the field. Every
-// `tau` steps the handoff checks finiteness (orbit.jl:171-172) and runs the reducer chain in order
-// (orbit.jl:139-145), each reducer seeing the frame as replaced by the reducers before it.
+// `tau` steps the handoff checks finiteness (orbit.jl:53-55) and runs the reducer chain in order
+// (orbit.jl:23-32,56-57), each reducer seeing the frame as replaced by the reducers before it.
 #pragma once
 #include <stdint.h>
 #include "cl_core.hpp"

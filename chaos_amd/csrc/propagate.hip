@@ -103,7 +103,7 @@ static int set_err(cl_ctx* ctx, int code, const char* msg) {
   return code;
 }
 
-// Validation shared by both launch entry points (src/Tangent/propagate.jl:279-289, orbit.jl:187-191).
+// Validation shared by both launch entry points (src/Tangent/propagate.jl:279-289, orbit.jl:70-74).
 // WeightedBirkhoff (the last reducer of its chain): the horizon sums held in the state array, H (2 + 5K) words.
 static int wb_index(const cl_chain* ch) {
   for (int r = 0; r < ch->n && r < CL_MAX_REDUCERS; ++r)

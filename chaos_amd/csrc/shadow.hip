@@ -35,7 +35,7 @@ struct ShadowGeom<StdMap> {
     y[1] = x[1] + d[1];
   }
 };
-// 2 pi as the wrapped map's step forms it: 2 * (pi_hi + pi_lo) in Float64 (wrapped_standard_map.jl:106)
+// 2 pi as the wrapped map's step forms it: 2 * (pi_hi + pi_lo) in Float64 (wrapped_standard_map.jl:40)
 CL_HD double shadow_two_pi() { return 2.0 * (3.141592653589793 + 1.2246467991473532e-16); }
 CL_HD double minimal_image(double u, double tp) {
   const double h = tp / 2.0;

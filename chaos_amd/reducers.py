@@ -18,17 +18,17 @@ class Orthonormalizer:
 
 
 class CGS(Orthonormalizer):
-    """Classical Gram-Schmidt (src/Lyapunov/orthonormalize.jl:250-251)."""
+    """Classical Gram-Schmidt (src/Lyapunov/orthonormalize.jl:8,38-47)."""
     code, name = L.CL_QR_CGS, "CGS"
 
 
 class MGS(Orthonormalizer):
-    """Modified Gram-Schmidt (src/Lyapunov/orthonormalize.jl:252-253)."""
+    """Modified Gram-Schmidt (src/Lyapunov/orthonormalize.jl:10,49-58)."""
     code, name = L.CL_QR_MGS, "MGS"
 
 
 class RGS(Orthonormalizer):
-    """Repeated Gram-Schmidt, eta = 1/sqrt 2, at most 4 passes (src/Lyapunov/orthonormalize.jl:254-255)."""
+    """Repeated Gram-Schmidt, eta = 1/sqrt 2, at most 4 passes (src/Lyapunov/orthonormalize.jl:12,21,60-86)."""
     code, name = L.CL_QR_RGS, "RGS"
 
 
@@ -74,7 +74,7 @@ class Reducer:
 
 
 class QRSpectrum(Reducer):
-    """QRSpectrum(alg; orthogonality_tol=8.0) (src/Lyapunov/qr_spectrum.jl:161-165)."""
+    """QRSpectrum(alg; orthogonality_tol=8.0) (src/Lyapunov/qr_spectrum.jl:14-18)."""
     kind = L.CL_RED_QR
 
     def __init__(self, alg, orthogonality_tol=8.0):
@@ -95,7 +95,7 @@ class QRSpectrum(Reducer):
                          ("value", "<f8"), ("threshold", "<f8"), ("step", "<i8")])
 
     def final(self, st):
-        """(n, sum, comp, lambda, G_max, orth_max, count_max), qr_spectrum.jl:235-240."""
+        """(n, sum, comp, lambda, G_max, orth_max, count_max), qr_spectrum.jl:88-93."""
         n = st["n"]
         tot = _acc(st["sums"])
         lam = np.where(n[:, None] == 0, 0.0, tot / np.where(n == 0, 1, n)[:, None])
@@ -111,7 +111,7 @@ class QRSpectrum(Reducer):
 
 
 class Benettin(Reducer):
-    """Benettin() for k = 1 (src/Lyapunov/benettin.jl:143-185)."""
+    """Benettin() for k = 1 (src/Lyapunov/benettin.jl:11-53)."""
     kind = L.CL_RED_BENETTIN
 
     def check(self, d, k):
@@ -218,7 +218,7 @@ class RLI(Reducer):
 
 
 class SALI(Reducer):
-    """SALI() for k = 2, replaces W by unit columns (src/Indicators/sali.jl:147-201)."""
+    """SALI() for k = 2, replaces W by unit columns (src/Indicators/sali.jl:14-68)."""
     kind = L.CL_RED_SALI
 
     def check(self, d, k):
@@ -241,7 +241,7 @@ class SALI(Reducer):
 
 
 class FLI(Reducer):
-    """FLI(; replace=True) for k = 2 (src/Indicators/fli.jl:292-362)."""
+    """FLI(; replace=True) for k = 2 (src/Indicators/fli.jl:26-96)."""
     kind = L.CL_RED_FLI
 
     def __init__(self, replace=True):
@@ -300,7 +300,7 @@ class OFLI(Reducer):
 
 
 class GALI(Reducer):
-    """GALI() for 2 <= k <= d (src/Indicators/gali.jl:190-374)."""
+    """GALI() for 2 <= k <= d (src/Indicators/gali.jl:17-201)."""
     kind = L.CL_RED_GALI
 
     def check(self, d, k):
@@ -409,7 +409,7 @@ def eps_pair(lam):
 
 def check_residual(what, value, threshold, *, step, orbit, backend="cpu"):
     """Raise NumericalHealthError when `value` is not finite or exceeds `threshold`, reporting an infinite
-    threshold for a non-finite value (residuals.jl:46-50)."""
+    threshold for a non-finite value (src/Diagnostics/residuals.jl:36-44)."""
     from .tangent import NumericalHealthError
     value, threshold = float(value), float(threshold)
     if np.isfinite(value) and value <= threshold:

@@ -57,7 +57,7 @@ class StandardMapChain:
 
 class WrappedStandardMap:
     """WrappedStandardMap(K): the standard map on [0, 2pi)^2 with exact winding counts, state (x, p, m, n)
-    (src/Systems/wrapped_standard_map.jl:75-123)."""
+    (src/Systems/wrapped_standard_map.jl:9-81)."""
     dimension = 2
     state_length = 4
 
@@ -93,7 +93,7 @@ class StandardMapK:
 class HenonHeilesGauss:
     """IntegratorMap(Gauss(Float64, 3; tol, maxiter), HenonHeilesFlow(), h): one step of the s = 3 Gauss-Legendre
     method on H = |p|^2/2 + |q|^2/2 + q1^2 q2 - q2^3/3, state (q1, q2, p1, p2)
-    (src/Integrators/integrator_map.jl:12-23, src/Integrators/gauss.jl:49-69, src/Systems/flow_models.jl:134,205-207).
+    (src/Integrators/integrator_map.jl:12-23, src/Integrators/gauss.jl:49-69, src/Systems/flow_models.jl:30,101-103).
     `tol=None` is the reference default acceptance 8 eps max(1, |Y|)."""
     dimension = 4
     state_length = 4
@@ -122,7 +122,7 @@ class HenonHeilesGauss:
 
 
 def hh_energy(z):
-    """H(z) of the Henon-Heiles flow in the reference's operation order (src/Systems/flow_models.jl:205)."""
+    """H(z) of the Henon-Heiles flow in the reference's operation order (src/Systems/flow_models.jl:101)."""
     z = np.asarray(z, dtype=np.float64)
     q1, q2, p1, p2 = z[..., 0], z[..., 1], z[..., 2], z[..., 3]
     return ((p1 * p1 + p2 * p2) / 2 + (q1 * q1 + q2 * q2) / 2 + (q1 * q1) * q2) - (q2 * q2) * q2 / 3
@@ -145,14 +145,14 @@ def _reduce(u, tp):
 
 def torus_state(z):
     """State (x, p, m, n) of WrappedStandardMap for the lifted point z = (X, P)
-    (src/Systems/wrapped_standard_map.jl:131-136)."""
+    (src/Systems/wrapped_standard_map.jl:59-70)."""
     x, m = _reduce(float(z[0]), TWO_PI)
     p, n = _reduce(float(z[1]), TWO_PI)
     return (x, p, m, n)
 
 
 class PairMap:
-    """PairMap(sys): the product of a standard map with itself (src/Systems/pair.jl:1-77), state (state A; state B),
+    """PairMap(sys): the product of a standard map with itself (src/Systems/pair.jl:1-68), state (state A; state B),
     dimension 4; each half steps by the base map and the tangent acts block by block. RunMonitor reads orbit A."""
     dimension = 4
 

@@ -49,12 +49,12 @@ extern "C" {
 
 /* ------------------------------------------------------------------ system descriptors */
 #define CL_SYS_STANDARD_MAP 1          /* StandardMap(K), src/Systems/standard_map.jl:8-43 */
-#define CL_SYS_WRAPPED_STANDARD_MAP 2  /* WrappedStandardMap(K), src/Systems/wrapped_standard_map.jl:75-123 */
+#define CL_SYS_WRAPPED_STANDARD_MAP 2  /* WrappedStandardMap(K), src/Systems/wrapped_standard_map.jl:9-81 */
 #define CL_SYS_STANDARD_MAP_CHAIN 3    /* StandardMapChain(N, K, eps; boundary), src/Systems/standard_map_chain.jl:11-53 */
 #define CL_SYS_HENON_HEILES_GAUSS 4    /* IntegratorMap(Gauss(Float64, s; tol), HenonHeilesFlow(), h), src/Integrators/integrator_map.jl:12-23 */
 #define CL_SYS_STANDARD_MAP_K 5        /* StandardMap with its K carried per orbit: state (x, p, K); no reference
                                           counterpart (the reference holds one K per system, standard_map.jl:8-10) */
-#define CL_SYS_PAIR_STANDARD_MAP 6     /* PairMap(StandardMap(K)), src/Systems/pair.jl:1-77: state (A; B), s = 4, d = 4 */
+#define CL_SYS_PAIR_STANDARD_MAP 6     /* PairMap(StandardMap(K)), src/Systems/pair.jl:1-68: state (A; B), s = 4, d = 4 */
 #define CL_SYS_PAIR_WRAPPED_STANDARD_MAP 7 /* PairMap(WrappedStandardMap(K)): s = 8, d = 4 */
 
 typedef struct {
@@ -71,13 +71,13 @@ typedef struct {
 } cl_system;
 
 /* ------------------------------------------------------------------ reducer chain */
-#define CL_RED_QR 1        /* QRSpectrum(alg; orthogonality_tol), src/Lyapunov/qr_spectrum.jl:161-243 */
-#define CL_RED_BENETTIN 2  /* Benettin(), src/Lyapunov/benettin.jl:143-185 */
+#define CL_RED_QR 1        /* QRSpectrum(alg; orthogonality_tol), src/Lyapunov/qr_spectrum.jl:14-96 */
+#define CL_RED_BENETTIN 2  /* Benettin(), src/Lyapunov/benettin.jl:11-53 */
 #define CL_RED_MEGNO 3     /* MEGNO(; replace), src/Indicators/megno.jl:29-113 */
-#define CL_RED_SALI 4      /* SALI(), src/Indicators/sali.jl:147-201 */
-#define CL_RED_FLI 5       /* FLI(; replace), src/Indicators/fli.jl:292-362 */
+#define CL_RED_SALI 4      /* SALI(), src/Indicators/sali.jl:14-68 */
+#define CL_RED_FLI 5       /* FLI(; replace), src/Indicators/fli.jl:26-96 */
 #define CL_RED_OFLI 6      /* OFLI(flow; replace), src/Indicators/ofli.jl:23-126 */
-#define CL_RED_GALI 7      /* GALI(), src/Indicators/gali.jl:190-374 */
+#define CL_RED_GALI 7      /* GALI(), src/Indicators/gali.jl:17-201 */
 #define CL_RED_MONITOR 9   /* ResidualMonitor(sys, det_tol, orth_tol), src/Diagnostics/monitor.jl:9-54 */
 #define CL_RED_RUNMONITOR 10 /* RunMonitor(sys), the recorder's non-raising monitor, src/Experiments/run_record.jl:25-43 */
 #define CL_RED_STRETCHING 11 /* StretchingNumbers(; range, bins, interval, min_per_cell, replace), k = 2, with a

@@ -7,12 +7,12 @@
 //                         KernelAbstractions CPU backend, src/Tangent/propagate.jl:59-60 (16 orbits per group)
 //   fused product         src/Foundation/product.jl:9-20
 //   CompensatedSum/add    src/Foundation/Foundation.jl:48-58;   safe_norm  src/Foundation/norm.jl:8-12
-//   StandardMap           src/Systems/standard_map.jl:18-43;   WrappedStandardMap  wrapped_standard_map.jl:82-123
-//   QRSpectrum            src/Lyapunov/qr_spectrum.jl:169-240; orthonormalize.jl:281-394; householder.jl:17-123
-//   Benettin              src/Lyapunov/benettin.jl:146-183
+//   StandardMap           src/Systems/standard_map.jl:18-43;   WrappedStandardMap  wrapped_standard_map.jl:16-57
+//   QRSpectrum            src/Lyapunov/qr_spectrum.jl:22-93; orthonormalize.jl:38-151; householder.jl:17-123
+//   Benettin              src/Lyapunov/benettin.jl:14-48
 //   MEGNO                 src/Indicators/megno.jl:35-111;       SALI sali.jl:150-199;   FLI fli.jl:299-360
 //   OFLI                  src/Indicators/ofli.jl:30-124 (direction field: the flow's, or the map displacement)
-//   GALI                  src/Indicators/gali.jl:193-372
+//   GALI                  src/Indicators/gali.jl:20-199
 //   LogNorm (test reducer) test/tangent-core/tc_helpers.jl:56-70
 //   ResidualMonitor       src/Diagnostics/monitor.jl:44-54; residuals src/Diagnostics/residuals.jl:4-36,
 //                         src/Systems/residuals.jl:19-33
@@ -40,6 +40,11 @@
 typedef __float128 quad;
 
 enum { MATH_LIBM = 0, MATH_CL = 1 };
+
+// Planted defects for the negative controls of the closed-form tests (tests/test_oracle_pins.py): a test switches one
+// on, shows that the closed form it checks then fails, and switches it off. 0 (the default) is the oracle as written.
+enum { DEFECT_NONE = 0, DEFECT_GALI_RAW = 1, DEFECT_OFLI_NOPROJ = 2, DEFECT_MEGNO_Q2 = 3, DEFECT_GALI_LASTFACTOR = 4 };
+static int g_defect = DEFECT_NONE;
 
 // ------------------------------------------------------------------ arithmetic policies
 struct F64Libm {
@@ -805,7 +810,7 @@ static MSums<T> megno_add(MSums<T> m, T a, int64_t n) {
   o.R1 = R1;
   o.R2 = R2;
   o.Q = acc_add<P>(m.Q, (T)2 * acc_total(R1) / tn);
-  o.Q2 = acc_add<P>(m.Q2, (T)3 * acc_total(R2));
+  o.Q2 = acc_add<P>(m.Q2, (T)(g_defect == DEFECT_MEGNO_Q2 ? 2 : 3) * acc_total(R2));
   return o;
 }
 
@@ -1146,7 +1151,7 @@ static void red_ofli(const cl_reducer& r, OfliSt<T>& st, int64_t n, const T* fld
     al[j] = snorm<P>(w, d);
     T dw = vdot(u, w, d), v[8];
     for (int i = 0; i < d; ++i) v[i] = w[i] - dw * u[i];
-    be[j] = snorm<P>(v, d);
+    be[j] = g_defect == DEFECT_OFLI_NOPROJ ? al[j] : snorm<P>(v, d);
   }
   for (int j = 0; j < k; ++j)
     if (!(P::finite(al[j]) && al[j] > 0)) {
@@ -1211,7 +1216,7 @@ static void red_gali(GaliSt<T>& st, int64_t n, T* W, int d, int k) {
   T U[64];
   for (int j = 0; j < k; ++j)
     for (int i = 0; i < d; ++i) {
-      U[j * d + i] = W[j * d + i] / a[j];
+      U[j * d + i] = g_defect == DEFECT_GALI_RAW ? W[j * d + i] : W[j * d + i] / a[j];
       W[j * d + i] = U[j * d + i];
     }
   T g = 1;
@@ -1226,7 +1231,7 @@ static void red_gali(GaliSt<T>& st, int64_t n, T* W, int d, int k) {
     T s = rr == 0 ? (T)1 : rr;
     for (int i = 0; i < d; ++i) U[j * d + i] /= s;
     if (j > 0) {
-      g *= rr;
+      if (!(g_defect == DEFECT_GALI_LASTFACTOR && j == k - 1)) g *= rr;
       st.gali[j - 1] = g;
     }
   }
@@ -1277,19 +1282,30 @@ struct MapSys {
   int kind;
   T K;
   int s;     // state length
-  T A[4];    // linear map, column-major (kind OR_SYS_LINEAR)
+  T A[64];   // linear map, column-major dl x dl (kind OR_SYS_LINEAR)
+  int dl = 2;  // dimension of the linear map (2..8)
   // tangent at x applied to W (d x k), explicit-fma product; then the step
   void step(T* x, T* const* W, int nf, int k) const {
     if (kind == OR_SYS_LINEAR) {
+      // fused_product (product.jl:9-20): C[i,j] = A[i,1] W[1,j], then fma(A[i,q], W[q,j], C[i,j]) for q = 2..d
       for (int f = 0; f < nf; ++f)
         for (int j = 0; j < k; ++j) {
-          T w0 = W[f][2 * j], w1 = W[f][2 * j + 1];
-          W[f][2 * j] = P::fma(A[2], w1, A[0] * w0);
-          W[f][2 * j + 1] = P::fma(A[3], w1, A[1] * w0);
+          T w[8], c[8];
+          for (int q = 0; q < dl; ++q) w[q] = W[f][dl * j + q];
+          for (int i = 0; i < dl; ++i) {
+            c[i] = A[i] * w[0];
+            for (int q = 1; q < dl; ++q) c[i] = P::fma(A[q * dl + i], w[q], c[i]);
+          }
+          for (int i = 0; i < dl; ++i) W[f][dl * j + i] = c[i];
         }
-      T x0 = x[0], x1 = x[1];
-      x[0] = A[0] * x0 + A[2] * x1;
-      x[1] = A[1] * x0 + A[3] * x1;
+      // x' = A x, SMatrix * SVector order: A[i,1] x1 + A[i,2] x2 + ...
+      T xo[8], xn[8];
+      for (int q = 0; q < dl; ++q) xo[q] = x[q];
+      for (int i = 0; i < dl; ++i) {
+        xn[i] = A[i] * xo[0];
+        for (int q = 1; q < dl; ++q) xn[i] = xn[i] + A[q * dl + i] * xo[q];
+      }
+      for (int i = 0; i < dl; ++i) x[i] = xn[i];
       return;
     }
     T sn = P::sin(x[0]), cs = P::cos(x[0]);
@@ -1336,11 +1352,11 @@ struct MapSys {
       }
     }
   }
-  int jd() const { return 2; }  // the size of the one-step tangent the monitors read
+  int jd() const { return kind == OR_SYS_LINEAR ? dl : 2; }  // the size of the one-step tangent the monitors read
   // the one-step tangent at x, column-major: [[1 + K cos x, 1], [K cos x, 1]] (standard_map.jl:30-34)
   void jac(const T* x, T* J) const {
     if (kind == OR_SYS_LINEAR) {
-      for (int i = 0; i < 4; ++i) J[i] = A[i];
+      for (int i = 0; i < dl * dl; ++i) J[i] = A[i];
       return;
     }
     const T Kx = kind == CL_SYS_STANDARD_MAP_K ? x[2] : K;
@@ -1350,8 +1366,17 @@ struct MapSys {
     J[2] = (T)1;
     J[3] = (T)1;
   }
-  // map displacement field at x (OFLI direction on a map)
+  // map displacement field F(x) - x at x (OFLI direction on a map); (A - I) x for the linear map, formed with the
+  // diagonal reduced first so that a shear [[I, hI], [0, I]] gives exactly (h p, 0), the free particle's direction
   void field(const T* x, T* f) const {
+    if (kind == OR_SYS_LINEAR) {
+      for (int i = 0; i < dl; ++i) {
+        T ax = (A[i] - (i == 0 ? (T)1 : (T)0)) * x[0];
+        for (int q = 1; q < dl; ++q) ax = ax + (A[q * dl + i] - (q == i ? (T)1 : (T)0)) * x[q];
+        f[i] = ax;
+      }
+      return;
+    }
     T ks = (kind == CL_SYS_STANDARD_MAP_K ? x[2] : K) * P::sin(x[0]);
     f[0] = x[1] + ks;
     f[1] = ks;
@@ -1359,7 +1384,7 @@ struct MapSys {
 };
 
 
-// The product of a map with itself (src/Systems/pair.jl:1-77): state (A; B), each half stepped by the base map with
+// The product of a map with itself (src/Systems/pair.jl:1-68): state (A; B), each half stepped by the base map with
 // its own block of the frame's rows (1..2 for A, 3..4 for B), the monitor reading orbit A (run_indicator_record.jl:6-10).
 template <class P, class T>
 struct PairSys {
@@ -1623,7 +1648,7 @@ template <class P, class T, class SYS>
 static void red_monitor(const cl_reducer& r, MonSt<T>& st, int64_t n, const SYS& sys, const T* x, const T* W, int d,
                         int k) {
   if (st.code) return;
-  T J[16];
+  T J[64] = {};
   sys.jac(x, J);
   T mx = P::abs(J[0]);
   for (int e = 1; e < d * d; ++e) mx = jmax<P>(mx, P::abs(J[e]));
@@ -1650,7 +1675,7 @@ static void red_monitor(const cl_reducer& r, MonSt<T>& st, int64_t n, const SYS&
 // max |M| of the one-step tangent; never fails
 template <class P, class T, class SYS>
 static void red_runmonitor(MonSt<T>& st, const SYS& sys, const T* x, int d) {
-  T J[16];
+  T J[64] = {};
   sys.jac(x, J);
   T mx = P::abs(J[0]);
   for (int e = 1; e < d * d; ++e) mx = jmax<P>(mx, P::abs(J[e]));
@@ -1761,7 +1786,8 @@ static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, in
   ms.kind = sys->kind;
   ms.K = (T)sys->K;
   ms.s = s;
-  for (int i = 0; i < 4; ++i) ms.A[i] = A ? (T)A[i] : (T)0;
+  ms.dl = (A && sys->kind == OR_SYS_LINEAR) ? d : 2;
+  for (int i = 0; i < ms.dl * ms.dl; ++i) ms.A[i] = A ? (T)A[i] : (T)0;
   const bool pair = sys->kind == CL_SYS_PAIR_STANDARD_MAP || sys->kind == CL_SYS_PAIR_WRAPPED_STANDARD_MAP;
   PairSys<P, T> ps;
   ps.base.kind = sys->kind == CL_SYS_PAIR_STANDARD_MAP ? CL_SYS_STANDARD_MAP : CL_SYS_WRAPPED_STANDARD_MAP;
@@ -1987,7 +2013,7 @@ static int record_all(const cl_system* sys, const cl_chain* ch, int64_t m, int s
 // ------------------------------------------------------------------ two-trajectory estimator
 // The reference orbit and a shadow at distance d0, stepped together and placed back at d0 along the separation
 // every tau steps (_shadow_body, src/Lyapunov/two_trajectory.jl:45-82). Separation and placement
-// (src/Lyapunov/separation.jl:9-40): the plain difference and ref + delta on the lifted map and the linear map; on
+// (src/Lyapunov/separation.jl:9-38): the plain difference and ref + delta on the lifted map and the linear map; on
 // the wrapped map the minimal image of each coordinate in (-pi, pi] and the wrapped placement whose reduction counts
 // are added to the reference's.
 struct ShadowRec {  // ShadowEnd{s, Float64} (two_trajectory.jl:11-19)
@@ -2413,6 +2439,23 @@ int or_propagate_ws(const cl_system* sys, const double* A, const cl_chain* ch, i
                                 A, ws);
   return propagate_all<F64Libm>(sys, ch, m, s, d, k, N, tau, x, W, nullptr, (uint8_t*)states, code, hstep, nthreads,
                                 A, ws);
+}
+
+// Switch a planted defect on (id > 0) or off (0); see g_defect.
+void or_plant_defect(int id) { g_defect = id; }
+
+// A linear map x' = A x of dimension d (2..8, A column-major d x d), k <= d columns: the closed-form fixtures of the
+// reference's suites in more than two dimensions (the free particle of test/ofli/of_ac01_05.jl:1-40 with A = [[I, hI],
+// [0, I]], prescribed frames for GALI with A = I). math as in or_propagate; quad != 0 runs in __float128.
+int or_propagate_linear_d(const double* A, int d, const cl_chain* ch, int64_t m, int k, int64_t N, int tau, double* x,
+                          double* W, void* states, int8_t* code, int64_t* hstep, int quad_) {
+  cl_system sys;
+  memset(&sys, 0, sizeof(sys));
+  sys.kind = OR_SYS_LINEAR;
+  if (d < 2 || d > 8 || k < 1 || k > d || N % tau) return 10;
+  if (quad_)
+    return propagate_all<Quad>(&sys, ch, m, d, d, k, N, tau, x, W, nullptr, (uint8_t*)states, code, hstep, 1, A);
+  return propagate_all<F64Cl>(&sys, ch, m, d, d, k, N, tau, x, W, nullptr, (uint8_t*)states, code, hstep, 1, A);
 }
 
 // A linear map x' = A x (A column-major 2 x 2): the closed-form fixtures of the reference's suites

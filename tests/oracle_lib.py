@@ -31,6 +31,10 @@ def load():
                                         I, P, P, P, P, P, P, I]
         o.or_propagate_linear.restype = I
         o.or_propagate_linear.argtypes = [P, ctypes.POINTER(L.cl_chain), I64, I, I64, I, P, P, P, P, P, I]
+        o.or_propagate_linear_d.restype = I
+        o.or_propagate_linear_d.argtypes = [P, I, ctypes.POINTER(L.cl_chain), I64, I, I64, I, P, P, P, P, P, I]
+        o.or_plant_defect.restype = None
+        o.or_plant_defect.argtypes = [I]
         o.or_initial_deviations.restype = I
         o.or_initial_deviations.argtypes = [ctypes.c_uint64, I64, I64, I, I, P]
         o.or_propagate_chain.restype = I
@@ -155,6 +159,42 @@ def run_linear(A, chain, x0, W0, N, tau, k, quad=False):
                                _ptr(hstep), int(quad))
     assert rc == 0, rc
     return dict(x=x, W=W.transpose(0, 2, 1).copy(), states=states[:, :nb], code=code, hstep=hstep)
+
+
+def run_linear_d(A, chain, x0, W0, N, tau, k, quad=False):
+    """The linear map x' = A x in d = A.shape[0] dimensions (2..8): x0 (m, d), W0 (m, d, k). Returns x, W (m, d, k),
+    raw states, code and hstep."""
+    o = load()
+    A = np.asarray(A, dtype=np.float64)
+    d = A.shape[0]
+    x = np.ascontiguousarray(np.array(x0, dtype=np.float64).reshape(-1, d))
+    m = x.shape[0]
+    W = np.array(np.asarray(W0, dtype=np.float64).reshape(m, d, k).transpose(0, 2, 1), order="C", copy=True)
+    Ac = np.ascontiguousarray(A.T.ravel())  # column-major
+    nb = o.or_state_bytes(ctypes.byref(chain), d, k)
+    states = np.zeros((m, max(nb, 1)), dtype=np.uint8)
+    code = np.zeros(m, dtype=np.int8)
+    hstep = np.zeros(m, dtype=np.int64)
+    rc = o.or_propagate_linear_d(_ptr(Ac), d, ctypes.byref(chain), m, k, N, tau, _ptr(x), _ptr(W), _ptr(states),
+                                 _ptr(code), _ptr(hstep), int(quad))
+    assert rc == 0, rc
+    return dict(x=x, W=W.transpose(0, 2, 1).copy(), states=states[:, :nb], code=code, hstep=hstep)
+
+
+class planted:
+    """Context manager: the oracle runs with a planted defect (oracle.cpp g_defect) inside the block."""
+    GALI_RAW, OFLI_NOPROJ, MEGNO_Q2, GALI_LASTFACTOR = 1, 2, 3, 4
+
+    def __init__(self, defect):
+        self.defect = defect
+
+    def __enter__(self):
+        load().or_plant_defect(self.defect)
+        return self
+
+    def __exit__(self, *exc):
+        load().or_plant_defect(0)
+        return False
 
 
 def frames(seed, orbit0, m, d, k):

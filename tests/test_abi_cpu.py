@@ -64,7 +64,7 @@ def test_state_bytes_agree(chain):
 
 def test_state_bytes_reference_layouts():
     """Sizes of the reference's isbits states (Julia C-alignment): QRState{Float64,2} 104 B, MEGNOState 200 B,
-    SALIState 72 B, FLIState 96 B, BenettinState 48 B (src/Lyapunov/qr_spectrum.jl:169-180 and the indicator files)."""
+    SALIState 72 B, FLIState 96 B, BenettinState 48 B (src/Lyapunov/qr_spectrum.jl:22-33 and the indicator files)."""
     d = {QRSpectrum(MGS()): 104, MEGNO(): 200, SALI(): 72, FLI(): 96}
     for r, b in d.items():
         assert r.dtype(2, 2).itemsize == b
@@ -116,7 +116,7 @@ def test_host_math_accuracy():
 
 
 def test_chunk_validation_errors():
-    """Argument errors come back as code 10 before any device work (src/Tangent/orbit.jl:187-191)."""
+    """Argument errors come back as code 10 before any device work (src/Tangent/orbit.jl:70-74)."""
     lib = L.load()
     sysd = StandardMap(1.0).descriptor()
     ch, _ = make_chain(())

@@ -192,8 +192,8 @@ def test_k0_shear_closed_form_on_gpu():
 
 def test_health_errors_match_reference_rules():
     """A non-finite start raises NumericalHealthError('state') at the first handoff; a zero frame column stops a QR
-    spectrum with an orthonormalisation breakdown at its pivot (src/Tangent/orbit.jl:171-174,
-    src/Lyapunov/qr_spectrum.jl:190-195); codes and steps equal the oracle's."""
+    spectrum with an orthonormalisation breakdown at its pivot (src/Tangent/orbit.jl:53-57,
+    src/Lyapunov/qr_spectrum.jl:44-60); codes and steps equal the oracle's."""
     say("health: non-finite start")
     x0 = np.array([[0.1, 0.2], [np.inf, 0.0], [0.3, 0.4]])
     with pytest.raises(NumericalHealthError) as e:

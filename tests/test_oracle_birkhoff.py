@@ -214,7 +214,7 @@ def test_frozen_horizons_and_failure():
 
 
 def test_index_rules():
-    """birkhoff_index (wbirkhoff.jl:182-199): swapping the pair negates delta and keeps the digits; S = 0 gives NaN
+    """birkhoff_index (wbirkhoff.jl:144-171): swapping the pair negates delta and keeps the digits; S = 0 gives NaN
     digits with zero and below_floor set; a denormal S floors phi at floatmin; an overflowing delta gives NaN digits
     and no flags; reldig is NaN where m <= phi and dig is then absdig."""
     a, b, S = 0.3, 0.3 + 1e-9, 1.0
