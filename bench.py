@@ -231,6 +231,75 @@ def c3_flops(N, k=16, tau=10):
     return 6 * N * 20 + 8 * N * k + (3 * 2 * (2 * N) * k * k + k ** 3 / 3) / tau
 
 
+def qr_lambdas(dst, k):
+    """lambda_i = total(sums_i) / n of QRState{Float64, k} rows (device int64 words [m][2k + 9])."""
+    w = dst.cpu().numpy().view(np.float64)
+    n = dst[:, 2 * k].cpu().numpy().astype(np.float64)
+    s = w[:, 0:2 * k:2] + w[:, 1:2 * k:2]
+    return s / np.where(n > 0, n, 1.0)[:, None]
+
+
+def c3_mixed(a, lib, ctx, chain, sets, m, k, tau, desc):
+    """C3 as one persistent launch over the five lattices per chunk (cl_propagate_mixed), chunks of --c3-chunk
+    steps captured in one graph."""
+    import torch
+    from chaos_amd import _lib as L
+    st = torch.cuda.current_stream()
+    lats = (L.cl_lattice * len(sets))()
+    for j, S in enumerate(sets):
+        lats[j].N, lats[j].m, lats[j].Ns = S["N"], m, S["Ns"].ctypes.data
+        lats[j].dx, lats[j].dW, lats[j].dstates = S["dx"].data_ptr(), S["dW"].data_ptr(), S["dst"].data_ptr()
+        lats[j].dcode, lats[j].dhstep = S["dc"].data_ptr(), S["dh"].data_ptr()
+    warm = [np.full(m, 20, dtype=np.int64) for _ in sets]
+
+    def run(Ns_list, chunk):
+        for j, S in enumerate(sets):
+            S["dx"].copy_(S["xm"])
+            assert lib.cl_initial_deviations_device(ctx, 3, 1, m, 2 * S["N"], k, S["dW"].data_ptr(), st.cuda_stream) == 0
+            lats[j].Ns = Ns_list[j].ctypes.data
+        rc = lib.cl_propagate_mixed(ctx, ctypes.byref(sets[0]["sd"]), ctypes.byref(chain), 1, len(sets), lats, k, tau,
+                                    chunk, st.cuda_stream)
+        assert rc == 0, lib.cl_last_error(ctx)
+
+    run(warm, 0)
+    torch.cuda.synchronize()
+    log("warmup done")
+    Nsl = [S["Ns"] for S in sets]
+    t0 = time.perf_counter()
+    evs = []
+    for _ in range(a.steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        run(Nsl, a.c3_chunk)
+        e1.record(st)
+        evs.append((e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    span = float(np.mean([x.elapsed_time(y) for x, y in evs]))
+    steps_n = {S["N"]: float(S["Ns"].sum()) for S in sets}
+    total = sum(steps_n.values())
+    flops = sum(c3_flops(N) * steps_n[N] for N in steps_n)
+    achieved = flops / (span * 1e-3) / 1e12
+    bad = int(sum(int((S["dc"] != 0).sum().item()) for S in sets))
+    lam8 = qr_lambdas(sets[0]["dst"], k)  # N = 8: k = 16 = 2N, the full spectrum
+    pair = np.max(np.abs(lam8 + lam8[:, ::-1]), axis=1)
+    nsteps = sets[0]["Ns"].astype(np.float64)
+    out = {
+        "metric": METRIC, "value": total * a.steps / wall, "unit": "orbit-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": 1, "ms_per_step": wall * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": desc + f"; one persistent launch per chunk of {a.c3_chunk or 'all'} steps",
+                   "orbit_steps_per_step": total, "nonfinite_orbits": bad,
+                   "pairing_n8": {"max": float(pair.max()), "max_times_n": float(np.max(pair * nsteps)),
+                                  "sum_max": float(np.max(np.abs(lam8.sum(axis=1))))}},
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel_ms": span},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return out
+
+
 def c3_main(a):
     import torch
     from chaos_amd import StandardMapChain, QRSpectrum, HouseholderQR, _lib as L
@@ -269,6 +338,8 @@ def c3_main(a):
                                      sh)
         assert rc == 0, lib.cl_last_error(ctx)
 
+    if a.c3_mode == "mixed":
+        return c3_mixed(a, lib, ctx, chain, sets, m, k, tau, desc)
     for S in sets:  # warmup: 20 steps per orbit
         run(S, np.full(m, 20, dtype=np.int64))
     torch.cuda.synchronize()
@@ -717,6 +788,9 @@ def main():
     ap.add_argument("--c5-epochs", type=int, default=8)
     ap.add_argument("--cpu-orbits", type=int, default=32768, help="orbits per K in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--c3-mode", default="streams", choices=["streams", "mixed"],
+                    help="c3: one launch per lattice on five streams, or one persistent launch over all five")
+    ap.add_argument("--c3-chunk", type=int, default=0, help="c3 mixed: steps per graph-captured chunk (0: one chunk)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 1 GiB device-to-device (the known-size kernel that calibrates FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args()

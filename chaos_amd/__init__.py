@@ -8,7 +8,7 @@ from .systems import PairMap, pair_start, pair_frame, pair_distance, SeparationE
 from .reducers import (QRSpectrum, CGS, MGS, RGS, HouseholderQR, AugmentedQR, GivensQR, Benettin, MEGNO, SALI, FLI,
                        OFLI, GALI, ResidualMonitor, RunMonitor, monitor_exponents, det_threshold, symp_threshold, sum_threshold,
                        eps_sum, eps_pair, check_residual)
-from .tangent import propagate, propagate_ragged, PropagationResult, NumericalHealthError, initial_deviations
+from .tangent import propagate, propagate_ragged, propagate_mixed, PropagationResult, NumericalHealthError, initial_deviations
 from .two_trajectory import propagate_shadow
 from .reducers import RLI
 from .birkhoff import WeightedPhaseAdvance, PlanarObservable, LiftedAngleObservable, circle_distance, frequency_index
@@ -16,7 +16,7 @@ from .reducers import StretchingNumbers, stretching_range, spectral_distance, sp
 from .birkhoff import (WeightedBirkhoff, CosObservable, SinObservable, LiftedObservable, weighted_birkhoff,
                        birkhoff_index)
 
-__all__ = ["StandardMap", "StandardMapK", "StandardMapChain", "WrappedStandardMap", "HenonHeilesGauss", "hh_energy", "torus_state", "QRSpectrum", "CGS", "MGS", "RGS", "HouseholderQR",
+__all__ = ["propagate_mixed", "StandardMap", "StandardMapK", "StandardMapChain", "WrappedStandardMap", "HenonHeilesGauss", "hh_energy", "torus_state", "QRSpectrum", "CGS", "MGS", "RGS", "HouseholderQR",
            "AugmentedQR", "GivensQR", "Benettin", "MEGNO", "SALI", "FLI", "OFLI", "GALI", "ResidualMonitor", "RunMonitor", "monitor_exponents", "det_threshold",
            "symp_threshold", "sum_threshold", "eps_sum", "eps_pair", "check_residual", "propagate", "propagate_ragged",
            "PropagationResult", "NumericalHealthError", "initial_deviations", "propagate_shadow",

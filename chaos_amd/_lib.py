@@ -62,6 +62,12 @@ class cl_chain(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32), ("flags", ctypes.c_int32), ("r", cl_reducer * CL_MAX_REDUCERS)]
 
 
+class cl_lattice(ctypes.Structure):
+    _fields_ = [("N", ctypes.c_int32), ("pad_", ctypes.c_int32), ("m", ctypes.c_int64), ("Ns", ctypes.c_void_p),
+                ("dx", ctypes.c_void_p), ("dW", ctypes.c_void_p), ("dstates", ctypes.c_void_p),
+                ("dcode", ctypes.c_void_p), ("dhstep", ctypes.c_void_p)]
+
+
 class cl_probe_t(ctypes.Structure):
     _fields_ = [("loadable", ctypes.c_int32), ("functional", ctypes.c_int32), ("compiles", ctypes.c_int32),
                 ("gpu_faster", ctypes.c_int32), ("device", ctypes.c_char * 64)]
@@ -92,6 +98,8 @@ SIGNATURES = {
                                        I64, I64, I, P, P, P, P, P, P, P, I64, I, P]),
     "cl_propagate_ragged": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I64, I, I, I, P, P, I,
                                 P, P, P, P, P, P]),
+    "cl_propagate_mixed": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I, ctypes.POINTER(cl_lattice),
+                               I, I, I64, P]),
     "cl_record_rows": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I64, I, I, I, I, P, P, P, P, P,
                            P]),
     "cl_ind_record_rows": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I64, I, I, I, P, P, P, P, P,

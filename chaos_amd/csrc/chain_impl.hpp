@@ -294,7 +294,14 @@ __device__ inline void osync() {
 // (orthonormalize.jl:112-127), orth_loss (qr_spectrum.jl:4) and the log sums. The reflector loops are
 // unrolled (column indices are compile-time register indices); each thread keeps its rows of the reflectors
 // in LDS (Us, one slot per thread and reflector), the triangle R and its inverse too, per orbit (Ro, Rio).
-template <int NS, int KC>
+// Upper-triangular k x k factors in LDS, packed by columns: entry (r, l), r <= l, at l (l + 1) / 2 + r.
+template <int KC>
+struct Tri {
+  static constexpr int SIZE = KC * (KC + 1) / 2;
+  static __device__ __attribute__((always_inline)) constexpr int at(int r, int l) { return l * (l + 1) / 2 + r; }
+};
+
+template <int NS, int KC, int BLOCK>
 __device__ __attribute__((always_inline)) inline void qr_handoff(const Orb<NS>& o, const ChainArgs& a, int64_t n,
                                                                  dvec<KC>& Wq, dvec<KC>& Wp, double* Wo,
                                                                  CSum (&sums)[(KC + NS - 1) / NS], QRChainState& qs,
@@ -310,7 +317,7 @@ __device__ __attribute__((always_inline)) inline void qr_handoff(const Orb<NS>& 
     Wo[c * D + NS + t] = Wp[c];
   }
   int count = 0;
-  constexpr int BLOCK = Geo<NS>::BLOCK;
+  using TR = Tri<KC>;
   const int tid = (int)threadIdx.x;
 #pragma unroll
   for (int j = 0; j < NREF; ++j) {
@@ -359,14 +366,16 @@ __device__ __attribute__((always_inline)) inline void qr_handoff(const Orb<NS>& 
       if (t == j - NS) Wp[j] = rho;
     }
   }
-  // the upper triangle of the swept matrix (rows < k), unsigned, column-major with zeros below
+  // the upper triangle of the swept matrix (rows < k), unsigned, packed by columns
   if (t < KC) {
 #pragma unroll
-    for (int l = 0; l < KC; ++l) Ro[l * KC + t] = l >= t ? Wq[l] : 0.0;
+    for (int l = 0; l < KC; ++l)
+      if (l >= t) Ro[TR::at(t, l)] = Wq[l];
   }
   if (NS < KC && NS + t < KC) {
 #pragma unroll
-    for (int l = 0; l < KC; ++l) Ro[l * KC + NS + t] = l >= NS + t ? Wp[l] : 0.0;
+    for (int l = 0; l < KC; ++l)
+      if (l >= NS + t) Ro[TR::at(NS + t, l)] = Wp[l];
   }
   osync<NS>();
   // sign fix and pivot check (householder.jl:55-65); R(r, l) = sgn_r A(r, l)
@@ -374,7 +383,7 @@ __device__ __attribute__((always_inline)) inline void qr_handoff(const Orb<NS>& 
   int piv = 0;
 #pragma unroll
   for (int j = 0; j < KC; ++j) {
-    const double dj = Ro[j * KC + j];
+    const double dj = Ro[TR::at(j, j)];
     if (dj < 0.0) neg |= 1u << j;
     const double pv = (dj < 0.0 ? -1.0 : 1.0) * dj;
     if (piv == 0 && !(cl_isfinite(pv) && pv > 0.0)) piv = j + 1;
@@ -385,10 +394,10 @@ __device__ __attribute__((always_inline)) inline void qr_handoff(const Orb<NS>& 
     double mR = cl_fabs(Ro[0]);
 #pragma unroll 1
     for (int l = 0; l < KC; ++l)
-      for (int r = 0; r <= l; ++r) mR = fmax(mR, cl_fabs(Ro[l * KC + r]));
+      for (int r = 0; r <= l; ++r) mR = fmax(mR, cl_fabs(Ro[TR::at(r, l)]));
     const bool sc = (mR > 1e100 || mR < 1e-100) && cl_isfinite(mR) && mR > 0.0;
     auto getR = [&](int r, int l) -> double {
-      const double v = sgn(r) * Ro[l * KC + r];
+      const double v = sgn(r) * Ro[TR::at(r, l)];
       return sc ? v / mR : v;
     };
     // R^-1 column cc by its owner thread(s), rows from the bottom up (orthonormalize.jl:116-124)
@@ -416,7 +425,8 @@ __device__ __attribute__((always_inline)) inline void qr_handoff(const Orb<NS>& 
             if (l == r) Ri[l] = v;
         }
 #pragma unroll
-        for (int r = 0; r < KC; ++r) Rio[cc * KC + r] = Ri[r];
+        for (int r = 0; r < KC; ++r)
+          if (r <= cc) Rio[TR::at(r, cc)] = Ri[r];
       }
     }
     osync<NS>();
@@ -431,7 +441,7 @@ __device__ __attribute__((always_inline)) inline void qr_handoff(const Orb<NS>& 
 #pragma unroll 1
     for (int l = 1; l < KC; ++l)
       for (int r = 0; r <= l; ++r) {
-        const double u = getR(r, l), ui = Rio[l * KC + r];
+        const double u = getR(r, l), ui = Rio[TR::at(r, l)];
         sa = sa + u * u;
         sb = sb + ui * ui;
       }
@@ -464,7 +474,7 @@ __device__ __attribute__((always_inline)) inline void qr_handoff(const Orb<NS>& 
     Wp[l] = Wp[l] * sgn(l);
   }
   if (piv != 0) {
-    const double pv = sgn(piv - 1) * Ro[(piv - 1) * KC + piv - 1];
+    const double pv = sgn(piv - 1) * Ro[TR::at(piv - 1, piv - 1)];
     qs.code = 1;
     qs.column = piv;
     qs.value = pv;
@@ -504,7 +514,7 @@ __device__ __attribute__((always_inline)) inline void qr_handoff(const Orb<NS>& 
 #pragma unroll
       for (int s2 = 0; s2 < SPT; ++s2) {
         const int cc = t + s2 * NS;
-        if (cc < KC) sums[s2] = csum_add(sums[s2], cl_log(sgn(cc) * Ro[cc * KC + cc]));
+        if (cc < KC) sums[s2] = csum_add(sums[s2], cl_log(sgn(cc) * Ro[TR::at(cc, cc)]));
       }
       qs.n = n;
       qs.G_max = cl_jmax(qs.G_max, gw_val);
@@ -516,22 +526,17 @@ __device__ __attribute__((always_inline)) inline void qr_handoff(const Orb<NS>& 
   osync<NS>();  // Ro / Rio / Us are rewritten at the next handoff
 }
 
-// N >= 64: ask the allocator for two waves per SIMD (256 registers, a few spilled): measured on the C3 workload,
-// N = 128 925 ms against 1001 ms and N = 64 552 against 573; at N = 32 the spills cost more than the second
-// wave (484 against 453), so smaller N keep the whole register file.
-template <int NS, int KC, bool RING, bool HASQR>
-__global__ void __launch_bounds__(Geo<NS>::BLOCK) __attribute__((amdgpu_waves_per_eu(NS >= 64 ? 2 : 1)))
-chain_kernel(ChainArgs a) {
+// The orbits of one thread block: orbit slots g0 .. g0 + nslots - 1 of the launch order (slot = thread / N), each
+// run from the launch's step n0 for `steps` steps or, with per-orbit step counts, to min(N_i, n0 + steps); an orbit
+// whose count is reached (N_i <= n0) or that failed before n0 is retired and left untouched. LDS: xb (N = 128 only,
+// 128 k doubles), Us (2 NREF BLOCK doubles), Rs and Ris (one packed triangle per orbit slot each).
+template <int NS, int KC, bool RING, bool HASQR, int BLOCK>
+__device__ __attribute__((always_inline)) inline void chain_orbits(const ChainArgs& a, int64_t g0, int nslots,
+                                                                   double* xb, double* Us, double* Rs, double* Ris) {
   constexpr int D = 2 * NS;
-  constexpr int NREF = KC < D ? KC : KC - 1;
   constexpr int SPT = (KC + NS - 1) / NS;  // QR sums owned per thread
-  constexpr int OPB = Geo<NS>::OPB;
   constexpr int WORDS = 2 * KC + 9;
-  constexpr int BLOCK = Geo<NS>::BLOCK;
-  __shared__ double xb[NS > 64 ? 128 * KC : 1];
-  __shared__ double Us[HASQR ? 2 * NREF * BLOCK : 1];
-  __shared__ double Rs[HASQR ? OPB * KC * KC : 1];
-  __shared__ double Ris[HASQR ? OPB * KC * KC : 1];
+  using TR = Tri<KC>;
 
   Orb<NS> o;
   o.xb = xb;
@@ -539,14 +544,15 @@ chain_kernel(ChainArgs a) {
   o.base = ((int)threadIdx.x & 63) - ((int)threadIdx.x & 63) % NS;
   const int t = o.t;
   const int slot = (int)threadIdx.x / NS;
-  double* Ro = Rs + (HASQR ? slot * KC * KC : 0);
-  double* Rio = Ris + (HASQR ? slot * KC * KC : 0);
-  const int64_t g = (int64_t)blockIdx.x * OPB + slot;
-  const bool exists = g < a.m;
+  double* Ro = Rs + (HASQR ? slot * TR::SIZE : 0);
+  double* Rio = Ris + (HASQR ? slot * TR::SIZE : 0);
+  const int64_t g = g0 + slot;
+  const bool exists = slot < nslots && g < a.m;
   const int64_t i = exists ? (a.order ? a.order[g] : g) : 0;
-  bool active = exists && (a.n0 == 0 || a.code[i] == STATUS_OK);
   const int64_t Ni = exists ? (a.Ns ? a.Ns[i] : a.N) : 0;
-  const int64_t Hi = active ? (a.Ns ? Ni : a.steps) / a.tau : 0;
+  bool active = exists && (a.n0 == 0 || a.code[i] == STATUS_OK) && (!a.Ns || Ni > a.n0);
+  const int64_t nend = a.Ns ? (Ni < a.n0 + a.steps ? Ni : a.n0 + a.steps) : a.n0 + a.steps;
+  const int64_t Hi = active ? (nend - a.n0) / a.tau : 0;
   // handoffs of the group: every lane of a 64-lane group (or the 128-thread group) loops the same count
   int64_t Hmax = Hi;
   if constexpr (NS <= 64) {
@@ -649,7 +655,7 @@ chain_kernel(ChainArgs a) {
         code = badx ? STATUS_STATE : STATUS_W;
         done = true;
       } else if (HASQR && qs.code == 0) {
-        qr_handoff<NS, KC>(o, a, n, Wq, Wp, Wo, sums, qs, Us, Ro, Rio);
+        qr_handoff<NS, KC, BLOCK>(o, a, n, Wq, Wp, Wo, sums, qs, Us, Ro, Rio);
         if (qs.code != 0) {
           code = STATUS_REDUCER;
           done = true;
@@ -704,6 +710,23 @@ chain_kernel(ChainArgs a) {
     a.code[i] = code;
     a.hstep[i] = n;
   }
+}
+
+// N >= 64: ask the allocator for two waves per SIMD (256 registers, a few spilled): measured on the C3 workload,
+// N = 128 925 ms against 1001 ms and N = 64 552 against 573; at N = 32 the spills cost more than the second
+// wave (484 against 453), so smaller N keep the whole register file.
+template <int NS, int KC, bool RING, bool HASQR>
+__global__ void __launch_bounds__(Geo<NS>::BLOCK) __attribute__((amdgpu_waves_per_eu(NS >= 64 ? 2 : 1)))
+chain_kernel(ChainArgs a) {
+  constexpr int D = 2 * NS;
+  constexpr int NREF = KC < D ? KC : KC - 1;
+  constexpr int OPB = Geo<NS>::OPB;
+  constexpr int BLOCK = Geo<NS>::BLOCK;
+  __shared__ double xb[NS > 64 ? 128 * KC : 1];
+  __shared__ double Us[HASQR ? 2 * NREF * BLOCK : 1];
+  __shared__ double Rs[HASQR ? OPB * Tri<KC>::SIZE : 1];
+  __shared__ double Ris[HASQR ? OPB * Tri<KC>::SIZE : 1];
+  chain_orbits<NS, KC, RING, HASQR, BLOCK>(a, (int64_t)blockIdx.x * OPB, OPB, xb, Us, Rs, Ris);
 }
 
 template <int NS, int KC, bool RING, bool HASQR>

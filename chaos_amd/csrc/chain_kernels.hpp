@@ -18,6 +18,34 @@ struct ChainArgs {
   double K, eps, tol;
 };
 
+// The persistent launch over several lattices (chain_mixed.hip): one class per site count, work items of up to
+// `slots` orbits of one class taken from a device counter in list order.
+struct MixedClass {
+  ChainArgs a;  // the class's buffers, per-orbit step counts and launch order (n0, steps are the chunk's)
+  int N;        // sites
+  int slots;    // orbits per work item
+};
+struct MixedItem {
+  int32_t cls;     // class index
+  int32_t nslots;  // orbits in the item
+  int64_t g0;      // first orbit slot (index into the class's launch order)
+};
+struct MixedArgs {
+  const MixedClass* cls;   // device
+  const MixedItem* items;  // device, in the order they are taken
+  int64_t nitems;
+  unsigned* counter;       // device work counter of this launch, zero at its start
+  int64_t n0, steps;       // the chunk: global steps n0 .. n0 + steps (each orbit stops at its own count)
+};
+// Orbits per work item of the persistent launch for N sites (its LDS holds this many packed R triangles).
+int mixed_slots(int N);
+// Zeroes n work counters (a fill kernel, so the launch can be captured in a graph).
+int mixed_zero(unsigned* counters, int n, hipStream_t st);
+// Compute units of the current device (queried once; call before a graph capture).
+int mixed_cus();
+// One persistent launch over the items (ring or open chains, QRSpectrum(HouseholderQR()), k = 16).
+int mixed_launch(const MixedArgs& a, bool ring, hipStream_t st);
+
 // 1 when a kernel is built for N sites and k columns.
 int chain_supported(int N, int k);
 // Launches the kernel for (N, ring, with or without QRSpectrum(HouseholderQR())); 0 or an error code.

@@ -1,0 +1,107 @@
+// chain_mixed.hip - one persistent launch over coupled-chain lattices of several site counts (N in {8, 16, 32, 64,
+// 128}) with uneven per-orbit step counts. Each 128-thread group takes work items from a device counter until the
+// list is exhausted; an item is up to `slots` orbits of one lattice, and the group runs them with that lattice's body
+// (chain_orbits, the per-N kernels' own code, so an orbit's bits do not depend on the route). The host orders the
+// items longest first, so the uneven lengths of every lattice pack onto the compute units together, and runs the
+// launch in step chunks: an orbit whose count is reached, or that failed, retires and its items exit at once in
+// later chunks.
+#include "chain_impl.hpp"
+
+namespace cl {
+
+namespace chain_detail {
+
+constexpr int MIX_BLOCK = 128;
+constexpr int MIX_KC = 16;
+// LDS pool (doubles): Us for 128 threads and 16 reflectors, then N = 128's exchange buffer and its two packed
+// triangles; smaller N put their triangles where the exchange buffer would be (at most MIX_SLOTS_MAX orbits).
+constexpr int MIX_US = 2 * MIX_KC * MIX_BLOCK;
+constexpr int MIX_XB = 128 * MIX_KC;
+constexpr int MIX_TRI = Tri<MIX_KC>::SIZE;
+constexpr int MIX_POOL = MIX_US + MIX_XB + 2 * MIX_TRI;
+constexpr int MIX_SLOTS_MAX = (MIX_POOL - MIX_US) / (2 * MIX_TRI);
+
+template <int NS, bool RING>
+__device__ __attribute__((always_inline)) inline void mixed_item(const ChainArgs& a, const MixedItem& w, double* pool) {
+  double* Us = pool;
+  if constexpr (NS > 64) {
+    chain_orbits<NS, MIX_KC, RING, true, MIX_BLOCK>(a, w.g0, w.nslots, pool + MIX_US, Us, pool + MIX_US + MIX_XB,
+                                                     pool + MIX_US + MIX_XB + MIX_TRI);
+  } else {
+    double* Rs = pool + MIX_US;
+    chain_orbits<NS, MIX_KC, RING, true, MIX_BLOCK>(a, w.g0, w.nslots, nullptr, Us, Rs, Rs + MIX_SLOTS_MAX * MIX_TRI);
+  }
+}
+
+template <bool RING>
+__global__ void __launch_bounds__(MIX_BLOCK) __attribute__((amdgpu_waves_per_eu(1))) chain_mixed_kernel(MixedArgs m) {
+  __shared__ double pool[MIX_POOL];
+  __shared__ unsigned item_s;
+  for (;;) {
+    __syncthreads();  // the previous item's LDS use is over
+    if (threadIdx.x == 0) item_s = atomicAdd(m.counter, 1u);
+    __syncthreads();
+    const int64_t it = (int64_t)item_s;
+    if (it >= m.nitems) return;  // every thread of the group reads the same value: the exit is uniform
+    const MixedItem w = m.items[it];
+    const MixedClass& c = m.cls[w.cls];
+    ChainArgs a = c.a;
+    a.n0 = m.n0;
+    a.steps = m.steps;
+    switch (c.N) {
+      case 8: mixed_item<8, RING>(a, w, pool); break;
+      case 16: mixed_item<16, RING>(a, w, pool); break;
+      case 32: mixed_item<32, RING>(a, w, pool); break;
+      case 64: mixed_item<64, RING>(a, w, pool); break;
+      case 128: mixed_item<128, RING>(a, w, pool); break;
+      default: break;
+    }
+  }
+}
+
+__global__ void k_zero_counters(unsigned* c, int n) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) c[i] = 0u;
+}
+
+}  // namespace chain_detail
+
+int mixed_slots(int N) {
+  using namespace chain_detail;
+  if (N > 64) return 1;
+  const int s = MIX_BLOCK / N;
+  return s < MIX_SLOTS_MAX ? s : MIX_SLOTS_MAX;
+}
+
+int mixed_zero(unsigned* counters, int n, hipStream_t st) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(chain_detail::k_zero_counters, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, counters, n);
+  return hipGetLastError() == hipSuccess ? 0 : CL_ERR_DEVICE_LOST;
+}
+
+int mixed_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+      cus = n;
+    if (cus <= 0) cus = 256;
+  }
+  return cus;
+}
+
+int mixed_launch(const MixedArgs& a, bool ring, hipStream_t st) {
+  using namespace chain_detail;
+  if (a.nitems <= 0) return 0;
+  // three groups per compute unit (the LDS pool allows three), never more groups than items
+  int64_t grid = (int64_t)mixed_cus() * 3;
+  if (grid > a.nitems) grid = a.nitems;
+  if (ring)
+    hipLaunchKernelGGL(chain_mixed_kernel<true>, dim3((unsigned)grid), dim3(MIX_BLOCK), 0, st, a);
+  else
+    hipLaunchKernelGGL(chain_mixed_kernel<false>, dim3((unsigned)grid), dim3(MIX_BLOCK), 0, st, a);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (e == hipErrorOutOfMemory ? CL_ERR_OOM : CL_ERR_DEVICE_LOST);
+}
+
+}  // namespace cl

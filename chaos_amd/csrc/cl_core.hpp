@@ -86,20 +86,22 @@ CL_HD double dot(const double* a, const double* b) {
 }
 
 // ------------------------------------------------------------------------------ frames
-template <int D, int K>
+// A d x k deviation frame. E is double, or float for the FP32-tangent element type (the frame is carried in Float32
+// between handoffs and widened to double for the reducers, which orthonormalise and sum logs in FP64).
+template <int D, int K, class E = double>
 struct Frame {
-  double w[D * K];  // column-major: w[col * D + row]
-  CL_HD double& at(int r, int c) { return w[c * D + r]; }
-  CL_HD double at(int r, int c) const { return w[c * D + r]; }
-  CL_HD double* col(int c) { return w + c * D; }
-  CL_HD const double* col(int c) const { return w + c * D; }
+  E w[D * K];  // column-major: w[col * D + row]
+  CL_HD E& at(int r, int c) { return w[c * D + r]; }
+  CL_HD E at(int r, int c) const { return w[c * D + r]; }
+  CL_HD E* col(int c) { return w + c * D; }
+  CL_HD const E* col(int c) const { return w + c * D; }
 };
 
-template <int D, int K>
-CL_HD bool frame_finite(const Frame<D, K>& W) {
+template <int D, int K, class E>
+CL_HD bool frame_finite(const Frame<D, K, E>& W) {
   bool ok = true;
 #pragma unroll
-  for (int i = 0; i < D * K; ++i) ok = ok && cl_isfinite(W.w[i]);
+  for (int i = 0; i < D * K; ++i) ok = ok && cl_isfinite((double)W.w[i]);
   return ok;
 }
 
@@ -114,6 +116,20 @@ CL_HD void map_tangent_apply(double c, Frame<2, K2>& W) {
     double w0 = W.at(0, j), w1 = W.at(1, j);
     W.at(0, j) = cl_fma(1.0, w1, m11 * w0);
     W.at(1, j) = cl_fma(1.0, w1, c * w0);
+  }
+}
+
+// The same product on a Float32 frame: the tangent's entries are formed in FP64 from the FP64 orbit (1 + c and c,
+// exactly as above) and rounded once to Float32; the product is the explicit-fma one in Float32 (product.jl:9-20 is
+// defined for Float32 as for Float64).
+template <int K2>
+CL_HD void map_tangent_apply(double c, Frame<2, K2, float>& W) {
+  const float m11 = (float)(1.0 + c), m21 = (float)c;
+#pragma unroll
+  for (int j = 0; j < K2; ++j) {
+    float w0 = W.at(0, j), w1 = W.at(1, j);
+    W.at(0, j) = __builtin_fmaf(1.0f, w1, m11 * w0);
+    W.at(1, j) = __builtin_fmaf(1.0f, w1, m21 * w0);
   }
 }
 
@@ -139,8 +155,8 @@ struct StdMap {
   // the one parameter c = K cos x of the tangent at the current point
   CL_HD double tan_c(const Aux& a) const { return K * a.cs; }
   // W <- M W with fused_product order: C[i,j] = M[i,1] W[1,j], then fma(M[i,2], W[2,j], acc).
-  template <int K2>
-  CL_HD void tangent_apply(const Aux& a, Frame<2, K2>& W) const { map_tangent_apply(tan_c(a), W); }
+  template <int K2, class E>
+  CL_HD void tangent_apply(const Aux& a, Frame<2, K2, E>& W) const { map_tangent_apply(tan_c(a), W); }
   // the one-step tangent at the current point, column-major (the ResidualMonitor reads it)
   CL_HD void jacobian(const Aux& a, double* J) const { map_jacobian(tan_c(a), J); }
   CL_HD void forward(const Aux& a, double* x) const {
@@ -171,8 +187,8 @@ struct StdMapK {
     a.K = x[2];
   }
   CL_HD double tan_c(const Aux& a) const { return a.K * a.cs; }
-  template <int K2>
-  CL_HD void tangent_apply(const Aux& a, Frame<2, K2>& W) const { map_tangent_apply(tan_c(a), W); }
+  template <int K2, class E>
+  CL_HD void tangent_apply(const Aux& a, Frame<2, K2, E>& W) const { map_tangent_apply(tan_c(a), W); }
   // the one-step tangent at the current point, column-major (the ResidualMonitor reads it)
   CL_HD void jacobian(const Aux& a, double* J) const { map_jacobian(tan_c(a), J); }
   CL_HD void forward(const Aux& a, double* x) const {
@@ -197,8 +213,8 @@ struct WrappedStdMap {
   };
   CL_HD void aux(const double* x, Aux& a) const { cl_sincos(x[0], &a.sn, &a.cs); }
   CL_HD double tan_c(const Aux& a) const { return K * a.cs; }
-  template <int K2>
-  CL_HD void tangent_apply(const Aux& a, Frame<2, K2>& W) const { map_tangent_apply(tan_c(a), W); }
+  template <int K2, class E>
+  CL_HD void tangent_apply(const Aux& a, Frame<2, K2, E>& W) const { map_tangent_apply(tan_c(a), W); }
   // the one-step tangent at the current point, column-major (the ResidualMonitor reads it)
   CL_HD void jacobian(const Aux& a, double* J) const { map_jacobian(tan_c(a), J); }
   static CL_HD void reduce(double u, double tp, double* r, double* a) {
@@ -255,9 +271,9 @@ struct PairMap {
     base.aux(x, au.a);
     base.aux(x + B::S, au.b);
   }
-  template <int K2>
-  CL_HD void tangent_apply(const Aux& au, Frame<D, K2>& W) const {
-    Frame<B::D, K2> wa, wb;
+  template <int K2, class E>
+  CL_HD void tangent_apply(const Aux& au, Frame<D, K2, E>& W) const {
+    Frame<B::D, K2, E> wa, wb;
 #pragma unroll
     for (int j = 0; j < K2; ++j)
 #pragma unroll

@@ -370,8 +370,24 @@ CL_HD int8_t frame_status(const OrbitCtx<Sys, K, NF>& c) {
   return wf ? STATUS_OK : STATUS_W;
 }
 
+// Status of the point and of carried Float32 frames (the FP32-tangent element type).
+template <class Sys, int K, int NF, class Ctx>
+CL_HD int8_t frame_status_f(const Ctx& c, const Frame<Sys::D, K, float> (&Wf)[NF]) {
+  bool xf = true;
+#pragma unroll
+  for (int i = 0; i < Sys::S; ++i) xf = xf && cl_isfinite(c.x[i]);
+  if (!xf) return STATUS_STATE;
+  bool wf = true;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) wf = wf && frame_finite(Wf[f]);
+  return wf ? STATUS_OK : STATUS_W;
+}
+
 // ------------------------------------------------------------------------------ the kernel
-template <class Sys, int K, int NF, class Chain>
+// F32T: the FP32-tangent element type. The frames are carried in Float32 between handoffs (the input frames are
+// rounded to Float32 at the start of the launch); at each handoff they are widened exactly to double, the chain runs
+// in FP64 as for the Float64 type, and the frames it leaves are rounded back to Float32. The orbit stays FP64.
+template <class Sys, int K, int NF, class Chain, bool F32T = false>
 __global__ void __launch_bounds__(256) orbit_kernel(KArgs a, Sys sys, Chain chain0) {
   constexpr int D = Sys::D;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -380,6 +396,7 @@ __global__ void __launch_bounds__(256) orbit_kernel(KArgs a, Sys sys, Chain chai
   if (!(a.n0 == 0 || a.code[i] == STATUS_OK)) return;
   OrbitCtx<Sys, K, NF> c;
   c.sys = sys;
+  Frame<D, K, float> Wf[NF];
 #pragma unroll
   for (int j = 0; j < Sys::S; ++j) c.x[j] = a.x[(int64_t)j * a.m + i];
 #pragma unroll
@@ -387,6 +404,12 @@ __global__ void __launch_bounds__(256) orbit_kernel(KArgs a, Sys sys, Chain chai
   if (NF > 1) {
 #pragma unroll
     for (int e = 0; e < D * K; ++e) c.W[NF - 1].w[e] = a.W1[(int64_t)e * a.m + i];
+  }
+  if constexpr (F32T) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int e = 0; e < D * K; ++e) Wf[f].w[e] = (float)c.W[f].w[e];
   }
   Chain ch = chain0;
   c.ws = a.ws ? a.ws + i : nullptr;
@@ -407,20 +430,36 @@ __global__ void __launch_bounds__(256) orbit_kernel(KArgs a, Sys sys, Chain chai
   for (int64_t h = 0; h < nh; ++h) {
     for (int t = 0; t < a.tau; ++t) {
 #pragma unroll
-      for (int f = 0; f < NF; ++f) sys.tangent_apply(c.aux, c.W[f]);
+      for (int f = 0; f < NF; ++f) {
+        if constexpr (F32T) sys.tangent_apply(c.aux, Wf[f]);
+        else sys.tangent_apply(c.aux, c.W[f]);
+      }
       sys.forward(c.aux, c.x);
       sys.aux(c.x, c.aux);
     }
     n += a.tau;
-    code = frame_status(c);
+    if constexpr (F32T) code = frame_status_f<Sys, K, NF>(c, Wf);
+    else code = frame_status(c);
     if (code != STATUS_OK) {
       early = true;
       break;
+    }
+    if constexpr (F32T) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int e = 0; e < D * K; ++e) c.W[f].w[e] = (double)Wf[f].w[e];
     }
 #pragma unroll
     for (int f = 0; f < NF; ++f) c.logs_valid[f] = false;
     chain_step(ch, n, c);
     trace_hook(ch, c, n, a, i);
+    if constexpr (F32T) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int e = 0; e < D * K; ++e) Wf[f].w[e] = (float)c.W[f].w[e];
+    }
     if (chain_failed(ch)) {
       code = STATUS_REDUCER;
       early = true;
@@ -430,10 +469,21 @@ __global__ void __launch_bounds__(256) orbit_kernel(KArgs a, Sys sys, Chain chai
 #pragma unroll
       for (int f = 0; f < NF; ++f)
 #pragma unroll
-        for (int e = 0; e < D * K; ++e) c.W[f].w[e] = (e % (D + 1) == 0) ? 1.0 : 0.0;
+        for (int e = 0; e < D * K; ++e) {
+          c.W[f].w[e] = (e % (D + 1) == 0) ? 1.0 : 0.0;
+          Wf[f].w[e] = (e % (D + 1) == 0) ? 1.0f : 0.0f;
+        }
     }
   }
-  if (!early) code = (n == a.N) ? frame_status(c) : STATUS_OK;
+  if constexpr (F32T) {
+    if (!early) code = (n == a.N) ? frame_status_f<Sys, K, NF>(c, Wf) : STATUS_OK;
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int e = 0; e < D * K; ++e) c.W[f].w[e] = (double)Wf[f].w[e];
+  } else {
+    if (!early) code = (n == a.N) ? frame_status(c) : STATUS_OK;
+  }
 #pragma unroll
   for (int j = 0; j < Sys::S; ++j) a.x[(int64_t)j * a.m + i] = c.x[j];
 #pragma unroll

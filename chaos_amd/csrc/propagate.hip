@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
 #include <mutex>
 #include <vector>
 #include "registry.hpp"
@@ -24,9 +25,10 @@ struct cl_ctx {
 
 static char g_create_err[256] = "";
 
+
 // ------------------------------------------------------------------------------ instance table
 static std::once_flag g_once;
-static Instance g_inst[256];
+static Instance g_inst[384];
 static int g_ninst = 0;
 
 static PipeInstance g_pipe[64];
@@ -34,8 +36,9 @@ static int g_npipe = 0;
 
 static void load_table() {
   std::call_once(g_once, [] {
-    register_instances_map(g_inst, &g_ninst, 256);
-    register_instances_flow(g_inst, &g_ninst, 256);
+    register_instances_map(g_inst, &g_ninst, 384);
+    register_instances_flow(g_inst, &g_ninst, 384);
+    register_instances_f32(g_inst, &g_ninst, 384);
     register_pipes(g_pipe, &g_npipe, 64);
   });
 }
@@ -53,11 +56,11 @@ static const PipeInstance* find_pipe(const cl_system* sys, const cl_chain* ch, i
   return nullptr;
 }
 
-static const Instance* find_instance(const cl_system* sys, const cl_chain* ch, int d, int k) {
+static const Instance* find_instance(const cl_system* sys, const cl_chain* ch, int d, int k, int elt = CL_ELT_F64) {
   load_table();
   for (int i = 0; i < g_ninst; ++i) {
     const Instance& in = g_inst[i];
-    if (in.sys_kind != sys->kind || in.d != d || in.k != k || in.nred != ch->n) continue;
+    if (in.elt != elt || in.sys_kind != sys->kind || in.d != d || in.k != k || in.nred != ch->n) continue;
     bool same = true;
     for (int r = 0; r < ch->n && same; ++r) {
       const cl_reducer& q = ch->r[r];
@@ -196,7 +199,8 @@ static int validate(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int e
     return set_err(ctx, CL_ERR_UNSUPPORTED, "Gauss kernels are built for s = 3 stages");
   if (s != state_dim(sys->kind)) return set_err(ctx, CL_ERR_ARGUMENT, "state length does not match the system");
   if (!(k >= 1 && k <= d)) return set_err(ctx, CL_ERR_ARGUMENT, "k must be in 1:d");
-  if (elt != CL_ELT_F64) return set_err(ctx, CL_ERR_UNSUPPORTED, "element type has no kernel for this system");
+  if (elt != CL_ELT_F64 && elt != CL_ELT_F64_F32TAN)
+    return set_err(ctx, CL_ERR_UNSUPPORTED, "element type has no kernel for this system");
   int nws = 0;
   for (int r = 0; r < ch->n; ++r) {
     const cl_reducer& q = ch->r[r];
@@ -210,8 +214,8 @@ static int validate(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int e
   if (nws > 1) return set_err(ctx, CL_ERR_ARGUMENT, "at most one reducer of a chain may request a workspace");
   if (const char* why = wb_check(ch, s)) return set_err(ctx, CL_ERR_ARGUMENT, why);
   if (const char* why = pa_check(ch, s)) return set_err(ctx, CL_ERR_ARGUMENT, why);
-  const Instance* in = find_instance(sys, ch, d, k);
-  if (!in) return set_err(ctx, CL_ERR_UNSUPPORTED, "no kernel instance for this system/chain/k");
+  const Instance* in = find_instance(sys, ch, d, k, elt);
+  if (!in) return set_err(ctx, CL_ERR_UNSUPPORTED, "no kernel instance for this system/chain/k/element type");
   *out = in;
   return CL_OK;
 }
@@ -346,7 +350,7 @@ extern "C" int cl_probe(int elt, cl_probe_t* out) {
   if (hipGetDeviceProperties(&p, 0) != hipSuccess) return CL_ERR_NO_DEVICE;
   out->functional = strncmp(p.gcnArchName, "gfx950", 6) == 0;
   snprintf(out->device, sizeof(out->device), "%s (%s)", p.name, p.gcnArchName);
-  out->compiles = out->functional && elt == CL_ELT_F64;
+  out->compiles = out->functional && (elt == CL_ELT_F64 || elt == CL_ELT_F64_F32TAN);
   out->gpu_faster = out->compiles;
   return out->functional ? CL_OK : CL_ERR_NO_DEVICE;
 }
@@ -400,14 +404,14 @@ extern "C" int cl_birkhoff_weights(int64_t N, int order, double* w) {
 
 extern "C" int64_t cl_state_bytes(const cl_system* sys, const cl_chain* ch, int elt, int d, int k) {
   if (!sys || !ch || ch->n < 0 || ch->n > CL_MAX_REDUCERS) return -CL_ERR_ARGUMENT;
-  if (elt != CL_ELT_F64) return -CL_ERR_UNSUPPORTED;
+  if (elt != CL_ELT_F64 && elt != CL_ELT_F64_F32TAN) return -CL_ERR_UNSUPPORTED;
   if (sys->kind == CL_SYS_STANDARD_MAP_CHAIN) {
     int words = 0;
     bool qr = false;
     int rc = validate_chain(nullptr, sys, ch, elt, 0, d, d, k, 1, &words, &qr);
     return rc ? -(int64_t)rc : (int64_t)words * 8;
   }
-  const Instance* in = find_instance(sys, ch, d, k);
+  const Instance* in = find_instance(sys, ch, d, k, elt);
   if (!in) return -CL_ERR_UNSUPPORTED;
   if (wb_check(ch, state_dim(sys->kind))) return -CL_ERR_ARGUMENT;
   return ((int64_t)in->words + wb_mem_words(ch)) * 8;
@@ -534,8 +538,8 @@ static int launch_split(const Instance* in, const cl_system* sys, const cl_chain
                         int64_t n0, int64_t steps, int64_t N, int tau, hipStream_t st) {
   cl_chain ca, cb;
   if (!split_chains(ch, &ca, &cb)) return CL_ERR_UNSUPPORTED;
-  const Instance* ia = find_instance(sys, &ca, d, k);
-  const Instance* ib = find_instance(sys, &cb, d, k);
+  const Instance* ia = find_instance(sys, &ca, d, k, in->elt);
+  const Instance* ib = find_instance(sys, &cb, d, k, in->elt);
   if (!ia || !ib || ia->nf != 1 || ib->nf != 1 || ia->words + ib->words != in->words) return CL_ERR_UNSUPPORTED;
   if (m == 0) return CL_OK;
   const int DK = d * k, words = in->words;
@@ -603,7 +607,7 @@ static int launch_pipe(const Instance* in, const cl_system* sys, const cl_chain*
                        double* dx, double* dW, uint64_t* dst, int8_t* dcode, int64_t* dhstep, int64_t n0,
                        int64_t steps, int64_t N, int tau, hipStream_t st) {
   const PipeInstance* pi = find_pipe(sys, ch, d, k);
-  if (!pi || in->nf != 1 || tau > PIPE_TAU_MAX || !pipe_enabled(m)) return CL_ERR_UNSUPPORTED;
+  if (!pi || in->elt != CL_ELT_F64 || in->nf != 1 || tau > PIPE_TAU_MAX || !pipe_enabled(m)) return CL_ERR_UNSUPPORTED;
   if (m == 0) return CL_OK;
   uint8_t* rerun = nullptr;
   hipError_t e = hipMallocAsync((void**)&rerun, (size_t)m, st);
@@ -846,6 +850,186 @@ extern "C" int cl_propagate_ragged(cl_ctx* ctx, const cl_system* sys, const cl_c
   e = hipFreeAsync(dbuf, st);
   if (rc) return set_err(ctx, rc, "chain kernel launch failed");
   return e == hipSuccess ? CL_OK : set_err(ctx, map_err(e), "stream-ordered free failed");
+}
+
+// ------------------------------------------------------------------------------ mixed lattices, one persistent launch
+// The host side of cl_propagate_mixed: validation, the per-lattice launch orders (longest first), the work list
+// (items of up to mixed_slots(N) orbits, longest item first over all lattices, weighted by N: a chain step costs in
+// proportion to its sites) and the device staging image, copied in one transfer.
+struct MixedPlan {
+  std::vector<char> image;  // staging image: classes | items | Ns per lattice | orders per lattice | counters
+  size_t off_items = 0, off_counters = 0;
+  int64_t nitems = 0, Nmax = 0;
+  int nchunks = 0;
+};
+
+static int mixed_plan(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int nlat, const cl_lattice* lat,
+                      int k, int tau, int64_t chunk, MixedPlan& P, std::vector<MixedClass>& cls) {
+  if (!sys || !ch || (nlat > 0 && !lat)) return set_err(ctx, CL_ERR_ARGUMENT, "null descriptor");
+  if (sys->kind != CL_SYS_STANDARD_MAP_CHAIN) return set_err(ctx, CL_ERR_UNSUPPORTED, "the mixed launch runs chains");
+  if (elt != CL_ELT_F64) return set_err(ctx, CL_ERR_UNSUPPORTED, "element type has no kernel for this system");
+  if (!(ch->n == 1 && ch->r[0].kind == CL_RED_QR && ch->r[0].alg == CL_QR_HOUSEHOLDER && ch->r[0].frame == 0))
+    return set_err(ctx, CL_ERR_UNSUPPORTED, "the mixed launch runs QRSpectrum(HouseholderQR())");
+  if (k != 16) return set_err(ctx, CL_ERR_UNSUPPORTED, "the mixed launch is built for k = 16");
+  if (nlat < 1 || nlat > 64) return set_err(ctx, CL_ERR_ARGUMENT, "1 to 64 lattices");
+  if (tau < 1 || chunk < 0 || chunk % tau) return set_err(ctx, CL_ERR_ARGUMENT, "chunk must be a multiple of tau");
+  const int words = 2 * k + 9;
+  size_t tot_orbits = 0;
+  for (int c = 0; c < nlat; ++c) {
+    const cl_lattice& L = lat[c];
+    if (!chain_supported(L.N, k)) return set_err(ctx, CL_ERR_UNSUPPORTED, "no chain kernel for this N");
+    if (sys->ring && L.N < 3) return set_err(ctx, CL_ERR_ARGUMENT, "a ring needs N >= 3");
+    if (L.m < 0) return set_err(ctx, CL_ERR_ARGUMENT, "negative orbit count");
+    if (L.m > 0 && (!L.Ns || !L.dx || !L.dW || !L.dstates || !L.dcode || !L.dhstep))
+      return set_err(ctx, CL_ERR_ARGUMENT, "null buffer");
+    for (int64_t i = 0; i < L.m; ++i) {
+      if (!(L.Ns[i] >= 1 && L.Ns[i] % tau == 0))
+        return set_err(ctx, CL_ERR_ARGUMENT, "each N_i must be a positive multiple of tau");
+      P.Nmax = L.Ns[i] > P.Nmax ? L.Ns[i] : P.Nmax;
+    }
+    tot_orbits += (size_t)L.m;
+  }
+  const int64_t len = chunk == 0 ? P.Nmax : chunk;
+  P.nchunks = P.Nmax == 0 ? 0 : (int)((P.Nmax + len - 1) / len);
+  if (P.nchunks > 100000) return set_err(ctx, CL_ERR_ARGUMENT, "more than 10^5 chunks");
+  // launch orders (longest first, ties by orbit index) and the work list
+  struct Item { MixedItem w; int64_t cost; };
+  std::vector<Item> items;
+  std::vector<std::vector<int64_t>> order((size_t)nlat);
+  for (int c = 0; c < nlat; ++c) {
+    const cl_lattice& L = lat[c];
+    std::vector<int64_t>& o = order[(size_t)c];
+    o.resize((size_t)L.m);
+    for (int64_t i = 0; i < L.m; ++i) o[(size_t)i] = i;
+    std::stable_sort(o.begin(), o.end(), [&](int64_t a, int64_t b) { return L.Ns[a] > L.Ns[b]; });
+    const int sl = mixed_slots(L.N);
+    for (int64_t g = 0; g < L.m; g += sl) {
+      Item it;
+      it.w.cls = c;
+      it.w.nslots = (int32_t)(L.m - g < sl ? L.m - g : sl);
+      it.w.g0 = g;
+      it.cost = L.Ns[o[(size_t)g]] * (int64_t)L.N;
+      items.push_back(it);
+    }
+  }
+  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
+  P.nitems = (int64_t)items.size();
+  auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  size_t off = al(sizeof(MixedClass) * (size_t)nlat);
+  P.off_items = off;
+  off = al(off + sizeof(MixedItem) * items.size());
+  const size_t off_ns = off;
+  off = al(off + 8 * tot_orbits);
+  const size_t off_ord = off;
+  off = al(off + 8 * tot_orbits);
+  P.off_counters = off;
+  off = al(off + 4 * (size_t)(P.nchunks > 0 ? P.nchunks : 1));
+  P.image.assign(off, 0);
+  for (size_t j = 0; j < items.size(); ++j)
+    memcpy(P.image.data() + P.off_items + j * sizeof(MixedItem), &items[j].w, sizeof(MixedItem));
+  cls.resize((size_t)nlat);
+  size_t o_ns = off_ns, o_ord = off_ord;
+  for (int c = 0; c < nlat; ++c) {
+    const cl_lattice& L = lat[c];
+    if (L.m > 0) {
+      memcpy(P.image.data() + o_ns, L.Ns, 8 * (size_t)L.m);
+      memcpy(P.image.data() + o_ord, order[(size_t)c].data(), 8 * (size_t)L.m);
+    }
+    MixedClass& mc = cls[(size_t)c];
+    memset(&mc, 0, sizeof(mc));
+    mc.N = L.N;
+    mc.slots = mixed_slots(L.N);
+    mc.a.x = L.dx;
+    mc.a.W = L.dW;
+    mc.a.st = L.dstates;
+    mc.a.code = L.dcode;
+    mc.a.hstep = L.dhstep;
+    mc.a.Ns = (const int64_t*)(uintptr_t)o_ns;   // offsets; rebased onto the device address by mixed_run
+    mc.a.order = (const int64_t*)(uintptr_t)o_ord;
+    mc.a.m = L.m;
+    mc.a.N = P.Nmax;
+    mc.a.tau = tau;
+    mc.a.K = sys->K;
+    mc.a.eps = sys->eps;
+    mc.a.tol = ch->r[0].tol;
+    o_ns += 8 * (size_t)L.m;
+    o_ord += 8 * (size_t)L.m;
+  }
+  (void)words;
+  return CL_OK;
+}
+
+// Copies the staging image to `base` (device memory of at least P.image.size() bytes) and runs the chunks on st:
+// one graph holding the counter fill and every chunk's launch (CL_GRAPH=0: the same launches directly).
+static int mixed_run(cl_ctx* ctx, MixedPlan& P, std::vector<MixedClass>& cls, char* base, bool ring, int64_t chunk,
+                     hipStream_t st, bool allow_graph = true) {
+  for (MixedClass& mc : cls) {
+    mc.a.Ns = (const int64_t*)(base + (uintptr_t)mc.a.Ns);
+    mc.a.order = (const int64_t*)(base + (uintptr_t)mc.a.order);
+  }
+  memcpy(P.image.data(), cls.data(), sizeof(MixedClass) * cls.size());
+  hipError_t e = hipMemcpyAsync(base, P.image.data(), P.image.size(), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return set_err(ctx, map_err(e), "copy of the work list failed");
+  if (P.nchunks == 0) return CL_OK;
+  const int64_t len = chunk == 0 ? P.Nmax : chunk;
+  unsigned* counters = (unsigned*)(base + P.off_counters);
+  auto enqueue = [&](hipStream_t s) -> int {
+    int rc = mixed_zero(counters, P.nchunks, s);
+    for (int c = 0; c < P.nchunks && !rc; ++c) {
+      MixedArgs a;
+      a.cls = (const MixedClass*)base;
+      a.items = (const MixedItem*)(base + P.off_items);
+      a.nitems = P.nitems;
+      a.counter = counters + c;
+      a.n0 = (int64_t)c * len;
+      a.steps = P.Nmax - a.n0 < len ? P.Nmax - a.n0 : len;
+      rc = mixed_launch(a, ring, s);
+    }
+    return rc;
+  };
+  const char* ge = getenv("CL_GRAPH");
+  const bool graph = allow_graph && P.nchunks > 1 && !(ge && ge[0] == '0');
+  if (!graph) {
+    int rc = enqueue(st);
+    return rc ? set_err(ctx, rc, "mixed chain launch failed") : CL_OK;
+  }
+  // capture on the context's own stream (never the legacy default stream), replay on st
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge2 = nullptr;
+  e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return set_err(ctx, map_err(e), "graph capture failed to start");
+  int rc = enqueue(ctx->stream);
+  hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+  if (rc || e2 != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    return set_err(ctx, rc ? rc : map_err(e2), "graph capture of the chunks failed");
+  }
+  e = hipGraphInstantiate(&ge2, g, nullptr, nullptr, 0);
+  if (e == hipSuccess) e = hipGraphLaunch(ge2, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (ge2) hipGraphExecDestroy(ge2);
+  hipGraphDestroy(g);
+  return e == hipSuccess ? CL_OK : set_err(ctx, map_err(e), "graph launch of the chunks failed");
+}
+
+extern "C" int cl_propagate_mixed(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int nlat,
+                                  const cl_lattice* lat, int k, int tau, int64_t chunk, void* stream) {
+  if (!ctx) return CL_ERR_ARGUMENT;
+  MixedPlan P;
+  std::vector<MixedClass> cls;
+  int rc = mixed_plan(ctx, sys, ch, elt, nlat, lat, k, tau, chunk, P, cls);
+  if (rc) return rc;
+  hipSetDevice(ctx->device);
+  mixed_cus();
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  char* base = nullptr;
+  hipError_t e = hipMallocAsync((void**)&base, P.image.size(), st);
+  if (e != hipSuccess) return set_err(ctx, map_err(e), "device allocation failed");
+  rc = mixed_run(ctx, P, cls, base, sys->ring != 0, chunk, st);
+  hipFreeAsync(base, st);
+  e = hipStreamSynchronize(st);
+  if (rc) return rc;
+  return e == hipSuccess ? CL_OK : set_err(ctx, map_err(e), "mixed chain launch failed");
 }
 
 // The chunk entry point for the chain: host and device buffers share the orbit-major layout.

@@ -12,6 +12,7 @@ namespace cl {
 
 struct Instance {
   int sys_kind, d, k, nf, words, nred;
+  int elt;  // CL_ELT_F64 or CL_ELT_F64_F32TAN
   int red[CL_MAX_REDUCERS][3];  // kind, alg, frame
   int (*launch)(const cl_system*, const cl_chain*, const KArgs&, hipStream_t);
 };
@@ -60,14 +61,14 @@ inline HHGauss make_sys<HHGauss>(const cl_system& s) {
 // orbits) uses 64-lane groups so its waves spread over as many CUs as possible.
 inline int block_for(int64_t m) { return m >= 65536 ? 256 : 64; }
 
-template <class Sys, int K, int NF, class Chain>
+template <class Sys, int K, int NF, class Chain, bool F32T = false>
 int launch_impl(const cl_system* s, const cl_chain* d, const KArgs& a, hipStream_t st) {
   Sys sys = make_sys<Sys>(*s);
   Chain ch;
   chain_setup(ch, *d, 0);
   int block = block_for(a.m);
   int64_t grid = (a.m + block - 1) / block;
-  hipLaunchKernelGGL((orbit_kernel<Sys, K, NF, Chain>), dim3((unsigned)grid), dim3(block), 0, st, a, sys, ch);
+  hipLaunchKernelGGL((orbit_kernel<Sys, K, NF, Chain, F32T>), dim3((unsigned)grid), dim3(block), 0, st, a, sys, ch);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (e == hipErrorOutOfMemory ? CL_ERR_OOM : CL_ERR_DEVICE_LOST);
 }
@@ -138,9 +139,10 @@ void fill_key(Instance& in, const Cons<H, T>& c, int i) {
   fill_key(in, c.tail, i + 1);
 }
 
-template <int SYSK, class Sys, int K, int NF, class Chain>
+template <int SYSK, class Sys, int K, int NF, class Chain, bool F32T = false>
 Instance make_instance() {
   Instance in;
+  in.elt = F32T ? CL_ELT_F64_F32TAN : CL_ELT_F64;
   in.sys_kind = SYSK;
   in.d = Sys::D;
   in.k = K;
@@ -150,13 +152,14 @@ Instance make_instance() {
   for (int i = 0; i < CL_MAX_REDUCERS; ++i) in.red[i][0] = in.red[i][1] = in.red[i][2] = 0;
   Chain c;
   fill_key(in, c, 0);
-  in.launch = &launch_impl<Sys, K, NF, Chain>;
+  in.launch = &launch_impl<Sys, K, NF, Chain, F32T>;
   return in;
 }
 
 // Each instance translation unit appends its entries.
 void register_instances_map(Instance* out, int* n, int cap);
 void register_instances_flow(Instance* out, int* n, int cap);
+void register_instances_f32(Instance* out, int* n, int cap);
 
 // Wave-specialised pipeline kernels for small map ensembles whose chain is one QRSpectrum on frame 1, alone or
 // followed by RunMonitor (pipe_kernel.hpp). `launch` marks in `rerun` the orbits the caller must rerun with the fused kernel.

@@ -133,12 +133,21 @@ def check_health(code, hstep, x, W, states, allr, orbit0=1):
     raise NumericalHealthError("state" if c == 1 else "deviation matrix", first, math.inf, int(hstep[i]), orbit, "gpu")
 
 
+ELEMENT_TYPES = {"f64": L.CL_ELT_F64, "f32tan": L.CL_ELT_F64_F32TAN}
+PRECISION_LABELS = {"f64": "Float64", "f32tan": "Float64 orbit, Float32 tangents"}
+
+
 def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None, launch_steps=None,
-              check=True, keep_states=False, reset_frame=False):
+              check=True, keep_states=False, reset_frame=False, elt="f64"):
     """Propagate every orbit of `x0` (shape (m, s)) for N steps with k deviation vectors on the GPU.
     reset_frame=True restarts the frame from the unit columns after every handoff (the indicator recorder's window
-    kinds, src/Experiments/run_indicator_device.jl:66-80); W0 is then the unit columns."""
+    kinds, src/Experiments/run_indicator_device.jl:66-80); W0 is then the unit columns.
+    elt="f32tan" carries the frames in Float32 between handoffs (FP64 orbit and Jacobian, FP64 reducers: the
+    orthonormalisation and the compensated log sums), on the standard-map family."""
     import torch
+    if elt not in ELEMENT_TYPES:
+        raise ValueError(f"elt must be one of {sorted(ELEMENT_TYPES)}, got {elt!r}")
+    E = ELEMENT_TYPES[elt]
     N, tau, k = int(N), int(tau), int(k)
     if not (tau >= 1 and N >= 1 and N % tau == 0):
         raise ValueError(f"N must be a positive multiple of tau, got N = {N}, tau = {tau}")
@@ -168,7 +177,7 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
     if reset_frame:
         W0 = np.broadcast_to(np.eye(d)[:, :k], (m, d, k))
     nws = workspace_entries(allr, d, k)
-    nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, d, k)
+    nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), E, d, k)
     if nbytes < 0:
         raise L.HIPError(-nbytes, "no kernel instance for this system, k and reducer chain")
     words = nbytes // 8
@@ -177,6 +186,8 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
     if sysd.kind == L.CL_SYS_STANDARD_MAP_CHAIN and nws:
         raise L.HIPError(3, "chain kernels run no workspace reducer")
     if sysd.kind == L.CL_SYS_STANDARD_MAP_CHAIN:
+        if E != L.CL_ELT_F64:
+            raise L.HIPError(3, "chain kernels are built for Float64")
         if reducers2:
             raise L.HIPError(3, "chain kernels carry one frame")
         return _propagate_orbit_major(lib, ctx, sys, sysd, chain, allr, x0, N, k, tau, seed, W0, launch_steps,
@@ -202,12 +213,12 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
     while n0 < N:
         steps = min(L_, N - n0)
         if dws is not None:
-            rc = lib.cl_propagate_device_ws(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, m, s, d, k,
+            rc = lib.cl_propagate_device_ws(ctx, ctypes.byref(sysd), ctypes.byref(chain), E, m, s, d, k,
                                             n0, steps, N, tau, dx.data_ptr(), dW.data_ptr(),
                                             dW2.data_ptr() if dW2 is not None else None, dst.data_ptr(),
                                             dcode.data_ptr(), dh.data_ptr(), dws.data_ptr(), stream)
         else:
-            rc = lib.cl_propagate_device(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, m, s, d, k, n0,
+            rc = lib.cl_propagate_device(ctx, ctypes.byref(sysd), ctypes.byref(chain), E, m, s, d, k, n0,
                                          steps, N, tau, dx.data_ptr(), dW.data_ptr(),
                                          dW2.data_ptr() if dW2 is not None else None, dst.data_ptr(),
                                          dcode.data_ptr(), dh.data_ptr(), stream)
@@ -226,7 +237,7 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
     ws = dws.cpu().numpy().view(np.uint32).T.copy() if dws is not None else None
     if check:
         check_health(code, hstep, x, W, states, allr)
-    prov = dict(backend="gpu", device=device_name(), backend_fallback_reason="", precision="Float64",
+    prov = dict(backend="gpu", device=device_name(), backend_fallback_reason="", precision=PRECISION_LABELS[elt],
                 seed=seed, tau=tau, k=k, chunk_size=m, gpu_chunks=1, gpu_chunk_size=m, launch_steps=L_,
                 launches_per_chunk=launches)
     for r, _ in allr:
@@ -321,6 +332,75 @@ def propagate_ragged(sys, x0, Ns, *, k, tau=1, seed, reducers=(), W0=None, order
         order = np.argsort(-Ns, kind="stable")
     return _propagate_orbit_major(lib, ctx, sys, sysd, chain, allr, x0, 0, k, tau, seed, W0, None, nbytes // 8,
                                   check, keep_states, Ns=Ns, order=order)
+
+
+def propagate_mixed(lattices, *, K, eps, boundary="ring", k=16, tau=1, seed, reducers=None, chunk=0, check=True,
+                    keep_states=False):
+    """Several coupled-chain lattices in one persistent launch per chunk (`cl_propagate_mixed`): `lattices` is a list of
+    (N, x0, Ns), x0 of shape (m, 2N), Ns the per-orbit step counts (positive multiples of tau). All share K, eps and
+    the boundary; the chain is QRSpectrum(HouseholderQR()) with k = 16. `chunk` is the step length of a chunk (0: one
+    chunk). Seeded frames use orbit ids 1..m within each lattice, as `propagate_ragged` on that lattice alone, whose
+    results this gives bit for bit. Returns one PropagationResult per lattice."""
+    import torch
+    from .systems import StandardMapChain
+    from .reducers import QRSpectrum, HouseholderQR
+    tau, k = int(tau), int(k)
+    if reducers is None:
+        reducers = (QRSpectrum(HouseholderQR()),)
+    lib, ctx = _context()
+    chain, allr = make_chain(reducers)
+    dev = torch.device("cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    systems, bufs, lats, keep = [], [], (L.cl_lattice * max(len(lattices), 1))(), []
+    for j, (N, x0, Ns) in enumerate(lattices):
+        sys = StandardMapChain(N, K, eps, boundary)
+        x0 = np.ascontiguousarray(np.asarray(x0, dtype=np.float64))
+        m, s = x0.shape
+        d = sys.dimension
+        Ns = np.ascontiguousarray(np.asarray(Ns, dtype=np.int64))
+        if s != sys.state_length or Ns.shape != (m,) or np.any(Ns < 1) or np.any(Ns % tau):
+            raise ValueError("each lattice needs x0 of shape (m, 2N) and one positive multiple of tau per orbit")
+        sysd = sys.descriptor()
+        nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, d, k)
+        if nbytes < 0:
+            raise L.HIPError(-nbytes, "no kernel instance for this system, k and reducer chain")
+        words = nbytes // 8
+        dx = torch.from_numpy(x0.copy()).to(dev)
+        dW = torch.empty((m, k, d), dtype=torch.float64, device=dev)
+        rc = lib.cl_initial_deviations_device(ctx, int(seed) & (2 ** 64 - 1), 1, m, d, k, dW.data_ptr(), stream)
+        if rc:
+            raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
+        dst = torch.zeros((m, max(words, 1)), dtype=torch.int64, device=dev)
+        dcode = torch.zeros(m, dtype=torch.int8, device=dev)
+        dh = torch.zeros(m, dtype=torch.int64, device=dev)
+        lats[j].N, lats[j].m, lats[j].Ns = int(N), m, Ns.ctypes.data
+        lats[j].dx, lats[j].dW, lats[j].dstates = dx.data_ptr(), dW.data_ptr(), dst.data_ptr()
+        lats[j].dcode, lats[j].dhstep = dcode.data_ptr(), dh.data_ptr()
+        systems.append((sys, m, d, words))
+        bufs.append((dx, dW, dst, dcode, dh))
+        keep.append(Ns)
+    sysd = StandardMapChain(max(8, int(lattices[0][0])), K, eps, boundary).descriptor()
+    rc = lib.cl_propagate_mixed(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, len(lattices), lats, k, tau, int(chunk), stream)
+    if rc:
+        raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
+    torch.cuda.synchronize()
+    out = []
+    for (sys, m, d, words), (dx, dW, dst, dcode, dh), Ns in zip(systems, bufs, keep):
+        x = dx.cpu().numpy()
+        W = dW.cpu().numpy().transpose(0, 2, 1).copy()
+        raw = dst[:, :words].cpu().numpy().copy().view(np.uint8).reshape(m, words * 8)
+        code, hstep = dcode.cpu().numpy(), dh.cpu().numpy()
+        states = decode_states(raw, allr, d, k)
+        if check:
+            check_health(code, hstep, x, W, states, allr)
+        prov = dict(backend="gpu", device=device_name(), backend_fallback_reason="", precision="Float64", seed=seed,
+                    tau=tau, k=k, chunk_size=m, gpu_chunks=1, gpu_chunk_size=m, launch_steps=int(chunk),
+                    launches_per_chunk=1, lattices=len(lattices))
+        for r, _ in allr:
+            prov.update(r.provenance())
+        out.append(PropagationResult(x=x, W=W, finals=finals_of(states, allr, d, k), provenance=prov, code=code,
+                                     hstep=hstep, states=raw if keep_states else None))
+    return out
 
 
 def device_name():

@@ -226,6 +226,34 @@ int cl_propagate_ragged(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain
                         int64_t m, int s, int d, int k, const int64_t *Ns, const int64_t *order, int tau,
                         double *dx, double *dW, uint64_t *dstates, int8_t *dcode, int64_t *dhstep, void *stream);
 
+/* One lattice of the mixed launch: N sites (8, 16, 32, 64 or 128), m orbits with their step counts (host array Ns[m],
+ * positive multiples of tau) and their device buffers in the chain's orbit-major layout (x[i][2N], W[i][k][2N],
+ * states[i][2k + 9] words, code[i], hstep[i]); the states start fresh. */
+typedef struct {
+  int32_t N;
+  int32_t pad_;
+  int64_t m;
+  const int64_t *Ns;
+  double *dx;
+  double *dW;
+  uint64_t *dstates;
+  int8_t *dcode;
+  int64_t *dhstep;
+} cl_lattice;
+
+/* Full runs of several coupled-chain lattices in ONE persistent launch (per chunk): every lattice shares the system's
+ * K, eps and boundary (sys->N is ignored: each lattice carries its own) and the chain QRSpectrum(HouseholderQR())
+ * with k = 16. Work items of a few orbits of one lattice are taken from a device counter, longest first over all
+ * lattices. The run is split into chunks of `chunk` steps (a positive multiple of tau; 0: one chunk), captured in one
+ * HIP graph when there is more than one; an orbit whose count is reached, or that failed, retires and is skipped by
+ * the later chunks. Every orbit's result is bit for bit that of cl_propagate_ragged on its lattice alone.
+ * Synchronous (returns after the work on `stream` (NULL: the context's stream) has finished); the device staging of
+ * the step counts, the launch orders and the work list is allocated and freed by the call.
+ * No reference counterpart (the reference runs one StandardMapChain per call, src/Systems/standard_map_chain.jl:11-53,
+ * and relaunches in step chunks, src/Tangent/propagate.jl:144-193). */
+int cl_propagate_mixed(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain, int elt, int nlat,
+                       const cl_lattice *lat, int k, int tau, int64_t chunk, void *stream);
+
 /* The recorder's rows (replaces the row writes of _record_kernel!, src/Experiments/run_device.jl:9-37,41-72):
  * after the launch (cl_propagate_device) that ends at grid step g of G, row g of the orbit-major record array
  * dR[m][G][k + 7 + s] is written for every orbit still running: lambda (k), det_max, symp_max, orth, the running

@@ -1284,6 +1284,7 @@ struct MapSys {
   int s;     // state length
   T A[64];   // linear map, column-major dl x dl (kind OR_SYS_LINEAR)
   int dl = 2;  // dimension of the linear map (2..8)
+  bool f32 = false;  // FP32-tangent element type: frames carried in Float32 (standard maps)
   // tangent at x applied to W (d x k), explicit-fma product; then the step
   void step(T* x, T* const* W, int nf, int k) const {
     if (kind == OR_SYS_LINEAR) {
@@ -1311,12 +1312,23 @@ struct MapSys {
     T sn = P::sin(x[0]), cs = P::cos(x[0]);
     const T Kx = kind == CL_SYS_STANDARD_MAP_K ? x[2] : K;  // the per-orbit K rides in the state
     T c = Kx * cs, m11 = (T)1 + c;
-    for (int f = 0; f < nf; ++f)
-      for (int j = 0; j < k; ++j) {
-        T w0 = W[f][2 * j], w1 = W[f][2 * j + 1];
-        W[f][2 * j] = P::fma((T)1, w1, m11 * w0);
-        W[f][2 * j + 1] = P::fma((T)1, w1, c * w0);
-      }
+    if (f32) {
+      // FP32 tangents: the entries formed in FP64 and rounded once to Float32, the explicit-fma product in Float32
+      const float m11f = (float)m11, cf = (float)c;
+      for (int f = 0; f < nf; ++f)
+        for (int j = 0; j < k; ++j) {
+          const float w0 = (float)W[f][2 * j], w1 = (float)W[f][2 * j + 1];
+          W[f][2 * j] = (T)fmaf(1.0f, w1, m11f * w0);
+          W[f][2 * j + 1] = (T)fmaf(1.0f, w1, cf * w0);
+        }
+    } else {
+      for (int f = 0; f < nf; ++f)
+        for (int j = 0; j < k; ++j) {
+          T w0 = W[f][2 * j], w1 = W[f][2 * j + 1];
+          W[f][2 * j] = P::fma((T)1, w1, m11 * w0);
+          W[f][2 * j + 1] = P::fma((T)1, w1, c * w0);
+        }
+    }
     if (kind == CL_SYS_STANDARD_MAP || kind == CL_SYS_STANDARD_MAP_K) {
       T p = x[1] + Kx * sn;
       x[0] = x[0] + p;
@@ -1390,6 +1402,7 @@ template <class P, class T>
 struct PairSys {
   MapSys<P, T> base;
   int s;  // the pair's state length, twice the base's
+  bool f32 = false;
   int jd() const { return 2; }
   void step(T* x, T* const* W, int nf, int k) const {
     T wa[2][32], wb[2][32];
@@ -1433,6 +1446,7 @@ static const double OR_GB_LO[3] = {-0x1.c71c71c71c71cp-57, 0x1.c71c71c71c71cp-56
 template <class P, class T>
 struct HHSys {
   int s = 4, d = 4;
+  bool f32 = false;
   int jd() const { return 4; }
   T h, tol;  // tol NaN: 8 eps max(1, |Y|)
   int maxiter;
@@ -1763,6 +1777,9 @@ static int8_t run_orbit_seg(const SYS& sys, const cl_chain& ch, int d, int k, in
         case CL_RED_PHASE_ADVANCE: red_pa<P>(rd, st[r].pa, n, x); break;
       }
     }
+    if (sys.f32)  // the frames the chain leaves are rounded back to Float32
+      for (int f = 0; f < nf; ++f)
+        for (int e = 0; e < d * k; ++e) W[f][e] = (T)(float)W[f][e];
     for (int r = 0; r < ch.n; ++r)
       if (red_failed(ch.r[r], st[r])) {
         *hstep = n;
@@ -1779,10 +1796,11 @@ static int8_t run_orbit_seg(const SYS& sys, const cl_chain& ch, int d, int k, in
 template <class P>
 static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int d, int k, int64_t N,
                          int tau, double* x, double* W, double* W2, uint8_t* states, int8_t* code, int64_t* hstep,
-                         int nthreads, const double* A = nullptr, uint32_t* ws = nullptr) {
+                         int nthreads, const double* A = nullptr, uint32_t* ws = nullptr, bool f32 = false) {
   typedef typename P::T T;
   const int64_t nws = chain_ws(*ch);
   MapSys<P, T> ms;
+  ms.f32 = f32;
   ms.kind = sys->kind;
   ms.K = (T)sys->K;
   ms.s = s;
@@ -1815,9 +1833,9 @@ static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, in
       for (int64_t i = lo; i < hi; ++i) {
         T xx[8], w0[64], w1[64];
         for (int c = 0; c < s; ++c) xx[c] = (T)x[i * s + c];
-        for (int e = 0; e < d * k; ++e) {
-          w0[e] = (T)W[i * d * k + e];
-          w1[e] = W2 ? (T)W2[i * d * k + e] : (T)0;
+        for (int e = 0; e < d * k; ++e) {  // FP32 tangents: the input frames rounded to Float32
+          w0[e] = f32 ? (T)(float)W[i * d * k + e] : (T)W[i * d * k + e];
+          w1[e] = W2 ? (f32 ? (T)(float)W2[i * d * k + e] : (T)W2[i * d * k + e]) : (T)0;
         }
         uint32_t* wsi = (ws && nws > 0) ? ws + i * nws : nullptr;
         code[i] = sys->kind == CL_SYS_HENON_HEILES_GAUSS
@@ -2336,6 +2354,24 @@ int or_propagate(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int
   if (math == MATH_CL)
     return propagate_all<F64Cl>(sys, ch, m, s, d, k, N, tau, x, W, W2, (uint8_t*)states, code, hstep, nthreads);
   return propagate_all<F64Libm>(sys, ch, m, s, d, k, N, tau, x, W, W2, (uint8_t*)states, code, hstep, nthreads);
+}
+
+// or_propagate with an element type: CL_ELT_F64, or CL_ELT_F64_F32TAN on the standard maps (FP64 orbit and Jacobian,
+// Float32 frames between handoffs, FP64 reducers; the frames rounded to Float32 on entry and after each handoff's
+// chain).
+int or_propagate_elt(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int d, int k, int64_t N, int tau,
+                     double* x, double* W, double* W2, void* states, int8_t* code, int64_t* hstep, int math,
+                     int nthreads, int elt) {
+  if (!or_dims_ok(sys, d, k) || N % tau) return 10;
+  const bool map = sys->kind == CL_SYS_STANDARD_MAP || sys->kind == CL_SYS_WRAPPED_STANDARD_MAP ||
+                   sys->kind == CL_SYS_STANDARD_MAP_K;
+  if (elt == CL_ELT_F64) return or_propagate(sys, ch, m, s, d, k, N, tau, x, W, W2, states, code, hstep, math, nthreads);
+  if (elt != CL_ELT_F64_F32TAN || !map) return 3;
+  if (math == MATH_CL)
+    return propagate_all<F64Cl>(sys, ch, m, s, d, k, N, tau, x, W, W2, (uint8_t*)states, code, hstep, nthreads,
+                                nullptr, nullptr, true);
+  return propagate_all<F64Libm>(sys, ch, m, s, d, k, N, tau, x, W, W2, (uint8_t*)states, code, hstep, nthreads,
+                                nullptr, nullptr, true);
 }
 
 // The recorder on a standard map (lifted, s = 2) or a wrapped standard map (s = 4) with the chain (estimator,

@@ -26,6 +26,9 @@ def load():
         o.or_propagate.restype = I
         o.or_propagate.argtypes = [ctypes.POINTER(L.cl_system), ctypes.POINTER(L.cl_chain), I64, I, I, I, I64, I,
                                    P, P, P, P, P, P, I, I]
+        o.or_propagate_elt.restype = I
+        o.or_propagate_elt.argtypes = [ctypes.POINTER(L.cl_system), ctypes.POINTER(L.cl_chain), I64, I, I, I, I64, I,
+                                       P, P, P, P, P, P, I, I, I]
         o.or_propagate_quad.restype = I
         o.or_propagate_quad.argtypes = [ctypes.POINTER(L.cl_system), ctypes.POINTER(L.cl_chain), I64, I, I, I, I64,
                                         I, P, P, P, P, P, P, I]
@@ -78,7 +81,7 @@ def _ptr(a):
     return None if a is None else a.ctypes.data
 
 
-def run(sysd, chain, x0, W0, N, tau, k, W2=None, math=MATH_CL, nthreads=None, quad=False):
+def run(sysd, chain, x0, W0, N, tau, k, W2=None, math=MATH_CL, nthreads=None, quad=False, elt=L.CL_ELT_F64):
     """Full N-step runs. x0 (m, s), W0 (m, d, k) as numpy; returns dict of x, W, W2, raw states, code, hstep."""
     o = load()
     x = np.ascontiguousarray(np.array(x0, dtype=np.float64))
@@ -95,6 +98,9 @@ def run(sysd, chain, x0, W0, N, tau, k, W2=None, math=MATH_CL, nthreads=None, qu
     if quad:
         rc = o.or_propagate_quad(ctypes.byref(sysd), ctypes.byref(chain), m, s, d, k, N, tau, _ptr(x), _ptr(W),
                                  _ptr(W2a), _ptr(states), _ptr(code), _ptr(hstep), nt)
+    elif elt != L.CL_ELT_F64:
+        rc = o.or_propagate_elt(ctypes.byref(sysd), ctypes.byref(chain), m, s, d, k, N, tau, _ptr(x), _ptr(W),
+                                _ptr(W2a), _ptr(states), _ptr(code), _ptr(hstep), math, nt, elt)
     else:
         rc = o.or_propagate(ctypes.byref(sysd), ctypes.byref(chain), m, s, d, k, N, tau, _ptr(x), _ptr(W), _ptr(W2a),
                             _ptr(states), _ptr(code), _ptr(hstep), math, nt)
