@@ -100,6 +100,8 @@ SIGNATURES = {
                                 P, P, P, P, P, P]),
     "cl_propagate_mixed": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I, ctypes.POINTER(cl_lattice),
                                I, I, I64, P]),
+    "cl_propagate_mixed_noalloc": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I,
+                                       ctypes.POINTER(cl_lattice), I, I, I64, P]),
     "cl_record_rows": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I64, I, I, I, I, P, P, P, P, P,
                            P]),
     "cl_ind_record_rows": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I64, I, I, I, P, P, P, P, P,

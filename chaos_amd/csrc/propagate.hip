@@ -25,6 +25,16 @@ struct cl_ctx {
 
 static char g_create_err[256] = "";
 
+// Scratch of cl_propagate_mixed_noalloc: the device staging of the work list, step counts, launch orders and work
+// counters, a file-scope device array of a fixed size (each GPU the library runs on holds one copy until the
+// process ends). Host code reaches it only through the address hipGetSymbolAddress returns for the device, taken
+// once per device by cl_create with that device current (g_scratch_ptr); the array's name is never used as a
+// device address on the host.
+static constexpr size_t CL_SCRATCH_BYTES = (size_t)4 << 20;
+static __device__ __align__(256) char scratch[CL_SCRATCH_BYTES];
+static constexpr int CL_MAX_DEVICES = 64;
+static void* g_scratch_ptr[CL_MAX_DEVICES];
+static std::mutex g_scratch_mu;
 
 // ------------------------------------------------------------------------------ instance table
 static std::once_flag g_once;
@@ -370,6 +380,13 @@ extern "C" cl_ctx* cl_create(int device) {
   if (hipSetDevice(device) != hipSuccess) {
     snprintf(g_create_err, sizeof(g_create_err), "hipSetDevice(%d) failed", device);
     return nullptr;
+  }
+  if (device < CL_MAX_DEVICES) {  // the scratch array's address on this device, once per device
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+      void* p = nullptr;
+      if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_ptr[device] = p;
+    }
   }
   cl_ctx* c = (cl_ctx*)calloc(1, sizeof(cl_ctx));
   c->device = device;
@@ -1028,6 +1045,29 @@ extern "C" int cl_propagate_mixed(cl_ctx* ctx, const cl_system* sys, const cl_ch
   rc = mixed_run(ctx, P, cls, base, sys->ring != 0, chunk, st);
   hipFreeAsync(base, st);
   e = hipStreamSynchronize(st);
+  if (rc) return rc;
+  return e == hipSuccess ? CL_OK : set_err(ctx, map_err(e), "mixed chain launch failed");
+}
+
+// cl_propagate_mixed without any device allocation: the staging image goes to this device's scratch array (at most
+// CL_SCRATCH_BYTES) and the chunks are launched directly on the stream (no graph is instantiated). Calls that overlap
+// on one device share the array and race.
+extern "C" int cl_propagate_mixed_noalloc(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int nlat,
+                                          const cl_lattice* lat, int k, int tau, int64_t chunk, void* stream) {
+  if (!ctx) return CL_ERR_ARGUMENT;
+  MixedPlan P;
+  std::vector<MixedClass> cls;
+  int rc = mixed_plan(ctx, sys, ch, elt, nlat, lat, k, tau, chunk, P, cls);
+  if (rc) return rc;
+  if (P.image.size() > CL_SCRATCH_BYTES)
+    return set_err(ctx, CL_ERR_ARGUMENT, "work list, step counts and orders exceed the 4 MiB scratch (about 2e5 orbits)");
+  char* base = ctx->device < CL_MAX_DEVICES ? (char*)g_scratch_ptr[ctx->device] : nullptr;
+  if (!base) return set_err(ctx, CL_ERR_DEVICE_LOST, "the scratch array has no address on this device");
+  hipSetDevice(ctx->device);
+  mixed_cus();
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  rc = mixed_run(ctx, P, cls, base, sys->ring != 0, chunk, st, false);
+  hipError_t e = hipStreamSynchronize(st);
   if (rc) return rc;
   return e == hipSuccess ? CL_OK : set_err(ctx, map_err(e), "mixed chain launch failed");
 }

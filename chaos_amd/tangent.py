@@ -335,7 +335,7 @@ def propagate_ragged(sys, x0, Ns, *, k, tau=1, seed, reducers=(), W0=None, order
 
 
 def propagate_mixed(lattices, *, K, eps, boundary="ring", k=16, tau=1, seed, reducers=None, chunk=0, check=True,
-                    keep_states=False):
+                    keep_states=False, _noalloc=False):
     """Several coupled-chain lattices in one persistent launch per chunk (`cl_propagate_mixed`): `lattices` is a list of
     (N, x0, Ns), x0 of shape (m, 2N), Ns the per-orbit step counts (positive multiples of tau). All share K, eps and
     the boundary; the chain is QRSpectrum(HouseholderQR()) with k = 16. `chunk` is the step length of a chunk (0: one
@@ -380,7 +380,8 @@ def propagate_mixed(lattices, *, K, eps, boundary="ring", k=16, tau=1, seed, red
         bufs.append((dx, dW, dst, dcode, dh))
         keep.append(Ns)
     sysd = StandardMapChain(max(8, int(lattices[0][0])), K, eps, boundary).descriptor()
-    rc = lib.cl_propagate_mixed(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, len(lattices), lats, k, tau, int(chunk), stream)
+    fn = lib.cl_propagate_mixed_noalloc if _noalloc else lib.cl_propagate_mixed
+    rc = fn(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, len(lattices), lats, k, tau, int(chunk), stream)
     if rc:
         raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
     torch.cuda.synchronize()

@@ -254,6 +254,15 @@ typedef struct {
 int cl_propagate_mixed(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain, int elt, int nlat,
                        const cl_lattice *lat, int k, int tau, int64_t chunk, void *stream);
 
+/* cl_propagate_mixed without any device allocation, for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc). No counterpart in the reference. The staging of the work list, step counts, launch orders and work counters lives in a fixed 4 MiB array
+ * of the library on each device (about 2e5 orbits; larger requests return 10 before any device work); the chunks are
+ * launched directly on the stream, no graph is instantiated. The array keeps the last call's contents and every
+ * field a call reads is rewritten by that call (counters cleared by a fill kernel). Calls that overlap on one device
+ * share the array and race: serialise them. Host-side planning uses ordinary heap memory. Synchronous. */
+int cl_propagate_mixed_noalloc(cl_ctx *ctx, const cl_system *sys, const cl_chain *chain, int elt, int nlat,
+                               const cl_lattice *lat, int k, int tau, int64_t chunk, void *stream);
+
 /* The recorder's rows (replaces the row writes of _record_kernel!, src/Experiments/run_device.jl:9-37,41-72):
  * after the launch (cl_propagate_device) that ends at grid step g of G, row g of the orbit-major record array
  * dR[m][G][k + 7 + s] is written for every orbit still running: lambda (k), det_max, symp_max, orth, the running
