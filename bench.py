@@ -60,6 +60,37 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def host_cores():
+    """The CPU the baseline may use: os.cpu_count(), the affinity set, and the cgroup CPU quota (cpu.max) when one is
+    set. The oracle runs one thread per usable core: the affinity set, capped by the quota (a GPU box gives one GPU's
+    job a share of its cores; threads beyond the quota would only time-share it)."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = max(1, int(math.ceil(int(q) / int(per))))
+    except (OSError, ValueError):
+        pass
+    threads = min(aff, quota) if quota else aff
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(cpu_count=os.cpu_count(), affinity=aff, cgroup_quota=quota, threads=threads, model=model)
+
+
+def nthreads():
+    return host_cores()["threads"]
+
+
+nthreads_ = nthreads
+
+
 def workload(name):
     from chaos_amd import MEGNO, FLI, OFLI, SALI, GALI, QRSpectrum, HouseholderQR, MGS, GivensQR
     from chaos_amd import StandardMap, HenonHeilesGauss
@@ -86,7 +117,7 @@ def workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def gpu_run(wl, steps, warmup):
+def gpu_run(wl, steps, warmup, elt=1):
     import torch
     from chaos_amd import QRSpectrum
     from chaos_amd.tangent import make_chain, _context
@@ -97,7 +128,7 @@ def gpu_run(wl, steps, warmup):
     chain, allr = make_chain(wl["red"], wl["red2"])
     nf = 2 if wl["red2"] else 1
     sd0 = wl["systems"][0].descriptor()
-    words = lib.cl_state_bytes(ctypes.byref(sd0), ctypes.byref(chain), 1, d, k) // 8
+    words = lib.cl_state_bytes(ctypes.byref(sd0), ctypes.byref(chain), elt, d, k) // 8
     dev = torch.device("cuda")
     x_master = torch.from_numpy(np.ascontiguousarray(x0.T)).to(dev)
     dx = torch.empty_like(x_master)
@@ -121,7 +152,7 @@ def gpu_run(wl, steps, warmup):
             dW2.copy_(dW)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        rc = lib.cl_propagate_device(ctx, ctypes.byref(sd), ctypes.byref(chain), 1, m, s, d, k, 0, n, n, tau,
+        rc = lib.cl_propagate_device(ctx, ctypes.byref(sd), ctypes.byref(chain), elt, m, s, d, k, 0, n, n, tau,
                                      dx.data_ptr(), dW.data_ptr(), dW2.data_ptr() if dW2 is not None else None,
                                      dst.data_ptr(), dcode.data_ptr(), dh.data_ptr(), sh)
         e1.record(stream)
@@ -152,7 +183,8 @@ def gpu_run(wl, steps, warmup):
         fin = finals_of(decode_states(raw, allr, d, k), allr, d, k)
         qi = [j for j, (r, _) in enumerate(allr) if isinstance(r, QRSpectrum)][0]
         lam = dict(idx=idx, lam=fin[qi]["lambda"], system=wl["systems"][last])
-    return dict(wall=wall, kernel_ms=kms, orbit_steps=float(m) * N, m=m, bad=bad, lam=lam)
+    return dict(wall=wall, kernel_ms=kms, orbit_steps=float(m) * N, m=m, bad=bad, lam=lam,
+                x_final=dx.cpu().numpy().T.copy() if wl["name"] == "c4" else None)
 
 
 def cpu_baseline(wl, budget_orbits):
@@ -165,22 +197,23 @@ def cpu_baseline(wl, budget_orbits):
     m = len(xs)
     W0 = O.frames(wl["seed"], 1, len(x0), wl["d"], wl["k"])[::stride]
     chain, _ = make_chain(wl["red"], wl["red2"])
-    nthreads = min(16, os.cpu_count() or 1)
+    nt = nthreads()
     total, t = 0.0, 0.0
     O.gauss_iterations()
     for sysm in wl["systems"]:
         t0 = time.perf_counter()
         O.run(sysm.descriptor(), chain, xs, W0, wl["N"], wl["tau"], wl["k"],
-              W2=W0 if wl["red2"] else None, math=O.MATH_LIBM, nthreads=nthreads)
+              W2=W0 if wl["red2"] else None, math=O.MATH_LIBM, nthreads=nt)
         t += time.perf_counter() - t0
         total += float(m) * wl["N"]
         log(f"cpu baseline {sysm!r}: {m} orbits done")
     it, st = O.gauss_iterations()
     ns = len(wl["systems"])
-    out = dict(value=total / t, unit="orbit-steps/s", cores=nthreads, kind="port",
+    out = dict(value=total / t, unit="orbit-steps/s", cores=nt, kind="port",
                sample=(f"every {stride}th orbit" if stride > 1 else "every orbit") +
                       f" ({m} orbits x {wl['N']} steps x {ns} system(s)), "
-                      f"oracle/oracle.cpp with glibc sin/cos/log, {nthreads} threads, {t:.1f} s")
+                      f"oracle/oracle.cpp with glibc sin/cos/log, {nt} threads, {t:.1f} s",
+               host=host_cores())
     if st:
         out["gauss_iterations_per_step"] = it / st
     return out
@@ -219,6 +252,149 @@ def flops_per_orbit_step(wl, cpu):
     it = cpu.get("gauss_iterations_per_step") if cpu else None
     it = it if it else C4_ITERATIONS_DEFAULT
     return 780.0 * it + 2300.0 + 75.0
+
+
+def c4_energy_drift(wl, xf):
+    """|H(z_N) - H(z_0)| of the Henon-Heiles launch per energy: max and rms over its orbits (the last timed launch ran
+    all N steps from the workload's starts; non-finite orbits excluded and counted)."""
+    from chaos_amd import hh_energy
+    x0 = wl["x0"][:, :4]
+    m = len(x0) // 3
+    out = {}
+    for j, E in enumerate(("1/12", "1/8", "1/6")):
+        sl = slice(j * m, (j + 1) * m)
+        H0 = hh_energy(x0[sl])
+        H1 = hh_energy(xf[sl])
+        d = np.abs(H1 - H0)
+        ok = np.isfinite(d)
+        out[f"E={E}"] = {"max": float(d[ok].max()) if ok.any() else None,
+                         "rms": float(np.sqrt(np.mean(d[ok] ** 2))) if ok.any() else None,
+                         "nonfinite": int((~ok).sum())}
+    return out
+
+
+def ks_two_sample(a, b):
+    """Two-sample Kolmogorov-Smirnov statistic D and its 1% critical value 1.63 sqrt((n + m) / (n m))."""
+    a, b = np.sort(np.asarray(a)), np.sort(np.asarray(b))
+    z = np.concatenate([a, b])
+    D = float(np.max(np.abs(np.searchsorted(a, z, side="right") / len(a) - np.searchsorted(b, z, side="right") / len(b))))
+    return D, 1.63 * math.sqrt((len(a) + len(b)) / (len(a) * len(b)))
+
+
+def accuracy_protocol(wl, same_stride=1024, ind_orbits=96):
+    """The metric's "at equal lambda error vs Double64" on C2, for the Float64 kernels and the Float32-tangent ones.
+    (a) same orbit (the AC4 method, test/tangent-core/tc_ac01_04.jl:66-88): every `same_stride`-th grid orbit of each K,
+        the GPU's QR exponents against the oracle's __float128 Jacobians, frame products and QR along the same FP64
+        orbit; max, median, and the counts above 1e-6 and 1e-10.
+    (b) independent orbit (the reference's AC12 protocol, verify/gpu.jl:414-465): `ind_orbits` grid orbits per K run in
+        full by the __float128 oracle (113-bit significand, finer than Double64's 106 bits), the whole chain. Orbits
+        whose __float128 MEGNO Y (column 1) is below 4 form the regular class (MEGNO tends to 2 on quasi-periodic
+        orbits): per-indicator maximum errors and the counts above 1e-10 and 1e-6 of lambda_1; the rest is the chaotic
+        class: two-sample KS of lambda_1 (D against 1.63 sqrt((n+m)/(nm))) and the 4-SE test of the means."""
+    from chaos_amd import propagate, QRSpectrum
+    from chaos_amd.tangent import make_chain, decode_states, finals_of
+    from tests import oracle_lib as O
+    nt = nthreads()
+    x0_all, N, tau = wl["x0"], wl["N"], wl["tau"]
+    W_all = O.frames(wl["seed"], 1, len(x0_all), 2, 2)
+    red, red2 = wl["red"], wl["red2"]
+    chain, allr = make_chain(red, red2)
+    qr_chain, _ = make_chain(tuple(r for r in red2 if isinstance(r, QRSpectrum)))
+    qi = len(red)  # the QR spectrum's position in the chain (first reducer of the second frame)
+    out = {"same_orbit": {}, "independent": {}, "orbits_same": 0, "orbits_independent": 0}
+    t0 = time.perf_counter()
+    for sysm in wl["systems"]:
+        K = sysm.K
+        idx = np.arange(0, len(x0_all), same_stride)
+        xs, Ws = x0_all[idx], W_all[idx]
+        g = {e: propagate(sysm, xs, N, k=2, tau=tau, seed=wl["seed"], reducers=red, reducers2=red2, W0=Ws,
+                          check=False, elt=e) for e in ("f64", "f32tan")}
+        lq = O.same_orbit_quad(K, qr_chain, xs, Ws, N, tau, 2, nthreads=nt)
+        row = {}
+        for e, res in g.items():
+            dl = np.abs(res.finals[qi]["lambda"] - lq)
+            row[e] = {"max": float(dl.max()), "median": float(np.median(dl)), "above_1e-6": int(np.sum(dl.max(1) > 1e-6)),
+                      "above_1e-10": int(np.sum(dl.max(1) > 1e-10))}
+        out["same_orbit"][f"K={K}"] = row
+        out["orbits_same"] += len(idx)
+        # independent orbits: the full chain in __float128
+        jdx = np.linspace(0, len(x0_all) - 1, ind_orbits).astype(np.int64)
+        xi, Wi = x0_all[jdx], W_all[jdx]
+        q = O.run(sysm.descriptor(), chain, xi, Wi, N, tau, 2, W2=Wi, quad=True, nthreads=nt)
+        fq = finals_of(decode_states(q["states"], allr, 2, 2), allr, 2, 2)
+        reg = fq[0]["Y"][:, 0] < 4.0
+        rowi = {"regular": int(reg.sum()), "chaotic": int((~reg).sum())}
+        for e in ("f64", "f32tan"):
+            r = propagate(sysm, xi, N, k=2, tau=tau, seed=wl["seed"], reducers=red, reducers2=red2, W0=Wi,
+                          check=False, elt=e).finals
+            d = {}
+            if reg.any():
+                dl = np.abs(r[qi]["lambda"][reg] - fq[qi]["lambda"][reg])
+                d["lambda_max"] = float(dl.max())
+                d["lambda1_above_1e-10"] = int(np.sum(dl[:, 0] > 1e-10))
+                d["lambda1_above_1e-6"] = int(np.sum(dl[:, 0] > 1e-6))
+                d["megno_Y_max"] = float(np.max(np.abs(r[0]["Y"][reg] - fq[0]["Y"][reg])))
+                d["megno_Ybar_max"] = float(np.max(np.abs(r[0]["Ybar"][reg] - fq[0]["Ybar"][reg])))
+                d["fli_max"] = float(np.max(np.abs(r[1]["fli"][reg] - fq[1]["fli"][reg])))
+                d["ofli_max"] = float(np.max(np.abs(r[2]["ofli"][reg] - fq[2]["ofli"][reg])))
+                d["sali_max"] = float(np.max(np.abs(r[3]["sali"][reg] - fq[3]["sali"][reg])))
+            ch = ~reg
+            if ch.sum() >= 10:
+                a_, b_ = r[qi]["lambda"][ch, 0], fq[qi]["lambda"][ch, 0]
+                D, Dc = ks_two_sample(a_, b_)
+                se = math.sqrt((np.var(a_) + np.var(b_)) / len(a_))
+                d["chaotic_ks"] = {"D": D, "D_crit_1pct": Dc, "pass": D <= Dc,
+                                   "mean_gpu": float(np.mean(a_)), "mean_quad": float(np.mean(b_)),
+                                   "means_in_SE": float(abs(np.mean(a_) - np.mean(b_)) / se) if se > 0 else 0.0}
+            rowi[e] = d
+        out["independent"][f"K={K}"] = rowi
+        out["orbits_independent"] += len(jdx)
+        log(f"accuracy K={K}: same-orbit {len(idx)} orbits, independent {len(jdx)} ({int(reg.sum())} regular)")
+    out["seconds"] = round(time.perf_counter() - t0, 1)
+    out["threads"] = nt
+    out["reference"] = ("__float128 oracle (113-bit significand) standing in for Double64 (106-bit); the reference's "
+                        "own AC12(a): 1-20 of 1024 regular orbits above 1e-6 between its FP64 backends and Double64")
+    return out
+
+
+def c1_block(a):
+    """C1, the reference's CPU-runnable case, measured beside the headline: the GPU rate, its roofline, and the
+    oracle on the host's cores over every orbit of the workload at its full step count; the target is >= 50x."""
+    wl = workload("c1")
+    g = gpu_run(wl, 10, 2)
+    value = g["orbit_steps"] * 10 / g["wall"]
+    avg_ms = float(np.mean(g["kernel_ms"]))
+    achieved = FLOPS_PER_ORBIT_STEP["c1"] * g["orbit_steps"] / (avg_ms * 1e-3) / 1e12
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(wl, 4096)
+    out = {"workload": wl["desc"], "value": value, "unit": "orbit-steps/s", "kernel_ms": avg_ms,
+           "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_PEAK_TFLOPS}, "cpu_baseline": cpu, "target_gpu_over_cpu": 50}
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    return out
+
+
+def lognorm_one_thread():
+    """The reference's CPU sanity figure: the column-normalising log-sum chain (LogNorm, test/tangent-core/
+    tc_helpers.jl:56-70), k = 2, K = 2 pi, tau = 1, on one thread of the oracle with glibc math, 1e3 steps from starts
+    uniform on the torus as the reference's record; its own 1-thread figure is 33.09 ns per orbit-step on a 5.4 GHz
+    desktop core (gpu-rtx5060-verify-log.txt:11)."""
+    from chaos_amd import StandardMap, _lib as L
+    from chaos_amd.inputs import torus_uniform
+    from tests import oracle_lib as O
+    ch = L.cl_chain()
+    ch.n = 1
+    ch.r[0].kind = 8
+    m, N = 16384, 1000  # the reference's record: 1e5 orbits x 1e3 steps (the orbits stay near their starts)
+    x0 = torus_uniform(7, m) * 2 * np.pi
+    W0 = O.frames(7, 1, m, 2, 2)
+    t0 = time.perf_counter()
+    O.run(StandardMap(2 * np.pi).descriptor(), ch, x0, W0, N, 1, 2, math=O.MATH_LIBM, nthreads=1)
+    dt = time.perf_counter() - t0
+    return {"ns_per_orbit_step": dt / (m * N) * 1e9, "reference_ns_1thread": 33.09,
+            "sample": f"{m} orbits x {N} steps, LogNorm k=2, K=2pi, tau=1, one thread, glibc sin/cos/log"}
 
 
 C3_NS = (8, 16, 32, 64, 128)
@@ -372,7 +548,7 @@ def c3_main(a):
     cpu = None
     if not a.no_cpu:
         from tests import oracle_lib as O
-        nthreads = min(16, os.cpu_count() or 1)
+        nthreads = nthreads_()
         tc, done = 0.0, 0.0
         for j, S in enumerate(sets):
             sel = np.arange(0, m, 32)
@@ -403,6 +579,12 @@ def c3_main(a):
 
 
 def c5_main(a):
+    out = c5_run(a)
+    print(json.dumps(out), flush=True)
+    return out
+
+
+def c5_run(a, cpu_leg=True):
     import torch
     from chaos_amd import surrogate as S
     n_train, n_ho, n_tri, N = (int(v) for v in a.c5_sizes.split(","))
@@ -444,13 +626,20 @@ def c5_main(a):
     torch.cuda.synchronize()
     t_tri = time.perf_counter() - t0
     wrong = float(np.mean(tri["label"] != (lab_plain == 1)))
+    # the same triage with the prefix pass on Float32 tangents (the continuation stays Float64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tri32 = S.triage_run(net, x0, 13, N, prefix, margin, log=log, elt=2)
+    torch.cuda.synchronize()
+    t_tri32 = time.perf_counter() - t0
+    wrong32 = float(np.mean(tri32["label"] != (lab_plain == 1)))
     cpu = None
-    if not a.no_cpu:
+    if cpu_leg and not a.no_cpu:
         from chaos_amd import QRSpectrum, MGS, StandardMapK
         from chaos_amd.tangent import make_chain
         from tests import oracle_lib as O
         ms = 16384
-        nthreads = min(16, os.cpu_count() or 1)
+        nthreads = nthreads_()
         chain, _ = make_chain((QRSpectrum(MGS()),))
         xs = x0[:: max(1, n_tri // ms)][:ms]
         W0 = O.frames(13, 1, n_tri, 2, 2)[:: max(1, n_tri // ms)][:ms]
@@ -471,12 +660,15 @@ def c5_main(a):
                       "continued_fraction": float(1.0 - tri["confident"].mean()),
                       "misclassified_vs_plain": wrong, "heldout": ev, "train_s": t_train, "data_s": t_data,
                       "chaotic_fraction_train": info["chaotic_fraction"],
-                      "final_loss": [float(v) for v in info["history"][-1]]},
+                      "final_loss": [float(v) for v in info["history"][-1]],
+                      "f32tan_prefix": {"speedup": t_plain / t_tri32, "triage_s": t_tri32,
+                                        "prefix_s": tri32["t_prefix"], "misclassified_vs_plain": wrong32,
+                                        "continued_fraction": float(1.0 - tri32["confident"].mean())}},
         "roofline": None, "cpu_baseline": cpu,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
-    print(json.dumps(out), flush=True)
+    return out
 
 
 # The recorder's cost as the reference measures it (verify/experiments-device.jl:37-58): 2048 orbits x 2e5 steps per
@@ -528,7 +720,7 @@ def rec_main(a):
         from chaos_amd.tangent import make_chain
         from tests import oracle_lib as O
         ms_ = max(1, m // 8)
-        nt = min(16, os.cpu_count() or 1)
+        nt = nthreads()
         tc, steps_c = 0.0, 0
         for meth in METHODS:
             k = vectors(meth)
@@ -728,7 +920,7 @@ def ind_main(a):
     cpu = None
     if not a.no_cpu:
         from tests import oracle_lib as O
-        nt = min(16, os.cpu_count() or 1)
+        nt = nthreads()
         tc, steps_c = 0.0, 0
         for n in ("st", "bk"):
             r = res[(n, 1024)]
@@ -788,6 +980,8 @@ def main():
     ap.add_argument("--c5-epochs", type=int, default=8)
     ap.add_argument("--cpu-orbits", type=int, default=32768, help="orbits per K in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="c2: skip the Float32-tangent, accuracy, C1 and C5 blocks of the default run")
     ap.add_argument("--c3-mode", default="streams", choices=["streams", "mixed"],
                     help="c3: one launch per lattice on five streams, or one persistent launch over all five")
     ap.add_argument("--c3-chunk", type=int, default=0, help="c3 mixed: steps per graph-captured chunk (0: one chunk)")
@@ -795,9 +989,9 @@ def main():
                     help="first copy 1 GiB device-to-device (the known-size kernel that calibrates FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU: the ensemble shards by orbit ranges with no exchange during propagation, "
-              f"so the figure at N GPUs is N times the one-GPU figure; multi-GPU runs are not supported "
-              f"(asked for --gpus {a.gpus}).", file=sys.stderr)
+        print(f"bench.py measures one GPU; multi-GPU runs are not supported (asked for --gpus {a.gpus}). The ensemble "
+              f"would shard by orbit ranges with no exchange during propagation, so throughput is expected to scale "
+              f"with the GPU count; that is not measured.", file=sys.stderr)
         sys.exit(2)
     if a.workload == "c5":
         return c5_main(a)
@@ -820,6 +1014,38 @@ def main():
     g = gpu_run(wl, a.steps, a.warmup)
     value = g["orbit_steps"] * a.steps / g["wall"]
     avg_ms = float(np.mean(g["kernel_ms"]))
+    extra = {}
+    if wl["name"] == "c4":
+        extra["energy_drift"] = {"f64": c4_energy_drift(wl, g["x_final"])}
+        if not a.no_extras:  # the compensated (double-double) orbit on the same workload
+            wdd = dict(wl)
+            wdd["x0"] = np.concatenate([wl["x0"], np.zeros_like(wl["x0"])], axis=1)
+            gdd = gpu_run(wdd, max(1, min(a.steps, 2)), 1, elt=3)
+            ms_dd = float(np.mean(gdd["kernel_ms"]))
+            extra["dd"] = {"value": gdd["orbit_steps"] * len(gdd["kernel_ms"]) / gdd["wall"], "kernel_ms": ms_dd,
+                           "over_f64": avg_ms / ms_dd, "nonfinite_orbits": gdd["bad"]}
+            extra["energy_drift"]["dd"] = c4_energy_drift(wl, gdd["x_final"][:, :4])
+        log("energy drift", extra["energy_drift"])
+    if wl["name"] == "c2" and not a.no_extras:
+        # the Float32-tangent form of the same workload (Float64 orbit and reducers)
+        g32 = gpu_run(wl, max(1, min(a.steps, 6)), 1, elt=2)
+        ms32 = float(np.mean(g32["kernel_ms"]))
+        a32 = FLOPS_PER_ORBIT_STEP["c2"] * g32["orbit_steps"] / (ms32 * 1e-3) / 1e12
+        extra["c2_f32tan"] = {"value": g32["orbit_steps"] * len(g32["kernel_ms"]) / g32["wall"], "kernel_ms": ms32,
+                              "over_f64": avg_ms / ms32, "nonfinite_orbits": g32["bad"],
+                              "roofline": {"bound": "fp64-valu", "achieved": a32, "peak": FP64_PEAK_TFLOPS,
+                                           "unit": "TFLOP/s", "frac": a32 / FP64_PEAK_TFLOPS}}
+        log("c2 f32tan", extra["c2_f32tan"])
+        if not a.no_cpu:
+            extra["accuracy"] = accuracy_protocol(wl)
+            extra["cpu_sanity"] = lognorm_one_thread()
+            log("cpu sanity", extra["cpu_sanity"])
+        extra["c1"] = c1_block(a)
+        log("c1", {k: v for k, v in extra["c1"].items() if k != "cpu_baseline"})
+        c5 = c5_run(a, cpu_leg=False)
+        extra["c5"] = {"value": c5["value"], "unit": c5["unit"], "workload": c5["config"]["workload"],
+                       **c5["surrogate"]}
+        log("c5", {k: extra["c5"][k] for k in ("speedup", "misclassified_vs_plain", "continued_fraction")})
     budget = {"c2": a.cpu_orbits, "c1": 4096, "c4": 2048}[wl["name"]]
     cpu = None if a.no_cpu else cpu_baseline(wl, budget)
     if cpu is not None:
@@ -839,6 +1065,7 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

@@ -65,7 +65,11 @@ struct HHGauss {
     f[3] = -((z[1] + z[0] * z[0]) - z[1] * z[1]);
   }
 
-  CL_HD void aux(const double* z, Aux& a) const {
+  CL_HD void aux(const double* z, Aux& a) const { aux_e(z, nullptr, a); }
+
+  // The stage solve at z with the correction e of a compensated state (null: none). The stage values are
+  // z + (e + h sum_j a_ij F_j) (gauss.jl:117); the start and the first field use z alone (gauss.jl:123-124).
+  CL_HD void aux_e(const double* z, const double* e, Aux& a) const {
     const double Ahi[3][3] = CL_GA_HI, Alo[3][3] = CL_GA_LO;
     const double bhi[3] = CL_GB_HI, blo[3] = CL_GB_LO;
     double f0[4];
@@ -95,7 +99,7 @@ struct HHGauss {
         for (int c = 0; c < 4; ++c) {
           double Fc[3] = {F[0][c], F[1][c], F[2][c]};
           double ss = h * dot_comp3(Ahi[i], Alo[i], Fc);
-          double yn = z[c] + (0.0 + ss);
+          double yn = z[c] + ((e ? e[c] : 0.0) + ss);
           double e = cl_fabs(yn - Y[i][c]);
           di = c == 0 ? e : cl_jmax(di, e);
           Y[i][c] = yn;
@@ -292,6 +296,29 @@ struct HHGauss {
   static CL_HD double energy(const double* z) {
     return (((z[2] * z[2] + z[3] * z[3]) / 2.0 + (z[0] * z[0] + z[1] * z[1]) / 2.0) + (z[0] * z[0]) * z[1]) -
            (z[1] * z[1]) * z[1] / 3.0;
+  }
+};
+
+// The Henon-Heiles Gauss map on a compensated state (value z, correction e): the reference's compensated summation
+// (advance_n(...; compensated = true), gauss.jl:149-158, stepper.jl:50-58) carried through the tangent stream. The
+// correction enters the stage values; the update is z, e = TwoSum(z, inc + e) (the _kahan rule), so the state is a
+// double-double pair that drifts by no systematic rounding bias. The Jacobian is that of the compensated stages.
+struct HHGaussComp : HHGauss {
+  static constexpr int S = 8;  // z[0..3], e[4..7]
+  CL_HD void aux(const double* x, Aux& a) const { aux_e(x, x + 4, a); }
+  CL_HD void forward(const Aux& a, double* x) const {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (a.ok) {
+        double s, err;
+        cl_two_sum(x[c], a.inc[c] + x[4 + c], &s, &err);
+        x[c] = s;
+        x[4 + c] = err;
+      } else {
+        x[c] = __builtin_nan("");
+        x[4 + c] = __builtin_nan("");
+      }
+    }
   }
 };
 

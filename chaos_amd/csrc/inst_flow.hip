@@ -31,7 +31,18 @@ typedef Cons<S_<OFLI<4, 1>>, Nil> F_ofli1;
 
 #define ADDF(CH, K, NF) if (*n < cap) out[(*n)++] = make_instance<CL_SYS_HENON_HEILES_GAUSS, HHGauss, K, NF, CH>();
 
+// the compensated (double-double) orbit of the element type CL_ELT_DD: state (z, e), s = 8
+template <class CH, int K, int NF>
+Instance make_dd() {
+  Instance in = make_instance<CL_SYS_HENON_HEILES_GAUSS, HHGaussComp, K, NF, CH>();
+  in.elt = CL_ELT_DD;
+  return in;
+}
+#define ADDD(CH, K, NF) if (*n < cap) out[(*n)++] = make_dd<CH, K, NF>();
+
 void register_instances_flow(Instance* out, int* n, int cap) {
+  ADDD(F_none, 4, 1) ADDD(F_giv4, 4, 1) ADDD(F_megno_gali4, 4, 1) ADDD(F_c4, 4, 2) ADDD(F_giv_mon4, 4, 1)
+  ADDD(F_ofli4, 4, 1)
   ADDF(F_none, 4, 1) ADDF(F_giv4, 4, 1) ADDF(F_hh4, 4, 1) ADDF(F_mgs4, 4, 1) ADDF(F_gali4, 4, 1)
   ADDF(F_megno4, 4, 1) ADDF(F_ofli4, 4, 1) ADDF(F_megno_gali4, 4, 1) ADDF(F_c4, 4, 2)
   ADDF(F_mon4, 4, 1) ADDF(F_giv_mon4, 4, 1)

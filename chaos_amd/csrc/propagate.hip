@@ -82,7 +82,8 @@ static const Instance* find_instance(const cl_system* sys, const cl_chain* ch, i
   return nullptr;
 }
 
-static int state_dim(int kind) {
+static int state_dim(int kind, int elt = CL_ELT_F64) {
+  if (elt == CL_ELT_DD && kind == CL_SYS_HENON_HEILES_GAUSS) return 8;  // value and correction
   switch (kind) {
     case CL_SYS_STANDARD_MAP: return 2;
     case CL_SYS_WRAPPED_STANDARD_MAP: return 4;
@@ -207,10 +208,10 @@ static int validate(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int e
   if (d != sd) return set_err(ctx, CL_ERR_ARGUMENT, "d is not the system dimension");
   if (sys->kind == CL_SYS_HENON_HEILES_GAUSS && sys->stages != 3)
     return set_err(ctx, CL_ERR_UNSUPPORTED, "Gauss kernels are built for s = 3 stages");
-  if (s != state_dim(sys->kind)) return set_err(ctx, CL_ERR_ARGUMENT, "state length does not match the system");
-  if (!(k >= 1 && k <= d)) return set_err(ctx, CL_ERR_ARGUMENT, "k must be in 1:d");
-  if (elt != CL_ELT_F64 && elt != CL_ELT_F64_F32TAN)
+  if (elt != CL_ELT_F64 && elt != CL_ELT_F64_F32TAN && elt != CL_ELT_DD)
     return set_err(ctx, CL_ERR_UNSUPPORTED, "element type has no kernel for this system");
+  if (s != state_dim(sys->kind, elt)) return set_err(ctx, CL_ERR_ARGUMENT, "state length does not match the system");
+  if (!(k >= 1 && k <= d)) return set_err(ctx, CL_ERR_ARGUMENT, "k must be in 1:d");
   int nws = 0;
   for (int r = 0; r < ch->n; ++r) {
     const cl_reducer& q = ch->r[r];
@@ -360,7 +361,7 @@ extern "C" int cl_probe(int elt, cl_probe_t* out) {
   if (hipGetDeviceProperties(&p, 0) != hipSuccess) return CL_ERR_NO_DEVICE;
   out->functional = strncmp(p.gcnArchName, "gfx950", 6) == 0;
   snprintf(out->device, sizeof(out->device), "%s (%s)", p.name, p.gcnArchName);
-  out->compiles = out->functional && (elt == CL_ELT_F64 || elt == CL_ELT_F64_F32TAN);
+  out->compiles = out->functional && (elt == CL_ELT_F64 || elt == CL_ELT_F64_F32TAN || elt == CL_ELT_DD);
   out->gpu_faster = out->compiles;
   return out->functional ? CL_OK : CL_ERR_NO_DEVICE;
 }
@@ -421,7 +422,7 @@ extern "C" int cl_birkhoff_weights(int64_t N, int order, double* w) {
 
 extern "C" int64_t cl_state_bytes(const cl_system* sys, const cl_chain* ch, int elt, int d, int k) {
   if (!sys || !ch || ch->n < 0 || ch->n > CL_MAX_REDUCERS) return -CL_ERR_ARGUMENT;
-  if (elt != CL_ELT_F64 && elt != CL_ELT_F64_F32TAN) return -CL_ERR_UNSUPPORTED;
+  if (elt != CL_ELT_F64 && elt != CL_ELT_F64_F32TAN && elt != CL_ELT_DD) return -CL_ERR_UNSUPPORTED;
   if (sys->kind == CL_SYS_STANDARD_MAP_CHAIN) {
     int words = 0;
     bool qr = false;

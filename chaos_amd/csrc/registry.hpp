@@ -57,6 +57,15 @@ inline HHGauss make_sys<HHGauss>(const cl_system& s) {
   return m;
 }
 
+template <>
+inline HHGaussComp make_sys<HHGaussComp>(const cl_system& s) {
+  HHGaussComp m;
+  m.h = s.h;
+  m.tol = s.tol;
+  m.maxiter = s.maxiter > 0 ? s.maxiter : 50;
+  return m;
+}
+
 // Launch geometry: a big ensemble fills the chip with 256-lane groups; a small one (below 64K
 // orbits) uses 64-lane groups so its waves spread over as many CUs as possible.
 inline int block_for(int64_t m) { return m >= 65536 ? 256 : 64; }

@@ -182,12 +182,16 @@ class SurrogateMLP:
                self.lib, self.ctx)
         return X
 
-    def train(self, X, y, t, w, epochs=4, batch=4096, lr=1e-3, wreg=0.1, seed=0, log=None):
-        """Mini-batch Adam over the columns of X (bf16, 768 x n); y, t, w FP32 device vectors."""
+    def train(self, X, y, t, w, epochs=4, batch=4096, lr=1e-3, wreg=0.1, seed=0, log=None, lr_final=0.05):
+        """Mini-batch Adam over the columns of X (bf16, 768 x n); y, t, w FP32 device vectors. The step size follows a
+        cosine from lr down to lr_final * lr over the run, so the last epochs settle instead of jumping between
+        minibatch minima (the network's confidence, and with it the triage's skipped fraction, depends on it)."""
         import torch
         n = X.shape[1]
         rng = np.random.default_rng(seed)
         hist = []
+        total = max(1, epochs * (n // batch))
+        it = 0
         for ep in range(epochs):
             perm = torch.from_numpy(rng.permutation(n)).cuda()
             Xp, yp, tp, wp = X[:, perm].contiguous(), y[perm].contiguous(), t[perm].contiguous(), w[perm].contiguous()
@@ -198,7 +202,9 @@ class SurrogateMLP:
                 acts, out = self.forward(xb, batch)
                 dout, part = self.loss(out, yp[s:s + batch], tp[s:s + batch], wp[s:s + batch], wreg)
                 gW, gb = self.backward(xb, batch, acts, dout)
-                self.adam(gW, gb, lr=lr)
+                cos = 0.5 * (1.0 + math.cos(math.pi * it / total))
+                self.adam(gW, gb, lr=lr * (lr_final + (1.0 - lr_final) * cos))
+                it += 1
                 tot += part.view(-1, 2).sum(0).double() / batch
                 nb += 1
             torch.cuda.synchronize()
@@ -224,9 +230,10 @@ class Ensemble:
     """Device buffers of an ensemble of the per-orbit-K standard map: x (3 x m), the indicator frame W and the
     QR frame W2 (4 x m each), the state words of the prefix chain (MEGNO, FLI | QR), codes and steps."""
 
-    def __init__(self, x0, seed):
+    def __init__(self, x0, seed, elt=1):
         import torch
         self.lib, self.ctx = _context()
+        self.elt = elt  # element type of the prefix pass (1 Float64; 2 Float64 orbit, Float32 tangents)
         self.m = m = len(x0)
         self.chain_pre, self.allr = make_chain((MEGNO(replace=False), FLI()), (QRSpectrum(MGS()),))
         self.chain_qr, _ = make_chain((QRSpectrum(MGS()),))
@@ -249,7 +256,7 @@ class Ensemble:
         import torch
         tr = torch.empty((3 * steps, self.m), dtype=torch.float32, device="cuda")
         rc = self.lib.cl_propagate_device_traced(
-            self.ctx, ctypes.byref(self.sd), ctypes.byref(self.chain_pre), 1, self.m, 3, 2, 2, 0, steps, N, 1,
+            self.ctx, ctypes.byref(self.sd), ctypes.byref(self.chain_pre), self.elt, self.m, 3, 2, 2, 0, steps, N, 1,
             self.x.data_ptr(), self.W.data_ptr(), self.W2.data_ptr(), self.st.data_ptr(), self.code.data_ptr(),
             self.h.data_ptr(), tr.data_ptr(), 0, steps, torch.cuda.current_stream().cuda_stream)
         _check(rc, self.lib, self.ctx)
@@ -359,14 +366,15 @@ def evaluate(net, run, margin=2.0):
                 accuracy_all=float(((logit > 0) == truth).mean()), m=int(m))
 
 
-def triage_run(net, x0, seed, N, prefix=256, margin=2.0, log=None):
+def triage_run(net, x0, seed, N, prefix=256, margin=2.0, log=None, elt=1):
     """The triaged method on an ensemble: prefix with features, the network, then the QR spectrum to N/10 and
     N for the orbits the network is not sure about. Returns dict(label, lam_N10, lam_N, confident, times);
-    lambda entries of confidently triaged orbits are NaN."""
+    lambda entries of confidently triaged orbits are NaN. elt = 2 runs the prefix with Float32 tangents (the
+    continuation stays Float64, so the continued orbits' exponents keep the Float64 accuracy)."""
     import torch
     t0 = time.perf_counter()
     m = len(x0)
-    ens = Ensemble(x0, seed)
+    ens = Ensemble(x0, seed, elt)
     tr = ens.prefix(prefix, N)
     logit = net.predict(net.normalise(tr, m), m)[0]
     conf = logit.abs() >= margin

@@ -133,8 +133,9 @@ def check_health(code, hstep, x, W, states, allr, orbit0=1):
     raise NumericalHealthError("state" if c == 1 else "deviation matrix", first, math.inf, int(hstep[i]), orbit, "gpu")
 
 
-ELEMENT_TYPES = {"f64": L.CL_ELT_F64, "f32tan": L.CL_ELT_F64_F32TAN}
-PRECISION_LABELS = {"f64": "Float64", "f32tan": "Float64 orbit, Float32 tangents"}
+ELEMENT_TYPES = {"f64": L.CL_ELT_F64, "f32tan": L.CL_ELT_F64_F32TAN, "dd": L.CL_ELT_DD}
+PRECISION_LABELS = {"f64": "Float64", "f32tan": "Float64 orbit, Float32 tangents",
+                    "dd": "compensated orbit (value + correction), Float64 tangents"}
 
 
 def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None, launch_steps=None,
@@ -143,7 +144,9 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
     reset_frame=True restarts the frame from the unit columns after every handoff (the indicator recorder's window
     kinds, src/Experiments/run_indicator_device.jl:66-80); W0 is then the unit columns.
     elt="f32tan" carries the frames in Float32 between handoffs (FP64 orbit and Jacobian, FP64 reducers: the
-    orthonormalisation and the compensated log sums), on the standard-map family."""
+    orthonormalisation and the compensated log sums), on the standard-map family. elt="dd" carries the orbit of the
+    Henon-Heiles Gauss map as a compensated pair (value, correction), the reference's compensated summation; x0 may
+    give the values only (the corrections start at 0) and the returned x has both halves."""
     import torch
     if elt not in ELEMENT_TYPES:
         raise ValueError(f"elt must be one of {sorted(ELEMENT_TYPES)}, got {elt!r}")
@@ -156,9 +159,14 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
         raise ValueError("x0 must have shape (orbits, state length)")
     m, s = x0.shape
     d = sys.dimension
+    if elt == "dd" and s == sys.state_length:
+        x0 = np.ascontiguousarray(np.concatenate([x0, np.zeros_like(x0)], axis=1))
+        m, s = x0.shape
     if s < d:
         raise ValueError(f"state length {s} is below the system dimension {d}")
-    if s != sys.state_length:
+    if elt == "dd" and s == 2 * sys.state_length:
+        pass
+    elif s != sys.state_length:
         raise ValueError(f"state length {s} does not match {sys!r} (needs {sys.state_length})")
     if not (1 <= k <= d):
         raise ValueError(f"k must be in 1:{d}, got {k}")

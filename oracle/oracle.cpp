@@ -1447,6 +1447,7 @@ template <class P, class T>
 struct HHSys {
   int s = 4, d = 4;
   bool f32 = false;
+  bool comp = false;  // compensated state (z, e), s = 8: the element type CL_ELT_DD
   int jd() const { return 4; }
   T h, tol;  // tol NaN: 8 eps max(1, |Y|)
   int maxiter;
@@ -1494,7 +1495,7 @@ struct HHSys {
         T di = 0;
         for (int c = 0; c < 4; ++c) {
           T Fc[3] = {F[0][c], F[1][c], F[2][c]};
-          T yn = x[c] + ((T)0 + h * cdot(OR_GA_HI[i], OR_GA_LO[i], Fc));
+          T yn = x[c] + ((comp ? x[4 + c] : (T)0) + h * cdot(OR_GA_HI[i], OR_GA_LO[i], Fc));
           T e = P::abs(yn - Y[i][c]);
           di = c == 0 ? e : jmax<P>(di, e);
           Y[i][c] = yn;
@@ -1614,6 +1615,20 @@ struct HHSys {
           O[j * 4 + i] = acc;
         }
       for (int e = 0; e < 4 * k; ++e) W[f][e] = O[e];
+    }
+    if (comp) {  // z, e = TwoSum(z, inc + e): the compensated update (stepper.jl:50-58)
+      for (int c = 0; c < 4; ++c) {
+        if (!ok) {
+          x[c] = x[4 + c] = (T)NAN;
+          continue;
+        }
+        T b = inc[c] + x[4 + c];
+        T sn = x[c] + b, bb = sn - x[c];
+        T err = (x[c] - (sn - bb)) + (b - bb);
+        x[c] = sn;
+        x[4 + c] = err;
+      }
+      return;
     }
     for (int c = 0; c < 4; ++c) x[c] = ok ? x[c] + inc[c] : (T)NAN;
   }
@@ -1796,9 +1811,10 @@ static int8_t run_orbit_seg(const SYS& sys, const cl_chain& ch, int d, int k, in
 template <class P>
 static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int d, int k, int64_t N,
                          int tau, double* x, double* W, double* W2, uint8_t* states, int8_t* code, int64_t* hstep,
-                         int nthreads, const double* A = nullptr, uint32_t* ws = nullptr, bool f32 = false) {
+                         int nthreads, const double* A = nullptr, uint32_t* ws = nullptr, int elt = CL_ELT_F64) {
   typedef typename P::T T;
   const int64_t nws = chain_ws(*ch);
+  const bool f32 = elt == CL_ELT_F64_F32TAN;
   MapSys<P, T> ms;
   ms.f32 = f32;
   ms.kind = sys->kind;
@@ -1817,6 +1833,8 @@ static int propagate_all(const cl_system* sys, const cl_chain* ch, int64_t m, in
   hs.h = (T)sys->h;
   hs.tol = (T)sys->tol;
   hs.maxiter = sys->maxiter > 0 ? sys->maxiter : 50;
+  hs.comp = elt == CL_ELT_DD;
+  hs.s = hs.comp ? 8 : 4;
   int sb = 0;
   int off[CL_MAX_REDUCERS];
   for (int r = 0; r < ch->n; ++r) {
@@ -2366,12 +2384,23 @@ int or_propagate_elt(const cl_system* sys, const cl_chain* ch, int64_t m, int s,
   const bool map = sys->kind == CL_SYS_STANDARD_MAP || sys->kind == CL_SYS_WRAPPED_STANDARD_MAP ||
                    sys->kind == CL_SYS_STANDARD_MAP_K;
   if (elt == CL_ELT_F64) return or_propagate(sys, ch, m, s, d, k, N, tau, x, W, W2, states, code, hstep, math, nthreads);
-  if (elt != CL_ELT_F64_F32TAN || !map) return 3;
+  const bool hh = sys->kind == CL_SYS_HENON_HEILES_GAUSS;
+  if (!((elt == CL_ELT_F64_F32TAN && map) || (elt == CL_ELT_DD && hh && s == 8))) return 3;
   if (math == MATH_CL)
     return propagate_all<F64Cl>(sys, ch, m, s, d, k, N, tau, x, W, W2, (uint8_t*)states, code, hstep, nthreads,
-                                nullptr, nullptr, true);
+                                nullptr, nullptr, elt);
   return propagate_all<F64Libm>(sys, ch, m, s, d, k, N, tau, x, W, W2, (uint8_t*)states, code, hstep, nthreads,
-                                nullptr, nullptr, true);
+                                nullptr, nullptr, elt);
+}
+
+// The compensated Henon-Heiles orbit in __float128 (the state's correction words carried as well), for the energy
+// and exponent comparisons of the double-double element type.
+int or_propagate_quad_elt(const cl_system* sys, const cl_chain* ch, int64_t m, int s, int d, int k, int64_t N, int tau,
+                          double* x, double* W, double* W2, void* states, int8_t* code, int64_t* hstep, int nthreads,
+                          int elt) {
+  if (!or_dims_ok(sys, d, k) || N % tau) return 10;
+  return propagate_all<Quad>(sys, ch, m, s, d, k, N, tau, x, W, W2, (uint8_t*)states, code, hstep, nthreads, nullptr,
+                             nullptr, elt);
 }
 
 // The recorder on a standard map (lifted, s = 2) or a wrapped standard map (s = 4) with the chain (estimator,

@@ -29,6 +29,9 @@ def load():
         o.or_propagate_elt.restype = I
         o.or_propagate_elt.argtypes = [ctypes.POINTER(L.cl_system), ctypes.POINTER(L.cl_chain), I64, I, I, I, I64, I,
                                        P, P, P, P, P, P, I, I, I]
+        o.or_propagate_quad_elt.restype = I
+        o.or_propagate_quad_elt.argtypes = [ctypes.POINTER(L.cl_system), ctypes.POINTER(L.cl_chain), I64, I, I, I, I64,
+                                            I, P, P, P, P, P, P, I, I]
         o.or_propagate_quad.restype = I
         o.or_propagate_quad.argtypes = [ctypes.POINTER(L.cl_system), ctypes.POINTER(L.cl_chain), I64, I, I, I, I64,
                                         I, P, P, P, P, P, P, I]
@@ -95,7 +98,10 @@ def run(sysd, chain, x0, W0, N, tau, k, W2=None, math=MATH_CL, nthreads=None, qu
     code = np.zeros(m, dtype=np.int8)
     hstep = np.zeros(m, dtype=np.int64)
     nt = nthreads or min(16, os.cpu_count() or 1)
-    if quad:
+    if quad and elt != L.CL_ELT_F64:
+        rc = o.or_propagate_quad_elt(ctypes.byref(sysd), ctypes.byref(chain), m, s, d, k, N, tau, _ptr(x), _ptr(W),
+                                     _ptr(W2a), _ptr(states), _ptr(code), _ptr(hstep), nt, elt)
+    elif quad:
         rc = o.or_propagate_quad(ctypes.byref(sysd), ctypes.byref(chain), m, s, d, k, N, tau, _ptr(x), _ptr(W),
                                  _ptr(W2a), _ptr(states), _ptr(code), _ptr(hstep), nt)
     elif elt != L.CL_ELT_F64:
