@@ -476,6 +476,51 @@ def c3_mixed(a, lib, ctx, chain, sets, m, k, tau, desc):
     return out
 
 
+def record_main(a):
+    """--workload record: the per-machine throughput record a Julia ChaoslabHIPExt reads (as ext/ChaoslabCUDAExt.jl:42-62
+    reads data/gpu-benchmark.toml), measured on the reference's record workload: the standard map at K = 2 pi, k = 2,
+    tau = 1, QRSpectrum(MGS()) (the log-sum chain's closest device chain), 1e5 orbits x 1e3 steps for gpu_ns_per_step,
+    one wave (64 orbits) x 1e4 steps for gpu_ns_per_step_wave, the oracle on the host's cores for cpu_ns_per_step.
+    Writes [<device>."Float64"] to --record-path (TOML) and prints the JSON line."""
+    import torch
+    from chaos_amd import StandardMap, QRSpectrum, MGS, propagate
+    from chaos_amd.tangent import device_name
+    from chaos_amd.inputs import torus_uniform
+    sysm = StandardMap(2 * math.pi)
+    red = (QRSpectrum(MGS()),)
+    res = {}
+    for key, m, N in (("gpu_ns_per_step", 100_000, 1000), ("gpu_ns_per_step_wave", 64, 10_000)):
+        x0 = torus_uniform(5, m) * 2 * math.pi
+        propagate(sysm, x0, 100, k=2, tau=1, seed=5, reducers=red, check=False)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        propagate(sysm, x0, N, k=2, tau=1, seed=5, reducers=red, check=False)
+        torch.cuda.synchronize()
+        res[key] = (time.perf_counter() - t0) / (m * N) * 1e9
+    cpu = None
+    if not a.no_cpu:
+        from tests import oracle_lib as O
+        from chaos_amd.tangent import make_chain
+        m, N = 16384, 1000
+        x0 = torus_uniform(5, m) * 2 * math.pi
+        nt = nthreads()
+        t0 = time.perf_counter()
+        O.run(sysm.descriptor(), make_chain(red)[0], x0, O.frames(5, 1, m, 2, 2), N, 1, 2, math=O.MATH_LIBM, nthreads=nt)
+        cpu = (time.perf_counter() - t0) / (m * N) * 1e9
+        res["cpu_ns_per_step"] = cpu
+        res["cpu_threads"] = nt
+    res["gpu_faster"] = bool(cpu is None or res["gpu_ns_per_step"] < cpu)
+    dev = device_name()
+    if a.record_path:
+        lines = [f'["{dev}"."Float64"]'] + [f"{k} = {str(v).lower() if isinstance(v, bool) else v}" for k, v in res.items()]
+        with open(a.record_path, "a") as fh:
+            fh.write("\n".join(lines) + "\n")
+    out = {"metric": "gpu-benchmark record", "value": res["gpu_ns_per_step"], "unit": "ns/orbit-step", "n_gpus": 1,
+           "steps": 1, "warmup": 1, "higher_is_better": False, "device": dev, "record": res}
+    print(json.dumps(out), flush=True)
+    return out
+
+
 def c3_main(a):
     import torch
     from chaos_amd import StandardMapChain, QRSpectrum, HouseholderQR, _lib as L
@@ -972,7 +1017,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "rec", "ind"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "rec", "ind", "record"])
+    ap.add_argument("--record-path", default=None, help="record: the TOML file the entry is appended to")
     ap.add_argument("--ind-orbits", default=1 << 20, type=int, help="ind: orbits of the large cases")
     ap.add_argument("--rec-sizes", default="2048,200000", help="rec: orbits, steps")
     ap.add_argument("--c5-sizes", default="1048576,262144,4194304,100000",
@@ -999,6 +1045,8 @@ def main():
         return c3_main(a)
     if a.workload == "rec":
         return rec_main(a)
+    if a.workload == "record":
+        return record_main(a)
     if a.workload == "ind":
         return ind_main(a)
     wl = workload(a.workload)
