@@ -68,6 +68,11 @@ class cl_lattice(ctypes.Structure):
                 ("dcode", ctypes.c_void_p), ("dhstep", ctypes.c_void_p)]
 
 
+class cl_tuning(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int32), ("split", ctypes.c_int32), ("pipe", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
+
+
 class cl_probe_t(ctypes.Structure):
     _fields_ = [("loadable", ctypes.c_int32), ("functional", ctypes.c_int32), ("compiles", ctypes.c_int32),
                 ("gpu_faster", ctypes.c_int32), ("device", ctypes.c_char * 64)]
@@ -85,6 +90,8 @@ SIGNATURES = {
     "cl_create": (P, [I]),
     "cl_destroy": (None, [P]),
     "cl_last_error": (ctypes.c_char_p, [P]),
+    "cl_set_tuning": (I, [P, ctypes.POINTER(cl_tuning)]),
+    "cl_get_tuning": (I, [P, ctypes.POINTER(cl_tuning)]),
     "cl_state_bytes": (I64, [ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I, I]),
     "cl_propagate_chunk": (I, [P, ctypes.POINTER(cl_system), ctypes.POINTER(cl_chain), I, I64, I, I, I, I64, I64,
                                I64, I, P, P, P, P, P, P, P]),

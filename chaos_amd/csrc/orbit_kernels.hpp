@@ -36,6 +36,8 @@ struct KArgs {
   const double* wtab;
   // restart the frame from the unit columns after every handoff (CL_CHAIN_RESET_FRAME)
   int reset_frame;
+  // lanes per group of the launch (64, 128 or 256), 0 for the size rule
+  int block;
 };
 
 // Per-orbit context the reducers read: the point, the carried sin/cos, the frames and a cache of

@@ -18,10 +18,14 @@ struct cl_ctx {
   int device;
   hipStream_t stream;
   char err[256];
+  cl_tuning tune;  // launch settings chosen by the caller's autotuner (zeros: the built-in rules)
   // grown-on-demand device buffers of the host entry point
   void* buf[14];
   size_t cap[14];
 };
+
+// The launch settings of the call in progress on this thread (the context's cl_tuning).
+static thread_local cl_tuning g_tune = {0, 0, 0, 0};
 
 static char g_create_err[256] = "";
 
@@ -411,6 +415,21 @@ extern "C" void cl_destroy(cl_ctx* c) {
 
 extern "C" const char* cl_last_error(cl_ctx* c) { return c ? c->err : g_create_err; }
 
+extern "C" int cl_set_tuning(cl_ctx* c, const cl_tuning* t) {
+  if (!c || !t) return CL_ERR_ARGUMENT;
+  if (!(t->block == 0 || t->block == 64 || t->block == 128 || t->block == 256) || t->split < 0 || t->split > 2 ||
+      t->pipe < 0 || t->pipe > 2)
+    return set_err(c, CL_ERR_ARGUMENT, "tuning: block 0/64/128/256, split and pipe 0 (automatic), 1 (on), 2 (off)");
+  c->tune = *t;
+  return CL_OK;
+}
+
+extern "C" int cl_get_tuning(cl_ctx* c, cl_tuning* t) {
+  if (!c || !t) return CL_ERR_ARGUMENT;
+  *t = c->tune;
+  return CL_OK;
+}
+
 // The context's stream, for the other translation units of the library (not part of the C ABI).
 extern "C" __attribute__((visibility("hidden"))) hipStream_t cl_ctx_stream(cl_ctx* c) { return c->stream; }
 
@@ -455,6 +474,7 @@ static int launch(const Instance* in, const cl_system* sys, const cl_chain* ch, 
   if (m == 0) return CL_OK;
   KArgs a;
   memset(&a, 0, sizeof(a));
+  a.block = g_tune.block;
   a.ws = ws;
   a.mask = mask;
   a.reset_frame = (ch->flags & CL_CHAIN_RESET_FRAME) ? 1 : 0;
@@ -523,6 +543,7 @@ static bool split_enabled(int kind) {
   const char* e = getenv("CL_SPLIT");
   if (e && e[0] == '0') return false;
   if (e && e[0] == '1') return true;
+  if (g_tune.split) return g_tune.split == 1;
   return kind == CL_SYS_STANDARD_MAP || kind == CL_SYS_WRAPPED_STANDARD_MAP || kind == CL_SYS_STANDARD_MAP_K;
 }
 
@@ -618,6 +639,7 @@ static bool pipe_enabled(int64_t m) {
   const char* e = getenv("CL_PIPE");
   if (e && e[0] == '0') return false;
   if (e && e[0] == '1') return true;
+  if (g_tune.pipe) return g_tune.pipe == 1;
   return m <= PIPE_MAX_ORBITS;
 }
 
@@ -777,6 +799,7 @@ extern "C" int cl_propagate_device(cl_ctx* ctx, const cl_system* sys, const cl_c
     return set_err(ctx, CL_ERR_ARGUMENT, "the chain requests a workspace: call cl_propagate_device_ws");
   hipSetDevice(ctx->device);
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  g_tune = ctx->tune;
   rc = launch_best(in, sys, ch, m, s, d, k, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st);
   if (rc) return set_err(ctx, rc, "kernel launch failed");
   return CL_OK;
@@ -798,6 +821,7 @@ extern "C" int cl_propagate_device_ws(cl_ctx* ctx, const cl_system* sys, const c
   if (m > 0 && !dws) return set_err(ctx, CL_ERR_ARGUMENT, "null workspace");
   hipSetDevice(ctx->device);
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  g_tune = ctx->tune;
   rc = launch_best(in, sys, ch, m, s, d, k, dx, dW, dW2, dst, dcode, dhstep, n0, steps, N, tau, st, dws);
   if (rc) return set_err(ctx, rc, "kernel launch failed");
   return CL_OK;
@@ -1263,6 +1287,7 @@ extern "C" int cl_propagate_chunk(cl_ctx* ctx, const cl_system* sys, const cl_ch
       hipLaunchKernelGGL(k_aos_to_soa32, dim3(grid_for(m * nws, B)), dim3(B), 0, st, wsa, dws, m, nws);
     }
   }
+  g_tune = ctx->tune;
   rc = launch_best(in, sys, ch, m, s, d, k, (double*)dx, (double*)dW, (double*)dW2, dst, dcode, dh, n0, steps, N,
                    tau, st, dws);
   if (rc) return set_err(ctx, rc, "kernel launch failed");

@@ -75,7 +75,7 @@ int launch_impl(const cl_system* s, const cl_chain* d, const KArgs& a, hipStream
   Sys sys = make_sys<Sys>(*s);
   Chain ch;
   chain_setup(ch, *d, 0);
-  int block = block_for(a.m);
+  int block = a.block ? a.block : block_for(a.m);
   int64_t grid = (a.m + block - 1) / block;
   hipLaunchKernelGGL((orbit_kernel<Sys, K, NF, Chain, F32T>), dim3((unsigned)grid), dim3(block), 0, st, a, sys, ch);
   hipError_t e = hipGetLastError();

@@ -353,9 +353,10 @@ def record_indicators(sys, x0, N, *, members=("FLI", "MEGNO", "SALI"), reader=No
     or one per orbit; default deviation_frame(direction, 2)); each row holds the members' series columns. The pair
     kind: `sys` a PairMap, x0 the pair states (pair_start), members with RLI (the study's spec RLI, MEGNO), W0 the
     block frame (4 x 2; default pair_frame of the direction); the point columns hold orbit A's state. A window
-    kind (`reader` a WeightedBirkhoff): the chain is RunMonitor then the reader, stepped with the constant deviation
-    column (1, 0) reset at every handoff; its rows hold the monitor and the point, and its tables come from the
-    reader's final state (finals_of on `states`). `sys` is a StandardMap (x0 of shape (m, 2)) or a
+    kind (`reader` a WeightedBirkhoff, kind "birkhoff", or a WeightedPhaseAdvance, kind "phase_advance",
+    run_indicator.jl:5-6): the chain is RunMonitor then the reader, stepped with the constant deviation column (1, 0)
+    reset at every handoff; its rows hold the monitor and the point, and its tables come from the reader's final
+    state (finals_of on `states`). `sys` is a StandardMap (x0 of shape (m, 2)) or a
     WrappedStandardMap (x0 of shape (m, 4)); `grid` the record steps (default: the study grid). Launches end at the
     grid steps and are split at `launch_steps` (a multiple of tau) when given."""
     import torch
@@ -368,6 +369,14 @@ def record_indicators(sys, x0, N, *, members=("FLI", "MEGNO", "SALI"), reader=No
         raise ValueError("a window kind runs on the map itself, not a pair")
     N, tau = int(N), int(tau)
     window = reader is not None
+    if window:
+        from .birkhoff import WeightedBirkhoff, WeightedPhaseAdvance
+        if isinstance(reader, WeightedPhaseAdvance):
+            window_kind = "phase_advance"
+        elif isinstance(reader, WeightedBirkhoff):
+            window_kind = "birkhoff"
+        else:
+            raise ValueError(f"a window kind's reader is a WeightedBirkhoff or a WeightedPhaseAdvance, got {reader!r}")
     k = 1 if window else 2
     if grid is None:
         grid = record_grid(N, tau, points_per_decade=20, first_steps=100, extra_fractions=(0.1, 0.5))
@@ -442,7 +451,7 @@ def record_indicators(sys, x0, N, *, members=("FLI", "MEGNO", "SALI"), reader=No
     if timing is not None:
         timing["device_ms"] = ev0.elapsed_time(ev1)
     raw = dst.cpu().numpy().T.copy().view(np.uint8).reshape(m, words * 8)
-    return IndRecords(kind="birkhoff" if window else ("pair" if pair else "tangent"), columns=columns, grid=grid,
+    return IndRecords(kind=window_kind if window else ("pair" if pair else "tangent"), columns=columns, grid=grid,
                       R=dR.cpu().numpy(), fail=dfail.cpu().numpy(), code=dcode.cpu().numpy(), hstep=dh.cpu().numpy(),
                       x=dx.cpu().numpy().T.copy(), W=dW.cpu().numpy().reshape(k, d, m).transpose(2, 1, 0).copy(),
                       states=raw, reducers=allr, s=ns)

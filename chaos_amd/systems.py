@@ -132,6 +132,12 @@ TWO_PI = 2.0 * (3.141592653589793 + 1.2246467991473532e-16)
 
 
 def _reduce(u, tp):
+    """(u mod tp in [0, tp), the count) as the reference's _reduce (wrapped_standard_map.jl:16-29). A non-finite u
+    follows Julia's arithmetic, where floor keeps NaN and +-Inf: the remainder is NaN and the count u / tp, so a
+    non-finite start gives a NaN state instead of a conversion error."""
+    if not math.isfinite(u):
+        a = u / tp
+        return u - tp * a, a
     a = math.floor(u / tp)
     r = u - tp * a
     if r < 0:
@@ -175,11 +181,16 @@ class PairMap:
 
 
 def _wrap_diff(u, tp):
-    """Minimal image of a coordinate difference (rli.jl:145): u - tp round(u / tp), ties to even."""
+    """Minimal image of a coordinate difference (rli.jl:145): u - tp round(u / tp), ties to even; a non-finite
+    difference stays non-finite (NaN), as Julia's round keeps NaN and Inf."""
+    if not math.isfinite(u):
+        return u - tp * (u / tp)
     return u - tp * round(u / tp)
 
 
 def _safe_norm2(a, b):
+    if math.isnan(a) or math.isnan(b):  # Julia's max propagates NaN
+        return math.nan
     m = max(abs(a), abs(b))
     if not (math.isfinite(m) and m > 0):
         return m

@@ -156,6 +156,20 @@ typedef struct {
 
 typedef struct cl_ctx cl_ctx;
 
+/* Launch settings an autotuner picks per context (zeros: the built-in rules): `block` lanes per group of the orbit
+ * kernel (64, 128 or 256; the rule is 256 from 65536 orbits on, else 64), `split` the two single-frame passes of a
+ * two-frame map chain and `pipe` the wave-specialised pipeline of small QR ensembles (1 on, 2 off). Every setting
+ * gives the same bits. No reference counterpart: the reference's launch plan is its measured one-wave throughput
+ * record (ext/ChaoslabCUDAExt.jl:42-62, src/Tangent/propagate.jl:144-193). */
+typedef struct {
+  int32_t block;
+  int32_t split;
+  int32_t pipe;
+  int32_t pad_;
+} cl_tuning;
+int cl_set_tuning(cl_ctx *ctx, const cl_tuning *t);
+int cl_get_tuning(cl_ctx *ctx, cl_tuning *t);
+
 /* Foundation.probe(::HIPDevice, T) (src/Foundation/Foundation.jl:92-93; ext/ChaoslabCUDAExt.jl:87-91). */
 int cl_probe(int elt, cl_probe_t *out);
 /* Foundation.free_memory(::HIPDevice) (ext/ChaoslabCUDAExt.jl:31); -1 without a device. */
