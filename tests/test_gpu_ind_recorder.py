@@ -144,3 +144,34 @@ def test_phase_advance_window_kind_equals_oracle(obs, wrapped):
     assert np.array_equal(rec.R.view(np.uint64), orc["R"].view(np.uint64))
     assert np.array_equal(rec.states, orc["states"])
     assert rec.fail[6] == 1 and np.mean(rec.fail == 0) > 0.99
+
+
+def test_phase_advance_planted_branch_cases():
+    """The planar increment's device-only branches against the oracle bit for bit: a start at the origin (a fixed
+    point of the standard map: every sample is 0, the increment NaN), a start about 1e-150 from it (the product of the
+    samples falls below 1e-280 and both are rescaled by powers of two first), and starts near the origin at K = -5,
+    where the tangent at the origin has two negative eigenvalues, so consecutive samples point nearly opposite and the
+    increments fall near the branch cut (near_pi counts above 0)."""
+    from chaos_amd import WeightedPhaseAdvance, PlanarObservable
+    from chaos_amd.tangent import decode_states, finals_of
+    m, tau, windows = 256, 1, (64,)
+    N = 2 * max(windows) + 1
+    sysm = StandardMap(-5.0)
+    rng = np.random.default_rng(43)
+    x0 = 1e-3 * rng.standard_normal((m, 2))
+    x0[0] = [0.0, 0.0]
+    x0[1] = [1e-150, -2e-150]
+    x0[2] = [3e-151, 1e-151]
+    pa = WeightedPhaseAdvance(window=windows, observable=PlanarObservable(1, 2), birkhoff_order=2)
+    grid = record_grid(N, tau, points_per_decade=10, first_steps=10, extra_fractions=(0.5,))
+    chain, _ = make_chain((RunMonitor(sysm), pa), reset_frame=True)
+    W1 = np.broadcast_to(np.array([[1.0], [0.0]]), (m, 2, 1))
+    orc = O.ind_record(sysm.descriptor(), chain, x0, W1, N, tau, 1, grid, 3 + 2)
+    say("ind phase advance planted", "m", m, "N", N)
+    rec = record_indicators(sysm, x0, N, reader=pa, tau=tau, grid=grid)
+    assert np.array_equal(rec.fail, orc["fail"]) and np.array_equal(rec.code, orc["code"])
+    assert np.array_equal(rec.R.view(np.uint64), orc["R"].view(np.uint64))
+    assert np.array_equal(rec.states, orc["states"])
+    fin = finals_of(decode_states(rec.states, rec.reducers, 2, 1), rec.reducers, 2, 1)[1]
+    assert np.any(np.asarray(fin["near_pi"]) > 0)
+    assert np.all(np.isnan(np.asarray(fin["nu1"])[0]))  # the origin: every increment NaN
