@@ -561,8 +561,10 @@ def c3_main(a):
         dh = torch.zeros(m, dtype=torch.int64, device="cuda")
         sets.append(dict(N=N, sys=sysm, sd=sysm.descriptor(), x0=x0, Ns=Ns, order=order, dx=dx, dW=dW, dst=dst,
                          dc=dc, dh=dh, xm=dx.clone()))
+    shared = torch.cuda.Stream() if a.c3_mode == "serial" else None
     for S in sets:
-        S["stream"] = torch.cuda.Stream()  # the five ensembles run concurrently, one stream each (default priority)
+        # the five ensembles run concurrently, one stream each (default priority); "serial": one after another
+        S["stream"] = shared if shared is not None else torch.cuda.Stream()
 
     def run(S, Ns):
         sh = S["stream"].cuda_stream
@@ -628,10 +630,14 @@ def c3_main(a):
         "metric": METRIC, "value": total * a.steps / wall, "unit": "orbit-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": 1, "ms_per_step": wall * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": desc, "orbit_steps_per_step": total, "kernel_ms_per_N": ms, "nonfinite_orbits": bad},
+        "config": {"workload": desc + ("; the five launches in turn on one stream" if a.c3_mode == "serial" else
+                                       "; the five launches on five streams"),
+                   "orbit_steps_per_step": total, "kernel_ms_per_N": ms, "nonfinite_orbits": bad},
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel_ms": span,
-                     "flops_per_orbit_step": {N: c3_flops(N) for N in C3_NS}},
+                     "flops_per_orbit_step": {N: c3_flops(N) for N in C3_NS},
+                     "frac_per_N": {N: c3_flops(N) * steps_n[N] / (ms[N] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS
+                                    for N in steps_n}},
         "cpu_baseline": cpu,
     }
     if cpu:
@@ -1044,8 +1050,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="c2: skip the Float32-tangent, accuracy, C1 and C5 blocks of the default run")
-    ap.add_argument("--c3-mode", default="streams", choices=["streams", "mixed"],
-                    help="c3: one launch per lattice on five streams, or one persistent launch over all five")
+    ap.add_argument("--c3-mode", default="streams", choices=["streams", "mixed", "serial"],
+                    help="c3: one launch per lattice on five streams, one persistent launch over all five, or the five "
+                         "launches one after another on one stream (each lattice's kernel alone on the device)")
     ap.add_argument("--c3-chunk", type=int, default=0, help="c3 mixed: steps per graph-captured chunk (0: one chunk)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 1 GiB device-to-device (the known-size kernel that calibrates FETCH_SIZE/WRITE_SIZE)")

@@ -123,6 +123,7 @@ SIGNATURES = {
     "cl_math_host": (None, [I64, P, P, P, P]),
     "cl_gw_host": (None, [I64, P, P, P, P, P]),
     "cl_math_device": (I, [P, I64, P, P, P, P, P]),
+    "cl_sqrt12_device": (I, [P, I64, P, P, P]),
     "cl_mlp_gemm": (I, [P, I, I, I, I, I, P, I, P, I, P, I, I, P, I, P, I, P]),
     "cl_mlp_rowsum": (I, [P, I, I, P, I, P, P]),
     "cl_mlp_loss": (I, [P, I, P, P, P, P, ctypes.c_float, P, P, P]),

@@ -64,7 +64,7 @@ CL_HD double safe_norm(const double* v, int st = 1) {
     // 1 + b^2 equals b^2 + 1, so the sum is the same in either order
     double other = cl_fabs(v[0]) >= cl_fabs(v[st]) ? v[st] : v[0];
     double b = other / m;
-    return m * cl_sqrt(1.0 + b * b);
+    return m * cl_sqrt12(1.0 + b * b);
   }
   double a = v[0] / m;
   double s = a * a;

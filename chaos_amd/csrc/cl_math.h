@@ -68,6 +68,26 @@ CL_HD int cl_signbit(double x) { return (int)(cl_bits(x) >> 63); }
 CL_HD double cl_fabs(double x) { return cl_from_bits(cl_bits(x) & 0x7fffffffffffffffull); }
 CL_HD double cl_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 CL_HD double cl_sqrt(double x) { return __builtin_sqrt(x); }
+/* sqrt(x) for x in [1, 2] (the 2-entry norms' sqrt(1 + b^2), |b| <= 1), correctly rounded like cl_sqrt. On the
+ * device, Goldschmidt's iteration from the hardware reciprocal square root y: g = x y, h = y / 2, r = 1/2 - h g,
+ * g += g r, h += h r, then twice g += (x - g^2) h with a fused residual. This is the general square root's arithmetic
+ * without its range scaling and special-value selects, which x in [1, 2] never needs; the device parity test compares
+ * it bit for bit with the host's IEEE square root. */
+CL_HD double cl_sqrt12(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = y * 0.5;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  double d = __builtin_fma(-g, g, x);
+  h = __builtin_fma(h, r, h);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, h, g);
+#else
+  return __builtin_sqrt(x);
+#endif
+}
 CL_HD double cl_floor(double x) { return __builtin_floor(x); }
 CL_HD double cl_rint(double x) { return __builtin_rint(x); }
 /* 2^e for e in [-1022, 1023] */

@@ -352,6 +352,12 @@ __global__ void k_math(int64_t n, const double* __restrict__ x, double* sn, doub
   lg[i] = cl_log(x[i]);
 }
 
+__global__ void k_sqrt12(int64_t n, const double* __restrict__ x, double* __restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  y[i] = cl_sqrt12(x[i]);
+}
+
 static unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // ------------------------------------------------------------------------------ probe / context
@@ -1399,5 +1405,14 @@ extern "C" int cl_math_device(cl_ctx* ctx, int64_t n, const double* dx, double* 
   hipSetDevice(ctx->device);
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
   hipLaunchKernelGGL(k_math, dim3(grid_for(n, 256)), dim3(256), 0, st, n, dx, dsn, dcs, dlg);
+  return map_err(hipGetLastError());
+}
+
+extern "C" int cl_sqrt12_device(cl_ctx* ctx, int64_t n, const double* dx, double* dy, void* stream) {
+  if (!ctx || n < 0) return CL_ERR_ARGUMENT;
+  if (n == 0) return CL_OK;
+  hipSetDevice(ctx->device);
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  hipLaunchKernelGGL(k_sqrt12, dim3(grid_for(n, 256)), dim3(256), 0, st, n, dx, dy);
   return map_err(hipGetLastError());
 }

@@ -338,6 +338,9 @@ int cl_initial_deviations_device(cl_ctx *ctx, uint64_t seed, int64_t orbit0, int
 void cl_math_host(int64_t n, const double *x, double *sn, double *cs, double *lg);
 int cl_math_device(cl_ctx *ctx, int64_t n, const double *dx, double *dsn, double *dcs, double *dlg,
                    void *stream);
+/* The square root the kernels' 2-entry norms take of 1 + b^2 in [1, 2], on the device (for its parity test against
+ * the host's IEEE square root; no counterpart in the reference, whose norm calls sqrt). */
+int cl_sqrt12_device(cl_ctx *ctx, int64_t n, const double *dx, double *dy, void *stream);
 /* G_w = ||R||_F ||R^-1||_F of n 2 x 2 upper-triangular factors R (column-major, 4 entries each) as the kernels
  * form it on the host: window_Gw (orthonormalize.jl:112-127) and the one-division estimate the kernels' checks
  * use to skip it (-1 outside its range); settled[i] is the fused QR step's decision (Householder, orth_loss 0)

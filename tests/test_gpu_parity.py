@@ -181,6 +181,32 @@ def test_device_math_and_frames_bitwise():
         assert np.array_equal(dW.cpu().numpy().T.reshape(m, k, d_), host)
 
 
+def test_device_sqrt12_is_correctly_rounded():
+    """The 2-entry norms' square root on [1, 2] (no range scaling, no special values) equals the host's IEEE square
+    root bit for bit: 2^24 uniform arguments, the end points and their neighbours, the squares of 26-bit numbers
+    (exact roots) and their neighbours, and arguments whose root lies within 2^-30 ulp of a rounding midpoint
+    (x = RN(m^2) for the midpoints m = (2a + 1) 2^-53 of [1, sqrt 2): the hardest cases for a rounding decision)."""
+    import torch
+    lib, ctx = _context()
+    rng = np.random.default_rng(12)
+    u = 1.0 + rng.random(1 << 24)
+    ends = np.array([1.0, 2.0, np.nextafter(1.0, 2.0), np.nextafter(2.0, 1.0), np.nextafter(np.nextafter(1.0, 2.0), 2.0)])
+    r26 = 1.0 + rng.integers(0, 1 << 26, 200000) * 2.0 ** -26
+    r26 = r26[r26 * r26 <= 2.0]
+    sq = r26 * r26  # exact (26 + 26 bits)
+    near = np.concatenate([sq, np.nextafter(sq, 0.0), np.nextafter(sq, 4.0)])
+    # midpoints (2a + 1) 2^-53 of [1, sqrt 2) and x = RN(midpoint^2) by exact integer arithmetic
+    odd = 2 * rng.integers(1 << 52, int((1 << 53) * 0.7071), 200000, dtype=np.int64) + 1
+    hard = np.array([math.ldexp(float(int(o) * int(o)), -106) for o in odd])
+    xs = np.concatenate([u, ends, near[(near >= 1.0) & (near <= 2.0)], hard[(hard >= 1.0) & (hard <= 2.0)]])
+    say("device sqrt12", len(xs))
+    dx = torch.from_numpy(xs).cuda()
+    dy = torch.empty_like(dx)
+    assert lib.cl_sqrt12_device(ctx, len(xs), dx.data_ptr(), dy.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(np.sqrt(xs).view(np.uint64), dy.cpu().numpy().view(np.uint64))
+
+
 def test_k0_shear_closed_form_on_gpu():
     """tc_ac01_04.jl:52-64 on the device: K = 0, W0 = I: W_n = [[1,n],[0,1]], x_n = x0 + n p0 bitwise."""
     for n in (1, 1000, 10 ** 6):
