@@ -54,7 +54,7 @@ struct HHGauss {
   struct Aux {
     bool ok;
     double inc[4];
-    double q[3][2];  // positions of the converged stages
+    double J[16];  // the one-step Jacobian at the converged stages, column-major (all NaN when the solve failed)
   };
 
   // J grad H for H = |p|^2/2 + |q|^2/2 + q1^2 q2 - q2^3/3 (src/Systems/flow_models.jl:101-103, flow_core.jl:75)
@@ -128,15 +128,25 @@ struct HHGauss {
       double Fc[3] = {F[0][c], F[1][c], F[2][c]};
       a.inc[c] = h * dot_comp3(bhi, blo, Fc);
     }
+    double q[3][2];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      a.q[i][0] = Y[i][0];
-      a.q[i][1] = Y[i][1];
+      q[i][0] = Y[i][0];
+      q[i][1] = Y[i][1];
+    }
+    step_jacobian(q, a.J);
+    if (!a.ok) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) a.J[e] = __builtin_nan("");
     }
   }
 
-  // The one-step Jacobian (4 x 4, column-major J[c * 4 + r]) from the converged stages.
-  CL_HD void step_jacobian(const Aux& a, double* J) const {
+  // The one-step Jacobian (4 x 4, column-major J[c * 4 + r]) from the converged stage positions q. B is factored
+  // once (partial pivoting, first largest |pivot|; the multipliers kept below the diagonal and rows swapped whole),
+  // then the four right-hand sides are solved one column at a time: all the row swaps first, then the forward
+  // elimination and back substitution. Every entry of a right-hand side sees the same multipliers and pivot rows in
+  // the same order as when the columns are eliminated alongside B, with a quarter of the live registers.
+  CL_HD void step_jacobian(const double (&q)[3][2], double* J) const {
     const double Ahi[3][3] = CL_GA_HI, Alo[3][3] = CL_GA_LO;
     const double bhi[3] = CL_GB_HI, blo[3] = CL_GB_LO;
     double ha[3][3], hb[3], H[3][2][2];
@@ -145,17 +155,17 @@ struct HHGauss {
 #pragma unroll
       for (int j = 0; j < 3; ++j) ha[i][j] = h * (Ahi[i][j] + Alo[i][j]);
       hb[i] = h * (bhi[i] + blo[i]);
-      H[i][0][0] = 1.0 + 2.0 * a.q[i][1];
-      H[i][1][0] = 2.0 * a.q[i][0];
-      H[i][0][1] = 2.0 * a.q[i][0];
-      H[i][1][1] = 1.0 - 2.0 * a.q[i][1];
+      H[i][0][0] = 1.0 + 2.0 * q[i][1];
+      H[i][1][0] = 2.0 * q[i][0];
+      H[i][0][1] = 2.0 * q[i][0];
+      H[i][1][1] = 1.0 - 2.0 * q[i][1];
     }
-    // B (6 x 6) and R (6 x 4), row-major, block (i, l) rows 2i..2i+1, columns 2l..2l+1
-    double B[6][6], R[6][4];
+    // B (6 x 6), row-major, block (i, l) rows 2i..2i+1, columns 2l..2l+1
+    double B[6][6];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int r = 0; r < 2; ++r) {
+      for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int l = 0; l < 3; ++l)
 #pragma unroll
@@ -165,16 +175,7 @@ struct HHGauss {
             for (int j = 0; j < 3; ++j) v = v + (ha[i][j] * ha[j][l]) * H[j][r][c];
             B[2 * i + r][2 * l + c] = v;
           }
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          double v = 0.0;
-#pragma unroll
-          for (int j = 0; j < 3; ++j) v = v + ha[i][j] * H[j][r][c];
-          R[2 * i + r][c] = -v;
-          R[2 * i + r][2 + c] = (r == c) ? 1.0 : 0.0;
-        }
-      }
-    // Gaussian elimination with partial pivoting (first largest |pivot| wins), then back substitution.
+    int pv[6];
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
       int piv = p;
@@ -187,6 +188,7 @@ struct HHGauss {
           piv = r;
         }
       }
+      pv[p] = piv;
 #pragma unroll
       for (int r = p + 1; r < 6; ++r) {  // swap rows p and piv without a dynamic register index
         if (r == piv) {
@@ -196,96 +198,100 @@ struct HHGauss {
             B[p][c] = B[r][c];
             B[r][c] = t;
           }
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            double t = R[p][c];
-            R[p][c] = R[r][c];
-            R[r][c] = t;
-          }
         }
       }
 #pragma unroll
       for (int r = p + 1; r < 6; ++r) {
-        double f = B[r][p] / B[p][p];
+        const double f = B[r][p] / B[p][p];
+        B[r][p] = f;
 #pragma unroll
         for (int c = p + 1; c < 6; ++c) B[r][c] = B[r][c] - f * B[p][c];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) R[r][c] = R[r][c] - f * R[p][c];
       }
     }
-    double Qv[6][4];
 #pragma unroll
-    for (int r = 5; r >= 0; --r)
+    for (int c = 0; c < 4; ++c) {
+      // right-hand side column c: -h sum_j a_ij H_j E_q (c < 2), E_p (c >= 2)
+      double v[6];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        double v = R[r][c];
-#pragma unroll
-        for (int t = r + 1; t < 6; ++t) v = v - B[r][t] * Qv[t][c];
-        Qv[r][c] = v / B[r][r];
-      }
-    // P_i = E_q + sum_j ha_ij Q_j; J = I + sum_i hb_i [Q_i; -H_i P_i]
-    double Jm[4][4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) Jm[r][c] = (r == c) ? 1.0 : 0.0;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      double P[2][4];
-#pragma unroll
-      for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          double v = (r == c) ? 1.0 : 0.0;
-#pragma unroll
-          for (int j = 0; j < 3; ++j) v = v + ha[i][j] * Qv[2 * j + r][c];
-          P[r][c] = v;
-        }
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-          Jm[r][c] = Jm[r][c] + hb[i] * Qv[2 * i + r][c];
-          double hp = H[i][r][0] * P[0][c] + H[i][r][1] * P[1][c];
-          Jm[2 + r][c] = Jm[2 + r][c] + hb[i] * (-hp);
+          if (c < 2) {
+            double w = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) w = w + ha[i][j] * H[j][r][c];
+            v[2 * i + r] = -w;
+          } else {
+            v[2 * i + r] = (r == c - 2) ? 1.0 : 0.0;
+          }
+        }
+#pragma unroll
+      for (int p = 0; p < 6; ++p)
+#pragma unroll
+        for (int r = p + 1; r < 6; ++r)
+          if (r == pv[p]) {
+            double t = v[p];
+            v[p] = v[r];
+            v[r] = t;
+          }
+#pragma unroll
+      for (int p = 0; p < 6; ++p)
+#pragma unroll
+        for (int r = p + 1; r < 6; ++r) v[r] = v[r] - B[r][p] * v[p];
+      double Qv[6];
+#pragma unroll
+      for (int r = 5; r >= 0; --r) {
+        double w = v[r];
+#pragma unroll
+        for (int t = r + 1; t < 6; ++t) w = w - B[r][t] * Qv[t];
+        Qv[r] = w / B[r][r];
+      }
+      // P_i = E_q + sum_j ha_ij Q_j; J = I + sum_i hb_i [Q_i; -H_i P_i]
+      double Jc[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Jc[r] = (r == c) ? 1.0 : 0.0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        double P[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          double w = (r == c) ? 1.0 : 0.0;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) w = w + ha[i][j] * Qv[2 * j + r];
+          P[r] = w;
+        }
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          Jc[r] = Jc[r] + hb[i] * Qv[2 * i + r];
+          double hp = H[i][r][0] * P[0] + H[i][r][1] * P[1];
+          Jc[2 + r] = Jc[2 + r] + hb[i] * (-hp);
         }
       }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) J[c * 4 + r] = Jc[r];
     }
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) J[c * 4 + r] = Jm[r][c];
   }
 
   // W <- J_step W with the fused_product order (src/Foundation/product.jl:9-20); a failed stage solve
   // makes the tangent all NaN, as the reference's `W * NaN` (gauss.jl:179).
   template <int K2>
   CL_HD void tangent_apply(const Aux& a, Frame<4, K2>& W) const {
-    double J[16];
-    step_jacobian(a, J);
-    if (!a.ok) {
-#pragma unroll
-      for (int e = 0; e < 16; ++e) J[e] = __builtin_nan("");
-    }
     Frame<4, K2> O;
 #pragma unroll
     for (int j = 0; j < K2; ++j)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        double acc = J[i] * W.at(0, j);
+        double acc = a.J[i] * W.at(0, j);
 #pragma unroll
-        for (int q = 1; q < 4; ++q) acc = cl_fma(J[q * 4 + i], W.at(q, j), acc);
+        for (int q = 1; q < 4; ++q) acc = cl_fma(a.J[q * 4 + i], W.at(q, j), acc);
         O.at(i, j) = acc;
       }
     W = O;
   }
   // the one-step tangent at the current point, column-major (NaN when the stage solve failed)
   CL_HD void jacobian(const Aux& a, double* J) const {
-    step_jacobian(a, J);
-    if (!a.ok) {
 #pragma unroll
-      for (int e = 0; e < 16; ++e) J[e] = __builtin_nan("");
-    }
+    for (int e = 0; e < 16; ++e) J[e] = a.J[e];
   }
   CL_HD void forward(const Aux& a, double* z) const {
 #pragma unroll
