@@ -386,11 +386,54 @@ CL_HD int8_t frame_status_f(const Ctx& c, const Frame<Sys::D, K, float> (&Wf)[NF
 }
 
 // ------------------------------------------------------------------------------ the kernel
+// Systems whose step needs most of the register file (the Gauss stage solve and its Jacobian) park the frames in
+// LDS while the step solves its stages (a slot per thread, written after the tangent product and read back before
+// the next one), so that they are not live across the solve, and ask for two waves per SIMD. On the Henon-Heiles
+// workload (196608 orbits, two 4 x 4 frames, MEGNO + GALI and the Givens QR spectrum) the launch takes 5.31 s this
+// way against 6.05 s with the frames in registers at one wave per SIMD, 6.12 s parked at one wave, 7.20 s parked at
+// three, and 5.55 s (6.00 s unparked) when the reducer states also go to HBM between handoffs. Other systems keep
+// their frames in registers for the whole launch.
+template <class Sys>
+struct OrbitPlan {
+  static constexpr bool park_frames = false;
+  static constexpr int waves = 1;
+};
+template <>
+struct OrbitPlan<HHGauss> {
+  static constexpr bool park_frames = true;
+  static constexpr int waves = 2;
+};
+template <>
+struct OrbitPlan<HHGaussComp> : OrbitPlan<HHGauss> {};
+
+// The frames' LDS slots (dynamic shared memory, NF d k doubles per thread): entry e of thread t at
+// park[e * blockDim + t], conflict-free 8-byte accesses. The empty asm with a memory clobber keeps the compiler from
+// forwarding the stored values in registers past the solve.
+template <int D, int K, int NF>
+__device__ __attribute__((always_inline)) inline void park_store(double* park, const Frame<D, K> (&W)[NF]) {
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int e = 0; e < D * K; ++e) park[(f * D * K + e) * blockDim.x + threadIdx.x] = W[f].w[e];
+  __asm__ volatile("" ::: "memory");
+}
+template <int D, int K, int NF>
+__device__ __attribute__((always_inline)) inline void park_load(const double* park, Frame<D, K> (&W)[NF]) {
+  __asm__ volatile("" ::: "memory");
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int e = 0; e < D * K; ++e) W[f].w[e] = park[(f * D * K + e) * blockDim.x + threadIdx.x];
+}
+
 // F32T: the FP32-tangent element type. The frames are carried in Float32 between handoffs (the input frames are
 // rounded to Float32 at the start of the launch); at each handoff they are widened exactly to double, the chain runs
 // in FP64 as for the Float64 type, and the frames it leaves are rounded back to Float32. The orbit stays FP64.
 template <class Sys, int K, int NF, class Chain, bool F32T = false>
-__global__ void __launch_bounds__(256) orbit_kernel(KArgs a, Sys sys, Chain chain0) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OrbitPlan<Sys>::waves)))
+orbit_kernel(KArgs a, Sys sys, Chain chain0) {
+  constexpr bool PF = OrbitPlan<Sys>::park_frames && !F32T;
+  extern __shared__ double park[];
   constexpr int D = Sys::D;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.m) return;
@@ -437,7 +480,9 @@ __global__ void __launch_bounds__(256) orbit_kernel(KArgs a, Sys sys, Chain chai
         else sys.tangent_apply(c.aux, c.W[f]);
       }
       sys.forward(c.aux, c.x);
+      if constexpr (PF) park_store<D, K, NF>(park, c.W);
       sys.aux(c.x, c.aux);
+      if constexpr (PF) park_load<D, K, NF>(park, c.W);
     }
     n += a.tau;
     if constexpr (F32T) code = frame_status_f<Sys, K, NF>(c, Wf);

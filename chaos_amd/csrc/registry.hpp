@@ -77,7 +77,9 @@ int launch_impl(const cl_system* s, const cl_chain* d, const KArgs& a, hipStream
   chain_setup(ch, *d, 0);
   int block = a.block ? a.block : block_for(a.m);
   int64_t grid = (a.m + block - 1) / block;
-  hipLaunchKernelGGL((orbit_kernel<Sys, K, NF, Chain, F32T>), dim3((unsigned)grid), dim3(block), 0, st, a, sys, ch);
+  // the frames' LDS slots of systems that park them during the step (OrbitPlan), sized by the group
+  const size_t lds = OrbitPlan<Sys>::park_frames && !F32T ? (size_t)NF * Sys::D * K * block * sizeof(double) : 0;
+  hipLaunchKernelGGL((orbit_kernel<Sys, K, NF, Chain, F32T>), dim3((unsigned)grid), dim3(block), lds, st, a, sys, ch);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (e == hipErrorOutOfMemory ? CL_ERR_OOM : CL_ERR_DEVICE_LOST);
 }
