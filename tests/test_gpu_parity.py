@@ -280,6 +280,35 @@ def test_henon_heiles_gauss_bitwise(chains):
         compare(res, orc, allr, 4, k)
 
 
+@pytest.mark.parametrize("block", [0, 128])
+def test_henon_heiles_large_groups_bitwise(block):
+    """The Gauss kernel parks its frames in LDS slots sized by the group: at 65536 orbits the launch takes 256-lane
+    groups (block 0: the built-in rule), or the 128 lanes a tuned setting asks for. The two-frame C4 chain (MEGNO +
+    GALI | Givens QR spectrum) equals the oracle bit for bit on every 256th orbit."""
+    from chaos_amd import HenonHeilesGauss
+    from chaos_amd.autotune import set_tuning
+    from chaos_amd.inputs import hh_section_starts
+    red, red2 = (MEGNO(replace=False), GALI()), (QRSpectrum(GivensQR()),)
+    m, N = 65536, 40
+    say("henon-heiles large groups", "block", block, "m", m)
+    sysm = HenonHeilesGauss(h=0.01, tol=1e-15)
+    x0 = hh_section_starts(7, m, 1 / 8)
+    set_tuning(block=block)
+    try:
+        res = propagate(sysm, x0, N, k=4, tau=10, seed=8, reducers=red, reducers2=red2, check=False, keep_states=True)
+    finally:
+        set_tuning()
+    sel = np.arange(0, m, 256)
+    ch, allr = make_chain(red, red2)
+    W0 = O.frames(8, 1, m, 4, 4)[sel]
+    orc = O.run(sysm.descriptor(), ch, x0[sel], W0, N, 10, 4, W2=W0, math=O.MATH_CL)
+    assert np.array_equal(res.x[sel].view(np.uint64), orc["x"].view(np.uint64))
+    assert np.array_equal(res.W[sel].view(np.uint64), orc["W"].view(np.uint64))
+    assert np.array_equal(res.W2[sel].view(np.uint64), orc["W2"].view(np.uint64))
+    assert np.array_equal(res.states[sel], orc["states"])
+    assert np.array_equal(res.code[sel], orc["code"]) and np.array_equal(res.hstep[sel], orc["hstep"])
+
+
 def test_split_passes_with_failures_equal_fused_and_oracle():
     """A two-frame chain runs as two single-frame passes; orbits that fail in either pass are rerun by the fused
     kernel. With failures planted in one frame at a time (a zero column on either frame, overflow on either frame,
