@@ -1354,9 +1354,24 @@ struct Stretch {
     const int64_t c = (int64_t)cl_floor((z - lo) / ((hi - lo) / (double)bins)) + 1;
     return c < (int64_t)bins ? c : (int64_t)bins;
   }
+  // The counts of the launch so far, staged in 8-bit counters (entry e at cnt[e * cs]; null: counted in ws directly),
+  // added to the workspace and cleared. Staged counts stay below 256: a flush follows every 255th handoff.
+  static constexpr int64_t STAGE_HANDOFFS = 255;
+  CL_HD void flush(uint8_t* cnt, int cs, uint32_t* ws, int64_t stride) const {
+    if (!cnt) return;
+    const int64_t e = 2 * ((int64_t)bins + 2);
+    for (int64_t j = 0; j < e; ++j) {
+      const uint32_t v = cnt[j * cs];
+      if (v) {
+        ws[j * stride] += v;
+        cnt[j * cs] = 0;
+      }
+    }
+  }
   // ws: the orbit's first workspace entry, entries `stride` apart
   template <int K>
-  CL_HD void step_logs(State& st, int64_t n, Frame<D, K>& W, ColLogs& c, uint32_t* ws, int64_t stride) const {
+  CL_HD void step_logs(State& st, int64_t n, Frame<D, K>& W, ColLogs& c, uint32_t* ws, int64_t stride,
+                       uint8_t* cnt = nullptr, int cs = 0) const {
     if (st.code != 0) return;
     if (!(c.ok1 && c.ok2)) {
       st.column = c.ok1 ? 2 : 1;
@@ -1374,8 +1389,15 @@ struct Stretch {
       return;
     }
     const double tau = (double)(n / K1);
-    ws[cell(c.l1 / tau) * stride] += 1u;
-    ws[((int64_t)bins + 2 + cell(c.l2 / tau)) * stride] += 1u;
+    const int64_t e1 = cell(c.l1 / tau), e2 = (int64_t)bins + 2 + cell(c.l2 / tau);
+    if (cnt) {
+      cnt[e1 * cs] += 1;
+      cnt[e2 * cs] += 1;
+      if (K1 % STAGE_HANDOFFS == 0) flush(cnt, cs, ws, stride);
+    } else {
+      ws[e1 * stride] += 1u;
+      ws[e2 * stride] += 1u;
+    }
     st.s1 = csum_add(st.s1, c.l1);
     st.s2 = csum_add(st.s2, c.l2);
     st.q1 = csum_add(st.q1, c.l1 * c.l1);

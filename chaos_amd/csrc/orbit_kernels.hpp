@@ -8,6 +8,7 @@
 // (orbit.jl:23-32,56-57), each reducer seeing the frame as replaced by the reducers before it.
 #pragma once
 #include <stdint.h>
+#include <type_traits>
 #include "cl_core.hpp"
 #include "cl_gauss.hpp"
 #include "../../include/chaoslab_hip.h"
@@ -38,6 +39,8 @@ struct KArgs {
   int reset_frame;
   // lanes per group of the launch (64, 128 or 256), 0 for the size rule
   int block;
+  // StretchingNumbers: stage the counts in LDS (dynamic shared memory, 2 (bins + 2) bytes per lane)
+  int ws_lds;
 };
 
 // Per-orbit context the reducers read: the point, the carried sin/cos, the frames and a cache of
@@ -52,6 +55,8 @@ struct OrbitCtx {
   ColLogs logs[NF];
   uint32_t* ws;  // the orbit's first workspace entry (entries m apart)
   int64_t wstride;
+  uint8_t* cnt;  // this lane's staged StretchingNumbers counts in LDS (entries cstride apart), or null
+  int cstride;
   double* mem;   // the orbit's first WeightedBirkhoff horizon word in the state array (words m apart)
   const double* wtab;
 };
@@ -147,7 +152,7 @@ CL_HD void slot_step(Slot<Monitor<D, K>, FR>& s, int64_t n, Ctx& c) {
 template <int D, int FR, class Ctx>
 CL_HD void slot_step(Slot<Stretch<D>, FR>& s, int64_t n, Ctx& c) {
   ensure_logs<D, 2, 0>(c, FR);
-  s.r.step_logs(s.st, n, c.W[FR], c.logs[FR], c.ws, c.wstride);
+  s.r.step_logs(s.st, n, c.W[FR], c.logs[FR], c.ws, c.wstride, c.cnt, c.cstride);
   if (s.r.replace) c.logs_valid[FR] = false;
 }
 
@@ -180,6 +185,33 @@ CL_HD void chain_ws_init(const Cons<H, T>& ch, Ctx& c) {
   slot_ws_init(ch.head, c);
   chain_ws_init(ch.tail, c);
 }
+
+// The staged StretchingNumbers counts of a launch, added to the workspace at its end.
+template <class R, int FR, class Ctx>
+CL_HD void slot_ws_flush(const Slot<R, FR>&, Ctx&) {}
+template <int D, int FR, class Ctx>
+CL_HD void slot_ws_flush(const Slot<Stretch<D>, FR>& s, Ctx& c) {
+  s.r.flush(c.cnt, c.cstride, c.ws, c.wstride);
+}
+template <class Ctx>
+CL_HD void chain_ws_flush(const Nil&, Ctx&) {}
+template <class H, class T, class Ctx>
+CL_HD void chain_ws_flush(const Cons<H, T>& ch, Ctx& c) {
+  slot_ws_flush(ch.head, c);
+  chain_ws_flush(ch.tail, c);
+}
+// Whether a chain holds a StretchingNumbers slot, and its number of bins (the staging size).
+template <class Chain>
+struct HasStretch : std::false_type {};
+template <int D, int FR, class T>
+struct HasStretch<Cons<Slot<Stretch<D>, FR>, T>> : std::true_type {};
+template <class H, class T>
+struct HasStretch<Cons<H, T>> : HasStretch<T> {};
+template <int D, int FR, class T>
+CL_HD int stretch_bins(const Cons<Slot<Stretch<D>, FR>, T>& ch) { return ch.head.r.bins; }
+template <class H, class T>
+CL_HD int stretch_bins(const Cons<H, T>& ch) { return stretch_bins(ch.tail); }
+CL_HD inline int stretch_bins(const Nil&) { return 0; }
 
 template <int D, int FR, class Ctx>
 CL_HD void slot_step(Slot<RunMon<D>, FR>& s, int64_t n, Ctx& c) {
@@ -459,6 +491,15 @@ orbit_kernel(KArgs a, Sys sys, Chain chain0) {
   Chain ch = chain0;
   c.ws = a.ws ? a.ws + i : nullptr;
   c.wstride = a.m;
+  c.cnt = nullptr;
+  c.cstride = (int)blockDim.x;
+  if constexpr (HasStretch<Chain>::value) {
+    if (a.ws_lds && c.ws) {
+      c.cnt = reinterpret_cast<uint8_t*>(park) + threadIdx.x;
+      const int e = 2 * (stretch_bins(chain0) + 2);
+      for (int j = 0; j < e; ++j) c.cnt[j * c.cstride] = 0;
+    }
+  }
   c.mem = (double*)(a.st + (int64_t)WBOffset<Chain>::value * a.m + i);
   c.wtab = a.wtab;
   if (a.n0 == 0) {
@@ -531,6 +572,7 @@ orbit_kernel(KArgs a, Sys sys, Chain chain0) {
   } else {
     if (!early) code = (n == a.N) ? frame_status(c) : STATUS_OK;
   }
+  if constexpr (HasStretch<Chain>::value) chain_ws_flush(ch, c);
 #pragma unroll
   for (int j = 0; j < Sys::S; ++j) a.x[(int64_t)j * a.m + i] = c.x[j];
 #pragma unroll

@@ -78,8 +78,18 @@ int launch_impl(const cl_system* s, const cl_chain* d, const KArgs& a, hipStream
   int block = a.block ? a.block : block_for(a.m);
   int64_t grid = (a.m + block - 1) / block;
   // the frames' LDS slots of systems that park them during the step (OrbitPlan), sized by the group
-  const size_t lds = OrbitPlan<Sys>::park_frames && !F32T ? (size_t)NF * Sys::D * K * block * sizeof(double) : 0;
-  hipLaunchKernelGGL((orbit_kernel<Sys, K, NF, Chain, F32T>), dim3((unsigned)grid), dim3(block), lds, st, a, sys, ch);
+  size_t lds = OrbitPlan<Sys>::park_frames && !F32T ? (size_t)NF * Sys::D * K * block * sizeof(double) : 0;
+  KArgs b = a;
+  b.ws_lds = 0;
+  if constexpr (HasStretch<Chain>::value) {
+    // StretchingNumbers' counts staged in LDS, 2 (bins + 2) bytes per lane, when they fit in 64 KiB per group
+    const size_t need = (size_t)2 * ((size_t)stretch_bins(ch) + 2) * (size_t)block;
+    if (a.ws && need <= 65536 && lds == 0) {
+      lds = need;
+      b.ws_lds = 1;
+    }
+  }
+  hipLaunchKernelGGL((orbit_kernel<Sys, K, NF, Chain, F32T>), dim3((unsigned)grid), dim3(block), lds, st, b, sys, ch);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (e == hipErrorOutOfMemory ? CL_ERR_OOM : CL_ERR_DEVICE_LOST);
 }

@@ -121,6 +121,32 @@ def test_refuses_workspace_chain_without_workspace():
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("block,bins", [(128, 100), (256, 200), (64, 500)])
+def test_staged_and_direct_counts(block, bins):
+    """The counts are staged in 8-bit LDS counters (flushed every 255 handoffs and at the launch's end) when
+    2 (bins + 2) bytes per lane fit in 64 KiB per group, and go to the workspace directly otherwise: 128 lanes x 100
+    cells and 64 x 500 stage, 256 x 200 does not. Both equal the oracle bit for bit over 600 handoffs, a launch
+    split included."""
+    from chaos_amd.autotune import set_tuning
+    m, N = 4096, 600
+    sysm = StandardMap(2.5)
+    x0 = ensemble(m, False, 17)
+    W0 = O.frames(5, 1, m, 2, 2)
+    sr = StretchingNumbers(range=stretching_range(sysm), bins=bins, interval=1)
+    chain, _ = make_chain((sr,))
+    orc = O.run_ws(sysm.descriptor(), chain, sr.workspace(2, 2), x0, W0, N, 1, 2)
+    set_tuning(block=block)
+    try:
+        for L_ in (None, 260):
+            say("stretching staging", "block", block, "bins", bins, "launch", L_)
+            res = propagate(sysm, x0, N, k=2, tau=1, seed=5, reducers=(sr,), W0=W0, launch_steps=L_, check=False,
+                            keep_states=True)
+            assert np.array_equal(res.workspace, orc["ws"]) and np.array_equal(res.states, orc["states"])
+            assert np.array_equal(res.code, orc["code"]) and np.array_equal(res.hstep, orc["hstep"])
+    finally:
+        set_tuning()
+
+
 def test_full_size_sample():
     """2^20 orbits of the wrapped map at K = 6 (the study's wrapped mode), 1000 steps, tau = 1, 100 cells: a sample
     of 2048 orbits equals the oracle bit for bit; every column's counts total the handoffs; the spectrum's mean is
