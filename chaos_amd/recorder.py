@@ -28,6 +28,15 @@ VARIATIONAL = ("L1",) + tuple(QR_METHODS)   # the methods with a reducer chain o
 METHODS = ("L1", "L2") + tuple(QR_METHODS)
 
 
+def _to_host(t):
+    """A device tensor's values in host memory, copied through a pinned buffer: one DMA transfer rather than the
+    runtime's staging kernels for pageable memory (the row buffers reach hundreds of MB at 65536 orbits)."""
+    import torch
+    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    h.copy_(t)
+    return h.numpy()
+
+
 def estimator(method, orthogonality_tol=8.0):
     """The estimator of a method code (_estimator, run_record.jl:49-53)."""
     if method == "L1":
@@ -220,7 +229,7 @@ def record(sys, x0, N, *, method, tau=1, grid=None, W0=None, direction="pi/4", o
     if timing is not None:
         timing["device_ms"] = ev0.elapsed_time(ev1)
     raw = dst.cpu().numpy().T.copy().view(np.uint8).reshape(m, words * 8)
-    return Records(method=method, grid=grid, R=dR.cpu().numpy(), fail=dfail.cpu().numpy(), code=dcode.cpu().numpy(),
+    return Records(method=method, grid=grid, R=_to_host(dR), fail=dfail.cpu().numpy(), code=dcode.cpu().numpy(),
                    hstep=dh.cpu().numpy(), x=dx.cpu().numpy().T.copy(),
                    W=dW.cpu().numpy().reshape(k, 2, m).transpose(2, 1, 0).copy(), states=raw, k=k, s=s)
 
@@ -268,7 +277,7 @@ def _record_shadow(sys, x0, N, tau, grid, u0, d0, launch_steps, timing):
         timing["device_ms"] = ev0.elapsed_time(ev1)
     raw = de.cpu().numpy().T.copy().view(np.uint64)
     e = decode_ends(raw, s)
-    return Records(method="L2", grid=grid, R=dR.cpu().numpy(), fail=dfail.cpu().numpy(), code=e["code"],
+    return Records(method="L2", grid=grid, R=_to_host(dR), fail=dfail.cpu().numpy(), code=e["code"],
                    hstep=e["step"], x=e["x"], W=e["y"], states=raw, k=1, s=s)
 
 
@@ -452,6 +461,6 @@ def record_indicators(sys, x0, N, *, members=("FLI", "MEGNO", "SALI"), reader=No
         timing["device_ms"] = ev0.elapsed_time(ev1)
     raw = dst.cpu().numpy().T.copy().view(np.uint8).reshape(m, words * 8)
     return IndRecords(kind=window_kind if window else ("pair" if pair else "tangent"), columns=columns, grid=grid,
-                      R=dR.cpu().numpy(), fail=dfail.cpu().numpy(), code=dcode.cpu().numpy(), hstep=dh.cpu().numpy(),
+                      R=_to_host(dR), fail=dfail.cpu().numpy(), code=dcode.cpu().numpy(), hstep=dh.cpu().numpy(),
                       x=dx.cpu().numpy().T.copy(), W=dW.cpu().numpy().reshape(k, d, m).transpose(2, 1, 0).copy(),
                       states=raw, reducers=allr, s=ns)
