@@ -566,11 +566,16 @@ def c3_main(a):
         # the five ensembles run concurrently, one stream each (default priority); "serial": one after another
         S["stream"] = shared if shared is not None else torch.cuda.Stream()
 
-    def run(S, Ns):
+    def setup(S):  # the starts and the seeded frames (part of the timed work)
         sh = S["stream"].cuda_stream
         with torch.cuda.stream(S["stream"]):
             S["dx"].copy_(S["xm"])
         assert lib.cl_initial_deviations_device(ctx, 3, 1, m, 2 * S["N"], k, S["dW"].data_ptr(), sh) == 0
+
+    def run(S, Ns, with_setup=True):
+        sh = S["stream"].cuda_stream
+        if with_setup:
+            setup(S)
         rc = lib.cl_propagate_ragged(ctx, ctypes.byref(S["sd"]), ctypes.byref(chain), 1, m, 2 * S["N"], 2 * S["N"], k,
                                      Ns.ctypes.data, S["order"].ctypes.data, tau, S["dx"].data_ptr(),
                                      S["dW"].data_ptr(), S["dst"].data_ptr(), S["dc"].data_ptr(), S["dh"].data_ptr(),
@@ -590,12 +595,19 @@ def c3_main(a):
         ev0.record()
         for S in sets:
             S["stream"].wait_event(ev0)
+        # every lattice's starts and frames first, so that no frame kernel waits behind another lattice's chain
+        # kernel for its compute units
+        evs = {}
         for S in sets:
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0 = torch.cuda.Event(enable_timing=True)
             e0.record(S["stream"])
-            run(S, S["Ns"])
+            setup(S)
+            evs[S["N"]] = e0
+        for S in (sets[::-1] if a.c3_order == "descending" else sets):
+            e1 = torch.cuda.Event(enable_timing=True)
+            run(S, S["Ns"], with_setup=False)
             e1.record(S["stream"])
-            per_n.setdefault(S["N"], []).append((ev0, e0, e1))
+            per_n.setdefault(S["N"], []).append((ev0, evs[S["N"]], e1))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = {N: float(np.mean([x.elapsed_time(z) for _, x, z in v])) for N, v in per_n.items()}
@@ -1054,6 +1066,8 @@ def main():
                     help="c3: one launch per lattice on five streams, one persistent launch over all five, or the five "
                          "launches one after another on one stream (each lattice's kernel alone on the device)")
     ap.add_argument("--c3-chunk", type=int, default=0, help="c3 mixed: steps per graph-captured chunk (0: one chunk)")
+    ap.add_argument("--c3-order", default="ascending", choices=["ascending", "descending"],
+                    help="c3 streams: launch the lattices' chain kernels in ascending or descending N")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 1 GiB device-to-device (the known-size kernel that calibrates FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args()

@@ -1332,11 +1332,46 @@ static void frames_host(uint64_t seed, int64_t orbit0, int64_t m, double* W) {
 
 #define FRAME_SIZES(X) X(2, 1) X(2, 2) X(4, 1) X(4, 2) X(4, 3) X(4, 4)
 
-// Frames of the chain sizes (d > 4): one thread per orbit, built in place in the orbit-major output.
-__global__ void k_frames_dyn(uint64_t seed, int64_t orbit0, int64_t m, int d, int k, double* __restrict__ W) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Frames of the chain sizes (d > 4): one 64-lane group per orbit, the d x k frame in LDS (d k doubles). The lanes
+// draw the entries and apply each projection and scaling over the rows; the dot products and the norm's sum run in
+// row order in every lane (the same serial sums and maximum as seeded_frame_dyn, so the bits are the host's).
+__global__ void __launch_bounds__(64) k_frames_dyn(uint64_t seed, int64_t orbit0, int64_t m, int d, int k,
+                                                   double* __restrict__ W) {
+  extern __shared__ double G[];
+  const int64_t i = blockIdx.x;
   if (i >= m) return;
-  seeded_frame_dyn(seed, orbit0 + i, d, k, W + i * (int64_t)d * k);
+  const int lane = (int)threadIdx.x;
+  const uint64_t key = orbit_key(seed, orbit0 + i);
+  for (int t = lane; t < d * k; t += 64) G[t] = normal_draw(key, (uint64_t)t);
+  __syncthreads();
+  for (int j = 0; j < k; ++j) {
+    double* g = G + j * d;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int q = 0; q < j; ++q) {
+        const double* gq = G + q * d;
+        double c = gq[0] * g[0];
+        for (int l = 1; l < d; ++l) c = c + gq[l] * g[l];
+        __syncthreads();  // every lane has read g before it changes
+        for (int l = lane; l < d; l += 64) g[l] = g[l] - c * gq[l];
+        __syncthreads();
+      }
+    double mx = cl_fabs(g[0]);
+    for (int l = 1; l < d; ++l) mx = cl_jmax(mx, cl_fabs(g[l]));
+    double nr = mx;
+    if (cl_isfinite(mx) && mx > 0.0) {
+      double s = (g[0] / mx) * (g[0] / mx);
+      for (int l = 1; l < d; ++l) {
+        const double b = g[l] / mx;
+        s = s + b * b;
+      }
+      nr = mx * cl_sqrt(s);
+    }
+    __syncthreads();
+    for (int l = lane; l < d; l += 64) g[l] = g[l] / nr;
+    __syncthreads();
+  }
+  double* out = W + i * (int64_t)d * k;
+  for (int t = lane; t < d * k; t += 64) out[t] = G[t];
 }
 
 static bool dyn_frame_ok(int d, int k) { return d > 4 && d <= 256 && k >= 1 && k <= d && k <= 16; }
@@ -1368,7 +1403,8 @@ extern "C" int cl_initial_deviations_device(cl_ctx* ctx, uint64_t seed, int64_t 
   FRAME_SIZES(X)
 #undef X
   if (dyn_frame_ok(d, k)) {
-    hipLaunchKernelGGL(k_frames_dyn, dim3(grid_for(m, 64)), dim3(64), 0, st, seed, orbit0, m, d, k, dW);
+    hipLaunchKernelGGL(k_frames_dyn, dim3((unsigned)m), dim3(64), (size_t)d * k * sizeof(double), st, seed, orbit0, m, d,
+                       k, dW);
     return map_err(hipGetLastError());
   }
   return set_err(ctx, CL_ERR_UNSUPPORTED, "no frame kernel for this d, k");

@@ -87,6 +87,21 @@ def test_chain_origin_spectrum():
     assert np.max(np.abs(g.finals[0]["lambda"] - want)) < 10.0 / 4000
 
 
+@pytest.mark.parametrize("d,k", [(6, 3), (16, 16), (48, 5), (128, 16), (256, 16)])
+def test_device_frames_of_chain_sizes_equal_host(d, k):
+    """The frames of d > 4 (one 64-lane group per orbit, the frame in LDS) equal the host rule bit for bit."""
+    import torch
+    m = 37
+    say("chain frames", d, k)
+    lib, ctx = _context()
+    dW = torch.empty((m, k, d), dtype=torch.float64, device="cuda")
+    assert lib.cl_initial_deviations_device(ctx, 11, 5, m, d, k, dW.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    hW = np.empty((m, k, d))
+    assert lib.cl_initial_deviations(11, 5, m, d, k, hW.ctypes.data) == 0
+    assert np.array_equal(dW.cpu().numpy(), hW)
+
+
 def test_chain_host_chunk_and_frames():
     import torch
     N, m = 64, 4
