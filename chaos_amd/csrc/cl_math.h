@@ -102,13 +102,15 @@ CL_HD uint64_t cl_mulhi64(uint64_t a, uint64_t b) {
 }
 
 /* Julia's min/max on floats: NaN wins, -0.0 < +0.0. */
+/* (the conditions are combined with | rather than ||, so that no branch splits a reducer's straight-line code;
+ * x != x is x's NaN test) */
 CL_HD double cl_jmin(double x, double y) {
-  int takey = (y < x) || (cl_signbit(y) > cl_signbit(x));
-  return takey ? (cl_isnan(x) ? x : y) : (cl_isnan(y) ? y : x);
+  const int takey = (y < x) | (cl_signbit(y) > cl_signbit(x));
+  return takey ? (x != x ? x : y) : (y != y ? y : x);
 }
 CL_HD double cl_jmax(double x, double y) {
-  int takey = (y > x) || (cl_signbit(y) < cl_signbit(x));
-  return takey ? (cl_isnan(x) ? x : y) : (cl_isnan(y) ? y : x);
+  const int takey = (y > x) | (cl_signbit(y) < cl_signbit(x));
+  return takey ? (x != x ? x : y) : (y != y ? y : x);
 }
 
 /* ---------------------------------------------------------------- error-free transforms */
