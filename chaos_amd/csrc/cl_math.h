@@ -237,14 +237,8 @@ CL_HD int cl_rem_pio2_large(double x, double *rh, double *rl) {
 }
 
 /* x = q pi/2 + (rh + rl) for finite x; returns q mod 4. */
-CL_HD int cl_rem_pio2(double x, double *rh, double *rl) {
-  double ax = cl_fabs(x);
-  if (ax <= 0x1.921fb54442d18p-1) { /* pi/4 */
-    *rh = x;
-    *rl = 0.0;
-    return 0;
-  }
-  if (ax >= 0x1p50) return cl_rem_pio2_large(x, rh, rl);
+/* Cody and Waite's reduction for pi/4 < |x| < 2^50 */
+CL_HD int cl_rem_pio2_mid(double x, double *rh, double *rl) {
   double qd = cl_rint(x * CL_INV_PIO2);
   double p1, e1;
   cl_two_prod(qd, CL_PIO2_1, &p1, &e1);
@@ -260,6 +254,16 @@ CL_HD int cl_rem_pio2(double x, double *rh, double *rl) {
   cl_two_sum(s2, lo, rh, rl);
   int64_t qi = (int64_t)qd;
   return (int)(qi & 3);
+}
+CL_HD int cl_rem_pio2(double x, double *rh, double *rl) {
+  double ax = cl_fabs(x);
+  if (ax <= 0x1.921fb54442d18p-1) { /* pi/4 */
+    *rh = x;
+    *rl = 0.0;
+    return 0;
+  }
+  if (ax >= 0x1p50) return cl_rem_pio2_large(x, rh, rl);
+  return cl_rem_pio2_mid(x, rh, rl);
 }
 
 /* ---------------------------------------------------------------- sin / cos kernels, |r| <= 1 */
@@ -284,13 +288,20 @@ CL_HD double cl_cos_kernel(double rh, double rl) {
 
 /* sin and cos of one argument (one reduction). Non-finite x gives x - x (NaN). */
 CL_HD void cl_sincos(double x, double *sn, double *cs) {
-  if (!cl_isfinite(x)) {
-    *sn = x - x;
-    *cs = x - x;
-    return;
-  }
+  /* one test for the usual case pi/4 < |x| < 2^50; then non-finite, small and large arguments */
   double rh, rl;
-  int q = cl_rem_pio2(x, &rh, &rl);
+  int q;
+  const double ax = cl_fabs(x);
+  if (ax > 0x1.921fb54442d18p-1 && ax < 0x1p50) {
+    q = cl_rem_pio2_mid(x, &rh, &rl);
+  } else {
+    if (!cl_isfinite(x)) {
+      *sn = x - x;
+      *cs = x - x;
+      return;
+    }
+    q = cl_rem_pio2(x, &rh, &rl);
+  }
   double s = cl_sin_kernel(rh, rl), c = cl_cos_kernel(rh, rl);
   switch (q) {
     case 0: *sn = s; *cs = c; break;
@@ -336,16 +347,20 @@ CL_HD double cl_log_normal(double x, int k0) {
 }
 
 CL_HD double cl_log(double x) {
+  /* one test for the usual case, a positive finite x (a subnormal one scaled by 2^54 first); then the special
+   * values: NaN gives x, x < 0 NaN, +-0 -inf, +inf +inf */
+  if (x > 0.0 && x <= 0x1.fffffffffffffp+1023) {
+    int k0 = 0;
+    if (x < 0x1p-1022) { /* subnormal */
+      x = x * 0x1p54;
+      k0 = -54;
+    }
+    return cl_log_normal(x, k0);
+  }
   if (cl_isnan(x)) return x;
   if (x < 0.0) return __builtin_nan("");
   if (x == 0.0) return -__builtin_inf();
-  if (!cl_isfinite(x)) return x;
-  int k0 = 0;
-  if (x < 0x1p-1022) { /* subnormal */
-    x = x * 0x1p54;
-    k0 = -54;
-  }
-  return cl_log_normal(x, k0);
+  return x;
 }
 
 /* x is positive, normal and finite (cl_log_normal's domain) */
