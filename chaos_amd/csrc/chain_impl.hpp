@@ -482,28 +482,44 @@ __device__ __attribute__((always_inline)) inline void qr_handoff(const Orb<NS>& 
     qs.threshold = cl_isfinite(pv) ? 0.0 : __builtin_inf();
     qs.step = n;
   } else {
-    // orth_loss: largest |Q'Q - I| over the entries a <= b (the matrix is symmetric entry for entry)
+    // orth_loss: largest |Q'Q - I| over the entries l <= b (the matrix is symmetric entry for entry). Each entry
+    // is its own butterfly sum, so only the upper triangle is formed, two columns to a call (b and KC - 2 - b; the
+    // last column and, for even KC, the middle one alone): KC entries per call, k(k+1)/2 sums in all.
     double orth = 0.0, onan = 0.0;
-    bool seen_nan = false;
-#pragma unroll 1
-    for (int b = 0; b < KC; ++b) {
-      const double qb = Wq[b], pb = Wp[b];
+    int nan_at = KC * KC;  // b KC + l of the first NaN in column order, the one Julia's max returns
+    constexpr int NG = KC / 2 + 1;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      const int b1 = g < NG - 1 ? g : KC - 1;
+      const int b2 = (g < NG - 1 && KC - 2 - g > g) ? KC - 2 - g : -1;
+      const int ne = b1 + 1 + (b2 >= 0 ? b2 + 1 : 0);
       double v[KC];
+      {
+        const double qb = Wq[b1], pb = Wp[b1];
 #pragma unroll
-      for (int l = 0; l < KC; ++l) v[l] = cl_fma(Wp[l], pb, Wq[l] * qb);
-      o.template sum<KC>(v);  // all k entries: a run-time b keeps this loop small; entries l > b are unused
+        for (int l = 0; l < KC; ++l)
+          if (l <= b1) v[l] = cl_fma(Wp[l], pb, Wq[l] * qb);
+      }
+      if (b2 >= 0) {
+        const double qb = Wq[b2], pb = Wp[b2];
 #pragma unroll
-      for (int l = 0; l < KC; ++l)
-        if (l <= b) {
-          const double er = cl_fabs(l == b ? v[l] - 1.0 : v[l] - 0.0);
-          if (!seen_nan && cl_isnan(er)) {
-            seen_nan = true;
+        for (int l = 0; l < KC; ++l)
+          if (l <= b2) v[b1 + 1 + l] = cl_fma(Wp[l], pb, Wq[l] * qb);
+      }
+      o.template sum<KC>(v, 0, ne - 1);
+#pragma unroll
+      for (int e = 0; e < KC; ++e)
+        if (e < ne) {
+          const int b = e <= b1 ? b1 : b2, l = e <= b1 ? e : e - b1 - 1;
+          const double er = cl_fabs(l == b ? v[e] - 1.0 : v[e] - 0.0);
+          if (cl_isnan(er) && b * KC + l < nan_at) {
+            nan_at = b * KC + l;
             onan = er;
           }
-          orth = (b == 0 && l == 0) ? er : fmax(orth, er);
+          orth = fmax(orth, er);  // er >= +0, so starting from +0 equals starting from the first entry
         }
     }
-    if (seen_nan) orth = onan;  // Julia's max: NaN wins
+    if (nan_at < KC * KC) orth = onan;  // Julia's max: NaN wins
     const double thr = a.tol * CL_EPS64 * (double)KC * 1.0;
     if (!(orth <= thr)) {
       qs.code = 4;
