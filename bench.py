@@ -484,7 +484,8 @@ def c3_mixed(a, lib, ctx, chain, sets, m, k, tau, desc):
                    "orbit_steps_per_step": total, "nonfinite_orbits": bad,
                    "pairing_n8": {"max": float(pair.max()), "max_times_n": float(np.max(pair * nsteps)),
                                   "sum_max": float(np.max(np.abs(lam8.sum(axis=1))))}},
-        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64-valu" if a.c3_qr == "sites" else "fp64-valu+mfma", "achieved": achieved,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel_ms": span},
         "cpu_baseline": None,
     }
@@ -542,10 +543,14 @@ def c3_main(a):
     from chaos_amd import StandardMapChain, QRSpectrum, HouseholderQR, _lib as L
     from chaos_amd.inputs import chain_starts, chain_step_counts
     from chaos_amd.tangent import make_chain, _context
+    from chaos_amd import autotune
     lib, ctx = _context()
+    autotune.set_tuning(chain_qr={"auto": 0, "blocked": 1, "sites": 2}[a.c3_qr])
     m, k, tau = 2048, 16, 10
+    order = ("blocked HouseholderQR on the FP64 matrix cores (two panels of 8, compact WY trailing update)"
+             if a.c3_qr != "sites" else "unblocked HouseholderQR, sums over sites as lane butterflies")
     desc = (f"coupled standard-map chain, ring, K=1.5, eps=0.1, N in {list(C3_NS)} x {m} orbits, steps U[2000,20000] "
-            f"per orbit (multiples of tau={tau}), k={k}, QRSpectrum(HouseholderQR()), FP64")
+            f"per orbit (multiples of tau={tau}), k={k}, QRSpectrum(HouseholderQR()), FP64; {order}")
     log("workload", desc)
     chain, _ = make_chain((QRSpectrum(HouseholderQR()),))
     sets = []
@@ -645,7 +650,8 @@ def c3_main(a):
         "config": {"workload": desc + ("; the five launches in turn on one stream" if a.c3_mode == "serial" else
                                        "; the five launches on five streams"),
                    "orbit_steps_per_step": total, "kernel_ms_per_N": ms, "nonfinite_orbits": bad},
-        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64-valu" if a.c3_qr == "sites" else "fp64-valu+mfma", "achieved": achieved,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel_ms": span,
                      "flops_per_orbit_step": {N: c3_flops(N) for N in C3_NS},
                      "frac_per_N": {N: c3_flops(N) * steps_n[N] / (ms[N] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS
@@ -1062,6 +1068,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="c2: skip the Float32-tangent, accuracy, C1 and C5 blocks of the default run")
+    ap.add_argument("--c3-qr", default="auto", choices=["auto", "blocked", "sites"],
+                    help="c3: the HouseholderQR order (auto: the library's rule, the blocked one on the matrix cores)")
     ap.add_argument("--c3-mode", default="streams", choices=["streams", "mixed", "serial"],
                     help="c3: one launch per lattice on five streams, one persistent launch over all five, or the five "
                          "launches one after another on one stream (each lattice's kernel alone on the device)")

@@ -70,7 +70,7 @@ class cl_lattice(ctypes.Structure):
 
 class cl_tuning(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("split", ctypes.c_int32), ("pipe", ctypes.c_int32),
-                ("pad_", ctypes.c_int32)]
+                ("chain_qr", ctypes.c_int32)]
 
 
 class cl_probe_t(ctypes.Structure):

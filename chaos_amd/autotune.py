@@ -24,10 +24,11 @@ from .tangent import _context, make_chain, ELEMENT_TYPES
 BLOCKS = (64, 128, 256)
 
 
-def set_tuning(block=0, split=0, pipe=0):
-    """Set the context's launch settings (0: the built-in rule; split/pipe 1 on, 2 off)."""
+def set_tuning(block=0, split=0, pipe=0, chain_qr=0):
+    """Set the context's launch settings (0: the built-in rule; split/pipe 1 on, 2 off; chain_qr 1 the coupled chain's
+    blocked HouseholderQR on the matrix cores, 2 its lane-butterfly sweep)."""
     lib, ctx = _context()
-    t = L.cl_tuning(int(block), int(split), int(pipe), 0)
+    t = L.cl_tuning(int(block), int(split), int(pipe), int(chain_qr))
     rc = lib.cl_set_tuning(ctx, ctypes.byref(t))
     if rc:
         raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
@@ -37,7 +38,7 @@ def get_tuning():
     lib, ctx = _context()
     t = L.cl_tuning()
     lib.cl_get_tuning(ctx, ctypes.byref(t))
-    return dict(block=t.block, split=t.split, pipe=t.pipe)
+    return dict(block=t.block, split=t.split, pipe=t.pipe, chain_qr=t.chain_qr)
 
 
 def _candidates(sysd, nf, pipe_ok, taus):
@@ -72,7 +73,7 @@ def tune(sys, x0, *, k, tau, reducers, reducers2=(), trial_steps=None, reps=2, t
         db = json.load(open(cache))
         if key in db:
             best = db[key]["best"]
-            set_tuning(best["block"], best["split"], best["pipe"])
+            set_tuning(best["block"], best["split"], best["pipe"], get_tuning()["chain_qr"])
             return dict(db[key], cached=True)
     words = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), E, d, k) // 8
     dev = torch.device("cuda")
@@ -92,7 +93,7 @@ def tune(sys, x0, *, k, tau, reducers, reducers2=(), trial_steps=None, reps=2, t
     old = get_tuning()
 
     def run(c):
-        set_tuning(c["block"], c["split"], c["pipe"])
+        set_tuning(c["block"], c["split"], c["pipe"], old["chain_qr"])
         dx.copy_(xm)
         assert lib.cl_initial_deviations_device(ctx, seed, 1, m, d, k, W.data_ptr(), stream.cuda_stream) == 0
         if W2 is not None:
@@ -118,10 +119,10 @@ def tune(sys, x0, *, k, tau, reducers, reducers2=(), trial_steps=None, reps=2, t
             ms = min(a.elapsed_time(b) for a, b in evs)
             table.append(dict(c, ns_per_orbit_step=ms * 1e6 / (m * trial_steps)))
     finally:
-        set_tuning(old["block"], old["split"], old["pipe"])
+        set_tuning(old["block"], old["split"], old["pipe"], old["chain_qr"])
     default = table[-1]
     best = min(table[:-1], key=lambda r: r["ns_per_orbit_step"])
-    set_tuning(best["block"], best["split"], best["pipe"])
+    set_tuning(best["block"], best["split"], best["pipe"], old["chain_qr"])
     out = dict(best=best, default=default, gain=default["ns_per_orbit_step"] / best["ns_per_orbit_step"],
                table=table, trial_steps=trial_steps, orbits=m, seconds=round(time.perf_counter() - t0, 2))
     if log:

@@ -37,18 +37,20 @@ struct MixedArgs {
   unsigned* counter;       // device work counter of this launch, zero at its start
   int64_t n0, steps;       // the chunk: global steps n0 .. n0 + steps (each orbit stops at its own count)
 };
-// Orbits per work item of the persistent launch for N sites (its LDS holds this many packed R triangles).
-int mixed_slots(int N);
+// Orbits per work item of the persistent launch for N sites (its LDS holds this many orbits' matrices), for the
+// blocked order or the lane-butterfly sweep.
+int mixed_slots(int N, bool blocked);
 // Zeroes n work counters (a fill kernel, so the launch can be captured in a graph).
 int mixed_zero(unsigned* counters, int n, hipStream_t st);
 // Compute units of the current device (queried once; call before a graph capture).
 int mixed_cus();
-// One persistent launch over the items (ring or open chains, QRSpectrum(HouseholderQR()), k = 16).
-int mixed_launch(const MixedArgs& a, bool ring, hipStream_t st);
+// One persistent launch over the items (ring or open chains, QRSpectrum(HouseholderQR()), k = 16, either order).
+int mixed_launch(const MixedArgs& a, bool ring, bool blocked, hipStream_t st);
 
 // 1 when a kernel is built for N sites and k columns.
 int chain_supported(int N, int k);
-// Launches the kernel for (N, ring, with or without QRSpectrum(HouseholderQR())); 0 or an error code.
-int chain_launch(const ChainArgs& a, int N, bool ring, bool qr, hipStream_t st);
+// Launches the kernel for (N, ring, with or without QRSpectrum(HouseholderQR()), the blocked order on the matrix cores
+// or the lane-butterfly sweep); 0 or an error code.
+int chain_launch(const ChainArgs& a, int N, bool ring, bool qr, bool blocked, hipStream_t st);
 
 }  // namespace cl

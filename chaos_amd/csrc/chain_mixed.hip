@@ -20,24 +20,41 @@ constexpr int MIX_XB = 128 * MIX_KC;
 constexpr int MIX_TRI = Tri<MIX_KC>::SIZE;
 constexpr int MIX_POOL = MIX_US + MIX_XB + 2 * MIX_TRI;
 constexpr int MIX_SLOTS_MAX = (MIX_POOL - MIX_US) / (2 * MIX_TRI);
+// The blocked order (qr_handoff_blk): the frames of the group (32 doubles per thread), then N = 128's exchange buffer and
+// its two 16 x 16 matrices and triangle, or for smaller N those of at most MIX_BLK_SLOTS orbits (three groups per
+// compute unit fit the LDS either way).
+constexpr int MIX_XS = 32 * MIX_BLOCK;
+constexpr int MIX_BLK_SLOT = 2 * MSZ + MIX_TRI;
+constexpr int MIX_BLK_SLOTS = 4;
+constexpr int MIX_POOL_BLK = MIX_XS + (MIX_XB + MIX_BLK_SLOT > MIX_BLK_SLOTS * MIX_BLK_SLOT ? MIX_XB + MIX_BLK_SLOT
+                                                                                           : MIX_BLK_SLOTS * MIX_BLK_SLOT);
 
-template <int NS, bool RING>
+template <int NS, bool RING, bool BLK>
 __device__ __attribute__((always_inline)) inline void mixed_item(const ChainArgs& a, const MixedItem& w, double* pool) {
-  double* Us = pool;
-  if constexpr (NS > 64) {
-    chain_orbits<NS, MIX_KC, RING, true, MIX_BLOCK>(a, w.g0, w.nslots, pool + MIX_US, Us, pool + MIX_US + MIX_XB,
-                                                     pool + MIX_US + MIX_XB + MIX_TRI);
+  if constexpr (BLK) {
+    constexpr int SL = NS > 64 ? 1 : (MIX_BLOCK / NS < MIX_BLK_SLOTS ? MIX_BLOCK / NS : MIX_BLK_SLOTS);
+    double* xb = NS > 64 ? pool + MIX_XS : nullptr;
+    double* Gs = pool + MIX_XS + (NS > 64 ? MIX_XB : 0);
+    double* Ts = Gs + SL * MSZ;
+    double* Rs = Ts + SL * MSZ;
+    chain_orbits<NS, MIX_KC, RING, true, MIX_BLOCK, true, SL>(a, w.g0, w.nslots, xb, pool, Rs, nullptr, Gs, Ts);
   } else {
-    double* Rs = pool + MIX_US;
-    chain_orbits<NS, MIX_KC, RING, true, MIX_BLOCK>(a, w.g0, w.nslots, nullptr, Us, Rs, Rs + MIX_SLOTS_MAX * MIX_TRI);
+    double* Us = pool;
+    if constexpr (NS > 64) {
+      chain_orbits<NS, MIX_KC, RING, true, MIX_BLOCK>(a, w.g0, w.nslots, pool + MIX_US, Us, pool + MIX_US + MIX_XB,
+                                                       pool + MIX_US + MIX_XB + MIX_TRI);
+    } else {
+      double* Rs = pool + MIX_US;
+      chain_orbits<NS, MIX_KC, RING, true, MIX_BLOCK>(a, w.g0, w.nslots, nullptr, Us, Rs, Rs + MIX_SLOTS_MAX * MIX_TRI);
+    }
   }
 }
 
 // Two waves per SIMD (256 registers, about 1.3 KB spilled per lane): on the C3 workload (the five lattices, 2048
 // orbits each, uneven lengths) one launch takes 995 ms this way and 1095 ms with the whole register file per wave.
-template <bool RING>
+template <bool RING, bool BLK>
 __global__ void __launch_bounds__(MIX_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) chain_mixed_kernel(MixedArgs m) {
-  __shared__ double pool[MIX_POOL];
+  __shared__ double pool[BLK ? MIX_POOL_BLK : MIX_POOL];
   __shared__ unsigned item_s;
   for (;;) {
     __syncthreads();  // the previous item's LDS use is over
@@ -51,11 +68,11 @@ __global__ void __launch_bounds__(MIX_BLOCK) __attribute__((amdgpu_waves_per_eu(
     a.n0 = m.n0;
     a.steps = m.steps;
     switch (c.N) {
-      case 8: mixed_item<8, RING>(a, w, pool); break;
-      case 16: mixed_item<16, RING>(a, w, pool); break;
-      case 32: mixed_item<32, RING>(a, w, pool); break;
-      case 64: mixed_item<64, RING>(a, w, pool); break;
-      case 128: mixed_item<128, RING>(a, w, pool); break;
+      case 8: mixed_item<8, RING, BLK>(a, w, pool); break;
+      case 16: mixed_item<16, RING, BLK>(a, w, pool); break;
+      case 32: mixed_item<32, RING, BLK>(a, w, pool); break;
+      case 64: mixed_item<64, RING, BLK>(a, w, pool); break;
+      case 128: mixed_item<128, RING, BLK>(a, w, pool); break;
       default: break;
     }
   }
@@ -68,11 +85,11 @@ __global__ void k_zero_counters(unsigned* c, int n) {
 
 }  // namespace chain_detail
 
-int mixed_slots(int N) {
+int mixed_slots(int N, bool blocked) {
   using namespace chain_detail;
   if (N > 64) return 1;
-  const int s = MIX_BLOCK / N;
-  return s < MIX_SLOTS_MAX ? s : MIX_SLOTS_MAX;
+  const int s = MIX_BLOCK / N, cap = blocked ? MIX_BLK_SLOTS : MIX_SLOTS_MAX;
+  return s < cap ? s : cap;
 }
 
 int mixed_zero(unsigned* counters, int n, hipStream_t st) {
@@ -92,16 +109,18 @@ int mixed_cus() {
   return cus;
 }
 
-int mixed_launch(const MixedArgs& a, bool ring, hipStream_t st) {
+int mixed_launch(const MixedArgs& a, bool ring, bool blocked, hipStream_t st) {
   using namespace chain_detail;
   if (a.nitems <= 0) return 0;
   // three groups per compute unit (the LDS pool allows three), never more groups than items
   int64_t grid = (int64_t)mixed_cus() * 3;
   if (grid > a.nitems) grid = a.nitems;
   if (ring)
-    hipLaunchKernelGGL(chain_mixed_kernel<true>, dim3((unsigned)grid), dim3(MIX_BLOCK), 0, st, a);
+    hipLaunchKernelGGL((blocked ? chain_mixed_kernel<true, true> : chain_mixed_kernel<true, false>), dim3((unsigned)grid),
+                       dim3(MIX_BLOCK), 0, st, a);
   else
-    hipLaunchKernelGGL(chain_mixed_kernel<false>, dim3((unsigned)grid), dim3(MIX_BLOCK), 0, st, a);
+    hipLaunchKernelGGL((blocked ? chain_mixed_kernel<false, true> : chain_mixed_kernel<false, false>),
+                       dim3((unsigned)grid), dim3(MIX_BLOCK), 0, st, a);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (e == hipErrorOutOfMemory ? CL_ERR_OOM : CL_ERR_DEVICE_LOST);
 }

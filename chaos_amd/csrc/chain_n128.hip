@@ -3,8 +3,8 @@
 
 namespace cl {
 
-int launch_chain_n128(const ChainArgs& a, bool ring, bool qr, hipStream_t st) {
-  return chain_detail::launch_n<128>(a, ring, qr, st);
+int launch_chain_n128(const ChainArgs& a, bool ring, bool qr, bool blocked, hipStream_t st) {
+  return chain_detail::launch_n<128>(a, ring, qr, blocked, st);
 }
 
 }  // namespace cl

@@ -424,8 +424,9 @@ extern "C" const char* cl_last_error(cl_ctx* c) { return c ? c->err : g_create_e
 extern "C" int cl_set_tuning(cl_ctx* c, const cl_tuning* t) {
   if (!c || !t) return CL_ERR_ARGUMENT;
   if (!(t->block == 0 || t->block == 64 || t->block == 128 || t->block == 256) || t->split < 0 || t->split > 2 ||
-      t->pipe < 0 || t->pipe > 2)
-    return set_err(c, CL_ERR_ARGUMENT, "tuning: block 0/64/128/256, split and pipe 0 (automatic), 1 (on), 2 (off)");
+      t->pipe < 0 || t->pipe > 2 || t->chain_qr < 0 || t->chain_qr > 2)
+    return set_err(c, CL_ERR_ARGUMENT,
+                   "tuning: block 0/64/128/256, split, pipe and chain_qr 0 (automatic), 1 (on), 2 (off)");
   c->tune = *t;
   return CL_OK;
 }
@@ -772,6 +773,10 @@ extern "C" int cl_ind_row_width(const cl_chain* ch, int s) {
   return cols + 3 + s;
 }
 
+// The order of the chain's HouseholderQR for this context's calls: the blocked factorisation on the matrix cores
+// unless the tuning asks for the lane-butterfly sweep.
+static bool chain_blocked(const cl_ctx* c) { return c->tune.chain_qr != 2; }
+
 static int chain_device(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m, int s, int d,
                         int k, int64_t n0, int64_t steps, int64_t N, int tau, double* dx, double* dW, uint64_t* dst,
                         int8_t* dcode, int64_t* dhstep, hipStream_t st) {
@@ -782,7 +787,7 @@ static int chain_device(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, i
   if (!(N >= 1 && N % tau == 0 && steps >= 0 && steps % tau == 0 && n0 >= 0 && n0 % tau == 0 && n0 + steps <= N))
     return set_err(ctx, CL_ERR_ARGUMENT, "launch steps must be a multiple of tau within N");
   rc = chain_launch(chain_args(sys, ch, m, dx, dW, dst, dcode, dhstep, n0, steps, N, tau), sys->N, sys->ring != 0,
-                    qr, st);
+                    qr, chain_blocked(ctx), st);
   return rc ? set_err(ctx, rc, "chain kernel launch failed") : CL_OK;
 }
 
@@ -894,7 +899,7 @@ extern "C" int cl_propagate_ragged(cl_ctx* ctx, const cl_system* sys, const cl_c
   ChainArgs a = chain_args(sys, ch, m, dx, dW, dst, dcode, dhstep, 0, Nmax, Nmax, tau);
   a.Ns = dbuf;
   a.order = order ? dbuf + m : nullptr;
-  rc = chain_launch(a, sys->N, sys->ring != 0, qr, st);
+  rc = chain_launch(a, sys->N, sys->ring != 0, qr, chain_blocked(ctx), st);
   e = hipFreeAsync(dbuf, st);
   if (rc) return set_err(ctx, rc, "chain kernel launch failed");
   return e == hipSuccess ? CL_OK : set_err(ctx, map_err(e), "stream-ordered free failed");
@@ -950,7 +955,7 @@ static int mixed_plan(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int
     o.resize((size_t)L.m);
     for (int64_t i = 0; i < L.m; ++i) o[(size_t)i] = i;
     std::stable_sort(o.begin(), o.end(), [&](int64_t a, int64_t b) { return L.Ns[a] > L.Ns[b]; });
-    const int sl = mixed_slots(L.N);
+    const int sl = mixed_slots(L.N, chain_blocked(ctx));
     for (int64_t g = 0; g < L.m; g += sl) {
       Item it;
       it.w.cls = c;
@@ -986,7 +991,7 @@ static int mixed_plan(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int
     MixedClass& mc = cls[(size_t)c];
     memset(&mc, 0, sizeof(mc));
     mc.N = L.N;
-    mc.slots = mixed_slots(L.N);
+    mc.slots = mixed_slots(L.N, chain_blocked(ctx));
     mc.a.x = L.dx;
     mc.a.W = L.dW;
     mc.a.st = L.dstates;
@@ -1031,7 +1036,7 @@ static int mixed_run(cl_ctx* ctx, MixedPlan& P, std::vector<MixedClass>& cls, ch
       a.counter = counters + c;
       a.n0 = (int64_t)c * len;
       a.steps = P.Nmax - a.n0 < len ? P.Nmax - a.n0 : len;
-      rc = mixed_launch(a, ring, s);
+      rc = mixed_launch(a, ring, chain_blocked(ctx), s);
     }
     return rc;
   };

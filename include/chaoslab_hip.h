@@ -158,14 +158,17 @@ typedef struct cl_ctx cl_ctx;
 
 /* Launch settings an autotuner picks per context (zeros: the built-in rules): `block` lanes per group of the orbit
  * kernel (64, 128 or 256; the rule is 256 from 65536 orbits on, else 64), `split` the two single-frame passes of a
- * two-frame map chain and `pipe` the wave-specialised pipeline of small QR ensembles (1 on, 2 off). Every setting
- * gives the same bits. No reference counterpart: the reference's launch plan is its measured one-wave throughput
- * record (ext/ChaoslabCUDAExt.jl:42-62, src/Tangent/propagate.jl:144-193). */
+ * two-frame map chain and `pipe` the wave-specialised pipeline of small QR ensembles (1 on, 2 off). These give the
+ * same bits under every setting. `chain_qr` picks the order of the coupled chain's HouseholderQR (k = 16): 1 the
+ * blocked factorisation on the FP64 matrix cores (two panels of 8 reflectors, compact WY trailing update), 2 the
+ * unblocked sweep with the sums over sites as lane butterflies, 0 the built-in rule (blocked); the two orders agree to
+ * rounding, each bit for bit with its restatement in the oracle. No reference counterpart: the reference's launch plan
+ * is its measured one-wave throughput record (ext/ChaoslabCUDAExt.jl:42-62, src/Tangent/propagate.jl:144-193). */
 typedef struct {
   int32_t block;
   int32_t split;
   int32_t pipe;
-  int32_t pad_;
+  int32_t chain_qr;
 } cl_tuning;
 int cl_set_tuning(cl_ctx *ctx, const cl_tuning *t);
 int cl_get_tuning(cl_ctx *ctx, cl_tuning *t);

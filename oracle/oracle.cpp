@@ -43,7 +43,8 @@ enum { MATH_LIBM = 0, MATH_CL = 1 };
 
 // Planted defects for the negative controls of the closed-form tests (tests/test_oracle_pins.py): a test switches one
 // on, shows that the closed form it checks then fails, and switches it off. 0 (the default) is the oracle as written.
-enum { DEFECT_NONE = 0, DEFECT_GALI_RAW = 1, DEFECT_OFLI_NOPROJ = 2, DEFECT_MEGNO_Q2 = 3, DEFECT_GALI_LASTFACTOR = 4 };
+enum { DEFECT_NONE = 0, DEFECT_GALI_RAW = 1, DEFECT_OFLI_NOPROJ = 2, DEFECT_MEGNO_Q2 = 3, DEFECT_GALI_LASTFACTOR = 4,
+       DEFECT_BLK_TNOTRANS = 5, DEFECT_BLK_TSIGN = 6 };
 static int g_defect = DEFECT_NONE;
 
 // ------------------------------------------------------------------ arithmetic policies

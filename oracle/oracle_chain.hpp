@@ -17,9 +17,20 @@
 //                  v[t] <- v[t] + v[t ^ o] for o = 1, 2, 4, ..., N/2 (pairs of sites first), whose result every site
 //                  holds (IEEE addition commutes). In this order rows that hold structural zeros (rows above the reflector's pivot, and
 //                  columns a reflector leaves unchanged) are skipped; the results differ from RED_ROWS by rounding only.
-// The reference's left fold is the restatement; RED_SITES is what the device kernel computes bit for bit.
+//   RED_BLOCKED (2): the blocked Householder factorisation of the device's matrix-core path (two panels of 8 columns):
+//                  each panel is swept in the RED_SITES order over its own columns only, each column's norm from one
+//                  reduction (chain_panel); after the first panel the
+//                  trailing 8 columns take the panel's reflectors at once in compact WY form, A <- A - U (T' (U'A))
+//                  (Schreiber and Van Loan, "A storage-efficient WY representation for products of Householder
+//                  transformations", SIAM J. Sci. Stat. Comput. 10 (1989) 53-57); Q = E - U (T U_top') over all
+//                  reflectors, and orth_loss from the Gram matrix Q'Q. Every product that sums over the 2N rows is a
+//                  fused multiply-add chain over the rows in order from +0 (the FP64 matrix instruction's own order:
+//                  four fma per instruction, k = 0..3), one chain per 128 rows, the chains' results added in order.
+//                  Mathematically the same factorisation; the results differ from RED_ROWS by rounding only.
+// The reference's left fold is the restatement; RED_SITES and RED_BLOCKED are what the device kernels compute bit for
+// bit.
 
-enum { RED_ROWS = 0, RED_SITES = 1 };
+enum { RED_ROWS = 0, RED_SITES = 1, RED_BLOCKED = 2 };
 
 template <class T>
 static T butterfly(std::vector<T>& v) {
@@ -149,7 +160,151 @@ static void chain_householder(T* A, int N, int k, int mode, ChainFact<T>& f) {
   }
 }
 
-// largest |Q'Q - I| (qr_spectrum.jl:4): entries accumulated with fma in row order, or per site then the butterfly
+// ---------------------------------------------------------------- RED_BLOCKED
+// Gram matrix G = X'X of the first k columns of a d x k column-major X: G[i][j] (i + 16 j) an fma chain over the rows
+// in order from +0, one chain per 128 rows, the chains' results added in order. G[i][j] equals G[j][i] bit for bit.
+template <class P, class T>
+static void chain_gram(const T* X, int d, int k, T* G) {
+  for (int j = 0; j < 16; ++j)
+    for (int i = 0; i < 16; ++i) {
+      T tot = (T)0;
+      for (int rb = 0; rb < d; rb += 128) {
+        T acc = (T)0;
+        if (i < k && j < k)
+          for (int r = rb; r < rb + 128 && r < d; ++r) acc = P::fma(X[(size_t)i * d + r], X[(size_t)j * d + r], acc);
+        tot = rb == 0 ? acc : tot + acc;
+      }
+      G[i + 16 * j] = tot;
+    }
+}
+
+// Upper-triangular factor of the compact WY form of H_0 H_1 ... H_{nb-1}, H_j = I - 2 u_j u_j' (a reflector that was
+// not applied has u_j = 0): T[j][j] = 2, T[m][j] = -2 (sum over p < j of T[m][p] G[p][j]), the sum an fma chain in
+// p order from +0 that includes the zeros T[m][p], p < m (they leave the chain at +0: G is finite); zero below the
+// diagonal. T (i + 16 j) for i, j < nb.
+template <class P, class T>
+static void chain_tmat(const T* G, int nb, T* Tm) {
+  for (int e = 0; e < 256; ++e) Tm[e] = (T)0;
+  for (int j = 0; j < nb; ++j) {
+    Tm[j + 16 * j] = (T)2;
+    for (int m = 0; m < j; ++m) {
+      T z = (T)0;
+      for (int p = 0; p < j; ++p) z = P::fma(Tm[m + 16 * p], G[p + 16 * j], z);
+      Tm[m + 16 * j] = (T)(g_defect == DEFECT_BLK_TSIGN ? 2 : -2) * z;
+    }
+  }
+}
+
+// One panel of the sweep (columns j0 .. j1 - 1), the reflectors applied to the panel's columns only; writes u_j into U
+// and rho into the diagonal. The column norm c: m the largest |entry| over rows >= j; for m in [2^-400, 2^400] the
+// square root of the plain sum of squares (no square overflows or falls below the normal range), in the RED_SITES
+// order, else safe_norm (norm.jl:8-12) in the RED_SITES order; ||v|| of v = a + sign(a_j) c e_j from its closed form
+// sqrt(2c (c + |a_j|)), so one reduction gives m, the sum and a_j together. The same reflector as the reference's to
+// rounding.
+template <class P, class T>
+static void chain_panel(T* A, int N, int j0, int j1, int nref, T* U, int& count) {
+  const int d = 2 * N;
+  std::vector<T> sq(d), u(d);
+  for (int j = j0; j < j1 && j < nref; ++j) {
+    T* col = A + (size_t)j * d;
+    T m = P::abs(col[j]);
+    for (int r = j + 1; r < d; ++r) m = jmax<P>(m, P::abs(col[r]));
+    T c;
+    if (!(P::finite(m) && m > 0)) {
+      c = m;
+    } else if (m >= (T)0x1p-400 && m <= (T)0x1p400) {
+      for (int r = 0; r < d; ++r) sq[r] = r >= j ? col[r] * col[r] : (T)0;
+      c = P::sqrt(chain_sum(sq.data(), N, j, RED_SITES));
+    } else {
+      c = chain_snorm<P>(col, N, j, RED_SITES);
+    }
+    T rho;
+    if (P::finite(c) && c > 0) {
+      T s = col[j] < 0 ? (T)-1 : (T)1;
+      const T vj = col[j] + s * c;
+      const T nv = P::sqrt(((T)2 * c) * (c + P::abs(col[j])));
+      for (int r = 0; r < d; ++r) u[r] = r > j ? col[r] / nv : (r == j ? vj / nv : (T)0);
+      for (int l = j + 1; l < j1; ++l) {
+        T* Al = A + (size_t)l * d;
+        T w = chain_dot(u.data(), Al, N, j, RED_SITES);
+        for (int r = j; r < d; ++r) Al[r] = Al[r] - ((T)2 * u[r]) * w;
+      }
+      for (int r = 0; r < d; ++r) U[(size_t)j * d + r] = u[r];
+      ++count;
+      rho = -s * c;
+    } else {
+      rho = c;
+    }
+    col[j] = rho;
+  }
+}
+
+// HouseholderQR of a d x 16 frame A (column-major; overwritten) in the RED_BLOCKED order.
+template <class P, class T>
+static void chain_householder_blocked(T* A, int N, int k, ChainFact<T>& f) {
+  const int d = 2 * N;
+  const int nref = k < d ? k : k - 1;
+  std::vector<T> U((size_t)d * k, (T)0), X((size_t)d * k);
+  T G[256], Tm[256], Y[256], M[256];
+  int count = 0;
+  // panel 1, then the trailing columns 8..15 take H_7 ... H_0 = I - U T' U' at once: A <- A - U (T' W), W = U'A
+  chain_panel<P>(A, N, 0, 8, nref, U.data(), count);
+  for (int c = 0; c < k; ++c)
+    for (int r = 0; r < d; ++r) X[(size_t)c * d + r] = c < 8 ? U[(size_t)c * d + r] : A[(size_t)c * d + r];
+  chain_gram<P>(X.data(), d, k, G);
+  chain_tmat<P>(G, 8, Tm);
+  for (int m = 0; m < 8; ++m)
+    for (int c = 8; c < k; ++c) {
+      T y = (T)0;
+      for (int p = 0; p < 8; ++p)  // T' W, the zeros included
+        y = P::fma(g_defect == DEFECT_BLK_TNOTRANS ? Tm[m + 16 * p] : Tm[p + 16 * m], G[p + 16 * c], y);
+      Y[m + 16 * c] = y;
+    }
+  for (int c = 8; c < k; ++c)
+    for (int r = 0; r < d; ++r) {
+      T acc = A[(size_t)c * d + r];
+      for (int m = 0; m < 8; ++m) acc = P::fma(-U[(size_t)m * d + r], Y[m + 16 * c], acc);
+      A[(size_t)c * d + r] = acc;
+    }
+  // panel 2
+  chain_panel<P>(A, N, 8, k, nref, U.data(), count);
+  // Q = H_0 ... H_15 E = E - U (T U_top'), U_top the first 16 rows of U
+  chain_gram<P>(U.data(), d, k, G);
+  chain_tmat<P>(G, k, Tm);
+  for (int m = 0; m < k; ++m)
+    for (int c = 0; c < k; ++c) {
+      T acc = (T)0;
+      for (int p = 0; p < k; ++p) acc = P::fma(Tm[m + 16 * p], U[(size_t)p * d + c], acc);  // the zeros included
+      M[m + 16 * c] = acc;
+    }
+  // sign fix and status (householder.jl:55-65)
+  f.Q.assign((size_t)d * k, (T)0);
+  f.R.assign((size_t)k * k, (T)0);
+  std::vector<T> sg(k);
+  for (int j = 0; j < k; ++j) sg[j] = A[(size_t)j * d + j] < 0 ? (T)-1 : (T)1;
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i <= j; ++i) f.R[(size_t)j * k + i] = sg[i] * A[(size_t)j * d + i];
+  for (int c = 0; c < k; ++c)
+    for (int r = 0; r < d; ++r) {
+      T acc = r == c ? (T)1 : (T)0;
+      for (int m = 0; m < k; ++m) acc = P::fma(-U[(size_t)m * d + r], M[m + 16 * c], acc);
+      f.Q[(size_t)c * d + r] = acc * sg[c];
+    }
+  f.count = count;
+  f.status = 0;
+  f.column = 0;
+  for (int j = 0; j < k; ++j) {
+    T p = f.R[(size_t)j * k + j];
+    if (!(P::finite(p) && p > 0)) {
+      f.status = 1;
+      f.column = j + 1;
+      break;
+    }
+  }
+}
+
+// largest |Q'Q - I| (qr_spectrum.jl:4): entries accumulated with fma in row order, per site then the butterfly, or
+// (RED_BLOCKED) the Gram matrix's chains
 template <class P, class T>
 static T chain_orth_loss(const T* Q, int N, int k, int mode) {
   const int d = 2 * N;
@@ -159,7 +314,14 @@ static T chain_orth_loss(const T* Q, int N, int k, int mode) {
     for (int a = 0; a < k; ++a) {
       const T *qa = Q + (size_t)a * d, *qb = Q + (size_t)b * d;
       T acc;
-      if (mode == RED_ROWS) {
+      if (mode == RED_BLOCKED) {
+        acc = (T)0;
+        for (int rb = 0; rb < d; rb += 128) {
+          T part = (T)0;
+          for (int r = rb; r < rb + 128 && r < d; ++r) part = P::fma(qa[r], qb[r], part);
+          acc = rb == 0 ? part : acc + part;
+        }
+      } else if (mode == RED_ROWS) {
         acc = qa[0] * qb[0];
         for (int r = 1; r < d; ++r) acc = P::fma(qa[r], qb[r], acc);
       } else {
@@ -187,7 +349,10 @@ static void chain_qr_step(const cl_reducer& r, ChainQRSt<T>& st, int64_t n, T* W
   if (st.code) return;
   ChainFact<T> f;
   std::vector<T> A(W, W + (size_t)2 * N * k);
-  chain_householder<P>(A.data(), N, k, mode, f);
+  if (mode == RED_BLOCKED)
+    chain_householder_blocked<P>(A.data(), N, k, f);
+  else
+    chain_householder<P>(A.data(), N, k, mode, f);
   auto fail = [&](int code, int64_t col, T v, T thr) {
     st.code = (int8_t)code;
     st.column = col;

@@ -117,7 +117,7 @@ def run(sysd, chain, x0, W0, N, tau, k, W2=None, math=MATH_CL, nthreads=None, qu
     return out
 
 
-RED_ROWS, RED_SITES = 0, 1
+RED_ROWS, RED_SITES, RED_BLOCKED = 0, 1, 2
 
 
 def run_chain(sysd, chain, x0, W0, N, tau, k, Ns=None, math=MATH_CL, mode=RED_SITES, nthreads=None):
@@ -195,7 +195,7 @@ def run_linear_d(A, chain, x0, W0, N, tau, k, quad=False):
 
 class planted:
     """Context manager: the oracle runs with a planted defect (oracle.cpp g_defect) inside the block."""
-    GALI_RAW, OFLI_NOPROJ, MEGNO_Q2, GALI_LASTFACTOR = 1, 2, 3, 4
+    GALI_RAW, OFLI_NOPROJ, MEGNO_Q2, GALI_LASTFACTOR, BLK_TNOTRANS, BLK_TSIGN = 1, 2, 3, 4, 5, 6
 
     def __init__(self, defect):
         self.defect = defect

@@ -1,18 +1,28 @@
-"""The coupled standard-map chain kernels on the GPU against the oracle, bit for bit (the oracle's site-butterfly
-reduction order, which the kernels use), for every built N, both boundaries, launch splits, per-orbit step counts
-and the host chunk entry point."""
+"""The coupled standard-map chain kernels on the GPU against the oracle, bit for bit, for every built N, both
+boundaries, launch splits, per-orbit step counts and the host chunk entry point, under both orders of the
+HouseholderQR: the blocked factorisation on the matrix cores (the oracle's RED_BLOCKED) and the lane-butterfly sweep
+(RED_SITES)."""
 import ctypes
 import sys
 
 import numpy as np
 import pytest
 
-from chaos_amd import StandardMapChain, QRSpectrum, HouseholderQR, propagate, propagate_ragged, _lib as L
+from chaos_amd import StandardMapChain, QRSpectrum, HouseholderQR, propagate, propagate_ragged, autotune, _lib as L
 from chaos_amd.tangent import make_chain, _context
 from tests import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
 K = 16
+
+
+@pytest.fixture(params=[(1, O.RED_BLOCKED), (2, O.RED_SITES)], ids=["blocked", "sites"])
+def order(request):
+    """The chain's QR order on the context (cl_tuning.chain_qr) and the oracle mode that restates it."""
+    qr, mode = request.param
+    autotune.set_tuning(chain_qr=qr)
+    yield mode
+    autotune.set_tuning()
 
 
 def say(*a):
@@ -27,16 +37,16 @@ def starts(seed, m, N):
 
 @pytest.mark.parametrize("ring", [True, False], ids=["ring", "open"])
 @pytest.mark.parametrize("N", [8, 16, 32, 64, 128])
-def test_chain_qr_bitwise(N, ring):
+def test_chain_qr_bitwise(N, ring, order):
     m = 13 if N <= 64 else 3
-    say("chain qr", N, ring, m)
+    say("chain qr", N, ring, m, order)
     sysm = StandardMapChain(N, 1.5, 0.1, "ring" if ring else "open")
     x0 = starts(N + ring, m, N)
     W0 = O.frames(5, 1, m, 2 * N, K)
     chain, _ = make_chain((QRSpectrum(HouseholderQR()),))
     g = propagate(sysm, x0, 300, k=K, tau=10, seed=5, reducers=(QRSpectrum(HouseholderQR()),), W0=W0,
                   keep_states=True)
-    o = O.run_chain(sysm.descriptor(), chain, x0, W0, 300, 10, K, mode=O.RED_SITES)
+    o = O.run_chain(sysm.descriptor(), chain, x0, W0, 300, 10, K, mode=order)
     assert np.array_equal(g.x, o["x"])
     assert np.array_equal(g.W, o["W"])
     assert np.array_equal(g.states, o["states"])
@@ -46,8 +56,8 @@ def test_chain_qr_bitwise(N, ring):
 
 
 @pytest.mark.parametrize("N", [8, 128])
-def test_chain_no_reducer_and_launch_split(N):
-    say("chain split", N)
+def test_chain_no_reducer_and_launch_split(N, order):
+    say("chain split", N, order)
     sysm = StandardMapChain(N, 1.5, 0.1, "ring")
     x0 = starts(3, 5, N)
     W0 = O.frames(3, 1, 5, 2 * N, K)
@@ -61,8 +71,8 @@ def test_chain_no_reducer_and_launch_split(N):
 
 
 @pytest.mark.parametrize("N", [16, 128])
-def test_chain_ragged_bitwise(N):
-    say("chain ragged", N)
+def test_chain_ragged_bitwise(N, order):
+    say("chain ragged", N, order)
     m = 9
     sysm = StandardMapChain(N, 1.5, 0.1, "ring")
     x0 = starts(11, m, N)
@@ -70,16 +80,16 @@ def test_chain_ragged_bitwise(N):
     Ns = 10 * np.random.default_rng(1).integers(3, 40, m)
     r = (QRSpectrum(HouseholderQR()),)
     g = propagate_ragged(sysm, x0, Ns, k=K, tau=10, seed=11, reducers=r, W0=W0, keep_states=True)
-    o = O.run_chain(sysm.descriptor(), make_chain(r)[0], x0, W0, 0, 10, K, Ns=Ns)
+    o = O.run_chain(sysm.descriptor(), make_chain(r)[0], x0, W0, 0, 10, K, Ns=Ns, mode=order)
     assert np.array_equal(g.x, o["x"]) and np.array_equal(g.W, o["W"])
     assert np.array_equal(g.states, o["states"]) and np.array_equal(g.hstep, Ns)
 
 
-def test_chain_origin_spectrum():
+def test_chain_origin_spectrum(order):
     """At the origin the chain's spectrum is +-ln mu_j of its modes (see tests/test_oracle_chain.py)."""
     from tests.test_oracle_chain import origin_exponents
     N = 32
-    say("chain origin", N)
+    say("chain origin", N, order)
     sysm = StandardMapChain(N, 1.5, 0.1, "ring")
     g = propagate(sysm, np.zeros((2, 2 * N)), 4000, k=K, tau=10, seed=2, reducers=(QRSpectrum(HouseholderQR()),))
     assert np.all(g.x == 0)
@@ -102,10 +112,10 @@ def test_device_frames_of_chain_sizes_equal_host(d, k):
     assert np.array_equal(dW.cpu().numpy(), hW)
 
 
-def test_chain_host_chunk_and_frames():
+def test_chain_host_chunk_and_frames(order):
     import torch
     N, m = 64, 4
-    say("chain chunk", N)
+    say("chain chunk", N, order)
     lib, ctx = _context()
     sysm = StandardMapChain(N, 1.5, 0.1, "open")
     sd = sysm.descriptor()
@@ -129,6 +139,6 @@ def test_chain_host_chunk_and_frames():
                                 x.ctypes.data, W.ctypes.data, None, st.ctypes.data, code.ctypes.data, h.ctypes.data,
                                 None)
     assert rc == 0
-    o = O.run_chain(sd, chain, x0, hW.transpose(0, 2, 1), 100, 10, K)
+    o = O.run_chain(sd, chain, x0, hW.transpose(0, 2, 1), 100, 10, K, mode=order)
     assert np.array_equal(x, o["x"]) and np.array_equal(W.transpose(0, 2, 1), o["W"])
     assert np.array_equal(st.view(np.uint8).reshape(m, -1), o["states"])

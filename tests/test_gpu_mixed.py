@@ -8,13 +8,22 @@ import sys
 import numpy as np
 import pytest
 
-from chaos_amd import StandardMapChain, QRSpectrum, HouseholderQR, propagate_ragged, propagate_mixed
+from chaos_amd import StandardMapChain, QRSpectrum, HouseholderQR, propagate_ragged, propagate_mixed, autotune
 from chaos_amd.tangent import make_chain
 from tests import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
 K = 16
 R = (QRSpectrum(HouseholderQR()),)
+
+
+@pytest.fixture(params=[(1, O.RED_BLOCKED), (2, O.RED_SITES)], ids=["blocked", "sites"])
+def order(request):
+    """The chain's QR order on the context (cl_tuning.chain_qr) and the oracle mode that restates it."""
+    qr, mode = request.param
+    autotune.set_tuning(chain_qr=qr)
+    yield mode
+    autotune.set_tuning()
 
 
 def say(*a):
@@ -40,8 +49,8 @@ def check_equal(a, b):
 
 @pytest.mark.parametrize("boundary", ["ring", "open"])
 @pytest.mark.parametrize("chunk", [0, 60, 130])
-def test_mixed_equals_ragged_and_oracle(boundary, chunk):
-    say("mixed", boundary, chunk)
+def test_mixed_equals_ragged_and_oracle(boundary, chunk, order):
+    say("mixed", boundary, chunk, order)
     lat = lattices(7, [(8, 11), (16, 9), (32, 7), (64, 5), (128, 3)], 30, 400, 10)
     got = propagate_mixed(lat, K=1.5, eps=0.1, boundary=boundary, k=K, tau=10, seed=3, chunk=chunk, keep_states=True)
     for (N, x0, Ns), g in zip(lat, got):
@@ -51,7 +60,7 @@ def test_mixed_equals_ragged_and_oracle(boundary, chunk):
         assert np.array_equal(g.hstep, Ns)
         if N <= 32:
             W0 = O.frames(3, 1, len(x0), 2 * N, K)
-            o = O.run_chain(sysm.descriptor(), make_chain(R)[0], x0, W0, 0, 10, K, Ns=Ns)
+            o = O.run_chain(sysm.descriptor(), make_chain(R)[0], x0, W0, 0, 10, K, Ns=Ns, mode=order)
             assert np.array_equal(g.x, o["x"]) and np.array_equal(g.W, o["W"]) and np.array_equal(g.states, o["states"])
 
 
@@ -71,7 +80,7 @@ def test_mixed_graph_replays_equal_direct_launches():
         check_equal(x, z)
 
 
-def test_mixed_retires_failed_orbit():
+def test_mixed_retires_failed_orbit(order):
     """A non-finite start fails at its first handoff (code 1, the reference's state status) and is skipped after;
     its neighbours in the same work item run on unchanged."""
     say("mixed failure")

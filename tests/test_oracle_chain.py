@@ -47,7 +47,7 @@ def test_origin_is_fixed_and_spectrum_is_exact(N, ring):
     x0 = np.zeros((m, 2 * N))
     W0 = O.frames(3, 1, m, 2 * N, k)
     n = 4000
-    for mode in (O.RED_ROWS, O.RED_SITES):
+    for mode in (O.RED_ROWS, O.RED_SITES, O.RED_BLOCKED):
         r = O.run_chain(sysd, QR[0], x0, W0, n, 10, k, mode=mode)
         assert np.all(r["x"] == 0) and np.all(r["code"] == 0)
         lam = lambdas(r, k)
@@ -120,9 +120,12 @@ def test_modes_agree_and_symplectic_sum():
     W0 = O.frames(7, 1, 4, 2 * N, k)
     a = lambdas(O.run_chain(sysd, QR[0], x0, W0, n, 10, k, mode=O.RED_ROWS), k)
     b = lambdas(O.run_chain(sysd, QR[0], x0, W0, n, 10, k, mode=O.RED_SITES), k)
+    c = lambdas(O.run_chain(sysd, QR[0], x0, W0, n, 10, k, mode=O.RED_BLOCKED), k)
     # rounding level: the last columns span directions contracted by about exp(-(lambda_1 - lambda_16) tau) = 3e-4
     # per handoff relative to the first, which amplifies a rounding difference in the factorisation by the inverse
     assert np.max(np.abs(a - b)) < 1e-11
+    assert np.max(np.abs(a - c)) < 1e-11
+    assert np.max(np.abs(c.sum(axis=1))) < 1e-10
     # det M = 1 exactly; the sum collects the rounding of 300 handoffs, each amplified by cond(M^tau) ~ 5e3 in the
     # contracted columns. A wrong index or sign in the band or the reflectors moves it by O(0.1).
     assert np.max(np.abs(a.sum(axis=1))) < 1e-10
@@ -143,3 +146,26 @@ def test_ragged_step_counts():
         s = O.run_chain(sysd, QR[0], x0[i:i + 1], W0[i:i + 1], int(Ns[i]), 10, k)
         assert np.array_equal(r["states"][i], s["states"][0]) and np.array_equal(r["W"][i], s["W"][0])
         assert r["hstep"][i] == Ns[i]
+
+
+@pytest.mark.parametrize("N", [16, 128])
+def test_blocked_order_against_row_order_and_planted_defects(N):
+    """The blocked factorisation (two panels, compact WY trailing update, Q = E - U T U_top') gives the row order's
+    frame to rounding after one handoff and its spectrum to rounding over 100 handoffs (N = 128: the Gram sums in two
+    128-row chains). Negative controls: T instead of T' in the trailing update, or the wrong sign of T's off-diagonal
+    entries, moves the frame by O(1) or fails the orthogonality check."""
+    k = 16
+    sysd = StandardMapChain(N, 1.5, 0.1, "ring").descriptor()
+    rng = np.random.default_rng(21)
+    x0 = rng.uniform(0, 2 * np.pi, (3, 2 * N))
+    W0 = O.frames(21, 1, 3, 2 * N, k)
+    r1 = O.run_chain(sysd, QR[0], x0, W0, 10, 10, k, mode=O.RED_ROWS)
+    b1 = O.run_chain(sysd, QR[0], x0, W0, 10, 10, k, mode=O.RED_BLOCKED)
+    assert np.all(b1["code"] == 0) and np.max(np.abs(b1["W"] - r1["W"])) < 1e-12
+    a = lambdas(O.run_chain(sysd, QR[0], x0, W0, 1000, 10, k, mode=O.RED_ROWS), k)
+    c = lambdas(O.run_chain(sysd, QR[0], x0, W0, 1000, 10, k, mode=O.RED_BLOCKED), k)
+    assert np.max(np.abs(a - c)) < 1e-9
+    for defect in (O.planted.BLK_TNOTRANS, O.planted.BLK_TSIGN):
+        with O.planted(defect):
+            d1 = O.run_chain(sysd, QR[0], x0, W0, 10, 10, k, mode=O.RED_BLOCKED)
+        assert np.any(d1["code"] != 0) or np.max(np.abs(d1["W"] - r1["W"])) > 1e-3, defect
