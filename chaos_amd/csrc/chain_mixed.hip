@@ -29,8 +29,12 @@ constexpr int MIX_BLK_SLOTS = 4;
 constexpr int MIX_POOL_BLK = MIX_XS + (MIX_XB + MIX_BLK_SLOT > MIX_BLK_SLOTS * MIX_BLK_SLOT ? MIX_XB + MIX_BLK_SLOT
                                                                                            : MIX_BLK_SLOTS * MIX_BLK_SLOT);
 
+// Each lattice's body is its own function: inlined into the one kernel, the five bodies spilled 1.7 KB per lane even
+// with the whole register file (C3, blocked order: 741 ms, against 512 ms as separate functions; the butterfly order
+// 640 -> 617 ms).
+#define CL_MIX_ITEM_ATTR __attribute__((noinline))
 template <int NS, bool RING, bool BLK>
-__device__ __attribute__((always_inline)) inline void mixed_item(const ChainArgs& a, const MixedItem& w, double* pool) {
+__device__ CL_MIX_ITEM_ATTR void mixed_item(const ChainArgs& a, const MixedItem& w, double* pool) {
   if constexpr (BLK) {
     constexpr int SL = NS > 64 ? 1 : (MIX_BLOCK / NS < MIX_BLK_SLOTS ? MIX_BLOCK / NS : MIX_BLK_SLOTS);
     double* xb = NS > 64 ? pool + MIX_XS : nullptr;
@@ -50,10 +54,15 @@ __device__ __attribute__((always_inline)) inline void mixed_item(const ChainArgs
   }
 }
 
-// Two waves per SIMD (256 registers, about 1.3 KB spilled per lane): on the C3 workload (the five lattices, 2048
-// orbits each, uneven lengths) one launch takes 995 ms this way and 1095 ms with the whole register file per wave.
+// The lane-butterfly order at two waves per SIMD (256 registers): on the C3 workload (the five lattices, 2048 orbits
+// each, uneven lengths) 617 ms. The blocked order at one wave per SIMD (the whole register file): 512 ms, against
+// 683 ms at two.
+#ifndef CL_MIX_BLK_WAVES
+#define CL_MIX_BLK_WAVES 1
+#endif
 template <bool RING, bool BLK>
-__global__ void __launch_bounds__(MIX_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) chain_mixed_kernel(MixedArgs m) {
+__global__ void __launch_bounds__(MIX_BLOCK) __attribute__((amdgpu_waves_per_eu(BLK ? CL_MIX_BLK_WAVES : 2)))
+chain_mixed_kernel(MixedArgs m) {
   __shared__ double pool[BLK ? MIX_POOL_BLK : MIX_POOL];
   __shared__ unsigned item_s;
   for (;;) {
@@ -112,8 +121,9 @@ int mixed_cus() {
 int mixed_launch(const MixedArgs& a, bool ring, bool blocked, hipStream_t st) {
   using namespace chain_detail;
   if (a.nitems <= 0) return 0;
-  // three groups per compute unit (the LDS pool allows three), never more groups than items
-  int64_t grid = (int64_t)mixed_cus() * 3;
+  // three groups per compute unit (the LDS pool allows three; two for the blocked order at one wave per SIMD), never
+  // more groups than items
+  int64_t grid = (int64_t)mixed_cus() * (blocked && CL_MIX_BLK_WAVES == 1 ? 2 : 3);
   if (grid > a.nitems) grid = a.nitems;
   if (ring)
     hipLaunchKernelGGL((blocked ? chain_mixed_kernel<true, true> : chain_mixed_kernel<true, false>), dim3((unsigned)grid),
