@@ -538,7 +538,7 @@ def record_main(a):
     return out
 
 
-def c3_main(a):
+def c3_main(a, emit=True):
     import torch
     from chaos_amd import StandardMapChain, QRSpectrum, HouseholderQR, _lib as L
     from chaos_amd.inputs import chain_starts, chain_step_counts
@@ -602,17 +602,14 @@ def c3_main(a):
             S["stream"].wait_event(ev0)
         # every lattice's starts and frames first, so that no frame kernel waits behind another lattice's chain
         # kernel for its compute units
-        evs = {}
         for S in sets:
-            e0 = torch.cuda.Event(enable_timing=True)
-            e0.record(S["stream"])
             setup(S)
-            evs[S["N"]] = e0
         for S in (sets[::-1] if a.c3_order == "descending" else sets):
-            e1 = torch.cuda.Event(enable_timing=True)
+            es, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            es.record(S["stream"])  # the lattice's chain kernel alone: from its own start on its stream
             run(S, S["Ns"], with_setup=False)
             e1.record(S["stream"])
-            per_n.setdefault(S["N"], []).append((ev0, evs[S["N"]], e1))
+            per_n.setdefault(S["N"], []).append((ev0, es, e1))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = {N: float(np.mean([x.elapsed_time(z) for _, x, z in v])) for N, v in per_n.items()}
@@ -660,7 +657,9 @@ def c3_main(a):
     }
     if cpu:
         out["gpu_over_cpu"] = out["value"] / cpu["value"]
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def c5_main(a):
@@ -1067,7 +1066,7 @@ def main():
     ap.add_argument("--cpu-orbits", type=int, default=32768, help="orbits per K in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="c2: skip the Float32-tangent, accuracy, C1 and C5 blocks of the default run")
+                    help="c2: skip the Float32-tangent, accuracy, C1, C5 and C3 blocks of the default run")
     ap.add_argument("--c3-qr", default="auto", choices=["auto", "blocked", "sites"],
                     help="c3: the HouseholderQR order (auto: the library's rule, the blocked one on the matrix cores)")
     ap.add_argument("--c3-mode", default="streams", choices=["streams", "mixed", "serial"],
@@ -1141,6 +1140,14 @@ def main():
         extra["c5"] = {"value": c5["value"], "unit": c5["unit"], "workload": c5["config"]["workload"],
                        **c5["surrogate"]}
         log("c5", {k: extra["c5"][k] for k in ("speedup", "misclassified_vs_plain", "continued_fraction")})
+        # the coupled chains (C3) on five streams in the library's QR order (the blocked one on the matrix cores)
+        a3 = argparse.Namespace(**vars(a))
+        a3.steps, a3.c3_mode, a3.c3_qr, a3.no_cpu, a3.c3_order = 2, "streams", "auto", True, "ascending"
+        c3 = c3_main(a3, emit=False)
+        extra["c3"] = {"value": c3["value"], "unit": c3["unit"], "workload": c3["config"]["workload"],
+                       "kernel_ms_per_N": c3["config"]["kernel_ms_per_N"], "roofline": c3["roofline"],
+                       "nonfinite_orbits": c3["config"]["nonfinite_orbits"]}
+        log("c3", {"value": c3["value"], "span_ms": c3["roofline"]["kernel_ms"], "frac": c3["roofline"]["frac"]})
     budget = {"c2": a.cpu_orbits, "c1": 4096, "c4": 2048}[wl["name"]]
     cpu = None if a.no_cpu else cpu_baseline(wl, budget)
     if cpu is not None:
