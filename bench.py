@@ -431,7 +431,7 @@ def qr_lambdas(dst, k):
     return s / np.where(n > 0, n, 1.0)[:, None]
 
 
-def c3_mixed(a, lib, ctx, chain, sets, m, k, tau, desc):
+def c3_mixed(a, lib, ctx, chain, sets, m, k, tau, desc, E=1):
     """C3 as one persistent launch over the five lattices per chunk (cl_propagate_mixed), chunks of --c3-chunk
     steps captured in one graph."""
     import torch
@@ -449,7 +449,7 @@ def c3_mixed(a, lib, ctx, chain, sets, m, k, tau, desc):
             S["dx"].copy_(S["xm"])
             assert lib.cl_initial_deviations_device(ctx, 3, 1, m, 2 * S["N"], k, S["dW"].data_ptr(), st.cuda_stream) == 0
             lats[j].Ns = Ns_list[j].ctypes.data
-        rc = lib.cl_propagate_mixed(ctx, ctypes.byref(sets[0]["sd"]), ctypes.byref(chain), 1, len(sets), lats, k, tau,
+        rc = lib.cl_propagate_mixed(ctx, ctypes.byref(sets[0]["sd"]), ctypes.byref(chain), E, len(sets), lats, k, tau,
                                     chunk, st.cuda_stream)
         assert rc == 0, lib.cl_last_error(ctx)
 
@@ -549,8 +549,10 @@ def c3_main(a, emit=True):
     m, k, tau = 2048, 16, 10
     order = ("blocked HouseholderQR on the FP64 matrix cores (two panels of 8, compact WY trailing update)"
              if a.c3_qr != "sites" else "unblocked HouseholderQR, sums over sites as lane butterflies")
+    E = {"f64": 1, "f32tan": 2}[a.c3_elt]
+    prec = "FP64" if a.c3_elt == "f64" else "FP64 orbit, Float32 tangents, FP64 factorisation and log sums"
     desc = (f"coupled standard-map chain, ring, K=1.5, eps=0.1, N in {list(C3_NS)} x {m} orbits, steps U[2000,20000] "
-            f"per orbit (multiples of tau={tau}), k={k}, QRSpectrum(HouseholderQR()), FP64; {order}")
+            f"per orbit (multiples of tau={tau}), k={k}, QRSpectrum(HouseholderQR()), {prec}; {order}")
     log("workload", desc)
     chain, _ = make_chain((QRSpectrum(HouseholderQR()),))
     sets = []
@@ -581,14 +583,14 @@ def c3_main(a, emit=True):
         sh = S["stream"].cuda_stream
         if with_setup:
             setup(S)
-        rc = lib.cl_propagate_ragged(ctx, ctypes.byref(S["sd"]), ctypes.byref(chain), 1, m, 2 * S["N"], 2 * S["N"], k,
+        rc = lib.cl_propagate_ragged(ctx, ctypes.byref(S["sd"]), ctypes.byref(chain), E, m, 2 * S["N"], 2 * S["N"], k,
                                      Ns.ctypes.data, S["order"].ctypes.data, tau, S["dx"].data_ptr(),
                                      S["dW"].data_ptr(), S["dst"].data_ptr(), S["dc"].data_ptr(), S["dh"].data_ptr(),
                                      sh)
         assert rc == 0, lib.cl_last_error(ctx)
 
     if a.c3_mode == "mixed":
-        return c3_mixed(a, lib, ctx, chain, sets, m, k, tau, desc)
+        return c3_mixed(a, lib, ctx, chain, sets, m, k, tau, desc, E)
     for S in sets:  # warmup: 20 steps per orbit
         run(S, np.full(m, 20, dtype=np.int64))
     torch.cuda.synchronize()
@@ -596,6 +598,8 @@ def c3_main(a, emit=True):
     per_n = {}
     t0 = time.perf_counter()
     for _ in range(a.steps):
+        # the previous step's kernels on the five (non-blocking) streams are done before this step's start event
+        torch.cuda.synchronize()
         ev0 = torch.cuda.Event(enable_timing=True)
         ev0.record()
         for S in sets:
@@ -659,6 +663,7 @@ def c3_main(a, emit=True):
         out["gpu_over_cpu"] = out["value"] / cpu["value"]
     if emit:
         print(json.dumps(out), flush=True)
+    out["_lambdas"] = {S["N"]: qr_lambdas(S["dst"], k) for S in sets}  # for the caller's same-orbit comparison
     return out
 
 
@@ -1069,6 +1074,8 @@ def main():
                     help="c2: skip the Float32-tangent, accuracy, C1, C5 and C3 blocks of the default run")
     ap.add_argument("--c3-qr", default="auto", choices=["auto", "blocked", "sites"],
                     help="c3: the HouseholderQR order (auto: the library's rule, the blocked one on the matrix cores)")
+    ap.add_argument("--c3-elt", default="f64", choices=["f64", "f32tan"],
+                    help="c3: FP64 frames or Float32 tangents (FP64 orbit, factorisation and log sums)")
     ap.add_argument("--c3-mode", default="streams", choices=["streams", "mixed", "serial"],
                     help="c3: one launch per lattice on five streams, one persistent launch over all five, or the five "
                          "launches one after another on one stream (each lattice's kernel alone on the device)")
@@ -1142,12 +1149,26 @@ def main():
         log("c5", {k: extra["c5"][k] for k in ("speedup", "misclassified_vs_plain", "continued_fraction")})
         # the coupled chains (C3) on five streams in the library's QR order (the blocked one on the matrix cores)
         a3 = argparse.Namespace(**vars(a))
-        a3.steps, a3.c3_mode, a3.c3_qr, a3.no_cpu, a3.c3_order = 2, "streams", "auto", True, "ascending"
+        a3.steps, a3.c3_mode, a3.c3_qr, a3.no_cpu, a3.c3_order, a3.c3_elt = 2, "streams", "auto", True, "ascending", "f64"
         c3 = c3_main(a3, emit=False)
         extra["c3"] = {"value": c3["value"], "unit": c3["unit"], "workload": c3["config"]["workload"],
                        "kernel_ms_per_N": c3["config"]["kernel_ms_per_N"], "roofline": c3["roofline"],
                        "nonfinite_orbits": c3["config"]["nonfinite_orbits"]}
         log("c3", {"value": c3["value"], "span_ms": c3["roofline"]["kernel_ms"], "frac": c3["roofline"]["frac"]})
+        # the Float32-tangent variant of the same ensembles: rate and the same-orbit error of the spectrum
+        a3.c3_elt = "f32tan"
+        c3f = c3_main(a3, emit=False)
+        err = {}
+        for N, l64 in c3["_lambdas"].items():
+            d = np.abs(c3f["_lambdas"][N] - l64)
+            err[N] = {"lambda_1_max": float(d[:, 0].max()), "lambda_1_median": float(np.median(d[:, 0])),
+                      "all_max": float(d.max()), "all_median": float(np.median(d))}
+        extra["c3"]["f32tan"] = {"value": c3f["value"], "over_f64": c3f["value"] / c3["value"],
+                                 "span_ms": c3f["roofline"]["kernel_ms"], "frac": c3f["roofline"]["frac"],
+                                 "kernel_ms_per_N": c3f["config"]["kernel_ms_per_N"],
+                                 "nonfinite_orbits": c3f["config"]["nonfinite_orbits"],
+                                 "lambda_error_vs_f64_same_orbit": err}
+        log("c3 f32tan", {k: v for k, v in extra["c3"]["f32tan"].items() if k != "kernel_ms_per_N"})
     budget = {"c2": a.cpu_orbits, "c1": 4096, "c4": 2048}[wl["name"]]
     cpu = None if a.no_cpu else cpu_baseline(wl, budget)
     if cpu is not None:

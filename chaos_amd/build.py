@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchaoslab_hip.so")
-SOURCES = ["chain_mixed.hip", "chain_n128.hip", "chain_n64.hip", "chain_n32.hip", "chain_n16.hip", "chain_n8.hip", "chain_kernels.hip",
+SOURCES = ["chain_mixed.hip", "chain_mixed_f32.hip", "chain_n128.hip", "chain_n128f.hip", "chain_n64.hip", "chain_n64f.hip",
+           "chain_n32.hip", "chain_n32f.hip", "chain_n16.hip", "chain_n16f.hip", "chain_n8.hip", "chain_n8f.hip", "chain_kernels.hip",
            "propagate.hip", "inst_maps.hip", "inst_flow.hip", "inst_f32.hip", "inst_pipe.hip", "record.hip", "shadow.hip", "mlp.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CL_OFFLOAD_ARCH", "gfx950")

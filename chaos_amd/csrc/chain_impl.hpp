@@ -23,6 +23,7 @@
 // ragged entry point; groups are dispatched in order, which balances uneven orbits over the CUs.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "cl_core.hpp"
 #include "chain_kernels.hpp"
 #include "../../include/chaoslab_hip.h"
@@ -35,6 +36,11 @@ namespace chain_detail {
 // is an indexed register move rather than a private-memory array.
 template <int K>
 using dvec = double __attribute__((ext_vector_type(K)));
+template <int K>
+using fvec = float __attribute__((ext_vector_type(K)));
+// Frame registers of the element type: Float64, or Float32 tangents (CL_ELT_F64_F32TAN).
+template <class FT, int K>
+using frame_vec = typename std::conditional<std::is_same<FT, float>::value, fvec<K>, dvec<K>>::type;
 
 template <int NS>
 struct Geo {
@@ -264,17 +270,18 @@ struct Orb {
     }
   }
   // values of v at the left and right neighbour sites (ring order inside the orbit)
-  template <int M, class V>
+  template <int M, class V, class E = double>
   __device__ __attribute__((always_inline)) void nbr(const V& v, V& l, V& r) const {
     if constexpr (NS > 64) {
+      E* xe = reinterpret_cast<E*>(xb);  // the exchange buffer holds elements of the frame's type
       lds_barrier();
 #pragma unroll
-      for (int j = 0; j < M; ++j) xb[j * 128 + t] = v[j];
+      for (int j = 0; j < M; ++j) xe[j * 128 + t] = v[j];
       lds_barrier();
 #pragma unroll
       for (int j = 0; j < M; ++j) {
-        l[j] = xb[j * 128 + ((t + NS - 1) & (NS - 1))];
-        r[j] = xb[j * 128 + ((t + 1) & (NS - 1))];
+        l[j] = xe[j * 128 + ((t + NS - 1) & (NS - 1))];
+        r[j] = xe[j * 128 + ((t + 1) & (NS - 1))];
       }
     } else {
       const int sl = base + ((t + NS - 1) & (NS - 1)), sr = base + ((t + 1) & (NS - 1));
@@ -1064,7 +1071,8 @@ __device__ __attribute__((always_inline)) inline void qr_handoff_blk(const Orb<N
 // run from the launch's step n0 for `steps` steps or, with per-orbit step counts, to min(N_i, n0 + steps); an orbit
 // whose count is reached (N_i <= n0) or that failed before n0 is retired and left untouched. LDS: xb (N = 128 only,
 // 128 k doubles), Us (2 NREF BLOCK doubles), Rs and Ris (one packed triangle per orbit slot each).
-template <int NS, int KC, bool RING, bool HASQR, int BLOCK, bool BLK = false, int SLOTS = Geo<NS>::OPB>
+template <int NS, int KC, bool RING, bool HASQR, int BLOCK, bool BLK = false, int SLOTS = Geo<NS>::OPB,
+          class FT = double>
 __device__ __attribute__((always_inline)) inline void chain_orbits(const ChainArgs& a, int64_t g0, int nslots,
                                                                    double* xb, double* Us, double* Rs, double* Ris,
                                                                    double* Gs = nullptr, double* Ts = nullptr) {
@@ -1102,9 +1110,11 @@ __device__ __attribute__((always_inline)) inline void chain_orbits(const ChainAr
   double* Wo = a.W + i * (int64_t)D * KC;
   uint64_t* so = a.st + i * (int64_t)WORDS;
   double q = 0.0, p = 0.0;
-  dvec<KC> Wq, Wp;
+  // Float32 tangents: the frame is rounded once at the launch start and carried in Float32 between handoffs
+  using FV = frame_vec<FT, KC>;
+  FV Wq, Wp;
 #pragma unroll
-  for (int c = 0; c < KC; ++c) Wq[c] = Wp[c] = 0.0;
+  for (int c = 0; c < KC; ++c) Wq[c] = Wp[c] = (FT)0;
   CSum sums[SPT];
   QRChainState qs;
   qs.n = qs.count_max = qs.column = qs.step = 0;
@@ -1117,8 +1127,8 @@ __device__ __attribute__((always_inline)) inline void chain_orbits(const ChainAr
     p = xo[NS + t];
 #pragma unroll
     for (int c = 0; c < KC; ++c) {
-      Wq[c] = Wo[c * D + t];
-      Wp[c] = Wo[c * D + NS + t];
+      Wq[c] = (FT)Wo[c * D + t];
+      Wp[c] = (FT)Wo[c * D + NS + t];
     }
     if (HASQR && a.n0 > 0) {
 #pragma unroll
@@ -1159,14 +1169,16 @@ __device__ __attribute__((always_inline)) inline void chain_orbits(const ChainAr
         double cL = hasL ? a.eps * bl[1] : 0.0;
         double cR = hasR ? a.eps * cb : 0.0;
         double dg = (gq + cL) + cR, gl = -cL, gr = -cR;
-        dvec<KC> wl, wr;
-        o.template nbr<KC, dvec<KC>>(Wq, wl, wr);
+        // the band's entries in the frame's type (formed in FP64, rounded once), the product in that type
+        const FT dgf = (FT)dg, glf = (FT)gl, grf = (FT)gr;
+        FV wl, wr;
+        o.template nbr<KC, FV, FT>(Wq, wl, wr);
 #pragma unroll
         for (int c = 0; c < KC; ++c) {
-          double gw = dg * Wq[c];
-          if (hasL) gw = gw + gl * wl[c];
-          if (hasR) gw = gw + gr * wr[c];
-          double np = gw + Wp[c];
+          FT gw = dgf * Wq[c];
+          if (hasL) gw = gw + glf * wl[c];
+          if (hasR) gw = gw + grf * wr[c];
+          FT np = gw + Wp[c];
           Wq[c] = (Wq[c] + gw) + Wp[c];
           Wp[c] = np;
         }
@@ -1194,7 +1206,22 @@ __device__ __attribute__((always_inline)) inline void chain_orbits(const ChainAr
         if constexpr (BLK) {
           want = true;
         } else {
-          qr_handoff<NS, KC, BLOCK>(o, a, n, Wq, Wp, Wo, sums, qs, Us, Ro, Rio);
+          if constexpr (std::is_same<FT, double>::value) {
+            qr_handoff<NS, KC, BLOCK>(o, a, n, Wq, Wp, Wo, sums, qs, Us, Ro, Rio);
+          } else {  // the factorisation in FP64 on the exactly widened frame, its Q rounded back
+            dvec<KC> Wqd, Wpd;
+#pragma unroll
+            for (int c = 0; c < KC; ++c) {
+              Wqd[c] = (double)Wq[c];
+              Wpd[c] = (double)Wp[c];
+            }
+            qr_handoff<NS, KC, BLOCK>(o, a, n, Wqd, Wpd, Wo, sums, qs, Us, Ro, Rio);
+#pragma unroll
+            for (int c = 0; c < KC; ++c) {
+              Wq[c] = (FT)Wqd[c];
+              Wp[c] = (FT)Wpd[c];
+            }
+          }
           if (qs.code != 0) {
             code = STATUS_REDUCER;
             done = true;
@@ -1208,7 +1235,22 @@ __device__ __attribute__((always_inline)) inline void chain_orbits(const ChainAr
       // one orbit, whose two waves agree on `want`)
       if (NS > 64 ? want : __builtin_amdgcn_ballot_w64(want) != 0) {
         BlkLds L{Us, Gs, Ts, Rs, xb};
-        qr_handoff_blk<NS, KC, BLOCK, SLOTS>(o, a, n, Wq, Wp, Wo, sums, qs, L, want);
+        if constexpr (std::is_same<FT, double>::value) {
+          qr_handoff_blk<NS, KC, BLOCK, SLOTS>(o, a, n, Wq, Wp, Wo, sums, qs, L, want);
+        } else {  // the factorisation in FP64 on the exactly widened frame, its Q rounded back
+          dvec<KC> Wqd, Wpd;
+#pragma unroll
+          for (int c = 0; c < KC; ++c) {
+            Wqd[c] = (double)Wq[c];
+            Wpd[c] = (double)Wp[c];
+          }
+          qr_handoff_blk<NS, KC, BLOCK, SLOTS>(o, a, n, Wqd, Wpd, Wo, sums, qs, L, want);
+#pragma unroll
+          for (int c = 0; c < KC; ++c) {
+            Wq[c] = (FT)Wqd[c];
+            Wp[c] = (FT)Wpd[c];
+          }
+        }
         if (want && qs.code != 0) {
           code = STATUS_REDUCER;
           done = true;
@@ -1234,8 +1276,8 @@ __device__ __attribute__((always_inline)) inline void chain_orbits(const ChainAr
   if (!qr_failed) {
 #pragma unroll
     for (int c = 0; c < KC; ++c) {
-      Wo[c * D + t] = Wq[c];
-      Wo[c * D + NS + t] = Wp[c];
+      Wo[c * D + t] = (double)Wq[c];
+      Wo[c * D + NS + t] = (double)Wp[c];
     }
   }
   if (HASQR) {
@@ -1268,7 +1310,7 @@ __device__ __attribute__((always_inline)) inline void chain_orbits(const ChainAr
 // N >= 64: ask the allocator for two waves per SIMD (256 registers, a few spilled): measured on the C3 workload,
 // N = 128 925 ms against 1001 ms and N = 64 552 against 573; at N = 32 the spills cost more than the second
 // wave (484 against 453), so smaller N keep the whole register file.
-template <int NS, int KC, bool RING, bool HASQR>
+template <int NS, int KC, bool RING, bool HASQR, class FT>
 __global__ void __launch_bounds__(Geo<NS>::BLOCK) __attribute__((amdgpu_waves_per_eu(NS >= 64 ? 2 : 1)))
 chain_kernel(ChainArgs a) {
   constexpr int D = 2 * NS;
@@ -1279,7 +1321,7 @@ chain_kernel(ChainArgs a) {
   __shared__ double Us[HASQR ? 2 * NREF * BLOCK : 1];
   __shared__ double Rs[HASQR ? OPB * Tri<KC>::SIZE : 1];
   __shared__ double Ris[HASQR ? OPB * Tri<KC>::SIZE : 1];
-  chain_orbits<NS, KC, RING, HASQR, BLOCK>(a, (int64_t)blockIdx.x * OPB, OPB, xb, Us, Rs, Ris);
+  chain_orbits<NS, KC, RING, HASQR, BLOCK, false, OPB, FT>(a, (int64_t)blockIdx.x * OPB, OPB, xb, Us, Rs, Ris);
 }
 
 // The blocked order on the matrix cores (qr_handoff_blk). LDS per group: the frames (2N x 16 per orbit, 32 BLOCK
@@ -1290,7 +1332,7 @@ chain_kernel(ChainArgs a) {
 #ifndef CL_BLK_WAVES
 #define CL_BLK_WAVES 1
 #endif
-template <int NS, int KC, bool RING>
+template <int NS, int KC, bool RING, class FT>
 __global__ void __launch_bounds__(Geo<NS>::BLOCK) __attribute__((amdgpu_waves_per_eu(NS >= 64 ? CL_BLK_WAVES : 1)))
 chain_kernel_blk(ChainArgs a) {
   constexpr int OPB = Geo<NS>::OPB;
@@ -1300,25 +1342,27 @@ chain_kernel_blk(ChainArgs a) {
   __shared__ double Gs[OPB * MSZ];
   __shared__ double Ts[OPB * MSZ];
   __shared__ double Rs[OPB * Tri<KC>::SIZE];
-  chain_orbits<NS, KC, RING, true, BLOCK, true>(a, (int64_t)blockIdx.x * OPB, OPB, xb, Xs, Rs, nullptr, Gs, Ts);
+  chain_orbits<NS, KC, RING, true, BLOCK, true, OPB, FT>(a, (int64_t)blockIdx.x * OPB, OPB, xb, Xs, Rs, nullptr, Gs,
+                                                         Ts);
 }
 
-template <int NS, int KC, bool RING, bool HASQR>
+template <int NS, int KC, bool RING, bool HASQR, class FT>
 int launch_chain(const ChainArgs& a, hipStream_t st, bool blocked) {
   constexpr int B = Geo<NS>::BLOCK, OPB = Geo<NS>::OPB;
   int64_t grid = (a.m + OPB - 1) / OPB;
   if (HASQR && blocked)
-    hipLaunchKernelGGL((chain_kernel_blk<NS, KC, RING>), dim3((unsigned)grid), dim3(B), 0, st, a);
+    hipLaunchKernelGGL((chain_kernel_blk<NS, KC, RING, FT>), dim3((unsigned)grid), dim3(B), 0, st, a);
   else
-    hipLaunchKernelGGL((chain_kernel<NS, KC, RING, HASQR>), dim3((unsigned)grid), dim3(B), 0, st, a);
+    hipLaunchKernelGGL((chain_kernel<NS, KC, RING, HASQR, FT>), dim3((unsigned)grid), dim3(B), 0, st, a);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (e == hipErrorOutOfMemory ? CL_ERR_OOM : CL_ERR_DEVICE_LOST);
 }
 
-template <int NS>
+template <int NS, class FT>
 int launch_n(const ChainArgs& a, bool ring, bool qr, bool blocked, hipStream_t st) {
-  if (ring) return qr ? launch_chain<NS, 16, true, true>(a, st, blocked) : launch_chain<NS, 16, true, false>(a, st, false);
-  return qr ? launch_chain<NS, 16, false, true>(a, st, blocked) : launch_chain<NS, 16, false, false>(a, st, false);
+  if (ring)
+    return qr ? launch_chain<NS, 16, true, true, FT>(a, st, blocked) : launch_chain<NS, 16, true, false, FT>(a, st, false);
+  return qr ? launch_chain<NS, 16, false, true, FT>(a, st, blocked) : launch_chain<NS, 16, false, false, FT>(a, st, false);
 }
 
 }  // namespace chain_detail

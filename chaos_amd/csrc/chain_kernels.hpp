@@ -44,13 +44,14 @@ int mixed_slots(int N, bool blocked);
 int mixed_zero(unsigned* counters, int n, hipStream_t st);
 // Compute units of the current device (queried once; call before a graph capture).
 int mixed_cus();
-// One persistent launch over the items (ring or open chains, QRSpectrum(HouseholderQR()), k = 16, either order).
-int mixed_launch(const MixedArgs& a, bool ring, bool blocked, hipStream_t st);
+// One persistent launch over the items (ring or open chains, QRSpectrum(HouseholderQR()), k = 16, either order,
+// Float64 frames or Float32 tangents).
+int mixed_launch(const MixedArgs& a, bool ring, bool blocked, bool f32, hipStream_t st);
 
 // 1 when a kernel is built for N sites and k columns.
 int chain_supported(int N, int k);
 // Launches the kernel for (N, ring, with or without QRSpectrum(HouseholderQR()), the blocked order on the matrix cores
-// or the lane-butterfly sweep); 0 or an error code.
-int chain_launch(const ChainArgs& a, int N, bool ring, bool qr, bool blocked, hipStream_t st);
+// or the lane-butterfly sweep, Float64 frames or Float32 tangents); 0 or an error code.
+int chain_launch(const ChainArgs& a, int N, bool ring, bool qr, bool blocked, bool f32, hipStream_t st);
 
 }  // namespace cl

@@ -4,7 +4,7 @@
 namespace cl {
 
 int launch_chain_n8(const ChainArgs& a, bool ring, bool qr, bool blocked, hipStream_t st) {
-  return chain_detail::launch_n<8>(a, ring, qr, blocked, st);
+  return chain_detail::launch_n<8, double>(a, ring, qr, blocked, st);
 }
 
 }  // namespace cl

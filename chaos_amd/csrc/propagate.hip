@@ -267,7 +267,8 @@ static int validate_chain(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch,
   if (!(k >= 1 && k <= d)) return set_err(ctx, CL_ERR_ARGUMENT, "k must be in 1:d");
   if (tau < 1) return set_err(ctx, CL_ERR_ARGUMENT, "tau must be positive");
   if (m < 0) return set_err(ctx, CL_ERR_ARGUMENT, "negative orbit count");
-  if (elt != CL_ELT_F64) return set_err(ctx, CL_ERR_UNSUPPORTED, "element type has no kernel for this system");
+  if (elt != CL_ELT_F64 && elt != CL_ELT_F64_F32TAN)
+    return set_err(ctx, CL_ERR_UNSUPPORTED, "element type has no kernel for this system");
   if (!chain_supported(N, k)) return set_err(ctx, CL_ERR_UNSUPPORTED, "no chain kernel for this N and k");
   if (ch->n == 0) {
     *qr = false;
@@ -787,7 +788,7 @@ static int chain_device(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, i
   if (!(N >= 1 && N % tau == 0 && steps >= 0 && steps % tau == 0 && n0 >= 0 && n0 % tau == 0 && n0 + steps <= N))
     return set_err(ctx, CL_ERR_ARGUMENT, "launch steps must be a multiple of tau within N");
   rc = chain_launch(chain_args(sys, ch, m, dx, dW, dst, dcode, dhstep, n0, steps, N, tau), sys->N, sys->ring != 0,
-                    qr, chain_blocked(ctx), st);
+                    qr, chain_blocked(ctx), elt == CL_ELT_F64_F32TAN, st);
   return rc ? set_err(ctx, rc, "chain kernel launch failed") : CL_OK;
 }
 
@@ -899,7 +900,7 @@ extern "C" int cl_propagate_ragged(cl_ctx* ctx, const cl_system* sys, const cl_c
   ChainArgs a = chain_args(sys, ch, m, dx, dW, dst, dcode, dhstep, 0, Nmax, Nmax, tau);
   a.Ns = dbuf;
   a.order = order ? dbuf + m : nullptr;
-  rc = chain_launch(a, sys->N, sys->ring != 0, qr, chain_blocked(ctx), st);
+  rc = chain_launch(a, sys->N, sys->ring != 0, qr, chain_blocked(ctx), elt == CL_ELT_F64_F32TAN, st);
   e = hipFreeAsync(dbuf, st);
   if (rc) return set_err(ctx, rc, "chain kernel launch failed");
   return e == hipSuccess ? CL_OK : set_err(ctx, map_err(e), "stream-ordered free failed");
@@ -914,13 +915,16 @@ struct MixedPlan {
   size_t off_items = 0, off_counters = 0;
   int64_t nitems = 0, Nmax = 0;
   int nchunks = 0;
+  bool f32 = false;  // Float32 tangents
 };
 
 static int mixed_plan(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int nlat, const cl_lattice* lat,
                       int k, int tau, int64_t chunk, MixedPlan& P, std::vector<MixedClass>& cls) {
   if (!sys || !ch || (nlat > 0 && !lat)) return set_err(ctx, CL_ERR_ARGUMENT, "null descriptor");
   if (sys->kind != CL_SYS_STANDARD_MAP_CHAIN) return set_err(ctx, CL_ERR_UNSUPPORTED, "the mixed launch runs chains");
-  if (elt != CL_ELT_F64) return set_err(ctx, CL_ERR_UNSUPPORTED, "element type has no kernel for this system");
+  if (elt != CL_ELT_F64 && elt != CL_ELT_F64_F32TAN)
+    return set_err(ctx, CL_ERR_UNSUPPORTED, "element type has no kernel for this system");
+  P.f32 = elt == CL_ELT_F64_F32TAN;
   if (!(ch->n == 1 && ch->r[0].kind == CL_RED_QR && ch->r[0].alg == CL_QR_HOUSEHOLDER && ch->r[0].frame == 0))
     return set_err(ctx, CL_ERR_UNSUPPORTED, "the mixed launch runs QRSpectrum(HouseholderQR())");
   if (k != 16) return set_err(ctx, CL_ERR_UNSUPPORTED, "the mixed launch is built for k = 16");
@@ -1036,7 +1040,7 @@ static int mixed_run(cl_ctx* ctx, MixedPlan& P, std::vector<MixedClass>& cls, ch
       a.counter = counters + c;
       a.n0 = (int64_t)c * len;
       a.steps = P.Nmax - a.n0 < len ? P.Nmax - a.n0 : len;
-      rc = mixed_launch(a, ring, chain_blocked(ctx), s);
+      rc = mixed_launch(a, ring, chain_blocked(ctx), P.f32, s);
     }
     return rc;
   };

@@ -144,7 +144,7 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
     reset_frame=True restarts the frame from the unit columns after every handoff (the indicator recorder's window
     kinds, src/Experiments/run_indicator_device.jl:66-80); W0 is then the unit columns.
     elt="f32tan" carries the frames in Float32 between handoffs (FP64 orbit and Jacobian, FP64 reducers: the
-    orthonormalisation and the compensated log sums), on the standard-map family. elt="dd" carries the orbit of the
+    orthonormalisation and the compensated log sums), on the standard-map family and the coupled chain. elt="dd" carries the orbit of the
     Henon-Heiles Gauss map as a compensated pair (value, correction), the reference's compensated summation; x0 may
     give the values only (the corrections start at 0) and the returned x has both halves."""
     import torch
@@ -194,12 +194,12 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
     if sysd.kind == L.CL_SYS_STANDARD_MAP_CHAIN and nws:
         raise L.HIPError(3, "chain kernels run no workspace reducer")
     if sysd.kind == L.CL_SYS_STANDARD_MAP_CHAIN:
-        if E != L.CL_ELT_F64:
-            raise L.HIPError(3, "chain kernels are built for Float64")
+        if E not in (L.CL_ELT_F64, L.CL_ELT_F64_F32TAN):
+            raise L.HIPError(3, "chain kernels are built for Float64 and Float32 tangents")
         if reducers2:
             raise L.HIPError(3, "chain kernels carry one frame")
         return _propagate_orbit_major(lib, ctx, sys, sysd, chain, allr, x0, N, k, tau, seed, W0, launch_steps,
-                                      words, check, keep_states)
+                                      words, check, keep_states, elt=elt)
     dx = torch.from_numpy(np.ascontiguousarray(x0.T)).to(dev)
     dW = torch.empty((d * k, m), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
@@ -255,9 +255,10 @@ def propagate(sys, x0, N, *, k, tau=1, seed, reducers=(), reducers2=(), W0=None,
 
 
 def _propagate_orbit_major(lib, ctx, sys, sysd, chain, allr, x0, N, k, tau, seed, W0, launch_steps, words, check,
-                           keep_states, Ns=None, order=None):
+                           keep_states, Ns=None, order=None, elt="f64"):
     """The chain system: device buffers in the orbit-major layout (x[i][s], W[i][k][d], states[i][words])."""
     import torch
+    E = ELEMENT_TYPES[elt]
     m, s = x0.shape
     d = sys.dimension
     dev = torch.device("cuda")
@@ -278,7 +279,7 @@ def _propagate_orbit_major(lib, ctx, sys, sysd, chain, allr, x0, N, k, tau, seed
     if Ns is not None:
         Na = np.ascontiguousarray(np.asarray(Ns, dtype=np.int64))
         oa = None if order is None else np.ascontiguousarray(np.asarray(order, dtype=np.int64))
-        rc = lib.cl_propagate_ragged(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, m, s, d, k,
+        rc = lib.cl_propagate_ragged(ctx, ctypes.byref(sysd), ctypes.byref(chain), E, m, s, d, k,
                                      Na.ctypes.data, None if oa is None else oa.ctypes.data, tau, dx.data_ptr(),
                                      dW.data_ptr(), dst.data_ptr(), dcode.data_ptr(), dh.data_ptr(), stream)
         if rc:
@@ -289,7 +290,7 @@ def _propagate_orbit_major(lib, ctx, sys, sysd, chain, allr, x0, N, k, tau, seed
         n0 = 0
         while n0 < N:
             steps = min(L_, N - n0)
-            rc = lib.cl_propagate_device(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, m, s, d, k, n0,
+            rc = lib.cl_propagate_device(ctx, ctypes.byref(sysd), ctypes.byref(chain), E, m, s, d, k, n0,
                                          steps, N, tau, dx.data_ptr(), dW.data_ptr(), None, dst.data_ptr(),
                                          dcode.data_ptr(), dh.data_ptr(), stream)
             if rc:
@@ -305,7 +306,7 @@ def _propagate_orbit_major(lib, ctx, sys, sysd, chain, allr, x0, N, k, tau, seed
     states = decode_states(raw, allr, d, k)
     if check:
         check_health(code, hstep, x, W, states, allr)
-    prov = dict(backend="gpu", device=device_name(), backend_fallback_reason="", precision="Float64",
+    prov = dict(backend="gpu", device=device_name(), backend_fallback_reason="", precision=PRECISION_LABELS[elt],
                 seed=seed, tau=tau, k=k, chunk_size=m, gpu_chunks=1, gpu_chunk_size=m, launch_steps=L_,
                 launches_per_chunk=launches)
     for r, _ in allr:
@@ -315,10 +316,10 @@ def _propagate_orbit_major(lib, ctx, sys, sysd, chain, allr, x0, N, k, tau, seed
 
 
 def propagate_ragged(sys, x0, Ns, *, k, tau=1, seed, reducers=(), W0=None, order=None, check=True,
-                     keep_states=False):
+                     keep_states=False, elt="f64"):
     """Chain ensembles with one step count per orbit (Ns[i], each a positive multiple of tau): every orbit is
     the same as a `propagate` call of its own length. `order` (a permutation) sets the launch order; by default
-    the longest orbits go first."""
+    the longest orbits go first. elt="f32tan": Float32 tangents (as in `propagate`)."""
     tau, k = int(tau), int(k)
     x0 = np.ascontiguousarray(np.asarray(x0, dtype=np.float64))
     m, s = x0.shape
@@ -333,22 +334,22 @@ def propagate_ragged(sys, x0, Ns, *, k, tau=1, seed, reducers=(), W0=None, order
     lib, ctx = _context()
     sysd = sys.descriptor()
     chain, allr = make_chain(reducers)
-    nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, d, k)
+    nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), ELEMENT_TYPES[elt], d, k)
     if nbytes < 0:
         raise L.HIPError(-nbytes, "no kernel instance for this system, k and reducer chain")
     if order is None:
         order = np.argsort(-Ns, kind="stable")
     return _propagate_orbit_major(lib, ctx, sys, sysd, chain, allr, x0, 0, k, tau, seed, W0, None, nbytes // 8,
-                                  check, keep_states, Ns=Ns, order=order)
+                                  check, keep_states, Ns=Ns, order=order, elt=elt)
 
 
 def propagate_mixed(lattices, *, K, eps, boundary="ring", k=16, tau=1, seed, reducers=None, chunk=0, check=True,
-                    keep_states=False, _noalloc=False):
+                    keep_states=False, _noalloc=False, elt="f64"):
     """Several coupled-chain lattices in one persistent launch per chunk (`cl_propagate_mixed`): `lattices` is a list of
     (N, x0, Ns), x0 of shape (m, 2N), Ns the per-orbit step counts (positive multiples of tau). All share K, eps and
     the boundary; the chain is QRSpectrum(HouseholderQR()) with k = 16. `chunk` is the step length of a chunk (0: one
     chunk). Seeded frames use orbit ids 1..m within each lattice, as `propagate_ragged` on that lattice alone, whose
-    results this gives bit for bit. Returns one PropagationResult per lattice."""
+    results this gives bit for bit. Returns one PropagationResult per lattice. elt="f32tan": Float32 tangents."""
     import torch
     from .systems import StandardMapChain
     from .reducers import QRSpectrum, HouseholderQR
@@ -369,7 +370,7 @@ def propagate_mixed(lattices, *, K, eps, boundary="ring", k=16, tau=1, seed, red
         if s != sys.state_length or Ns.shape != (m,) or np.any(Ns < 1) or np.any(Ns % tau):
             raise ValueError("each lattice needs x0 of shape (m, 2N) and one positive multiple of tau per orbit")
         sysd = sys.descriptor()
-        nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, d, k)
+        nbytes = lib.cl_state_bytes(ctypes.byref(sysd), ctypes.byref(chain), ELEMENT_TYPES[elt], d, k)
         if nbytes < 0:
             raise L.HIPError(-nbytes, "no kernel instance for this system, k and reducer chain")
         words = nbytes // 8
@@ -389,7 +390,8 @@ def propagate_mixed(lattices, *, K, eps, boundary="ring", k=16, tau=1, seed, red
         keep.append(Ns)
     sysd = StandardMapChain(max(8, int(lattices[0][0])), K, eps, boundary).descriptor()
     fn = lib.cl_propagate_mixed_noalloc if _noalloc else lib.cl_propagate_mixed
-    rc = fn(ctx, ctypes.byref(sysd), ctypes.byref(chain), L.CL_ELT_F64, len(lattices), lats, k, tau, int(chunk), stream)
+    rc = fn(ctx, ctypes.byref(sysd), ctypes.byref(chain), ELEMENT_TYPES[elt], len(lattices), lats, k, tau, int(chunk),
+            stream)
     if rc:
         raise L.HIPError(rc, lib.cl_last_error(ctx).decode())
     torch.cuda.synchronize()
@@ -402,7 +404,8 @@ def propagate_mixed(lattices, *, K, eps, boundary="ring", k=16, tau=1, seed, red
         states = decode_states(raw, allr, d, k)
         if check:
             check_health(code, hstep, x, W, states, allr)
-        prov = dict(backend="gpu", device=device_name(), backend_fallback_reason="", precision="Float64", seed=seed,
+        prov = dict(backend="gpu", device=device_name(), backend_fallback_reason="", precision=PRECISION_LABELS[elt],
+                    seed=seed,
                     tau=tau, k=k, chunk_size=m, gpu_chunks=1, gpu_chunk_size=m, launch_steps=int(chunk),
                     launches_per_chunk=1, lattices=len(lattices))
         for r, _ in allr:

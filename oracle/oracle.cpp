@@ -2289,19 +2289,22 @@ extern "C" {
 // The coupled standard-map chain (kind CL_SYS_STANDARD_MAP_CHAIN) with no reducer or QRSpectrum(HouseholderQR()),
 // k <= 16, full runs from fresh states. Buffers are orbit-major: x[i][2N], W[i][k][2N] (column-major frames),
 // states[i][16k + 72]. Ns (optional) gives each orbit its own step count (a multiple of tau), else N for all.
-// mode RED_ROWS (0) is the reference's row-order reductions, RED_SITES (1) the device kernel's.
+// mode RED_ROWS (0) is the reference's row-order reductions, RED_SITES (1) and RED_BLOCKED (2) the device kernels'.
+// elt CL_ELT_F64 (1) or CL_ELT_F64_F32TAN (2): Float32 tangents (chain_orbit_f32).
 int or_propagate_chain(const cl_system* sys, const cl_chain* ch, int64_t m, int k, const int64_t* Ns, int64_t N,
                        int tau, double* x, double* W, void* states, int8_t* code, int64_t* hstep, int math, int mode,
-                       int nthreads) {
+                       int nthreads, int elt) {
   if (sys->kind != CL_SYS_STANDARD_MAP_CHAIN || sys->N < 2 || sys->N > 128 || (sys->ring && sys->N < 3)) return 10;
   if (k < 1 || k > 16 || k > 2 * sys->N || tau < 1) return 10;
-  if (mode == RED_SITES && (sys->N & (sys->N - 1))) return 10;  // the butterfly needs a power-of-two site count
+  if (mode != RED_ROWS && (sys->N & (sys->N - 1))) return 10;  // the butterfly needs a power-of-two site count
+  if (elt != CL_ELT_F64 && elt != CL_ELT_F64_F32TAN) return 3;
+  const bool f32 = elt == CL_ELT_F64_F32TAN;
   if (ch->n > 1 || (ch->n == 1 && !(ch->r[0].kind == CL_RED_QR && ch->r[0].alg == CL_QR_HOUSEHOLDER))) return 3;
   for (int64_t i = 0; i < m; ++i)
     if ((Ns ? Ns[i] : N) % tau) return 10;
   if (math == MATH_CL)
-    return chain_all<F64Cl>(sys, ch, m, k, Ns, N, tau, x, W, (uint8_t*)states, code, hstep, mode, nthreads);
-  return chain_all<F64Libm>(sys, ch, m, k, Ns, N, tau, x, W, (uint8_t*)states, code, hstep, mode, nthreads);
+    return chain_all<F64Cl>(sys, ch, m, k, Ns, N, tau, x, W, (uint8_t*)states, code, hstep, mode, nthreads, f32);
+  return chain_all<F64Libm>(sys, ch, m, k, Ns, N, tau, x, W, (uint8_t*)states, code, hstep, mode, nthreads, f32);
 }
 
 // Same-orbit check of the standard map's QR spectrum (the AC4 method, test/tangent-core/tc_ac01_04.jl:66-88):

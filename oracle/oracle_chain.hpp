@@ -408,8 +408,11 @@ struct ChainSys {
   T K, eps;
   int left(int j) const { return j > 0 ? j - 1 : (ring ? N - 1 : -1); }
   int right(int j) const { return j < N - 1 ? j + 1 : (ring ? 0 : -1); }
-  // W <- ChainTangent(x) * W, then x <- forward(x); x = (q_1..q_N, p_1..p_N), W d x k column-major
-  void step(T* x, T* W, int k, std::vector<T>& tmp) const {
+  // W <- ChainTangent(x) * W, then x <- forward(x); x = (q_1..q_N, p_1..p_N), W d x k column-major. F is the frame's
+  // type: T, or float for Float32 tangents (the band's entries formed in T and rounded to float once, the product in
+  // float in the same operation order).
+  template <class F>
+  void step(T* x, F* W, int k, std::vector<F>& tmp) const {
     const int d = 2 * N;
     tmp.resize((size_t)d * k);
     const T* q = x;
@@ -419,11 +422,12 @@ struct ChainSys {
       T cl = kl >= 0 ? eps * P::cos(q[j] - q[kl]) : (T)0;
       T cr = kr >= 0 ? eps * P::cos(q[j] - q[kr]) : (T)0;
       T dg = (g + cl) + cr, gl = -cl, gr = -cr;
+      const F dgf = (F)dg, glf = (F)gl, grf = (F)gr;
       for (int c = 0; c < k; ++c) {
-        const T* Wc = W + (size_t)c * d;
-        T gw = dg * Wc[j];
-        if (kl >= 0) gw = gw + gl * Wc[kl];
-        if (kr >= 0) gw = gw + gr * Wc[kr];
+        const F* Wc = W + (size_t)c * d;
+        F gw = dgf * Wc[j];
+        if (kl >= 0) gw = gw + glf * Wc[kl];
+        if (kr >= 0) gw = gw + grf * Wc[kr];
         tmp[(size_t)c * d + N + j] = gw + Wc[N + j];
         tmp[(size_t)c * d + j] = (Wc[j] + gw) + Wc[N + j];
       }
@@ -444,7 +448,52 @@ struct ChainSys {
   }
 };
 
-// One orbit of the chain with the QR-spectrum chain (or no reducer), full run of Nsteps from fresh states.
+// One orbit of the chain with the QR-spectrum chain (or no reducer), full run of Nsteps from fresh states. Float32
+// tangents (f32): the frame is rounded to float once at the start and carried in float between handoffs; each
+// factorisation runs in T on the exactly widened frame and its Q is rounded back (the frame a failed factorisation
+// leaves is the widened one it was given).
+template <class P, class T>
+static int8_t chain_orbit_f32(const ChainSys<P, T>& sys, const cl_chain& ch, int k, int64_t Nsteps, int tau, T* x, T* W,
+                              ChainQRSt<T>& st, int64_t* hstep, int mode) {
+  const int d = 2 * sys.N;
+  const size_t dk = (size_t)d * k;
+  std::vector<float> Wf(dk), tmp;
+  for (size_t e = 0; e < dk; ++e) Wf[e] = (float)W[e];
+  auto status = [&]() -> int8_t {
+    for (int c = 0; c < d; ++c)
+      if (!P::finite(x[c])) return 1;
+    for (size_t e = 0; e < dk; ++e)
+      if (!std::isfinite(Wf[e])) return 2;
+    return 0;
+  };
+  auto widen = [&]() {
+    for (size_t e = 0; e < dk; ++e) W[e] = (T)Wf[e];
+  };
+  int64_t n = 0;
+  for (int64_t h = 0; h < Nsteps / tau; ++h) {
+    for (int t = 0; t < tau; ++t) sys.step(x, Wf.data(), k, tmp);
+    n += tau;
+    int8_t code = status();
+    if (code) {
+      widen();
+      *hstep = n;
+      return code;
+    }
+    if (ch.n > 0) {
+      widen();
+      chain_qr_step<P>(ch.r[0], st, n, W, sys.N, k, mode);
+      if (st.code) {
+        *hstep = n;
+        return 3;
+      }
+      for (size_t e = 0; e < dk; ++e) Wf[e] = (float)W[e];
+    }
+  }
+  widen();
+  *hstep = n;
+  return n == Nsteps ? status() : (int8_t)0;
+}
+
 template <class P, class T>
 static int8_t chain_orbit(const ChainSys<P, T>& sys, const cl_chain& ch, int k, int64_t Nsteps, int tau, T* x, T* W,
                           ChainQRSt<T>& st, int64_t* hstep, int mode) {
@@ -481,7 +530,7 @@ static int8_t chain_orbit(const ChainSys<P, T>& sys, const cl_chain& ch, int k, 
 template <class P>
 static int chain_all(const cl_system* sys, const cl_chain* ch, int64_t m, int k, const int64_t* Ns, int64_t N,
                      int tau, double* x, double* W, uint8_t* states, int8_t* code, int64_t* hstep, int mode,
-                     int nthreads) {
+                     int nthreads, bool f32 = false) {
   typedef typename P::T T;
   ChainSys<P, T> cs;
   cs.N = sys->N;
@@ -504,7 +553,8 @@ static int chain_all(const cl_system* sys, const cl_chain* ch, int64_t m, int k,
       for (int s = 0; s < 16; ++s) st.sums[s].s = st.sums[s].c = (T)0;
       st.G_max = st.orth_max = st.value = st.threshold = (T)0;
       int64_t Ni = Ns ? Ns[i] : N;
-      code[i] = chain_orbit<P, T>(cs, *ch, k, Ni, tau, xx.data(), ww.data(), st, &hstep[i], mode);
+      code[i] = f32 ? chain_orbit_f32<P, T>(cs, *ch, k, Ni, tau, xx.data(), ww.data(), st, &hstep[i], mode)
+                    : chain_orbit<P, T>(cs, *ch, k, Ni, tau, xx.data(), ww.data(), st, &hstep[i], mode);
       for (int c = 0; c < d; ++c) x[i * d + c] = dbl(xx[c]);
       for (size_t e = 0; e < (size_t)d * k; ++e) W[i * d * k + e] = dbl(ww[e]);
       if (states && sb) chain_write_qr(st, k, states + i * sb);

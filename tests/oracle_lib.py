@@ -45,7 +45,7 @@ def load():
         o.or_initial_deviations.argtypes = [ctypes.c_uint64, I64, I64, I, I, P]
         o.or_propagate_chain.restype = I
         o.or_propagate_chain.argtypes = [ctypes.POINTER(L.cl_system), ctypes.POINTER(L.cl_chain), I64, I, P, I64, I,
-                                         P, P, P, P, P, I, I, I]
+                                         P, P, P, P, P, I, I, I, I]
         o.or_same_orbit_quad.restype = I
         o.or_same_orbit_quad.argtypes = [ctypes.c_double, ctypes.POINTER(L.cl_chain), I64, I, I64, I, P, P, P, I]
         o.or_gauss_iterations.restype = None
@@ -120,8 +120,9 @@ def run(sysd, chain, x0, W0, N, tau, k, W2=None, math=MATH_CL, nthreads=None, qu
 RED_ROWS, RED_SITES, RED_BLOCKED = 0, 1, 2
 
 
-def run_chain(sysd, chain, x0, W0, N, tau, k, Ns=None, math=MATH_CL, mode=RED_SITES, nthreads=None):
-    """Chain runs from fresh states: x0 (m, 2N), W0 (m, 2N, k); Ns optional per-orbit step counts."""
+def run_chain(sysd, chain, x0, W0, N, tau, k, Ns=None, math=MATH_CL, mode=RED_SITES, nthreads=None, elt=1):
+    """Chain runs from fresh states: x0 (m, 2N), W0 (m, 2N, k); Ns optional per-orbit step counts; elt 1 Float64,
+    2 Float32 tangents."""
     o = load()
     x = np.ascontiguousarray(np.array(x0, dtype=np.float64))
     m, d = x.shape
@@ -133,7 +134,7 @@ def run_chain(sysd, chain, x0, W0, N, tau, k, Ns=None, math=MATH_CL, mode=RED_SI
     Na = None if Ns is None else np.ascontiguousarray(np.asarray(Ns, dtype=np.int64))
     nt = nthreads or min(16, os.cpu_count() or 1)
     rc = o.or_propagate_chain(ctypes.byref(sysd), ctypes.byref(chain), m, k, _ptr(Na), int(N), tau, _ptr(x), _ptr(W),
-                              _ptr(states), _ptr(code), _ptr(hstep), math, mode, nt)
+                              _ptr(states), _ptr(code), _ptr(hstep), math, mode, nt, int(elt))
     assert rc == 0, rc
     return dict(x=x, W=W.transpose(0, 2, 1).copy(), states=states[:, :nb], code=code, hstep=hstep)
 

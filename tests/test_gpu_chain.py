@@ -142,3 +142,41 @@ def test_chain_host_chunk_and_frames(order):
     o = O.run_chain(sd, chain, x0, hW.transpose(0, 2, 1), 100, 10, K, mode=order)
     assert np.array_equal(x, o["x"]) and np.array_equal(W.transpose(0, 2, 1), o["W"])
     assert np.array_equal(st.view(np.uint8).reshape(m, -1), o["states"])
+
+
+@pytest.mark.parametrize("ring", [True, False], ids=["ring", "open"])
+@pytest.mark.parametrize("N", [8, 16, 32, 64, 128])
+def test_chain_f32tan_bitwise(N, ring, order):
+    """Float32 tangents (elt f32tan): the Float64 orbit, the frame in Float32 between handoffs, each factorisation in
+    FP64 on the widened frame; bit for bit against the oracle's policy (elt 2), launch splits included."""
+    m = 7 if N <= 64 else 3
+    say("chain f32tan", N, ring, m, order)
+    sysm = StandardMapChain(N, 1.5, 0.1, "ring" if ring else "open")
+    x0 = starts(50 + N, m, N)
+    W0 = O.frames(9, 1, m, 2 * N, K)
+    r = (QRSpectrum(HouseholderQR()),)
+    g = propagate(sysm, x0, 300, k=K, tau=10, seed=9, reducers=r, W0=W0, keep_states=True, elt="f32tan")
+    o = O.run_chain(sysm.descriptor(), make_chain(r)[0], x0, W0, 300, 10, K, mode=order, elt=2)
+    assert np.array_equal(g.x, o["x"]) and np.array_equal(g.W, o["W"])
+    assert np.array_equal(g.states, o["states"])
+    assert np.array_equal(g.code, o["code"]) and np.array_equal(g.hstep, o["hstep"])
+    assert g.provenance["precision"] == "Float64 orbit, Float32 tangents"
+    if N in (8, 128):
+        s = propagate(sysm, x0, 300, k=K, tau=10, seed=9, reducers=r, W0=W0, keep_states=True, elt="f32tan",
+                      launch_steps=70)
+        assert np.array_equal(s.states, g.states) and np.array_equal(s.W, g.W)
+
+
+@pytest.mark.parametrize("N", [16, 128])
+def test_chain_f32tan_ragged(N, order):
+    say("chain f32tan ragged", N, order)
+    m = 9
+    sysm = StandardMapChain(N, 1.5, 0.1, "ring")
+    x0 = starts(12, m, N)
+    W0 = O.frames(12, 1, m, 2 * N, K)
+    Ns = 10 * np.random.default_rng(2).integers(3, 40, m)
+    r = (QRSpectrum(HouseholderQR()),)
+    g = propagate_ragged(sysm, x0, Ns, k=K, tau=10, seed=12, reducers=r, W0=W0, keep_states=True, elt="f32tan")
+    o = O.run_chain(sysm.descriptor(), make_chain(r)[0], x0, W0, 0, 10, K, Ns=Ns, mode=order, elt=2)
+    assert np.array_equal(g.x, o["x"]) and np.array_equal(g.W, o["W"])
+    assert np.array_equal(g.states, o["states"]) and np.array_equal(g.hstep, Ns)

@@ -108,3 +108,14 @@ def test_mixed_pairing_residual_n8():
     assert np.max(np.abs(lam.sum(axis=1))) < 1e-9
     pair = np.max(np.abs(lam + lam[:, ::-1]), axis=1)
     assert np.max(pair) < 16.0 / 2000 and np.mean(pair) < 4.0 / 2000
+
+
+def test_mixed_f32tan_equals_ragged(order):
+    """Float32 tangents in the persistent launch: every lattice equals its ragged kernel bit for bit."""
+    say("mixed f32tan", order)
+    lat = lattices(17, [(8, 9), (16, 7), (32, 5), (64, 3), (128, 2)], 30, 300, 10)
+    got = propagate_mixed(lat, K=1.5, eps=0.1, k=K, tau=10, seed=5, chunk=100, keep_states=True, elt="f32tan")
+    for (N, x0, Ns), g in zip(lat, got):
+        r = propagate_ragged(StandardMapChain(N, 1.5, 0.1, "ring"), x0, Ns, k=K, tau=10, seed=5, reducers=R,
+                             keep_states=True, elt="f32tan")
+        check_equal(g, r)

@@ -169,3 +169,31 @@ def test_blocked_order_against_row_order_and_planted_defects(N):
         with O.planted(defect):
             d1 = O.run_chain(sysd, QR[0], x0, W0, 10, 10, k, mode=O.RED_BLOCKED)
         assert np.any(d1["code"] != 0) or np.max(np.abs(d1["W"] - r1["W"])) > 1e-3, defect
+
+
+@pytest.mark.parametrize("N", [8, 32, 128])
+def test_f32_tangent_policy(N):
+    """Float32 tangents (elt 2): the orbit is the Float64 orbit bit for bit, the frames handed back hold Float32
+    values, the leading exponents stay within 1e-6 of the Float64 run on the same orbit and the whole spectrum within
+    1e-2 (the trailing columns carry the rounding of the frame amplified by the alignment over tau steps); at the
+    origin, with tau inside the reference's tau_max(Float32) = floor(ln(1/eps32) / (2 (lambda_1 - lambda_k)))
+    (tangent-core-spec.md:45-48), the spectrum is +-ln mu_j within 10/n as for Float64."""
+    k = 16
+    sysd = StandardMapChain(N, 1.5, 0.1, "ring").descriptor()
+    rng = np.random.default_rng(7)
+    x0 = rng.uniform(0, 2 * np.pi, (3, 2 * N))
+    W0 = O.frames(7, 1, 3, 2 * N, k)
+    for mode in (O.RED_BLOCKED, O.RED_SITES):
+        a = O.run_chain(sysd, QR[0], x0, W0, 3000, 10, k, mode=mode)
+        b = O.run_chain(sysd, QR[0], x0, W0, 3000, 10, k, mode=mode, elt=2)
+        assert np.array_equal(a["x"], b["x"]) and np.all(b["code"] == 0)
+        assert np.array_equal(b["W"], b["W"].astype(np.float32).astype(np.float64))
+        la, lb = lambdas(a, k), lambdas(b, k)
+        assert not np.array_equal(la, lb)  # the frames are rounded
+        assert np.max(np.abs(la[:, :2] - lb[:, :2])) < 1e-6
+        assert np.max(np.abs(la - lb)) < 1e-2
+    n, tau = 4000, 2 if N == 8 else 10  # N = 8, k = 2N: lambda_1 - lambda_16 = 2.3 at the origin, tau_max = 3
+    r = O.run_chain(sysd, QR[0], np.zeros((2, 2 * N)), O.frames(3, 1, 2, 2 * N, k), n, tau, k, mode=O.RED_BLOCKED,
+                    elt=2)
+    assert np.all(r["x"] == 0) and np.all(r["code"] == 0)
+    assert np.max(np.abs(lambdas(r, k) - origin_exponents(N, 1.5, 0.1, True)[:k])) < 10.0 / n
