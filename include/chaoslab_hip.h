@@ -44,7 +44,8 @@ extern "C" {
 
 /* element types: the orbit and frame arithmetic of a run */
 #define CL_ELT_F64 1         /* Float64 orbit and tangents */
-#define CL_ELT_F64_F32TAN 2  /* Float64 orbit, Float32 tangent frames, FP64 reorthonormalisation and log sums */
+#define CL_ELT_F64_F32TAN 2  /* Float64 orbit, Float32 tangent frames, FP64 reorthonormalisation and log sums
+                                (the standard-map family and the coupled chain) */
 #define CL_ELT_DD 3          /* Double64-equivalent orbit (double-double on fma), Float64 tangents */
 
 /* ------------------------------------------------------------------ system descriptors */
