@@ -739,6 +739,17 @@ def c5_run(a, cpu_leg=True):
                    sample=f"plain method on {len(xs)} of the triaged orbits x {N} steps, oracle/oracle.cpp with "
                           f"glibc sin/cos/log, {nthreads} threads, {tc:.1f} s")
     value = float(n_tri) * N / t_tri
+    # the dominant part of the triage is the continuation of the ambiguous orbits: QRSpectrum(MGS()) k = 2 at the C1
+    # census per orbit-step, over the wall time of that pass (host work included, so a lower bound); the network's
+    # training throughput beside it, forward + backward at 3 x 2 (768*256 + 2*256*256 + 256*2) flops per sample-epoch
+    cont_steps = float((1.0 - tri["confident"].mean()) * n_tri * (N - prefix))
+    a_rest = FLOPS_PER_ORBIT_STEP["c1"] * cont_steps / max(tri["t_rest"], 1e-9) / 1e12
+    mlp_flops = 3.0 * 2.0 * (768 * 256 + 2 * 256 * 256 + 256 * 2) * n_train * a.c5_epochs
+    a_mlp = mlp_flops / max(t_train, 1e-9) / 1e12
+    roof = {"bound": "fp64-valu", "achieved": a_rest, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": a_rest / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "continuation of the ambiguous orbits",
+            "mlp_training": {"bound": "mfma", "achieved": a_mlp, "peak": 2500.0, "unit": "TFLOP/s (bf16 dense)",
+                             "frac": a_mlp / 2500.0, "seconds": t_train}}
     out = {
         "metric": METRIC, "value": value, "unit": "orbit-steps/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": t_tri * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -753,7 +764,7 @@ def c5_run(a, cpu_leg=True):
                       "f32tan_prefix": {"speedup": t_plain / t_tri32, "triage_s": t_tri32,
                                         "prefix_s": tri32["t_prefix"], "misclassified_vs_plain": wrong32,
                                         "continued_fraction": float(1.0 - tri32["confident"].mean())}},
-        "roofline": None, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
@@ -1145,15 +1156,19 @@ def main():
         log("c1", {k: v for k, v in extra["c1"].items() if k != "cpu_baseline"})
         c5 = c5_run(a, cpu_leg=False)
         extra["c5"] = {"value": c5["value"], "unit": c5["unit"], "workload": c5["config"]["workload"],
-                       **c5["surrogate"]}
+                       **c5["surrogate"], "roofline": c5["roofline"]}
         log("c5", {k: extra["c5"][k] for k in ("speedup", "misclassified_vs_plain", "continued_fraction")})
         # the coupled chains (C3) on five streams in the library's QR order (the blocked one on the matrix cores)
         a3 = argparse.Namespace(**vars(a))
         a3.steps, a3.c3_mode, a3.c3_qr, a3.no_cpu, a3.c3_order, a3.c3_elt = 2, "streams", "auto", True, "ascending", "f64"
         c3 = c3_main(a3, emit=False)
+        lam8 = c3["_lambdas"][8]  # N = 8: k = 16 = 2N, the full spectrum
+        pair = np.max(np.abs(lam8 + lam8[:, ::-1]), axis=1)
         extra["c3"] = {"value": c3["value"], "unit": c3["unit"], "workload": c3["config"]["workload"],
                        "kernel_ms_per_N": c3["config"]["kernel_ms_per_N"], "roofline": c3["roofline"],
-                       "nonfinite_orbits": c3["config"]["nonfinite_orbits"]}
+                       "nonfinite_orbits": c3["config"]["nonfinite_orbits"],
+                       "pairing_n8": {"max": float(pair.max()), "median": float(np.median(pair)),
+                                      "sum_max": float(np.max(np.abs(lam8.sum(axis=1))))}}
         log("c3", {"value": c3["value"], "span_ms": c3["roofline"]["kernel_ms"], "frac": c3["roofline"]["frac"]})
         # the Float32-tangent variant of the same ensembles: rate and the same-orbit error of the spectrum
         a3.c3_elt = "f32tan"
