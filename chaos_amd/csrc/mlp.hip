@@ -6,7 +6,12 @@
 // GEMM: 256-thread groups own a 64 x 64 tile of C; four wave64s each own 32 x 32 as 2 x 2 tiles of
 // v_mfma_f32_16x16x32_bf16 (gfx950): lane l supplies A row (l mod 16) and B column (l mod 16), the eight
 // K values (l div 16) * 8 .. +7 of a 32-wide K step, and receives C rows 4 (l div 16) .. +3 of column
-// (l mod 16). Tiles of A and B are staged through LDS K-contiguous; edges are zero-filled.
+// (l mod 16). Tiles of A and B (64 x 64 K values each) are staged through LDS K-contiguous, read from HBM as
+// 16-byte vectors of eight values, and the next K tile's vectors are loaded into registers while the matrix
+// cores work on the current one; edges and unaligned operands fall back to guarded 2-byte loads and zeros.
+// A product with few output tiles and a long K (the weight gradient dW = dZ A^T, K = the batch) is split over K:
+// each split writes its FP32 partial tile to the context's workspace and a second kernel adds the partials in
+// split order, so the result does not depend on scheduling.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "../../include/chaoslab_hip.h"
@@ -16,6 +21,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 struct cl_ctx;
 extern "C" hipStream_t cl_ctx_stream(cl_ctx* ctx);
+extern "C" void* cl_ctx_mlp_workspace(cl_ctx* ctx, size_t bytes);
 
 namespace {
 
@@ -28,98 +34,142 @@ __device__ inline uint16_t f2bf(float f) {
   return (uint16_t)(u >> 16);
 }
 
-constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8;  // LDS rows padded by 8 bf16 (16 B)
+constexpr int BM = 64, BN = 64, BK = 64, LDK = BK + 8;  // LDS rows padded by 8 bf16 (16 B)
+
+// Eight consecutive bf16 values p[0..7] as one 16-byte vector: a vector load when all eight are in range and
+// `vec` (16-byte aligned rows), else guarded scalar loads with zeros past `n` valid values.
+__device__ inline uint4 ld8(const uint16_t* __restrict__ p, int n, bool vec) {
+  if (vec && n >= 8) return *reinterpret_cast<const uint4*>(p);
+  uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int e = 0; e < 8; ++e)
+    if (e < n) w[e >> 1] |= (uint32_t)p[e] << ((e & 1) * 16);
+  return uint4{w[0], w[1], w[2], w[3]};
+}
+
+// One operand's share of a K tile: two vectors per thread. K-contiguous (A untransposed, B transposed): vector
+// v = tid + 256 i covers row v / 8, K values (v mod 8) * 8 .. +7. Otherwise (stored K x rows): K value v / 8,
+// rows (v mod 8) * 8 .. +7.
+template <bool KCONTIG>
+__device__ inline void tile_load(uint4 (&r)[2], const uint16_t* __restrict__ X, int ld, int rows, int r0, int k0,
+                                 int kend, bool vec, int tid) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int v = tid + 256 * i;
+    if (KCONTIG) {
+      const int row = r0 + (v >> 3), gk = k0 + (v & 7) * 8;
+      const int n = row < rows ? kend - gk : 0;
+      r[i] = ld8(X + (int64_t)(row < rows ? row : 0) * ld + gk, n, vec);
+    } else {
+      const int gk = k0 + (v >> 3), row = r0 + (v & 7) * 8;
+      const int n = gk < kend ? rows - row : 0;
+      r[i] = ld8(X + (int64_t)(gk < kend ? gk : k0) * ld + row, n, vec);
+    }
+  }
+}
+
+template <bool KCONTIG>
+__device__ inline void tile_store(uint16_t* __restrict__ S, const uint4 (&r)[2], int tid) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int v = tid + 256 * i;
+    if (KCONTIG) {
+      *reinterpret_cast<uint4*>(S + (v >> 3) * LDK + (v & 7) * 8) = r[i];
+    } else {
+      const int kk = v >> 3, row = (v & 7) * 8;
+      const uint32_t w[4] = {r[i].x, r[i].y, r[i].z, r[i].w};
+#pragma unroll
+      for (int e = 0; e < 8; ++e) S[(row + e) * LDK + kk] = (uint16_t)(w[e >> 1] >> ((e & 1) * 16));
+    }
+  }
+}
+
+// EPI: 0 plain, 1 bias[m] then ReLU, 2 bias[m], 3 multiply by (H(m, n) > 0) (ReLU backward).
+template <int EPI, bool OUT_F32>
+__device__ inline void epi_store(float x, int gm, int gn, void* __restrict__ C, int ldc, const float* __restrict__ bias,
+                                 const uint16_t* __restrict__ H, int ldh) {
+  if (EPI == 1 || EPI == 2) x += bias[gm];
+  if (EPI == 1) x = x > 0.f ? x : 0.f;
+  if (EPI == 3) x = bf2f(H[(int64_t)gm * ldh + gn]) > 0.f ? x : 0.f;
+  if (OUT_F32)
+    reinterpret_cast<float*>(C)[(int64_t)gm * ldc + gn] = x;
+  else
+    reinterpret_cast<uint16_t*>(C)[(int64_t)gm * ldc + gn] = f2bf(x);
+}
 
 // C(m, n) = sum_k A(m, k) B(k, n); A(m, k) = TA ? A[k lda + m] : A[m lda + k]; B(k, n) = TB ? B[n ldb + k] : B[k ldb + n].
-// EPI: 0 plain, 1 bias[m] then ReLU, 2 bias[m], 3 multiply by (H(m, n) > 0) (ReLU backward).
-template <bool TA, bool TB, int EPI, bool OUT_F32>
-__global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
-                                              const uint16_t* __restrict__ B, int ldb, void* __restrict__ C, int ldc,
-                                              const float* __restrict__ bias, const uint16_t* __restrict__ H, int ldh) {
+// Split z = blockIdx.z sums k in [z kc, min(K, (z + 1) kc)); with SPLIT its raw FP32 tile goes to
+// ws[z M N + m N + n], else the epilogue writes C.
+template <bool TA, bool TB, int EPI, bool OUT_F32, bool SPLIT>
+__global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, int kc, const uint16_t* __restrict__ A, int lda,
+                                              int vec_a, const uint16_t* __restrict__ B, int ldb, int vec_b,
+                                              void* __restrict__ C, int ldc, const float* __restrict__ bias,
+                                              const uint16_t* __restrict__ H, int ldh, float* __restrict__ ws) {
   __shared__ __attribute__((aligned(16))) uint16_t As[BM * LDK];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LDK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int bm = blockIdx.y * BM, bn = blockIdx.x * BN;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int kbeg = blockIdx.z * kc, kend = min(K, kbeg + kc);
   floatx4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    // stage A: 64 x 32 values, 8 per thread
-    {
-      if (!TA) {  // row-major M x K: a thread reads 8 consecutive k of one row
-        int r = tid >> 2, kk = (tid & 3) * 8;
-        int gm = bm + r;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          int gk = k0 + kk + e;
-          As[r * LDK + kk + e] = (gm < M && gk < K) ? A[(int64_t)gm * lda + gk] : (uint16_t)0;
-        }
-      } else {  // stored K x M: a thread reads 8 consecutive m of one k
-        int kk = tid >> 3, r = (tid & 7) * 8;
-        int gk = k0 + kk;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          int gm = bm + r + e;
-          As[(r + e) * LDK + kk] = (gm < M && gk < K) ? A[(int64_t)gk * lda + gm] : (uint16_t)0;
-        }
-      }
-      if (!TB) {  // row-major K x N: 8 consecutive n of one k
-        int kk = tid >> 3, c = (tid & 7) * 8;
-        int gk = k0 + kk;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          int gn = bn + c + e;
-          Bs[(c + e) * LDK + kk] = (gn < N && gk < K) ? B[(int64_t)gk * ldb + gn] : (uint16_t)0;
-        }
-      } else {  // stored N x K: 8 consecutive k of one n
-        int c = tid >> 2, kk = (tid & 3) * 8;
-        int gn = bn + c;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          int gk = k0 + kk + e;
-          Bs[c * LDK + kk + e] = (gn < N && gk < K) ? B[(int64_t)gn * ldb + gk] : (uint16_t)0;
-        }
-      }
-    }
+  uint4 ra[2], rb[2];
+  tile_load<!TA>(ra, A, lda, M, bm, kbeg, kend, vec_a != 0, tid);
+  tile_load<TB>(rb, B, ldb, N, bn, kbeg, kend, vec_b != 0, tid);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    tile_store<!TA>(As, ra, tid);
+    tile_store<TB>(Bs, rb, tid);
     __syncthreads();
-    const int kq = (lane >> 4) * 8;
-    bf16x8 af[2], bfr[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const uint16_t* pa = As + (wm + i * 16 + (lane & 15)) * LDK + kq;
-      const uint16_t* pb = Bs + (wn + i * 16 + (lane & 15)) * LDK + kq;
-      af[i] = *reinterpret_cast<const bf16x8*>(pa);
-      bfr[i] = *reinterpret_cast<const bf16x8*>(pb);
+    if (k0 + BK < kend) {  // the next tile's loads stay in flight over this tile's products
+      tile_load<!TA>(ra, A, lda, M, bm, k0 + BK, kend, vec_a != 0, tid);
+      tile_load<TB>(rb, B, ldb, N, bn, k0 + BK, kend, vec_b != 0, tid);
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int ks = 0; ks < BK; ks += 32) {
+      const int kq = ks + (lane >> 4) * 8;
+      bf16x8 af[2], bfr[2];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+      for (int i = 0; i < 2; ++i) {
+        af[i] = *reinterpret_cast<const bf16x8*>(As + (wm + i * 16 + (lane & 15)) * LDK + kq);
+        bfr[i] = *reinterpret_cast<const bf16x8*>(Bs + (wn + i * 16 + (lane & 15)) * LDK + kq);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
     __syncthreads();
   }
-  // epilogue
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        int gm = bm + wm + i * 16 + (lane >> 4) * 4 + v;
-        int gn = bn + wn + j * 16 + (lane & 15);
+        const int gm = bm + wm + i * 16 + (lane >> 4) * 4 + v;
+        const int gn = bn + wn + j * 16 + (lane & 15);
         if (gm >= M || gn >= N) continue;
-        float x = acc[i][j][v];
-        if (EPI == 1 || EPI == 2) x += bias[gm];
-        if (EPI == 1) x = x > 0.f ? x : 0.f;
-        if (EPI == 3) x = bf2f(H[(int64_t)gm * ldh + gn]) > 0.f ? x : 0.f;
-        if (OUT_F32)
-          reinterpret_cast<float*>(C)[(int64_t)gm * ldc + gn] = x;
+        if (SPLIT)
+          ws[((int64_t)blockIdx.z * M + gm) * N + gn] = acc[i][j][v];
         else
-          reinterpret_cast<uint16_t*>(C)[(int64_t)gm * ldc + gn] = f2bf(x);
+          epi_store<EPI, OUT_F32>(acc[i][j][v], gm, gn, C, ldc, bias, H, ldh);
       }
+}
+
+// C(m, n) = epilogue(sum over splits z = 0, 1, ... of ws[z M N + m N + n]), added in split order.
+template <int EPI, bool OUT_F32>
+__global__ void __launch_bounds__(256) k_split_sum(int M, int N, int S, const float* __restrict__ ws,
+                                                   void* __restrict__ C, int ldc, const float* __restrict__ bias,
+                                                   const uint16_t* __restrict__ H, int ldh) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x, mn = (int64_t)M * N;
+  if (i >= mn) return;
+  float x = ws[i];
+  for (int z = 1; z < S; ++z) x += ws[z * mn + i];
+  epi_store<EPI, OUT_F32>(x, (int)(i / N), (int)(i % N), C, ldc, bias, H, ldh);
 }
 
 // Row sums of an M x N bf16 matrix in FP32 (the bias gradient); one 256-thread group per row.
@@ -235,11 +285,37 @@ __global__ void k_f32_to_bf16(int64_t n, const float* __restrict__ s, uint16_t* 
   if (i < n) d[i] = f2bf(s[i]);
 }
 
+// Splits over K for an M x N x K product with fewer output tiles than half the compute units of a 256-CU device:
+// enough groups for two per compute unit, each split at least 256 K values, split lengths a multiple of BK.
+// Depends on the shape only.
+void split_plan(int M, int N, int K, int* S, int* kc) {
+  const int64_t tiles = (int64_t)((N + BN - 1) / BN) * ((M + BM - 1) / BM);
+  int64_t s = tiles >= 128 ? 1 : (512 + tiles - 1) / tiles;
+  if (s > K / 256) s = K / 256;
+  if (s < 1) s = 1;
+  int64_t c = (K + s - 1) / s;
+  c = (c + BK - 1) / BK * BK;
+  *kc = (int)c;
+  *S = (int)((K + c - 1) / c);
+}
+
+bool vec_ok(const uint16_t* p, int ld) { return ld % 8 == 0 && ((uintptr_t)p & 15) == 0; }
+
 template <bool TA, bool TB, int EPI, bool F32>
-void gemm_launch(int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B, int ldb, void* C, int ldc,
-                 const float* bias, const uint16_t* H, int ldh, hipStream_t st) {
-  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
-  hipLaunchKernelGGL((k_gemm<TA, TB, EPI, F32>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, bias, H, ldh);
+void gemm_launch(int M, int N, int K, int S, int kc, float* ws, const uint16_t* A, int lda, const uint16_t* B, int ldb,
+                 void* C, int ldc, const float* bias, const uint16_t* H, int ldh, hipStream_t st) {
+  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, S);
+  const int va = vec_ok(A, lda), vb = vec_ok(B, ldb);
+  if (S == 1) {
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI, F32, false>), grid, dim3(256), 0, st, M, N, K, kc, A, lda, va, B, ldb, vb,
+                       C, ldc, bias, H, ldh, (float*)nullptr);
+    return;
+  }
+  hipLaunchKernelGGL((k_gemm<TA, TB, 0, true, true>), grid, dim3(256), 0, st, M, N, K, kc, A, lda, va, B, ldb, vb,
+                     (void*)nullptr, 0, (const float*)nullptr, (const uint16_t*)nullptr, 0, ws);
+  const int64_t mn = (int64_t)M * N;
+  hipLaunchKernelGGL((k_split_sum<EPI, F32>), dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, M, N, S, ws, C,
+                     ldc, bias, H, ldh);
 }
 
 int rc_of(hipError_t e) { return e == hipSuccess ? CL_OK : (e == hipErrorOutOfMemory ? CL_ERR_OOM : CL_ERR_DEVICE_LOST); }
@@ -253,20 +329,27 @@ extern "C" int cl_mlp_gemm(cl_ctx* ctx, int transA, int transB, int M, int N, in
   if ((epi == 1 || epi == 2) && !bias) return CL_ERR_ARGUMENT;
   if (epi == 3 && !H) return CL_ERR_ARGUMENT;
   hipStream_t st = stream ? (hipStream_t)stream : cl_ctx_stream(ctx);
-#define DISPATCH(TA, TB)                                                                                         \
-  if (transA == TA && transB == TB) {                                                                            \
-    if (c_f32) {                                                                                                 \
-      if (epi == 0) gemm_launch<TA, TB, 0, true>(M, N, K, A, lda, B, ldb, C, ldc, bias, H, ldh, st);             \
-      else if (epi == 1) gemm_launch<TA, TB, 1, true>(M, N, K, A, lda, B, ldb, C, ldc, bias, H, ldh, st);        \
-      else if (epi == 2) gemm_launch<TA, TB, 2, true>(M, N, K, A, lda, B, ldb, C, ldc, bias, H, ldh, st);        \
-      else gemm_launch<TA, TB, 3, true>(M, N, K, A, lda, B, ldb, C, ldc, bias, H, ldh, st);                      \
-    } else {                                                                                                     \
-      if (epi == 0) gemm_launch<TA, TB, 0, false>(M, N, K, A, lda, B, ldb, C, ldc, bias, H, ldh, st);            \
-      else if (epi == 1) gemm_launch<TA, TB, 1, false>(M, N, K, A, lda, B, ldb, C, ldc, bias, H, ldh, st);       \
-      else if (epi == 2) gemm_launch<TA, TB, 2, false>(M, N, K, A, lda, B, ldb, C, ldc, bias, H, ldh, st);       \
-      else gemm_launch<TA, TB, 3, false>(M, N, K, A, lda, B, ldb, C, ldc, bias, H, ldh, st);                     \
-    }                                                                                                            \
-    return rc_of(hipGetLastError());                                                                             \
+  int S = 1, kc = K;
+  split_plan(M, N, K, &S, &kc);
+  float* ws = nullptr;
+  if (S > 1) {
+    ws = (float*)cl_ctx_mlp_workspace(ctx, (size_t)S * M * N * sizeof(float));
+    if (!ws) return CL_ERR_OOM;
+  }
+#define DISPATCH(TA, TB)                                                                                              \
+  if (transA == TA && transB == TB) {                                                                                 \
+    if (c_f32) {                                                                                                      \
+      if (epi == 0) gemm_launch<TA, TB, 0, true>(M, N, K, S, kc, ws, A, lda, B, ldb, C, ldc, bias, H, ldh, st);       \
+      else if (epi == 1) gemm_launch<TA, TB, 1, true>(M, N, K, S, kc, ws, A, lda, B, ldb, C, ldc, bias, H, ldh, st);  \
+      else if (epi == 2) gemm_launch<TA, TB, 2, true>(M, N, K, S, kc, ws, A, lda, B, ldb, C, ldc, bias, H, ldh, st);  \
+      else gemm_launch<TA, TB, 3, true>(M, N, K, S, kc, ws, A, lda, B, ldb, C, ldc, bias, H, ldh, st);                \
+    } else {                                                                                                          \
+      if (epi == 0) gemm_launch<TA, TB, 0, false>(M, N, K, S, kc, ws, A, lda, B, ldb, C, ldc, bias, H, ldh, st);      \
+      else if (epi == 1) gemm_launch<TA, TB, 1, false>(M, N, K, S, kc, ws, A, lda, B, ldb, C, ldc, bias, H, ldh, st); \
+      else if (epi == 2) gemm_launch<TA, TB, 2, false>(M, N, K, S, kc, ws, A, lda, B, ldb, C, ldc, bias, H, ldh, st); \
+      else gemm_launch<TA, TB, 3, false>(M, N, K, S, kc, ws, A, lda, B, ldb, C, ldc, bias, H, ldh, st);               \
+    }                                                                                                                 \
+    return rc_of(hipGetLastError());                                                                                  \
   }
   DISPATCH(0, 0)
   DISPATCH(1, 0)

@@ -359,7 +359,10 @@ void cl_gw_host(int64_t n, const double *R, const double *G_max, double *quick, 
 
 /* C = epi(op(A) op(B)): op(A) is M x K (A stored K x M when transA), op(B) is K x N (B stored N x K when
  * transB); BF16 inputs, FP32 accumulation on the matrix cores. epi: 0 none, 1 + bias[m] then ReLU,
- * 2 + bias[m], 3 zero where H[m][n] <= 0 (ReLU backward). C is FP32 when c_f32, else BF16 (round to nearest even). */
+ * 2 + bias[m], 3 zero where H[m][n] <= 0 (ReLU backward). C is FP32 when c_f32, else BF16 (round to nearest even).
+ * A shape with few 64 x 64 output tiles and a long K is split over K: the FP32 partials go to a device buffer the
+ * context owns (grown on demand; CL_ERR_OOM if that fails) and are added in split order, so results depend on the
+ * shape only. Calls of one context that split K must not run concurrently on different streams. */
 int cl_mlp_gemm(cl_ctx *ctx, int transA, int transB, int M, int N, int K, const uint16_t *A, int lda,
                 const uint16_t *B, int ldb, void *C, int ldc, int c_f32, const float *bias, int epi,
                 const uint16_t *H, int ldh, void *stream);

@@ -27,7 +27,9 @@ def host_bf16(t):
 
 
 @pytest.mark.parametrize("tA,tB", [(0, 0), (1, 0), (0, 1), (1, 1)])
-@pytest.mark.parametrize("shape", [(64, 64, 32), (70, 100, 50), (2, 256, 256), (256, 4096, 768), (768, 300, 96)])
+# (256, 768, 4096), (2, 256, 4096) and (70, 100, 1000) are split over K (the last with a partial final split)
+@pytest.mark.parametrize("shape", [(64, 64, 32), (70, 100, 50), (2, 256, 256), (256, 4096, 768), (768, 300, 96),
+                                   (256, 768, 4096), (2, 256, 4096), (70, 100, 1000)])
 def test_gemm_against_numpy(tA, tB, shape):
     import torch
     from chaos_amd.surrogate import SurrogateMLP
