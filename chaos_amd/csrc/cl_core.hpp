@@ -58,14 +58,21 @@ CL_HD double safe_norm(const double* v, int st = 1) {
   double m = cl_fabs(v[0]);
 #pragma unroll
   for (int i = 1; i < N; ++i) m = __builtin_fmax(m, cl_fabs(v[i * st]));
-  if (!(cl_isfinite(m) && m > 0.0)) return m;
   if (N == 2) {
     // the larger entry over m is exactly +-1, its square exactly 1: one division instead of two, and
     // 1 + b^2 equals b^2 + 1, so the sum is the same in either order
-    double other = cl_fabs(v[0]) >= cl_fabs(v[st]) ? v[st] : v[0];
-    double b = other / m;
+    const double other = cl_fabs(v[0]) >= cl_fabs(v[st]) ? v[st] : v[0];
+    if (m >= 0x1p-400 && m <= 0x1p400) {
+      // |b| <= 1: the plain division is x / m wherever |b| >= 2^-400, and a smaller |b| (or a zero of either sign)
+      // leaves 1 + b^2 = 1 either way
+      const double b = cl_div_plain(other, m);
+      return m * cl_sqrt12(1.0 + b * b);
+    }
+    if (!(cl_isfinite(m) && m > 0.0)) return m;
+    const double b = other / m;
     return m * cl_sqrt12(1.0 + b * b);
   }
+  if (!(cl_isfinite(m) && m > 0.0)) return m;
   double a = v[0] / m;
   double s = a * a;
 #pragma unroll
@@ -879,7 +886,10 @@ CL_HD MegnoSums megno_add(MegnoSums m, double a, int64_t n) {
   o.s = csum_add(m.s, a);
   o.R1 = R1;
   o.R2 = R2;
-  o.Q = csum_add(m.Q, 2.0 * csum_total(R1) / tn);
+  // 2 R1 / n by the plain division: every term n a added to R1 is 0 or a multiple of 2^-105 of magnitude at least
+  // 2^-53 (a is a log: never -0 and never below 2^-53 in magnitude unless 0), so R1 is +0 or at least 2^-105 in
+  // magnitude and the quotient lies in the division's exact range
+  o.Q = csum_add(m.Q, cl_div_plain(2.0 * csum_total(R1), tn));
   o.Q2 = csum_add(m.Q2, 3.0 * csum_total(R2));
   return o;
 }
@@ -954,7 +964,7 @@ struct SALI {
     double d_minus = safe_norm<D>(dm), d_plus = safe_norm<D>(dp);
     st.s1 = csum_add(st.s1, c.l1);
     st.s2 = csum_add(st.s2, c.l2);
-    st.sali = cl_jmin(d_minus, d_plus);
+    st.sali = cl_min_num(d_minus, d_plus);  // two norms: non-negative, never NaN or -0
     st.n = n;
 #pragma unroll
     for (int i = 0; i < D; ++i) {
@@ -1100,7 +1110,9 @@ struct OFLI {
     }
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      if (st.code == 0 && !(cl_isfinite(al[j]) && al[j] > 0.0)) {
+      // with the shared column norms their finiteness flags are this test's outcome
+      const bool okj = (K == 2 && lc) ? (j == 0 ? lc->ok1 : lc->ok2) : (cl_isfinite(al[j]) && al[j] > 0.0);
+      if (st.code == 0 && !okj) {
         st.value = al[j]; st.step = n; st.code = 1; st.column = (int8_t)(j + 1);
       }
     }
@@ -1115,9 +1127,11 @@ struct OFLI {
       CSum na = csum_add(st.acc[i], la);
       double g = be[i] > 0.0 ? csum_total(st.acc[i]) + cl_log(be[i]) : -__builtin_inf();
       double sn = be[i] / al[i], S = csum_total(na);
-      sinmin = cl_jmin(sinmin, sn);
+      // sn >= +0 and g (finite or -inf, never -0: no log here returns -0 and the sums start at +0) are never NaN,
+      // nor are the running values they meet, so Julia's min and max are the hardware's
+      sinmin = cl_min_num(sinmin, sn);
       nminf += be[i] > 0.0 ? 0 : 1;
-      inst = cl_jmax(inst, g);
+      inst = cl_max_num(inst, g);
       if (g > gB) { gB = g; stepB = n; colB = (int8_t)(i + 1); sinB = sn; }
       if (i == 0 && g > g1) { g1 = g; step1 = n; sin1 = sn; }
       if (i == 0 && S > f1) f1 = S;

@@ -112,6 +112,30 @@ CL_HD double cl_jmax(double x, double y) {
   const int takey = (y > x) | (cl_signbit(y) < cl_signbit(x));
   return takey ? (x != x ? x : y) : (y != y ? y : x);
 }
+/* Julia's min and max where neither operand is NaN and the pair is never -0.0 against +0.0: there they equal IEEE
+ * minNum/maxNum, one hardware instruction. */
+CL_HD double cl_min_num(double x, double y) { return __builtin_fmin(x, y); }
+CL_HD double cl_max_num(double x, double y) { return __builtin_fmax(x, y); }
+
+/* x / y for y in [2^-400, 2^400] and x with |x / y| in [2^-400, 2^400] or x = +0: the device division's arithmetic
+ * (the hardware reciprocal, two Newton steps, the product and one fused residual correction) without the operand
+ * range scaling and special-value fix-up steps, which are identities on these operands, so the quotient is the
+ * correctly rounded one, bit for bit. Outside that range (a smaller nonzero quotient, a -0 numerator) the result may
+ * differ from x / y; callers use it only where such a quotient cannot change what they compute. */
+CL_HD double cl_div_plain(double x, double y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(y);
+  double e = __builtin_fma(-y, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-y, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  const double q = x * r;
+  const double d = __builtin_fma(-y, q, x);
+  return __builtin_fma(d, r, q);
+#else
+  return x / y;
+#endif
+}
 
 /* ---------------------------------------------------------------- error-free transforms */
 /* s + e = a + b exactly (Knuth's TwoSum, no ordering assumption). */
@@ -335,7 +359,7 @@ CL_HD double cl_log_normal(double x, int k0) {
     k += 1;
   }
   double f = m - 1.0;
-  double s = f / (2.0 + f);
+  double s = cl_div_plain(f, 2.0 + f); /* f is +0 or |f| >= 2^-53, 2 + f in [1.7, 2.5] */
   double z = s * s;
   double R = z * (0x1.5555555555558p-1 + z * (0x1.999999999529cp-2 + z * (0x1.2492492df8420p-2 +
              z * (0x1.c71c62dd6bbfcp-3 + z * (0x1.7462b6c858cc6p-3 + z * (0x1.39fe1f2d19f7ap-3 +
@@ -347,16 +371,10 @@ CL_HD double cl_log_normal(double x, int k0) {
 }
 
 CL_HD double cl_log(double x) {
-  /* one test for the usual case, a positive finite x (a subnormal one scaled by 2^54 first); then the special
-   * values: NaN gives x, x < 0 NaN, +-0 -inf, +inf +inf */
-  if (x > 0.0 && x <= 0x1.fffffffffffffp+1023) {
-    int k0 = 0;
-    if (x < 0x1p-1022) { /* subnormal */
-      x = x * 0x1p54;
-      k0 = -54;
-    }
-    return cl_log_normal(x, k0);
-  }
+  /* one test for the usual case, a positive normal finite x; then a positive subnormal x, scaled by 2^54 first, and
+   * the special values: NaN gives x, x < 0 NaN, +-0 -inf, +inf +inf */
+  if (x >= 0x1p-1022 && x <= 0x1.fffffffffffffp+1023) return cl_log_normal(x, 0);
+  if (x > 0.0 && x < 0x1p-1022) return cl_log_normal(x * 0x1p54, -54);
   if (cl_isnan(x)) return x;
   if (x < 0.0) return __builtin_nan("");
   if (x == 0.0) return -__builtin_inf();
