@@ -724,7 +724,7 @@ CL_HD double orth_loss(const Frame<D, K>& Q) {
 #pragma unroll
       for (int l = 1; l < D; ++l) acc = cl_fma(Q.at(l, i), Q.at(l, j), acc);
       double e = cl_fabs(i == j ? acc - 1.0 : acc - 0.0);
-      mx = first ? e : cl_jmax(mx, e);
+      mx = first ? e : cl_max_nonneg(mx, e);  // absolute values: cl_jmax's result
       first = false;
     }
   return mx;
@@ -818,7 +818,7 @@ struct QRSpectrum {
 #pragma unroll
     for (int i = 0; i < K; ++i) st.sums[i] = csum_add(st.sums[i], cl_log(f.R[i * K + i]));
     st.n = n;
-    st.orth_max = cl_jmax(st.orth_max, orth);
+    st.orth_max = cl_max_nonneg(st.orth_max, orth);  // both >= +0 or NaN: cl_jmax's result
     st.count_max = st.count_max > f.count ? st.count_max : f.count;
     st.column = 0;
     W = f.Q;
@@ -1125,7 +1125,11 @@ struct OFLI {
     for (int i = 0; i < K; ++i) {
       double la = (K == 2 && lc) ? (i == 0 ? lc->l1 : lc->l2) : cl_log(al[i]);
       CSum na = csum_add(st.acc[i], la);
-      double g = be[i] > 0.0 ? csum_total(st.acc[i]) + cl_log(be[i]) : -__builtin_inf();
+      // a positive normal be (the usual case) goes straight to the log's arithmetic
+      const double tot = csum_total(st.acc[i]);
+      double g;
+      if (cl_is_pos_normal(be[i])) g = tot + cl_log_normal(be[i], 0);
+      else g = be[i] > 0.0 ? tot + cl_log(be[i]) : -__builtin_inf();
       double sn = be[i] / al[i], S = csum_total(na);
       // sn >= +0 and g (finite or -inf, never -0: no log here returns -0 and the sums start at +0) are never NaN,
       // nor are the running values they meet, so Julia's min and max are the hardware's

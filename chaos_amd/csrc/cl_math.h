@@ -116,6 +116,11 @@ CL_HD double cl_jmax(double x, double y) {
  * minNum/maxNum, one hardware instruction. */
 CL_HD double cl_min_num(double x, double y) { return __builtin_fmin(x, y); }
 CL_HD double cl_max_num(double x, double y) { return __builtin_fmax(x, y); }
+/* Julia's max of two values that are each >= +0 or NaN (absolute values and their running maxima), NaN included:
+ * such doubles are ordered as the unsigned integers of their bit patterns, with every NaN pattern above +inf's, so
+ * one 64-bit compare and a select replace cl_jmax's tests. (Two NaNs of different bit patterns could give the other
+ * NaN than cl_jmax; a device kernel's NaNs all carry the one default pattern.) */
+CL_HD double cl_max_nonneg(double x, double y) { return cl_bits(y) > cl_bits(x) ? y : x; }
 
 /* x / y for y in [2^-400, 2^400] and x with |x / y| in [2^-400, 2^400] or x = +0: the device division's arithmetic
  * (the hardware reciprocal, two Newton steps, the product and one fused residual correction) without the operand
@@ -370,10 +375,19 @@ CL_HD double cl_log_normal(double x, int k0) {
   return dk * CL_LN2_HI + (f - (hfsq - lo));
 }
 
+/* x is positive, normal and finite (cl_log_normal's domain); on the device one class test */
+CL_HD int cl_is_pos_normal(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_class(x, 0x100);
+#else
+  return x >= 0x1p-1022 && x <= 0x1.fffffffffffffp+1023;
+#endif
+}
+
 CL_HD double cl_log(double x) {
   /* one test for the usual case, a positive normal finite x; then a positive subnormal x, scaled by 2^54 first, and
    * the special values: NaN gives x, x < 0 NaN, +-0 -inf, +inf +inf */
-  if (x >= 0x1p-1022 && x <= 0x1.fffffffffffffp+1023) return cl_log_normal(x, 0);
+  if (cl_is_pos_normal(x)) return cl_log_normal(x, 0);
   if (x > 0.0 && x < 0x1p-1022) return cl_log_normal(x * 0x1p54, -54);
   if (cl_isnan(x)) return x;
   if (x < 0.0) return __builtin_nan("");
@@ -381,8 +395,6 @@ CL_HD double cl_log(double x) {
   return x;
 }
 
-/* x is positive, normal and finite (cl_log_normal's domain) */
-CL_HD int cl_is_pos_normal(double x) { return x >= 0x1p-1022 && x <= 0x1.fffffffffffffp+1023; }
 
 /* ---------------------------------------------------------------- atan2 */
 /* atan2(y, x) in (-pi, pi]: u = min(|x|, |y|) / max(|x|, |y|) in [0, 1]; with c = k/8 (k = rint(8u)),
