@@ -73,6 +73,14 @@ class cl_tuning(ctypes.Structure):
                 ("chain_qr", ctypes.c_int32)]
 
 
+CL_MLP_MAX_LAYERS = 8
+
+
+class cl_mlp_net(ctypes.Structure):
+    _fields_ = [("nlayers", ctypes.c_int32), ("sizes", ctypes.c_int32 * (CL_MLP_MAX_LAYERS + 1))] + [
+        (nm, ctypes.c_void_p * CL_MLP_MAX_LAYERS) for nm in ("W", "Wb", "b", "mW", "vW", "mb", "vb")]
+
+
 class cl_probe_t(ctypes.Structure):
     _fields_ = [("loadable", ctypes.c_int32), ("functional", ctypes.c_int32), ("compiles", ctypes.c_int32),
                 ("gpu_faster", ctypes.c_int32), ("device", ctypes.c_char * 64)]
@@ -132,6 +140,8 @@ SIGNATURES = {
     "cl_mlp_normalize": (I, [P, I, I64, P, P, P, P, P]),
     "cl_mlp_feature_stats": (I, [P, I, I64, P, P, P, P]),
     "cl_f32_to_bf16": (I, [P, I64, P, P, P]),
+    "cl_mlp_train_step": (I, [P, P, P, I, I, P, P, P, ctypes.c_float, I, ctypes.c_float, ctypes.c_float,
+                              ctypes.c_float, ctypes.c_float, P, P]),
 }
 
 _lib = None

@@ -22,6 +22,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 struct cl_ctx;
 extern "C" hipStream_t cl_ctx_stream(cl_ctx* ctx);
 extern "C" void* cl_ctx_mlp_workspace(cl_ctx* ctx, size_t bytes);
+extern "C" void* cl_ctx_mlp_step_buffer(cl_ctx* ctx, size_t bytes);
 
 namespace {
 
@@ -280,6 +281,15 @@ __global__ void __launch_bounds__(256) k_feature_stats(int F, int64_t Bn, const 
   }
 }
 
+// loss_sum[c] += (sum over the groups of partial[2 g + c], in double) / B: the step's batch-mean loss terms.
+__global__ void k_loss_accum(int ng, int Bn, const float* __restrict__ partial, double* __restrict__ loss_sum) {
+  if (threadIdx.x >= 2) return;
+  const int c = threadIdx.x;
+  double s = 0.0;
+  for (int g = 0; g < ng; ++g) s += (double)partial[2 * g + c];
+  loss_sum[c] += s / (double)Bn;
+}
+
 __global__ void k_f32_to_bf16(int64_t n, const float* __restrict__ s, uint16_t* __restrict__ d) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = f2bf(s[i]);
@@ -406,4 +416,89 @@ extern "C" int cl_f32_to_bf16(cl_ctx* ctx, int64_t n, const float* src, uint16_t
   hipStream_t st = stream ? (hipStream_t)stream : cl_ctx_stream(ctx);
   hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, src, dst);
   return rc_of(hipGetLastError());
+}
+
+// One training step in the order of the host-driven calls (cl_mlp_gemm forward with bias and ReLU, the last layer in
+// FP32; cl_mlp_loss; per layer from the last: the weight gradient, the bias gradient, the masked input gradient;
+// cl_mlp_adam on every weight matrix then its bias), so the parameters after it are those calls' bit for bit. The
+// intermediates live in a buffer the context owns (grown on demand).
+extern "C" int cl_mlp_train_step(cl_ctx* ctx, const cl_mlp_net* net, const uint16_t* X, int ldx, int Bn,
+                                 const float* y, const float* t, const float* w, float wreg, int step, float lr,
+                                 float b1, float b2, float eps, double* loss_sum, void* stream) {
+  if (!ctx || !net || !X || !y || !t || !w || !loss_sum || Bn <= 0 || ldx < Bn || step < 1) return CL_ERR_ARGUMENT;
+  const int L = net->nlayers;
+  if (L < 1 || L > CL_MLP_MAX_LAYERS) return CL_ERR_ARGUMENT;
+  int maxin = 0;
+  for (int l = 0; l <= L; ++l)
+    if (net->sizes[l] <= 0) return CL_ERR_ARGUMENT;
+  for (int l = 0; l < L; ++l) {
+    if (!net->W[l] || !net->Wb[l] || !net->b[l] || !net->mW[l] || !net->vW[l] || !net->mb[l] || !net->vb[l])
+      return CL_ERR_ARGUMENT;
+    if (l > 0 && net->sizes[l] > maxin) maxin = net->sizes[l];
+  }
+  if (net->sizes[L] != 2) return CL_ERR_ARGUMENT;  // the loss reads the logit and the predicted exponent
+  hipStream_t st = stream ? (hipStream_t)stream : cl_ctx_stream(ctx);
+  // buffer plan: activations, output, output gradient, loss partials, weight and bias gradients, two input gradients
+  const int ng = (Bn + 255) / 256;
+  size_t off = 0;
+  auto take = [&off](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  size_t o_act[CL_MLP_MAX_LAYERS], o_gw[CL_MLP_MAX_LAYERS], o_gb[CL_MLP_MAX_LAYERS];
+  for (int l = 0; l + 1 < L; ++l) o_act[l] = take((size_t)net->sizes[l + 1] * Bn * 2);
+  const size_t o_out = take((size_t)2 * Bn * 4), o_dout = take((size_t)2 * Bn * 2), o_part = take((size_t)2 * ng * 4);
+  for (int l = 0; l < L; ++l) {
+    o_gw[l] = take((size_t)net->sizes[l + 1] * net->sizes[l] * 4);
+    o_gb[l] = take((size_t)net->sizes[l + 1] * 4);
+  }
+  const size_t o_da0 = take((size_t)maxin * Bn * 2), o_da1 = take((size_t)maxin * Bn * 2);
+  char* base = (char*)cl_ctx_mlp_step_buffer(ctx, off);
+  if (!base) return CL_ERR_OOM;
+  auto bf = [base](size_t o) { return (uint16_t*)(base + o); };
+  auto f32 = [base](size_t o) { return (float*)(base + o); };
+  int rc = CL_OK;
+  // forward
+  const uint16_t* A = X;
+  int lda = ldx;
+  for (int l = 0; l < L && !rc; ++l) {
+    const int M = net->sizes[l + 1], K = net->sizes[l];
+    if (l == L - 1) {
+      rc = cl_mlp_gemm(ctx, 0, 0, M, Bn, K, net->Wb[l], K, A, lda, f32(o_out), Bn, 1, net->b[l], 2, nullptr, 0, st);
+    } else {
+      rc = cl_mlp_gemm(ctx, 0, 0, M, Bn, K, net->Wb[l], K, A, lda, bf(o_act[l]), Bn, 0, net->b[l], 1, nullptr, 0, st);
+      A = bf(o_act[l]);
+      lda = Bn;
+    }
+  }
+  if (!rc) rc = cl_mlp_loss(ctx, Bn, f32(o_out), y, t, w, wreg, bf(o_dout), f32(o_part), st);
+  if (!rc) {
+    hipLaunchKernelGGL(k_loss_accum, dim3(1), dim3(64), 0, st, ng, Bn, f32(o_part), loss_sum);
+    rc = rc_of(hipGetLastError());
+  }
+  // backward
+  const uint16_t* dZ = bf(o_dout);
+  for (int l = L - 1; l >= 0 && !rc; --l) {
+    const int M = net->sizes[l + 1], K = net->sizes[l];
+    const uint16_t* Ap = l == 0 ? X : bf(o_act[l - 1]);
+    const int ldp = l == 0 ? ldx : Bn;
+    rc = cl_mlp_gemm(ctx, 0, 1, M, K, Bn, dZ, Bn, Ap, ldp, f32(o_gw[l]), K, 1, nullptr, 0, nullptr, 0, st);
+    if (!rc) rc = cl_mlp_rowsum(ctx, M, Bn, dZ, Bn, f32(o_gb[l]), st);
+    if (!rc && l > 0) {
+      uint16_t* dA = bf(((L - 1 - l) & 1) ? o_da1 : o_da0);  // never the buffer dZ is read from
+      rc = cl_mlp_gemm(ctx, 1, 0, K, Bn, M, net->Wb[l], K, dZ, Bn, dA, Bn, 0, nullptr, 3, bf(o_act[l - 1]), Bn, st);
+      dZ = dA;
+    }
+  }
+  // Adam, weights then bias of each layer
+  for (int l = 0; l < L && !rc; ++l) {
+    const int64_t nw = (int64_t)net->sizes[l + 1] * net->sizes[l];
+    rc = cl_mlp_adam(ctx, nw, net->W[l], net->Wb[l], f32(o_gw[l]), net->mW[l], net->vW[l], step, lr, b1, b2, eps, 0.f,
+                     st);
+    if (!rc)
+      rc = cl_mlp_adam(ctx, net->sizes[l + 1], net->b[l], nullptr, f32(o_gb[l]), net->mb[l], net->vb[l], step, lr, b1,
+                       b2, eps, 0.f, st);
+  }
+  return rc;
 }

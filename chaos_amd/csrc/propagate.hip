@@ -857,12 +857,18 @@ extern "C" int cl_propagate_device_traced(cl_ctx* ctx, const cl_system* sys, con
   return CL_OK;
 }
 
-enum { B_X, B_W, B_W2, B_ST, B_CODE, B_H, B_XA, B_WA, B_W2A, B_STA, B_WS, B_WSA, B_MLP };
+enum { B_X, B_W, B_W2, B_ST, B_CODE, B_H, B_XA, B_WA, B_W2A, B_STA, B_WS, B_WSA, B_MLP, B_MLPS };
 
 // The surrogate GEMM's split-K partial sums (mlp.hip): one buffer per context, grown on demand, so GEMMs of one
 // context that split K must not overlap (the surrogate issues them in order on one stream). Null on failure.
 extern "C" __attribute__((visibility("hidden"))) void* cl_ctx_mlp_workspace(cl_ctx* c, size_t bytes) {
   return grow(c, B_MLP, bytes) == CL_OK ? c->buf[B_MLP] : nullptr;
+}
+
+// The surrogate training step's activations, gradients and loss partials (cl_mlp_train_step): one buffer per
+// context, grown on demand; steps of one context run in order on one stream. Null on failure.
+extern "C" __attribute__((visibility("hidden"))) void* cl_ctx_mlp_step_buffer(cl_ctx* c, size_t bytes) {
+  return grow(c, B_MLPS, bytes) == CL_OK ? c->buf[B_MLPS] : nullptr;
 }
 
 extern "C" int cl_propagate_ragged(cl_ctx* ctx, const cl_system* sys, const cl_chain* ch, int elt, int64_t m, int s,

@@ -164,6 +164,30 @@ class SurrogateMLP:
                                         self.mb[l].data_ptr(), self.vb[l].data_ptr(), self.t, lr, b1, b2, eps, 0.0, st),
                    self.lib, self.ctx)
 
+    def _net(self):
+        """The cl_mlp_net descriptor of this network's device tensors (their addresses do not change)."""
+        if getattr(self, "_netd", None) is None:
+            n = L.cl_mlp_net()
+            n.nlayers = self.nlayers
+            for i, v in enumerate(self.sizes):
+                n.sizes[i] = v
+            for l in range(self.nlayers):
+                for nm, lst in (("W", self.W), ("Wb", self.Wb), ("b", self.b), ("mW", self.mW), ("vW", self.vW),
+                                ("mb", self.mb), ("vb", self.vb)):
+                    getattr(n, nm)[l] = lst[l].data_ptr()
+            self._netd = n
+        return self._netd
+
+    def train_step(self, X, ldx, Bn, y, t, w, wreg, lr, loss_sum, b1=0.9, b2=0.999, eps=1e-8):
+        """One step on the Bn columns at X (a device address, row stride ldx) with targets at y, t, w (device
+        addresses): the library's cl_mlp_train_step, the same kernels as forward, loss, backward and adam called one
+        by one, in one call. loss_sum (device FP64, 2) gains the batch-mean loss terms."""
+        import torch
+        self.t += 1
+        _check(self.lib.cl_mlp_train_step(self.ctx, ctypes.byref(self._net()), X, ldx, Bn, y, t, w, wreg, self.t, lr,
+                                          b1, b2, eps, loss_sum.data_ptr(), torch.cuda.current_stream().cuda_stream),
+               self.lib, self.ctx)
+
     def set_normalisation(self, trace, Bn):
         import torch
         F = trace.shape[0]
@@ -182,10 +206,13 @@ class SurrogateMLP:
                self.lib, self.ctx)
         return X
 
-    def train(self, X, y, t, w, epochs=4, batch=4096, lr=1e-3, wreg=0.1, seed=0, log=None, lr_final=0.05):
+    def train(self, X, y, t, w, epochs=4, batch=4096, lr=1e-3, wreg=0.1, seed=0, log=None, lr_final=0.05,
+              native=True):
         """Mini-batch Adam over the columns of X (bf16, 768 x n); y, t, w FP32 device vectors. The step size follows a
         cosine from lr down to lr_final * lr over the run, so the last epochs settle instead of jumping between
-        minibatch minima (the network's confidence, and with it the triage's skipped fraction, depends on it)."""
+        minibatch minima (the network's confidence, and with it the triage's skipped fraction, depends on it).
+        `native` runs each step as one cl_mlp_train_step call; False issues its kernels one call at a time (the
+        same parameters bit for bit)."""
         import torch
         n = X.shape[1]
         rng = np.random.default_rng(seed)
@@ -198,14 +225,19 @@ class SurrogateMLP:
             tot = torch.zeros(2, dtype=torch.float64, device="cuda")
             nb = 0
             for s in range(0, n - batch + 1, batch):
-                xb = Xp[:, s:s + batch].contiguous()
-                acts, out = self.forward(xb, batch)
-                dout, part = self.loss(out, yp[s:s + batch], tp[s:s + batch], wp[s:s + batch], wreg)
-                gW, gb = self.backward(xb, batch, acts, dout)
                 cos = 0.5 * (1.0 + math.cos(math.pi * it / total))
-                self.adam(gW, gb, lr=lr * (lr_final + (1.0 - lr_final) * cos))
+                step_lr = lr * (lr_final + (1.0 - lr_final) * cos)
+                if native:  # one library call per step; the batch is a column range of Xp (row stride n)
+                    self.train_step(Xp.data_ptr() + 2 * s, n, batch, yp.data_ptr() + 4 * s, tp.data_ptr() + 4 * s,
+                                    wp.data_ptr() + 4 * s, wreg, step_lr, tot)
+                else:
+                    xb = Xp[:, s:s + batch].contiguous()
+                    acts, out = self.forward(xb, batch)
+                    dout, part = self.loss(out, yp[s:s + batch], tp[s:s + batch], wp[s:s + batch], wreg)
+                    gW, gb = self.backward(xb, batch, acts, dout)
+                    self.adam(gW, gb, lr=step_lr)
+                    tot += part.view(-1, 2).double().sum(0) / batch
                 it += 1
-                tot += part.view(-1, 2).sum(0).double() / batch
                 nb += 1
             torch.cuda.synchronize()
             l = (tot / max(nb, 1)).cpu().numpy()

@@ -383,6 +383,30 @@ int cl_mlp_feature_stats(cl_ctx *ctx, int F, int64_t B, const float *trace, floa
 /* dst = bf16(src), n values. */
 int cl_f32_to_bf16(cl_ctx *ctx, int64_t n, const float *src, uint16_t *dst, void *stream);
 
+/* A dense ReLU network for cl_mlp_train_step: layer l maps sizes[l] inputs to sizes[l + 1] outputs with FP32 master
+ * weights W[l] (sizes[l+1] x sizes[l], row-major), their BF16 copy Wb[l], bias b[l] and the Adam moments of both. */
+#define CL_MLP_MAX_LAYERS 8
+typedef struct cl_mlp_net {
+  int32_t nlayers;
+  int32_t sizes[CL_MLP_MAX_LAYERS + 1];
+  float *W[CL_MLP_MAX_LAYERS];
+  uint16_t *Wb[CL_MLP_MAX_LAYERS];
+  float *b[CL_MLP_MAX_LAYERS];
+  float *mW[CL_MLP_MAX_LAYERS];
+  float *vW[CL_MLP_MAX_LAYERS];
+  float *mb[CL_MLP_MAX_LAYERS];
+  float *vb[CL_MLP_MAX_LAYERS];
+} cl_mlp_net;
+
+/* One training step on the B columns of X (sizes[0] x B, row stride ldx): forward (bias + ReLU, the last layer FP32
+ * with sizes[nlayers] = 2), the loss of cl_mlp_loss, backward, and Adam (step >= 1, no weight decay) on every weight
+ * matrix and bias; loss_sum[0..1] (device, FP64) gains the batch-mean BCE and regression terms. The same kernels in
+ * the same order as the separate calls, so the parameters equal theirs bit for bit. The intermediates use a device
+ * buffer the context owns (grown on demand; CL_ERR_OOM if that fails): steps of one context must not overlap. */
+int cl_mlp_train_step(cl_ctx *ctx, const cl_mlp_net *net, const uint16_t *X, int ldx, int B, const float *y,
+                      const float *t, const float *w, float wreg, int step, float lr, float b1, float b2, float eps,
+                      double *loss_sum, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -169,3 +169,28 @@ def test_training_reduces_loss_on_separable_data():
     hist = net.train(dev_bf16(X), torch.from_numpy(y).cuda(), torch.from_numpy(t).cuda(),
                      torch.ones(n, device="cuda"), epochs=3, batch=2048, lr=1e-3)
     assert hist[-1][0] < 0.5 * hist[0][0]
+
+
+def test_native_step_equals_one_call_per_kernel():
+    """cl_mlp_train_step leaves every parameter, BF16 copy and Adam moment bit for bit where the separate calls do,
+    on batches taken as column ranges of a wider matrix (row stride > batch), and its loss sums match theirs."""
+    import torch
+    from chaos_amd.surrogate import SurrogateMLP
+    say("native training step")
+    rng = np.random.default_rng(11)
+    n = 3 * 2048 + 512
+    X = rng.standard_normal((768, n)).astype(np.float32)
+    s = rng.standard_normal(768) @ X
+    y = torch.from_numpy((s > 0).astype(np.float32)).cuda()
+    t = torch.from_numpy((s / 30).astype(np.float32)).cuda()
+    w = torch.from_numpy(rng.random(n).astype(np.float32) + 0.5).cuda()
+    Xd = dev_bf16(X)
+    nets = [SurrogateMLP(seed=9), SurrogateMLP(seed=9)]
+    hist = [net.train(Xd, y, t, w, epochs=2, batch=2048, lr=2e-3, native=nat) for net, nat in zip(nets, (True, False))]
+    a, b = nets
+    for l in range(a.nlayers):
+        for nm in ("W", "Wb", "b", "mW", "vW", "mb", "vb"):
+            x, z = getattr(a, nm)[l].cpu().numpy(), getattr(b, nm)[l].cpu().numpy()
+            assert x.tobytes() == z.tobytes(), (nm, l)
+    assert a.t == b.t == 6
+    np.testing.assert_allclose(np.array(hist[0]), np.array(hist[1]), rtol=1e-6)
