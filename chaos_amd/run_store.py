@@ -425,13 +425,12 @@ def chunk_files(store):
 def classify_store(store, chunk_hashes, panel_sha256):
     """Files of the store by kind (run_resume.jl:24-44): `keep` {ordinal: path} (complete, chunk hash and panel hash
     equal to the plan's, not a pilot), `halted` (kept but written by a control-bound stop) and `stale` (everything
-    else, including unreadable files and ordinals beyond the plan). `chunk_hashes` maps ordinal 1..M to the plan's
-    chunk hash."""
+    else, including unreadable files and ordinals the plan does not have). `chunk_hashes` maps each ordinal of the
+    plan (1..M) to its chunk hash."""
     keep, halted, stale = {}, {}, []
-    M = len(chunk_hashes)
     for f in chunk_files(store):
         ordn = int(_CHUNK_RE.match(os.path.basename(f)).group(1))
-        if not 1 <= ordn <= M:
+        if ordn not in chunk_hashes:
             stale.append(f)
             continue
         try:
@@ -492,7 +491,7 @@ def pending_chunks(store, chunk_hashes, panel_sha256, stale="error"):
         for f in st:
             move_aside(store, f, "stale")
             moved.append(os.path.basename(f))
-    pending = [o for o in range(1, len(chunk_hashes) + 1) if o not in keep]
+    pending = sorted(o for o in chunk_hashes if o not in keep)
     return pending, keep, removed, moved
 
 
