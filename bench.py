@@ -254,6 +254,30 @@ def flops_per_orbit_step(wl, cpu):
     return 780.0 * it + 2300.0 + 75.0
 
 
+def c4_block(a):
+    """C4 in the default line: one timed launch of the Henon-Heiles workload (196608 orbits x 1e5 steps) with the FP64
+    orbit and one with the compensated (double-double) orbit, their energy drift per energy and the census roofline
+    (Gauss iterations per step at the oracle's measured mean)."""
+    wl = workload("c4")
+    log("c4 block:", wl["desc"])
+    g = gpu_run(wl, 1, 1)
+    ms = float(np.mean(g["kernel_ms"]))
+    wdd = dict(wl)
+    wdd["x0"] = np.concatenate([wl["x0"], np.zeros_like(wl["x0"])], axis=1)
+    gdd = gpu_run(wdd, 1, 1, elt=3)
+    ms_dd = float(np.mean(gdd["kernel_ms"]))
+    fpos = flops_per_orbit_step(wl, None)
+    achieved = fpos * g["orbit_steps"] / (ms * 1e-3) / 1e12
+    return {"value": g["orbit_steps"] / g["wall"], "unit": "orbit-steps/s", "workload": wl["desc"], "kernel_ms": ms,
+            "nonfinite_orbits": g["bad"],
+            "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "flops_per_orbit_step": fpos,
+                         "gauss_iterations_per_step": C4_ITERATIONS_DEFAULT},
+            "dd": {"value": gdd["orbit_steps"] / gdd["wall"], "kernel_ms": ms_dd, "over_f64": ms / ms_dd,
+                   "nonfinite_orbits": gdd["bad"]},
+            "energy_drift": {"f64": c4_energy_drift(wl, g["x_final"]), "dd": c4_energy_drift(wl, gdd["x_final"][:, :4])}}
+
+
 def c4_energy_drift(wl, xf):
     """|H(z_N) - H(z_0)| of the Henon-Heiles launch per energy: max and rms over its orbits (the last timed launch ran
     all N steps from the workload's starts; non-finite orbits excluded and counted)."""
@@ -1082,7 +1106,7 @@ def main():
     ap.add_argument("--cpu-orbits", type=int, default=32768, help="orbits per K in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="c2: skip the Float32-tangent, accuracy, C1, C5 and C3 blocks of the default run")
+                    help="c2: skip the Float32-tangent, accuracy, C1, C5, C3 and C4 blocks of the default run")
     ap.add_argument("--c3-qr", default="auto", choices=["auto", "blocked", "sites"],
                     help="c3: the HouseholderQR order (auto: the library's rule, the blocked one on the matrix cores)")
     ap.add_argument("--c3-elt", default="f64", choices=["f64", "f32tan"],
@@ -1184,6 +1208,8 @@ def main():
                                  "nonfinite_orbits": c3f["config"]["nonfinite_orbits"],
                                  "lambda_error_vs_f64_same_orbit": err}
         log("c3 f32tan", {k: v for k, v in extra["c3"]["f32tan"].items() if k != "kernel_ms_per_N"})
+        extra["c4"] = c4_block(a)
+        log("c4", {k: extra["c4"][k] for k in ("value", "kernel_ms", "energy_drift")})
     budget = {"c2": a.cpu_orbits, "c1": 4096, "c4": 2048}[wl["name"]]
     cpu = None if a.no_cpu else cpu_baseline(wl, budget)
     if cpu is not None:
