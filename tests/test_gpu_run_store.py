@@ -54,18 +54,20 @@ class _Kill(Exception):
     pass
 
 
-@pytest.mark.parametrize("wrapped", [False, True], ids=["lifted", "wrapped"])
-def test_device_records_through_the_store_and_resume(tmp_path, wrapped):
+@pytest.mark.parametrize("wrapped,method", [(False, "L4"), (True, "L4"), (True, "L2")],
+                         ids=["lifted-L4", "wrapped-L4", "wrapped-L2"])
+def test_device_records_through_the_store_and_resume(tmp_path, wrapped, method):
     try:
         h5.load()
     except h5.H5Error as e:
         pytest.skip(f"no HDF5 library: {e}")
-    K, m, N, method = 0.97, 96, 3000, "L4"
+    K, m, N = 0.97, 96, 3000
     sysm = WrappedStandardMap(K) if wrapped else StandardMap(K)
     z = torus_uniform(11, m)
     x0 = np.array([torus_state(p) for p in z]) if wrapped else z
+    opts = {"eps": "1e-8"} if method == "L2" else {}
     say("record", "wrapped" if wrapped else "lifted", m, "orbits", N, "steps", method)
-    rec = record(sysm, x0, N, method=method, tau=1)
+    rec = record(sysm, x0, N, method=method, tau=1, **opts)
     ids = np.arange(1, m + 1)
     half = m // 2
     chunks = {1: (np.arange(half), ids[:half]), 2: (np.arange(half, m), ids[half:])}
@@ -81,6 +83,7 @@ def test_device_records_through_the_store_and_resume(tmp_path, wrapped):
             assert o.t.tobytes() == u["t"].tobytes() and o.lyap[method].tobytes() == u["lyap"].tobytes()
             assert o.state.tobytes() == np.ascontiguousarray(u["state"]).tobytes()
             assert (o.winding is None) == (not wrapped)
+            assert sorted(o.diagnostics) == sorted(RS.diagnostic_names(method))
             if wrapped:
                 assert o.winding.tobytes() == u["winding"].tobytes()
     say("chunk files equal the device rows")
@@ -97,7 +100,7 @@ def test_device_records_through_the_store_and_resume(tmp_path, wrapped):
     pending, keep, removed, moved = RS.pending_chunks(st, hashes, "panel")
     assert pending == [2] and removed == ["chunk-00002.h5.tmp"] and moved == []
     rows2, oids2 = chunks[2]
-    rec2 = record(sysm, x0[rows2], N, method=method, tau=1)
+    rec2 = record(sysm, x0[rows2], N, method=method, tau=1, **opts)
     RS.write_chunk(RS.chunk_path(st, 2), _root(2, "c2"), _chunk(rec2, np.arange(len(rows2)), oids2, method, wrapped, K))
     for c in hashes:
         assert _canon(RS.chunk_path(st, c)) == _canon(RS.chunk_path(ref, c))
