@@ -50,7 +50,9 @@ FLOPS_PER_ORBIT_STEP = {"c2": 500.0, "c1": 174.0}
 # (indicator pass 24792.9 / 590963.3 KiB, QR pass 24693.1 / 164881.8 KiB); summaries under profiles/.
 TRAFFIC_BYTES_PER_LAUNCH = {"c2": (2 * (24792.88 + 24693.06) + 590963.25 + 164881.81) * 1024,
                             # the ind workload's bk launch at 2^20 orbits (58419373.6 / 226179433.0 KiB)
-                            "ind_bk": (2 * 58419373.56 + 226179433.0) * 1024}
+                            "ind_bk": (2 * 58419373.56 + 226179433.0) * 1024,
+                            # the c4 launch (5.2041e8 / 1.7745e9 KiB): the Gauss kernel's register spills reach HBM
+                            "c4": (2 * 5.2041e8 + 1.7745e9) * 1024}
 # Mean Gauss fixed-point iterations per step on the c4 workload, measured with the oracle on its sample
 # (used when the CPU leg is skipped).
 C4_ITERATIONS_DEFAULT = 6.3
@@ -271,7 +273,9 @@ def c4_block(a):
     return {"value": g["orbit_steps"] / g["wall"], "unit": "orbit-steps/s", "workload": wl["desc"], "kernel_ms": ms,
             "nonfinite_orbits": g["bad"],
             "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "flops_per_orbit_step": fpos,
+                         "frac": achieved / FP64_PEAK_TFLOPS, "traffic": TRAFFIC_BYTES_PER_LAUNCH["c4"],
+                         "traffic_note": "register spills of the Gauss kernel (about 1.2e8 B algorithmic)",
+                         "flops_per_orbit_step": fpos,
                          "gauss_iterations_per_step": C4_ITERATIONS_DEFAULT},
             "dd": {"value": gdd["orbit_steps"] / gdd["wall"], "kernel_ms": ms_dd, "over_f64": ms / ms_dd,
                    "nonfinite_orbits": gdd["bad"]},
