@@ -167,3 +167,40 @@ def test_chain_state_bytes_and_frames():
         W = np.empty((3, 16, d))
         assert lib.cl_initial_deviations(9, 1, 3, d, 16, W.ctypes.data) == 0
         assert np.array_equal(W.transpose(0, 2, 1), O.frames(9, 1, 3, d, 16))
+
+
+def test_mlp_train_step_validation_errors():
+    """cl_mlp_train_step rejects a null context and a malformed network descriptor with code 10 before it touches the
+    context or the device (the context below is a zeroed stand-in that the checks never reach)."""
+    lib = L.load()
+    X = np.zeros((4, 8), np.uint16)
+    v = np.zeros(8, np.float32)
+    s = np.zeros(2, np.float64)
+    ptrs = [np.zeros(64, np.float32) for _ in range(7)]
+
+    def net(nlayers, sizes, null_slot=None):
+        n = L.cl_mlp_net()
+        n.nlayers = nlayers
+        for i, z in enumerate(sizes):
+            n.sizes[i] = z
+        for l in range(L.CL_MLP_MAX_LAYERS):
+            for j, nm in enumerate(("W", "Wb", "b", "mW", "vW", "mb", "vb")):
+                getattr(n, nm)[l] = None if (l, nm) == null_slot else ptrs[j].ctypes.data
+        return n
+
+    fake = ctypes.create_string_buffer(4096)
+
+    def call(ctx, n, Bn=8, ldx=8, step=1):
+        return lib.cl_mlp_train_step(ctx, ctypes.byref(n), X.ctypes.data, ldx, Bn, v.ctypes.data, v.ctypes.data,
+                                     v.ctypes.data, 0.1, step, 1e-3, 0.9, 0.999, 1e-8, s.ctypes.data, None)
+
+    ok = net(2, (4, 3, 2))
+    assert call(None, ok) == L.CL_ERR_ARGUMENT
+    assert call(fake, net(0, (4,))) == L.CL_ERR_ARGUMENT
+    assert call(fake, net(L.CL_MLP_MAX_LAYERS + 1, (4,) * 9)) == L.CL_ERR_ARGUMENT
+    assert call(fake, net(2, (4, 3, 3))) == L.CL_ERR_ARGUMENT            # the loss needs two outputs
+    assert call(fake, net(2, (4, 0, 2))) == L.CL_ERR_ARGUMENT
+    assert call(fake, net(2, (4, 3, 2), null_slot=(1, "vb"))) == L.CL_ERR_ARGUMENT
+    assert call(fake, ok, Bn=0) == L.CL_ERR_ARGUMENT
+    assert call(fake, ok, ldx=4) == L.CL_ERR_ARGUMENT                     # row stride below the batch
+    assert call(fake, ok, step=0) == L.CL_ERR_ARGUMENT
